@@ -1,0 +1,228 @@
+"""Benchmark: implicit Newton steps/s on the 128^3 jittered neo-Hooke block (6.44 M dofs),
+mixed precision (headline) and FP64, each Newton step = assembly + Dirichlet + Jacobi-PCG to
+1e-10 relative + state update, relative Newton residual target 1e-8.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu-baseline]
+
+Prints one JSON line.  `value` is the mixed-precision Newton steps/s over exactly K timed
+Newton iterations (inputs resident in HBM); `roofline` is the FP32 node-block SpMV of the
+mixed inner solve, timed live with HIP events on the solver's stream; `cpu_baseline` is the
+CPU FP64 oracle timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+ASSEMBLY_FLOP_PER_ELEMENT = 45536  # canonical count: 8 QPs x (B^T D B 4320 + geometric 528 +
+#                                    kinematics 576 + f_int 168 + constitutive 100)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(nnz, nnzb, nodes, n, value_bytes):
+    """Algorithmic bytes of one node-block SpMV: values once, one int32 block column per 9
+    values, block row pointers, x read once, y written once."""
+    return value_bytes * nnz + 4 * nnzb + 4 * (nodes + 1) + 2 * value_bytes * n
+
+
+def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
+    sim = dict(p.simulation)
+    sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": precision})
+    from neon_amd import _capi
+
+    if warmup > 0:
+        sm = neon_amd.static_matrix(mesh, dict(sim, max_total_iterations=warmup), p.boundaries)
+        sm.solve()
+    spmv_class = _capi.KERNEL_SPMV_FP32 if precision == "mixed" else _capi.KERNEL_SPMV_FP64
+    ctx.profile(True)
+    for c in range(4):
+        ctx.profile_read(c, reset=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sm = neon_amd.static_matrix(mesh, dict(sim, max_total_iterations=steps), p.boundaries)
+    sm.solve()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    spmv_count, spmv_ms = ctx.profile_read(spmv_class)
+    asm_count, asm_ms = ctx.profile_read(_capi.KERNEL_ASSEMBLY)
+    ctx.profile(False)
+    timed = dict(seconds=t1 - t0, records=sm.records, timing=sm.timing, spmv_count=spmv_count, spmv_ms=spmv_ms,
+                 asm_count=asm_count, asm_ms=asm_ms)
+    log(f"[{precision}] {steps} Newton steps in {t1 - t0:.3f} s; CG its "
+        f"{[r['solver_iterations'] for r in sm.records]}; spmv {spmv_count} x {spmv_ms / max(spmv_count, 1):.4f} ms")
+    # first load step to the 1e-8 relative residual
+    t0 = time.perf_counter()
+    sm = neon_amd.static_matrix(mesh, dict(sim, max_load_steps=1), p.boundaries)
+    sm.solve()
+    t1 = time.perf_counter()
+    timed["to_tolerance"] = dict(seconds=t1 - t0, newton_iterations=len(sm.records),
+                                 final_force_norm=sm.records[-1]["force_norm"],
+                                 final_displacement_norm=sm.records[-1]["displacement_norm"],
+                                 cg_iterations=[r["solver_iterations"] for r in sm.records])
+    log(f"[{precision}] load step 1 to 1e-8: {len(sm.records)} Newton its, {t1 - t0:.3f} s, "
+        f"force norms {[f'{r['force_norm']:.2e}' for r in sm.records]}")
+    return timed
+
+
+def cpu_baseline(problems, n_gpu, cg_its_per_step, nnz_gpu, n_elem_gpu):
+    """Oracle (C++ FP64 port of the reference CPU path) on a 48^3 sample of the same
+    recipe: one state update + assembly (parallel element loop, binary-search + atomic
+    scatter) + serial f_int, and 200 Jacobi-PCG iterations with an OpenMP SpMV; scaled to the
+    full mesh by element and nnz counts and to a Newton step by the GPU FP64 run's mean CG
+    iterations per Newton step."""
+    from oracle import oracle as orc
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    ns = 48
+    p = problems.block_rollers(ns)
+    m = orc.Model(p.X, p.conn, p.params, model=p.model)
+    rng = np.random.default_rng(0)
+    u = 1e-3 * rng.normal(size=p.n_dofs)
+    rp, ci = m.pattern()
+    t0 = time.perf_counter()
+    m.update(u)
+    t1 = time.perf_counter()
+    val, f = m.assemble(rp, ci, threads=threads)
+    t2 = time.perf_counter()
+    b = -f
+    orc.enforce_dirichlet(rp, ci, val, b, p.fixed_dofs())
+    t3 = time.perf_counter()
+    its_sample = 200
+    _, _, its, _ = orc.cg_eigen(rp, ci, val, b, 1e-300, its_sample, threads=threads)
+    t4 = time.perf_counter()
+    per_it = (t4 - t3) / max(its, 1)
+    scale_e = n_elem_gpu / p.n_elem
+    scale_nnz = nnz_gpu / len(ci)
+    step = (t1 - t0 + t2 - t1) * scale_e + (t3 - t2) * scale_nnz + per_it * scale_nnz * cg_its_per_step
+    spmv_b = 12 * len(ci) + 4 * (p.n_dofs + 1) + 16 * p.n_dofs
+    return {
+        "value": 1.0 / step,
+        "unit": "Newton steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{ns}^3 block ({p.n_elem} elements, {len(ci)} nnz): 1 state update + assembly + f_int "
+                   f"({t2 - t0:.2f} s) and {its} Jacobi-PCG iterations ({t4 - t3:.2f} s), extrapolated to "
+                   f"{n_elem_gpu} elements / {nnz_gpu} nnz and {cg_its_per_step:.0f} CG iterations per Newton step"),
+        "cg_iteration_ms_full_mesh": 1e3 * per_it * scale_nnz,
+        "spmv_gbs": spmv_b / per_it / 1e9,
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precisions", default="fp64,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design only "
+                                   "(see the sharding section of DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import neon_amd
+    from neon_amd import problems
+
+    t0 = time.perf_counter()
+    p = problems.block_rollers(args.n, residual_tolerance=1e-8, displacement_tolerance=1e-8, norm_mode=1,
+                               solver_tolerance=1e-10, max_iterations=200000)
+    ctx = neon_amd.Context(0)
+    mesh = neon_amd.DeviceMesh(p.X, p.conn, p.params, model=p.model, context=ctx)
+    nnz = mesh.nnz()
+    nnzb = nnz // 9
+    log(f"setup {time.perf_counter() - t0:.1f} s: {p.n_elem} elements, {p.n_dofs} dofs, nnz {nnz}")
+
+    results = {}
+    for prec in args.precisions.split(","):
+        results[prec] = run_precision(neon_amd, problems, ctx, mesh, p, prec, args.steps, args.warmup)
+
+    head = "mixed" if "mixed" in results else "fp64"
+    r = results[head]
+    value = args.steps / r["seconds"]
+    vb = 4 if head == "mixed" else 8
+    bytes_spmv = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vb)
+    spmv_avg_ms = r["spmv_ms"] / max(r["spmv_count"], 1)
+    achieved = bytes_spmv / (spmv_avg_ms * 1e-3) / 1e9
+    line = {
+        "metric": "Newton steps/s & SpMV GB/s at 6.3M dofs: mixed vs FP64 to 1e-8 residual",
+        "value": value,
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner PCG + f64 refinement/residuals" if head == "mixed" else "f64",
+        "data": "synthetic (seeded jittered structured hex8 mesh, per-element mu ~ U[0.5,1.5] MPa)",
+        "config": {"workload": f"neo-Hooke block {args.n}^3 hex8, rollers x=0/y=0/z=0, top u_z = 20% in 10 "
+                               f"steps; Newton steps of load step 1; Jacobi-PCG 1e-10; Newton target 1e-8",
+                   "elements": p.n_elem, "dofs": p.n_dofs, "nnz": nnz},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"spmv_nb_kernel<{'float' if vb == 4 else 'double'}> (node-block CSR SpMV + p.Ap)",
+                     "algorithmic_bytes": bytes_spmv, "avg_ms": spmv_avg_ms, "launches": r["spmv_count"]},
+    }
+    for prec, rr in results.items():
+        vbp = 4 if prec == "mixed" else 8
+        bs = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vbp)
+        avg = rr["spmv_ms"] / max(rr["spmv_count"], 1)
+        asm_avg = rr["asm_ms"] / max(rr["asm_count"], 1)
+        its = [x["solver_iterations"] for x in rr["records"]]
+        line[prec] = {
+            "newton_steps_per_s": args.steps / rr["seconds"],
+            "ms_per_newton_step": 1e3 * rr["seconds"] / args.steps,
+            "spmv_gbs": bs / (avg * 1e-3) / 1e9,
+            "spmv_avg_ms": avg,
+            "spmv_hbm_frac": bs / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "spmv_csr_equivalent_gbs": (12 if vbp == 8 else 8) * nnz / (avg * 1e-3) / 1e9,
+            "cg_iterations_per_newton_step": its,
+            "assembly_avg_ms": asm_avg,
+            "assembly_gflops": ASSEMBLY_FLOP_PER_ELEMENT * p.n_elem / (asm_avg * 1e-3) / 1e9 if asm_avg > 0 else None,
+            "assembly_fp64_peak_frac": (ASSEMBLY_FLOP_PER_ELEMENT * p.n_elem / (asm_avg * 1e-3) / 1e12
+                                        / FP64_PEAK_TFLOPS) if asm_avg > 0 else None,
+            "phase_ms": {k: rr["timing"][k] for k in ("assemble_ms", "dirichlet_ms", "solve_ms", "update_ms",
+                                                      "total_ms")},
+            "to_1e-8": rr["to_tolerance"],
+        }
+    if "mixed" in results and "fp64" in results:
+        line["mixed_vs_fp64_speedup"] = {
+            "newton_steps_per_s": line["mixed"]["newton_steps_per_s"] / line["fp64"]["newton_steps_per_s"],
+            "time_to_1e-8": results["fp64"]["to_tolerance"]["seconds"] / results["mixed"]["to_tolerance"]["seconds"],
+        }
+    if not args.no_cpu_baseline:
+        its64 = [x["solver_iterations"] for x in results.get("fp64", r)["records"]]
+        line["cpu_baseline"] = cpu_baseline(problems, args.n, float(np.mean(its64)), nnz, p.n_elem)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+/*
+ * neon_hip.h — C ABI of the MI355X (gfx950) implicit Newton step for hex8 solid
+ * mechanics.  Plain pointers and sizes only; every entry point returns an int status.
+ *
+ * The entry points replace these interfaces of dbeurle/neon (paths relative to the
+ * reference repository root):
+ *
+ *   neon_krylov_solve_csr      conjugate_gradient_cuda::solve
+ *                              (src/solver/linear/conjugate_gradient_cuda.cpp:72-192) and
+ *                              biconjugate_gradient_stabilised_cuda::solve
+ *                              (src/solver/linear/biconjugate_gradient_stabilised_cuda.cpp:31-200),
+ *                              behind linear_solver::solve (src/solver/linear/linear_solver.hpp:18)
+ *   neon_mesh_create           mechanics::solid::mesh / submesh construction
+ *                              (src/mesh/mechanics/solid/mesh.cpp:25-46, submesh.cpp:26-52) with
+ *                              the constitutive model from make_constitutive_model
+ *                              (src/constitutive/constitutive_model_factory.cpp:26-164)
+ *   neon_mesh_pattern          compute_sparsity_pattern (src/assembler/sparsity_pattern.hpp:18-48)
+ *   neon_mesh_set_displacement material_coordinates::update_current_configuration
+ *                              (src/mesh/material_coordinates.cpp:23-27)
+ *   neon_mesh_update_internal_variables
+ *                              submesh::update_internal_variables (submesh.cpp:194-269) ->
+ *                              constitutive_model::update_internal_variables
+ *                              (src/constitutive/constitutive_model.hpp:36)
+ *   neon_mesh_save_internal_variables
+ *                              mesh::save_internal_variables (mesh.cpp:73-77)
+ *   neon_mesh_assemble         static_matrix::assemble_stiffness + compute_internal_force
+ *                              (src/assembler/mechanics/static_matrix.hpp:189-203, 243-277) over
+ *                              submesh::tangent_stiffness / internal_force (submesh.cpp:66-160)
+ *   neon_mesh_enforce_dirichlet static_matrix::enforce_dirichlet_conditions (static_matrix.hpp:279-320)
+ *   neon_mesh_solve            linear_solver::solve on the device-resident tangent (static_matrix.hpp:432)
+ *   neon_newton_solve          static_matrix::solve (static_matrix.hpp:145-187, 402-467)
+ *   neon_adjoint_gradient      no counterpart in the reference (new: adjoint of the converged state)
+ *
+ * Threading: one host thread per context; a context owns one HIP stream and every
+ * device buffer it allocates.  Host pointers are borrowed for the duration of a call.
+ */
+#ifndef NEON_HIP_H
+#define NEON_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: > 0 recoverable (neon's computational_error), < 0 fatal ---- */
+#define NEON_OK 0
+#define NEON_NONCONVERGED 1      /* Krylov or Newton iteration limit */
+#define NEON_NAN_OR_INF 2        /* non-finite stress or Krylov scalar (FE_INVALID) */
+#define NEON_NEGATIVE_JACOBIAN 3 /* det F < 0 at a quadrature point */
+#define NEON_RETURN_MAP_FAILED 4 /* J2 radial return did not converge in 50 iterations */
+#define NEON_BREAKDOWN 5         /* BiCGSTAB omega == 0 */
+#define NEON_FATAL -1            /* neon's std::domain_error (e.g. too many cutbacks) */
+#define NEON_BAD_ARGUMENT -2
+#define NEON_HIP_ERROR -3
+#define NEON_OOM -4
+#define NEON_NO_DEVICE -5
+
+/* ---- enumerations ---- */
+#define NEON_MODEL_NEOHOOKE 0 /* compressible_neohooke */
+#define NEON_MODEL_J2 1       /* small_strain_J2_plasticity */
+
+#define NEON_TANGENT_REFERENCE 0  /* neon's tangents (neo-Hooke D without 1/J) */
+#define NEON_TANGENT_CONSISTENT 1 /* exact Jacobian of f_int (D/J for neo-Hooke) */
+
+#define NEON_SOLVER_CG 0
+#define NEON_SOLVER_BICGSTAB 1
+
+#define NEON_STOP_RELATIVE 0               /* CPU rule: ||r|| < tol ||b|| (Eigen) */
+#define NEON_STOP_ABSOLUTE_PRECONDITIONED 1 /* CUDA rule: sqrt(r.z) <= tol */
+
+#define NEON_PRECISION_FP64 0  /* FP64 values, vectors and arithmetic */
+#define NEON_PRECISION_MIXED 1 /* FP32 tangent + inner PCG, FP64 refinement/dots/residuals */
+
+typedef struct neon_context neon_context;
+
+/* ---- context ---- */
+int neon_device_count(int* count);
+int neon_context_create(int device, neon_context** out);
+int neon_context_destroy(neon_context* ctx);
+/* Text of the last error on this context (valid until the next call). */
+const char* neon_context_last_error(const neon_context* ctx);
+/* Wait for all work on the context's stream. */
+int neon_context_synchronize(neon_context* ctx);
+
+/* ---- kernel timing (HIP events on the context's stream; no counterpart in the reference,
+ * whose timing is std::chrono prints) ----
+ * While enabled, every SpMV launch of the Krylov solvers and every assembly call is
+ * bracketed by events; counts and summed milliseconds accumulate per kernel class until
+ * read with reset.  Classes: */
+#define NEON_KERNEL_SPMV_FP64 0     /* node-block SpMV, FP64 values/vectors */
+#define NEON_KERNEL_SPMV_FP32 1     /* node-block SpMV, FP32 values/vectors (mixed inner) */
+#define NEON_KERNEL_RESIDUAL_FP64 2 /* b - K x in FP64 (mixed reliable updates) */
+#define NEON_KERNEL_ASSEMBLY 3      /* one full tangent + f_int assembly (all colours) */
+#define NEON_KERNEL_CLASSES 4
+int neon_context_profile(neon_context* ctx, int enable);
+int neon_context_profile_read(neon_context* ctx, int kernel_class, int reset, int64_t* count, double* total_ms);
+
+/* ---- linear solver on a host CSR matrix (drop-in for the CUDA solvers) ----
+ * A is square, row-major CSR, int32 indices, FP64 values.  x is overwritten.  With
+ * precision NEON_PRECISION_MIXED the inner iterations run in FP32 with FP64 reliable
+ * updates of b - A x (only NEON_SOLVER_CG).  iterations / error receive the solver's
+ * iteration count and final residual measure in the stopping rule's own form. */
+int neon_krylov_solve_csr(neon_context* ctx, int solver, int criterion, int precision, int32_t n, int64_t nnz,
+                          const int32_t* row_ptr, const int32_t* col, const double* val, const double* b,
+                          double* x, double tolerance, int32_t max_iterations, int32_t* iterations,
+                          double* error);
+
+/* ---- device-resident mesh, constitutive state and tangent ----
+ * X: 3 per node, node-interleaved.  conn: 8 per element in hex8 order
+ * (src/interpolations/hexahedron.cpp:21-28).  params: 4 per element
+ * (bulk modulus, shear modulus, yield stress, hardening modulus); neo-Hooke uses the
+ * first two.  One mesh per context. */
+int neon_mesh_create(neon_context* ctx, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
+                     int32_t model, const double* params, int32_t tangent_mode);
+/* Replace the per-element parameters (4 per element) without rebuilding the pattern. */
+int neon_mesh_set_parameters(neon_context* ctx, const double* params);
+/* Sparsity pattern: nnz always; row_ptr (3N+1) and col (nnz) when non-NULL. */
+int neon_mesh_pattern(neon_context* ctx, int64_t* nnz, int32_t* row_ptr, int32_t* col);
+/* u: 3N displacements (host). */
+int neon_mesh_set_displacement(neon_context* ctx, const double* u);
+int neon_mesh_update_internal_variables(neon_context* ctx, double time_step);
+int neon_mesh_save_internal_variables(neon_context* ctx, int32_t converged);
+/* Per-QP state, index e*8+l.  Any pointer may be NULL.  sigma/eps_p 9 per QP (row-major
+ * 3x3), D 36 per QP (6x6 Voigt, engineering shear in strain), detF/ebar/vm 1 per QP. */
+int neon_mesh_qp_state(neon_context* ctx, double* sigma, double* D, double* detF, double* eps_p, double* ebar,
+                       double* vm);
+/* Assemble the tangent (FP64 or FP32 values per precision) and FP64 f_int on the device. */
+int neon_mesh_assemble(neon_context* ctx, int32_t precision);
+/* Download the assembled tangent values (CSR order, widened to FP64) and/or f_int. */
+int neon_mesh_download(neon_context* ctx, double* val, double* f_int);
+/* Zero the rows and columns of the fixed dofs, keeping the diagonal; if b is non-NULL
+ * (host, length 3N) its fixed entries are zeroed too. */
+int neon_mesh_enforce_dirichlet(neon_context* ctx, int32_t n_fixed, const int32_t* fixed, double* b);
+/* Solve (device tangent) x = b with the given solver; b and x are host arrays of 3N. */
+int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double tolerance, int32_t max_iterations,
+                    const double* b, double* x, int32_t* iterations, double* error);
+
+/* ---- Newton driver (static_matrix::solve) ---- */
+typedef struct {
+    int32_t n_dofs;      /* number of constrained dofs */
+    const int32_t* dofs; /* global dof = 3 node + component */
+    int32_t n_points;    /* piecewise-linear time history */
+    const double* times;
+    const double* values;
+} neon_dirichlet;
+
+typedef struct {
+    double period, initial, minimum, maximum; /* "time" block */
+    int32_t adaptive;
+    double residual_tolerance, displacement_tolerance; /* "nonlinear_options" */
+    int32_t max_newton_iterations;                     /* "linear_iterations" (default 10) */
+    int32_t absolute_norm;                             /* "absolute_tolerance" given */
+    int32_t norm_mode;                                 /* 0 neon's norms, 1 always relative */
+    double solver_tolerance;                           /* "linear_solver" block */
+    int32_t solver_max_iterations;
+    int32_t solver_criterion; /* NEON_STOP_* */
+    int32_t solver;           /* NEON_SOLVER_* */
+    int32_t precision;        /* NEON_PRECISION_* */
+    int32_t max_load_steps;   /* stop after this many converged increments (0: all) */
+    int32_t max_total_iterations; /* stop after this many Newton iterations in all (0: no limit) */
+} neon_newton_options;
+
+typedef struct {
+    int32_t step;              /* converged increments before this iteration */
+    int32_t iteration;         /* Newton iteration in this attempt */
+    int32_t solver_iterations; /* Krylov iterations of this Newton iteration */
+    int32_t status;            /* NEON_OK or the recoverable code that ended the attempt */
+    double step_time;
+    double displacement_norm;
+    double force_norm;
+} neon_newton_record;
+
+typedef struct {
+    double assemble_ms, dirichlet_ms, solve_ms, update_ms, total_ms;
+    int64_t spmv_count;        /* SpMV launches in the Krylov phase */
+    int64_t refinement_count;  /* FP64 residual recomputations (mixed precision) */
+} neon_newton_timing;
+
+/* u_out: 3N final displacement (host).  records may be NULL. */
+int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int32_t n_bc,
+                      const neon_dirichlet* bcs, double* u_out, neon_newton_record* records,
+                      int32_t record_capacity, int32_t* n_records, neon_newton_timing* timing);
+
+/* ---- adjoint (no counterpart in the reference) ----
+ * For the converged state held by the context (consistent tangent), with objective
+ * J(u) whose gradient dJdu (3N, host) is given and the constrained dofs listed in
+ * fixed: solves K^T lambda = dJ/du with the same Krylov solver (K symmetric) and returns
+ * dJ/dp_e = -lambda^T dR/dp_e for p_e = the per-element shear modulus with the bulk
+ * modulus tied by bulk = ratio * shear (grad: n_elem). */
+int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed, const int32_t* fixed,
+                          double bulk_over_shear, double tolerance, int32_t max_iterations, double* grad,
+                          int32_t* iterations);
+
+/* ---- synthetic hex8 meshes (host only, no device use) ----
+ * Structured nx*ny*nz block of size Lx*Ly*Lz; node n = i + (nx+1)(j + (ny+1)k).  Nodes are
+ * jittered by U(-jitter h, +jitter h) per component with a counter-based SplitMix64 stream
+ * keyed by seed, except normal to a boundary face.  Elements whose centroid lies within
+ * hole_radius of the z-parallel axis through the block centre are removed (hole_radius
+ * <= 0: none) and unused nodes dropped (node numbering stays increasing). */
+int neon_mesh_box_size(int32_t nx, int32_t ny, int32_t nz, double Lx, double Ly, double Lz, double hole_radius,
+                       int32_t* n_nodes, int32_t* n_elem);
+int neon_mesh_box(int32_t nx, int32_t ny, int32_t nz, double Lx, double Ly, double Lz, double jitter,
+                  uint64_t seed, double hole_radius, double* X, int32_t* conn, int32_t* node_ijk);
+/* Uniform(lo, hi) per index from the counter-based SplitMix64 stream (n values). */
+int neon_random_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
+/* Standard normal per index (Box-Muller on two stream values). */
+int neon_random_normal(uint64_t seed, int64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

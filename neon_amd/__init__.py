@@ -1,0 +1,350 @@
+"""neon_amd — MI355X (gfx950) implicit Newton step for hex8 solid mechanics.
+
+Python mirror of the reference's host interfaces over the C ABI of ``include/neon_hip.h``:
+
+* :class:`conjugate_gradient_hip` / :class:`biconjugate_gradient_stabilised_hip` — the
+  ``linear_solver`` subclasses selected like ``conjugate_gradient_cuda``
+  (src/solver/linear/linear_solver.hpp:13-56, src/solver/linear/conjugate_gradient_cuda.hpp:17-59),
+  with :func:`make_linear_solver` following src/solver/linear/linear_solver_factory.cpp:67-174;
+* :class:`DeviceMesh` — the device-resident ``mesh``/``submesh``/``constitutive_model`` state
+  (src/mesh/mechanics/solid/mesh.hpp:37-80, submesh.hpp:38-67);
+* :class:`static_matrix` — the Newton driver (src/assembler/mechanics/static_matrix.hpp:29-107).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import (  # noqa: F401
+    MODEL_J2,
+    MODEL_NEOHOOKE,
+    PRECISION_FP64,
+    PRECISION_MIXED,
+    SOLVER_BICGSTAB,
+    SOLVER_CG,
+    STOP_ABSOLUTE_PRECONDITIONED,
+    STOP_RELATIVE,
+    TANGENT_CONSISTENT,
+    TANGENT_REFERENCE,
+    NeonError,
+)
+
+__all__ = [
+    "Context",
+    "DeviceMesh",
+    "conjugate_gradient_hip",
+    "biconjugate_gradient_stabilised_hip",
+    "make_linear_solver",
+    "static_matrix",
+    "computational_error",
+]
+
+
+class computational_error(RuntimeError):
+    """Recoverable numerical failure (neon's ``computational_error``, src/exceptions.hpp:14-17)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{_capi.STATUS_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+def _raise(code: int, ctx_handle) -> None:
+    if code == _capi.NEON_OK:
+        return
+    msg = _capi.lib().neon_context_last_error(ctx_handle) if ctx_handle else b""
+    msg = msg.decode() if msg else ""
+    if code > 0:
+        raise computational_error(code, msg)
+    raise NeonError(code, msg)
+
+
+class Context:
+    """One device, one HIP stream (``neon_context_create``)."""
+
+    def __init__(self, device: int = 0):
+        self.handle = C.c_void_p()
+        code = _capi.lib().neon_context_create(device, C.byref(self.handle))
+        if code != _capi.NEON_OK:
+            raise NeonError(code, "neon_context_create failed")
+
+    def close(self):
+        if self.handle:
+            _capi.lib().neon_context_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, code: int) -> None:
+        _raise(code, self.handle)
+
+    def synchronize(self) -> None:
+        self.check(_capi.lib().neon_context_synchronize(self.handle))
+
+    def profile(self, enable: bool) -> None:
+        """Time SpMV launches and assemblies with HIP events on the context's stream."""
+        self.check(_capi.lib().neon_context_profile(self.handle, int(enable)))
+
+    def profile_read(self, kernel_class: int, reset: bool = True):
+        """(launches, summed milliseconds) for a kernel class (``_capi.KERNEL_*``)."""
+        cnt, ms = C.c_int64(0), C.c_double(0.0)
+        self.check(_capi.lib().neon_context_profile_read(self.handle, kernel_class, int(reset), C.byref(cnt),
+                                                         C.byref(ms)))
+        return cnt.value, ms.value
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _csr_arrays(A):
+    """(row_ptr, col, val) from a scipy CSR matrix or a tuple."""
+    if isinstance(A, tuple):
+        rp, ci, v = A
+    else:
+        A = A.tocsr()
+        rp, ci, v = A.indptr, A.indices, A.data
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return rp, ci, v
+
+
+class iterative_linear_solver:
+    """``iterative_linear_solver`` (src/solver/linear/linear_solver.hpp:27-56): defaults
+    tolerance 1e-5 and 2000 iterations."""
+
+    solver_kind = SOLVER_CG
+
+    def __init__(self, residual_tolerance: float = 1.0e-5, max_iterations: int = 2000,
+                 criterion: int = STOP_ABSOLUTE_PRECONDITIONED, precision: int = PRECISION_FP64,
+                 context: Context | None = None):
+        self.residual_tolerance = float(residual_tolerance)
+        self.max_iterations = int(max_iterations)
+        self.criterion = criterion
+        self.precision = precision
+        self.ctx = context or default_context()
+        self.iterations = 0
+        self.error = 0.0
+
+    def solve(self, A, x: np.ndarray, b: np.ndarray) -> None:
+        """``solve(sparse_matrix const& A, vector& x, vector const& b)``: x is overwritten.
+        Raises :class:`computational_error` where neon throws it."""
+        rp, ci, v = _csr_arrays(A)
+        n = len(rp) - 1
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.zeros(n)
+        its = C.c_int32(0)
+        err = C.c_double(0.0)
+        code = _capi.lib().neon_krylov_solve_csr(
+            self.ctx.handle, self.solver_kind, self.criterion, self.precision, n, int(rp[-1]),
+            _capi.i32(rp), _capi.i32(ci), _capi.f64(v), _capi.f64(b), _capi.f64(out),
+            self.residual_tolerance, self.max_iterations, C.byref(its), C.byref(err))
+        self.iterations, self.error = its.value, err.value
+        x[...] = out
+        self.ctx.check(code)
+
+
+class conjugate_gradient_hip(iterative_linear_solver):
+    """Drop-in for ``conjugate_gradient_cuda`` (src/solver/linear/conjugate_gradient_cuda.cpp:72-192)."""
+
+    solver_kind = SOLVER_CG
+
+
+class biconjugate_gradient_stabilised_hip(iterative_linear_solver):
+    """Drop-in for ``biconjugate_gradient_stabilised_cuda``
+    (src/solver/linear/biconjugate_gradient_stabilised_cuda.cpp:31-200)."""
+
+    solver_kind = SOLVER_BICGSTAB
+
+
+def make_linear_solver(solver_data: dict, is_symmetric: bool, context: Context | None = None):
+    """The ``"iterative"`` / ``"device": "gpu"`` branch of make_linear_solver
+    (src/solver/linear/linear_solver_factory.cpp:67-174) with ``backend.type == "hip"``;
+    ``backend.precision`` selects ``"fp64"`` (default) or ``"mixed"``."""
+    if "type" not in solver_data:
+        raise ValueError('A "linear_solver" "type" must be specified')
+    if solver_data["type"] != "iterative":
+        raise ValueError("only the iterative solvers run on the HIP backend")
+    if solver_data.get("device", "cpu") != "gpu":
+        raise ValueError('the HIP solvers need "device": "gpu"')
+    backend = solver_data.get("backend")
+    if backend is None:
+        raise ValueError('The gpu linear solvers require a "backend" object')
+    if backend.get("type") != "hip":
+        raise ValueError("Incorrect backend option specified.")
+    precision = {"fp64": PRECISION_FP64, "mixed": PRECISION_MIXED}[backend.get("precision", "fp64")]
+    kwargs = {}
+    if "tolerance" in solver_data:
+        kwargs["residual_tolerance"] = solver_data["tolerance"]
+    if "maximum_iterations" in solver_data:
+        kwargs["max_iterations"] = solver_data["maximum_iterations"]
+    cls = conjugate_gradient_hip if is_symmetric else biconjugate_gradient_stabilised_hip
+    return cls(precision=precision, context=context, **kwargs)
+
+
+class DeviceMesh:
+    """Device-resident hex8 mesh, constitutive state and tangent (one per context)."""
+
+    def __init__(self, X: np.ndarray, conn: np.ndarray, params: np.ndarray, model: int = MODEL_NEOHOOKE,
+                 tangent: int = TANGENT_REFERENCE, context: Context | None = None):
+        self.ctx = context or Context(0)
+        self.X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1)
+        self.params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        self.n_nodes = len(self.X) // 3
+        self.n_elem = len(self.conn) // 8
+        self.model = model
+        self.ctx.check(_capi.lib().neon_mesh_create(
+            self.ctx.handle, self.n_nodes, _capi.f64(self.X), self.n_elem, _capi.i32(self.conn), model,
+            _capi.f64(self.params), tangent))
+
+    @property
+    def n_dofs(self) -> int:
+        return 3 * self.n_nodes
+
+    def set_parameters(self, params: np.ndarray) -> None:
+        self.params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        self.ctx.check(_capi.lib().neon_mesh_set_parameters(self.ctx.handle, _capi.f64(self.params)))
+
+    def pattern(self):
+        nnz = C.c_int64(0)
+        self.ctx.check(_capi.lib().neon_mesh_pattern(self.ctx.handle, C.byref(nnz), None, None))
+        rp = np.zeros(self.n_dofs + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        self.ctx.check(_capi.lib().neon_mesh_pattern(self.ctx.handle, C.byref(nnz), _capi.i32(rp), _capi.i32(ci)))
+        return rp, ci
+
+    def nnz(self) -> int:
+        nnz = C.c_int64(0)
+        self.ctx.check(_capi.lib().neon_mesh_pattern(self.ctx.handle, C.byref(nnz), None, None))
+        return nnz.value
+
+    def set_displacement(self, u: np.ndarray) -> None:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self.ctx.check(_capi.lib().neon_mesh_set_displacement(self.ctx.handle, _capi.f64(u)))
+
+    def update_internal_variables(self, dt: float = 0.0) -> None:
+        self.ctx.check(_capi.lib().neon_mesh_update_internal_variables(self.ctx.handle, dt))
+
+    def save_internal_variables(self, converged: bool) -> None:
+        self.ctx.check(_capi.lib().neon_mesh_save_internal_variables(self.ctx.handle, int(converged)))
+
+    def qp_state(self):
+        Q = 8 * self.n_elem
+        sigma, D = np.zeros(9 * Q), np.zeros(36 * Q)
+        detF, eps_p, ebar, vm = np.zeros(Q), np.zeros(9 * Q), np.zeros(Q), np.zeros(Q)
+        self.ctx.check(_capi.lib().neon_mesh_qp_state(self.ctx.handle, _capi.f64(sigma), _capi.f64(D), _capi.f64(detF),
+                                                      _capi.f64(eps_p), _capi.f64(ebar), _capi.f64(vm)))
+        return dict(sigma=sigma.reshape(Q, 3, 3), D=D.reshape(Q, 6, 6), detF=detF, eps_p=eps_p.reshape(Q, 3, 3),
+                    ebar=ebar, vm=vm)
+
+    def assemble(self, precision: int = PRECISION_FP64) -> None:
+        self.ctx.check(_capi.lib().neon_mesh_assemble(self.ctx.handle, precision))
+
+    def download(self, stiffness: bool = True, internal_force: bool = True):
+        val = np.zeros(self.nnz()) if stiffness else None
+        f = np.zeros(self.n_dofs) if internal_force else None
+        self.ctx.check(_capi.lib().neon_mesh_download(self.ctx.handle, _capi.f64(val), _capi.f64(f)))
+        return val, f
+
+    def enforce_dirichlet(self, fixed: np.ndarray, b: np.ndarray | None = None) -> None:
+        fixed = np.ascontiguousarray(fixed, dtype=np.int32)
+        self.ctx.check(_capi.lib().neon_mesh_enforce_dirichlet(self.ctx.handle, len(fixed), _capi.i32(fixed),
+                                                               _capi.f64(b)))
+
+    def solve(self, b: np.ndarray, solver: int = SOLVER_CG, criterion: int = STOP_RELATIVE, tol: float = 1e-10,
+              max_iterations: int = 10000):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n_dofs)
+        its, err = C.c_int32(0), C.c_double(0.0)
+        code = _capi.lib().neon_mesh_solve(self.ctx.handle, solver, criterion, tol, max_iterations, _capi.f64(b),
+                                           _capi.f64(x), C.byref(its), C.byref(err))
+        self.ctx.check(code)
+        return x, its.value, err.value
+
+
+def _bc_structs(boundaries):
+    """boundaries: list of dicts {dofs, times, values} -> ctypes array (+ keepalive)."""
+    keep = []
+    arr = (_capi.Dirichlet * max(1, len(boundaries)))()
+    for k, bc in enumerate(boundaries):
+        d = np.ascontiguousarray(bc["dofs"], dtype=np.int32)
+        t = np.ascontiguousarray(bc["times"], dtype=np.float64)
+        v = np.ascontiguousarray(bc["values"], dtype=np.float64)
+        keep += [d, t, v]
+        arr[k] = _capi.Dirichlet(len(d), _capi.i32(d), len(t), _capi.f64(t), _capi.f64(v))
+    return arr, keep
+
+
+class static_matrix:
+    """Newton driver (``static_matrix::solve``, src/assembler/mechanics/static_matrix.hpp:145-467)
+    on a :class:`DeviceMesh`.  ``simulation`` uses neon's keys: ``time``
+    {period, increments{initial, minimum, maximum, adaptive}}, ``nonlinear_options``
+    {displacement_tolerance, residual_tolerance, linear_iterations, absolute_tolerance},
+    ``linear_solver`` {tolerance, maximum_iterations, backend{precision}}; plus ``criterion``,
+    ``norm_mode`` and ``max_load_steps`` extensions."""
+
+    def __init__(self, mesh: DeviceMesh, simulation: dict, boundaries: list):
+        self.mesh = mesh
+        nl = simulation["nonlinear_options"]
+        if "displacement_tolerance" not in nl:
+            raise ValueError("displacement_tolerance not specified in nonlinear_options")
+        if "residual_tolerance" not in nl:
+            raise ValueError("residual_tolerance not specified in nonlinear_options")
+        t = simulation["time"]
+        inc = t["increments"]
+        ls = simulation.get("linear_solver", {})
+        backend = ls.get("backend", {})
+        o = _capi.NewtonOptions()
+        o.period = t["period"]
+        o.initial = inc["initial"]
+        o.minimum = inc["minimum"]
+        o.maximum = inc["maximum"]
+        o.adaptive = int(bool(inc["adaptive"]))
+        o.residual_tolerance = nl["residual_tolerance"]
+        o.displacement_tolerance = nl["displacement_tolerance"]
+        o.max_newton_iterations = nl.get("linear_iterations", 10)
+        o.absolute_norm = int("absolute_tolerance" in nl)
+        o.norm_mode = simulation.get("norm_mode", 0)
+        o.solver_tolerance = ls.get("tolerance", 1.0e-5)
+        o.solver_max_iterations = ls.get("maximum_iterations", 2000)
+        o.solver_criterion = simulation.get("criterion", STOP_RELATIVE)
+        o.solver = SOLVER_CG if ls.get("symmetric", True) else SOLVER_BICGSTAB
+        o.precision = {"fp64": PRECISION_FP64, "mixed": PRECISION_MIXED}[backend.get("precision", "fp64")]
+        o.max_load_steps = simulation.get("max_load_steps", 0)
+        o.max_total_iterations = simulation.get("max_total_iterations", 0)
+        self.options = o
+        self.boundaries = boundaries
+        self.records: list[dict] = []
+        self.timing: dict = {}
+
+    def solve(self, capacity: int = 4096) -> np.ndarray:
+        m = self.mesh
+        u = np.zeros(m.n_dofs)
+        recs = (_capi.NewtonRecord * capacity)()
+        nrec = C.c_int32(0)
+        tm = _capi.NewtonTiming()
+        bcs, keep = _bc_structs(self.boundaries)
+        code = _capi.lib().neon_newton_solve(m.ctx.handle, C.byref(self.options), len(self.boundaries), bcs,
+                                             _capi.f64(u), recs, capacity, C.byref(nrec), C.byref(tm))
+        del keep
+        self.records = [
+            dict(step=r.step, iteration=r.iteration, solver_iterations=r.solver_iterations, status=r.status,
+                 step_time=r.step_time, displacement_norm=r.displacement_norm, force_norm=r.force_norm)
+            for r in recs[: min(nrec.value, capacity)]
+        ]
+        self.timing = {name: getattr(tm, name) for name, _ in _capi.NewtonTiming._fields_}
+        m.ctx.check(code)
+        return u

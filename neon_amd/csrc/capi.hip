@@ -1,0 +1,346 @@
+// C ABI of neon_hip.h: argument checks, exception -> status translation, host<->device
+// staging.  The product path has no CPU fallback: without a device every compute entry
+// point returns NEON_NO_DEVICE.
+#include "engine.hpp"
+#include "newton.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+using namespace neon_amd;
+
+namespace
+{
+template <typename F>
+int guarded(neon_context* ctx, F&& f)
+{
+    if (!ctx) return NEON_BAD_ARGUMENT;
+    try
+    {
+        NEON_CHECK_HIP(hipSetDevice(ctx->device));
+        int const r = f();
+        if (r == NEON_OK) ctx->error.clear();
+        return r;
+    }
+    catch (status_error const& e)
+    {
+        ctx->error = e.what();
+        return e.code;
+    }
+    catch (std::bad_alloc const&)
+    {
+        ctx->error = "host allocation failed";
+        return NEON_OOM;
+    }
+    catch (std::exception const& e)
+    {
+        ctx->error = e.what();
+        return NEON_FATAL;
+    }
+}
+
+int need_mesh(neon_context* ctx)
+{
+    if (!ctx->mesh) throw status_error(NEON_BAD_ARGUMENT, "no mesh on this context");
+    return NEON_OK;
+}
+
+int lanes_for(double avg)
+{
+    int l = 1;
+    while (l < 64 && l < avg) l <<= 1;
+    return l;
+}
+} // namespace
+
+extern "C" {
+
+int neon_device_count(int* count)
+{
+    if (!count) return NEON_BAD_ARGUMENT;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return NEON_OK;
+}
+
+int neon_context_create(int device, neon_context** out)
+{
+    if (!out) return NEON_BAD_ARGUMENT;
+    *out = nullptr;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0) return NEON_NO_DEVICE;
+    if (device < 0 || device >= c) return NEON_BAD_ARGUMENT;
+    auto* ctx = new (std::nothrow) neon_context;
+    if (!ctx) return NEON_OOM;
+    ctx->device = device;
+    int const r = guarded(ctx, [&] {
+        NEON_CHECK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->engine = new krylov_engine(ctx->stream);
+        ctx->engine->prof = &ctx->profiler;
+        return NEON_OK;
+    });
+    if (r != NEON_OK)
+    {
+        neon_context_destroy(ctx);
+        return r;
+    }
+    *out = ctx;
+    return NEON_OK;
+}
+
+int neon_context_destroy(neon_context* ctx)
+{
+    if (!ctx) return NEON_BAD_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx->mesh;
+    delete ctx->engine;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return NEON_OK;
+}
+
+const char* neon_context_last_error(const neon_context* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int neon_context_synchronize(neon_context* ctx)
+{
+    return guarded(ctx, [&] {
+        NEON_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        return NEON_OK;
+    });
+}
+
+int neon_context_profile(neon_context* ctx, int enable)
+{
+    return guarded(ctx, [&] {
+        ctx->profiler.discard();
+        ctx->profiler.enabled = enable != 0;
+        return NEON_OK;
+    });
+}
+
+int neon_context_profile_read(neon_context* ctx, int kernel_class, int reset, int64_t* count, double* total_ms)
+{
+    return guarded(ctx, [&] {
+        ctx->profiler.read(kernel_class, reset != 0, count, total_ms);
+        return NEON_OK;
+    });
+}
+
+int neon_krylov_solve_csr(neon_context* ctx, int solver, int criterion, int precision, int32_t n, int64_t nnz,
+                          const int32_t* row_ptr, const int32_t* col, const double* val, const double* b, double* x,
+                          double tolerance, int32_t max_iterations, int32_t* iterations, double* error)
+{
+    return guarded(ctx, [&] {
+        if (n <= 0 || nnz < 0 || !row_ptr || !col || !val || !b || !x) throw status_error(NEON_BAD_ARGUMENT, "bad CSR arguments");
+        if (solver != NEON_SOLVER_CG && solver != NEON_SOLVER_BICGSTAB) throw status_error(NEON_BAD_ARGUMENT, "unknown solver");
+        if (criterion != NEON_STOP_RELATIVE && criterion != NEON_STOP_ABSOLUTE_PRECONDITIONED)
+            throw status_error(NEON_BAD_ARGUMENT, "unknown stopping rule");
+        if (row_ptr[0] != 0 || row_ptr[n] != nnz) throw status_error(NEON_BAD_ARGUMENT, "row_ptr does not span nnz");
+        for (int32_t r = 0; r < n; ++r)
+            if (row_ptr[r + 1] < row_ptr[r]) throw status_error(NEON_BAD_ARGUMENT, "row_ptr not monotone");
+        for (int64_t p = 0; p < nnz; ++p)
+            if (col[p] < 0 || col[p] >= n) throw status_error(NEON_BAD_ARGUMENT, "column index out of range");
+        hipStream_t const s = ctx->stream;
+        device_array<int32_t> rp, ci;
+        device_array<double> v, bd, xd, minv;
+        device_array<float> v32;
+        rp.resize(size_t(n) + 1);
+        ci.resize(std::max<int64_t>(nnz, 1));
+        v.resize(std::max<int64_t>(nnz, 1));
+        bd.resize(n);
+        xd.resize(n);
+        minv.resize(n);
+        NEON_CHECK_HIP(hipMemcpyAsync(rp.data(), row_ptr, (size_t(n) + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        NEON_CHECK_HIP(hipMemcpyAsync(ci.data(), col, nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        NEON_CHECK_HIP(hipMemcpyAsync(v.data(), val, nnz * sizeof(double), hipMemcpyHostToDevice, s));
+        NEON_CHECK_HIP(hipMemcpyAsync(bd.data(), b, n * sizeof(double), hipMemcpyHostToDevice, s));
+        linear_operator A;
+        A.n = n;
+        A.node_block = false;
+        A.rp = rp.data();
+        A.ci = ci.data();
+        A.lanes_per_row = lanes_for(double(nnz) / n);
+        A.v64 = v.data();
+        solve_result r;
+        if (precision == NEON_PRECISION_FP64)
+        {
+            ctx->engine->jacobi64(A, minv.data(), criterion == NEON_STOP_RELATIVE ? 1 : 0);
+            r = solver == NEON_SOLVER_CG
+                    ? ctx->engine->cg64(A, bd.data(), xd.data(), minv.data(), criterion, tolerance, max_iterations)
+                    : ctx->engine->bicgstab64(A, bd.data(), xd.data(), minv.data(), criterion, tolerance, max_iterations);
+        }
+        else
+        {
+            if (solver != NEON_SOLVER_CG) throw status_error(NEON_BAD_ARGUMENT, "mixed precision supports CG only");
+            v32.resize(std::max<int64_t>(nnz, 1));
+            convert_to_float(s, nnz, v.data(), v32.data());
+            A.v64 = nullptr;
+            A.v32 = v32.data();
+            r = ctx->engine->cg_mixed(A, bd.data(), xd.data(), criterion, tolerance, max_iterations);
+        }
+        NEON_CHECK_HIP(hipMemcpyAsync(x, xd.data(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+        NEON_CHECK_HIP(hipStreamSynchronize(s));
+        if (iterations) *iterations = r.iterations;
+        if (error) *error = r.error;
+        return r.status;
+    });
+}
+
+int neon_mesh_create(neon_context* ctx, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
+                     int32_t model, const double* params, int32_t tangent_mode)
+{
+    return guarded(ctx, [&] {
+        delete ctx->mesh;
+        ctx->mesh = nullptr;
+        ctx->mesh = new device_mesh(ctx->stream, n_nodes, X, n_elem, conn, model, params, tangent_mode);
+        ctx->mesh->prof = &ctx->profiler;
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_set_parameters(neon_context* ctx, const double* params)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!params) throw status_error(NEON_BAD_ARGUMENT, "null parameters");
+        ctx->mesh->set_parameters(params);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_pattern(neon_context* ctx, int64_t* nnz, int32_t* row_ptr, int32_t* col)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (nnz) *nnz = ctx->mesh->nnz();
+        ctx->mesh->pattern(row_ptr, col);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_set_displacement(neon_context* ctx, const double* u)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!u) throw status_error(NEON_BAD_ARGUMENT, "null displacement");
+        ctx->mesh->set_displacement_host(u);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_update_internal_variables(neon_context* ctx, double)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        return ctx->mesh->update_internal_variables(ctx->mesh->u());
+    });
+}
+
+int neon_mesh_save_internal_variables(neon_context* ctx, int32_t converged)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        ctx->mesh->save_internal_variables(converged != 0);
+        NEON_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_qp_state(neon_context* ctx, double* sigma, double* D, double* detF, double* eps_p, double* ebar, double* vm)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        ctx->mesh->download_qp_state(sigma, D, detF, eps_p, ebar, vm);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_assemble(neon_context* ctx, int32_t precision)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (precision != NEON_PRECISION_FP64 && precision != NEON_PRECISION_MIXED)
+            throw status_error(NEON_BAD_ARGUMENT, "unknown precision");
+        ctx->mesh->assemble(ctx->mesh->u(), precision);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_download(neon_context* ctx, double* val, double* f_int)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        ctx->mesh->download(val, f_int, 0);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_enforce_dirichlet(neon_context* ctx, int32_t n_fixed, const int32_t* fixed, double* b)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        device_mesh& m = *ctx->mesh;
+        if (n_fixed < 0 || (n_fixed > 0 && !fixed)) throw status_error(NEON_BAD_ARGUMENT, "bad fixed dof list");
+        for (int32_t i = 0; i < n_fixed; ++i)
+            if (fixed[i] < 0 || fixed[i] >= m.dofs()) throw status_error(NEON_BAD_ARGUMENT, "fixed dof out of range");
+        if (!m.has_k64() && !m.has_k32()) throw status_error(NEON_BAD_ARGUMENT, "no tangent assembled");
+        device_array<int32_t> f;
+        f.resize(std::max(1, n_fixed));
+        if (n_fixed > 0)
+            NEON_CHECK_HIP(hipMemcpyAsync(f.data(), fixed, n_fixed * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        m.enforce_dirichlet(n_fixed, f.data(), nullptr, m.has_k64() ? NEON_PRECISION_FP64 : NEON_PRECISION_MIXED);
+        NEON_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        if (b)
+            for (int32_t i = 0; i < n_fixed; ++i) b[fixed[i]] = 0.0;
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double tolerance, int32_t max_iterations,
+                    const double* b, double* x, int32_t* iterations, double* error)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        device_mesh& m = *ctx->mesh;
+        if (!b || !x) throw status_error(NEON_BAD_ARGUMENT, "null vector");
+        int64_t const n = m.dofs();
+        device_array<double> bd, xd, minv;
+        bd.resize(n);
+        xd.resize(n);
+        minv.resize(n);
+        NEON_CHECK_HIP(hipMemcpyAsync(bd.data(), b, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        int const precision = m.has_k64() ? NEON_PRECISION_FP64 : NEON_PRECISION_MIXED;
+        solve_result const r = device_solve(ctx, solver, criterion, precision, tolerance, max_iterations, bd.data(),
+                                            xd.data(), minv.data());
+        NEON_CHECK_HIP(hipMemcpyAsync(x, xd.data(), n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        NEON_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        if (iterations) *iterations = r.iterations;
+        if (error) *error = r.error;
+        return r.status;
+    });
+}
+
+int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int32_t n_bc, const neon_dirichlet* bcs,
+                      double* u_out, neon_newton_record* records, int32_t record_capacity, int32_t* n_records,
+                      neon_newton_timing* timing)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!options || !u_out || n_bc < 0 || (n_bc > 0 && !bcs)) throw status_error(NEON_BAD_ARGUMENT, "bad arguments");
+        if (options->precision != NEON_PRECISION_FP64 && options->precision != NEON_PRECISION_MIXED)
+            throw status_error(NEON_BAD_ARGUMENT, "unknown precision");
+        return newton_solve(ctx, options, n_bc, bcs, u_out, records, record_capacity, n_records, timing);
+    });
+}
+
+int neon_adjoint_gradient(neon_context* ctx, const double*, int32_t, const int32_t*, double, double, int32_t, double*,
+                          int32_t*)
+{
+    return guarded(ctx, [&]() -> int { throw status_error(NEON_BAD_ARGUMENT, "adjoint is not available in this build"); });
+}
+
+} // extern "C"

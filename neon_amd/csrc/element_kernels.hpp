@@ -1,0 +1,474 @@
+// Device kernels of the hex8 element path: per-quadrature-point deformation measures and
+// constitutive update, element tangent + internal force with scatter into the node-block
+// CSR, and Dirichlet row/column zeroing.
+//
+// Reference behaviour (paths relative to the neon repository root):
+//   deformation measures   src/mesh/mechanics/solid/submesh.cpp:210-269
+//   neo-Hooke              src/constitutive/mechanics/solid/compressible_neohooke.cpp:17-90
+//   J2 radial return       src/constitutive/mechanics/solid/small_strain_J2_plasticity.cpp:30-163
+//   element K, f_int       src/mesh/mechanics/solid/submesh.cpp:66-160
+//   Dirichlet zeroing      src/assembler/mechanics/static_matrix.hpp:279-320
+#pragma once
+
+#include "common.hpp"
+#include "neon_hip.h"
+
+namespace neon_amd
+{
+// Per-QP stored data, structure-of-arrays with stride Q = 8 E (index k*Q + qp):
+//   sigma : 6  Cauchy stress, kinetic Voigt order (11, 22, 33, 23, 13, 12)
+//   coef  : 9  tangent coefficients (A, Bt, C, n11, n22, n33, n23, n13, n12) such that
+//              D = A m m^T + Bt diag(2,2,2,1,1,1) + C n n^T  (m = (1,1,1,0,0,0),
+//              n in kinetic Voigt order); neo-Hooke: A = lambda, Bt = mu - lambda ln J, C = 0;
+//              J2: A = lambda + c1/3, Bt = G - c1/2, C = c2 with c1, c2 of the algorithmic
+//              tangent (small_strain_J2_plasticity.cpp:95-106).
+//   detF  : 1
+//   J2 history: eps_p 6 (tensor components 11,22,33,23,13,12), ebar 1, vm 1; committed
+//   copies eps_p_old, ebar_old.
+enum
+{
+    COEF_A = 0,
+    COEF_B = 1,
+    COEF_C = 2,
+    COEF_N = 3
+};
+
+struct hex8_tables
+{
+    double dN[8][8][3]; // [qp][node][xi]
+    double w[8];
+};
+
+__constant__ hex8_tables c_hex8;
+
+struct qp_state
+{
+    double* sigma;
+    double* coef;
+    double* detF;
+    double* eps_p;
+    double* ebar;
+    double* vm;
+    int64_t Q;
+};
+
+struct element_params
+{
+    const double* p; // 4 per element: bulk, shear, yield, hardening
+};
+
+__device__ inline double det3(const double* a)
+{
+    return a[0] * (a[4] * a[8] - a[7] * a[5]) - a[3] * (a[1] * a[8] - a[7] * a[2])
+           + a[6] * (a[1] * a[5] - a[4] * a[2]);
+}
+
+__device__ inline void inv3(const double* a, double* r)
+{
+    double const c00 = a[4] * a[8] - a[5] * a[7];
+    double const c01 = a[5] * a[6] - a[3] * a[8];
+    double const c02 = a[3] * a[7] - a[4] * a[6];
+    double const det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double const id = 1.0 / det;
+    r[0] = c00 * id;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    r[3] = c01 * id;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    r[6] = c02 * id;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+// ---------------------------------------------------- constitutive update per QP
+// One thread per quadrature point (qp = 8 e + l).
+template <int MODEL>
+__global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const int32_t* __restrict__ conn,
+                                                              const double* __restrict__ X,
+                                                              const double* __restrict__ u, element_params prm,
+                                                              qp_state st, int tangent_mode,
+                                                              uint32_t* __restrict__ status)
+{
+    int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (qp >= 8ll * E) return;
+    int64_t const e = qp >> 3;
+    int const l = int(qp & 7);
+    int64_t const Q = st.Q;
+
+    double F0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fx[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double dXe[8][3];
+    for (int a = 0; a < 8; ++a)
+    {
+        int64_t const nd = conn[8 * e + a];
+        for (int i = 0; i < 3; ++i)
+        {
+            double const Xi = X[3 * nd + i];
+            double const xi = Xi + u[3 * nd + i]; // x = X + u (material_coordinates.cpp:23-27)
+            dXe[a][i] = xi - Xi;
+            for (int j = 0; j < 3; ++j)
+            {
+                F0[i * 3 + j] += Xi * c_hex8.dN[l][a][j];
+                Fx[i * 3 + j] += xi * c_hex8.dN[l][a][j];
+            }
+        }
+    }
+    double F0i[9];
+    inv3(F0, F0i);
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 8; ++a)
+    {
+        double B0[3];
+        for (int j = 0; j < 3; ++j)
+            B0[j] = c_hex8.dN[l][a][0] * F0i[0 * 3 + j] + c_hex8.dN[l][a][1] * F0i[1 * 3 + j]
+                    + c_hex8.dN[l][a][2] * F0i[2 * 3 + j];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) H[i * 3 + j] += dXe[a][i] * B0[j];
+    }
+    double F[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            F[i * 3 + j] = Fx[i * 3 + 0] * F0i[0 * 3 + j] + Fx[i * 3 + 1] * F0i[1 * 3 + j] + Fx[i * 3 + 2] * F0i[2 * 3 + j];
+    double const J = det3(F);
+    st.detF[qp] = J;
+    if (signbit(J)) atomicOr(status, STATUS_NEGATIVE_JACOBIAN);
+
+    double const bulk = prm.p[4 * e], shear = prm.p[4 * e + 1];
+    double const lambda = bulk - 2.0 * shear / 3.0;
+    double sig[6];
+    double cA, cB, cC = 0.0, nv[6] = {0, 0, 0, 0, 0, 0};
+    if constexpr (MODEL == NEON_MODEL_NEOHOOKE)
+    {
+        // sigma = (lambda ln J I + mu (F F^T - I)) / J ; D: lambda, mu' = mu - lambda ln J
+        double const lnJ = log(J);
+        double b[6];
+        b[0] = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+        b[1] = F[3] * F[3] + F[4] * F[4] + F[5] * F[5];
+        b[2] = F[6] * F[6] + F[7] * F[7] + F[8] * F[8];
+        b[3] = F[3] * F[6] + F[4] * F[7] + F[5] * F[8];
+        b[4] = F[0] * F[6] + F[1] * F[7] + F[2] * F[8];
+        b[5] = F[0] * F[3] + F[1] * F[4] + F[2] * F[5];
+        for (int k = 0; k < 3; ++k) sig[k] = (lambda * lnJ + shear * (b[k] - 1.0)) / J;
+        for (int k = 3; k < 6; ++k) sig[k] = (shear * b[k]) / J;
+        cA = lambda;
+        cB = shear - lambda * lnJ;
+        if (tangent_mode == NEON_TANGENT_CONSISTENT)
+        {
+            cA /= J;
+            cB /= J;
+        }
+    }
+    else
+    {
+        // small-strain J2 with linear isotropic hardening, history updated in place
+        double const G = shear, sy = prm.p[4 * e + 2], Hh = prm.p[4 * e + 3];
+        double ep[6];
+        for (int k = 0; k < 6; ++k) ep[k] = st.eps_p[k * Q + qp];
+        double eb = st.ebar[qp];
+        // strain = sym(H) in tensor components (11,22,33,23,13,12)
+        double eps[6] = {H[0], H[4], H[8], 0.5 * (H[5] + H[7]), 0.5 * (H[2] + H[6]), 0.5 * (H[1] + H[3])};
+        double ee[6];
+        for (int k = 0; k < 6; ++k) ee[k] = eps[k] - ep[k];
+        double const tr = ee[0] + ee[1] + ee[2];
+        for (int k = 0; k < 6; ++k) sig[k] = 2.0 * G * ee[k];
+        for (int k = 0; k < 3; ++k) sig[k] += lambda * tr;
+        auto vonmises = [](const double* s) {
+            double const t = (s[0] + s[1] + s[2]) / 3.0;
+            double const d0 = s[0] - t, d1 = s[1] - t, d2 = s[2] - t;
+            double const nrm = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+            return sqrt(3.0 / 2.0) * sqrt(nrm);
+        };
+        double vm = vonmises(sig);
+        cA = lambda;
+        cB = G;
+        double const f0 = (vm - 3.0 * G * 0.0) - (sy + (eb + 0.0) * Hh);
+        if (f0 > 0.0)
+        {
+            double const vm_tr = vm;
+            double const t = (sig[0] + sig[1] + sig[2]) / 3.0;
+            double dev[6] = {sig[0] - t, sig[1] - t, sig[2] - t, sig[3], sig[4], sig[5]};
+            double const nd =
+                sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] + 2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5]));
+            for (int k = 0; k < 6; ++k) nv[k] = dev[k] / nd;
+            double dg = 0.0, f = f0;
+            int it = 0;
+            while (f > 1.0e-6 && it < 50)
+            {
+                dg += f / (3.0 * G + Hh);
+                f = (vm_tr - 3.0 * G * dg) - (sy + (eb + dg) * Hh);
+                ++it;
+            }
+            if (it == 50) atomicOr(status, STATUS_RETURN_MAP);
+            double const s32 = sqrt(3.0 / 2.0);
+            for (int k = 0; k < 6; ++k) ep[k] += dg * s32 * nv[k];
+            for (int k = 0; k < 6; ++k) sig[k] -= 2.0 * G * dg * s32 * nv[k];
+            vm = vonmises(sig);
+            eb += dg;
+            double const c1 = dg * 6.0 * G * G / vm_tr;
+            double const c2 = 6.0 * G * G * (dg / vm_tr - 1.0 / (3.0 * G + Hh));
+            cA = lambda + c1 / 3.0;
+            cB = G - 0.5 * c1;
+            cC = c2;
+            for (int k = 0; k < 6; ++k) st.eps_p[k * Q + qp] = ep[k];
+            st.ebar[qp] = eb;
+        }
+        st.vm[qp] = vm;
+    }
+    bool finite = isfinite(cA) && isfinite(cB);
+    for (int k = 0; k < 6; ++k)
+    {
+        st.sigma[k * Q + qp] = sig[k];
+        finite = finite && isfinite(sig[k]);
+    }
+    if (!finite) atomicOr(status, STATUS_NAN);
+    st.coef[COEF_A * Q + qp] = cA;
+    st.coef[COEF_B * Q + qp] = cB;
+    st.coef[COEF_C * Q + qp] = cC;
+    for (int k = 0; k < 6; ++k) st.coef[(COEF_N + k) * Q + qp] = nv[k];
+}
+
+// commit (old <- current) or revert (current <- old) of the J2 history
+__global__ __launch_bounds__(kBlock) void history_copy_kernel(int64_t n, const double* __restrict__ src,
+                                                              double* __restrict__ dst)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Expand the stored coefficients to the 6x6 Voigt tangent (test download).
+__global__ __launch_bounds__(kBlock) void expand_tangent_kernel(int64_t Q, const double* __restrict__ coef,
+                                                                const double* __restrict__ sigma,
+                                                                double* __restrict__ D, double* __restrict__ s9)
+{
+    int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (qp >= Q) return;
+    double const A = coef[COEF_A * Q + qp], B = coef[COEF_B * Q + qp], C = coef[COEF_C * Q + qp];
+    double n[6];
+    for (int k = 0; k < 6; ++k) n[k] = coef[(COEF_N + k) * Q + qp];
+    if (D)
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j)
+            {
+                double v = C * n[i] * n[j];
+                if (i < 3 && j < 3) v += A;
+                if (i == j) v += (i < 3 ? 2.0 : 1.0) * B;
+                D[36 * qp + 6 * i + j] = v;
+            }
+    if (s9)
+    {
+        double const s[6] = {sigma[0 * Q + qp], sigma[1 * Q + qp], sigma[2 * Q + qp],
+                             sigma[3 * Q + qp], sigma[4 * Q + qp], sigma[5 * Q + qp]};
+        double* o = s9 + 9 * qp;
+        o[0] = s[0];
+        o[4] = s[1];
+        o[8] = s[2];
+        o[5] = o[7] = s[3];
+        o[2] = o[6] = s[4];
+        o[1] = o[3] = s[5];
+    }
+}
+
+// ------------------------------------------------------------------ assembly
+// One wavefront (= one 64-thread block) per element at a time; blocks stride over the
+// elements of one colour, so no two elements in flight share a node and the K / f_int
+// read-modify-writes need no atomics.
+//
+// Per element:
+//   phase A, lane = (l, a): J_l = x dN_l (current configuration), det, inverse,
+//     g_{l,a} = J_l^{-T} dN_{l,a} (row a of dN J^-1) -> LDS; lanes a = 0..5 of QP l stage
+//     the QP coefficients, pre-multiplied by w_l det J_l.
+//   phase B, lane = (a, b): 3x3 block
+//     K_ab = sum_l [A g_a g_b^T + Bt (g_b g_a^T + (g_a.g_b) I) + C (N g_a)(N g_b)^T
+//                   + (g_a^T sigma g_b) I (finite deformation only)]
+//     which equals B_a^T D B_b (+ the geometric term) for the D above
+//     (submesh.cpp:136-160, 110-134); the 9 values are added into CSR through the
+//     element's slot table.
+//   phase C, lanes (a, i<3): f_a,i = sum_l (sigma g_a)_i w det J (submesh.cpp:82-108).
+template <typename KT, bool FINITE>
+__global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict__ elems, int32_t count,
+                                                      const int32_t* __restrict__ conn,
+                                                      const double* __restrict__ X,
+                                                      const double* __restrict__ u, qp_state st,
+                                                      const uint8_t* __restrict__ slots,
+                                                      const int32_t* __restrict__ bptr, KT* __restrict__ K,
+                                                      double* __restrict__ f_int)
+{
+    __shared__ double s_x[8][3];
+    __shared__ double s_g[8][8][3]; // [l][a][k]
+    __shared__ double s_c[8][16];   // [l]: A, Bt, C, n(6), sigma(6) scaled by w detJ (n unscaled)
+    __shared__ int32_t s_node[8];
+
+    int const lane = threadIdx.x;
+    int const hi = lane >> 3, lo = lane & 7;
+    int64_t const Q = st.Q;
+
+    for (int32_t idx = blockIdx.x; idx < count; idx += gridDim.x)
+    {
+        int64_t const e = elems[idx];
+        if (lane < 8)
+        {
+            int64_t const nd = conn[8 * e + lane];
+            s_node[lane] = int32_t(nd);
+            for (int i = 0; i < 3; ++i) s_x[lane][i] = X[3 * nd + i] + u[3 * nd + i];
+        }
+        __syncthreads();
+        // phase A: lane = (l = hi, a = lo)
+        {
+            int const l = hi, a = lo;
+            double J[9];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                {
+                    double s = 0.0;
+                    for (int c = 0; c < 8; ++c) s += s_x[c][i] * c_hex8.dN[l][c][j];
+                    J[i * 3 + j] = s;
+                }
+            double Ji[9];
+            inv3(J, Ji);
+            double const wdet = det3(J) * c_hex8.w[l];
+            for (int k = 0; k < 3; ++k)
+                s_g[l][a][k] = c_hex8.dN[l][a][0] * Ji[0 * 3 + k] + c_hex8.dN[l][a][1] * Ji[1 * 3 + k]
+                               + c_hex8.dN[l][a][2] * Ji[2 * 3 + k];
+            int64_t const qp = 8 * e + l;
+            // coefficient slots: a -> 0..7, a+8 -> 8..14
+            for (int c = a; c < 15; c += 8)
+            {
+                double v;
+                if (c < 9)
+                    v = st.coef[int64_t(c) * Q + qp];
+                else
+                    v = st.sigma[int64_t(c - 9) * Q + qp];
+                if (c < 3 || c >= 9) v *= wdet;
+                s_c[l][c] = v;
+            }
+        }
+        __syncthreads();
+        // phase B: lane = (a = hi, b = lo)
+        {
+            int const a = hi, b = lo;
+            double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int l = 0; l < 8; ++l)
+            {
+                double const ga0 = s_g[l][a][0], ga1 = s_g[l][a][1], ga2 = s_g[l][a][2];
+                double const gb0 = s_g[l][b][0], gb1 = s_g[l][b][1], gb2 = s_g[l][b][2];
+                double const A = s_c[l][0], Bt = s_c[l][1], C = s_c[l][2];
+                double const gab = ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
+                double diag = Bt * gab;
+                if constexpr (FINITE)
+                {
+                    // g_a^T sigma g_b, sigma (11,22,33,23,13,12) pre-scaled
+                    double const s0 = s_c[l][9], s1 = s_c[l][10], s2 = s_c[l][11], s3 = s_c[l][12],
+                                 s4 = s_c[l][13], s5 = s_c[l][14];
+                    double const t0 = s0 * gb0 + s5 * gb1 + s4 * gb2;
+                    double const t1 = s5 * gb0 + s1 * gb1 + s3 * gb2;
+                    double const t2 = s4 * gb0 + s3 * gb1 + s2 * gb2;
+                    diag += ga0 * t0 + ga1 * t1 + ga2 * t2;
+                }
+                double const ga[3] = {ga0, ga1, ga2}, gb[3] = {gb0, gb1, gb2};
+                double na[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
+                if (C != 0.0)
+                {
+                    // N g with N = [[n11,n12,n13],[n12,n22,n23],[n13,n23,n33]]
+                    double const n11 = s_c[l][3], n22 = s_c[l][4], n33 = s_c[l][5], n23 = s_c[l][6],
+                                 n13 = s_c[l][7], n12 = s_c[l][8];
+                    na[0] = n11 * ga0 + n12 * ga1 + n13 * ga2;
+                    na[1] = n12 * ga0 + n22 * ga1 + n23 * ga2;
+                    na[2] = n13 * ga0 + n23 * ga1 + n33 * ga2;
+                    nb[0] = n11 * gb0 + n12 * gb1 + n13 * gb2;
+                    nb[1] = n12 * gb0 + n22 * gb1 + n23 * gb2;
+                    nb[2] = n13 * gb0 + n23 * gb1 + n33 * gb2;
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                    {
+                        double v = A * ga[i] * gb[j] + Bt * gb[i] * ga[j] + C * na[i] * nb[j];
+                        if (i == j) v += diag;
+                        k[i * 3 + j] += v;
+                    }
+            }
+            int32_t const na_ = s_node[a];
+            int32_t const b0 = bptr[na_];
+            int64_t const deg = bptr[na_ + 1] - b0;
+            int64_t const base = 9ll * b0 + 3ll * slots[64 * e + 8 * a + b];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                {
+                    int64_t const pos = base + i * 3 * deg + j;
+                    K[pos] = KT(double(K[pos]) + k[i * 3 + j]);
+                }
+        }
+        // phase C: lane = (a = hi, i = lo < 3)
+        if (lo < 3)
+        {
+            int const a = hi, i = lo;
+            double f = 0.0;
+            for (int l = 0; l < 8; ++l)
+            {
+                double const s0 = s_c[l][9], s1 = s_c[l][10], s2 = s_c[l][11], s3 = s_c[l][12], s4 = s_c[l][13],
+                             s5 = s_c[l][14];
+                double const g0 = s_g[l][a][0], g1 = s_g[l][a][1], g2 = s_g[l][a][2];
+                double v;
+                if (i == 0)
+                    v = g0 * s0 + g1 * s5 + g2 * s4;
+                else if (i == 1)
+                    v = g0 * s5 + g1 * s1 + g2 * s3;
+                else
+                    v = g0 * s4 + g1 * s3 + g2 * s2;
+                f += v;
+            }
+            int64_t const pos = 3ll * s_node[a] + i;
+            f_int[pos] += f;
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------- Dirichlet
+// For each fixed dof d: zero row d and column d except the diagonal.  All writes are
+// zeros and the diagonal is never written, so concurrent dofs need no ordering; the
+// result equals the sequential loop of enforce_dirichlet_conditions.
+template <typename KT>
+__global__ __launch_bounds__(kBlock) void dirichlet_kernel(int32_t n_fixed, const int32_t* __restrict__ fixed,
+                                                           const int32_t* __restrict__ bptr,
+                                                           const int32_t* __restrict__ bcol, KT* __restrict__ K,
+                                                           double* __restrict__ b)
+{
+    int32_t const t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_fixed) return;
+    int32_t const d = fixed[t];
+    int32_t const nd = d / 3, i = d % 3;
+    int32_t const b0 = bptr[nd];
+    int32_t const deg = bptr[nd + 1] - b0;
+    if (b) b[d] = 0.0;
+    for (int32_t s = 0; s < deg; ++s)
+    {
+        int32_t const m = bcol[b0 + s];
+        // row d
+        for (int j = 0; j < 3; ++j)
+        {
+            if (m == nd && j == i) continue;
+            K[9ll * b0 + 3ll * i * deg + 3ll * s + j] = KT(0);
+        }
+        // column d in the rows of node m: find the slot of nd in m's block row
+        int32_t const c0 = bptr[m];
+        int32_t const cdeg = bptr[m + 1] - c0;
+        int32_t slot = -1;
+        for (int32_t q = 0; q < cdeg; ++q)
+            if (bcol[c0 + q] == nd)
+            {
+                slot = q;
+                break;
+            }
+        if (slot < 0) continue;
+        for (int j = 0; j < 3; ++j)
+        {
+            if (m == nd && j == i) continue;
+            K[9ll * c0 + 3ll * j * cdeg + 3ll * slot + i] = KT(0);
+        }
+    }
+}
+} // namespace neon_amd

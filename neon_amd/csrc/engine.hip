@@ -1,0 +1,959 @@
+// Host engine of the gfx950 implicit Newton step: Krylov drivers and the device mesh.
+#include "engine.hpp"
+#include "element_kernels.hpp"
+#include "krylov_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include <omp.h>
+
+namespace neon_amd
+{
+namespace
+{
+inline int grid_for(int64_t n, int per_block = kBlock)
+{
+    int64_t const g = (n + per_block - 1) / per_block;
+    return int(std::max<int64_t>(1, g));
+}
+
+// Round a block count to a multiple of 8 (one share per XCD) within [1, cap].
+inline int xcd_blocks(int64_t want, int cap)
+{
+    int64_t b = std::min<int64_t>(std::max<int64_t>(want, 1), cap);
+    if (b >= 8) b = (b / 8) * 8;
+    return int(b);
+}
+} // namespace
+
+// ================================================================ operator
+int linear_operator::spmv_blocks() const
+{
+    int64_t const rows = node_block ? nodes : n;
+    int64_t const rows_per_pass = node_block ? kBlock / 32 : kBlock / lanes_per_row;
+    // about 4 passes per block, at most kMaxPartials blocks
+    return xcd_blocks((rows + 4 * rows_per_pass - 1) / (4 * rows_per_pass), kMaxPartials);
+}
+
+namespace
+{
+template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const XT* x, YT* y, const double* b,
+                 double* part, const krylov_state* st, int gate = 0)
+{
+    int const blocks = A.spmv_blocks();
+    if (A.node_block)
+    {
+        nb_matrix m{A.nodes, A.bptr, A.bcol};
+        spmv_nb_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
+    }
+    else
+    {
+        csr_matrix m{int32_t(A.n), A.rp, A.ci};
+        switch (A.lanes_per_row)
+        {
+        case 1: spmv_csr_kernel<1, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate); break;
+        case 2: spmv_csr_kernel<2, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate); break;
+        case 4: spmv_csr_kernel<4, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate); break;
+        case 8: spmv_csr_kernel<8, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate); break;
+        case 16: spmv_csr_kernel<16, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate); break;
+        case 32: spmv_csr_kernel<32, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate); break;
+        default: spmv_csr_kernel<64, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate); break;
+        }
+    }
+    NEON_CHECK_HIP(hipGetLastError());
+}
+} // namespace
+
+void linear_operator::apply64(hipStream_t s, const double* x, double* y, double* part, const krylov_state* st) const
+{
+    if (!v64) throw status_error(NEON_BAD_ARGUMENT, "operator has no FP64 values");
+    if (part)
+        launch_spmv<double, double, double, double, true, 0>(*this, s, v64, x, y, nullptr, part, st);
+    else
+        launch_spmv<double, double, double, double, false, 0>(*this, s, v64, x, y, nullptr, nullptr, st);
+}
+
+void linear_operator::apply32(hipStream_t s, const float* x, float* y, double* part, const krylov_state* st) const
+{
+    if (!v32) throw status_error(NEON_BAD_ARGUMENT, "operator has no FP32 values");
+    if (part)
+        launch_spmv<float, float, float, float, true, 0>(*this, s, v32, x, y, nullptr, part, st);
+    else
+        launch_spmv<float, float, float, float, false, 0>(*this, s, v32, x, y, nullptr, nullptr, st);
+}
+
+void linear_operator::residual64(hipStream_t s, const double* b, const double* x, double* r, const krylov_state* st,
+                                 int gate) const
+{
+    if (v64)
+        launch_spmv<double, double, double, double, false, 1>(*this, s, v64, x, r, b, nullptr, st, gate);
+    else
+        launch_spmv<float, double, double, double, false, 1>(*this, s, v32, x, r, b, nullptr, st, gate);
+}
+
+// ================================================================ profiler
+kernel_profiler::~kernel_profiler()
+{
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+}
+
+int kernel_profiler::begin(hipStream_t s, int cls)
+{
+    if (!enabled || pairs_ >= kMaxPairs) return -1;
+    while (int(events_.size()) < 2 * (pairs_ + 1))
+    {
+        hipEvent_t e;
+        NEON_CHECK_HIP(hipEventCreate(&e));
+        events_.push_back(e);
+    }
+    if (int(cls_.size()) <= pairs_) cls_.resize(pairs_ + 1);
+    cls_[pairs_] = cls;
+    NEON_CHECK_HIP(hipEventRecord(events_[2 * pairs_], s));
+    return pairs_++;
+}
+
+void kernel_profiler::end(hipStream_t s, int token)
+{
+    if (token < 0) return;
+    NEON_CHECK_HIP(hipEventRecord(events_[2 * token + 1], s));
+}
+
+void kernel_profiler::collect(int64_t valid)
+{
+    for (int i = 0; i < pairs_; ++i)
+    {
+        if (valid >= 0 && i >= valid) break;
+        float t = 0.0f;
+        NEON_CHECK_HIP(hipEventSynchronize(events_[2 * i + 1]));
+        NEON_CHECK_HIP(hipEventElapsedTime(&t, events_[2 * i], events_[2 * i + 1]));
+        ms_[cls_[i]] += t;
+        count_[cls_[i]] += 1;
+    }
+    pairs_ = 0;
+}
+
+void kernel_profiler::read(int cls, bool reset, int64_t* count, double* ms)
+{
+    if (cls < 0 || cls >= NEON_KERNEL_CLASSES) throw status_error(NEON_BAD_ARGUMENT, "unknown kernel class");
+    if (count) *count = count_[cls];
+    if (ms) *ms = ms_[cls];
+    if (reset)
+    {
+        count_[cls] = 0;
+        ms_[cls] = 0.0;
+    }
+}
+
+// ================================================================== engine
+krylov_engine::krylov_engine(hipStream_t s) : s_(s)
+{
+    for (auto& p : part_) p.resize(kMaxPartials);
+    state_.resize(2);
+    scalar_.resize(1);
+    NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_state_), sizeof(krylov_state)));
+    NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_scalar_), sizeof(double)));
+}
+
+krylov_engine::~krylov_engine()
+{
+    if (host_state_) (void)hipHostFree(host_state_);
+    if (host_scalar_) (void)hipHostFree(host_scalar_);
+}
+
+int krylov_engine::vec_blocks(int64_t n) const { return xcd_blocks((n + 4 * kBlock - 1) / (4 * kBlock), 2048); }
+
+void krylov_engine::ensure(int64_t n)
+{
+    if (n == n_) return;
+    r_.resize(n);
+    p_.resize(n);
+    Ap_.resize(n);
+    n_ = n;
+}
+
+double krylov_engine::dot(int64_t n, const double* a, const double* b)
+{
+    int const nb = vec_blocks(n);
+    dot_kernel<<<nb, kBlock, 0, s_>>>(n, a, b, part_[0].data());
+    reduce_to_kernel<<<1, kBlock, 0, s_>>>(part_[0].data(), nb, scalar_.data());
+    NEON_CHECK_HIP(hipGetLastError());
+    NEON_CHECK_HIP(hipMemcpyAsync(host_scalar_, scalar_.data(), sizeof(double), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    return *host_scalar_;
+}
+
+void krylov_engine::jacobi64(const linear_operator& A, double* minv, int eigen_rule)
+{
+    csr_matrix m{int32_t(A.n), A.rp, A.ci};
+    jacobi_kernel<double><<<grid_for(A.n), kBlock, 0, s_>>>(m, A.v64, minv, eigen_rule);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+namespace
+{
+solve_result finish(const krylov_state& s, int criterion)
+{
+    solve_result r;
+    r.iterations = s.it;
+    r.error = criterion == NEON_STOP_RELATIVE ? (s.rhs2 > 0 ? std::sqrt(s.rr / s.rhs2) : 0.0) : std::sqrt(s.rz);
+    switch (s.done)
+    {
+    case KRYLOV_CONVERGED: r.status = NEON_OK; break;
+    case KRYLOV_LIMIT: r.status = NEON_NONCONVERGED; break;
+    case KRYLOV_NONFINITE: r.status = NEON_NAN_OR_INF; break;
+    case KRYLOV_BREAKDOWN: r.status = NEON_BREAKDOWN; break;
+    default: r.status = NEON_NONCONVERGED; break;
+    }
+    return r;
+}
+} // namespace
+
+// Jacobi-PCG in FP64.  Per iteration three kernels: SpMV fused with p.Ap, the x/r update
+// fused with r.r and r.z, and the p update with the convergence decision; scalars stay on
+// the device and the host polls the state every `poll_every` iterations.  Iterations past
+// convergence are no-ops, so the reported count is exact.
+solve_result krylov_engine::cg64(const linear_operator& A, const double* b, double* x, const double* minv,
+                                 int criterion, double tol, int32_t max_it)
+{
+    int64_t const n = A.n;
+    ensure(n);
+    int const nb = vec_blocks(n);
+    int const ns = A.spmv_blocks();
+    double* part_pAp = part_[0].data();
+    double* part_rr = part_[1].data();
+    double* part_rz = part_[2].data();
+    krylov_state* st = state_.data();
+
+    cg_init_kernel<<<nb, kBlock, 0, s_>>>(n, x, r_.data(), p_.data(), b, minv, part_rr, part_rz);
+    cg_init_state_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, st, criterion, tol, max_it);
+    NEON_CHECK_HIP(hipGetLastError());
+
+    solve_result res;
+    int64_t launched = 0;
+    for (;;)
+    {
+        NEON_CHECK_HIP(hipMemcpyAsync(host_state_, st + (launched & 1), sizeof(krylov_state), hipMemcpyDeviceToHost, s_));
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
+        if (host_state_->done) break;
+        for (int k = 0; k < poll_every; ++k, ++launched)
+        {
+            krylov_state* cur = st + (launched & 1);
+            krylov_state* nxt = st + ((launched + 1) & 1);
+            int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP64) : -1;
+            A.apply64(s_, p_.data(), Ap_.data(), part_pAp, cur);
+            if (prof) prof->end(s_, tok);
+            cg_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x, r_.data(), p_.data(), Ap_.data(), minv, part_pAp, ns, cur,
+                                                        part_rr, part_rz);
+            cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p_.data(), r_.data(), minv, part_rr, part_rz, nb, cur, nxt,
+                                                       criterion, tol, max_it);
+        }
+        NEON_CHECK_HIP(hipGetLastError());
+    }
+    res = finish(*host_state_, criterion);
+    // SpMV launches that did work: Eigen's rule counts the converging iteration's SpMV
+    // outside `iterations`; the CUDA rule counts every iteration.
+    res.spmv = std::min<int64_t>(
+        launched, res.iterations + (criterion == NEON_STOP_RELATIVE && host_state_->done == KRYLOV_CONVERGED ? 1 : 0));
+    if (prof) prof->collect(res.spmv);
+    return res;
+}
+
+// Jacobi-BiCGSTAB in FP64 (the unsymmetric selection, not on the symmetric hot path): vector
+// work on the device, the handful of scalars per iteration reduced on the device and read by
+// the host.  criterion NEON_STOP_ABSOLUTE_PRECONDITIONED follows
+// biconjugate_gradient_stabilised_cuda::solve (biconjugate_gradient_stabilised_cuda.cpp:31-200);
+// NEON_STOP_RELATIVE follows Eigen's BiCGSTAB with its restart rule (linear_solver.cpp:111-138).
+solve_result krylov_engine::bicgstab64(const linear_operator& A, const double* b, double* x, const double* minv,
+                                       int criterion, double tol, int32_t max_it)
+{
+    int64_t const n = A.n;
+    ensure(n);
+    r0_.resize(n);
+    v_.resize(n);
+    s_vec_.resize(n);
+    t_.resize(n);
+    ph_.resize(n);
+    sh_.resize(n);
+    double* r = r_.data();
+    double* p = p_.data();
+    double* r0 = r0_.data();
+    double* v = v_.data();
+    double* sv = s_vec_.data();
+    double* t = t_.data();
+    double* y = ph_.data();
+    double* z = sh_.data();
+    int const g = grid_for(n);
+    size_t const bytes = n * sizeof(double);
+    solve_result res;
+    auto spmv = [&](const double* in, double* out) {
+        A.apply64(s_, in, out, nullptr, nullptr);
+        ++res.spmv;
+    };
+    NEON_CHECK_HIP(hipMemsetAsync(x, 0, bytes, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(r0, b, bytes, hipMemcpyDeviceToDevice, s_));
+    NEON_CHECK_HIP(hipMemsetAsync(v, 0, bytes, s_));
+    NEON_CHECK_HIP(hipMemsetAsync(p, 0, bytes, s_));
+
+    if (criterion == NEON_STOP_ABSOLUTE_PRECONDITIONED)
+    {
+        double residual = dot(n, r, r0), residual_old = 0.0, alpha = 0.0, omega = 0.0;
+        int32_t k = 0;
+        int status = NEON_OK;
+        while (std::sqrt(residual) > tol && k < max_it)
+        {
+            ++k;
+            if (k == 1)
+                NEON_CHECK_HIP(hipMemcpyAsync(p, r, bytes, hipMemcpyDeviceToDevice, s_));
+            else
+            {
+                double const beta = residual / residual_old * alpha / omega;
+                lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, p, -omega, v, p); // p - omega v
+                lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, beta, p, p);   // r + beta p
+            }
+            mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, p, y);
+            spmv(y, v);
+            alpha = residual / dot(n, r0, v);
+            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, -alpha, v, sv);
+            double const norm_s = std::sqrt(dot(n, sv, sv));
+            if (norm_s < tol)
+            {
+                lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, alpha, y, x);
+                residual = norm_s * norm_s;
+                break;
+            }
+            mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, sv, z);
+            spmv(z, t);
+            double const ts = dot(n, t, sv), tt = dot(n, t, t);
+            omega = ts / tt;
+            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, alpha, y, x);
+            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, omega, z, x);
+            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, sv, -omega, t, r);
+            residual_old = residual;
+            residual = dot(n, r0, r);
+            if (omega == 0.0) // is_approx(omega, 0.0) holds only for an exact zero
+            {
+                status = NEON_BREAKDOWN;
+                break;
+            }
+        }
+        NEON_CHECK_HIP(hipGetLastError());
+        res.iterations = k;
+        res.error = std::sqrt(residual);
+        res.status = status != NEON_OK ? status : (k >= max_it ? NEON_NONCONVERGED : NEON_OK);
+        if (!std::isfinite(residual) && res.status == NEON_OK) res.status = NEON_NAN_OR_INF;
+        return res;
+    }
+
+    // Eigen
+    double const rhs2 = dot(n, r, r);
+    if (rhs2 == 0.0)
+    {
+        res.status = NEON_OK;
+        return res;
+    }
+    double r0_sq = rhs2;
+    double rho = 1, alpha = 1, w = 1;
+    double const tol2 = tol * tol * rhs2;
+    double const eps2 = DBL_EPSILON * DBL_EPSILON;
+    int32_t i = 0, restarts = 0;
+    double rr = rhs2;
+    while (rr > tol2 && i < max_it)
+    {
+        double const rho_old = rho;
+        rho = dot(n, r0, r);
+        if (std::abs(rho) < eps2 * r0_sq)
+        {
+            A.residual64(s_, b, x, r);
+            ++res.spmv;
+            NEON_CHECK_HIP(hipMemcpyAsync(r0, r, bytes, hipMemcpyDeviceToDevice, s_));
+            rho = r0_sq = dot(n, r, r);
+            if (restarts++ == 0) i = 0;
+        }
+        double const beta = (rho / rho_old) * (alpha / w);
+        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, p, -w, v, p);
+        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, beta, p, p);
+        mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, p, y);
+        spmv(y, v);
+        alpha = rho / dot(n, r0, v);
+        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, -alpha, v, sv);
+        mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, sv, z);
+        spmv(z, t);
+        double const tmp = dot(n, t, t);
+        w = tmp > 0.0 ? dot(n, t, sv) / tmp : 0.0;
+        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, alpha, y, w, z, y); // alpha y + w z
+        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, 1.0, y, x);
+        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, sv, -w, t, r);
+        rr = dot(n, r, r);
+        ++i;
+    }
+    NEON_CHECK_HIP(hipGetLastError());
+    res.iterations = i;
+    res.error = std::sqrt(rr / rhs2);
+    res.status = !std::isfinite(rr) ? NEON_NAN_OR_INF : (i >= max_it ? NEON_NONCONVERGED : NEON_OK);
+    return res;
+}
+
+void krylov_engine::ensure_mixed(int64_t n)
+{
+    if (n == n_mixed_) return;
+    r32_.resize(n);
+    p32_.resize(n);
+    Ap32_.resize(n);
+    x32_.resize(n);
+    minv32_.resize(n);
+    r64_.resize(n);
+    n_mixed_ = n;
+}
+
+// Mixed-precision Jacobi-PCG: FP32 recurrence and tangent values, FP64 dot products and
+// FP64 reliable updates of r = b - K x (see mixed_* kernels).  Relative stopping rule on the
+// true FP64 residual; iterations counted as Eigen counts them.
+solve_result krylov_engine::cg_mixed(const linear_operator& A, const double* b, double* x, int criterion, double tol,
+                                     int32_t max_it)
+{
+    if (criterion != NEON_STOP_RELATIVE)
+        throw status_error(NEON_BAD_ARGUMENT, "mixed precision uses the relative stopping rule");
+    if (!A.v32) throw status_error(NEON_BAD_ARGUMENT, "mixed precision needs FP32 tangent values");
+    int64_t const n = A.n;
+    ensure_mixed(n);
+    int const nb = vec_blocks(n);
+    int const ns = A.spmv_blocks();
+    double* part_pAp = part_[0].data();
+    double* part_rr = part_[1].data();
+    double* part_rz = part_[2].data();
+    double* part_rr_t = part_[3].data();
+    double* part_rz_t = part_[4].data();
+    krylov_state* st = state_.data();
+    float* minv = minv32_.data();
+
+    // Jacobi inverse (FP32) from the FP32 diagonal
+    if (A.node_block)
+        throw status_error(NEON_BAD_ARGUMENT, "cg_mixed: node-block operators take their Jacobi inverse from the mesh");
+    csr_matrix m{int32_t(n), A.rp, A.ci};
+    jacobi32_kernel<float><<<grid_for(n), kBlock, 0, s_>>>(m, A.v32, minv, 1);
+    return cg_mixed_run(A, b, x, tol, max_it, nb, ns, part_pAp, part_rr, part_rz, part_rr_t, part_rz_t, st, minv);
+}
+
+solve_result krylov_engine::cg_mixed_nb(const linear_operator& A, const double* b, double* x, const double* minv_in,
+                                        double tol, int32_t max_it)
+{
+    int64_t const n = A.n;
+    ensure_mixed(n);
+    convert_kernel<double, float><<<grid_for(n), kBlock, 0, s_>>>(n, minv_in, minv32_.data());
+    return cg_mixed_run(A, b, x, tol, max_it, vec_blocks(n), A.spmv_blocks(), part_[0].data(), part_[1].data(),
+                        part_[2].data(), part_[3].data(), part_[4].data(), state_.data(), minv32_.data());
+}
+
+solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double* b, double* x, double tol,
+                                         int32_t max_it, int nb, int ns, double* part_pAp, double* part_rr,
+                                         double* part_rz, double* part_rr_t, double* part_rz_t, krylov_state* st,
+                                         float* minv)
+{
+    int64_t const n = A.n;
+    float* r32 = r32_.data();
+    float* p32 = p32_.data();
+    float* Ap32 = Ap32_.data();
+    float* x32 = x32_.data();
+    double* r64 = r64_.data();
+    mixed_init_kernel<<<nb, kBlock, 0, s_>>>(n, b, x, r32, x32, p32, minv, part_rr, part_rz);
+    mixed_init_state_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, st, tol, max_it);
+    NEON_CHECK_HIP(hipGetLastError());
+    int64_t launched = 0;
+    for (;;)
+    {
+        NEON_CHECK_HIP(hipMemcpyAsync(host_state_, st + (launched & 1), sizeof(krylov_state), hipMemcpyDeviceToHost, s_));
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
+        if (host_state_->done) break;
+        for (int k = 0; k < poll_every; ++k, ++launched)
+        {
+            krylov_state* cur = st + (launched & 1);
+            krylov_state* nxt = st + ((launched + 1) & 1);
+            int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP32) : -1;
+            A.apply32(s_, p32, Ap32, part_pAp, cur);
+            if (prof) prof->end(s_, tok);
+            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, part_rr, part_rz);
+            mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
+            mixed_fold_kernel<<<grid_for(n), kBlock, 0, s_>>>(n, x, x32, cur, 0);
+            A.residual64(s_, b, x, r64, cur, 1);
+            mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
+            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, nb, cur, nxt, max_it);
+        }
+        NEON_CHECK_HIP(hipGetLastError());
+    }
+    // fold whatever the recurrence holds since the last reliable update
+    mixed_fold_kernel<<<grid_for(n), kBlock, 0, s_>>>(n, x, x32, st, 1);
+    NEON_CHECK_HIP(hipGetLastError());
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    solve_result res = finish(*host_state_, NEON_STOP_RELATIVE);
+    res.refinements = host_state_->restarts;
+    int64_t const inner = res.iterations + (host_state_->done == KRYLOV_CONVERGED ? 1 : 0);
+    res.spmv = inner + res.refinements;
+    if (prof) prof->collect(inner);
+    return res;
+}
+
+void convert_to_float(hipStream_t s, int64_t n, const double* x, float* y)
+{
+    if (n <= 0) return;
+    convert_kernel<double, float><<<grid_for(n), kBlock, 0, s>>>(n, x, y);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+// ==================================================================== mesh
+void upload_hex8_tables()
+{
+    static bool done = false;
+    if (done) return;
+    hex8_tables t{};
+    static const double node[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                      {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    double const qp = 1.0 / std::sqrt(3.0);
+    for (int l = 0; l < 8; ++l)
+    {
+        double const xi = node[l][0] * qp, eta = node[l][1] * qp, zeta = node[l][2] * qp;
+        t.w[l] = 1.0;
+        for (int a = 0; a < 8; ++a)
+        {
+            double const xa = node[a][0], ya = node[a][1], za = node[a][2];
+            t.dN[l][a][0] = 1.0 / 8.0 * xa * (1.0 + ya * eta) * (1.0 + za * zeta);
+            t.dN[l][a][1] = 1.0 / 8.0 * (1.0 + xa * xi) * ya * (1.0 + za * zeta);
+            t.dN[l][a][2] = 1.0 / 8.0 * (1.0 + xa * xi) * (1.0 + ya * eta) * za;
+        }
+    }
+    NEON_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_hex8), &t, sizeof(t)));
+    done = true;
+}
+
+device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
+                         int32_t model, const double* params, int32_t tangent_mode)
+    : s_(s), N_(n_nodes), E_(n_elem), model_(model), tangent_mode_(tangent_mode)
+{
+    if (n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params)
+        throw status_error(NEON_BAD_ARGUMENT, "empty mesh or null pointer");
+    if (model != NEON_MODEL_NEOHOOKE && model != NEON_MODEL_J2) throw status_error(NEON_BAD_ARGUMENT, "unknown model");
+    if (tangent_mode != NEON_TANGENT_REFERENCE && tangent_mode != NEON_TANGENT_CONSISTENT)
+        throw status_error(NEON_BAD_ARGUMENT, "unknown tangent mode");
+    for (int64_t i = 0; i < 8ll * n_elem; ++i)
+        if (conn[i] < 0 || conn[i] >= n_nodes) throw status_error(NEON_BAD_ARGUMENT, "connectivity out of range");
+    upload_hex8_tables();
+
+    // node -> element incidence
+    std::vector<int32_t> inc_ptr(size_t(N_) + 1, 0), inc;
+    for (int64_t i = 0; i < 8ll * E_; ++i) inc_ptr[conn[i] + 1]++;
+    for (int32_t nd = 0; nd < N_; ++nd) inc_ptr[nd + 1] += inc_ptr[nd];
+    inc.resize(8ull * E_);
+    {
+        std::vector<int32_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int64_t e = 0; e < E_; ++e)
+            for (int a = 0; a < 8; ++a) inc[fill[conn[8 * e + a]]++] = int32_t(e);
+    }
+    // node adjacency (sorted unique) -> block pattern, as compute_sparsity_pattern
+    // (src/assembler/sparsity_pattern.hpp:18-48) yields for 3 dofs per node
+    std::vector<int32_t> deg(N_, 0);
+#pragma omp parallel
+    {
+        std::vector<int32_t> buf;
+#pragma omp for schedule(dynamic, 4096)
+        for (int32_t nd = 0; nd < N_; ++nd)
+        {
+            buf.clear();
+            for (int32_t k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k)
+                for (int a = 0; a < 8; ++a) buf.push_back(conn[8ll * inc[k] + a]);
+            std::sort(buf.begin(), buf.end());
+            deg[nd] = int32_t(std::unique(buf.begin(), buf.end()) - buf.begin());
+        }
+    }
+    bptr_h_.assign(size_t(N_) + 1, 0);
+    for (int32_t nd = 0; nd < N_; ++nd) bptr_h_[nd + 1] = bptr_h_[nd] + deg[nd];
+    int64_t const nnzb = bptr_h_[N_];
+    nnz_ = 9 * nnzb;
+    if (nnz_ > int64_t(INT32_MAX)) throw status_error(NEON_BAD_ARGUMENT, "nnz exceeds the int32 CSR index range");
+    bcol_h_.resize(nnzb);
+    std::vector<int32_t> self(N_);
+#pragma omp parallel
+    {
+        std::vector<int32_t> buf;
+#pragma omp for schedule(dynamic, 4096)
+        for (int32_t nd = 0; nd < N_; ++nd)
+        {
+            buf.clear();
+            for (int32_t k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k)
+                for (int a = 0; a < 8; ++a) buf.push_back(conn[8ll * inc[k] + a]);
+            std::sort(buf.begin(), buf.end());
+            buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+            std::copy(buf.begin(), buf.end(), bcol_h_.begin() + bptr_h_[nd]);
+            self[nd] = int32_t(std::lower_bound(buf.begin(), buf.end(), nd) - buf.begin());
+        }
+    }
+    // element slot table: slot of node b in node a's block row
+    std::vector<uint8_t> slots(64ull * E_);
+    bool overflow = false;
+#pragma omp parallel for schedule(static) reduction(|| : overflow)
+    for (int64_t e = 0; e < E_; ++e)
+        for (int a = 0; a < 8; ++a)
+        {
+            int32_t const na = conn[8 * e + a];
+            const int32_t* beg = bcol_h_.data() + bptr_h_[na];
+            const int32_t* end = bcol_h_.data() + bptr_h_[na + 1];
+            for (int b = 0; b < 8; ++b)
+            {
+                int64_t const s = std::lower_bound(beg, end, conn[8 * e + b]) - beg;
+                if (s > 255) overflow = true;
+                slots[64 * e + 8 * a + b] = uint8_t(s);
+            }
+        }
+    if (overflow) throw status_error(NEON_BAD_ARGUMENT, "node degree above 255");
+    // greedy element colouring: elements of one colour share no node
+    std::vector<int8_t> colour(E_, -1);
+    int ncol = 0;
+    for (int64_t e = 0; e < E_; ++e)
+    {
+        uint64_t used = 0;
+        for (int a = 0; a < 8; ++a)
+        {
+            int32_t const nd = conn[8 * e + a];
+            for (int32_t k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k)
+            {
+                int8_t const c = colour[inc[k]];
+                if (c >= 0) used |= 1ull << c;
+            }
+        }
+        int c = 0;
+        while (c < 64 && (used >> c & 1ull)) ++c;
+        if (c == 64) throw status_error(NEON_BAD_ARGUMENT, "element colouring needs more than 64 colours");
+        colour[e] = int8_t(c);
+        ncol = std::max(ncol, c + 1);
+    }
+    colour_offsets_.assign(ncol + 1, 0);
+    for (int64_t e = 0; e < E_; ++e) colour_offsets_[colour[e] + 1]++;
+    for (int c = 0; c < ncol; ++c) colour_offsets_[c + 1] += colour_offsets_[c];
+    std::vector<int32_t> elems(E_);
+    {
+        std::vector<int32_t> fill(colour_offsets_.begin(), colour_offsets_.end() - 1);
+        for (int64_t e = 0; e < E_; ++e) elems[fill[colour[e]]++] = int32_t(e);
+    }
+
+    X_.resize(3ull * N_);
+    u_.resize(3ull * N_);
+    fint_.resize(3ull * N_);
+    params_.resize(4ull * E_);
+    conn_.resize(8ull * E_);
+    elems_.resize(E_);
+    bptr_.resize(size_t(N_) + 1);
+    bcol_.resize(nnzb);
+    self_slot_.resize(N_);
+    slots_.resize(64ull * E_);
+    NEON_CHECK_HIP(hipMemcpyAsync(X_.data(), X, 3ull * N_ * sizeof(double), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemsetAsync(u_.data(), 0, 3ull * N_ * sizeof(double), s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(params_.data(), params, 4ull * E_ * sizeof(double), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(conn_.data(), conn, 8ull * E_ * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(elems_.data(), elems.data(), size_t(E_) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(bptr_.data(), bptr_h_.data(), (size_t(N_) + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(bcol_.data(), bcol_h_.data(), size_t(nnzb) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(self_slot_.data(), self.data(), size_t(N_) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(slots_.data(), slots.data(), slots.size(), hipMemcpyHostToDevice, s_));
+
+    int64_t const Q = 8ll * E_;
+    sigma_.resize(6 * Q);
+    coef_.resize(9 * Q);
+    detF_.resize(Q);
+    NEON_CHECK_HIP(hipMemsetAsync(sigma_.data(), 0, 6 * Q * sizeof(double), s_));
+    NEON_CHECK_HIP(hipMemsetAsync(coef_.data(), 0, 9 * Q * sizeof(double), s_));
+    if (model_ == NEON_MODEL_J2)
+    {
+        eps_p_.resize(6 * Q);
+        ebar_.resize(Q);
+        vm_.resize(Q);
+        eps_p_old_.resize(6 * Q);
+        ebar_old_.resize(Q);
+        NEON_CHECK_HIP(hipMemsetAsync(eps_p_.data(), 0, 6 * Q * sizeof(double), s_));
+        NEON_CHECK_HIP(hipMemsetAsync(ebar_.data(), 0, Q * sizeof(double), s_));
+        NEON_CHECK_HIP(hipMemsetAsync(vm_.data(), 0, Q * sizeof(double), s_));
+        NEON_CHECK_HIP(hipMemsetAsync(eps_p_old_.data(), 0, 6 * Q * sizeof(double), s_));
+        NEON_CHECK_HIP(hipMemsetAsync(ebar_old_.data(), 0, Q * sizeof(double), s_));
+    }
+    status_.resize(1);
+    NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&status_host_), sizeof(uint32_t)));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
+void device_mesh::set_parameters(const double* params)
+{
+    NEON_CHECK_HIP(hipMemcpyAsync(params_.data(), params, 4ull * E_ * sizeof(double), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
+void device_mesh::set_displacement_host(const double* u)
+{
+    NEON_CHECK_HIP(hipMemcpyAsync(u_.data(), u, 3ull * N_ * sizeof(double), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
+int device_mesh::update_internal_variables(const double* u_dev)
+{
+    int64_t const Q = 8ll * E_;
+    qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q};
+    NEON_CHECK_HIP(hipMemsetAsync(status_.data(), 0, sizeof(uint32_t), s_));
+    element_params prm{params_.data()};
+    if (model_ == NEON_MODEL_NEOHOOKE)
+        update_state_kernel<NEON_MODEL_NEOHOOKE><<<grid_for(Q), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_dev, prm, st,
+                                                                                  tangent_mode_, status_.data());
+    else
+        update_state_kernel<NEON_MODEL_J2><<<grid_for(Q), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_dev, prm, st,
+                                                                            tangent_mode_, status_.data());
+    NEON_CHECK_HIP(hipGetLastError());
+    NEON_CHECK_HIP(hipMemcpyAsync(status_host_, status_.data(), sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    uint32_t const f = *status_host_;
+    // submesh.cpp:239-269 checks det F first, then the constitutive update, then FE_INVALID
+    if (f & STATUS_NEGATIVE_JACOBIAN) return NEON_NEGATIVE_JACOBIAN;
+    if (f & STATUS_RETURN_MAP) return NEON_RETURN_MAP_FAILED;
+    if (f & STATUS_NAN) return NEON_NAN_OR_INF;
+    return NEON_OK;
+}
+
+// internal_variables::commit / revert (src/constitutive/internal_variables.hpp:203-214).  Only
+// the J2 history is path dependent; every other variable is recomputed from u before use.
+void device_mesh::save_internal_variables(bool converged)
+{
+    if (model_ != NEON_MODEL_J2) return;
+    int64_t const Q = 8ll * E_;
+    if (converged)
+    {
+        history_copy_kernel<<<grid_for(6 * Q), kBlock, 0, s_>>>(6 * Q, eps_p_.data(), eps_p_old_.data());
+        history_copy_kernel<<<grid_for(Q), kBlock, 0, s_>>>(Q, ebar_.data(), ebar_old_.data());
+    }
+    else
+    {
+        history_copy_kernel<<<grid_for(6 * Q), kBlock, 0, s_>>>(6 * Q, eps_p_old_.data(), eps_p_.data());
+        history_copy_kernel<<<grid_for(Q), kBlock, 0, s_>>>(Q, ebar_old_.data(), ebar_.data());
+    }
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+void device_mesh::assemble(const double* u_dev, int precision)
+{
+    int64_t const Q = 8ll * E_;
+    qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q};
+    bool const finite = model_ == NEON_MODEL_NEOHOOKE; // is_finite_deformation()
+    hipEvent_t e0, e1;
+    NEON_CHECK_HIP(hipEventCreate(&e0));
+    NEON_CHECK_HIP(hipEventCreate(&e1));
+    NEON_CHECK_HIP(hipEventRecord(e0, s_));
+    NEON_CHECK_HIP(hipMemsetAsync(fint_.data(), 0, 3ull * N_ * sizeof(double), s_));
+    if (precision == NEON_PRECISION_FP64)
+    {
+        k64_.resize(nnz_);
+        NEON_CHECK_HIP(hipMemsetAsync(k64_.data(), 0, nnz_ * sizeof(double), s_));
+    }
+    else
+    {
+        k32_.resize(nnz_);
+        NEON_CHECK_HIP(hipMemsetAsync(k32_.data(), 0, nnz_ * sizeof(float), s_));
+    }
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    for (int c = 0; c < colours(); ++c)
+    {
+        int32_t const off = colour_offsets_[c];
+        int32_t const cnt = colour_offsets_[c + 1] - off;
+        if (cnt == 0) continue;
+        int const blocks = int(std::min<int64_t>(cnt, int64_t(cus) * 16));
+        const int32_t* el = elems_.data() + off;
+        if (precision == NEON_PRECISION_FP64)
+        {
+            if (finite)
+                assemble_kernel<double, true><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
+                                                                      slots_.data(), bptr_.data(), k64_.data(), fint_.data());
+            else
+                assemble_kernel<double, false><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
+                                                                       slots_.data(), bptr_.data(), k64_.data(), fint_.data());
+        }
+        else
+        {
+            if (finite)
+                assemble_kernel<float, true><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
+                                                                     slots_.data(), bptr_.data(), k32_.data(), fint_.data());
+            else
+                assemble_kernel<float, false><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
+                                                                      slots_.data(), bptr_.data(), k32_.data(), fint_.data());
+        }
+    }
+    NEON_CHECK_HIP(hipGetLastError());
+    NEON_CHECK_HIP(hipEventRecord(e1, s_));
+    NEON_CHECK_HIP(hipEventSynchronize(e1));
+    NEON_CHECK_HIP(hipEventElapsedTime(&last_assemble_ms, e0, e1));
+    if (prof) prof->add(NEON_KERNEL_ASSEMBLY, last_assemble_ms);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    k64_valid_ = precision == NEON_PRECISION_FP64;
+    k32_valid_ = precision != NEON_PRECISION_FP64;
+}
+
+void device_mesh::enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision)
+{
+    if (n_fixed <= 0) return;
+    if (precision == NEON_PRECISION_FP64)
+        dirichlet_kernel<double><<<grid_for(n_fixed), kBlock, 0, s_>>>(n_fixed, fixed_dev, bptr_.data(), bcol_.data(),
+                                                                         k64_.data(), b_dev);
+    else
+        dirichlet_kernel<float><<<grid_for(n_fixed), kBlock, 0, s_>>>(n_fixed, fixed_dev, bptr_.data(), bcol_.data(),
+                                                                        k32_.data(), b_dev);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+void device_mesh::pattern(int32_t* row_ptr, int32_t* col) const
+{
+    if (row_ptr)
+    {
+        row_ptr[0] = 0;
+        for (int32_t nd = 0; nd < N_; ++nd)
+        {
+            int32_t const d = bptr_h_[nd + 1] - bptr_h_[nd];
+            for (int i = 0; i < 3; ++i) row_ptr[3 * nd + i + 1] = 9 * bptr_h_[nd] + (i + 1) * 3 * d;
+        }
+    }
+    if (col)
+    {
+#pragma omp parallel for schedule(static)
+        for (int32_t nd = 0; nd < N_; ++nd)
+        {
+            int32_t const b0 = bptr_h_[nd], d = bptr_h_[nd + 1] - b0;
+            int64_t pos = 9ll * b0;
+            for (int i = 0; i < 3; ++i)
+                for (int32_t s = 0; s < d; ++s)
+                    for (int j = 0; j < 3; ++j) col[pos++] = 3 * bcol_h_[b0 + s] + j;
+        }
+    }
+}
+
+void device_mesh::download(double* val, double* f_int, int)
+{
+    if (val)
+    {
+        if (k64_valid_)
+            NEON_CHECK_HIP(hipMemcpyAsync(val, k64_.data(), nnz_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+        else if (k32_valid_)
+        {
+            device_array<double> tmp;
+            tmp.resize(nnz_);
+            convert_kernel<float, double><<<grid_for(nnz_), kBlock, 0, s_>>>(nnz_, k32_.data(), tmp.data());
+            NEON_CHECK_HIP(hipMemcpyAsync(val, tmp.data(), nnz_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+            NEON_CHECK_HIP(hipStreamSynchronize(s_));
+        }
+        else
+            throw status_error(NEON_BAD_ARGUMENT, "no tangent has been assembled");
+    }
+    if (f_int) NEON_CHECK_HIP(hipMemcpyAsync(f_int, fint_.data(), 3ull * N_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
+void device_mesh::download_qp_state(double* sigma, double* D, double* detF, double* eps_p, double* ebar, double* vm)
+{
+    int64_t const Q = 8ll * E_;
+    if (sigma || D)
+    {
+        device_array<double> d36, s9;
+        if (D) d36.resize(36 * Q);
+        if (sigma) s9.resize(9 * Q);
+        expand_tangent_kernel<<<grid_for(Q), kBlock, 0, s_>>>(Q, coef_.data(), sigma_.data(), D ? d36.data() : nullptr,
+                                                               sigma ? s9.data() : nullptr);
+        NEON_CHECK_HIP(hipGetLastError());
+        if (D) NEON_CHECK_HIP(hipMemcpyAsync(D, d36.data(), 36 * Q * sizeof(double), hipMemcpyDeviceToHost, s_));
+        if (sigma) NEON_CHECK_HIP(hipMemcpyAsync(sigma, s9.data(), 9 * Q * sizeof(double), hipMemcpyDeviceToHost, s_));
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    }
+    if (detF) NEON_CHECK_HIP(hipMemcpyAsync(detF, detF_.data(), Q * sizeof(double), hipMemcpyDeviceToHost, s_));
+    if (model_ == NEON_MODEL_J2)
+    {
+        if (eps_p)
+        {
+            std::vector<double> e6(6 * Q);
+            NEON_CHECK_HIP(hipMemcpyAsync(e6.data(), eps_p_.data(), 6 * Q * sizeof(double), hipMemcpyDeviceToHost, s_));
+            NEON_CHECK_HIP(hipStreamSynchronize(s_));
+            for (int64_t q = 0; q < Q; ++q)
+            {
+                double* o = eps_p + 9 * q;
+                o[0] = e6[0 * Q + q];
+                o[4] = e6[1 * Q + q];
+                o[8] = e6[2 * Q + q];
+                o[5] = o[7] = e6[3 * Q + q];
+                o[2] = o[6] = e6[4 * Q + q];
+                o[1] = o[3] = e6[5 * Q + q];
+            }
+        }
+        if (ebar) NEON_CHECK_HIP(hipMemcpyAsync(ebar, ebar_.data(), Q * sizeof(double), hipMemcpyDeviceToHost, s_));
+        if (vm) NEON_CHECK_HIP(hipMemcpyAsync(vm, vm_.data(), Q * sizeof(double), hipMemcpyDeviceToHost, s_));
+    }
+    else
+    {
+        if (eps_p) std::fill(eps_p, eps_p + 9 * Q, 0.0);
+        if (ebar) std::fill(ebar, ebar + Q, 0.0);
+        if (vm) std::fill(vm, vm + Q, 0.0);
+    }
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
+linear_operator device_mesh::op(int precision) const
+{
+    linear_operator A;
+    A.n = 3ll * N_;
+    A.node_block = true;
+    A.nodes = N_;
+    A.bptr = bptr_.data();
+    A.bcol = bcol_.data();
+    if (precision == NEON_PRECISION_FP64)
+    {
+        if (!k64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP64 tangent assembled");
+        A.v64 = k64_.data();
+    }
+    else
+    {
+        if (!k32_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP32 tangent assembled");
+        A.v32 = k32_.data();
+    }
+    return A;
+}
+
+namespace
+{
+template <typename VT, typename OT>
+__global__ __launch_bounds__(kBlock) void nb_jacobi_kernel(int32_t N, const int32_t* __restrict__ bptr,
+                                                           const int32_t* __restrict__ self,
+                                                           const VT* __restrict__ val, OT* __restrict__ minv,
+                                                           int eigen_rule)
+{
+    int64_t const r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= 3ll * N) return;
+    int64_t const nd = r / 3;
+    int const i = int(r % 3);
+    int32_t const b0 = bptr[nd];
+    int64_t const d = bptr[nd + 1] - b0;
+    double const v = double(val[9ll * b0 + 3ll * i * d + 3ll * self[nd] + i]);
+    double const inv = eigen_rule ? (v != 0.0 ? 1.0 / v : 1.0) : 1.0 / v;
+    minv[r] = OT(inv);
+}
+} // namespace
+
+void device_mesh::jacobi(double* minv, int eigen_rule, int precision)
+{
+    if (precision == NEON_PRECISION_FP64)
+        nb_jacobi_kernel<double, double><<<grid_for(3ll * N_), kBlock, 0, s_>>>(N_, bptr_.data(), self_slot_.data(),
+                                                                                  k64_.data(), minv, eigen_rule);
+    else
+        nb_jacobi_kernel<float, double><<<grid_for(3ll * N_), kBlock, 0, s_>>>(N_, bptr_.data(), self_slot_.data(),
+                                                                                 k32_.data(), minv, eigen_rule);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+void device_mesh::parameter_gradient(const double*, const double*, double, double*)
+{
+    throw status_error(NEON_BAD_ARGUMENT, "adjoint is not available in this build");
+}
+} // namespace neon_amd

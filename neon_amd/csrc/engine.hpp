@@ -1,0 +1,242 @@
+// Host-side engine of the MI355X implicit Newton step: device buffers, the
+// device-resident mesh/constitutive state/tangent, and the Krylov drivers.  Everything
+// here is called by the C ABI in capi.cpp and by the Newton driver in newton.cpp.
+#pragma once
+
+#include "common.hpp"
+#include "neon_hip.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace neon_amd
+{
+struct krylov_state;
+
+template <typename T>
+class device_array
+{
+public:
+    device_array() = default;
+    device_array(device_array const&) = delete;
+    device_array& operator=(device_array const&) = delete;
+    ~device_array() { release(); }
+
+    void resize(size_t n)
+    {
+        if (n == count) return;
+        release();
+        if (n)
+        {
+            hipError_t const e = hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T));
+            if (e != hipSuccess)
+            {
+                ptr = nullptr;
+                throw status_error(e == hipErrorOutOfMemory ? NEON_OOM : NEON_HIP_ERROR,
+                                   "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+            }
+        }
+        count = n;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    T* data() const { return ptr; }
+    size_t size() const { return count; }
+
+private:
+    T* ptr = nullptr;
+    size_t count = 0;
+};
+
+// Event pairs around selected launches on one stream; elapsed times are summed per kernel
+// class once the stream has been synchronised.
+class kernel_profiler
+{
+public:
+    ~kernel_profiler();
+    bool enabled = false;
+    // returns a token (< 0 when not recording)
+    int begin(hipStream_t s, int cls);
+    void end(hipStream_t s, int token);
+    // accumulate pairs [0, pairs) whose launch ordinal < valid_launches (all if < 0), then clear
+    void collect(int64_t valid_launches = -1);
+    void read(int cls, bool reset, int64_t* count, double* ms);
+    void discard() { pairs_ = 0; }
+    void add(int cls, double ms)
+    {
+        if (!enabled) return;
+        count_[cls] += 1;
+        ms_[cls] += ms;
+    }
+
+private:
+    static constexpr int kMaxPairs = 16384;
+    std::vector<hipEvent_t> events_;
+    std::vector<int> cls_;
+    int pairs_ = 0;
+    int64_t count_[NEON_KERNEL_CLASSES] = {};
+    double ms_[NEON_KERNEL_CLASSES] = {};
+};
+
+struct solve_result
+{
+    int status = NEON_OK;
+    int32_t iterations = 0;
+    double error = 0.0;
+    int64_t spmv = 0;
+    int64_t refinements = 0;
+};
+
+// y = A x for a square matrix held on the device in one of two index layouts.
+struct linear_operator
+{
+    int64_t n = 0;
+    bool node_block = false;
+    // node-block layout
+    int32_t nodes = 0;
+    const int32_t* bptr = nullptr;
+    const int32_t* bcol = nullptr;
+    // generic CSR layout
+    const int32_t* rp = nullptr;
+    const int32_t* ci = nullptr;
+    int lanes_per_row = 8;
+    // values
+    const double* v64 = nullptr;
+    const float* v32 = nullptr;
+
+    int spmv_blocks() const;
+    // FP64: y = A x (values v64); with part, also block partials of x.y (or w.y if w).
+    void apply64(hipStream_t s, const double* x, double* y, double* part, const krylov_state* st) const;
+    // FP32 values/vectors, FP32 accumulation, FP64 partials of x.y
+    void apply32(hipStream_t s, const float* x, float* y, double* part, const krylov_state* st) const;
+    // r = b - A x in FP64 arithmetic from the stored values (v64 if set, else v32)
+    // (skipped when st is given and the state is done, or, with gate, no reliable update is due)
+    void residual64(hipStream_t s, const double* b, const double* x, double* r, const krylov_state* st = nullptr,
+                    int gate = 0) const;
+};
+
+class krylov_engine
+{
+public:
+    explicit krylov_engine(hipStream_t s);
+    ~krylov_engine();
+
+    // Jacobi inverse of the operator's diagonal (FP64 values).
+    void jacobi64(const linear_operator& A, double* minv, int eigen_rule);
+
+    solve_result cg64(const linear_operator& A, const double* b, double* x, const double* minv, int criterion,
+                      double tol, int32_t max_it);
+    solve_result bicgstab64(const linear_operator& A, const double* b, double* x, const double* minv,
+                            int criterion, double tol, int32_t max_it);
+    // FP32 inner CG with FP64 reliable updates r = b - A x (relative rule only).
+    solve_result cg_mixed(const linear_operator& A, const double* b, double* x, int criterion, double tol,
+                          int32_t max_it);
+    // same, for a node-block operator whose Jacobi inverse (FP64, from the FP32 diagonal) is given
+    solve_result cg_mixed_nb(const linear_operator& A, const double* b, double* x, const double* minv, double tol,
+                             int32_t max_it);
+
+    double dot(int64_t n, const double* a, const double* b);
+
+    hipStream_t stream() const { return s_; }
+    int vec_blocks(int64_t n) const;
+    kernel_profiler* prof = nullptr;
+    // iterations between convergence polls from the host
+    int poll_every = 16;
+    // reliable-update trigger of the mixed-precision CG: ||r32|| < delta ||r at last update||
+    double reliable_delta = 0.1;
+
+private:
+    void ensure(int64_t n);
+    void ensure_mixed(int64_t n);
+    solve_result cg_mixed_run(const linear_operator& A, const double* b, double* x, double tol, int32_t max_it, int nb,
+                              int ns, double* part_pAp, double* part_rr, double* part_rz, double* part_rr_t,
+                              double* part_rz_t, krylov_state* st, float* minv);
+    hipStream_t s_;
+    device_array<double> part_[5];
+    device_array<krylov_state> state_;
+    krylov_state* host_state_ = nullptr;
+    double* host_scalar_ = nullptr;
+    device_array<double> scalar_;
+    int64_t n_ = 0, n_mixed_ = 0;
+    device_array<double> r_, p_, Ap_;
+    device_array<double> r0_, v_, s_vec_, t_, ph_, sh_;
+    device_array<float> r32_, p32_, Ap32_, x32_, minv32_;
+    device_array<double> x64_, r64_;
+};
+
+// Device-resident mesh: geometry, constitutive state, pattern, tangent and f_int.
+class device_mesh
+{
+public:
+    device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn, int32_t model,
+                const double* params, int32_t tangent_mode);
+
+    int32_t nodes() const { return N_; }
+    int32_t elements() const { return E_; }
+    int64_t dofs() const { return 3ll * N_; }
+    int64_t nnz() const { return nnz_; }
+    int32_t model() const { return model_; }
+    int32_t colours() const { return int32_t(colour_offsets_.size()) - 1; }
+
+    void set_parameters(const double* params);
+    void set_displacement_host(const double* u);
+    // u (device) -> state; returns NEON_OK or the recoverable code
+    int update_internal_variables(const double* u_dev);
+    void save_internal_variables(bool converged);
+    void assemble(const double* u_dev, int precision);
+    void enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision);
+    void pattern(int32_t* row_ptr, int32_t* col) const;
+    void download(double* val, double* f_int, int precision_hint);
+    void download_qp_state(double* sigma, double* D, double* detF, double* eps_p, double* ebar, double* vm);
+    // operator over the assembled tangent (FP64 values, or FP32 values for mixed)
+    linear_operator op(int precision) const;
+    // Jacobi inverse from the node-block diagonal
+    void jacobi(double* minv, int eigen_rule, int precision);
+    // dJ/dp per element for p = shear modulus with bulk = ratio * shear (adjoint)
+    void parameter_gradient(const double* u_dev, const double* lambda_dev, double ratio, double* grad_dev);
+
+    double* u() { return u_.data(); }
+    double* f_int() { return fint_.data(); }
+    hipStream_t stream() const { return s_; }
+    bool has_k64() const { return k64_valid_; }
+    bool has_k32() const { return k32_valid_; }
+
+    // last assemble time split (ms) measured with events
+    float last_assemble_ms = 0.0f;
+    kernel_profiler* prof = nullptr;
+
+private:
+    hipStream_t s_;
+    int32_t N_, E_, model_, tangent_mode_;
+    int64_t nnz_ = 0;
+    std::vector<int32_t> bptr_h_, bcol_h_;
+    std::vector<int32_t> colour_offsets_;
+    device_array<double> X_, u_, params_, fint_;
+    device_array<int32_t> conn_, elems_, bptr_, bcol_, self_slot_;
+    device_array<uint8_t> slots_;
+    device_array<double> k64_;
+    device_array<float> k32_;
+    bool k64_valid_ = false, k32_valid_ = false;
+    device_array<double> sigma_, coef_, detF_, eps_p_, ebar_, vm_, eps_p_old_, ebar_old_;
+    device_array<uint32_t> status_;
+    uint32_t* status_host_ = nullptr;
+};
+
+void upload_hex8_tables();
+void convert_to_float(hipStream_t s, int64_t n, const double* x, float* y);
+} // namespace neon_amd
+
+struct neon_context
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    neon_amd::krylov_engine* engine = nullptr;
+    neon_amd::device_mesh* mesh = nullptr;
+    neon_amd::kernel_profiler profiler;
+};

@@ -1,0 +1,625 @@
+// Device kernels of the Jacobi-preconditioned Krylov solvers: CSR SpMV in two layouts,
+// fused vector updates with grid reductions, and the device-resident scalar state.
+//
+// The algorithms follow neon's two solver families:
+//   * Eigen's ConjugateGradient behind conjugate_gradient::solve
+//     (src/solver/linear/linear_solver.cpp:67-109): relative stop ||r||^2 < tol^2 ||b||^2,
+//   * conjugate_gradient_cuda::solve (src/solver/linear/conjugate_gradient_cuda.cpp:95-186):
+//     absolute stop sqrt(r.z) <= tol,
+// with every scalar kept on the device (the CUDA code synchronises on each cuBLAS dot).
+#pragma once
+
+#include "common.hpp"
+#include "neon_hip.h"
+
+#include <cfloat>
+
+namespace neon_amd
+{
+// Scalars of one Krylov iteration, ping-ponged between two device slots so that the
+// kernel that computes iteration k+1's scalars never overwrites what others still read.
+struct krylov_state
+{
+    double rz;        // r . z  (Eigen's absNew, CUDA's "residual")
+    double rr;        // r . r
+    double rhs2;      // b . b
+    double threshold; // Eigen: max(tol^2 ||b||^2, DBL_MIN)
+    double alpha, beta, omega, rho; // BiCGSTAB scalars (alpha reused by CG for reports)
+    double aux0, aux1;               // reliable-update bookkeeping (mixed precision)
+    int32_t it;   // iterations as the reference counts them
+    int32_t done; // 0 running, 1 converged, 2 iteration limit, 3 non-finite, 4 breakdown
+    int32_t restarts;
+    int32_t refine; // mixed precision: a reliable update is due this iteration
+};
+
+enum : int32_t
+{
+    KRYLOV_RUNNING = 0,
+    KRYLOV_CONVERGED = 1,
+    KRYLOV_LIMIT = 2,
+    KRYLOV_NONFINITE = 3,
+    KRYLOV_BREAKDOWN = 4,
+    KRYLOV_REFINE = 5, // mixed precision: a reliable update is due
+};
+
+// ----------------------------------------------------------------- SpMV layouts
+// Node-block CSR: the tangent of a 3-dof-per-node mesh stores its values in ordinary
+// row-major CSR order (sorted columns), but every dof row of node n has the same
+// column set {3m+j : m in adj(n), j < 3}.  The kernel therefore reads one int32 block
+// column per 9 values instead of one per value:
+//   row 3n+i, block s (node m = bcol[bptr[n]+s]), column 3m+j
+//   -> val[9 bptr[n] + i (3 deg n) + 3 s + j].
+struct nb_matrix
+{
+    int32_t nodes;
+    const int32_t* bptr;
+    const int32_t* bcol;
+};
+
+struct csr_matrix
+{
+    int32_t rows;
+    const int32_t* rp;
+    const int32_t* ci;
+};
+
+// 32 lanes per node row; y receives the three dof rows, reduced across the 32 lanes.
+template <typename VT, typename XT, typename AT>
+__device__ inline void nb_row(nb_matrix A, const VT* __restrict__ val, const XT* __restrict__ x, int32_t n,
+                              int lane, AT& y0, AT& y1, AT& y2)
+{
+    int32_t const b0 = A.bptr[n];
+    int32_t const d = A.bptr[n + 1] - b0;
+    int64_t const base = 9ll * b0;
+    int64_t const rs = 3ll * d;
+    AT s0 = 0, s1 = 0, s2 = 0;
+    for (int s = lane; s < d; s += 32)
+    {
+        int64_t const m = A.bcol[b0 + s];
+        AT const x0 = AT(x[3 * m]), x1 = AT(x[3 * m + 1]), x2 = AT(x[3 * m + 2]);
+        const VT* v = val + base + 3 * s;
+        s0 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
+        v += rs;
+        s1 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
+        v += rs;
+        s2 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
+    }
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1)
+    {
+        s0 += __shfl_xor(s0, off, 32);
+        s1 += __shfl_xor(s1, off, 32);
+        s2 += __shfl_xor(s2, off, 32);
+    }
+    y0 = s0;
+    y1 = s1;
+    y2 = s2;
+}
+
+// y = A x (node-block layout); with DOT, also writes the block partial of x . y.
+// MODE 0: y = Ax.  MODE 1: y = b - Ax (residual; b given in `bvec`).
+template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* __restrict__ val,
+                                                         const XT* __restrict__ x, YT* __restrict__ y,
+                                                         const double* __restrict__ bvec,
+                                                         double* __restrict__ part,
+                                                         const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate && !st->refine))) return;
+    int64_t beg, end;
+    xcd_chunk(A.nodes, beg, end);
+    int const g = threadIdx.x >> 5, l = threadIdx.x & 31;
+    double dot = 0.0;
+    for (int64_t n = beg + g; n < end; n += kBlock / 32)
+    {
+        AT y0, y1, y2;
+        nb_row<VT, XT, AT>(A, val, x, int32_t(n), l, y0, y1, y2);
+        if (l < 3)
+        {
+            AT const yi = l == 0 ? y0 : (l == 1 ? y1 : y2);
+            int64_t const r = 3 * n + l;
+            if constexpr (MODE == 1)
+                y[r] = YT(bvec[r] - double(yi));
+            else
+                y[r] = YT(yi);
+            if constexpr (DOT) dot += double(x[r]) * double(YT(yi));
+        }
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<kBlock>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
+// Generic CSR (any sparsity): LPR lanes per row.
+template <int LPR, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(kBlock) void spmv_csr_kernel(csr_matrix A, const VT* __restrict__ val,
+                                                          const XT* __restrict__ x, YT* __restrict__ y,
+                                                          const double* __restrict__ bvec,
+                                                          double* __restrict__ part,
+                                                          const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate && !st->refine))) return;
+    int64_t beg, end;
+    xcd_chunk(A.rows, beg, end);
+    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    double dot = 0.0;
+    for (int64_t r = beg + g; r < end; r += kBlock / LPR)
+    {
+        AT s = 0;
+        for (int32_t p = A.rp[r] + l; p < A.rp[r + 1]; p += LPR) s += AT(val[p]) * AT(x[A.ci[p]]);
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+        if (l == 0)
+        {
+            if constexpr (MODE == 1)
+                y[r] = YT(bvec[r] - double(s));
+            else
+                y[r] = YT(s);
+            if constexpr (DOT) dot += double(x[r]) * double(YT(s));
+        }
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<kBlock>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
+// Jacobi inverse from the CSR diagonal.  Eigen's DiagonalPreconditioner uses 1 where the
+// diagonal is zero or missing; the CUDA solver inverts blindly (cwiseInverse).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void jacobi_kernel(csr_matrix A, const double* __restrict__ val,
+                                                        T* __restrict__ minv, int eigen_rule)
+{
+    int64_t const r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= A.rows) return;
+    int32_t lo = A.rp[r], hi = A.rp[r + 1] - 1;
+    double d = 0.0;
+    bool found = false;
+    while (lo <= hi)
+    {
+        int32_t const mid = (lo + hi) >> 1;
+        int32_t const c = A.ci[mid];
+        if (c == r)
+        {
+            d = val[mid];
+            found = true;
+            break;
+        }
+        if (c < r)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    double inv;
+    if (eigen_rule)
+        inv = (found && d != 0.0) ? 1.0 / d : 1.0;
+    else
+        inv = 1.0 / d;
+    minv[r] = T(inv);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void jacobi32_kernel(csr_matrix A, const float* __restrict__ val,
+                                                          T* __restrict__ minv, int eigen_rule)
+{
+    int64_t const r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= A.rows) return;
+    double d = 0.0;
+    bool found = false;
+    for (int32_t p = A.rp[r]; p < A.rp[r + 1]; ++p)
+        if (A.ci[p] == r)
+        {
+            d = double(val[p]);
+            found = true;
+            break;
+        }
+    double const inv = eigen_rule ? ((found && d != 0.0) ? 1.0 / d : 1.0) : 1.0 / d;
+    minv[r] = T(inv);
+}
+
+// ------------------------------------------------------------------ CG (FP64)
+// init: x = 0, r = b, p = M^-1 r; partials of r.r and r.z
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t n, double* __restrict__ x, double* __restrict__ r,
+                                                         double* __restrict__ p, const double* __restrict__ b,
+                                                         const double* __restrict__ minv,
+                                                         double* __restrict__ part_rr,
+                                                         double* __restrict__ part_rz)
+{
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        double const ri = b[i];
+        double const zi = minv[i] * ri;
+        x[i] = 0.0;
+        r[i] = ri;
+        p[i] = zi;
+        rr += ri * ri;
+        rz += ri * zi;
+    }
+    rr = block_sum<kBlock>(rr);
+    rz = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0)
+    {
+        part_rr[blockIdx.x] = rr;
+        part_rz[blockIdx.x] = rz;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void cg_init_state_kernel(const double* __restrict__ part_rr,
+                                                               const double* __restrict__ part_rz, int npart,
+                                                               krylov_state* __restrict__ st, int criterion,
+                                                               double tol, int max_it)
+{
+    double const rr = reduce_partials<kBlock>(part_rr, npart);
+    double const rz = reduce_partials<kBlock>(part_rz, npart);
+    if (threadIdx.x != 0) return;
+    krylov_state s{};
+    s.rz = rz;
+    s.rr = rr;
+    s.rhs2 = rr;
+    s.it = 0;
+    s.done = KRYLOV_RUNNING;
+    if (criterion == NEON_STOP_RELATIVE)
+    {
+        s.threshold = fmax(tol * tol * rr, DBL_MIN);
+        if (rr == 0.0 || rr < s.threshold) s.done = KRYLOV_CONVERGED;
+    }
+    else
+    {
+        s.threshold = 0.0;
+        if (!(sqrt(rz) > tol)) s.done = KRYLOV_CONVERGED;
+    }
+    if (s.done == KRYLOV_RUNNING && max_it <= 0) s.done = KRYLOV_LIMIT;
+    if (!isfinite(rr) || !isfinite(rz)) s.done = KRYLOV_NONFINITE;
+    *st = s;
+}
+
+// x += alpha p, r -= alpha Ap with alpha = rz / (p.Ap); partials of r.r and r.z
+__global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double* __restrict__ x,
+                                                              double* __restrict__ r,
+                                                              const double* __restrict__ p,
+                                                              const double* __restrict__ Ap,
+                                                              const double* __restrict__ minv,
+                                                              const double* __restrict__ part_pAp, int npart,
+                                                              const krylov_state* __restrict__ st,
+                                                              double* __restrict__ part_rr,
+                                                              double* __restrict__ part_rz)
+{
+    if (st->done) return;
+    double const pAp = reduce_partials<kBlock>(part_pAp, npart);
+    double const alpha = st->rz / pAp;
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        x[i] += alpha * p[i];
+        double const ri = r[i] - alpha * Ap[i];
+        r[i] = ri;
+        double const zi = minv[i] * ri;
+        rr += ri * ri;
+        rz += ri * zi;
+    }
+    rr = block_sum<kBlock>(rr);
+    rz = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0)
+    {
+        part_rr[blockIdx.x] = rr;
+        part_rz[blockIdx.x] = rz;
+    }
+}
+
+// Convergence decision, beta, p = z + beta p; block 0 writes the next scalar state.
+__global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* __restrict__ p,
+                                                             const double* __restrict__ r,
+                                                             const double* __restrict__ minv,
+                                                             const double* __restrict__ part_rr,
+                                                             const double* __restrict__ part_rz, int npart,
+                                                             const krylov_state* __restrict__ st,
+                                                             krylov_state* __restrict__ st_next, int criterion,
+                                                             double tol, int max_it)
+{
+    krylov_state s = *st;
+    if (s.done)
+    {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
+        return;
+    }
+    double const rr = reduce_partials<kBlock>(part_rr, npart);
+    double const rz = reduce_partials<kBlock>(part_rz, npart);
+    int32_t it_new;
+    int32_t done = KRYLOV_RUNNING;
+    if (criterion == NEON_STOP_RELATIVE)
+    {
+        bool const conv = rr < s.threshold; // Eigen breaks before counting the iteration
+        it_new = conv ? s.it : s.it + 1;
+        if (conv)
+            done = KRYLOV_CONVERGED;
+        else if (it_new >= max_it)
+            done = KRYLOV_LIMIT;
+    }
+    else
+    {
+        it_new = s.it + 1; // the CUDA loop counts at the top
+        if (it_new >= max_it)
+            done = KRYLOV_LIMIT;
+        else if (!(sqrt(rz) > tol))
+            done = KRYLOV_CONVERGED;
+    }
+    if (!isfinite(rr) || !isfinite(rz)) done = KRYLOV_NONFINITE;
+    double const beta = rz / s.rz;
+    if (done == KRYLOV_RUNNING)
+    {
+        int64_t beg, end;
+        xcd_chunk(n, beg, end);
+        for (int64_t i = beg + threadIdx.x; i < end; i += kBlock) p[i] = minv[i] * r[i] + beta * p[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        s.rr = rr;
+        s.rz = rz;
+        s.beta = beta;
+        s.it = it_new;
+        s.done = done;
+        *st_next = s;
+    }
+}
+
+// ------------------------------------------------------- generic vector kernels
+__global__ __launch_bounds__(kBlock) void dot_kernel(int64_t n, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* __restrict__ part)
+{
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double s = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock) s += a[i] * b[i];
+    s = block_sum<kBlock>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void reduce_to_kernel(const double* __restrict__ part, int npart,
+                                                           double* __restrict__ out)
+{
+    double const s = reduce_partials<kBlock>(part, npart);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// y = a x + b y
+__global__ __launch_bounds__(kBlock) void axpby_kernel(int64_t n, double a, const double* __restrict__ x, double b,
+                                                       double* __restrict__ y)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// z = x - y
+__global__ __launch_bounds__(kBlock) void sub_kernel(int64_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ z)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) z[i] = x[i] - y[i];
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(kBlock) void convert_kernel(int64_t n, const A* __restrict__ x, B* __restrict__ y)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) y[i] = B(x[i]);
+}
+
+// z = a x + b y
+__global__ __launch_bounds__(kBlock) void lincomb_kernel(int64_t n, double a, const double* __restrict__ x, double b,
+                                                         const double* __restrict__ y, double* __restrict__ z)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) z[i] = a * x[i] + b * y[i];
+}
+
+// z = d o x
+__global__ __launch_bounds__(kBlock) void mul_kernel(int64_t n, const double* __restrict__ d,
+                                                     const double* __restrict__ x, double* __restrict__ z)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) z[i] = d[i] * x[i];
+}
+
+// ------------------------------------------------ mixed precision CG (reliable updates)
+// FP32 recurrence (x32, r32, p32, FP32 tangent values, FP32-accumulated SpMV) with FP64
+// dot products, and FP64 reliable updates: when the recurrence residual has dropped by
+// `delta` since the last update (or meets the threshold), x64 += x32, r = b - K x64 is
+// recomputed in FP64 arithmetic and replaces r32; the search direction is kept.
+// Convergence is decided only on the true FP64 residual.
+__global__ __launch_bounds__(kBlock) void mixed_init_kernel(int64_t n, const double* __restrict__ b,
+                                                            double* __restrict__ x64, float* __restrict__ r32,
+                                                            float* __restrict__ x32, float* __restrict__ p32,
+                                                            const float* __restrict__ minv,
+                                                            double* __restrict__ part_rr,
+                                                            double* __restrict__ part_rz)
+{
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        double const bi = b[i];
+        float const ri = float(bi);
+        float const zi = minv[i] * ri;
+        x64[i] = 0.0;
+        x32[i] = 0.0f;
+        r32[i] = ri;
+        p32[i] = zi;
+        rr += bi * bi;
+        rz += double(ri) * double(zi);
+    }
+    rr = block_sum<kBlock>(rr);
+    rz = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0)
+    {
+        part_rr[blockIdx.x] = rr;
+        part_rz[blockIdx.x] = rz;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void mixed_init_state_kernel(const double* __restrict__ part_rr,
+                                                                  const double* __restrict__ part_rz, int npart,
+                                                                  krylov_state* __restrict__ st, double tol,
+                                                                  int max_it)
+{
+    double const rr = reduce_partials<kBlock>(part_rr, npart);
+    double const rz = reduce_partials<kBlock>(part_rz, npart);
+    if (threadIdx.x != 0) return;
+    krylov_state s{};
+    s.rz = rz;
+    s.rr = rr;
+    s.rhs2 = rr;
+    s.threshold = fmax(tol * tol * rr, DBL_MIN);
+    s.aux0 = sqrt(rr);
+    s.done = (rr == 0.0 || rr < s.threshold) ? KRYLOV_CONVERGED : KRYLOV_RUNNING;
+    if (s.done == KRYLOV_RUNNING && max_it <= 0) s.done = KRYLOV_LIMIT;
+    if (!isfinite(rr) || !isfinite(rz)) s.done = KRYLOV_NONFINITE;
+    *st = s;
+}
+
+__global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, float* __restrict__ x32,
+                                                                 float* __restrict__ r32,
+                                                                 const float* __restrict__ p32,
+                                                                 const float* __restrict__ Ap32,
+                                                                 const float* __restrict__ minv,
+                                                                 const double* __restrict__ part_pAp, int npart,
+                                                                 const krylov_state* __restrict__ st,
+                                                                 double* __restrict__ part_rr,
+                                                                 double* __restrict__ part_rz)
+{
+    if (st->done) return;
+    double const pAp = reduce_partials<kBlock>(part_pAp, npart);
+    float const alpha = float(st->rz / pAp);
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        x32[i] += alpha * p32[i];
+        float const ri = r32[i] - alpha * Ap32[i];
+        r32[i] = ri;
+        float const zi = minv[i] * ri;
+        rr += double(ri) * double(ri);
+        rz += double(ri) * double(zi);
+    }
+    rr = block_sum<kBlock>(rr);
+    rz = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0)
+    {
+        part_rr[blockIdx.x] = rr;
+        part_rz[blockIdx.x] = rz;
+    }
+}
+
+// single block: decide whether this iteration ends with a reliable update
+__global__ __launch_bounds__(kBlock) void mixed_decide_kernel(const double* __restrict__ part_rr,
+                                                              const double* __restrict__ part_rz, int npart,
+                                                              krylov_state* __restrict__ st, double delta)
+{
+    if (st->done) return;
+    double const rr = reduce_partials<kBlock>(part_rr, npart);
+    double const rz = reduce_partials<kBlock>(part_rz, npart);
+    if (threadIdx.x != 0) return;
+    bool const refine = rr < st->threshold || sqrt(rr) < delta * st->aux0 || !isfinite(rr) || !isfinite(rz);
+    st->refine = refine ? 1 : 0;
+    st->rr = rr;
+    st->aux1 = rz;
+}
+
+// x64 += x32, x32 = 0 (when a reliable update is due, or unconditionally with force)
+__global__ __launch_bounds__(kBlock) void mixed_fold_kernel(int64_t n, double* __restrict__ x64,
+                                                            float* __restrict__ x32,
+                                                            const krylov_state* __restrict__ st, int force)
+{
+    if (!force && (st->done || !st->refine)) return;
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n)
+    {
+        x64[i] += double(x32[i]);
+        x32[i] = 0.0f;
+    }
+}
+
+// r32 = float(r64) after the FP64 residual; partials of r64.r64 and r32.z32
+__global__ __launch_bounds__(kBlock) void mixed_refresh_kernel(int64_t n, const double* __restrict__ r64,
+                                                               float* __restrict__ r32,
+                                                               const float* __restrict__ minv,
+                                                               const krylov_state* __restrict__ st,
+                                                               double* __restrict__ part_rr,
+                                                               double* __restrict__ part_rz)
+{
+    if (st->done || !st->refine) return;
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        double const r = r64[i];
+        float const ri = float(r);
+        r32[i] = ri;
+        rr += r * r;
+        rz += double(ri) * double(minv[i] * ri);
+    }
+    rr = block_sum<kBlock>(rr);
+    rz = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0)
+    {
+        part_rr[blockIdx.x] = rr;
+        part_rz[blockIdx.x] = rz;
+    }
+}
+
+// convergence on the true residual, beta, p32 = z + beta p32; block 0 writes the next state
+__global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float* __restrict__ p32,
+                                                                const float* __restrict__ r32,
+                                                                const float* __restrict__ minv,
+                                                                const double* __restrict__ part_rr_true,
+                                                                const double* __restrict__ part_rz_true, int npart,
+                                                                const krylov_state* __restrict__ st,
+                                                                krylov_state* __restrict__ st_next, int max_it)
+{
+    krylov_state s = *st;
+    if (s.done)
+    {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
+        return;
+    }
+    double rr = s.rr, rz = s.aux1;
+    bool conv = false;
+    if (s.refine)
+    {
+        rr = reduce_partials<kBlock>(part_rr_true, npart);
+        rz = reduce_partials<kBlock>(part_rz_true, npart);
+        conv = rr < s.threshold;
+        s.aux0 = sqrt(rr);
+        s.restarts += 1; // counts reliable updates
+    }
+    int32_t const it_new = conv ? s.it : s.it + 1;
+    int32_t done = conv ? KRYLOV_CONVERGED : (it_new >= max_it ? KRYLOV_LIMIT : KRYLOV_RUNNING);
+    if (!isfinite(rr) || !isfinite(rz)) done = KRYLOV_NONFINITE;
+    float const beta = float(rz / s.rz);
+    if (done == KRYLOV_RUNNING)
+    {
+        int64_t beg, end;
+        xcd_chunk(n, beg, end);
+        for (int64_t i = beg + threadIdx.x; i < end; i += kBlock) p32[i] = minv[i] * r32[i] + beta * p32[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        s.rr = rr;
+        s.rz = rz;
+        s.it = it_new;
+        s.done = done;
+        s.refine = 0;
+        *st_next = s;
+    }
+}
+} // namespace neon_amd

@@ -1,0 +1,1340 @@
+// CPU FP64 oracle for the hex8 implicit Newton step.  TEST INFRASTRUCTURE ONLY:
+// the checker for the HIP path and the CPU baseline of the benchmark, never the
+// product path.  See neon_oracle.h for the reference files each part restates.
+#include "neon_oracle.h"
+
+#include <algorithm>
+#include <cfenv>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <queue>
+#include <vector>
+
+#include <omp.h>
+
+namespace
+{
+using std::vector;
+
+// ---------------------------------------------------------------- 3x3 helpers
+// Row-major 3x3.
+inline double det3(const double* a)
+{
+    return a[0] * (a[4] * a[8] - a[7] * a[5]) - a[3] * (a[1] * a[8] - a[7] * a[2])
+           + a[6] * (a[1] * a[5] - a[4] * a[2]);
+}
+
+// Inverse by cofactors (the method Eigen uses for fixed 3x3 matrices).
+inline void inv3(const double* a, double* r)
+{
+    double const c00 = a[4] * a[8] - a[5] * a[7];
+    double const c01 = a[5] * a[6] - a[3] * a[8];
+    double const c02 = a[3] * a[7] - a[4] * a[6];
+    double const det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double const id = 1.0 / det;
+    r[0] = c00 * id;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    r[3] = c01 * id;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    r[6] = c02 * id;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+inline void matmul3(const double* a, const double* b, double* c)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += a[i * 3 + k] * b[k * 3 + j];
+            c[i * 3 + j] = s;
+        }
+}
+
+// --------------------------------------------------------------- hex8 tables
+// hexahedron8::precompute_shape_functions (src/interpolations/hexahedron.cpp:16-67)
+// at the 8-point rule of src/quadrature/hexahedron/hexahedron_quadrature.cpp:27-38.
+struct hex8
+{
+    double N[8][8];
+    double dN[8][8][3];
+    double w[8];
+    hex8()
+    {
+        static const double node[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                          {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+        double const qp = 1.0 / std::sqrt(3.0);
+        for (int l = 0; l < 8; ++l)
+        {
+            double const xi = node[l][0] * qp, eta = node[l][1] * qp, zeta = node[l][2] * qp;
+            w[l] = 1.0;
+            for (int a = 0; a < 8; ++a)
+            {
+                double const xa = node[a][0], ya = node[a][1], za = node[a][2];
+                N[l][a] = 1.0 / 8.0 * (1.0 + xa * xi) * (1.0 + ya * eta) * (1.0 + za * zeta);
+                dN[l][a][0] = 1.0 / 8.0 * xa * (1.0 + ya * eta) * (1.0 + za * zeta);
+                dN[l][a][1] = 1.0 / 8.0 * (1.0 + xa * xi) * ya * (1.0 + za * zeta);
+                dN[l][a][2] = 1.0 / 8.0 * (1.0 + xa * xi) * (1.0 + ya * eta) * za;
+            }
+        }
+    }
+};
+const hex8& shape()
+{
+    static hex8 h;
+    return h;
+}
+
+// configuration (3x8, column a = node a) * dN (8x3): local_deformation_gradient
+// (src/numeric/deformation_gradient.hpp:15-18)
+inline void local_F(const double* xe /*node-major 8x3*/, const double (*dN)[3], double* J)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            double s = 0.0;
+            for (int a = 0; a < 8; ++a) s += xe[a * 3 + i] * dN[a][j];
+            J[i * 3 + j] = s;
+        }
+}
+
+// ------------------------------------------------------ constitutive updates
+// compressible_neohooke::update_internal_variables
+// (src/constitutive/mechanics/solid/compressible_neohooke.cpp:27-89)
+void neohooke(const double* F, double J, double bulk, double shear, double* sigma, double* D)
+{
+    double const lambda = bulk - 2.0 * shear / 3.0; // isotropic_elastic_property.hpp:28
+    double const lnJ = std::log(J);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            double b = 0.0;
+            for (int k = 0; k < 3; ++k) b += F[i * 3 + k] * F[j * 3 + k];
+            double const I = i == j ? 1.0 : 0.0;
+            sigma[i * 3 + j] = (lambda * lnJ * I + shear * (b - I)) / J;
+        }
+    double const mu = shear - lambda * lnJ;
+    std::fill(D, D + 36, 0.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) D[i * 6 + j] = lambda + (i == j ? 2.0 * mu : 0.0);
+    for (int i = 3; i < 6; ++i) D[i * 6 + i] = mu;
+}
+
+// isotropic_linear_elasticity::elastic_moduli
+// (src/constitutive/mechanics/solid/isotropic_linear_elasticity.cpp:55-67)
+void elastic_moduli(double bulk, double shear, double* C)
+{
+    double const lambda = bulk - 2.0 * shear / 3.0;
+    std::fill(C, C + 36, 0.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C[i * 6 + j] = lambda + (i == j ? 2.0 * shear : 0.0);
+    for (int i = 3; i < 6; ++i) C[i * 6 + i] = shear;
+}
+
+inline void deviatoric3(const double* a, double* d)
+{
+    double const t = (a[0] + a[4] + a[8]) / 3.0;
+    for (int i = 0; i < 9; ++i) d[i] = a[i];
+    d[0] -= t;
+    d[4] -= t;
+    d[8] -= t;
+}
+inline double norm9(const double* a)
+{
+    double s = 0.0;
+    for (int i = 0; i < 9; ++i) s += a[i] * a[i];
+    return std::sqrt(s);
+}
+// von_mises_stress (src/numeric/stress_routines.hpp:19-22)
+inline double von_mises(const double* s)
+{
+    double d[9];
+    deviatoric3(s, d);
+    return std::sqrt(3.0 / 2.0) * norm9(d);
+}
+
+// small_strain_J2_plasticity::update_internal_variables body for one QP
+// (src/constitutive/mechanics/solid/small_strain_J2_plasticity.cpp:45-92), with
+// perform_radial_return (:108-155), evaluate_yield_function (:157-163),
+// algorithmic_tangent (:95-106), isotropic_elastic_plastic::yield_stress (:31-36).
+int j2(const double* H, double bulk, double shear, double sy, double Hh, double* eps_p,
+       double* ebar, double* sigma, double* D, double* vm)
+{
+    double const G = shear;
+    double const lambda = bulk - 2.0 * shear / 3.0;
+    double strain[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) strain[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i]);
+    double ee[9];
+    for (int i = 0; i < 9; ++i) ee[i] = strain[i] - eps_p[i];
+    double const tr = ee[0] + ee[4] + ee[8];
+    for (int i = 0; i < 9; ++i) sigma[i] = 2.0 * G * ee[i];
+    sigma[0] += lambda * tr;
+    sigma[4] += lambda * tr;
+    sigma[8] += lambda * tr;
+    // compute_cauchy_stress = lambda tr(e) I + 2 G e (stress_routines.hpp:31-36); the
+    // order above adds the same two terms.
+    *vm = von_mises(sigma);
+    auto yield = [&](double vm_, double e_, double dg) { return (vm_ - 3.0 * G * dg) - (sy + (e_ + dg) * Hh); };
+    double C[36];
+    elastic_moduli(bulk, shear, C);
+    if (yield(*vm, *ebar, 0.0) <= 0.0)
+    {
+        std::memcpy(D, C, sizeof(C));
+        return ORC_OK;
+    }
+    double const vm_trial = *vm;
+    double dev[9];
+    deviatoric3(sigma, dev);
+    double const ndev = norm9(dev);
+    double normal[9];
+    for (int i = 0; i < 9; ++i) normal[i] = dev[i] / ndev;
+
+    double dg = 0.0;
+    double f = yield(vm_trial, *ebar, 0.0);
+    int it = 0;
+    while (f > 1.0e-6 && it < 50)
+    {
+        dg += f / (3.0 * G + Hh);
+        f = yield(vm_trial, *ebar, dg);
+        ++it;
+    }
+    if (it == 50) return ORC_RETURN_MAP_FAILED;
+
+    double const s32 = std::sqrt(3.0 / 2.0);
+    for (int i = 0; i < 9; ++i) eps_p[i] += dg * s32 * normal[i];
+    for (int i = 0; i < 9; ++i) sigma[i] -= 2.0 * G * dg * s32 * normal[i];
+    *vm = von_mises(sigma);
+    *ebar += dg;
+
+    // C_e - dg 6 G^2 / q_tr I_dev + 6 G^2 (dg/q_tr - 1/(3G+H)) n (x) n
+    static const double Idev[36] = {2.0 / 3, -1.0 / 3, -1.0 / 3, 0, 0, 0, -1.0 / 3, 2.0 / 3, -1.0 / 3, 0, 0, 0,
+                                    -1.0 / 3, -1.0 / 3, 2.0 / 3, 0, 0, 0, 0, 0, 0, 0.5, 0, 0,
+                                    0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0, 0, 0.5};
+    double const nv[6] = {normal[0], normal[4], normal[8], normal[5], normal[2], normal[1]};
+    double const c1 = dg * 6.0 * G * G / vm_trial;
+    double const c2 = 6.0 * G * G * (dg / vm_trial - 1.0 / (3.0 * G + Hh));
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) D[i * 6 + j] = C[i * 6 + j] - c1 * Idev[i * 6 + j] + c2 * nv[i] * nv[j];
+    return ORC_OK;
+}
+} // namespace
+
+// ======================================================================= model
+struct orc_model
+{
+    int32_t N = 0, E = 0;
+    vector<double> X;
+    vector<int32_t> conn;
+    int32_t model = 0;
+    int32_t tangent_mode = 0;
+    vector<double> params; // 4 per element
+    vector<double> x;      // current configuration
+    // per-QP variables (internal_variables); "old" copies for commit/revert of
+    // scalars and second order tensors (src/constitutive/internal_variables.hpp:203-214)
+    vector<double> Hg, F, sigma, detF, eps_p, ebar, vm;
+    vector<double> Hg_o, F_o, sigma_o, detF_o, eps_p_o, ebar_o, vm_o;
+    vector<double> D; // fourth-order tangent, not saved
+};
+
+extern "C" {
+
+int orc_hex8_tables(double* N, double* dN, double* w)
+{
+    hex8 const& h = shape();
+    if (N) std::memcpy(N, h.N, sizeof(h.N));
+    if (dN) std::memcpy(dN, h.dN, sizeof(h.dN));
+    if (w) std::memcpy(w, h.w, sizeof(h.w));
+    return ORC_OK;
+}
+
+int orc_neohooke_qp(const double* F, double bulk, double shear, double* sigma, double* D)
+{
+    neohooke(F, det3(F), bulk, shear, sigma, D);
+    return ORC_OK;
+}
+
+int orc_j2_qp(const double* H, double bulk, double shear, double yield, double hardening, double* eps_p,
+              double* ebar, double* sigma, double* D, double* von_mises_out)
+{
+    return j2(H, bulk, shear, yield, hardening, eps_p, ebar, sigma, D, von_mises_out);
+}
+
+int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn, int32_t model,
+                     const double* params, int32_t tangent_mode, orc_model** out)
+{
+    if (!out || n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params) return ORC_BAD_ARGUMENT;
+    if (model != ORC_MODEL_NEOHOOKE && model != ORC_MODEL_J2) return ORC_BAD_ARGUMENT;
+    for (int64_t i = 0; i < int64_t(n_elem) * 8; ++i)
+        if (conn[i] < 0 || conn[i] >= n_nodes) return ORC_BAD_ARGUMENT;
+    auto* m = new orc_model;
+    m->N = n_nodes;
+    m->E = n_elem;
+    m->X.assign(X, X + 3 * size_t(n_nodes));
+    m->x = m->X;
+    m->conn.assign(conn, conn + 8 * size_t(n_elem));
+    m->model = model;
+    m->tangent_mode = tangent_mode;
+    m->params.assign(params, params + 4 * size_t(n_elem));
+    size_t const Q = size_t(n_elem) * 8;
+    m->Hg.assign(9 * Q, 0.0);
+    m->F.assign(9 * Q, 0.0);
+    for (size_t q = 0; q < Q; ++q) m->F[9 * q] = m->F[9 * q + 4] = m->F[9 * q + 8] = 1.0; // submesh.cpp:45-47
+    m->sigma.assign(9 * Q, 0.0);
+    m->detF.assign(Q, 1.0);
+    m->eps_p.assign(9 * Q, 0.0);
+    m->ebar.assign(Q, 0.0);
+    m->vm.assign(Q, 0.0);
+    m->D.assign(36 * Q, 0.0);
+    if (model == ORC_MODEL_J2) // isotropic_linear_elasticity.cpp:22-23
+        for (size_t q = 0; q < Q; ++q)
+            elastic_moduli(m->params[4 * (q / 8)], m->params[4 * (q / 8) + 1], &m->D[36 * q]);
+    m->Hg_o = m->Hg;
+    m->F_o = m->F;
+    m->sigma_o = m->sigma;
+    m->detF_o = m->detF;
+    m->eps_p_o = m->eps_p;
+    m->ebar_o = m->ebar;
+    m->vm_o = m->vm;
+    *out = m;
+    return ORC_OK;
+}
+
+int orc_model_destroy(orc_model* m)
+{
+    delete m;
+    return ORC_OK;
+}
+
+int orc_model_n_dofs(const orc_model* m) { return 3 * m->N; }
+
+// mesh::update_internal_variables (src/mesh/mechanics/solid/mesh.cpp:57-71) ->
+// material_coordinates::update_current_configuration (src/mesh/material_coordinates.cpp:23-27)
+// -> submesh::update_internal_variables (submesh.cpp:194-208).
+int orc_update_internal_variables(orc_model* m, const double* u, double /*dt*/)
+{
+    std::feclearexcept(FE_ALL_EXCEPT);
+    for (int64_t i = 0; i < 3 * int64_t(m->N); ++i) m->x[i] = m->X[i] + u[i];
+    hex8 const& h = shape();
+
+    // update_deformation_measures (submesh.cpp:210-237)
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < m->E; ++e)
+    {
+        double Xe[24], xe[24];
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i)
+            {
+                Xe[a * 3 + i] = m->X[3 * size_t(m->conn[8 * e + a]) + i];
+                xe[a * 3 + i] = m->x[3 * size_t(m->conn[8 * e + a]) + i];
+            }
+        for (int l = 0; l < 8; ++l)
+        {
+            double F0[9], Fx[9], F0i[9];
+            local_F(Xe, h.dN[l], F0);
+            local_F(xe, h.dN[l], Fx);
+            inv3(F0, F0i);
+            double B0t[8][3]; // rhea * F_0^{-1}
+            for (int a = 0; a < 8; ++a)
+                for (int j = 0; j < 3; ++j)
+                {
+                    double s = 0.0;
+                    for (int k = 0; k < 3; ++k) s += h.dN[l][a][k] * F0i[k * 3 + j];
+                    B0t[a][j] = s;
+                }
+            double* H = &m->Hg[9 * (8 * e + l)];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                {
+                    double s = 0.0;
+                    for (int a = 0; a < 8; ++a) s += (xe[a * 3 + i] - Xe[a * 3 + i]) * B0t[a][j];
+                    H[i * 3 + j] = s;
+                }
+            matmul3(Fx, F0i, &m->F[9 * (8 * e + l)]);
+        }
+    }
+    // update_Jacobian_determinants (submesh.cpp:239-269)
+    size_t const Q = size_t(m->E) * 8;
+    for (size_t q = 0; q < Q; ++q) m->detF[q] = det3(&m->F[9 * q]);
+    for (size_t q = 0; q < Q; ++q)
+        if (std::signbit(m->detF[q])) return ORC_NEGATIVE_JACOBIAN;
+
+    int status = ORC_OK;
+    if (m->model == ORC_MODEL_NEOHOOKE)
+    {
+        for (size_t q = 0; q < Q; ++q)
+        {
+            double const* p = &m->params[4 * (q / 8)];
+            neohooke(&m->F[9 * q], m->detF[q], p[0], p[1], &m->sigma[9 * q], &m->D[36 * q]);
+            if (m->tangent_mode == ORC_TANGENT_CONSISTENT)
+                for (int k = 0; k < 36; ++k) m->D[36 * q + k] /= m->detF[q];
+        }
+    }
+    else
+    {
+        int failed = 0;
+#pragma omp parallel for schedule(static) reduction(| : failed)
+        for (int64_t q = 0; q < int64_t(Q); ++q)
+        {
+            double const* p = &m->params[4 * (q / 8)];
+            int const s = j2(&m->Hg[9 * q], p[0], p[1], p[2], p[3], &m->eps_p[9 * q], &m->ebar[q],
+                             &m->sigma[9 * q], &m->D[36 * q], &m->vm[q]);
+            if (s != ORC_OK) failed |= 1;
+        }
+        if (failed) status = ORC_RETURN_MAP_FAILED;
+    }
+    if (status != ORC_OK) return status;
+    // FE_INVALID test of submesh.cpp:204-207.  The per-QP loops run on worker threads whose
+    // floating-point flags the calling thread cannot see, so non-finite results are scanned.
+    if (std::fetestexcept(FE_INVALID)) return ORC_NAN_OR_INF;
+    for (size_t k = 0; k < 9 * Q; ++k)
+        if (!std::isfinite(m->sigma[k])) return ORC_NAN_OR_INF;
+    return ORC_OK;
+}
+
+// submesh::save_internal_variables (submesh.cpp:54-64)
+int orc_save_internal_variables(orc_model* m, int32_t converged)
+{
+    if (converged)
+    {
+        m->Hg_o = m->Hg;
+        m->F_o = m->F;
+        m->sigma_o = m->sigma;
+        m->detF_o = m->detF;
+        m->eps_p_o = m->eps_p;
+        m->ebar_o = m->ebar;
+        m->vm_o = m->vm;
+    }
+    else
+    {
+        m->Hg = m->Hg_o;
+        m->F = m->F_o;
+        m->sigma = m->sigma_o;
+        m->detF = m->detF_o;
+        m->eps_p = m->eps_p_o;
+        m->ebar = m->ebar_o;
+        m->vm = m->vm_o;
+    }
+    return ORC_OK;
+}
+
+int orc_get_qp_state(const orc_model* m, double* sigma, double* D, double* detF, double* eps_p, double* ebar,
+                     double* vm)
+{
+    size_t const Q = size_t(m->E) * 8;
+    if (sigma) std::memcpy(sigma, m->sigma.data(), 9 * Q * sizeof(double));
+    if (D) std::memcpy(D, m->D.data(), 36 * Q * sizeof(double));
+    if (detF) std::memcpy(detF, m->detF.data(), Q * sizeof(double));
+    if (eps_p) std::memcpy(eps_p, m->eps_p.data(), 9 * Q * sizeof(double));
+    if (ebar) std::memcpy(ebar, m->ebar.data(), Q * sizeof(double));
+    if (vm) std::memcpy(vm, m->vm.data(), Q * sizeof(double));
+    return ORC_OK;
+}
+
+} // extern "C"
+
+namespace
+{
+// submesh::tangent_stiffness = material_tangent_stiffness + geometric_tangent_stiffness
+// (submesh.cpp:66-80, 110-160) and submesh::internal_force (submesh.cpp:82-108), using the
+// dense B (6x24) of symmetric_gradient<3> (src/numeric/gradient_operator.hpp:38-57).
+void element_matrices(const orc_model& m, int64_t e, double* ke, double* fe)
+{
+    hex8 const& h = shape();
+    double xe[24];
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) xe[a * 3 + i] = m.x[3 * size_t(m.conn[8 * e + a]) + i];
+    bool const want_k = ke != nullptr; // internal_force alone skips the stiffness
+    if (want_k) std::fill(ke, ke + 576, 0.0);
+    std::fill(fe, fe + 24, 0.0);
+    double kgeo[64];
+    std::fill(kgeo, kgeo + 64, 0.0);
+    bool const finite = m.model == ORC_MODEL_NEOHOOKE; // is_finite_deformation()
+    for (int l = 0; l < 8; ++l)
+    {
+        double J[9], Ji[9];
+        local_F(xe, h.dN[l], J);
+        double const detJ = det3(J);
+        inv3(J, Ji);
+        double Bt[8][3]; // dN * J^{-1}
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k)
+            {
+                double s = 0.0;
+                for (int j = 0; j < 3; ++j) s += h.dN[l][a][j] * Ji[j * 3 + k];
+                Bt[a][k] = s;
+            }
+        double const wl = h.w[l];
+        double const* sg = &m.sigma[9 * (8 * e + l)];
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i)
+            {
+                double s = 0.0;
+                for (int j = 0; j < 3; ++j) s += Bt[a][j] * sg[j * 3 + i];
+                fe[a * 3 + i] += s * detJ * wl;
+            }
+        if (!want_k) continue;
+        double B[6][24];
+        std::memset(B, 0, sizeof(B));
+        for (int a = 0; a < 8; ++a)
+        {
+            int const b = 3 * a;
+            B[0][b] = Bt[a][0];
+            B[4][b] = Bt[a][2];
+            B[5][b] = Bt[a][1];
+            B[1][b + 1] = Bt[a][1];
+            B[3][b + 1] = Bt[a][2];
+            B[5][b + 1] = Bt[a][0];
+            B[2][b + 2] = Bt[a][2];
+            B[3][b + 2] = Bt[a][1];
+            B[4][b + 2] = Bt[a][0];
+        }
+        double const* D = &m.D[36 * (8 * e + l)];
+        double DB[6][24];
+        for (int i = 0; i < 6; ++i)
+            for (int c = 0; c < 24; ++c)
+            {
+                double s = 0.0;
+                for (int k = 0; k < 6; ++k) s += D[i * 6 + k] * B[k][c];
+                DB[i][c] = s;
+            }
+        for (int r = 0; r < 24; ++r)
+            for (int c = 0; c < 24; ++c)
+            {
+                double s = 0.0;
+                for (int k = 0; k < 6; ++k) s += B[k][r] * DB[k][c];
+                ke[r * 24 + c] += s * detJ * wl;
+            }
+        if (finite)
+            for (int a = 0; a < 8; ++a)
+                for (int b = 0; b < 8; ++b)
+                {
+                    double s = 0.0;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) s += Bt[a][i] * sg[i * 3 + j] * Bt[b][j];
+                    kgeo[a * 8 + b] += s * detJ * wl;
+                }
+    }
+    if (finite && want_k) // identity_expansion_inplace<3> (src/numeric/tensor_operations.hpp:122-135)
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b)
+                for (int k = 0; k < 3; ++k) ke[(a * 3 + k) * 24 + b * 3 + k] += kgeo[a * 8 + b];
+}
+
+// node adjacency of the element graph: sorted unique neighbour lists
+void node_adjacency(const orc_model& m, vector<vector<int32_t>>& adj)
+{
+    adj.assign(m.N, {});
+    for (int64_t e = 0; e < m.E; ++e)
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) adj[m.conn[8 * e + a]].push_back(m.conn[8 * e + b]);
+    for (auto& l : adj)
+    {
+        std::sort(l.begin(), l.end());
+        l.erase(std::unique(l.begin(), l.end()), l.end());
+    }
+}
+} // namespace
+
+extern "C" {
+
+int orc_element_matrices(const orc_model* m, int32_t element, double* ke, double* fe)
+{
+    if (element < 0 || element >= m->E) return ORC_BAD_ARGUMENT;
+    element_matrices(*m, element, ke, fe);
+    return ORC_OK;
+}
+
+// compute_sparsity_pattern (src/assembler/sparsity_pattern.hpp:18-48): the CSR pattern
+// of all element dof pairs with sorted, unique column indices per row; dof = 3 node + i
+// (src/mesh/dof_allocator.hpp:41-47).
+int64_t orc_sparsity_nnz(const orc_model* m)
+{
+    vector<vector<int32_t>> adj;
+    node_adjacency(*m, adj);
+    int64_t nnz = 0;
+    for (auto const& l : adj) nnz += 9 * int64_t(l.size());
+    return nnz;
+}
+
+int orc_sparsity_fill(const orc_model* m, int32_t* row_ptr, int32_t* col)
+{
+    vector<vector<int32_t>> adj;
+    node_adjacency(*m, adj);
+    int64_t pos = 0;
+    row_ptr[0] = 0;
+    for (int32_t n = 0; n < m->N; ++n)
+        for (int i = 0; i < 3; ++i)
+        {
+            for (int32_t b : adj[n])
+                for (int j = 0; j < 3; ++j) col[pos++] = 3 * b + j;
+            row_ptr[3 * n + i + 1] = int32_t(pos);
+        }
+    return ORC_OK;
+}
+
+// assemble_stiffness (static_matrix.hpp:243-277) with add_to's binary search and atomic
+// add (thread_safe_plugin.hpp:7-23); compute_internal_force serial loop (static_matrix.hpp:189-203).
+int orc_assemble(const orc_model* m, const int32_t* row_ptr, const int32_t* col, double* val, double* f_int,
+                 int32_t threads)
+{
+    int64_t const n = 3 * int64_t(m->N);
+    if (val)
+    {
+        std::fill(val, val + row_ptr[n], 0.0);
+        if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+        {
+            double ke[576], fe[24];
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t e = 0; e < m->E; ++e)
+            {
+                element_matrices(*m, e, ke, fe);
+                int32_t dofs[24];
+                for (int a = 0; a < 8; ++a)
+                    for (int i = 0; i < 3; ++i) dofs[a * 3 + i] = 3 * m->conn[8 * e + a] + i;
+                for (int b = 0; b < 24; ++b)
+                    for (int a = 0; a < 24; ++a)
+                    {
+                        int32_t const r = dofs[a], c = dofs[b];
+                        const int32_t* lo = std::lower_bound(col + row_ptr[r], col + row_ptr[r + 1], c);
+                        int64_t const p = lo - col;
+#pragma omp atomic
+                        val[p] += ke[a * 24 + b];
+                    }
+            }
+        }
+    }
+    if (f_int)
+    {
+        std::fill(f_int, f_int + n, 0.0);
+        double fe[24];
+        for (int64_t e = 0; e < m->E; ++e)
+        {
+            element_matrices(*m, e, nullptr, fe);
+            for (int a = 0; a < 8; ++a)
+                for (int i = 0; i < 3; ++i) f_int[3 * int64_t(m->conn[8 * e + a]) + i] += fe[a * 3 + i];
+        }
+    }
+    return ORC_OK;
+}
+
+// enforce_dirichlet_conditions (static_matrix.hpp:279-320)
+int orc_enforce_dirichlet(int32_t n, const int32_t* row_ptr, const int32_t* col, double* val, double* b,
+                          int32_t n_fixed, const int32_t* fixed)
+{
+    auto coeff_ref = [&](int32_t r, int32_t c) -> double* {
+        const int32_t* lo = std::lower_bound(col + row_ptr[r], col + row_ptr[r + 1], c);
+        if (lo == col + row_ptr[r + 1] || *lo != c) return nullptr;
+        return &val[lo - col];
+    };
+    for (int32_t k = 0; k < n_fixed; ++k)
+    {
+        int32_t const d = fixed[k];
+        if (d < 0 || d >= n) return ORC_BAD_ARGUMENT;
+        double* dp = coeff_ref(d, d);
+        double const diag = dp ? *dp : 0.0;
+        if (b) b[d] = 0.0;
+        for (int32_t p = row_ptr[d]; p < row_ptr[d + 1]; ++p)
+        {
+            val[p] = 0.0;
+            if (double* q = coeff_ref(col[p], d)) *q = 0.0;
+        }
+        if (dp) *dp = diag;
+    }
+    return ORC_OK;
+}
+
+int orc_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val, const double* x,
+             double* y, int32_t threads)
+{
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static)
+    for (int32_t r = 0; r < n; ++r)
+    {
+        double s = 0.0;
+        for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) s += val[p] * x[col[p]];
+        y[r] = s;
+    }
+    return ORC_OK;
+}
+
+} // extern "C"
+
+namespace
+{
+struct csr
+{
+    int32_t n;
+    const int32_t* rp;
+    const int32_t* ci;
+    const double* v;
+};
+
+double dot(const vector<double>& a, const vector<double>& b)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+void spmv(const csr& A, const double* x, double* y)
+{
+#pragma omp parallel for schedule(static)
+    for (int32_t r = 0; r < A.n; ++r)
+    {
+        double s = 0.0;
+        for (int32_t p = A.rp[r]; p < A.rp[r + 1]; ++p) s += A.v[p] * x[A.ci[p]];
+        y[r] = s;
+    }
+}
+
+// Eigen DiagonalPreconditioner: 1/diag, or 1 where the diagonal is missing or zero.
+vector<double> jacobi_inverse(const csr& A)
+{
+    vector<double> d(A.n, 1.0);
+    for (int32_t r = 0; r < A.n; ++r)
+        for (int32_t p = A.rp[r]; p < A.rp[r + 1]; ++p)
+            if (A.ci[p] == r && A.v[p] != 0.0) d[r] = 1.0 / A.v[p];
+    return d;
+}
+} // namespace
+
+extern "C" {
+
+// Eigen 3.3 internal::conjugate_gradient (IterativeLinearSolvers/ConjugateGradient.h), the
+// algorithm behind conjugate_gradient::solve (src/solver/linear/linear_solver.cpp:67-109):
+// x0 = 0, threshold tol^2 ||b||^2 on the unpreconditioned residual, iterations counted as
+// Eigen counts them.
+int orc_cg_eigen(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val, const double* b,
+                 double* x, double tol, int32_t max_it, int32_t threads, int32_t* iterations, double* error)
+{
+    if (threads > 0) omp_set_num_threads(threads);
+    std::feclearexcept(FE_ALL_EXCEPT);
+    csr A{n, row_ptr, col, val};
+    vector<double> invd = jacobi_inverse(A);
+    vector<double> r(b, b + n), p(n), z(n), tmp(n), xv(n, 0.0);
+    double const rhs2 = dot(r, r);
+    int32_t it = 0;
+    double err = 0.0;
+    if (rhs2 == 0.0)
+    {
+        std::fill(x, x + n, 0.0);
+        *iterations = 0;
+        *error = 0.0;
+        return ORC_OK;
+    }
+    double const threshold = std::max(tol * tol * rhs2, std::numeric_limits<double>::min());
+    double rn2 = rhs2;
+    if (rn2 < threshold)
+    {
+        err = std::sqrt(rn2 / rhs2);
+    }
+    else
+    {
+        for (int32_t i = 0; i < n; ++i) p[i] = invd[i] * r[i];
+        double absNew = dot(r, p);
+        while (it < max_it)
+        {
+            spmv(A, p.data(), tmp.data());
+            double const alpha = absNew / dot(p, tmp);
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i)
+            {
+                xv[i] += alpha * p[i];
+                r[i] -= alpha * tmp[i];
+            }
+            rn2 = dot(r, r);
+            if (rn2 < threshold) break;
+            for (int32_t i = 0; i < n; ++i) z[i] = invd[i] * r[i];
+            double const absOld = absNew;
+            absNew = dot(r, z);
+            double const beta = absNew / absOld;
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+            ++it;
+        }
+        err = std::sqrt(rn2 / rhs2);
+    }
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = it;
+    *error = err;
+    if (std::fetestexcept(FE_INVALID)) return ORC_NAN_OR_INF; // linear_solver.cpp:99-102
+    if (it >= max_it) return ORC_NONCONVERGED;                 // linear_solver.cpp:104-108
+    return ORC_OK;
+}
+
+// conjugate_gradient_cuda::solve (src/solver/linear/conjugate_gradient_cuda.cpp:95-186)
+int orc_cg_cuda(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val, const double* b,
+                double* x, double tol, int32_t max_it, int32_t threads, int32_t* iterations, double* error)
+{
+    if (threads > 0) omp_set_num_threads(threads);
+    csr A{n, row_ptr, col, val};
+    vector<double> minv(n);
+    for (int32_t r = 0; r < n; ++r)
+    {
+        double d = 0.0;
+        for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+            if (col[p] == r) d = val[p];
+        minv[r] = 1.0 / d; // A.diagonal().cwiseInverse()
+    }
+    vector<double> xv(n, 0.0), r(b, b + n), z(n), p(n), Ap(n);
+    for (int32_t i = 0; i < n; ++i) z[i] = minv[i] * r[i];
+    double residual = dot(r, z), residual_old = 0.0;
+    int32_t k = 0;
+    while (std::sqrt(residual) > tol && k < max_it)
+    {
+        ++k;
+        if (k == 1)
+            p = z;
+        else
+        {
+            double const beta = residual / residual_old;
+            for (int32_t i = 0; i < n; ++i) p[i] = beta * p[i] + z[i];
+        }
+        spmv(A, p.data(), Ap.data());
+        double const alpha = residual / dot(p, Ap);
+        for (int32_t i = 0; i < n; ++i)
+        {
+            xv[i] += alpha * p[i];
+            r[i] += -alpha * Ap[i];
+        }
+        residual_old = residual;
+        for (int32_t i = 0; i < n; ++i) z[i] = minv[i] * r[i];
+        residual = dot(z, r);
+    }
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = k;
+    *error = std::sqrt(residual);
+    if (k >= max_it) return ORC_NONCONVERGED;
+    return ORC_OK;
+}
+
+// Eigen 3.3 internal::bicgstab (IterativeLinearSolvers/BiCGSTAB.h) with its restart rule,
+// behind biconjugate_gradient_stabilised::solve (linear_solver.cpp:111-138).
+int orc_bicgstab_eigen(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                       const double* b, double* x, double tol, int32_t max_it, int32_t threads,
+                       int32_t* iterations, double* error)
+{
+    if (threads > 0) omp_set_num_threads(threads);
+    std::feclearexcept(FE_ALL_EXCEPT);
+    csr A{n, row_ptr, col, val};
+    vector<double> invd = jacobi_inverse(A);
+    vector<double> xv(n, 0.0), r(b, b + n), r0 = r, v(n, 0.0), p(n, 0.0), y(n), z(n), s(n), t(n);
+    double r0_sqnorm = dot(r0, r0);
+    double const rhs_sqnorm = dot(r, r);
+    if (rhs_sqnorm == 0.0)
+    {
+        std::fill(x, x + n, 0.0);
+        *iterations = 0;
+        *error = 0.0;
+        return ORC_OK;
+    }
+    double rho = 1, alpha = 1, w = 1;
+    double const tol2 = tol * tol * rhs_sqnorm;
+    double const eps2 = std::numeric_limits<double>::epsilon() * std::numeric_limits<double>::epsilon();
+    int32_t i = 0, restarts = 0;
+    while (dot(r, r) > tol2 && i < max_it)
+    {
+        double const rho_old = rho;
+        rho = dot(r0, r);
+        if (std::abs(rho) < eps2 * r0_sqnorm)
+        {
+            spmv(A, xv.data(), t.data());
+            for (int32_t k = 0; k < n; ++k) r[k] = b[k] - t[k];
+            r0 = r;
+            rho = r0_sqnorm = dot(r, r);
+            if (restarts++ == 0) i = 0;
+        }
+        double const beta = (rho / rho_old) * (alpha / w);
+        for (int32_t k = 0; k < n; ++k) p[k] = r[k] + beta * (p[k] - w * v[k]);
+        for (int32_t k = 0; k < n; ++k) y[k] = invd[k] * p[k];
+        spmv(A, y.data(), v.data());
+        alpha = rho / dot(r0, v);
+        for (int32_t k = 0; k < n; ++k) s[k] = r[k] - alpha * v[k];
+        for (int32_t k = 0; k < n; ++k) z[k] = invd[k] * s[k];
+        spmv(A, z.data(), t.data());
+        double const tmp = dot(t, t);
+        w = tmp > 0.0 ? dot(t, s) / tmp : 0.0;
+        for (int32_t k = 0; k < n; ++k)
+        {
+            xv[k] += alpha * y[k] + w * z[k];
+            r[k] = s[k] - w * t[k];
+        }
+        ++i;
+    }
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = i;
+    *error = std::sqrt(dot(r, r) / rhs_sqnorm);
+    if (std::fetestexcept(FE_INVALID)) return ORC_NAN_OR_INF;
+    if (i >= max_it) return ORC_NONCONVERGED;
+    return ORC_OK;
+}
+
+// biconjugate_gradient_stabilised_cuda::solve (biconjugate_gradient_stabilised_cuda.cpp:31-200)
+int orc_bicgstab_cuda(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                      const double* b, double* x, double tol, int32_t max_it, int32_t threads,
+                      int32_t* iterations, double* error)
+{
+    if (threads > 0) omp_set_num_threads(threads);
+    csr A{n, row_ptr, col, val};
+    vector<double> minv(n);
+    for (int32_t r = 0; r < n; ++r)
+    {
+        double d = 0.0;
+        for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+            if (col[p] == r) d = val[p];
+        minv[r] = 1.0 / d;
+    }
+    vector<double> xv(n, 0.0), r(b, b + n), r0 = r, p(n, 0.0), ph(n), v(n, 0.0), s(n), sh(n), t(n);
+    int32_t k = 0;
+    double residual_old = 0.0, residual = dot(r, r0);
+    double alpha = 0.0, omega = 0.0;
+    int status = ORC_OK;
+    while (std::sqrt(residual) > tol && k < max_it)
+    {
+        ++k;
+        if (k == 1)
+            p = r;
+        else
+        {
+            double const beta = residual / residual_old * alpha / omega;
+            for (int32_t i = 0; i < n; ++i)
+            {
+                v[i] = omega * v[i];
+                p[i] = p[i] - v[i];
+                p[i] = beta * p[i];
+                p[i] = p[i] + r[i];
+            }
+        }
+        for (int32_t i = 0; i < n; ++i) ph[i] = minv[i] * p[i];
+        spmv(A, ph.data(), v.data());
+        alpha = residual / dot(r0, v);
+        for (int32_t i = 0; i < n; ++i) s[i] = r[i] + (-alpha) * v[i];
+        double const norm_s = std::sqrt(dot(s, s));
+        if (norm_s < tol)
+        {
+            for (int32_t i = 0; i < n; ++i) xv[i] += alpha * ph[i];
+            residual = norm_s * norm_s;
+            break;
+        }
+        for (int32_t i = 0; i < n; ++i) sh[i] = minv[i] * s[i];
+        spmv(A, sh.data(), t.data());
+        double const t_dot_s = dot(t, s), t_dot_t = dot(t, t);
+        omega = t_dot_s / t_dot_t;
+        for (int32_t i = 0; i < n; ++i)
+        {
+            xv[i] += alpha * ph[i];
+            xv[i] += omega * sh[i];
+            r[i] = s[i] + (-omega) * t[i];
+        }
+        residual_old = residual;
+        residual = dot(r0, r);
+        if (omega == 0.0) // is_approx(omega, 0.0): relative difference to 0 is never <= 1e-10
+        {
+            status = ORC_BREAKDOWN;
+            break;
+        }
+    }
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = k;
+    *error = std::sqrt(residual);
+    if (status != ORC_OK) return status;
+    if (k >= max_it) return ORC_NONCONVERGED;
+    return ORC_OK;
+}
+
+// reverse_cuthill_mcgee::compute (src/graph/cuthill_mckee.hpp:45-97) over the adjacency
+// graph of src/graph/adjacency_graph.hpp (diagonal removed, children sorted by degree).
+int orc_rcm(int32_t n, const int32_t* row_ptr, const int32_t* col, int32_t* perm)
+{
+    vector<vector<int32_t>> adj(n);
+    for (int32_t r = 0; r < n; ++r)
+        for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+            if (col[p] != r) adj[r].push_back(col[p]);
+    for (auto& l : adj)
+        std::stable_sort(l.begin(), l.end(),
+                         [&](int32_t a, int32_t b) { return adj[a].size() < adj[b].size(); });
+    vector<char> done(n, 0);
+    int32_t count = 0;
+    std::queue<int32_t> q;
+    while (count != n)
+    {
+        int32_t best = n;
+        size_t deg = size_t(n);
+        for (int32_t r = 0; r < n; ++r)
+            if (adj[r].size() < deg && !done[r])
+            {
+                deg = adj[r].size();
+                best = r;
+            }
+        perm[count++] = best;
+        done[best] = 1;
+        for (int32_t c : adj[best])
+            if (!done[c]) q.push(c);
+        while (!q.empty())
+        {
+            int32_t const f = q.front();
+            if (!done[f])
+            {
+                done[f] = 1;
+                perm[count++] = f;
+                for (int32_t c : adj[f])
+                    if (!done[c]) q.push(c);
+            }
+            q.pop();
+        }
+    }
+    std::reverse(perm, perm + n);
+    return ORC_OK;
+}
+
+} // extern "C"
+
+// ============================================================ load stepping
+namespace
+{
+// is_approx (src/numeric/float_compare.hpp:20-32) with boost's relative_difference:
+// relative difference to zero is the largest double, so only the absolute test can pass.
+bool is_approx(double x, double y, double abs_tol = 0.0, double rel_tol = 1e-10)
+{
+    double rel;
+    if (x == 0.0 && y == 0.0)
+        rel = 0.0;
+    else if (x == 0.0 || y == 0.0)
+        rel = std::numeric_limits<double>::max();
+    else
+        rel = std::abs(x - y) / std::min(std::abs(x), std::abs(y));
+    return rel <= rel_tol || std::abs(x - y) <= abs_tol;
+}
+} // namespace
+
+// adaptive_time_step (src/solver/adaptive_time_step.cpp:12-165, adaptive_time_step.hpp:72-137)
+struct orc_time
+{
+    std::priority_queue<double, vector<double>, std::greater<double>> queue;
+    int32_t increment_limit = 5, successful = 0, consecutive_converged = 0, consecutive_unconverged = 0;
+    double initial = 1, final_time = 1, current = 1, total = 0, last_converged = 0, last_size = 0;
+    double minimum = 0, maximum = 0;
+    bool applied = false;
+    bool highly_nonlinear() const { return consecutive_unconverged > 0 || consecutive_converged < 4; }
+};
+
+extern "C" {
+
+int orc_time_create(const orc_time_options* opt, int32_t n_hist, const double* history, orc_time** out)
+{
+    if (!opt || n_hist <= 0 || !out) return ORC_BAD_ARGUMENT;
+    auto* t = new orc_time;
+    for (int32_t i = 0; i < n_hist; ++i) t->queue.push(history[i]);
+    if (is_approx(t->queue.top(), 0.0)) t->queue.pop();
+    t->initial = opt->initial;
+    t->minimum = opt->adaptive ? opt->minimum : opt->initial;
+    t->maximum = opt->adaptive ? opt->maximum : opt->initial;
+    t->final_time = opt->period;
+    t->current = std::min(t->queue.top(), t->initial);
+    *out = t;
+    return ORC_OK;
+}
+
+int orc_time_destroy(orc_time* t)
+{
+    delete t;
+    return ORC_OK;
+}
+
+int orc_time_update(orc_time* t, int32_t converged)
+{
+    if (converged)
+    {
+        if (t->current >= t->queue.top()) t->queue.pop();
+        t->last_size = t->current - t->last_converged;
+        t->last_converged = t->current;
+        t->applied = is_approx(t->current, t->final_time) || t->queue.empty();
+        t->consecutive_unconverged = 0;
+        t->consecutive_converged++;
+        t->successful++;
+        if (t->applied) return ORC_OK;
+        double const new_time = std::min(t->highly_nonlinear() ? t->last_size + t->current : 2.0 * t->current,
+                                          t->current + t->maximum);
+        t->current = std::min(t->queue.top(), std::min(new_time, t->final_time));
+        return ORC_OK;
+    }
+    double const dt = std::max(t->minimum, (t->current - t->last_converged) * 0.5);
+    t->current = t->last_converged + dt;
+    if (t->current > t->final_time) return ORC_FATAL;
+    if (t->current < 0.0) return ORC_FATAL;
+    t->consecutive_unconverged++;
+    t->consecutive_converged = 0;
+    if (t->consecutive_unconverged == t->increment_limit) return ORC_FATAL;
+    return ORC_OK;
+}
+
+int orc_time_query(const orc_time* t, double* step_time, double* last_step_time, int32_t* fully_applied,
+                   int32_t* successful)
+{
+    if (step_time) *step_time = t->current;
+    if (last_step_time) *last_step_time = t->last_converged;
+    if (fully_applied) *fully_applied = t->applied;
+    if (successful) *successful = t->successful;
+    return ORC_OK;
+}
+
+} // extern "C"
+
+// ============================================================= Newton driver
+namespace
+{
+// boundary::is_active / interpolate_prescribed_load (src/mesh/boundary/boundary.cpp:37-85)
+bool bc_active(const orc_dirichlet& bc, double t) { return t >= bc.times[0] && t <= bc.times[bc.n_points - 1]; }
+double bc_value(const orc_dirichlet& bc, double t)
+{
+    for (int32_t i = 0; i < bc.n_points; ++i)
+        if (is_approx(bc.times[i], t)) return bc.values[i];
+    for (int32_t i = 0; i + 1 < bc.n_points; ++i)
+        if (bc.times[i] < t && t < bc.times[i + 1])
+        {
+            double const f = (t - bc.times[i]) / (bc.times[i + 1] - bc.times[i]);
+            return (1.0 - f) * bc.values[i] + f * bc.values[i + 1];
+        }
+    return std::numeric_limits<double>::quiet_NaN();
+}
+
+double norm(const vector<double>& v) { return std::sqrt(dot(v, v)); }
+
+// P^T K P and P^T b with perm[new] = old (apply_permutation, linear_solver.cpp:39-44)
+void permute(int32_t n, const int32_t* rp, const int32_t* ci, const double* v, const vector<int32_t>& perm,
+             const vector<int32_t>& inv, vector<int32_t>& prp, vector<int32_t>& pci, vector<double>& pv)
+{
+    prp.assign(n + 1, 0);
+    for (int32_t r = 0; r < n; ++r) prp[r + 1] = prp[r] + (rp[perm[r] + 1] - rp[perm[r]]);
+    pci.resize(rp[n]);
+    pv.resize(rp[n]);
+#pragma omp parallel for schedule(static)
+    for (int32_t r = 0; r < n; ++r)
+    {
+        int32_t const o = perm[r];
+        int32_t const len = rp[o + 1] - rp[o];
+        vector<std::pair<int32_t, double>> row(len);
+        for (int32_t k = 0; k < len; ++k) row[k] = {inv[ci[rp[o] + k]], v[rp[o] + k]};
+        std::sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (int32_t k = 0; k < len; ++k)
+        {
+            pci[prp[r] + k] = row[k].first;
+            pv[prp[r] + k] = row[k].second;
+        }
+    }
+}
+} // namespace
+
+extern "C" {
+
+double orc_boundary_value(int32_t n_points, const double* times, const double* values, double t)
+{
+    orc_dirichlet bc{0, nullptr, n_points, times, values};
+    return bc_value(bc, t);
+}
+
+// static_matrix::solve / perform_equilibrium_iterations (static_matrix.hpp:145-187, 402-467)
+int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, const orc_dirichlet* bcs,
+                     double* u_out, orc_newton_record* records, int32_t cap, int32_t* n_records)
+{
+    if (!m || !opt || (n_bc > 0 && !bcs) || !u_out) return ORC_BAD_ARGUMENT;
+    int32_t const threads = opt->threads;
+    if (threads > 0) omp_set_num_threads(threads);
+    int32_t const n = 3 * m->N;
+    int32_t nrec = 0;
+
+    vector<double> hist;
+    for (int32_t k = 0; k < n_bc; ++k)
+        for (int32_t i = 0; i < bcs[k].n_points; ++i) hist.push_back(bcs[k].times[i]);
+    // mesh::time_history collects the boundary times into a std::set (mesh.cpp:146-172)
+    std::sort(hist.begin(), hist.end());
+    hist.erase(std::unique(hist.begin(), hist.end()), hist.end());
+    orc_time* ts = nullptr;
+    if (orc_time_create(&opt->time, int32_t(hist.size()), hist.data(), &ts) != ORC_OK) return ORC_BAD_ARGUMENT;
+
+    int64_t const nnz = orc_sparsity_nnz(m);
+    vector<int32_t> rp(n + 1), ci(nnz);
+    orc_sparsity_fill(m, rp.data(), ci.data());
+    vector<double> K(nnz);
+    vector<double> f_int(n, 0.0), f_ext(n, 0.0), u(n, 0.0), u_old(n, 0.0), delta(n, 0.0), mres(n, 0.0);
+    vector<int32_t> perm, inv;
+    vector<int32_t> prp, pci;
+    vector<double> pv, pb, px;
+
+    bool const relative = !opt->absolute_norm;
+    int const max_it = opt->max_newton_iterations > 0 ? opt->max_newton_iterations : 10;
+    double norm_initial_residual = 1.0;
+    int result = ORC_OK;
+
+    auto record = [&](int it, int solver_its, int status, double dn, double fn) {
+        if (records && nrec < cap)
+        {
+            double st;
+            int32_t succ;
+            orc_time_query(ts, &st, nullptr, nullptr, &succ);
+            records[nrec] = orc_newton_record{succ, it, solver_its, status, st, dn, fn};
+        }
+        ++nrec;
+    };
+
+    for (;;) // solve() and its recursive retry after a computational_error
+    {
+        int err = orc_update_internal_variables(m, u.data(), 0.0);
+        bool recovered = false;
+        while (err == ORC_OK)
+        {
+            int32_t applied;
+            orc_time_query(ts, nullptr, nullptr, &applied, nullptr);
+            if (applied) break;
+            std::fill(f_ext.begin(), f_ext.end(), 0.0); // compute_external_force: no Neumann loads
+
+            // perform_equilibrium_iterations
+            u = u_old;
+            double step_time, last_time;
+            orc_time_query(ts, &step_time, &last_time, nullptr, nullptr);
+            err = orc_update_internal_variables(m, u.data(), step_time - last_time);
+            if (err != ORC_OK) break;
+            int it = 0;
+            double dnorm = 0, fnorm = 0;
+            while (it < max_it)
+            {
+                orc_assemble(m, rp.data(), ci.data(), K.data(), f_int.data(), threads);
+                for (int32_t i = 0; i < n; ++i) mres[i] = f_ext[i] - f_int[i];
+                vector<int32_t> fixed;
+                if (it == 0) // apply_displacement_boundaries (static_matrix.hpp:322-351)
+                {
+                    vector<double> inc(n, 0.0), Kinc(n);
+                    vector<char> touched(n, 0);
+                    for (int32_t k = 0; k < n_bc; ++k)
+                    {
+                        if (!bc_active(bcs[k], step_time)) continue;
+                        double const du = bc_value(bcs[k], step_time) - bc_value(bcs[k], last_time);
+                        for (int32_t i = 0; i < bcs[k].n_dofs; ++i)
+                        {
+                            inc[bcs[k].dofs[i]] = du;
+                            touched[bcs[k].dofs[i]] = 1;
+                        }
+                    }
+                    spmv(csr{n, rp.data(), ci.data(), K.data()}, inc.data(), Kinc.data());
+                    for (int32_t i = 0; i < n; ++i)
+                    {
+                        mres[i] -= Kinc[i];
+                        u[i] += inc[i];
+                    }
+                    norm_initial_residual = norm(mres);
+                }
+                for (int32_t k = 0; k < n_bc; ++k)
+                    if (bc_active(bcs[k], step_time)) fixed.insert(fixed.end(), bcs[k].dofs, bcs[k].dofs + bcs[k].n_dofs);
+                orc_enforce_dirichlet(n, rp.data(), ci.data(), K.data(), mres.data(), int32_t(fixed.size()),
+                                      fixed.data());
+
+                int32_t its = 0;
+                double e = 0;
+                int s;
+                if (opt->rcm)
+                {
+                    if (perm.empty())
+                    {
+                        perm.resize(n);
+                        inv.resize(n);
+                        orc_rcm(n, rp.data(), ci.data(), perm.data());
+                        for (int32_t i = 0; i < n; ++i) inv[perm[i]] = i;
+                    }
+                    permute(n, rp.data(), ci.data(), K.data(), perm, inv, prp, pci, pv);
+                    pb.resize(n);
+                    px.resize(n);
+                    for (int32_t i = 0; i < n; ++i) pb[i] = mres[perm[i]];
+                }
+                const int32_t* srp = opt->rcm ? prp.data() : rp.data();
+                const int32_t* sci = opt->rcm ? pci.data() : ci.data();
+                const double* sv = opt->rcm ? pv.data() : K.data();
+                const double* sb = opt->rcm ? pb.data() : mres.data();
+                double* sx = opt->rcm ? px.data() : delta.data();
+                if (opt->solver == 0)
+                    s = opt->solver_criterion == ORC_STOP_RELATIVE
+                            ? orc_cg_eigen(n, srp, sci, sv, sb, sx, opt->solver_tolerance, opt->solver_max_iterations,
+                                           threads, &its, &e)
+                            : orc_cg_cuda(n, srp, sci, sv, sb, sx, opt->solver_tolerance, opt->solver_max_iterations,
+                                          threads, &its, &e);
+                else
+                    s = opt->solver_criterion == ORC_STOP_RELATIVE
+                            ? orc_bicgstab_eigen(n, srp, sci, sv, sb, sx, opt->solver_tolerance,
+                                                 opt->solver_max_iterations, threads, &its, &e)
+                            : orc_bicgstab_cuda(n, srp, sci, sv, sb, sx, opt->solver_tolerance,
+                                                opt->solver_max_iterations, threads, &its, &e);
+                if (opt->rcm)
+                    for (int32_t i = 0; i < n; ++i) delta[perm[i]] = px[i];
+                if (s != ORC_OK)
+                {
+                    record(it, its, s, 0, 0);
+                    err = s;
+                    break;
+                }
+                for (int32_t i = 0; i < n; ++i) u[i] += delta[i];
+                err = orc_update_internal_variables(m, u.data(), 0.0);
+                if (err != ORC_OK)
+                {
+                    record(it, its, err, 0, 0);
+                    break;
+                }
+                // update_relative_norms (static_matrix.hpp:385-400)
+                double const un = norm(u), fe = norm(f_ext), fi = norm(f_int);
+                bool const use_rel = opt->norm_mode == 1
+                                         ? true
+                                         : (relative && !is_approx(un, 0.0, 1.0) && !is_approx(std::max(fe, fi), 0.0, 1.0));
+                if (use_rel)
+                {
+                    dnorm = norm(delta) / un;
+                    fnorm = norm(mres) / std::max(norm_initial_residual, std::max(fe, fi));
+                }
+                else
+                {
+                    dnorm = norm(delta);
+                    fnorm = norm(mres);
+                }
+                record(it, its, 0, dnorm, fnorm);
+                if (dnorm <= opt->displacement_tolerance && fnorm <= opt->residual_tolerance) break;
+                ++it;
+            }
+            if (err != ORC_OK) break;
+            if (it == max_it)
+            {
+                err = ORC_NONCONVERGED;
+                break;
+            }
+            u_old = u;
+            orc_time_update(ts, 1);
+            orc_save_internal_variables(m, 1);
+        }
+        if (err == ORC_OK) break;
+        if (err < 0)
+        {
+            result = err;
+            break;
+        }
+        // catch (computational_error&): cutback, revert, reset u, retry (static_matrix.hpp:168-182)
+        if (orc_time_update(ts, 0) != ORC_OK)
+        {
+            result = ORC_FATAL;
+            break;
+        }
+        orc_save_internal_variables(m, 0);
+        u = u_old;
+        recovered = true;
+        (void)recovered;
+    }
+    std::copy(u.begin(), u.end(), u_out);
+    if (n_records) *n_records = nrec;
+    orc_time_destroy(ts);
+    return result;
+}
+
+} // extern "C"

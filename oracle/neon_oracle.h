@@ -1,0 +1,186 @@
+/*
+ * CPU FP64 oracle for the implicit Newton step of hex8 solid mechanics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against; it is never the thing measured or shipped.  Only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark may load it.
+ *
+ * It restates, in plain C++17 without Eigen/TBB, the algorithms of dbeurle/neon
+ * (file:line citations are relative to the reference repository root):
+ *   hex8 shape functions      src/interpolations/hexahedron.cpp:16-67
+ *   8-point Gauss rule        src/quadrature/hexahedron/hexahedron_quadrature.cpp:27-38
+ *   deformation measures      src/mesh/mechanics/solid/submesh.cpp:210-269
+ *   compressible neo-Hooke    src/constitutive/mechanics/solid/compressible_neohooke.cpp:17-90
+ *   small-strain J2           src/constitutive/mechanics/solid/small_strain_J2_plasticity.cpp:30-163
+ *   element K and f_int       src/mesh/mechanics/solid/submesh.cpp:66-160
+ *   sparsity pattern          src/assembler/sparsity_pattern.hpp:18-48
+ *   assembly (atomic add_to)  src/assembler/mechanics/static_matrix.hpp:243-277,
+ *                             src/numeric/thread_safe_plugin.hpp:7-23
+ *   Dirichlet handling        src/assembler/mechanics/static_matrix.hpp:279-351
+ *   Newton driver + norms     src/assembler/mechanics/static_matrix.hpp:145-467
+ *   load stepping / cutback   src/solver/adaptive_time_step.cpp:12-165
+ *   load interpolation        src/mesh/boundary/boundary.cpp:37-85
+ *   CPU CG / BiCGSTAB         src/solver/linear/linear_solver.cpp:39-138 (Eigen 3.3
+ *                             ConjugateGradient / BiCGSTAB, restated below)
+ *   CUDA CG / BiCGSTAB        src/solver/linear/conjugate_gradient_cuda.cpp:72-192,
+ *                             src/solver/linear/biconjugate_gradient_stabilised_cuda.cpp:31-200
+ *   RCM reordering            src/graph/cuthill_mckee.hpp:45-97
+ *
+ * Every entry point returns an int status (ORC_* codes below).
+ */
+#ifndef NEON_ORACLE_H
+#define NEON_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_OK = 0,
+    ORC_NONCONVERGED = 1,     /* Krylov or Newton iteration limit (computational_error) */
+    ORC_NAN_OR_INF = 2,       /* FE_INVALID after an update (computational_error) */
+    ORC_NEGATIVE_JACOBIAN = 3,/* det F sign bit set (computational_error) */
+    ORC_RETURN_MAP_FAILED = 4,/* radial return did not converge (computational_error) */
+    ORC_BREAKDOWN = 5,        /* BiCGSTAB omega ~ 0 (computational_error) */
+    ORC_FATAL = -1,           /* std::domain_error: too many cutbacks etc. */
+    ORC_BAD_ARGUMENT = -2
+};
+
+enum { ORC_MODEL_NEOHOOKE = 0, ORC_MODEL_J2 = 1 };
+
+/* Krylov stopping rule: Eigen's relative ||r|| < tol ||b|| (CPU solver) or the
+ * CUDA solver's absolute sqrt(r.z) <= tol. */
+enum { ORC_STOP_RELATIVE = 0, ORC_STOP_ABSOLUTE_PRECONDITIONED = 1 };
+
+/* Tangent: "reference" reproduces neon's tangents (neo-Hooke D without 1/J);
+ * "consistent" divides the neo-Hooke D by J so that K = d f_int / d u. */
+enum { ORC_TANGENT_REFERENCE = 0, ORC_TANGENT_CONSISTENT = 1 };
+
+/* ---- shape functions and single quadrature-point constitutive updates ---- */
+int orc_hex8_tables(double* N /*[8 qp][8 node]*/, double* dN /*[8 qp][8 node][3]*/,
+                    double* w /*[8]*/);
+
+/* neo-Hooke at one QP: F (row-major 3x3), J = det F. Outputs sigma (3x3) and D (6x6). */
+int orc_neohooke_qp(const double* F, double bulk, double shear, double* sigma, double* D);
+
+/* J2 at one QP: H (3x3 displacement gradient); eps_p (3x3) and ebar are in/out history. */
+int orc_j2_qp(const double* H, double bulk, double shear, double yield, double hardening,
+              double* eps_p, double* ebar, double* sigma, double* D, double* von_mises);
+
+/* ---- mesh-level model ---- */
+typedef struct orc_model orc_model;
+
+/* params: per element (bulk K, shear G, yield stress, hardening modulus). */
+int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
+                     int32_t model, const double* params, int32_t tangent_mode,
+                     orc_model** out);
+int orc_model_destroy(orc_model* m);
+int orc_model_n_dofs(const orc_model* m);
+
+/* submesh::update_internal_variables with x = X + u.  Returns ORC_NEGATIVE_JACOBIAN,
+ * ORC_NAN_OR_INF or ORC_RETURN_MAP_FAILED like the reference's computational_error. */
+int orc_update_internal_variables(orc_model* m, const double* u, double dt);
+int orc_save_internal_variables(orc_model* m, int32_t converged);
+
+/* Per-QP state downloads (QP index e*8 + l). */
+int orc_get_qp_state(const orc_model* m, double* sigma /*9/qp or NULL*/, double* D /*36/qp or NULL*/,
+                     double* detF /*1/qp or NULL*/, double* eps_p /*9/qp or NULL*/,
+                     double* ebar /*1/qp or NULL*/, double* vm /*1/qp or NULL*/);
+
+/* Element matrices at the current configuration (tangent_stiffness / internal_force). */
+int orc_element_matrices(const orc_model* m, int32_t element, double* ke /*576*/, double* fe /*24*/);
+
+/* compute_sparsity_pattern: pass row_ptr=NULL to get nnz only. */
+int64_t orc_sparsity_nnz(const orc_model* m);
+int orc_sparsity_fill(const orc_model* m, int32_t* row_ptr, int32_t* col);
+
+/* assemble_stiffness (parallel, binary search + atomic add) + compute_internal_force (serial). */
+int orc_assemble(const orc_model* m, const int32_t* row_ptr, const int32_t* col, double* val,
+                 double* f_int, int32_t threads);
+
+/* enforce_dirichlet_conditions on a CSR matrix and rhs. */
+int orc_enforce_dirichlet(int32_t n, const int32_t* row_ptr, const int32_t* col, double* val,
+                          double* b, int32_t n_fixed, const int32_t* fixed);
+
+/* ---- Krylov solvers ---- */
+int orc_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+             const double* x, double* y, int32_t threads);
+/* Eigen ConjugateGradient (Jacobi) with neon's relative criterion; iterations as Eigen reports. */
+int orc_cg_eigen(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                 const double* b, double* x, double tol, int32_t max_it, int32_t threads,
+                 int32_t* iterations, double* error);
+/* conjugate_gradient_cuda::solve algorithm (absolute sqrt(r.z) criterion). */
+int orc_cg_cuda(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                const double* b, double* x, double tol, int32_t max_it, int32_t threads,
+                int32_t* iterations, double* error);
+int orc_bicgstab_eigen(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                       const double* b, double* x, double tol, int32_t max_it, int32_t threads,
+                       int32_t* iterations, double* error);
+int orc_bicgstab_cuda(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                      const double* b, double* x, double tol, int32_t max_it, int32_t threads,
+                      int32_t* iterations, double* error);
+/* Reverse Cuthill-McKee permutation of the CSR graph (perm[new] = old). */
+int orc_rcm(int32_t n, const int32_t* row_ptr, const int32_t* col, int32_t* perm);
+
+/* ---- load stepping ---- */
+typedef struct {
+    double period;
+    double initial, minimum, maximum;
+    int32_t adaptive;
+} orc_time_options;
+
+typedef struct orc_time orc_time;
+int orc_time_create(const orc_time_options* opt, int32_t n_hist, const double* history, orc_time** out);
+int orc_time_destroy(orc_time* t);
+/* returns ORC_FATAL when neon would throw std::domain_error */
+int orc_time_update(orc_time* t, int32_t converged);
+int orc_time_query(const orc_time* t, double* step_time, double* last_step_time, int32_t* fully_applied,
+                   int32_t* successful);
+
+/* boundary::interpolate_prescribed_load for a time-value history (NaN outside it). */
+double orc_boundary_value(int32_t n_points, const double* times, const double* values, double t);
+
+/* ---- Newton driver (static_matrix::solve) ---- */
+typedef struct {
+    int32_t n_dofs;        /* number of constrained dofs */
+    const int32_t* dofs;   /* global dof = 3*node + component */
+    int32_t n_points;      /* time-value history */
+    const double* times;
+    const double* values;
+} orc_dirichlet;
+
+typedef struct {
+    orc_time_options time;
+    double residual_tolerance;
+    double displacement_tolerance;
+    int32_t max_newton_iterations; /* "linear_iterations", default 10 */
+    int32_t absolute_norm;         /* "absolute_tolerance" present */
+    int32_t norm_mode;             /* 0: neon (is_approx fallback), 1: always relative */
+    double solver_tolerance;
+    int32_t solver_max_iterations;
+    int32_t solver_criterion;      /* ORC_STOP_* */
+    int32_t solver;                /* 0 CG, 1 BiCGSTAB */
+    int32_t threads;
+    int32_t rcm;                   /* apply RCM + P^T K P per solve like neon's CPU CG */
+} orc_newton_options;
+
+typedef struct {
+    int32_t step;           /* successful increments before this iteration */
+    int32_t iteration;      /* Newton iteration within the attempt */
+    int32_t solver_iterations;
+    int32_t status;         /* 0, or the recoverable code that ended the attempt */
+    double step_time;
+    double displacement_norm;
+    double force_norm;
+} orc_newton_record;
+
+int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, const orc_dirichlet* bcs,
+                     double* u /* in: ignored (zero start); out: final displacement */,
+                     orc_newton_record* records, int32_t record_capacity, int32_t* n_records);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,292 @@
+"""ctypes binding of the CPU FP64 oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline of the
+benchmark.  Only tests, the smoke check and the benchmark's CPU-baseline leg import it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+P = C.POINTER
+i32p, f64p = P(C.c_int32), P(C.c_double)
+
+OK, NONCONVERGED, NAN_OR_INF, NEGATIVE_JACOBIAN, RETURN_MAP_FAILED, BREAKDOWN = 0, 1, 2, 3, 4, 5
+FATAL, BAD_ARGUMENT = -1, -2
+MODEL_NEOHOOKE, MODEL_J2 = 0, 1
+STOP_RELATIVE, STOP_ABSOLUTE_PRECONDITIONED = 0, 1
+TANGENT_REFERENCE, TANGENT_CONSISTENT = 0, 1
+
+
+class TimeOptions(C.Structure):
+    _fields_ = [("period", C.c_double), ("initial", C.c_double), ("minimum", C.c_double), ("maximum", C.c_double),
+                ("adaptive", C.c_int32)]
+
+
+class Dirichlet(C.Structure):
+    _fields_ = [("n_dofs", C.c_int32), ("dofs", i32p), ("n_points", C.c_int32), ("times", f64p),
+                ("values", f64p)]
+
+
+class NewtonOptions(C.Structure):
+    _fields_ = [("time", TimeOptions), ("residual_tolerance", C.c_double), ("displacement_tolerance", C.c_double),
+                ("max_newton_iterations", C.c_int32), ("absolute_norm", C.c_int32), ("norm_mode", C.c_int32),
+                ("solver_tolerance", C.c_double), ("solver_max_iterations", C.c_int32),
+                ("solver_criterion", C.c_int32), ("solver", C.c_int32), ("threads", C.c_int32),
+                ("rcm", C.c_int32)]
+
+
+class NewtonRecord(C.Structure):
+    _fields_ = [("step", C.c_int32), ("iteration", C.c_int32), ("solver_iterations", C.c_int32),
+                ("status", C.c_int32), ("step_time", C.c_double), ("displacement_norm", C.c_double),
+                ("force_norm", C.c_double)]
+
+
+_SIG = {
+    "orc_hex8_tables": [f64p, f64p, f64p],
+    "orc_neohooke_qp": [f64p, C.c_double, C.c_double, f64p, f64p],
+    "orc_j2_qp": [f64p, C.c_double, C.c_double, C.c_double, C.c_double, f64p, f64p, f64p, f64p, f64p],
+    "orc_model_create": [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, f64p, C.c_int32, P(C.c_void_p)],
+    "orc_model_destroy": [C.c_void_p],
+    "orc_update_internal_variables": [C.c_void_p, f64p, C.c_double],
+    "orc_save_internal_variables": [C.c_void_p, C.c_int32],
+    "orc_get_qp_state": [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p],
+    "orc_element_matrices": [C.c_void_p, C.c_int32, f64p, f64p],
+    "orc_sparsity_fill": [C.c_void_p, i32p, i32p],
+    "orc_assemble": [C.c_void_p, i32p, i32p, f64p, f64p, C.c_int32],
+    "orc_enforce_dirichlet": [C.c_int32, i32p, i32p, f64p, f64p, C.c_int32, i32p],
+    "orc_spmv": [C.c_int32, i32p, i32p, f64p, f64p, f64p, C.c_int32],
+    "orc_rcm": [C.c_int32, i32p, i32p, i32p],
+    "orc_time_create": [P(TimeOptions), C.c_int32, f64p, P(C.c_void_p)],
+    "orc_time_destroy": [C.c_void_p],
+    "orc_time_update": [C.c_void_p, C.c_int32],
+    "orc_time_query": [C.c_void_p, f64p, f64p, i32p, i32p],
+    "orc_newton_solve": [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), f64p, P(NewtonRecord), C.c_int32,
+                         i32p],
+}
+for _k in ("orc_cg_eigen", "orc_cg_cuda", "orc_bicgstab_eigen", "orc_bicgstab_cuda"):
+    _SIG[_k] = [C.c_int32, i32p, i32p, f64p, f64p, f64p, C.c_double, C.c_int32, C.c_int32, i32p, f64p]
+
+_lib = None
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        h = C.CDLL(str(LIB_PATH))
+        for name, args in _SIG.items():
+            fn = getattr(h, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        h.orc_boundary_value.argtypes = [C.c_int32, f64p, f64p, C.c_double]
+        h.orc_boundary_value.restype = C.c_double
+        h.orc_sparsity_nnz.argtypes = [C.c_void_p]
+        h.orc_sparsity_nnz.restype = C.c_int64
+        _lib = h
+    return _lib
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(f64p)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(i32p)
+
+
+def hex8_tables():
+    N, dN, w = np.zeros((8, 8)), np.zeros((8, 8, 3)), np.zeros(8)
+    lib().orc_hex8_tables(_f(N), _f(dN), _f(w))
+    return N, dN, w
+
+
+def neohooke_qp(F, bulk, shear):
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    s, D = np.zeros((3, 3)), np.zeros((6, 6))
+    lib().orc_neohooke_qp(_f(F), bulk, shear, _f(s), _f(D))
+    return s, D
+
+
+def j2_qp(H, bulk, shear, yield_stress, hardening, eps_p=None, ebar=0.0):
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    ep = np.zeros((3, 3)) if eps_p is None else np.array(eps_p, dtype=np.float64)
+    eb = np.array([ebar], dtype=np.float64)
+    s, D, vm = np.zeros((3, 3)), np.zeros((6, 6)), np.zeros(1)
+    code = lib().orc_j2_qp(_f(H), bulk, shear, yield_stress, hardening, _f(ep), _f(eb), _f(s), _f(D), _f(vm))
+    return dict(status=code, sigma=s, D=D, vm=vm[0], eps_p=ep, ebar=eb[0])
+
+
+class Model:
+    """Mesh-level oracle model (constitutive state + element matrices + assembly)."""
+
+    def __init__(self, X, conn, params, model=MODEL_NEOHOOKE, tangent=TANGENT_REFERENCE):
+        self.X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1)
+        self.params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        self.n_nodes, self.n_elem = len(self.X) // 3, len(self.conn) // 8
+        self.h = C.c_void_p()
+        code = lib().orc_model_create(self.n_nodes, _f(self.X), self.n_elem, _i(self.conn), model, _f(self.params),
+                                      tangent, C.byref(self.h))
+        if code != OK:
+            raise RuntimeError(f"orc_model_create failed: {code}")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_model_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def n_dofs(self):
+        return 3 * self.n_nodes
+
+    def update(self, u, dt=0.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().orc_update_internal_variables(self.h, _f(u), dt)
+
+    def save(self, converged):
+        lib().orc_save_internal_variables(self.h, int(converged))
+
+    def qp_state(self):
+        Q = 8 * self.n_elem
+        s, D, J, ep, eb, vm = (np.zeros(9 * Q), np.zeros(36 * Q), np.zeros(Q), np.zeros(9 * Q), np.zeros(Q),
+                               np.zeros(Q))
+        lib().orc_get_qp_state(self.h, _f(s), _f(D), _f(J), _f(ep), _f(eb), _f(vm))
+        return dict(sigma=s.reshape(Q, 3, 3), D=D.reshape(Q, 6, 6), detF=J, eps_p=ep.reshape(Q, 3, 3), ebar=eb, vm=vm)
+
+    def element(self, e):
+        ke, fe = np.zeros((24, 24)), np.zeros(24)
+        lib().orc_element_matrices(self.h, e, _f(ke), _f(fe))
+        return ke, fe
+
+    def pattern(self):
+        nnz = lib().orc_sparsity_nnz(self.h)
+        rp, ci = np.zeros(self.n_dofs + 1, np.int32), np.zeros(nnz, np.int32)
+        lib().orc_sparsity_fill(self.h, _i(rp), _i(ci))
+        return rp, ci
+
+    def assemble(self, rp, ci, threads=0):
+        val, f = np.zeros(len(ci)), np.zeros(self.n_dofs)
+        lib().orc_assemble(self.h, _i(rp), _i(ci), _f(val), _f(f), threads)
+        return val, f
+
+    def newton(self, options: NewtonOptions, boundaries, capacity=4096):
+        arr = (Dirichlet * max(1, len(boundaries)))()
+        keep = []
+        for k, bc in enumerate(boundaries):
+            d = np.ascontiguousarray(bc["dofs"], dtype=np.int32)
+            t = np.ascontiguousarray(bc["times"], dtype=np.float64)
+            v = np.ascontiguousarray(bc["values"], dtype=np.float64)
+            keep += [d, t, v]
+            arr[k] = Dirichlet(len(d), _i(d), len(t), _f(t), _f(v))
+        u = np.zeros(self.n_dofs)
+        recs = (NewtonRecord * capacity)()
+        n = C.c_int32(0)
+        code = lib().orc_newton_solve(self.h, C.byref(options), len(boundaries), arr, _f(u), recs, capacity,
+                                      C.byref(n))
+        records = [dict(step=r.step, iteration=r.iteration, solver_iterations=r.solver_iterations, status=r.status,
+                        step_time=r.step_time, displacement_norm=r.displacement_norm, force_norm=r.force_norm)
+                   for r in recs[: min(n.value, capacity)]]
+        return code, u, records
+
+
+def boundary_value(times, values, t):
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    return lib().orc_boundary_value(len(times), _f(times), _f(values), t)
+
+
+def enforce_dirichlet(rp, ci, val, b, fixed):
+    fixed = np.ascontiguousarray(fixed, dtype=np.int32)
+    lib().orc_enforce_dirichlet(len(rp) - 1, _i(rp), _i(ci), _f(val), _f(b), len(fixed), _i(fixed))
+
+
+def spmv(rp, ci, val, x, threads=0):
+    y = np.zeros(len(rp) - 1)
+    lib().orc_spmv(len(rp) - 1, _i(rp), _i(ci), _f(val), _f(np.ascontiguousarray(x, dtype=np.float64)), _f(y), threads)
+    return y
+
+
+def _krylov(name, rp, ci, val, b, tol, max_it, threads=0):
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(rp) - 1)
+    its, err = C.c_int32(0), C.c_double(0)
+    code = getattr(lib(), name)(len(rp) - 1, _i(rp), _i(ci), _f(val), _f(b), _f(x), tol, max_it, threads,
+                                C.byref(its), C.byref(err))
+    return code, x, its.value, err.value
+
+
+def cg_eigen(*a, **k):
+    return _krylov("orc_cg_eigen", *a, **k)
+
+
+def cg_cuda(*a, **k):
+    return _krylov("orc_cg_cuda", *a, **k)
+
+
+def bicgstab_eigen(*a, **k):
+    return _krylov("orc_bicgstab_eigen", *a, **k)
+
+
+def bicgstab_cuda(*a, **k):
+    return _krylov("orc_bicgstab_cuda", *a, **k)
+
+
+def rcm(rp, ci):
+    perm = np.zeros(len(rp) - 1, np.int32)
+    lib().orc_rcm(len(rp) - 1, _i(rp), _i(ci), _i(perm))
+    return perm
+
+
+class TimeStep:
+    def __init__(self, period, initial, minimum, maximum, adaptive, history):
+        opt = TimeOptions(period, initial, minimum, maximum, int(adaptive))
+        h = np.ascontiguousarray(history, dtype=np.float64)
+        self.h = C.c_void_p()
+        code = lib().orc_time_create(C.byref(opt), len(h), _f(h), C.byref(self.h))
+        if code != OK:
+            raise RuntimeError("orc_time_create failed")
+
+    def __del__(self):
+        try:
+            lib().orc_time_destroy(self.h)
+        except Exception:
+            pass
+
+    def update(self, converged):
+        return lib().orc_time_update(self.h, int(converged))
+
+    def query(self):
+        st, lt = C.c_double(), C.c_double()
+        fa, su = C.c_int32(), C.c_int32()
+        lib().orc_time_query(self.h, C.byref(st), C.byref(lt), C.byref(fa), C.byref(su))
+        return dict(step_time=st.value, last_step_time=lt.value, fully_applied=bool(fa.value), successful=su.value)
+
+
+def newton_options(sim: dict, criterion=STOP_RELATIVE, threads=0, rcm=0, solver=0):
+    t = sim["time"]
+    inc = t["increments"]
+    nl = sim["nonlinear_options"]
+    ls = sim.get("linear_solver", {})
+    return NewtonOptions(
+        TimeOptions(t["period"], inc["initial"], inc["minimum"], inc["maximum"], int(bool(inc["adaptive"]))),
+        nl["residual_tolerance"], nl["displacement_tolerance"], nl.get("linear_iterations", 10),
+        int("absolute_tolerance" in nl), sim.get("norm_mode", 0), ls.get("tolerance", 1e-5),
+        ls.get("maximum_iterations", 2000), criterion, solver, threads, rcm)

@@ -1,0 +1,120 @@
+"""HIP BiCGSTAB and mixed-precision PCG against the CPU FP64 oracle (through the C ABI).
+
+Mixed precision stops on the true FP64 residual of the FP32-valued tangent, so its solution
+is compared with the oracle's FP64 solve of that same FP32-rounded matrix (1e-8 relative to
+max|x|), and the true residual is checked directly (<= tol ||b||).  Newton in mixed precision
+converges to the FP64 f_int root; its displacement is compared with the FP64 oracle run
+(error stated per test)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import neon_amd
+from neon_amd import problems
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(p, scale=0.01, seed=3):
+    m = orc.Model(p.X, p.conn, p.params, model=p.model)
+    u = scale * np.random.default_rng(seed).normal(size=p.n_dofs)
+    assert m.update(u) == orc.OK
+    rp, ci = m.pattern()
+    val, f = m.assemble(rp, ci)
+    b = -f + 1e-3 * np.random.default_rng(seed + 1).normal(size=p.n_dofs)
+    orc.enforce_dirichlet(rp, ci, val, b, p.fixed_dofs())
+    return rp, ci, val, b
+
+
+@pytest.mark.parametrize("criterion", [neon_amd.STOP_RELATIVE, neon_amd.STOP_ABSOLUTE_PRECONDITIONED])
+def test_bicgstab_host_csr_parity(ctx, criterion):
+    p = problems.cube_clamped(6)
+    rp, ci, val, b = _system(p)
+    tol = 1e-10 if criterion == neon_amd.STOP_RELATIVE else 1e-12
+    ref = orc.bicgstab_eigen if criterion == neon_amd.STOP_RELATIVE else orc.bicgstab_cuda
+    code, x0, its0, _ = ref(rp, ci, val, b, tol, 20000)
+    assert code == orc.OK
+    s = neon_amd.biconjugate_gradient_stabilised_hip(tol, 20000, criterion=criterion, context=ctx)
+    x = np.zeros_like(b)
+    s.solve((rp, ci, val), x, b)
+    assert abs(s.iterations - its0) <= 1
+    np.testing.assert_allclose(x, x0, rtol=0, atol=1e-9 * np.abs(x0).max())
+
+
+def test_bicgstab_unsymmetric_system(ctx):
+    # an unsymmetric diagonally dominant matrix: BiCGSTAB converges where CG would not apply
+    rng = np.random.default_rng(9)
+    n = 400
+    A = sp.random(n, n, density=0.02, random_state=3, format="csr")
+    A = A + sp.diags(np.abs(A).sum(axis=1).A1 + 1.0)
+    A = A.tocsr()
+    A.sort_indices()
+    b = rng.normal(size=n)
+    s = neon_amd.make_linear_solver({"type": "iterative", "device": "gpu", "tolerance": 1e-12,
+                                     "backend": {"type": "hip"}}, is_symmetric=False, context=ctx)
+    assert isinstance(s, neon_amd.biconjugate_gradient_stabilised_hip)
+    x = np.zeros(n)
+    s.solve(A, x, b)
+    assert np.linalg.norm(A @ x - b) < 1e-10 * np.linalg.norm(b)
+
+
+def test_mixed_cg_host_csr(ctx):
+    p = problems.cube_clamped(10)
+    rp, ci, val, b = _system(p)
+    val32 = val.astype(np.float32).astype(np.float64)
+    A32 = sp.csr_matrix((val32, ci, rp))
+    code, x0, its0, _ = orc.cg_eigen(rp, ci, val32, b, 1e-10, 20000)
+    assert code == orc.OK
+    s = neon_amd.conjugate_gradient_hip(1e-10, 20000, criterion=neon_amd.STOP_RELATIVE,
+                                        precision=neon_amd.PRECISION_MIXED, context=ctx)
+    x = np.zeros_like(b)
+    s.solve((rp, ci, val), x, b)
+    assert np.linalg.norm(b - A32 @ x) <= 1e-10 * np.linalg.norm(b)
+    np.testing.assert_allclose(x, x0, rtol=0, atol=1e-8 * np.abs(x0).max())
+    # reliable updates keep the Krylov space: iteration count close to the FP64 solver's
+    assert s.iterations <= 1.25 * its0 + 5
+
+
+def test_mixed_device_tangent_solve(ctx):
+    p = problems.block_rollers(8)
+    mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    u = 0.005 * np.random.default_rng(6).normal(size=p.n_dofs)
+    mg.set_displacement(u)
+    mg.update_internal_variables()
+    mg.assemble(neon_amd.PRECISION_MIXED)
+    _, f = mg.download(stiffness=False)
+    rp, ci = mg.pattern()
+    b = -f.copy()
+    mg.enforce_dirichlet(p.fixed_dofs(), b)
+    val32, _ = mg.download(internal_force=False)
+    x, its, err = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
+    A32 = sp.csr_matrix((val32, ci, rp))
+    assert np.linalg.norm(b - A32 @ x) <= 1e-10 * np.linalg.norm(b)
+    # FP32 tangent entries equal the FP64 assembly rounded per element contribution
+    m = orc.Model(p.X, p.conn, p.params)
+    assert m.update(u) == orc.OK
+    val64, f64 = m.assemble(rp, ci)
+    np.testing.assert_allclose(f, f64, rtol=0, atol=1e-12 * np.abs(f64).max())
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    fixed = set(p.fixed_dofs().tolist())
+    free = np.array([r not in fixed and c not in fixed for r, c in zip(rows, ci)])
+    rowmax = np.maximum.reduceat(np.abs(val64), rp[:-1])[rows]
+    assert (np.abs(val32 - val64)[free] / rowmax[free]).max() < 1e-6
+
+
+def test_mixed_newton_block(ctx):
+    p = problems.block_rollers(6, residual_tolerance=1e-8, displacement_tolerance=1e-8, norm_mode=1,
+                               precision="mixed")
+    mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    sm = neon_amd.static_matrix(mg, p.simulation, p.boundaries)
+    u = sm.solve()
+    ref = orc.Model(p.X, p.conn, p.params)
+    code, u0, recs0 = ref.newton(orc.newton_options(p.simulation), p.boundaries)
+    assert code == orc.OK
+    assert all(r["status"] == 0 for r in sm.records)
+    assert len({r["step"] for r in sm.records}) == len({r["step"] for r in recs0})
+    # converged to the same equilibrium: error of the mixed path against the FP64 oracle
+    err = np.abs(u - u0).max() / np.abs(u0).max()
+    assert err < 1e-7, err
+    assert sm.records[-1]["force_norm"] <= 1e-8
