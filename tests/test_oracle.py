@@ -493,3 +493,39 @@ def test_j2_uniaxial_bar_matches_1d_return_map():
     ebar = (Emod * eps - sy) / (Emod + Hh)
     np.testing.assert_allclose(st["ebar"], ebar, rtol=1e-6)
     np.testing.assert_allclose(st["vm"], sy + Hh * ebar, rtol=1e-6)
+
+
+def test_j2_consistent_tangent_is_fd_jacobian_at_plastic_state():
+    # consistent mode: return map from the converged history, small strain on X, so the
+    # algorithmic tangent is the exact Jacobian of f_int at a fully plastic state
+    p = problems.notched_plate_j2(6, 3, 2, hole_cells=0.0, jitter=0.1)
+    m = orc.Model(p.X, p.conn, p.params, model=p.model, tangent=orc.TANGENT_CONSISTENT)
+    u0 = np.zeros(p.n_dofs)
+    u0[0::3] = 0.003 * p.X[:, 0]
+    u0 += 1e-4 * np.random.default_rng(0).normal(size=p.n_dofs)
+    rp, ci = m.pattern()
+    assert m.update(u0) == orc.OK
+    assert (m.qp_state()["ebar"] > 0).all()
+    val, _ = m.assemble(rp, ci)
+    K = sp.csr_matrix((val, ci, rp))
+    d = np.random.default_rng(1).normal(size=p.n_dofs)
+    h = 1e-7
+    m.update(u0 + h * d)
+    _, fp = m.assemble(rp, ci)
+    m.update(u0 - h * d)
+    _, fm = m.assemble(rp, ci)
+    fd = (fp - fm) / (2 * h)
+    assert np.linalg.norm(K @ d - fd) / np.linalg.norm(fd) < 1e-6
+
+
+def test_j2_consistent_newton_converges_quadratically():
+    p = problems.notched_plate_j2(12, 6, 3, steps=5, hole_cells=0.0, residual_tolerance=1e-8,
+                                  displacement_tolerance=1e-8, norm_mode=1)
+    m = orc.Model(p.X, p.conn, p.params, model=p.model, tangent=orc.TANGENT_CONSISTENT)
+    code, u, recs = m.newton(orc.newton_options(p.simulation), p.boundaries)
+    assert code == orc.OK
+    # after the elastic predictor the residual falls at least quadratically
+    step2 = [r["force_norm"] for r in recs if r["step"] == 2]
+    assert len(step2) <= 6
+    for a, b in zip(step2[1:-1], step2[2:]):
+        assert b < 30 * a * a
