@@ -73,7 +73,7 @@ def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
                                  final_displacement_norm=sm.records[-1]["displacement_norm"],
                                  cg_iterations=[r["solver_iterations"] for r in sm.records])
     log(f"[{precision}] load step 1 to 1e-8: {len(sm.records)} Newton its, {t1 - t0:.3f} s, "
-        f"force norms {[f'{r['force_norm']:.2e}' for r in sm.records]}")
+        f"force norms {['%.2e' % r['force_norm'] for r in sm.records]}")
     return timed
 
 

@@ -136,6 +136,18 @@ int neon_mesh_enforce_dirichlet(neon_context* ctx, int32_t n_fixed, const int32_
 int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double tolerance, int32_t max_iterations,
                     const double* b, double* x, int32_t* iterations, double* error);
 
+/* Time reps SpMVs y = K x (x = 1) of the assembled tangent in the given precision with
+ * `lanes` (16, 32) lanes per block row, the variant `flags` below and `blocks` workgroups
+ * (0: default); average milliseconds per SpMV.  No counterpart in the reference: a tuning and
+ * roofline probe. */
+#define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads */
+#define NEON_SPMV_SWEEP 2       /* workgroups of one XCD sweep its node range together */
+int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
+                         int32_t reps, double* avg_ms);
+/* Average milliseconds of a streaming read and of a copy of `bytes` bytes (calibration of the
+ * achievable HBM bandwidth and of the traffic counters). */
+int neon_stream_probe(neon_context* ctx, int64_t bytes, int32_t reps, double* read_ms, double* copy_ms);
+
 /* ---- Newton driver (static_matrix::solve) ---- */
 typedef struct {
     int32_t n_dofs;      /* number of constrained dofs */

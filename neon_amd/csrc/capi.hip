@@ -324,6 +324,67 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
     });
 }
 
+int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
+                         int32_t reps, double* avg_ms)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        device_mesh& m = *ctx->mesh;
+        if (lanes != 16 && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "lanes must be 16 or 32");
+        if (reps <= 0 || !avg_ms) throw status_error(NEON_BAD_ARGUMENT, "bad repetitions");
+        linear_operator A = m.op(precision);
+        A.nb_lanes = lanes;
+        A.nb_nt = (flags & NEON_SPMV_NONTEMPORAL) != 0;
+        A.nb_sweep = (flags & NEON_SPMV_SWEEP) != 0;
+        A.blocks_override = blocks;
+        int64_t const n = m.dofs();
+        hipStream_t const s = ctx->stream;
+        hipEvent_t e0, e1;
+        NEON_CHECK_HIP(hipEventCreate(&e0));
+        NEON_CHECK_HIP(hipEventCreate(&e1));
+        float ms = 0.0f;
+        if (precision == NEON_PRECISION_FP64)
+        {
+            device_array<double> x, y;
+            x.resize(n);
+            y.resize(n);
+            fill_ones(s, n, x.data());
+            A.apply64(s, x.data(), y.data(), nullptr, nullptr); // warm
+            NEON_CHECK_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) A.apply64(s, x.data(), y.data(), nullptr, nullptr);
+            NEON_CHECK_HIP(hipEventRecord(e1, s));
+            NEON_CHECK_HIP(hipEventSynchronize(e1));
+        }
+        else
+        {
+            device_array<float> x, y;
+            x.resize(n);
+            y.resize(n);
+            fill_ones(s, n, x.data());
+            A.apply32(s, x.data(), y.data(), nullptr, nullptr);
+            NEON_CHECK_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) A.apply32(s, x.data(), y.data(), nullptr, nullptr);
+            NEON_CHECK_HIP(hipEventRecord(e1, s));
+            NEON_CHECK_HIP(hipEventSynchronize(e1));
+        }
+        NEON_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_ms = double(ms) / reps;
+        return NEON_OK;
+    });
+}
+
+int neon_stream_probe(neon_context* ctx, int64_t bytes, int32_t reps, double* read_ms, double* copy_ms)
+{
+    return guarded(ctx, [&] {
+        if (bytes < 8 || reps <= 0) throw status_error(NEON_BAD_ARGUMENT, "bad probe arguments");
+        if (read_ms) *read_ms = stream_probe(ctx->stream, bytes, reps, false);
+        if (copy_ms) *copy_ms = stream_probe(ctx->stream, bytes, reps, true);
+        return NEON_OK;
+    });
+}
+
 int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int32_t n_bc, const neon_dirichlet* bcs,
                       double* u_out, neon_newton_record* records, int32_t record_capacity, int32_t* n_records,
                       neon_newton_timing* timing)

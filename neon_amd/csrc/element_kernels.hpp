@@ -50,6 +50,8 @@ struct qp_state
     double* ebar;
     double* vm;
     int64_t Q;
+    const double* eps_p_old; // committed history (consistent J2 returns from it)
+    const double* ebar_old;
 };
 
 struct element_params
@@ -162,9 +164,18 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
     {
         // small-strain J2 with linear isotropic hardening, history updated in place
         double const G = shear, sy = prm.p[4 * e + 2], Hh = prm.p[4 * e + 3];
+        // neon accumulates the history in place across Newton iterates; the consistent mode
+        // returns from the last converged history
+        bool const from_old = tangent_mode == NEON_TANGENT_CONSISTENT;
+        const double* ep_src = from_old ? st.eps_p_old : st.eps_p;
         double ep[6];
-        for (int k = 0; k < 6; ++k) ep[k] = st.eps_p[k * Q + qp];
-        double eb = st.ebar[qp];
+        for (int k = 0; k < 6; ++k) ep[k] = ep_src[k * Q + qp];
+        double eb = from_old ? st.ebar_old[qp] : st.ebar[qp];
+        if (from_old)
+        {
+            for (int k = 0; k < 6; ++k) st.eps_p[k * Q + qp] = ep[k];
+            st.ebar[qp] = eb;
+        }
         // strain = sym(H) in tensor components (11,22,33,23,13,12)
         double eps[6] = {H[0], H[4], H[8], 0.5 * (H[5] + H[7]), 0.5 * (H[2] + H[6]), 0.5 * (H[1] + H[3])};
         double ee[6];
@@ -309,7 +320,8 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
         {
             int64_t const nd = conn[8 * e + lane];
             s_node[lane] = int32_t(nd);
-            for (int i = 0; i < 3; ++i) s_x[lane][i] = X[3 * nd + i] + u[3 * nd + i];
+            // current configuration x = X + u; u == nullptr integrates on X (small strain)
+            for (int i = 0; i < 3; ++i) s_x[lane][i] = u ? X[3 * nd + i] + u[3 * nd + i] : X[3 * nd + i];
         }
         __syncthreads();
         // phase A: lane = (l = hi, a = lo)
@@ -388,6 +400,7 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
                         k[i * 3 + j] += v;
                     }
             }
+            // CSR order: row 3 n_a + i holds 3 deg(n_a) values, block slot s at 3 s
             int32_t const na_ = s_node[a];
             int32_t const b0 = bptr[na_];
             int64_t const deg = bptr[na_ + 1] - b0;
@@ -447,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void dirichlet_kernel(int32_t n_fixed, cons
     for (int32_t s = 0; s < deg; ++s)
     {
         int32_t const m = bcol[b0 + s];
-        // row d
+        // row d: entries (i, j) of block (nd, m)
         for (int j = 0; j < 3; ++j)
         {
             if (m == nd && j == i) continue;
@@ -464,6 +477,7 @@ __global__ __launch_bounds__(kBlock) void dirichlet_kernel(int32_t n_fixed, cons
                 break;
             }
         if (slot < 0) continue;
+        // column d: entries (j, i) of block (m, nd)
         for (int j = 0; j < 3; ++j)
         {
             if (m == nd && j == i) continue;

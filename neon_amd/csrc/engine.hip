@@ -33,7 +33,8 @@ inline int xcd_blocks(int64_t want, int cap)
 int linear_operator::spmv_blocks() const
 {
     int64_t const rows = node_block ? nodes : n;
-    int64_t const rows_per_pass = node_block ? kBlock / 32 : kBlock / lanes_per_row;
+    int64_t const rows_per_pass = node_block ? kBlock / nb_lanes : kBlock / lanes_per_row;
+    if (blocks_override > 0) return xcd_blocks(blocks_override, kMaxPartials);
     // about 4 passes per block, at most kMaxPartials blocks
     return xcd_blocks((rows + 4 * rows_per_pass - 1) / (4 * rows_per_pass), kMaxPartials);
 }
@@ -48,7 +49,21 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const X
     if (A.node_block)
     {
         nb_matrix m{A.nodes, A.bptr, A.bcol};
-        spmv_nb_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
+#define NEON_NB(L, NT, SW) \
+    spmv_nb_kernel<L, 1, NT, SW, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate)
+        int const key = (A.nb_lanes == 16 ? 0 : 4) + (A.nb_nt ? 2 : 0) + (A.nb_sweep ? 1 : 0);
+        switch (key)
+        {
+        case 0: NEON_NB(16, false, false); break;
+        case 1: NEON_NB(16, false, true); break;
+        case 2: NEON_NB(16, true, false); break;
+        case 3: NEON_NB(16, true, true); break;
+        case 4: NEON_NB(32, false, false); break;
+        case 5: NEON_NB(32, false, true); break;
+        case 6: NEON_NB(32, true, false); break;
+        default: NEON_NB(32, true, true); break;
+        }
+#undef NEON_NB
     }
     else
     {
@@ -497,6 +512,58 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
     return res;
 }
 
+namespace
+{
+template <typename T>
+__global__ __launch_bounds__(kBlock) void fill_kernel(int64_t n, T v, T* __restrict__ x)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+} // namespace
+
+void fill_ones(hipStream_t s, int64_t n, double* x)
+{
+    fill_kernel<double><<<grid_for(n), kBlock, 0, s>>>(n, 1.0, x);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+void fill_ones(hipStream_t s, int64_t n, float* x)
+{
+    fill_kernel<float><<<grid_for(n), kBlock, 0, s>>>(n, 1.0f, x);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+double stream_probe(hipStream_t s, int64_t bytes, int reps, bool copy)
+{
+    int64_t const n = bytes / 8;
+    device_array<double> a, b, part;
+    a.resize(n);
+    part.resize(4096);
+    if (copy) b.resize(n);
+    fill_ones(s, n, a.data());
+    hipEvent_t e0, e1;
+    NEON_CHECK_HIP(hipEventCreate(&e0));
+    NEON_CHECK_HIP(hipEventCreate(&e1));
+    auto launch = [&]() {
+        if (copy)
+            stream_copy_kernel<double><<<grid_for(n), kBlock, 0, s>>>(n, a.data(), b.data());
+        else
+            stream_sum_kernel<<<4096, kBlock, 0, s>>>(n, a.data(), part.data());
+    };
+    launch();
+    NEON_CHECK_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    NEON_CHECK_HIP(hipEventRecord(e1, s));
+    NEON_CHECK_HIP(hipEventSynchronize(e1));
+    NEON_CHECK_HIP(hipGetLastError());
+    float ms = 0.0f;
+    NEON_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return double(ms) / reps;
+}
+
 void convert_to_float(hipStream_t s, int64_t n, const double* x, float* y)
 {
     if (n <= 0) return;
@@ -697,7 +764,8 @@ void device_mesh::set_displacement_host(const double* u)
 int device_mesh::update_internal_variables(const double* u_dev)
 {
     int64_t const Q = 8ll * E_;
-    qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q};
+    qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q,
+                eps_p_old_.data(), ebar_old_.data()};
     NEON_CHECK_HIP(hipMemsetAsync(status_.data(), 0, sizeof(uint32_t), s_));
     element_params prm{params_.data()};
     if (model_ == NEON_MODEL_NEOHOOKE)
@@ -739,8 +807,12 @@ void device_mesh::save_internal_variables(bool converged)
 void device_mesh::assemble(const double* u_dev, int precision)
 {
     int64_t const Q = 8ll * E_;
-    qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q};
+    qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q,
+                eps_p_old_.data(), ebar_old_.data()};
     bool const finite = model_ == NEON_MODEL_NEOHOOKE; // is_finite_deformation()
+    // neon integrates the small-strain model on the current configuration (submesh.cpp:68,84);
+    // the consistent mode integrates it on X
+    if (model_ == NEON_MODEL_J2 && tangent_mode_ == NEON_TANGENT_CONSISTENT) u_dev = nullptr;
     hipEvent_t e0, e1;
     NEON_CHECK_HIP(hipEventCreate(&e0));
     NEON_CHECK_HIP(hipEventCreate(&e1));
@@ -837,18 +909,19 @@ void device_mesh::download(double* val, double* f_int, int)
 {
     if (val)
     {
+        // the device keeps the values in CSR order (FP64, or FP32 widened here)
+        if (!k64_valid_ && !k32_valid_) throw status_error(NEON_BAD_ARGUMENT, "no tangent has been assembled");
         if (k64_valid_)
             NEON_CHECK_HIP(hipMemcpyAsync(val, k64_.data(), nnz_ * sizeof(double), hipMemcpyDeviceToHost, s_));
-        else if (k32_valid_)
+        else
         {
             device_array<double> tmp;
             tmp.resize(nnz_);
             convert_kernel<float, double><<<grid_for(nnz_), kBlock, 0, s_>>>(nnz_, k32_.data(), tmp.data());
+            NEON_CHECK_HIP(hipGetLastError());
             NEON_CHECK_HIP(hipMemcpyAsync(val, tmp.data(), nnz_ * sizeof(double), hipMemcpyDeviceToHost, s_));
-            NEON_CHECK_HIP(hipStreamSynchronize(s_));
         }
-        else
-            throw status_error(NEON_BAD_ARGUMENT, "no tangent has been assembled");
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
     }
     if (f_int) NEON_CHECK_HIP(hipMemcpyAsync(f_int, fint_.data(), 3ull * N_ * sizeof(double), hipMemcpyDeviceToHost, s_));
     NEON_CHECK_HIP(hipStreamSynchronize(s_));

@@ -105,6 +105,12 @@ struct linear_operator
     const int32_t* rp = nullptr;
     const int32_t* ci = nullptr;
     int lanes_per_row = 8;
+    // node-block SpMV shape: lanes per block row (16 or 32), nontemporal value loads, XCD
+    // sweep node order, optional workgroup count (tuning)
+    int nb_lanes = 32;
+    bool nb_nt = false;
+    bool nb_sweep = false;
+    int blocks_override = 0;
     // values
     const double* v64 = nullptr;
     const float* v32 = nullptr;
@@ -229,6 +235,10 @@ private:
 
 void upload_hex8_tables();
 void convert_to_float(hipStream_t s, int64_t n, const double* x, float* y);
+void fill_ones(hipStream_t s, int64_t n, double* x);
+// average ms of a streaming read (copy=false) or copy (copy=true) of `bytes` bytes
+double stream_probe(hipStream_t s, int64_t bytes, int reps, bool copy);
+void fill_ones(hipStream_t s, int64_t n, float* x);
 } // namespace neon_amd
 
 struct neon_context

@@ -43,10 +43,9 @@ enum : int32_t
 };
 
 // ----------------------------------------------------------------- SpMV layouts
-// Node-block CSR: the tangent of a 3-dof-per-node mesh stores its values in ordinary
-// row-major CSR order (sorted columns), but every dof row of node n has the same
-// column set {3m+j : m in adj(n), j < 3}.  The kernel therefore reads one int32 block
-// column per 9 values instead of one per value:
+// Node-block CSR: the tangent of a 3-dof-per-node mesh keeps its values in ordinary
+// row-major CSR order (sorted columns), but every dof row of node n has the same column set
+// {3m+j : m in adj(n), j < 3}, so the kernels read one int32 block column per 9 values:
 //   row 3n+i, block s (node m = bcol[bptr[n]+s]), column 3m+j
 //   -> val[9 bptr[n] + i (3 deg n) + 3 s + j].
 struct nb_matrix
@@ -63,42 +62,81 @@ struct csr_matrix
     const int32_t* ci;
 };
 
-// 32 lanes per node row; y receives the three dof rows, reduced across the 32 lanes.
-template <typename VT, typename XT, typename AT>
-__device__ inline void nb_row(nb_matrix A, const VT* __restrict__ val, const XT* __restrict__ x, int32_t n,
-                              int lane, AT& y0, AT& y1, AT& y2)
+template <bool NT, typename T>
+__device__ inline T ld(const T* p)
 {
-    int32_t const b0 = A.bptr[n];
-    int32_t const d = A.bptr[n + 1] - b0;
-    int64_t const base = 9ll * b0;
-    int64_t const rs = 3ll * d;
-    AT s0 = 0, s1 = 0, s2 = 0;
-    for (int s = lane; s < d; s += 32)
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+// ROWS block rows per group of LPR lanes; one lane per 3x3 block (node pair): it reads the
+// block's column index, the three x entries of the column node and the 9 values (3 runs of 3
+// contiguous values, one per dof row), then the three row sums are reduced across the group.
+// NT streams the values with nontemporal loads (read once per product) so that x stays cached.
+template <int LPR, int ROWS, bool NT, typename VT, typename XT, typename AT>
+__device__ inline void nb_rows(nb_matrix A, const VT* __restrict__ val, const XT* __restrict__ x,
+                               const int32_t* n, const bool* ok, int lane, AT (*y)[3])
+{
+    int32_t b0[ROWS], d[ROWS];
+    AT s[ROWS][3];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
     {
-        int64_t const m = A.bcol[b0 + s];
-        AT const x0 = AT(x[3 * m]), x1 = AT(x[3 * m + 1]), x2 = AT(x[3 * m + 2]);
-        const VT* v = val + base + 3 * s;
-        s0 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
-        v += rs;
-        s1 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
-        v += rs;
-        s2 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
+        b0[r] = ok[r] ? A.bptr[n[r]] : 0;
+        d[r] = ok[r] ? A.bptr[n[r] + 1] - b0[r] : 0;
+        s[r][0] = s[r][1] = s[r][2] = AT(0);
+    }
+    int32_t dmax = d[0];
+#pragma unroll
+    for (int r = 1; r < ROWS; ++r) dmax = d[r] > dmax ? d[r] : dmax;
+    for (int q = lane; q < dmax; q += LPR)
+    {
+        VT v[ROWS][9];
+        AT xs[ROWS][3];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+        {
+            bool const on = q < d[r];
+            int64_t const m = on ? A.bcol[b0[r] + q] : 0;
+            int64_t const rs = 3ll * d[r];
+            const VT* p = val + 9ll * b0[r] + 3 * q;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) v[r][3 * i + j] = on ? ld<NT>(p + i * rs + j) : VT(0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) xs[r][j] = on ? AT(x[3 * m + j]) : AT(0);
+        }
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                s[r][i] += AT(v[r][3 * i]) * xs[r][0] + AT(v[r][3 * i + 1]) * xs[r][1] + AT(v[r][3 * i + 2]) * xs[r][2];
     }
 #pragma unroll
-    for (int off = 16; off > 0; off >>= 1)
-    {
-        s0 += __shfl_xor(s0, off, 32);
-        s1 += __shfl_xor(s1, off, 32);
-        s2 += __shfl_xor(s2, off, 32);
-    }
-    y0 = s0;
-    y1 = s1;
-    y2 = s2;
+    for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            AT t = s[r][i];
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, LPR);
+            y[r][i] = t;
+        }
 }
 
 // y = A x (node-block layout); with DOT, also writes the block partial of x . y.
 // MODE 0: y = Ax.  MODE 1: y = b - Ax (residual; b given in `bvec`).
-template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+// Node order: each XCD (workgroups b with b % 8 equal) owns one contiguous eighth of the
+// nodes.  SWEEP=false: every workgroup walks its own contiguous chunk of that eighth.
+// SWEEP=true: the workgroups of one XCD advance together through the eighth, workgroup j
+// taking node groups j, j + W, j + 2W, ... (W workgroups per XCD), so the x entries that the
+// concurrently running workgroups gather (neighbours within about one node plane) stay in
+// that XCD's L2 instead of spanning the whole eighth.
+template <int LPR, int ROWS, bool NT, bool SWEEP, typename VT, typename XT, typename YT, typename AT, bool DOT,
+          int MODE>
 __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* __restrict__ val,
                                                          const XT* __restrict__ x, YT* __restrict__ y,
                                                          const double* __restrict__ bvec,
@@ -106,23 +144,51 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
                                                          const krylov_state* __restrict__ st, int gate)
 {
     if (st && (st->done || (gate && !st->refine))) return;
-    int64_t beg, end;
-    xcd_chunk(A.nodes, beg, end);
-    int const g = threadIdx.x >> 5, l = threadIdx.x & 31;
-    double dot = 0.0;
-    for (int64_t n = beg + g; n < end; n += kBlock / 32)
+    constexpr int G = kBlock / LPR; // lane groups per block
+    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    int64_t start, end, stride;
+    if (SWEEP && (gridDim.x & 7) == 0)
     {
-        AT y0, y1, y2;
-        nb_row<VT, XT, AT>(A, val, x, int32_t(n), l, y0, y1, y2);
+        int64_t const per = (int64_t(A.nodes) + 7) / 8;
+        int64_t const beg = int64_t(blockIdx.x & 7) * per;
+        end = beg + per < A.nodes ? beg + per : A.nodes;
+        stride = int64_t(gridDim.x >> 3) * G;
+        start = beg + int64_t(blockIdx.x >> 3) * G + g;
+    }
+    else
+    {
+        int64_t beg;
+        xcd_chunk(A.nodes, beg, end);
+        stride = G;
+        start = beg + g;
+    }
+    double dot = 0.0;
+    for (int64_t n0 = start; n0 < end; n0 += stride * ROWS)
+    {
+        int32_t n[ROWS];
+        bool ok[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+        {
+            ok[r] = n0 + r * stride < end;
+            n[r] = ok[r] ? int32_t(n0 + r * stride) : 0;
+        }
+        AT yy[ROWS][3];
+        nb_rows<LPR, ROWS, NT, VT, XT, AT>(A, val, x, n, ok, l, yy);
         if (l < 3)
         {
-            AT const yi = l == 0 ? y0 : (l == 1 ? y1 : y2);
-            int64_t const r = 3 * n + l;
-            if constexpr (MODE == 1)
-                y[r] = YT(bvec[r] - double(yi));
-            else
-                y[r] = YT(yi);
-            if constexpr (DOT) dot += double(x[r]) * double(YT(yi));
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r)
+            {
+                if (!ok[r]) continue;
+                AT const yi = l == 0 ? yy[r][0] : (l == 1 ? yy[r][1] : yy[r][2]);
+                int64_t const row = 3ll * n[r] + l;
+                if constexpr (MODE == 1)
+                    y[row] = YT(bvec[row] - double(yi));
+                else
+                    y[row] = YT(yi);
+                if constexpr (DOT) dot += double(x[row]) * double(YT(yi));
+            }
         }
     }
     if constexpr (DOT)
@@ -130,6 +196,32 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
         dot = block_sum<kBlock>(dot);
         if (threadIdx.x == 0) part[blockIdx.x] = dot;
     }
+}
+
+// streaming read of n doubles in 16-byte loads (calibration of the achievable read bandwidth)
+__global__ __launch_bounds__(kBlock) void stream_sum_kernel(int64_t n, const double* __restrict__ v,
+                                                            double* __restrict__ out)
+{
+    int64_t beg, end;
+    xcd_chunk(n / 2, beg, end);
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2* v2 = reinterpret_cast<const d2*>(v);
+    double s = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        d2 const t = __builtin_nontemporal_load(v2 + i);
+        s += t.x + t.y;
+    }
+    s = block_sum<kBlock>(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// copy of n values (calibration of the FETCH_SIZE / WRITE_SIZE counters)
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void stream_copy_kernel(int64_t n, const VT* __restrict__ a, VT* __restrict__ b)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) b[i] = a[i];
 }
 
 // Generic CSR (any sparsity): LPR lanes per row.

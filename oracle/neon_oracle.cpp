@@ -382,6 +382,13 @@ int orc_update_internal_variables(orc_model* m, const double* u, double /*dt*/)
         for (int64_t q = 0; q < int64_t(Q); ++q)
         {
             double const* p = &m->params[4 * (q / 8)];
+            if (m->tangent_mode == ORC_TANGENT_CONSISTENT)
+            {
+                // consistent mode: return map from the last converged history (backward Euler
+                // from t_n), instead of neon's in-place accumulation across Newton iterates
+                std::copy(&m->eps_p_o[9 * q], &m->eps_p_o[9 * q] + 9, &m->eps_p[9 * q]);
+                m->ebar[q] = m->ebar_o[q];
+            }
             int const s = j2(&m->Hg[9 * q], p[0], p[1], p[2], p[3], &m->eps_p[9 * q], &m->ebar[q],
                              &m->sigma[9 * q], &m->D[36 * q], &m->vm[q]);
             if (s != ORC_OK) failed |= 1;
@@ -447,8 +454,12 @@ void element_matrices(const orc_model& m, int64_t e, double* ke, double* fe)
 {
     hex8 const& h = shape();
     double xe[24];
+    // neon integrates on the current configuration x (submesh.cpp:68,84), also for the
+    // small-strain J2 model; consistent mode integrates small strain on X.
+    bool const material_frame = m.model == ORC_MODEL_J2 && m.tangent_mode == ORC_TANGENT_CONSISTENT;
+    const std::vector<double>& cfg = material_frame ? m.X : m.x;
     for (int a = 0; a < 8; ++a)
-        for (int i = 0; i < 3; ++i) xe[a * 3 + i] = m.x[3 * size_t(m.conn[8 * e + a]) + i];
+        for (int i = 0; i < 3; ++i) xe[a * 3 + i] = cfg[3 * size_t(m.conn[8 * e + a]) + i];
     bool const want_k = ke != nullptr; // internal_force alone skips the stiffness
     if (want_k) std::fill(ke, ke + 576, 0.0);
     std::fill(fe, fe + 24, 0.0);
