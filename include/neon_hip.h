@@ -142,8 +142,14 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
  * roofline probe. */
 #define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads */
 #define NEON_SPMV_SWEEP 2       /* workgroups of one XCD sweep its node range together */
+#define NEON_SPMV_PIPELINE 4    /* software-pipelined rows (32 lanes, sweep order) */
 int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
                          int32_t reps, double* avg_ms);
+/* Shape of the SpMV that the solvers on this mesh's tangent use from now on (lanes, flags as
+ * above, workgroups for FP64 and for FP32 values; 0: by size).  The pipelined form applies to
+ * meshes whose nodes have at most 32 neighbours and falls back to the plain one otherwise.
+ * No counterpart in the reference: a tuning knob; results agree to rounding. */
+int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);
 /* Average milliseconds of a streaming read and of a copy of `bytes` bytes (calibration of the
  * achievable HBM bandwidth and of the traffic counters). */
 int neon_stream_probe(neon_context* ctx, int64_t bytes, int32_t reps, double* read_ms, double* copy_ms);
@@ -195,14 +201,16 @@ int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int
                       int32_t record_capacity, int32_t* n_records, neon_newton_timing* timing);
 
 /* ---- adjoint (no counterpart in the reference) ----
- * For the converged state held by the context (consistent tangent), with objective
- * J(u) whose gradient dJdu (3N, host) is given and the constrained dofs listed in
- * fixed: solves K^T lambda = dJ/du with the same Krylov solver (K symmetric) and returns
- * dJ/dp_e = -lambda^T dR/dp_e for p_e = the per-element shear modulus with the bulk
- * modulus tied by bulk = ratio * shear (grad: n_elem). */
+ * Gradient of an objective J(u) with respect to the per-element parameters at the
+ * displacement held by the context (the converged state of the last neon_newton_solve, or
+ * what neon_mesh_set_displacement set): K(u) is assembled with the consistent tangent, the
+ * constrained dofs `fixed` get the forward solve's Dirichlet rows and columns, K^T lambda =
+ * dJ/du (dJdu: 3N, host; K is symmetric) is solved with the Jacobi-PCG to the relative
+ * tolerance, and grad (4 per element, in the order of params) receives
+ * dJ/dp_e = -lambda^T d f_int / d p_e.  Needs the neo-Hooke model with
+ * NEON_TANGENT_CONSISTENT (the reference tangent is not dR/du).  iterations: PCG count. */
 int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed, const int32_t* fixed,
-                          double bulk_over_shear, double tolerance, int32_t max_iterations, double* grad,
-                          int32_t* iterations);
+                          double tolerance, int32_t max_iterations, double* grad, int32_t* iterations);
 
 /* ---- synthetic hex8 meshes (host only, no device use) ----
  * Structured nx*ny*nz block of size Lx*Ly*Lz; node n = i + (nx+1)(j + (ny+1)k).  Nodes are

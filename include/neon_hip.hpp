@@ -1,0 +1,360 @@
+/*
+ * neon_hip.hpp — C++17 host classes over the C ABI of neon_hip.h, shaped like dbeurle/neon's
+ * own interfaces so that code written against those switches by type name (paths relative to
+ * the reference repository root):
+ *
+ *   linear_solver, iterative_linear_solver  src/solver/linear/linear_solver.hpp:13-56
+ *   conjugate_gradient_hip                  conjugate_gradient_cuda (conjugate_gradient_cuda.hpp:17-27)
+ *   biconjugate_gradient_stabilised_hip     biconjugate_gradient_stabilised_cuda
+ *                                           (biconjugate_gradient_stabilised_cuda.hpp)
+ *   make_linear_solver                      src/solver/linear/linear_solver_factory.cpp:28-174
+ *   computational_error, hip_error          computational_error, cuda_error (src/exceptions.hpp:14-23)
+ *   device_mesh                             mechanics::solid::mesh + its constitutive models
+ *                                           (src/mesh/mechanics/solid/mesh.hpp:37-80)
+ *   static_matrix                           fem::static_matrix (src/assembler/mechanics/static_matrix.hpp:145-467)
+ *
+ * Recoverable failures (Krylov or Newton limits, non-finite values, det F < 0, radial return)
+ * throw computational_error, which neon's load-step driver catches to cut back; everything else
+ * throws hip_error (a std::domain_error, like neon's cuda_error).  Host containers are borrowed
+ * for the duration of a call; the context owns every device buffer.
+ */
+#pragma once
+
+#include "neon_hip.h"
+
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace neon_hip
+{
+/// A failure that a smaller load increment may cure (neon's computational_error).
+struct computational_error : public std::domain_error
+{
+    computational_error(int c, std::string const& what) : std::domain_error(what), code(c) {}
+    int code;
+};
+
+/// Any other failure: bad arguments, HIP runtime errors, exhausted cut-backs (neon's cuda_error).
+struct hip_error : public std::domain_error
+{
+    hip_error(int c, std::string const& what) : std::domain_error(what), code(c) {}
+    int code;
+};
+
+inline void check(neon_context* ctx, int code)
+{
+    if (code == NEON_OK) return;
+    std::string const msg = ctx ? neon_context_last_error(ctx) : (code == NEON_NO_DEVICE ? "no HIP device" : "");
+    if (code > 0) throw computational_error(code, msg);
+    throw hip_error(code, msg);
+}
+
+/// One device and one HIP stream; owns every device buffer made through it.
+class context
+{
+public:
+    explicit context(int device = 0) { check(nullptr, neon_context_create(device, &ctx_)); }
+    ~context()
+    {
+        if (ctx_) neon_context_destroy(ctx_);
+    }
+    context(context const&) = delete;
+    context& operator=(context const&) = delete;
+    neon_context* get() const { return ctx_; }
+
+    /// The process-wide context on device 0, created on first use (the CUDA solvers find
+    /// their device the same way, conjugate_gradient_cuda.cpp:15-51).
+    static context& default_context()
+    {
+        static context c(0);
+        return c;
+    }
+
+private:
+    neon_context* ctx_ = nullptr;
+};
+
+/// Row-major CSR with int32 indices and sorted columns: the layout of neon's sparse_matrix
+/// (Eigen::SparseMatrix<double, RowMajor>, src/numeric/sparse_matrix.hpp:15), borrowed.
+struct csr_view
+{
+    std::int32_t rows = 0;
+    std::int64_t nnz = 0;
+    const std::int32_t* row_ptr = nullptr;
+    const std::int32_t* col = nullptr;
+    const double* val = nullptr;
+};
+
+class linear_solver
+{
+public:
+    virtual ~linear_solver() = default;
+
+    virtual void solve(csr_view const& A, std::vector<double>& x, std::vector<double> const& b) = 0;
+
+    /// Notifies the solver of a change in the sparsity structure of A
+    void update_sparsity_pattern() { build_sparsity_pattern = true; }
+
+protected:
+    bool build_sparsity_pattern{true};
+};
+
+class iterative_linear_solver : public linear_solver
+{
+public:
+    explicit iterative_linear_solver() = default;
+    explicit iterative_linear_solver(double const tolerance) : residual_tolerance(tolerance) {}
+    explicit iterative_linear_solver(std::int32_t const iterations) : max_iterations(iterations) {}
+    explicit iterative_linear_solver(double const tolerance, std::int32_t const iterations)
+        : residual_tolerance(tolerance), max_iterations(iterations)
+    {
+    }
+
+    /// NEON_STOP_ABSOLUTE_PRECONDITIONED (the CUDA solvers' rule, the default) or
+    /// NEON_STOP_RELATIVE (the CPU solvers' rule)
+    int criterion = NEON_STOP_ABSOLUTE_PRECONDITIONED;
+    /// NEON_PRECISION_FP64 or NEON_PRECISION_MIXED (conjugate gradient only)
+    int precision = NEON_PRECISION_FP64;
+    /// device context; the process-wide one when null
+    context* ctx = nullptr;
+
+    /// Krylov iterations and final residual measure of the last solve
+    std::int32_t iterations() const { return iterations_; }
+    double error() const { return error_; }
+
+protected:
+    void run(int solver, csr_view const& A, std::vector<double>& x, std::vector<double> const& b)
+    {
+        if (b.size() != std::size_t(A.rows)) throw hip_error(NEON_BAD_ARGUMENT, "right-hand side size mismatch");
+        context& c = ctx ? *ctx : context::default_context();
+        x.assign(A.rows, 0.0);
+        check(c.get(), neon_krylov_solve_csr(c.get(), solver, criterion, precision, A.rows, A.nnz, A.row_ptr, A.col,
+                                             A.val, b.data(), x.data(), residual_tolerance, max_iterations,
+                                             &iterations_, &error_));
+        build_sparsity_pattern = false;
+    }
+
+    double residual_tolerance{1.0e-5};
+    std::int32_t max_iterations{2000};
+    std::int32_t iterations_ = 0;
+    double error_ = 0.0;
+};
+
+/// Jacobi-preconditioned conjugate gradient on the GPU (drop-in for conjugate_gradient_cuda).
+class conjugate_gradient_hip : public iterative_linear_solver
+{
+public:
+    using iterative_linear_solver::iterative_linear_solver;
+
+    void solve(csr_view const& A, std::vector<double>& x, std::vector<double> const& b) override
+    {
+        run(NEON_SOLVER_CG, A, x, b);
+    }
+};
+
+/// Jacobi-preconditioned BiCGSTAB on the GPU (drop-in for biconjugate_gradient_stabilised_cuda).
+class biconjugate_gradient_stabilised_hip : public iterative_linear_solver
+{
+public:
+    using iterative_linear_solver::iterative_linear_solver;
+
+    void solve(csr_view const& A, std::vector<double>& x, std::vector<double> const& b) override
+    {
+        run(NEON_SOLVER_BICGSTAB, A, x, b);
+    }
+};
+
+/// The fields of neon's "linear_solver" JSON object that select a GPU solver
+/// ({"type": "iterative", "device": "gpu", "backend": {"type": "hip"}, ...}).
+struct solver_options
+{
+    std::string type;
+    std::string device;
+    std::string backend;
+    std::string precision = "fp64";
+    std::optional<double> tolerance;
+    std::optional<std::int32_t> maximum_iterations;
+};
+
+namespace detail
+{
+template <typename Solver>
+std::unique_ptr<iterative_linear_solver> make_with(solver_options const& o)
+{
+    // make_iterative_solver (linear_solver_factory.cpp:28-65): the constructor matching the
+    // options given
+    if (o.tolerance && o.maximum_iterations) return std::make_unique<Solver>(*o.tolerance, *o.maximum_iterations);
+    if (o.tolerance) return std::make_unique<Solver>(*o.tolerance);
+    if (o.maximum_iterations) return std::make_unique<Solver>(*o.maximum_iterations);
+    return std::make_unique<Solver>();
+}
+} // namespace detail
+
+/// make_linear_solver (linear_solver_factory.cpp:67-174) for the GPU iterative solvers: conjugate
+/// gradient when the system is symmetric, BiCGSTAB otherwise.
+inline std::unique_ptr<linear_solver> make_linear_solver(solver_options const& o, bool const is_symmetric)
+{
+    if (o.type.empty())
+        throw std::domain_error("Did not find a linear solver type.  Did you try specifying \"type\" for the "
+                                "linear solver name?");
+    if (o.type != "iterative" || o.device != "gpu")
+        throw std::domain_error("this library provides the iterative gpu solvers only");
+    if (o.backend.empty()) throw std::domain_error("\"backend\" requires a \"type\" option");
+    if (o.backend != "hip") throw std::domain_error("Incorrect backend option specified.");
+    int precision;
+    if (o.precision == "fp64")
+        precision = NEON_PRECISION_FP64;
+    else if (o.precision == "mixed")
+        precision = NEON_PRECISION_MIXED;
+    else
+        throw std::domain_error("\"precision\" must be \"fp64\" or \"mixed\"");
+    if (precision == NEON_PRECISION_MIXED && !is_symmetric)
+        throw std::domain_error("mixed precision supports the conjugate gradient only");
+    std::unique_ptr<iterative_linear_solver> s;
+    if (is_symmetric)
+        s = detail::make_with<conjugate_gradient_hip>(o);
+    else
+        s = detail::make_with<biconjugate_gradient_stabilised_hip>(o);
+    s->precision = precision;
+    return s;
+}
+
+/// Device-resident hex8 mesh with its constitutive model, sparsity pattern and tangent.
+class device_mesh
+{
+public:
+    /// X: 3 per node; conn: 8 per element (hexahedron.cpp:21-28 order); params: 4 per element
+    /// (bulk, shear, yield stress, hardening modulus); model: NEON_MODEL_*; tangent: NEON_TANGENT_*
+    device_mesh(context& c, std::vector<double> const& X, std::vector<std::int32_t> const& conn,
+                std::vector<double> const& params, int model, int tangent = NEON_TANGENT_REFERENCE)
+        : ctx_(c), nodes_(std::int32_t(X.size() / 3)), elements_(std::int32_t(conn.size() / 8))
+    {
+        if (X.size() % 3 || conn.size() % 8 || params.size() != 4 * conn.size() / 8)
+            throw hip_error(NEON_BAD_ARGUMENT, "mesh array sizes disagree");
+        check(c.get(), neon_mesh_create(c.get(), nodes_, X.data(), elements_, conn.data(), model, params.data(),
+                                        tangent));
+    }
+
+    std::int32_t nodes() const { return nodes_; }
+    std::int32_t elements() const { return elements_; }
+    std::int64_t dofs() const { return 3ll * nodes_; }
+    context& ctx() const { return ctx_; }
+
+    void set_parameters(std::vector<double> const& params)
+    {
+        check(ctx_.get(), neon_mesh_set_parameters(ctx_.get(), params.data()));
+    }
+    /// x = X + u and the constitutive update at every quadrature point (mesh.cpp:57-71)
+    void update_internal_variables(std::vector<double> const& u, double const time_step = 0.0)
+    {
+        check(ctx_.get(), neon_mesh_set_displacement(ctx_.get(), u.data()));
+        check(ctx_.get(), neon_mesh_update_internal_variables(ctx_.get(), time_step));
+    }
+    /// commit (converged) or revert the history (mesh.cpp:73-77)
+    void save_internal_variables(bool const have_converged)
+    {
+        check(ctx_.get(), neon_mesh_save_internal_variables(ctx_.get(), have_converged ? 1 : 0));
+    }
+    /// gradient of an objective with respect to the per-element parameters (4 per element) at
+    /// the held state, through the adjoint of the consistent tangent
+    std::vector<double> adjoint_gradient(std::vector<double> const& dJdu, std::vector<std::int32_t> const& fixed,
+                                         double const tolerance = 1e-12, std::int32_t const max_iterations = 20000)
+    {
+        std::vector<double> grad(4 * std::size_t(elements_));
+        std::int32_t its = 0;
+        check(ctx_.get(), neon_adjoint_gradient(ctx_.get(), dJdu.data(), std::int32_t(fixed.size()), fixed.data(),
+                                                tolerance, max_iterations, grad.data(), &its));
+        return grad;
+    }
+
+private:
+    context& ctx_;
+    std::int32_t nodes_, elements_;
+};
+
+/// A prescribed displacement history on a set of dofs (boundary.cpp:50-85).
+struct dirichlet_boundary
+{
+    std::vector<std::int32_t> dofs;
+    std::vector<double> times, values;
+};
+
+/// The "time", "nonlinear_options" and "linear_solver" entries of a neon simulation.
+struct simulation_options
+{
+    double period = 1.0, initial = 1.0, minimum = 1.0, maximum = 1.0;
+    bool adaptive = false;
+    double residual_tolerance = 1e-3, displacement_tolerance = 1e-3;
+    std::int32_t newton_iterations = 10;
+    /// neon's "absolute_tolerance" key: absolute norms (static_matrix.hpp:385-400)
+    bool absolute_tolerance = false;
+    /// extension: relative norms without neon's fall-backs to absolute ones
+    bool always_relative = false;
+    solver_options linear;
+    bool symmetric = true;
+};
+
+/// Newton-Raphson with adaptive load stepping on the device (fem::static_matrix::solve).
+class static_matrix
+{
+public:
+    static_matrix(device_mesh& m, simulation_options const& o, std::vector<dirichlet_boundary> b)
+        : mesh_(m), options_(o), boundaries_(std::move(b))
+    {
+    }
+
+    /// Solves every load step; returns the converged displacement.  A recoverable failure that
+    /// survives the cut-backs throws hip_error (neon's "Too many reductions" domain_error).
+    std::vector<double> solve(std::int32_t const record_capacity = 4096)
+    {
+        solver_options const& l = options_.linear;
+        if (l.type != "iterative" || l.device != "gpu" || l.backend != "hip")
+            throw std::domain_error("the device Newton driver needs the iterative hip gpu solver");
+        neon_newton_options o{};
+        o.period = options_.period;
+        o.initial = options_.initial;
+        o.minimum = options_.minimum;
+        o.maximum = options_.maximum;
+        o.adaptive = options_.adaptive ? 1 : 0;
+        o.residual_tolerance = options_.residual_tolerance;
+        o.displacement_tolerance = options_.displacement_tolerance;
+        o.max_newton_iterations = options_.newton_iterations;
+        o.absolute_norm = options_.absolute_tolerance ? 1 : 0;
+        o.norm_mode = options_.always_relative ? 1 : 0;
+        o.solver_tolerance = l.tolerance.value_or(1.0e-5);
+        o.solver_max_iterations = l.maximum_iterations.value_or(2000);
+        o.solver_criterion = NEON_STOP_ABSOLUTE_PRECONDITIONED;
+        o.solver = options_.symmetric ? NEON_SOLVER_CG : NEON_SOLVER_BICGSTAB;
+        o.precision = l.precision == "mixed" ? NEON_PRECISION_MIXED : NEON_PRECISION_FP64;
+        if (o.precision == NEON_PRECISION_MIXED) o.solver_criterion = NEON_STOP_RELATIVE;
+        std::vector<neon_dirichlet> bcs;
+        for (auto const& b : boundaries_)
+            bcs.push_back(neon_dirichlet{std::int32_t(b.dofs.size()), b.dofs.data(), std::int32_t(b.times.size()),
+                                         b.times.data(), b.values.data()});
+        std::vector<double> u(mesh_.dofs());
+        records_.assign(record_capacity, neon_newton_record{});
+        std::int32_t n = 0;
+        int const code = neon_newton_solve(mesh_.ctx().get(), &o, std::int32_t(bcs.size()), bcs.data(), u.data(),
+                                           records_.data(), record_capacity, &n, &timing_);
+        records_.resize(std::min(n, record_capacity));
+        check(mesh_.ctx().get(), code == NEON_OK || code < 0 ? code : NEON_FATAL);
+        return u;
+    }
+
+    /// one record per Newton iteration (load step, iteration, Krylov count, norms)
+    std::vector<neon_newton_record> const& records() const { return records_; }
+    neon_newton_timing const& timing() const { return timing_; }
+
+private:
+    device_mesh& mesh_;
+    simulation_options options_;
+    std::vector<dirichlet_boundary> boundaries_;
+    std::vector<neon_newton_record> records_;
+    neon_newton_timing timing_{};
+};
+} // namespace neon_hip

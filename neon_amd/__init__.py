@@ -24,6 +24,9 @@ from ._capi import (  # noqa: F401
     PRECISION_MIXED,
     SOLVER_BICGSTAB,
     SOLVER_CG,
+    SPMV_NONTEMPORAL,
+    SPMV_PIPELINE,
+    SPMV_SWEEP,
     STOP_ABSOLUTE_PRECONDITIONED,
     STOP_RELATIVE,
     TANGENT_CONSISTENT,
@@ -263,6 +266,25 @@ class DeviceMesh:
         fixed = np.ascontiguousarray(fixed, dtype=np.int32)
         self.ctx.check(_capi.lib().neon_mesh_enforce_dirichlet(self.ctx.handle, len(fixed), _capi.i32(fixed),
                                                                _capi.f64(b)))
+
+    def spmv_config(self, lanes: int = 32, flags: int = _capi.SPMV_SWEEP, blocks_fp64: int = 1024,
+                    blocks_fp32: int = 2048) -> None:
+        """Shape of the node-block SpMV used by the solvers on this mesh (tuning knob)."""
+        self.ctx.check(_capi.lib().neon_mesh_spmv_config(self.ctx.handle, lanes, flags, blocks_fp64, blocks_fp32))
+
+    def adjoint_gradient(self, dJdu: np.ndarray, fixed: np.ndarray, tol: float = 1e-12,
+                         max_iterations: int = 20000):
+        """dJ/dparams (n_elem x 4) at the held state for an objective with gradient dJdu,
+        through the adjoint solve K^T lambda = dJ/du (consistent tangent, neo-Hooke)."""
+        dJdu = np.ascontiguousarray(dJdu, dtype=np.float64)
+        fixed = np.ascontiguousarray(fixed, dtype=np.int32)
+        grad = np.zeros((self.n_elem, 4))
+        its = C.c_int32(0)
+        code = _capi.lib().neon_adjoint_gradient(self.ctx.handle, _capi.f64(dJdu), len(fixed), _capi.i32(fixed), tol,
+                                                 max_iterations, _capi.f64(grad), C.byref(its))
+        self.adjoint_iterations = its.value
+        self.ctx.check(code)
+        return grad
 
     def solve(self, b: np.ndarray, solver: int = SOLVER_CG, criterion: int = STOP_RELATIVE, tol: float = 1e-10,
               max_iterations: int = 10000):

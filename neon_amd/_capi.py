@@ -40,6 +40,7 @@ KERNEL_RESIDUAL_FP64 = 2
 KERNEL_ASSEMBLY = 3
 SPMV_NONTEMPORAL = 1
 SPMV_SWEEP = 2
+SPMV_PIPELINE = 4
 
 STATUS_NAMES = {
     0: "OK",
@@ -142,16 +143,14 @@ SIGNATURES = {
     "neon_mesh_enforce_dirichlet": (C.c_int, [C.c_void_p, C.c_int32, i32p, f64p]),
     "neon_mesh_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int32, f64p, f64p, i32p, f64p]),
     "neon_mesh_spmv_bench": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p]),
+    "neon_mesh_spmv_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "neon_stream_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f64p, f64p]),
     "neon_newton_solve": (
         C.c_int,
         [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), f64p, P(NewtonRecord), C.c_int32, i32p,
          P(NewtonTiming)],
     ),
-    "neon_adjoint_gradient": (
-        C.c_int,
-        [C.c_void_p, f64p, C.c_int32, i32p, C.c_double, C.c_double, C.c_int32, f64p, i32p],
-    ),
+    "neon_adjoint_gradient": (C.c_int, [C.c_void_p, f64p, C.c_int32, i32p, C.c_double, C.c_int32, f64p, i32p]),
     "neon_mesh_box_size": (
         C.c_int,
         [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, i32p, i32p],

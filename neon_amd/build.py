@@ -67,5 +67,24 @@ def build(jobs: int = 4) -> pathlib.Path:
     return LIB
 
 
+HOST_TEST_SRC = ROOT / "tests" / "cpp" / "host_api_test.cpp"
+HOST_TEST = ROOT / "tests" / "cpp" / "host_api_test"
+
+
+def build_host_test() -> pathlib.Path:
+    """Compile the C++ host-class test program against ``libneon_hip.so`` with g++ (host code
+    only: it uses the C ABI through ``include/neon_hip.hpp``)."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = HOST_TEST.with_suffix(".tmp")
+    subprocess.run(
+        [cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(HOST_TEST_SRC), f"-L{HERE}", "-lneon_hip",
+         "-Wl,-rpath,$ORIGIN/../../neon_amd", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(tmp)],
+        check=True,
+    )
+    os.replace(tmp, HOST_TEST)
+    return HOST_TEST
+
+
 if __name__ == "__main__":
     print(build())
+    print(build_host_test())

@@ -324,6 +324,22 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
     });
 }
 
+int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if ((lanes != 16 && lanes != 32) || (flags & ~7) || blocks_fp64 < 0 || blocks_fp32 < 0)
+            throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
+        if ((flags & NEON_SPMV_PIPELINE) && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "the pipelined SpMV uses 32 lanes");
+        spmv_config& c = ctx->mesh->spmv;
+        c.lanes = lanes;
+        c.flags = flags;
+        c.blocks64 = blocks_fp64;
+        c.blocks32 = blocks_fp32;
+        return NEON_OK;
+    });
+}
+
 int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
                          int32_t reps, double* avg_ms)
 {
@@ -336,6 +352,8 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
         A.nb_lanes = lanes;
         A.nb_nt = (flags & NEON_SPMV_NONTEMPORAL) != 0;
         A.nb_sweep = (flags & NEON_SPMV_SWEEP) != 0;
+        A.nb_pipeline = (flags & NEON_SPMV_PIPELINE) != 0;
+        if (A.nb_pipeline && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "the pipelined SpMV uses 32 lanes");
         A.blocks_override = blocks;
         int64_t const n = m.dofs();
         hipStream_t const s = ctx->stream;
@@ -398,10 +416,49 @@ int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int
     });
 }
 
-int neon_adjoint_gradient(neon_context* ctx, const double*, int32_t, const int32_t*, double, double, int32_t, double*,
-                          int32_t*)
+int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed, const int32_t* fixed,
+                          double tolerance, int32_t max_iterations, double* grad, int32_t* iterations)
 {
-    return guarded(ctx, [&]() -> int { throw status_error(NEON_BAD_ARGUMENT, "adjoint is not available in this build"); });
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        device_mesh& m = *ctx->mesh;
+        if (!dJdu || !grad || n_fixed < 0 || (n_fixed > 0 && !fixed) || max_iterations <= 0 || !(tolerance > 0.0))
+            throw status_error(NEON_BAD_ARGUMENT, "bad adjoint arguments");
+        if (m.model() != NEON_MODEL_NEOHOOKE)
+            throw status_error(NEON_BAD_ARGUMENT, "the adjoint gradient supports the neo-Hooke model");
+        if (m.tangent_mode() != NEON_TANGENT_CONSISTENT)
+            throw status_error(NEON_BAD_ARGUMENT, "the adjoint needs the consistent tangent (exact dR/du)");
+        int64_t const n = m.dofs();
+        for (int32_t i = 0; i < n_fixed; ++i)
+            if (fixed[i] < 0 || fixed[i] >= n) throw status_error(NEON_BAD_ARGUMENT, "fixed dof out of range");
+        hipStream_t const s = ctx->stream;
+        device_array<double> b, lambda, minv, g;
+        device_array<int32_t> fx;
+        b.resize(n);
+        lambda.resize(n);
+        minv.resize(n);
+        g.resize(4ull * m.elements());
+        NEON_CHECK_HIP(hipMemcpyAsync(b.data(), dJdu, n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (n_fixed > 0)
+        {
+            fx.resize(n_fixed);
+            NEON_CHECK_HIP(hipMemcpyAsync(fx.data(), fixed, n_fixed * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        }
+        // K(u) at the held state, with the constrained rows and columns of the forward solve
+        int const st = m.update_internal_variables(m.u());
+        if (st != NEON_OK) return st;
+        m.assemble(m.u(), NEON_PRECISION_FP64);
+        m.enforce_dirichlet(n_fixed, fx.data(), b.data(), NEON_PRECISION_FP64);
+        // K^T lambda = dJ/du (K symmetric): the forward Jacobi-PCG, relative stopping rule
+        solve_result const r = device_solve(ctx, NEON_SOLVER_CG, NEON_STOP_RELATIVE, NEON_PRECISION_FP64, tolerance,
+                                            max_iterations, b.data(), lambda.data(), minv.data());
+        if (iterations) *iterations = r.iterations;
+        if (r.status != NEON_OK) return r.status;
+        m.parameter_gradient(m.u(), lambda.data(), g.data());
+        NEON_CHECK_HIP(hipMemcpyAsync(grad, g.data(), 4ull * m.elements() * sizeof(double), hipMemcpyDeviceToHost, s));
+        NEON_CHECK_HIP(hipStreamSynchronize(s));
+        return NEON_OK;
+    });
 }
 
 } // extern "C"

@@ -485,4 +485,84 @@ __global__ __launch_bounds__(kBlock) void dirichlet_kernel(int32_t n_fixed, cons
         }
     }
 }
+
+// ----------------------------------------------------------------- adjoint
+// dJ/dp_e = -lambda_e^T d f_e / d p_e for the compressible neo-Hooke element force
+// f_e,a = sum_l w_l detJx_l sigma_l L_a (L_a = dN_a Jx^{-1}, current configuration,
+// submesh.cpp:82-108) with sigma = (lambda ln J I + mu (b - I)) / J, lambda = K - 2 mu / 3
+// (compressible_neohooke.cpp:37-40).  sigma is linear in (K, mu):
+//   d sigma / dK  = ln J / J I,   d sigma / dmu = (b - I - 2/3 ln J I) / J,
+// so with G = sum_a lambda_a (x) L_a, lambda_e^T d f_e/dp = sum_l w_l detJx_l (d sigma/dp : G).
+// One thread per quadrature point; the 8 points of an element reduce by lane shuffles and
+// the first writes the 4 gradients of the element (bulk, shear, yield, hardening; the last
+// two are zero for this model).
+__global__ __launch_bounds__(kBlock) void neohooke_parameter_gradient_kernel(
+    int32_t E, const int32_t* __restrict__ conn, const double* __restrict__ X, const double* __restrict__ u,
+    const double* __restrict__ lambda, double* __restrict__ grad)
+{
+    int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    bool const on = qp < 8ll * E;
+    int64_t const e = on ? qp >> 3 : 0;
+    int const l = int(qp & 7);
+    double gK = 0.0, gmu = 0.0;
+    if (on)
+    {
+        double F0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fx[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double le[8][3];
+        for (int a = 0; a < 8; ++a)
+        {
+            int64_t const nd = conn[8 * e + a];
+            for (int i = 0; i < 3; ++i)
+            {
+                double const Xi = X[3 * nd + i];
+                double const xi = Xi + u[3 * nd + i];
+                le[a][i] = lambda[3 * nd + i];
+                for (int j = 0; j < 3; ++j)
+                {
+                    F0[i * 3 + j] += Xi * c_hex8.dN[l][a][j];
+                    Fx[i * 3 + j] += xi * c_hex8.dN[l][a][j];
+                }
+            }
+        }
+        double F0i[9], Fxi[9];
+        inv3(F0, F0i);
+        inv3(Fx, Fxi);
+        double F[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                F[i * 3 + j] = Fx[i * 3 + 0] * F0i[0 * 3 + j] + Fx[i * 3 + 1] * F0i[1 * 3 + j] + Fx[i * 3 + 2] * F0i[2 * 3 + j];
+        double const J = det3(F);
+        double const lnJ = log(J);
+        double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < 8; ++a)
+        {
+            double L[3];
+            for (int j = 0; j < 3; ++j)
+                L[j] = c_hex8.dN[l][a][0] * Fxi[0 * 3 + j] + c_hex8.dN[l][a][1] * Fxi[1 * 3 + j]
+                       + c_hex8.dN[l][a][2] * Fxi[2 * 3 + j];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) G[i * 3 + j] += le[a][i] * L[j];
+        }
+        double const trG = G[0] + G[4] + G[8];
+        double bG = 0.0; // (F F^T) : G
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                bG += (F[i * 3 + 0] * F[j * 3 + 0] + F[i * 3 + 1] * F[j * 3 + 1] + F[i * 3 + 2] * F[j * 3 + 2]) * G[i * 3 + j];
+        double const wdet = c_hex8.w[l] * det3(Fx);
+        gK = -wdet * lnJ / J * trG;
+        gmu = -wdet / J * (bG - trG - 2.0 / 3.0 * lnJ * trG);
+    }
+    for (int off = 4; off > 0; off >>= 1)
+    {
+        gK += __shfl_xor(gK, off, 8);
+        gmu += __shfl_xor(gmu, off, 8);
+    }
+    if (on && l == 0)
+    {
+        grad[4 * e] = gK;
+        grad[4 * e + 1] = gmu;
+        grad[4 * e + 2] = 0.0;
+        grad[4 * e + 3] = 0.0;
+    }
+}
 } // namespace neon_amd

@@ -34,7 +34,12 @@ int linear_operator::spmv_blocks() const
 {
     int64_t const rows = node_block ? nodes : n;
     int64_t const rows_per_pass = node_block ? kBlock / nb_lanes : kBlock / lanes_per_row;
-    if (blocks_override > 0) return xcd_blocks(blocks_override, kMaxPartials);
+    if (blocks_override > 0)
+    {
+        // at most one pass of lane groups over the rows
+        int64_t const needed = ((rows + rows_per_pass - 1) / rows_per_pass + 7) / 8 * 8;
+        return xcd_blocks(std::min<int64_t>(blocks_override, needed), kMaxPartials);
+    }
     // about 4 passes per block, at most kMaxPartials blocks
     return xcd_blocks((rows + 4 * rows_per_pass - 1) / (4 * rows_per_pass), kMaxPartials);
 }
@@ -51,17 +56,27 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const X
         nb_matrix m{A.nodes, A.bptr, A.bcol};
 #define NEON_NB(L, NT, SW) \
     spmv_nb_kernel<L, 1, NT, SW, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate)
-        int const key = (A.nb_lanes == 16 ? 0 : 4) + (A.nb_nt ? 2 : 0) + (A.nb_sweep ? 1 : 0);
-        switch (key)
+        if (A.nb_pipeline)
         {
-        case 0: NEON_NB(16, false, false); break;
-        case 1: NEON_NB(16, false, true); break;
-        case 2: NEON_NB(16, true, false); break;
-        case 3: NEON_NB(16, true, true); break;
-        case 4: NEON_NB(32, false, false); break;
-        case 5: NEON_NB(32, false, true); break;
-        case 6: NEON_NB(32, true, false); break;
-        default: NEON_NB(32, true, true); break;
+            if (A.nb_nt)
+                spmv_nb_pipe_kernel<true, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
+            else
+                spmv_nb_pipe_kernel<false, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
+        }
+        else
+        {
+            int const key = (A.nb_lanes == 16 ? 0 : 4) + (A.nb_nt ? 2 : 0) + (A.nb_sweep ? 1 : 0);
+            switch (key)
+            {
+            case 0: NEON_NB(16, false, false); break;
+            case 1: NEON_NB(16, false, true); break;
+            case 2: NEON_NB(16, true, false); break;
+            case 3: NEON_NB(16, true, true); break;
+            case 4: NEON_NB(32, false, false); break;
+            case 5: NEON_NB(32, false, true); break;
+            case 6: NEON_NB(32, true, false); break;
+            default: NEON_NB(32, true, true); break;
+            }
         }
 #undef NEON_NB
     }
@@ -636,7 +651,11 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
         }
     }
     bptr_h_.assign(size_t(N_) + 1, 0);
-    for (int32_t nd = 0; nd < N_; ++nd) bptr_h_[nd + 1] = bptr_h_[nd] + deg[nd];
+    for (int32_t nd = 0; nd < N_; ++nd)
+    {
+        bptr_h_[nd + 1] = bptr_h_[nd] + deg[nd];
+        max_degree_ = std::max<int32_t>(max_degree_, deg[nd]);
+    }
     int64_t const nnzb = bptr_h_[N_];
     nnz_ = 9 * nnzb;
     if (nnz_ > int64_t(INT32_MAX)) throw status_error(NEON_BAD_ARGUMENT, "nnz exceeds the int32 CSR index range");
@@ -991,6 +1010,12 @@ linear_operator device_mesh::op(int precision) const
         if (!k32_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP32 tangent assembled");
         A.v32 = k32_.data();
     }
+    A.nb_lanes = spmv.lanes;
+    A.nb_nt = (spmv.flags & NEON_SPMV_NONTEMPORAL) != 0;
+    A.nb_sweep = (spmv.flags & NEON_SPMV_SWEEP) != 0;
+    // the pipelined kernel holds one block per lane: rows of at most 32 blocks
+    A.nb_pipeline = (spmv.flags & NEON_SPMV_PIPELINE) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
+    A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
     return A;
 }
 
@@ -1025,8 +1050,12 @@ void device_mesh::jacobi(double* minv, int eigen_rule, int precision)
     NEON_CHECK_HIP(hipGetLastError());
 }
 
-void device_mesh::parameter_gradient(const double*, const double*, double, double*)
+void device_mesh::parameter_gradient(const double* u_dev, const double* lambda_dev, double* grad_dev)
 {
-    throw status_error(NEON_BAD_ARGUMENT, "adjoint is not available in this build");
+    if (model_ != NEON_MODEL_NEOHOOKE)
+        throw status_error(NEON_BAD_ARGUMENT, "the adjoint parameter gradient supports the neo-Hooke model");
+    neohooke_parameter_gradient_kernel<<<grid_for(8ll * E_), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_dev,
+                                                                             lambda_dev, grad_dev);
+    NEON_CHECK_HIP(hipGetLastError());
 }
 } // namespace neon_amd

@@ -110,6 +110,7 @@ struct linear_operator
     int nb_lanes = 32;
     bool nb_nt = false;
     bool nb_sweep = false;
+    bool nb_pipeline = false;
     int blocks_override = 0;
     // values
     const double* v64 = nullptr;
@@ -175,6 +176,17 @@ private:
     device_array<double> x64_, r64_;
 };
 
+// Shape of the node-block SpMV used by the Krylov solvers on a mesh tangent (the defaults
+// are the fastest measured at 6.4M dofs on MI355X): lanes per block row, NEON_SPMV_* flags,
+// workgroups for FP64 and FP32 values.
+struct spmv_config
+{
+    int lanes = 32;
+    int flags = NEON_SPMV_SWEEP;
+    int blocks64 = 1024;
+    int blocks32 = 2048;
+};
+
 // Device-resident mesh: geometry, constitutive state, pattern, tangent and f_int.
 class device_mesh
 {
@@ -203,8 +215,9 @@ public:
     linear_operator op(int precision) const;
     // Jacobi inverse from the node-block diagonal
     void jacobi(double* minv, int eigen_rule, int precision);
-    // dJ/dp per element for p = shear modulus with bulk = ratio * shear (adjoint)
-    void parameter_gradient(const double* u_dev, const double* lambda_dev, double ratio, double* grad_dev);
+    // adjoint: grad (4 per element) = -lambda^T d f_int / d params at displacement u
+    void parameter_gradient(const double* u_dev, const double* lambda_dev, double* grad_dev);
+    int32_t tangent_mode() const { return tangent_mode_; }
 
     double* u() { return u_.data(); }
     double* f_int() { return fint_.data(); }
@@ -214,12 +227,15 @@ public:
 
     // last assemble time split (ms) measured with events
     float last_assemble_ms = 0.0f;
+    spmv_config spmv;
+    int32_t max_degree() const { return max_degree_; }
     kernel_profiler* prof = nullptr;
 
 private:
     hipStream_t s_;
     int32_t N_, E_, model_, tangent_mode_;
     int64_t nnz_ = 0;
+    int32_t max_degree_ = 0;
     std::vector<int32_t> bptr_h_, bcol_h_;
     std::vector<int32_t> colour_offsets_;
     device_array<double> X_, u_, params_, fint_;

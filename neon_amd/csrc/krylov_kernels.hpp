@@ -198,6 +198,122 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
     }
 }
 
+// Software-pipelined node-block SpMV (32 lanes per block row, at most 32 blocks per row,
+// XCD sweep order as above).  Iteration t of a lane group gathers x for row t, requests the
+// row pointers of row t+2 and the column indices and 9 values per lane of row t+1, and only
+// then combines row t, so each iteration exposes one memory latency with the next row's
+// loads in flight.  Every load is unconditional from a clamped, valid address and masked
+// afterwards: a load under a branch would force a wait for all outstanding loads at the join.
+template <bool NT, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(kBlock) void spmv_nb_pipe_kernel(nb_matrix A, const VT* __restrict__ val,
+                                                              const XT* __restrict__ x, YT* __restrict__ y,
+                                                              const double* __restrict__ bvec,
+                                                              double* __restrict__ part,
+                                                              const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate && !st->refine))) return;
+    constexpr int LPR = 32;
+    constexpr int G = kBlock / LPR;
+    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    int64_t start, end, stride;
+    if ((gridDim.x & 7) == 0)
+    {
+        int64_t const per = (int64_t(A.nodes) + 7) / 8;
+        int64_t const beg = int64_t(blockIdx.x & 7) * per;
+        end = beg + per < A.nodes ? beg + per : A.nodes;
+        stride = int64_t(gridDim.x >> 3) * G;
+        start = beg + int64_t(blockIdx.x >> 3) * G + g;
+    }
+    else
+    {
+        int64_t beg;
+        xcd_chunk(A.nodes, beg, end);
+        stride = G;
+        start = beg + g;
+    }
+    int64_t const count = start < end ? (end - start + stride - 1) / stride : 0;
+    double dot = 0.0;
+    if (count > 0)
+    {
+        // row pointers of row t (clamped to the last row; degree 0 past the end)
+        auto meta = [&](int64_t t, int32_t& b0, int32_t& d) {
+            int64_t const n = start + (t < count ? t : count - 1) * stride;
+            int32_t const lo = A.bptr[n], hi = A.bptr[n + 1];
+            b0 = lo;
+            d = t < count ? hi - lo : 0;
+        };
+        // lane l's block of a row: column index and 9 values; lanes past the row re-read its
+        // first block (cached) and are masked through their x entries at use
+        auto fetch = [&](int32_t b0, int32_t d, VT* v, int32_t& col, bool& on) {
+            on = l < d;
+            int const q = on ? l : 0;
+            int64_t const rs = 3ll * (d > 0 ? d : 1);
+            col = A.bcol[b0 + q];
+            __builtin_amdgcn_sched_barrier(0);
+            const VT* p = val + 9ll * b0 + 3 * q;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) v[3 * i + j] = ld<NT>(p + i * rs + j);
+        };
+        int32_t b0n, dn; // row pointers of the next row to fetch
+        // one pipeline stage: combine row t (values vc, index cc) while row t+1 is fetched into vn
+        auto stage = [&](int64_t t, const VT* vc, int32_t cc, bool oc, VT* vn, int32_t& cn, bool& on) {
+            AT xs[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) xs[j] = AT(x[3ll * cc + j]);
+            int32_t b0nn, dnn;
+            meta(t + 2, b0nn, dnn);
+            // issue order x, pointers, indices, values: the next stage then waits only for the
+            // indices while the values are still in flight
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(b0n, dn, vn, cn, on);
+            int64_t const row0 = 3 * (start + t * stride);
+            double xr = 0.0, br = 0.0;
+            if constexpr (DOT) xr = double(x[row0 + (l < 3 ? l : 2)]);
+            if constexpr (MODE == 1) br = bvec[row0 + (l < 3 ? l : 2)];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) xs[j] = oc ? xs[j] : AT(0);
+            AT s0 = AT(vc[0]) * xs[0] + AT(vc[1]) * xs[1] + AT(vc[2]) * xs[2];
+            AT s1 = AT(vc[3]) * xs[0] + AT(vc[4]) * xs[1] + AT(vc[5]) * xs[2];
+            AT s2 = AT(vc[6]) * xs[0] + AT(vc[7]) * xs[1] + AT(vc[8]) * xs[2];
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1)
+            {
+                s0 += __shfl_xor(s0, off, LPR);
+                s1 += __shfl_xor(s1, off, LPR);
+                s2 += __shfl_xor(s2, off, LPR);
+            }
+            AT const yi = l == 0 ? s0 : (l == 1 ? s1 : s2);
+            YT const yo = MODE == 1 ? YT(br - double(yi)) : YT(yi);
+            if (l < 3) y[row0 + l] = yo;
+            if constexpr (DOT) dot += l < 3 ? xr * double(yo) : 0.0;
+            b0n = b0nn;
+            dn = dnn;
+        };
+        int32_t b0, d;
+        meta(0, b0, d);
+        meta(1, b0n, dn);
+        // two register sets, alternated by a 2x unrolled loop, so no loaded value is copied
+        VT va[9], vb[9];
+        int32_t ca, cb;
+        bool oa, ob;
+        fetch(b0, d, va, ca, oa);
+        int64_t t = 0;
+        for (; t + 1 < count; t += 2)
+        {
+            stage(t, va, ca, oa, vb, cb, ob);
+            stage(t + 1, vb, cb, ob, va, ca, oa);
+        }
+        if (t < count) stage(t, va, ca, oa, vb, cb, ob);
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<kBlock>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
 // streaming read of n doubles in 16-byte loads (calibration of the achievable read bandwidth)
 __global__ __launch_bounds__(kBlock) void stream_sum_kernel(int64_t n, const double* __restrict__ v,
                                                             double* __restrict__ out)

@@ -450,7 +450,8 @@ namespace
 // submesh::tangent_stiffness = material_tangent_stiffness + geometric_tangent_stiffness
 // (submesh.cpp:66-80, 110-160) and submesh::internal_force (submesh.cpp:82-108), using the
 // dense B (6x24) of symmetric_gradient<3> (src/numeric/gradient_operator.hpp:38-57).
-void element_matrices(const orc_model& m, int64_t e, double* ke, double* fe)
+// sigma_e: optional 8 x 9 Cauchy stresses replacing the element's stored ones (f_e only)
+void element_matrices(const orc_model& m, int64_t e, double* ke, double* fe, const double* sigma_e = nullptr)
 {
     hex8 const& h = shape();
     double xe[24];
@@ -481,7 +482,7 @@ void element_matrices(const orc_model& m, int64_t e, double* ke, double* fe)
                 Bt[a][k] = s;
             }
         double const wl = h.w[l];
-        double const* sg = &m.sigma[9 * (8 * e + l)];
+        double const* sg = sigma_e ? sigma_e + 9 * l : &m.sigma[9 * (8 * e + l)];
         for (int a = 0; a < 8; ++a)
             for (int i = 0; i < 3; ++i)
             {
@@ -558,6 +559,42 @@ int orc_element_matrices(const orc_model* m, int32_t element, double* ke, double
 {
     if (element < 0 || element >= m->E) return ORC_BAD_ARGUMENT;
     element_matrices(*m, element, ke, fe);
+    return ORC_OK;
+}
+
+// Adjoint parameter gradient (no counterpart in the reference): grad_e,k = -lambda_e .
+// d f_e / d p_k for the neo-Hooke element force at the current state.  The Cauchy stress
+// (compressible_neohooke.cpp:37-40) is linear in (bulk, shear), so d f_e / d bulk and
+// d f_e / d shear are exactly the element forces of the stresses evaluated with unit
+// parameters (1, 0) and (0, 1) at the stored F; yield and hardening get zero.
+int orc_parameter_gradient(const orc_model* m, const double* lambda, double* grad)
+{
+    if (!m || !lambda || !grad || m->model != ORC_MODEL_NEOHOOKE) return ORC_BAD_ARGUMENT;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < m->E; ++e)
+    {
+        double sk[72], sm[72], D[36], fk[24], fm[24];
+        for (int l = 0; l < 8; ++l)
+        {
+            size_t const q = 8 * size_t(e) + l;
+            neohooke(&m->F[9 * q], m->detF[q], 1.0, 0.0, sk + 9 * l, D);
+            neohooke(&m->F[9 * q], m->detF[q], 0.0, 1.0, sm + 9 * l, D);
+        }
+        element_matrices(*m, e, nullptr, fk, sk);
+        element_matrices(*m, e, nullptr, fm, sm);
+        double gk = 0.0, gm = 0.0;
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i)
+            {
+                double const la = lambda[3 * size_t(m->conn[8 * e + a]) + i];
+                gk += la * fk[3 * a + i];
+                gm += la * fm[3 * a + i];
+            }
+        grad[4 * e] = -gk;
+        grad[4 * e + 1] = -gm;
+        grad[4 * e + 2] = 0.0;
+        grad[4 * e + 3] = 0.0;
+    }
     return ORC_OK;
 }
 

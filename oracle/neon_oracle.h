@@ -91,6 +91,9 @@ int orc_get_qp_state(const orc_model* m, double* sigma /*9/qp or NULL*/, double*
 
 /* Element matrices at the current configuration (tangent_stiffness / internal_force). */
 int orc_element_matrices(const orc_model* m, int32_t element, double* ke /*576*/, double* fe /*24*/);
+/* Adjoint: grad (4 per element) = -lambda_e . d f_e / d params_e at the current state
+ * (neo-Hooke only). */
+int orc_parameter_gradient(const orc_model* m, const double* lambda, double* grad);
 
 /* compute_sparsity_pattern: pass row_ptr=NULL to get nnz only. */
 int64_t orc_sparsity_nnz(const orc_model* m);

@@ -58,6 +58,7 @@ _SIG = {
     "orc_save_internal_variables": [C.c_void_p, C.c_int32],
     "orc_get_qp_state": [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p],
     "orc_element_matrices": [C.c_void_p, C.c_int32, f64p, f64p],
+    "orc_parameter_gradient": [C.c_void_p, f64p, f64p],
     "orc_sparsity_fill": [C.c_void_p, i32p, i32p],
     "orc_assemble": [C.c_void_p, i32p, i32p, f64p, f64p, C.c_int32],
     "orc_enforce_dirichlet": [C.c_int32, i32p, i32p, f64p, f64p, C.c_int32, i32p],
@@ -172,6 +173,15 @@ class Model:
         ke, fe = np.zeros((24, 24)), np.zeros(24)
         lib().orc_element_matrices(self.h, e, _f(ke), _f(fe))
         return ke, fe
+
+    def parameter_gradient(self, lam):
+        """-lambda_e . d f_e / d params_e (n_elem x 4) at the current state (neo-Hooke)."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        g = np.zeros((self.n_elem, 4))
+        code = lib().orc_parameter_gradient(self.h, _f(lam), _f(g))
+        if code != OK:
+            raise RuntimeError(f"orc_parameter_gradient failed: {code}")
+        return g
 
     def pattern(self):
         nnz = lib().orc_sparsity_nnz(self.h)

@@ -1,0 +1,192 @@
+// C++ host classes of include/neon_hip.hpp on the GPU, against results that the mathematics
+// fixes:
+//   * the tridiagonal system of the reference's linear-solver test (test/linear_solvers.cpp:15-49),
+//     x = (1.375, 1.75, 1.625), through make_linear_solver's conjugate gradient and BiCGSTAB;
+//   * factory errors for unknown options;
+//   * a uniaxial-stress patch test: a jittered neo-Hooke box on rollers stretched by s along z
+//     deforms homogeneously, u = (F - I) X with F = diag(l, l, 1 + s), where the lateral stretch
+//     l solves lambda ln J + mu (l^2 - 1) = 0, J = l^2 (1 + s) (sigma_xx = 0 for
+//     compressible_neohooke.cpp:37-40); hex8 interpolates linear fields exactly, so the FE
+//     solution equals it to solver accuracy.
+// Exit status 0: all checks passed; 77: no HIP device (nothing run); 1: a check failed.
+#include "neon_hip.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+namespace
+{
+int failures = 0;
+
+void expect(bool ok, char const* what)
+{
+    std::printf("%s: %s\n", ok ? "ok  " : "FAIL", what);
+    if (!ok) ++failures;
+}
+
+void tridiagonal(neon_hip::context& ctx)
+{
+    std::vector<std::int32_t> rp{0, 2, 5, 7}, col{0, 1, 0, 1, 2, 1, 2};
+    std::vector<double> val{2.0, -1.0, -1.0, 2.0, -1.0, -1.0, 2.0}, b{1.0, 0.5, 1.5}, x;
+    neon_hip::csr_view A{3, 7, rp.data(), col.data(), val.data()};
+    neon_hip::solver_options o;
+    o.type = "iterative";
+    o.device = "gpu";
+    o.backend = "hip";
+    o.tolerance = 1e-12;
+    for (bool symmetric : {true, false})
+    {
+        auto s = neon_hip::make_linear_solver(o, symmetric);
+        auto* it = dynamic_cast<neon_hip::iterative_linear_solver*>(s.get());
+        it->ctx = &ctx;
+        s->solve(A, x, b);
+        bool const close = std::abs(x[0] - 1.375) < 1e-10 && std::abs(x[1] - 1.75) < 1e-10
+                           && std::abs(x[2] - 1.625) < 1e-10;
+        expect(close, symmetric ? "tridiagonal golden system, conjugate gradient" : "tridiagonal golden system, BiCGSTAB");
+        expect(symmetric == (dynamic_cast<neon_hip::conjugate_gradient_hip*>(s.get()) != nullptr),
+               "factory picks CG for symmetric and BiCGSTAB otherwise");
+    }
+    // a solve that cannot converge in 1 iteration is recoverable
+    neon_hip::conjugate_gradient_hip one(1e-14, 1);
+    one.ctx = &ctx;
+    one.criterion = NEON_STOP_RELATIVE;
+    bool recoverable = false;
+    try
+    {
+        one.solve(A, x, b);
+    }
+    catch (neon_hip::computational_error const& e)
+    {
+        recoverable = e.code == NEON_NONCONVERGED;
+    }
+    expect(recoverable, "iteration limit throws computational_error");
+}
+
+void factory_errors()
+{
+    neon_hip::solver_options o;
+    bool threw = false;
+    try
+    {
+        neon_hip::make_linear_solver(o, true);
+    }
+    catch (std::domain_error const&)
+    {
+        threw = true;
+    }
+    expect(threw, "missing solver type throws domain_error");
+    o.type = "iterative";
+    o.device = "gpu";
+    o.backend = "cuda";
+    threw = false;
+    try
+    {
+        neon_hip::make_linear_solver(o, true);
+    }
+    catch (std::domain_error const&)
+    {
+        threw = true;
+    }
+    expect(threw, "unknown backend throws domain_error");
+}
+
+double lateral_stretch(double bulk, double shear, double stretch)
+{
+    double const lambda = bulk - 2.0 * shear / 3.0;
+    double l = 1.0;
+    for (int k = 0; k < 50; ++k)
+    {
+        double const J = l * l * stretch;
+        double const r = lambda * std::log(J) + shear * (l * l - 1.0);
+        double const dr = lambda * 2.0 / l + 2.0 * shear * l;
+        l -= r / dr;
+    }
+    return l;
+}
+
+void patch_test(neon_hip::context& ctx)
+{
+    int const n = 6;
+    double const s = 0.05, bulk = 2.0, shear = 0.4;
+    std::int32_t nn = 0, ne = 0;
+    neon_mesh_box_size(n, n, n, 1.0, 1.0, 1.0, 0.0, &nn, &ne);
+    std::vector<double> X(3 * std::size_t(nn));
+    std::vector<std::int32_t> conn(8 * std::size_t(ne)), ijk(3 * std::size_t(nn));
+    neon_mesh_box(n, n, n, 1.0, 1.0, 1.0, 0.1, 0x6E656F6E01ull, 0.0, X.data(), conn.data(), ijk.data());
+    std::vector<double> params(4 * std::size_t(ne));
+    for (std::int32_t e = 0; e < ne; ++e)
+    {
+        params[4 * e] = bulk;
+        params[4 * e + 1] = shear;
+    }
+    neon_hip::device_mesh mesh(ctx, X, conn, params, NEON_MODEL_NEOHOOKE);
+    std::vector<neon_hip::dirichlet_boundary> bcs(4);
+    for (std::int32_t a = 0; a < nn; ++a)
+    {
+        if (ijk[3 * a] == 0) bcs[0].dofs.push_back(3 * a);
+        if (ijk[3 * a + 1] == 0) bcs[1].dofs.push_back(3 * a + 1);
+        if (ijk[3 * a + 2] == 0) bcs[2].dofs.push_back(3 * a + 2);
+        if (ijk[3 * a + 2] == n) bcs[3].dofs.push_back(3 * a + 2);
+    }
+    for (int k = 0; k < 4; ++k)
+    {
+        bcs[k].times = {0.0, 1.0};
+        bcs[k].values = {0.0, k == 3 ? s : 0.0};
+    }
+    neon_hip::simulation_options o;
+    o.initial = o.minimum = o.maximum = 0.5;
+    o.residual_tolerance = 1e-10;
+    o.displacement_tolerance = 1e-10;
+    o.always_relative = true;
+    o.linear.type = "iterative";
+    o.linear.device = "gpu";
+    o.linear.backend = "hip";
+    o.linear.tolerance = 1e-13;
+    o.linear.maximum_iterations = 5000;
+    neon_hip::static_matrix sm(mesh, o, bcs);
+    std::vector<double> const u = sm.solve();
+    double const l = lateral_stretch(bulk, shear, 1.0 + s);
+    double err = 0.0, umax = 0.0;
+    for (std::int32_t a = 0; a < nn; ++a)
+    {
+        double const ue[3] = {(l - 1.0) * X[3 * a], (l - 1.0) * X[3 * a + 1], s * X[3 * a + 2]};
+        for (int i = 0; i < 3; ++i)
+        {
+            err = std::max(err, std::abs(u[3 * a + i] - ue[i]));
+            umax = std::max(umax, std::abs(ue[i]));
+        }
+    }
+    std::printf("      patch test: lateral stretch %.12f, max |u - (F - I) X| / max|u| = %.3e, %zu Newton records\n", l,
+                err / umax, sm.records().size());
+    expect(err <= 1e-9 * umax, "uniaxial neo-Hooke patch test matches the closed form");
+    expect(sm.records().size() >= 2, "two load steps recorded");
+}
+} // namespace
+
+int main()
+{
+    factory_errors();
+    int devices = 0;
+    neon_device_count(&devices);
+    if (devices == 0)
+    {
+        bool loud = false;
+        try
+        {
+            neon_hip::context c(0);
+        }
+        catch (neon_hip::hip_error const& e)
+        {
+            loud = e.code == NEON_NO_DEVICE;
+        }
+        expect(loud, "context creation without a device throws hip_error(NO_DEVICE)");
+        std::printf("no HIP device: device checks not run\n");
+        return failures ? 1 : 77;
+    }
+    neon_hip::context ctx(0);
+    tridiagonal(ctx);
+    patch_test(ctx);
+    std::printf("%d failure(s)\n", failures);
+    return failures ? 1 : 0;
+}

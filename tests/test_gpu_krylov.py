@@ -136,3 +136,40 @@ def test_mixed_newton_block(ctx):
     err = np.abs(u - u0).max() / np.abs(u0).max()
     assert err < 1e-7, err
     assert sm.records[-1]["force_norm"] <= 1e-8
+
+
+SPMV_SHAPES = [
+    (32, 0), (16, 0), (32, neon_amd.SPMV_SWEEP), (32, neon_amd.SPMV_SWEEP | neon_amd.SPMV_NONTEMPORAL),
+    (32, neon_amd.SPMV_PIPELINE), (32, neon_amd.SPMV_PIPELINE | neon_amd.SPMV_NONTEMPORAL),
+]
+
+
+@pytest.mark.parametrize("precision", [neon_amd.PRECISION_FP64, neon_amd.PRECISION_MIXED])
+def test_spmv_shapes_agree(ctx, precision):
+    # every SpMV shape (lanes, XCD sweep, nontemporal, pipelined) solves the device tangent to
+    # the same answer as the plain shape and with the same Krylov count up to rounding;
+    # each solve's true residual meets the tolerance
+    p = problems.block_rollers(12)
+    mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    u = _smooth_state(p, 0.05, 7)
+    mg.set_displacement(u)
+    mg.update_internal_variables()
+    mg.assemble(precision)
+    _, f = mg.download(stiffness=False)
+    b = -f.copy()
+    mg.enforce_dirichlet(p.fixed_dofs(), b)
+    rp, ci = mg.pattern()
+    val, _ = mg.download(internal_force=False)
+    A = sp.csr_matrix((val, ci, rp))
+    results = []
+    for lanes, flags in SPMV_SHAPES:
+        for blocks in (0, 64):
+            mg.spmv_config(lanes, flags, blocks, blocks)
+            x, its, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
+            assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b), (lanes, flags, blocks)
+            results.append((x, its))
+    x0, its0 = results[0]
+    for x, its in results[1:]:
+        assert abs(its - its0) <= 2
+        assert np.linalg.norm(x - x0) <= 1e-8 * np.linalg.norm(x0)
+    mg.spmv_config()

@@ -524,8 +524,11 @@ def test_j2_consistent_newton_converges_quadratically():
     m = orc.Model(p.X, p.conn, p.params, model=p.model, tangent=orc.TANGENT_CONSISTENT)
     code, u, recs = m.newton(orc.newton_options(p.simulation), p.boundaries)
     assert code == orc.OK
-    # after the elastic predictor the residual falls at least quadratically
+    # after the elastic predictor the residual falls at least quadratically until it nears
+    # the floor set by the Krylov tolerance (1e-10 relative), where roundoff of the inner
+    # solves (which moves with the thread count) takes over
     step2 = [r["force_norm"] for r in recs if r["step"] == 2]
     assert len(step2) <= 6
     for a, b in zip(step2[1:-1], step2[2:]):
-        assert b < 30 * a * a
+        if a > 1e-5:
+            assert b < 30 * a * a
