@@ -589,8 +589,12 @@ void convert_to_float(hipStream_t s, int64_t n, const double* x, float* y)
 // ==================================================================== mesh
 void upload_hex8_tables()
 {
-    static bool done = false;
-    if (done) return;
+    // __constant__ memory is per device: upload once for each device used
+    static uint64_t done = 0;
+    int dev = 0;
+    NEON_CHECK_HIP(hipGetDevice(&dev));
+    uint64_t const bit = dev < 64 ? uint64_t(1) << dev : 0;
+    if (done & bit) return;
     hex8_tables t{};
     static const double node[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                                       {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
@@ -608,7 +612,7 @@ void upload_hex8_tables()
         }
     }
     NEON_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_hex8), &t, sizeof(t)));
-    done = true;
+    done |= bit;
 }
 
 device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,

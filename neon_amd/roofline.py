@@ -1,0 +1,80 @@
+"""Roofline probe of the node-block SpMV on the assembled tangent of the neo-Hooke block.
+
+    python -m neon_amd.roofline --n 128 --precision mixed --reps 20
+
+builds the n^3 block (the bench's mesh), assembles the tangent at u = 0 in the requested
+precision, runs `reps` SpMVs with the solver's default shape (`neon_mesh_spmv_bench`) and a
+streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the calibration kernel for the
+FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the mean SpMV time, its
+algorithmic bytes (values, block columns, block row pointers, x read once, y written once)
+and the achieved fraction of the 8 TB/s HBM peak.  Run it under `rocprofv3 --kernel-trace
+--stats` or `--pmc` to read the same kernels' durations and traffic counters.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+
+import numpy as np
+
+import neon_amd
+from neon_amd import _capi, problems
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(nnz: int, nodes: int, value_bytes: int) -> int:
+    n = 3 * nodes
+    return value_bytes * nnz + 4 * (nnz // 9) + 4 * (nodes + 1) + 2 * value_bytes * n
+
+
+def main(argv=None) -> dict:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--copy-mb", type=int, default=1024)
+    args = ap.parse_args(argv)
+
+    ctx = neon_amd.Context(0)
+    p = problems.block_rollers(args.n)
+    mesh = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    mesh.set_displacement(np.zeros(p.n_dofs))
+    mesh.update_internal_variables()
+    prec = _capi.PRECISION_FP64 if args.precision == "fp64" else _capi.PRECISION_MIXED
+    mesh.assemble(prec)
+    ms = C.c_double()
+    lib = _capi.lib()
+    # the solvers' default shape (spmv_config defaults): 32 lanes, XCD sweep order
+    cfg = dict(lanes=32, flags=_capi.SPMV_SWEEP, blocks=1024 if prec == _capi.PRECISION_FP64 else 2048)
+    ctx.check(lib.neon_mesh_spmv_bench(ctx.handle, prec, cfg["lanes"], cfg["flags"], cfg["blocks"], args.reps,
+                                       C.byref(ms)))
+    nnz = mesh.nnz()
+    vb = 8 if prec == _capi.PRECISION_FP64 else 4
+    b = spmv_bytes(nnz, p.n_nodes, vb)
+    gbs = b / (ms.value * 1e-3) / 1e9
+    read_ms, copy_ms = C.c_double(), C.c_double()
+    copy_bytes = args.copy_mb << 20
+    ctx.check(lib.neon_stream_probe(ctx.handle, copy_bytes, args.reps, C.byref(read_ms), C.byref(copy_ms)))
+    out = {
+        "workload": f"neo-Hooke block {args.n}^3, tangent at u = 0",
+        "precision": args.precision,
+        "nodes": p.n_nodes,
+        "dofs": p.n_dofs,
+        "nnz": nnz,
+        "shape": cfg,
+        "spmv_avg_ms": ms.value,
+        "spmv_algorithmic_bytes": b,
+        "spmv_gbs": gbs,
+        "spmv_hbm_frac": gbs / HBM_PEAK_GBS,
+        "copy_bytes_each_way": copy_bytes,
+        "stream_read_gbs": copy_bytes / (read_ms.value * 1e-3) / 1e9,
+        "stream_copy_gbs": 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9,
+    }
+    print(json.dumps(out), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -249,6 +249,51 @@ def test_negative_jacobian_is_recoverable_error():
 
 
 # ------------------------------------------------------------------ Dirichlet
+def _expand(nodes):
+    """dof list of a node list, 3 dofs per node (the expansion of test/dof_numbering.cpp)."""
+    return (3 * np.asarray(nodes)[:, None] + np.arange(3)[None, :]).reshape(-1)
+
+
+def test_dof_expansion_golden():
+    # test/dof_numbering.cpp:32-41: nodes (0,1,2,3) and (4,2,1,5)
+    assert list(_expand([0, 1, 2, 3])) == list(range(12))
+    assert list(_expand([4, 2, 1, 5])) == [12, 13, 14, 6, 7, 8, 3, 4, 5, 15, 16, 17]
+
+
+def test_sparsity_pattern_matches_brute_force():
+    m, X, conn = _small_model(n=3)
+    rp, ci = m.pattern()
+    n = m.n_dofs
+    dense = np.zeros((n, n), bool)
+    for e in conn:
+        d = _expand(e)
+        dense[np.ix_(d, d)] = True
+    ref = sp.csr_matrix(dense)
+    ref.sort_indices()
+    np.testing.assert_array_equal(rp, ref.indptr)
+    np.testing.assert_array_equal(ci, ref.indices)
+
+
+@pytest.mark.parametrize("model", [orc.MODEL_NEOHOOKE, orc.MODEL_J2])
+def test_assembly_is_the_dense_sum_of_element_matrices(model):
+    m, X, conn = _small_model(n=3, model=model)
+    u = 0.01 * np.random.default_rng(11).normal(size=m.n_dofs)
+    assert m.update(u) == orc.OK
+    rp, ci = m.pattern()
+    val, f = m.assemble(rp, ci)
+    n = m.n_dofs
+    K = np.zeros((n, n))
+    F = np.zeros(n)
+    for e, nodes in enumerate(conn):
+        ke, fe = m.element(e)
+        d = _expand(nodes)
+        K[np.ix_(d, d)] += ke
+        F[d] += fe
+    Ka = sp.csr_matrix((val, ci, rp), shape=(n, n)).toarray()
+    np.testing.assert_allclose(Ka, K, rtol=0, atol=1e-13 * np.abs(K).max())
+    np.testing.assert_allclose(f, F, rtol=0, atol=1e-13 * np.abs(F).max())
+
+
 def test_dirichlet_zeroes_rows_and_columns_keeps_diagonal():
     m, _, _ = _small_model()
     m.update(np.zeros(m.n_dofs))
