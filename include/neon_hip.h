@@ -143,6 +143,7 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
 #define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads */
 #define NEON_SPMV_SWEEP 2       /* workgroups of one XCD sweep its node range together */
 #define NEON_SPMV_PIPELINE 4    /* software-pipelined rows (32 lanes, sweep order) */
+#define NEON_SPMV_LDS 8         /* values staged through LDS in 16-byte loads (32 lanes, sweep order) */
 int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
                          int32_t reps, double* avg_ms);
 /* Shape of the SpMV that the solvers on this mesh's tangent use from now on (lanes, flags as
@@ -177,6 +178,8 @@ typedef struct {
     int32_t precision;        /* NEON_PRECISION_* */
     int32_t max_load_steps;   /* stop after this many converged increments (0: all) */
     int32_t max_total_iterations; /* stop after this many Newton iterations in all (0: no limit) */
+    int32_t record_trajectory;    /* keep u and the committed history of every converged step on the
+                                     device, for neon_adjoint_gradient on J2 models (0: off) */
 } neon_newton_options;
 
 typedef struct {
@@ -201,14 +204,21 @@ int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int
                       int32_t record_capacity, int32_t* n_records, neon_newton_timing* timing);
 
 /* ---- adjoint (no counterpart in the reference) ----
- * Gradient of an objective J(u) with respect to the per-element parameters at the
- * displacement held by the context (the converged state of the last neon_newton_solve, or
- * what neon_mesh_set_displacement set): K(u) is assembled with the consistent tangent, the
- * constrained dofs `fixed` get the forward solve's Dirichlet rows and columns, K^T lambda =
- * dJ/du (dJdu: 3N, host; K is symmetric) is solved with the Jacobi-PCG to the relative
- * tolerance, and grad (4 per element, in the order of params) receives
- * dJ/dp_e = -lambda^T d f_int / d p_e.  Needs the neo-Hooke model with
- * NEON_TANGENT_CONSISTENT (the reference tangent is not dR/du).  iterations: PCG count. */
+ * Gradient of an objective J(u) of the final displacement with respect to the per-element
+ * parameters (grad: 4 per element, in the order of params), through the adjoint of the
+ * consistent tangent (needs NEON_TANGENT_CONSISTENT; the reference tangent is not dR/du).
+ * dJdu (3N, host) is dJ/du at the displacement held by the context; the constrained dofs
+ * `fixed` get the forward solve's Dirichlet rows and columns; every adjoint solve is the
+ * Jacobi-PCG to the relative `tolerance` (K symmetric).  iterations: PCG iterations in all.
+ *  - neo-Hooke (path independent): one solve K lambda = dJ/du at the held state, then
+ *    dJ/dp_e = -lambda^T d f_e / d p_e.
+ *  - J2 (path dependent): the load history of the last neon_newton_solve, which must have
+ *    run with record_trajectory (u_n and the committed history h_{n-1} of every converged
+ *    step are checkpointed on the device; the radial returns are recomputed from them).
+ *    Backwards over the steps: K_n lambda_n = dJ/du [last step] + B^T (dh_n/d eps)^T mu_n,
+ *    mu_{n-1} = (dh_n/dh_{n-1})^T mu_n - (d sigma_n/dh_{n-1})^T (w detJ B lambda_n), and
+ *    dJ/dp += (dh_n/dp)^T mu_n - (d sigma_n/dp)^T (w detJ B lambda_n) for p = bulk, shear,
+ *    yield stress, hardening.  The context's final state is restored afterwards. */
 int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed, const int32_t* fixed,
                           double tolerance, int32_t max_iterations, double* grad, int32_t* iterations);
 

@@ -295,6 +295,8 @@ struct simulation_options
     bool absolute_tolerance = false;
     /// extension: relative norms without neon's fall-backs to absolute ones
     bool always_relative = false;
+    /// extension: keep the load history on the device for the J2 adjoint
+    bool record_trajectory = false;
     solver_options linear;
     bool symmetric = true;
 };
@@ -332,6 +334,7 @@ public:
         o.solver = options_.symmetric ? NEON_SOLVER_CG : NEON_SOLVER_BICGSTAB;
         o.precision = l.precision == "mixed" ? NEON_PRECISION_MIXED : NEON_PRECISION_FP64;
         if (o.precision == NEON_PRECISION_MIXED) o.solver_criterion = NEON_STOP_RELATIVE;
+        o.record_trajectory = options_.record_trajectory ? 1 : 0;
         std::vector<neon_dirichlet> bcs;
         for (auto const& b : boundaries_)
             bcs.push_back(neon_dirichlet{std::int32_t(b.dofs.size()), b.dofs.data(), std::int32_t(b.times.size()),

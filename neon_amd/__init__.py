@@ -24,6 +24,7 @@ from ._capi import (  # noqa: F401
     PRECISION_MIXED,
     SOLVER_BICGSTAB,
     SOLVER_CG,
+    SPMV_LDS,
     SPMV_NONTEMPORAL,
     SPMV_PIPELINE,
     SPMV_SWEEP,
@@ -316,7 +317,8 @@ class static_matrix:
     {period, increments{initial, minimum, maximum, adaptive}}, ``nonlinear_options``
     {displacement_tolerance, residual_tolerance, linear_iterations, absolute_tolerance},
     ``linear_solver`` {tolerance, maximum_iterations, backend{precision}}; plus ``criterion``,
-    ``norm_mode`` and ``max_load_steps`` extensions."""
+    ``norm_mode``, ``max_load_steps``, ``max_total_iterations`` and ``record_trajectory`` (keep the
+    load history on the device for the J2 adjoint) extensions."""
 
     def __init__(self, mesh: DeviceMesh, simulation: dict, boundaries: list):
         self.mesh = mesh
@@ -347,6 +349,7 @@ class static_matrix:
         o.precision = {"fp64": PRECISION_FP64, "mixed": PRECISION_MIXED}[backend.get("precision", "fp64")]
         o.max_load_steps = simulation.get("max_load_steps", 0)
         o.max_total_iterations = simulation.get("max_total_iterations", 0)
+        o.record_trajectory = int(bool(simulation.get("record_trajectory", False)))
         self.options = o
         self.boundaries = boundaries
         self.records: list[dict] = []

@@ -41,6 +41,7 @@ KERNEL_ASSEMBLY = 3
 SPMV_NONTEMPORAL = 1
 SPMV_SWEEP = 2
 SPMV_PIPELINE = 4
+SPMV_LDS = 8
 
 STATUS_NAMES = {
     0: "OK",
@@ -90,6 +91,7 @@ class NewtonOptions(C.Structure):
         ("precision", C.c_int32),
         ("max_load_steps", C.c_int32),
         ("max_total_iterations", C.c_int32),
+        ("record_trajectory", C.c_int32),
     ]
 
 

@@ -328,9 +328,10 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        if ((lanes != 16 && lanes != 32) || (flags & ~7) || blocks_fp64 < 0 || blocks_fp32 < 0)
+        if ((lanes != 16 && lanes != 32) || (flags & ~15) || blocks_fp64 < 0 || blocks_fp32 < 0)
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
-        if ((flags & NEON_SPMV_PIPELINE) && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "the pipelined SpMV uses 32 lanes");
+        if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS)) && lanes != 32)
+            throw status_error(NEON_BAD_ARGUMENT, "the pipelined and LDS-staged SpMVs use 32 lanes");
         spmv_config& c = ctx->mesh->spmv;
         c.lanes = lanes;
         c.flags = flags;
@@ -353,7 +354,11 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
         A.nb_nt = (flags & NEON_SPMV_NONTEMPORAL) != 0;
         A.nb_sweep = (flags & NEON_SPMV_SWEEP) != 0;
         A.nb_pipeline = (flags & NEON_SPMV_PIPELINE) != 0;
-        if (A.nb_pipeline && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "the pipelined SpMV uses 32 lanes");
+        A.nb_lds = (flags & NEON_SPMV_LDS) != 0;
+        if ((A.nb_pipeline || A.nb_lds) && lanes != 32)
+            throw status_error(NEON_BAD_ARGUMENT, "the pipelined and LDS-staged SpMVs use 32 lanes");
+        if ((A.nb_pipeline || A.nb_lds) && m.max_degree() > 32)
+            throw status_error(NEON_BAD_ARGUMENT, "the pipelined and LDS-staged SpMVs need rows of at most 32 blocks");
         A.blocks_override = blocks;
         int64_t const n = m.dofs();
         hipStream_t const s = ctx->stream;
@@ -424,8 +429,6 @@ int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed
         device_mesh& m = *ctx->mesh;
         if (!dJdu || !grad || n_fixed < 0 || (n_fixed > 0 && !fixed) || max_iterations <= 0 || !(tolerance > 0.0))
             throw status_error(NEON_BAD_ARGUMENT, "bad adjoint arguments");
-        if (m.model() != NEON_MODEL_NEOHOOKE)
-            throw status_error(NEON_BAD_ARGUMENT, "the adjoint gradient supports the neo-Hooke model");
         if (m.tangent_mode() != NEON_TANGENT_CONSISTENT)
             throw status_error(NEON_BAD_ARGUMENT, "the adjoint needs the consistent tangent (exact dR/du)");
         int64_t const n = m.dofs();
@@ -443,6 +446,14 @@ int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed
         {
             fx.resize(n_fixed);
             NEON_CHECK_HIP(hipMemcpyAsync(fx.data(), fixed, n_fixed * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        }
+        if (m.model() == NEON_MODEL_J2)
+        {
+            int const r = j2_adjoint(ctx, b.data(), n_fixed, fx.data(), tolerance, max_iterations, g.data(), iterations);
+            if (r != NEON_OK) return r;
+            NEON_CHECK_HIP(hipMemcpyAsync(grad, g.data(), 4ull * m.elements() * sizeof(double), hipMemcpyDeviceToHost, s));
+            NEON_CHECK_HIP(hipStreamSynchronize(s));
+            return NEON_OK;
         }
         // K(u) at the held state, with the constrained rows and columns of the forward solve
         int const st = m.update_internal_variables(m.u());

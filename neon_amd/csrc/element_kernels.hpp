@@ -565,4 +565,211 @@ __global__ __launch_bounds__(kBlock) void neohooke_parameter_gradient_kernel(
         grad[4 * e + 3] = 0.0;
     }
 }
+// ------------------------------------------------------- J2 load-history adjoint
+// Symmetric tensors in 6 components (11, 22, 33, 23, 13, 12); a : b is the full contraction.
+__device__ inline double ddot6(const double* a, const double* b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + 2.0 * (a[3] * b[3] + a[4] * b[4] + a[5] * b[5]);
+}
+
+// Vector-Jacobian product of the consistent-mode radial return (small_strain_J2_plasticity.cpp
+// :45-92 started from the committed history): (sigma, eps_p, ebar) = F(eps, eps_p_old,
+// ebar_old; K, G, sigma_y, H).  With e = eps - eps_p_old, d = dev e, N = d / |d|,
+// q = sqrt(3/2) 2G |d|, f = q - (sigma_y + H ebar_old) and, when f > 0, dgamma = f / (3G + H)
+// (the exact result of the reference's iteration for linear hardening):
+//   sigma = C : e - 2G sqrt(3/2) dgamma N,  eps_p = eps_p_old + sqrt(3/2) dgamma N,
+//   ebar = ebar_old + dgamma,  d N = (dev de - N (N : de)) / |d|.
+// Seeds sbar (sigma), epbar (eps_p), ebbar (ebar) give e_bar (= eps_bar), the history
+// adjoints and pbar = (K, G, sigma_y, H) adjoints.
+__device__ inline void j2_return_vjp(const double* eps, const double* ep_old, double eb_old, double K, double G,
+                                     double sy, double Hh, const double* sbar, const double* epbar, double ebbar,
+                                     double* e_bar, double* ep_old_bar, double* eb_old_bar, double* pbar)
+{
+    double const lambda = K - 2.0 * G / 3.0;
+    double e[6];
+    for (int k = 0; k < 6; ++k) e[k] = eps[k] - ep_old[k];
+    double const tr = e[0] + e[1] + e[2];
+    double const d[6] = {e[0] - tr / 3.0, e[1] - tr / 3.0, e[2] - tr / 3.0, e[3], e[4], e[5]};
+    double const dn = sqrt(ddot6(d, d));
+    double const a = sqrt(1.5);
+    double const f0 = a * 2.0 * G * dn - (sy + Hh * eb_old);
+    double const trs = sbar[0] + sbar[1] + sbar[2];
+    double Cs[6];
+    for (int k = 0; k < 6; ++k) Cs[k] = 2.0 * G * sbar[k] + (k < 3 ? lambda * trs : 0.0);
+    double const Kbar = trs * tr;
+    double const Gbar_el = 2.0 * ddot6(sbar, d);
+    if (!(f0 > 0.0))
+    {
+        for (int k = 0; k < 6; ++k)
+        {
+            e_bar[k] = Cs[k];
+            ep_old_bar[k] = epbar[k] - Cs[k];
+        }
+        *eb_old_bar = ebbar;
+        pbar[0] = Kbar;
+        pbar[1] = Gbar_el;
+        pbar[2] = 0.0;
+        pbar[3] = 0.0;
+        return;
+    }
+    double const h3 = 3.0 * G + Hh;
+    double const dg = f0 / h3;
+    double N[6];
+    for (int k = 0; k < 6; ++k) N[k] = d[k] / dn;
+    double const sN = ddot6(sbar, N), eN = ddot6(epbar, N);
+    double const gdg = -2.0 * G * a * sN + a * eN + ebbar; // adjoint of dgamma
+    double W[6];
+    for (int k = 0; k < 6; ++k) W[k] = dg * a * (epbar[k] - 2.0 * G * sbar[k]); // adjoint of N
+    double const WN = ddot6(W, N);
+    double const trW = W[0] + W[1] + W[2];
+    for (int k = 0; k < 6; ++k)
+    {
+        double const devW = W[k] - (k < 3 ? trW / 3.0 : 0.0);
+        e_bar[k] = Cs[k] + gdg * a * 2.0 * G / h3 * N[k] + (devW - WN * N[k]) / dn;
+        ep_old_bar[k] = epbar[k] - e_bar[k];
+    }
+    *eb_old_bar = ebbar - gdg * Hh / h3;
+    pbar[0] = Kbar;
+    pbar[1] = Gbar_el - 2.0 * a * dg * sN + gdg * (a * 2.0 * dn - 3.0 * dg) / h3;
+    pbar[2] = -gdg / h3;
+    pbar[3] = -gdg * (eb_old + dg) / h3;
+}
+
+// gradient operator g_a = dN_a J_X^{-1} (reference configuration) at QP l and w detJ_X
+__device__ inline double reference_gradients(const double (*Xe)[3], int l, double (*g)[3])
+{
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i * 3 + j] += Xe[a][i] * c_hex8.dN[l][a][j];
+    double Ji[9];
+    inv3(J, Ji);
+    for (int a = 0; a < 8; ++a)
+        for (int k = 0; k < 3; ++k)
+            g[a][k] = c_hex8.dN[l][a][0] * Ji[0 * 3 + k] + c_hex8.dN[l][a][1] * Ji[1 * 3 + k] + c_hex8.dN[l][a][2] * Ji[2 * 3 + k];
+    return c_hex8.w[l] * det3(J);
+}
+
+// symmetric part of sum_a v_a (x) g_a in 6 components
+__device__ inline void sym_gradient(const double (*ve)[3], const double (*g)[3], double* s)
+{
+    double G9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) G9[i * 3 + j] += ve[a][i] * g[a][j];
+    s[0] = G9[0];
+    s[1] = G9[4];
+    s[2] = G9[8];
+    s[3] = 0.5 * (G9[5] + G9[7]);
+    s[4] = 0.5 * (G9[2] + G9[6]);
+    s[5] = 0.5 * (G9[1] + G9[3]);
+}
+
+struct j2_history
+{
+    const double* ep_old; // 6 per QP, SoA k * Q + q: committed history h_{n-1}
+    const double* eb_old; // 1 per QP
+    double* mu_ep;        // 6 per QP: multiplier of eps_p_n (in), of eps_p_{n-1} (out)
+    double* mu_eb;        // 1 per QP
+    int64_t Q;
+};
+
+// rhs += sum_QP B^T (dh_n / d eps)^T mu_n, one thread per element of one colour (elements of
+// a colour share no node, so the read-modify-writes need no atomics)
+__global__ __launch_bounds__(kBlock) void j2_adjoint_rhs_kernel(const int32_t* __restrict__ elems, int32_t count,
+                                                                const int32_t* __restrict__ conn,
+                                                                const double* __restrict__ X,
+                                                                const double* __restrict__ u, element_params prm,
+                                                                j2_history h, double* __restrict__ rhs)
+{
+    int32_t const t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    int64_t const e = elems[t];
+    double Xe[8][3], ue[8][3], r[8][3];
+    for (int a = 0; a < 8; ++a)
+    {
+        int64_t const nd = conn[8 * e + a];
+        for (int i = 0; i < 3; ++i)
+        {
+            Xe[a][i] = X[3 * nd + i];
+            ue[a][i] = u[3 * nd + i];
+            r[a][i] = 0.0;
+        }
+    }
+    double const K = prm.p[4 * e], G = prm.p[4 * e + 1], sy = prm.p[4 * e + 2], Hh = prm.p[4 * e + 3];
+    double const zero[6] = {0, 0, 0, 0, 0, 0};
+    for (int l = 0; l < 8; ++l)
+    {
+        int64_t const q = 8 * e + l;
+        double g[8][3];
+        reference_gradients(Xe, l, g);
+        double eps[6], ep_old[6], mu[6];
+        sym_gradient(ue, g, eps);
+        for (int k = 0; k < 6; ++k)
+        {
+            ep_old[k] = h.ep_old[k * h.Q + q];
+            mu[k] = h.mu_ep[k * h.Q + q];
+        }
+        double eb_[6], epb[6], ebb, pb[4];
+        j2_return_vjp(eps, ep_old, h.eb_old[q], K, G, sy, Hh, zero, mu, h.mu_eb[q], eb_, epb, &ebb, pb);
+        double const E9[9] = {eb_[0], eb_[5], eb_[4], eb_[5], eb_[1], eb_[3], eb_[4], eb_[3], eb_[2]};
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i)
+                r[a][i] += E9[i * 3 + 0] * g[a][0] + E9[i * 3 + 1] * g[a][1] + E9[i * 3 + 2] * g[a][2];
+    }
+    for (int a = 0; a < 8; ++a)
+    {
+        int64_t const nd = conn[8 * e + a];
+        for (int i = 0; i < 3; ++i) rhs[3 * nd + i] += r[a][i];
+    }
+}
+
+// mu_{n-1} = (dh_n/dh_{n-1})^T mu_n - (d sigma_n/dh_{n-1})^T (w detJ B lambda_n) in place, and
+// grad_e += (dh_n/dp)^T mu_n - (d sigma_n/dp)^T (w detJ B lambda_n); one thread per element
+__global__ __launch_bounds__(kBlock) void j2_adjoint_history_kernel(int32_t E, const int32_t* __restrict__ conn,
+                                                                    const double* __restrict__ X,
+                                                                    const double* __restrict__ u,
+                                                                    const double* __restrict__ lambda,
+                                                                    element_params prm, j2_history h,
+                                                                    double* __restrict__ grad)
+{
+    int64_t const e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (e >= E) return;
+    double Xe[8][3], ue[8][3], le[8][3];
+    for (int a = 0; a < 8; ++a)
+    {
+        int64_t const nd = conn[8 * e + a];
+        for (int i = 0; i < 3; ++i)
+        {
+            Xe[a][i] = X[3 * nd + i];
+            ue[a][i] = u[3 * nd + i];
+            le[a][i] = lambda[3 * nd + i];
+        }
+    }
+    double const K = prm.p[4 * e], G = prm.p[4 * e + 1], sy = prm.p[4 * e + 2], Hh = prm.p[4 * e + 3];
+    double acc[4] = {0, 0, 0, 0};
+    for (int l = 0; l < 8; ++l)
+    {
+        int64_t const q = 8 * e + l;
+        double g[8][3];
+        double const wdet = reference_gradients(Xe, l, g);
+        double eps[6], gl[6], sbar[6], ep_old[6], mu[6];
+        sym_gradient(ue, g, eps);
+        sym_gradient(le, g, gl);
+        for (int k = 0; k < 6; ++k)
+        {
+            sbar[k] = -wdet * gl[k];
+            ep_old[k] = h.ep_old[k * h.Q + q];
+            mu[k] = h.mu_ep[k * h.Q + q];
+        }
+        double eb_[6], epb[6], ebb, pb[4];
+        j2_return_vjp(eps, ep_old, h.eb_old[q], K, G, sy, Hh, sbar, mu, h.mu_eb[q], eb_, epb, &ebb, pb);
+        for (int k = 0; k < 6; ++k) h.mu_ep[k * h.Q + q] = epb[k];
+        h.mu_eb[q] = ebb;
+        for (int k = 0; k < 4; ++k) acc[k] += pb[k];
+    }
+    for (int k = 0; k < 4; ++k) grad[4 * e + k] += acc[k];
+}
+
 } // namespace neon_amd
+

@@ -56,7 +56,9 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const X
         nb_matrix m{A.nodes, A.bptr, A.bcol};
 #define NEON_NB(L, NT, SW) \
     spmv_nb_kernel<L, 1, NT, SW, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate)
-        if (A.nb_pipeline)
+        if (A.nb_lds)
+            spmv_nb_lds_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
+        else if (A.nb_pipeline)
         {
             if (A.nb_nt)
                 spmv_nb_pipe_kernel<true, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
@@ -508,7 +510,7 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             if (prof) prof->end(s_, tok);
             mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, part_rr, part_rz);
             mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
-            mixed_fold_kernel<<<grid_for(n), kBlock, 0, s_>>>(n, x, x32, cur, 0);
+            mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, cur, 0);
             A.residual64(s_, b, x, r64, cur, 1);
             mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
             mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, nb, cur, nxt, max_it);
@@ -516,7 +518,7 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
         NEON_CHECK_HIP(hipGetLastError());
     }
     // fold whatever the recurrence holds since the last reliable update
-    mixed_fold_kernel<<<grid_for(n), kBlock, 0, s_>>>(n, x, x32, st, 1);
+    mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, st, 1);
     NEON_CHECK_HIP(hipGetLastError());
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
     solve_result res = finish(*host_state_, NEON_STOP_RELATIVE);
@@ -843,13 +845,13 @@ void device_mesh::assemble(const double* u_dev, int precision)
     NEON_CHECK_HIP(hipMemsetAsync(fint_.data(), 0, 3ull * N_ * sizeof(double), s_));
     if (precision == NEON_PRECISION_FP64)
     {
-        k64_.resize(nnz_);
-        NEON_CHECK_HIP(hipMemsetAsync(k64_.data(), 0, nnz_ * sizeof(double), s_));
+        k64_.resize(nnz_ + kValuePad);
+        NEON_CHECK_HIP(hipMemsetAsync(k64_.data(), 0, (nnz_ + kValuePad) * sizeof(double), s_));
     }
     else
     {
-        k32_.resize(nnz_);
-        NEON_CHECK_HIP(hipMemsetAsync(k32_.data(), 0, nnz_ * sizeof(float), s_));
+        k32_.resize(nnz_ + kValuePad);
+        NEON_CHECK_HIP(hipMemsetAsync(k32_.data(), 0, (nnz_ + kValuePad) * sizeof(float), s_));
     }
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
@@ -1019,6 +1021,7 @@ linear_operator device_mesh::op(int precision) const
     A.nb_sweep = (spmv.flags & NEON_SPMV_SWEEP) != 0;
     // the pipelined kernel holds one block per lane: rows of at most 32 blocks
     A.nb_pipeline = (spmv.flags & NEON_SPMV_PIPELINE) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
+    A.nb_lds = (spmv.flags & NEON_SPMV_LDS) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
     A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
     return A;
 }
@@ -1051,6 +1054,79 @@ void device_mesh::jacobi(double* minv, int eigen_rule, int precision)
     else
         nb_jacobi_kernel<float, double><<<grid_for(3ll * N_), kBlock, 0, s_>>>(N_, bptr_.data(), self_slot_.data(),
                                                                                  k32_.data(), minv, eigen_rule);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+void device_mesh::clear_trajectory()
+{
+    traj_u_.clear();
+    traj_ep_.clear();
+    traj_eb_.clear();
+}
+
+void device_mesh::record_step()
+{
+    int64_t const Q = 8ll * E_;
+    auto copy = [&](const double* src, int64_t count) {
+        auto a = std::make_unique<device_array<double>>();
+        a->resize(count);
+        NEON_CHECK_HIP(hipMemcpyAsync(a->data(), src, count * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        return a;
+    };
+    traj_u_.push_back(copy(u_.data(), 3ll * N_));
+    if (model_ == NEON_MODEL_J2)
+    {
+        traj_ep_.push_back(copy(eps_p_old_.data(), 6 * Q));
+        traj_eb_.push_back(copy(ebar_old_.data(), Q));
+    }
+}
+
+void device_mesh::restore_step(int32_t step)
+{
+    if (step < 0 || step >= trajectory_steps()) throw status_error(NEON_BAD_ARGUMENT, "no such recorded step");
+    int64_t const Q = 8ll * E_;
+    NEON_CHECK_HIP(hipMemcpyAsync(u_.data(), traj_u_[step]->data(), 3ll * N_ * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s_));
+    if (model_ == NEON_MODEL_J2)
+    {
+        NEON_CHECK_HIP(hipMemcpyAsync(eps_p_old_.data(), traj_ep_[step]->data(), 6 * Q * sizeof(double),
+                                      hipMemcpyDeviceToDevice, s_));
+        NEON_CHECK_HIP(hipMemcpyAsync(ebar_old_.data(), traj_eb_[step]->data(), Q * sizeof(double),
+                                      hipMemcpyDeviceToDevice, s_));
+    }
+}
+
+void device_mesh::j2_adjoint_begin()
+{
+    int64_t const Q = 8ll * E_;
+    mu_ep_.resize(6 * Q);
+    mu_eb_.resize(Q);
+    NEON_CHECK_HIP(hipMemsetAsync(mu_ep_.data(), 0, 6 * Q * sizeof(double), s_));
+    NEON_CHECK_HIP(hipMemsetAsync(mu_eb_.data(), 0, Q * sizeof(double), s_));
+}
+
+void device_mesh::j2_adjoint_rhs(double* rhs)
+{
+    int64_t const Q = 8ll * E_;
+    j2_history h{eps_p_old_.data(), ebar_old_.data(), mu_ep_.data(), mu_eb_.data(), Q};
+    element_params prm{params_.data()};
+    for (int c = 0; c + 1 < int(colour_offsets_.size()); ++c)
+    {
+        int32_t const count = colour_offsets_[c + 1] - colour_offsets_[c];
+        if (count <= 0) continue;
+        j2_adjoint_rhs_kernel<<<grid_for(count), kBlock, 0, s_>>>(elems_.data() + colour_offsets_[c], count,
+                                                                  conn_.data(), X_.data(), u_.data(), prm, h, rhs);
+        NEON_CHECK_HIP(hipGetLastError());
+    }
+}
+
+void device_mesh::j2_adjoint_history(const double* lambda, double* grad)
+{
+    int64_t const Q = 8ll * E_;
+    j2_history h{eps_p_old_.data(), ebar_old_.data(), mu_ep_.data(), mu_eb_.data(), Q};
+    element_params prm{params_.data()};
+    j2_adjoint_history_kernel<<<grid_for(E_), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_.data(), lambda, prm, h,
+                                                               grad);
     NEON_CHECK_HIP(hipGetLastError());
 }
 

@@ -7,6 +7,7 @@
 #include "neon_hip.h"
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -111,6 +112,7 @@ struct linear_operator
     bool nb_nt = false;
     bool nb_sweep = false;
     bool nb_pipeline = false;
+    bool nb_lds = false;
     int blocks_override = 0;
     // values
     const double* v64 = nullptr;
@@ -176,6 +178,10 @@ private:
     device_array<double> x64_, r64_;
 };
 
+// Tangent value arrays extend this many entries past nnz (zeroed): SpMV kernels that stream a
+// segment in 16-byte vectors may read up to one vector past its end.
+constexpr int64_t kValuePad = 16;
+
 // Shape of the node-block SpMV used by the Krylov solvers on a mesh tangent (the defaults
 // are the fastest measured at 6.4M dofs on MI355X): lanes per block row, NEON_SPMV_* flags,
 // workgroups for FP64 and FP32 values.
@@ -217,6 +223,19 @@ public:
     void jacobi(double* minv, int eigen_rule, int precision);
     // adjoint: grad (4 per element) = -lambda^T d f_int / d params at displacement u
     void parameter_gradient(const double* u_dev, const double* lambda_dev, double* grad_dev);
+
+    // load-history checkpoints (J2 adjoint): u_n and the committed history h_{n-1} of every
+    // converged step, kept on the device
+    void clear_trajectory();
+    void record_step();
+    int32_t trajectory_steps() const { return int32_t(traj_u_.size()); }
+    // u <- u_n, committed history <- h_{n-1}
+    void restore_step(int32_t step);
+    // J2 adjoint pieces at the state of restore_step + update_internal_variables:
+    // rhs += B^T (dh/d eps)^T mu;  mu <- history adjoint of the step, grad += parameter adjoint
+    void j2_adjoint_rhs(double* rhs);
+    void j2_adjoint_history(const double* lambda, double* grad);
+    void j2_adjoint_begin();
     int32_t tangent_mode() const { return tangent_mode_; }
 
     double* u() { return u_.data(); }
@@ -246,6 +265,8 @@ private:
     bool k64_valid_ = false, k32_valid_ = false;
     device_array<double> sigma_, coef_, detF_, eps_p_, ebar_, vm_, eps_p_old_, ebar_old_;
     device_array<uint32_t> status_;
+    std::vector<std::unique_ptr<device_array<double>>> traj_u_, traj_ep_, traj_eb_;
+    device_array<double> mu_ep_, mu_eb_;
     uint32_t* status_host_ = nullptr;
 };
 

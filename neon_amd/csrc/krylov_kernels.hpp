@@ -314,6 +314,98 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_pipe_kernel(nb_matrix A, const
     }
 }
 
+// LDS-staged node-block SpMV (32 lanes per block row, at most 32 blocks per row, XCD sweep
+// order).  The 8 node rows that one workgroup combines per iteration are consecutive, so their
+// values form one contiguous segment of the value array; the workgroup streams it into LDS
+// with 16-byte loads (every load instruction fully used), then each lane takes its block's
+// 9 values from LDS, gathers x and the row sums reduce across the 32 lanes as in
+// spmv_nb_kernel.  The value array must extend 16 bytes past nnz (loads round the segment end
+// up to a whole vector).
+template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(kBlock) void spmv_nb_lds_kernel(nb_matrix A, const VT* __restrict__ val,
+                                                             const XT* __restrict__ x, YT* __restrict__ y,
+                                                             const double* __restrict__ bvec,
+                                                             double* __restrict__ part,
+                                                             const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate && !st->refine))) return;
+    constexpr int LPR = 32;
+    constexpr int G = kBlock / LPR;              // node rows per iteration
+    constexpr int VPV = 16 / int(sizeof(VT));    // values per 16-byte vector
+    constexpr int CAP = G * 9 * LPR + 2 * VPV;   // segment of G rows of <= 32 blocks + alignment
+    typedef VT vec __attribute__((ext_vector_type(VPV)));
+    __shared__ __align__(16) VT sv[CAP];
+    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    int64_t const per = (int64_t(A.nodes) + 7) / 8;
+    int64_t beg, end, step, first;
+    if ((gridDim.x & 7) == 0)
+    {
+        beg = int64_t(blockIdx.x & 7) * per;
+        end = beg + per < A.nodes ? beg + per : A.nodes;
+        step = int64_t(gridDim.x >> 3) * G;
+        first = beg + int64_t(blockIdx.x >> 3) * G;
+    }
+    else
+    {
+        xcd_chunk(A.nodes, beg, end);
+        step = G;
+        first = beg;
+    }
+    double dot = 0.0;
+    for (int64_t n0 = first; n0 < end; n0 += step)
+    {
+        int64_t const rows = end - n0 < G ? end - n0 : G;
+        int64_t const s0 = 9ll * A.bptr[n0], s1 = 9ll * A.bptr[n0 + rows];
+        int64_t const a0 = s0 & ~int64_t(VPV - 1);
+        int const nv = int((s1 - a0 + VPV - 1) / VPV);
+        const vec* src = reinterpret_cast<const vec*>(val + a0);
+        vec* dst = reinterpret_cast<vec*>(sv);
+        for (int v = threadIdx.x; v < nv; v += kBlock) dst[v] = __builtin_nontemporal_load(src + v);
+        __syncthreads();
+        if (g < rows)
+        {
+            int64_t const n = n0 + g;
+            int32_t const b0 = A.bptr[n], d = A.bptr[n + 1] - b0;
+            bool const on = l < d;
+            int const q = on ? l : 0;
+            int64_t const m = A.bcol[b0 + q];
+            AT xs[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) xs[j] = on ? AT(x[3 * m + j]) : AT(0);
+            const VT* pv = sv + (9ll * b0 - a0) + 3 * q;
+            int const rs = 3 * d;
+            AT s0v = AT(pv[0]) * xs[0] + AT(pv[1]) * xs[1] + AT(pv[2]) * xs[2];
+            AT s1v = AT(pv[rs]) * xs[0] + AT(pv[rs + 1]) * xs[1] + AT(pv[rs + 2]) * xs[2];
+            AT s2v = AT(pv[2 * rs]) * xs[0] + AT(pv[2 * rs + 1]) * xs[1] + AT(pv[2 * rs + 2]) * xs[2];
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1)
+            {
+                s0v += __shfl_xor(s0v, off, LPR);
+                s1v += __shfl_xor(s1v, off, LPR);
+                s2v += __shfl_xor(s2v, off, LPR);
+            }
+            if (l < 3)
+            {
+                AT const yi = l == 0 ? s0v : (l == 1 ? s1v : s2v);
+                int64_t const row = 3 * n + l;
+                YT yo;
+                if constexpr (MODE == 1)
+                    yo = YT(bvec[row] - double(yi));
+                else
+                    yo = YT(yi);
+                y[row] = yo;
+                if constexpr (DOT) dot += double(x[row]) * double(yo);
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<kBlock>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
 // streaming read of n doubles in 16-byte loads (calibration of the achievable read bandwidth)
 __global__ __launch_bounds__(kBlock) void stream_sum_kernel(int64_t n, const double* __restrict__ v,
                                                             double* __restrict__ out)
@@ -748,8 +840,9 @@ __global__ __launch_bounds__(kBlock) void mixed_fold_kernel(int64_t n, double* _
                                                             const krylov_state* __restrict__ st, int force)
 {
     if (!force && (st->done || !st->refine)) return;
-    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i < n)
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
         x64[i] += double(x32[i]);
         x32[i] = 0.0f;

@@ -155,6 +155,7 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
     minv.resize(n);
     NEON_CHECK_HIP(hipMemsetAsync(u_old.data(), 0, n * sizeof(double), s));
     NEON_CHECK_HIP(hipMemsetAsync(m.u(), 0, n * sizeof(double), s));
+    if (o->record_trajectory) m.clear_trajectory();
     NEON_CHECK_HIP(hipMemsetAsync(delta.data(), 0, n * sizeof(double), s));
     double* u = m.u();
     int const max_it = o->max_newton_iterations > 0 ? o->max_newton_iterations : 10;
@@ -287,6 +288,7 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
                 break;
             }
             NEON_CHECK_HIP(hipMemcpyAsync(u_old.data(), u, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            if (o->record_trajectory) m.record_step();
             ts.update(true);
             m.save_internal_variables(true);
             ++steps_done;
@@ -308,4 +310,52 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
     if (timing) *timing = tm;
     return result;
 }
+int j2_adjoint(neon_context* ctx, const double* dJdu_dev, int32_t n_fixed, const int32_t* fixed_dev, double tol,
+               int32_t max_it, double* grad_dev, int32_t* iterations)
+{
+    device_mesh& m = *ctx->mesh;
+    hipStream_t const s = ctx->stream;
+    int32_t const steps = m.trajectory_steps();
+    if (steps == 0)
+        throw status_error(NEON_BAD_ARGUMENT, "no recorded load history: run neon_newton_solve with record_trajectory");
+    int64_t const n = m.dofs();
+    device_array<double> rhs, lam, minv;
+    rhs.resize(n);
+    lam.resize(n);
+    minv.resize(n);
+    NEON_CHECK_HIP(hipMemsetAsync(grad_dev, 0, 4ull * m.elements() * sizeof(double), s));
+    m.j2_adjoint_begin();
+    int32_t total = 0;
+    int status = NEON_OK;
+    for (int32_t step = steps - 1; step >= 0 && status == NEON_OK; --step)
+    {
+        // state of step n: u_n with the return map from the committed h_{n-1}
+        m.restore_step(step);
+        status = m.update_internal_variables(m.u());
+        if (status != NEON_OK) break;
+        m.assemble(m.u(), NEON_PRECISION_FP64);
+        if (step == steps - 1)
+            NEON_CHECK_HIP(hipMemcpyAsync(rhs.data(), dJdu_dev, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        else
+        {
+            NEON_CHECK_HIP(hipMemsetAsync(rhs.data(), 0, n * sizeof(double), s));
+            m.j2_adjoint_rhs(rhs.data());
+        }
+        m.enforce_dirichlet(n_fixed, fixed_dev, rhs.data(), NEON_PRECISION_FP64);
+        solve_result const r = device_solve(ctx, NEON_SOLVER_CG, NEON_STOP_RELATIVE, NEON_PRECISION_FP64, tol, max_it,
+                                            rhs.data(), lam.data(), minv.data());
+        total += r.iterations;
+        status = r.status;
+        if (status != NEON_OK) break;
+        m.j2_adjoint_history(lam.data(), grad_dev);
+    }
+    // back to the converged final state: u_N, history returned from h_{N-1} and committed
+    m.restore_step(steps - 1);
+    int const st = m.update_internal_variables(m.u());
+    m.save_internal_variables(true);
+    NEON_CHECK_HIP(hipStreamSynchronize(s));
+    if (iterations) *iterations = total;
+    return status != NEON_OK ? status : st;
+}
+
 } // namespace neon_amd

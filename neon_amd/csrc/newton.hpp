@@ -46,4 +46,10 @@ solve_result device_solve(neon_context* ctx, int solver, int criterion, int prec
 int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, const neon_dirichlet* bcs,
                  double* u_out, neon_newton_record* records, int32_t cap, int32_t* n_records,
                  neon_newton_timing* timing);
+
+// Adjoint of the recorded J2 load history (device buffers): grad (4 per element) of an
+// objective of the final displacement with gradient dJdu; returns a status, PCG iterations
+// summed into iterations.  The mesh's final converged state is restored.
+int j2_adjoint(neon_context* ctx, const double* dJdu_dev, int32_t n_fixed, const int32_t* fixed_dev, double tol,
+               int32_t max_it, double* grad_dev, int32_t* iterations);
 } // namespace neon_amd

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cfenv>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -223,6 +224,116 @@ int j2(const double* H, double bulk, double shear, double sy, double Hh, double*
         for (int j = 0; j < 6; ++j) D[i * 6 + j] = C[i * 6 + j] - c1 * Idev[i * 6 + j] + c2 * nv[i] * nv[j];
     return ORC_OK;
 }
+// The return map of j2() in consistent mode as a function of its inputs, templated on the
+// scalar so that complex-step differentiation gives its exact partial derivatives (adjoint
+// only).  Full 3x3 tensors; comparisons use real parts, so the complex evaluation takes the
+// same branch and iteration count as the real one.
+template <typename T>
+double re(T v)
+{
+    return std::real(v);
+}
+
+template <typename T>
+void j2_map(const T* strain, const T* ep_old, T eb_old, T bulk, T shear, T sy, T Hh, T* sigma, T* ep, T* eb)
+{
+    T const G = shear;
+    T const lambda = bulk - T(2.0) * shear / T(3.0);
+    T ee[9];
+    for (int i = 0; i < 9; ++i) ee[i] = strain[i] - ep_old[i];
+    T const tr = ee[0] + ee[4] + ee[8];
+    for (int i = 0; i < 9; ++i) sigma[i] = T(2.0) * G * ee[i];
+    sigma[0] += lambda * tr;
+    sigma[4] += lambda * tr;
+    sigma[8] += lambda * tr;
+    T const t = (sigma[0] + sigma[4] + sigma[8]) / T(3.0);
+    T dev[9];
+    for (int i = 0; i < 9; ++i) dev[i] = sigma[i];
+    dev[0] -= t;
+    dev[4] -= t;
+    dev[8] -= t;
+    T nn = T(0.0);
+    for (int i = 0; i < 9; ++i) nn += dev[i] * dev[i];
+    T const ndev = std::sqrt(nn);
+    T const vm_trial = std::sqrt(T(1.5)) * ndev;
+    for (int i = 0; i < 9; ++i) ep[i] = ep_old[i];
+    *eb = eb_old;
+    auto yield = [&](T dg) { return (vm_trial - T(3.0) * G * dg) - (sy + (eb_old + dg) * Hh); };
+    if (re(yield(T(0.0))) <= 0.0) return;
+    T dg = T(0.0);
+    T f = yield(dg);
+    for (int it = 0; re(f) > 1.0e-6 && it < 50; ++it)
+    {
+        dg += f / (T(3.0) * G + Hh);
+        f = yield(dg);
+    }
+    T const s32 = std::sqrt(T(1.5));
+    for (int i = 0; i < 9; ++i)
+    {
+        T const nrm = dev[i] / ndev;
+        ep[i] += dg * s32 * nrm;
+        sigma[i] -= T(2.0) * G * dg * s32 * nrm;
+    }
+    *eb = eb_old + dg;
+}
+
+// Vector-Jacobian product of j2_map at one point: given seeds (sig_bar, ep_bar: 3x3 tensors
+// paired by full contraction; eb_bar) returns the adjoints of strain (symmetric tensor E
+// with E : d strain), of ep_old (tensor), eb_old and the 4 parameters.  Each input
+// direction is a unit symmetric tensor ((e_i e_j + e_j e_i) / 2 off the diagonal), whose
+// adjoint equals the tensor component.
+void j2_map_vjp(const double* strain, const double* ep_old, double eb_old, const double* prm, const double* sig_bar,
+                const double* ep_bar, double eb_bar, double* strain_bar, double* ep_old_bar, double* eb_old_bar,
+                double* prm_bar)
+{
+    typedef std::complex<double> cplx;
+    double const h = 1e-30;
+    static const int pair[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+    auto evaluate = [&](int group, int k) {
+        cplx s[9], e0[9], ps[4], eb0 = eb_old;
+        for (int i = 0; i < 9; ++i)
+        {
+            s[i] = strain[i];
+            e0[i] = ep_old[i];
+        }
+        for (int i = 0; i < 4; ++i) ps[i] = prm[i];
+        cplx const d(0.0, h);
+        if (group < 2)
+        {
+            cplx* t = group == 0 ? s : e0;
+            int const a = pair[k][0], b = pair[k][1];
+            if (a == b)
+                t[3 * a + a] += d;
+            else
+            {
+                t[3 * a + b] += 0.5 * d;
+                t[3 * b + a] += 0.5 * d;
+            }
+        }
+        else if (group == 2)
+            eb0 += d;
+        else
+            ps[k] += d;
+        cplx sig[9], ep[9], eb;
+        j2_map<cplx>(s, e0, eb0, ps[0], ps[1], ps[2], ps[3], sig, ep, &eb);
+        double acc = eb_bar * eb.imag() / h;
+        for (int i = 0; i < 9; ++i) acc += (sig_bar[i] * sig[i].imag() + ep_bar[i] * ep[i].imag()) / h;
+        return acc;
+    };
+    for (int k = 0; k < 6; ++k)
+    {
+        int const a = pair[k][0], b = pair[k][1];
+        double const gs = evaluate(0, k), ge = evaluate(1, k);
+        strain_bar[3 * a + b] = strain_bar[3 * b + a] = gs;
+        ep_old_bar[3 * a + b] = ep_old_bar[3 * b + a] = ge;
+    }
+    *eb_old_bar = evaluate(2, 0);
+    for (int k = 0; k < 4; ++k) prm_bar[k] = evaluate(3, k);
+}
+
+// gradient operator of element e at QP l on the reference configuration: Bt = dN J_X^{-1}
+void reference_gradients(const orc_model& m, int64_t e, int l, double (*Bt)[3], double& wdetJ);
+
 } // namespace
 
 // ======================================================================= model
@@ -240,6 +351,8 @@ struct orc_model
     vector<double> Hg, F, sigma, detF, eps_p, ebar, vm;
     vector<double> Hg_o, F_o, sigma_o, detF_o, eps_p_o, ebar_o, vm_o;
     vector<double> D; // fourth-order tangent, not saved
+    // recorded load history: u_n and the committed history h_{n-1} of each converged step
+    vector<vector<double>> traj_u, traj_ep, traj_eb;
 };
 
 extern "C" {
@@ -1213,6 +1326,12 @@ int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, 
     orc_sparsity_fill(m, rp.data(), ci.data());
     vector<double> K(nnz);
     vector<double> f_int(n, 0.0), f_ext(n, 0.0), u(n, 0.0), u_old(n, 0.0), delta(n, 0.0), mres(n, 0.0);
+    if (opt->record_trajectory)
+    {
+        m->traj_u.clear();
+        m->traj_ep.clear();
+        m->traj_eb.clear();
+    }
     vector<int32_t> perm, inv;
     vector<int32_t> prp, pci;
     vector<double> pv, pb, px;
@@ -1359,6 +1478,12 @@ int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, 
                 break;
             }
             u_old = u;
+            if (opt->record_trajectory)
+            {
+                m->traj_u.push_back(u);
+                m->traj_ep.push_back(m->eps_p_o);
+                m->traj_eb.push_back(m->ebar_o);
+            }
             orc_time_update(ts, 1);
             orc_save_internal_variables(m, 1);
         }
@@ -1383,6 +1508,129 @@ int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, 
     if (n_records) *n_records = nrec;
     orc_time_destroy(ts);
     return result;
+}
+
+} // extern "C"
+
+namespace
+{
+void reference_gradients(const orc_model& m, int64_t e, int l, double (*Bt)[3], double& wdetJ)
+{
+    hex8 const& h = shape();
+    double Xe[24];
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) Xe[a * 3 + i] = m.X[3 * size_t(m.conn[8 * e + a]) + i];
+    double J[9], Ji[9];
+    local_F(Xe, h.dN[l], J);
+    inv3(J, Ji);
+    wdetJ = h.w[l] * det3(J);
+    for (int a = 0; a < 8; ++a)
+        for (int k = 0; k < 3; ++k)
+        {
+            double s = 0.0;
+            for (int j = 0; j < 3; ++j) s += h.dN[l][a][j] * Ji[j * 3 + k];
+            Bt[a][k] = s;
+        }
+}
+} // namespace
+
+extern "C" {
+
+int orc_trajectory_steps(const orc_model* m) { return m ? int(m->traj_u.size()) : 0; }
+
+// Discrete adjoint of the load history (consistent mode: the return map of step n starts
+// from the committed history h_{n-1} and strains are small on X).  With
+//   R_n(u_n, h_{n-1}) = f_int - f_ext = 0,  h_n = H(eps(u_n), h_{n-1}),
+// and multipliers lambda_n (dofs) and mu_n (per QP history), going backwards from n = N:
+//   K_n lambda_n = dJ/du [n = N] + sum_QP B^T (dH_n/d eps)^T mu_n,
+//   mu_{n-1} = (dH_n/dh_{n-1})^T mu_n - (d sigma_n/dh_{n-1})^T (w detJ B lambda_n),   mu_N = 0,
+//   dJ/dp += (dH_n/dp)^T mu_n - (d sigma_n/dp)^T (w detJ B lambda_n),
+// every partial derivative from j2_map_vjp.
+int orc_adjoint_j2(orc_model* m, const double* dJdu, int32_t n_fixed, const int32_t* fixed, double tolerance,
+                   double* grad)
+{
+    if (!m || !dJdu || !grad || (n_fixed > 0 && !fixed)) return ORC_BAD_ARGUMENT;
+    if (m->model != ORC_MODEL_J2 || m->tangent_mode != ORC_TANGENT_CONSISTENT) return ORC_BAD_ARGUMENT;
+    int const steps = int(m->traj_u.size());
+    if (steps == 0) return ORC_BAD_ARGUMENT;
+    int32_t const n = 3 * m->N;
+    size_t const Q = 8 * size_t(m->E);
+    // final state, restored at the end
+    vector<double> const ep_final = m->eps_p_o, eb_final = m->ebar_o, u_final = m->traj_u.back();
+
+    int64_t const nnz = orc_sparsity_nnz(m);
+    vector<int32_t> rp(n + 1), ci(nnz);
+    orc_sparsity_fill(m, rp.data(), ci.data());
+    vector<double> K(nnz), f(n), rhs(n), lam(n);
+    vector<double> mu_ep(9 * Q, 0.0), mu_eb(Q, 0.0);
+    std::fill(grad, grad + 4 * size_t(m->E), 0.0);
+    for (int step = steps - 1; step >= 0; --step)
+    {
+        m->eps_p_o = m->traj_ep[step];
+        m->ebar_o = m->traj_eb[step];
+        int const st = orc_update_internal_variables(m, m->traj_u[step].data(), 0.0);
+        if (st != ORC_OK) return st;
+        orc_assemble(m, rp.data(), ci.data(), K.data(), f.data(), 0);
+        // right-hand side: dJ/du at the last step, plus the history multipliers' pull-back
+        for (int32_t i = 0; i < n; ++i) rhs[i] = step == steps - 1 ? dJdu[i] : 0.0;
+        if (step < steps - 1)
+            for (int64_t e = 0; e < m->E; ++e)
+                for (int l = 0; l < 8; ++l)
+                {
+                    size_t const q = 8 * size_t(e) + l;
+                    double Bt[8][3], wdetJ;
+                    reference_gradients(*m, e, l, Bt, wdetJ);
+                    double strain[9], zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    const double* H = &m->Hg[9 * q];
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) strain[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]);
+                    double E[9], epb[9], ebb, pb[4];
+                    j2_map_vjp(strain, &m->eps_p_o[9 * q], m->ebar_o[q], &m->params[4 * e], zero, &mu_ep[9 * q],
+                               mu_eb[q], E, epb, &ebb, pb);
+                    for (int a = 0; a < 8; ++a)
+                        for (int i = 0; i < 3; ++i)
+                        {
+                            double s = 0.0;
+                            for (int j = 0; j < 3; ++j) s += E[3 * i + j] * Bt[a][j];
+                            rhs[3 * size_t(m->conn[8 * e + a]) + i] += s;
+                        }
+                }
+        orc_enforce_dirichlet(n, rp.data(), ci.data(), K.data(), rhs.data(), n_fixed, fixed);
+        int32_t its = 0;
+        double err = 0.0;
+        int const cg = orc_cg_eigen(n, rp.data(), ci.data(), K.data(), rhs.data(), lam.data(), tolerance, 1000000, 0,
+                                    &its, &err);
+        if (cg != ORC_OK) return cg;
+        // history multipliers of the previous step and the parameter gradient
+        for (int64_t e = 0; e < m->E; ++e)
+            for (int l = 0; l < 8; ++l)
+            {
+                size_t const q = 8 * size_t(e) + l;
+                double Bt[8][3], wdetJ;
+                reference_gradients(*m, e, l, Bt, wdetJ);
+                double gl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int a = 0; a < 8; ++a)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) gl[3 * i + j] += lam[3 * size_t(m->conn[8 * e + a]) + i] * Bt[a][j];
+                double sb[9], strain[9];
+                const double* H = &m->Hg[9 * q];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                    {
+                        sb[3 * i + j] = -wdetJ * 0.5 * (gl[3 * i + j] + gl[3 * j + i]);
+                        strain[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]);
+                    }
+                double E[9], epb[9], ebb, pb[4];
+                j2_map_vjp(strain, &m->eps_p_o[9 * q], m->ebar_o[q], &m->params[4 * e], sb, &mu_ep[9 * q], mu_eb[q],
+                           E, epb, &ebb, pb);
+                for (int i = 0; i < 9; ++i) mu_ep[9 * q + i] = epb[i];
+                mu_eb[q] = ebb;
+                for (int k = 0; k < 4; ++k) grad[4 * e + k] += pb[k];
+            }
+    }
+    m->eps_p_o = ep_final;
+    m->ebar_o = eb_final;
+    return orc_update_internal_variables(m, u_final.data(), 0.0);
 }
 
 } // extern "C"

@@ -167,6 +167,7 @@ typedef struct {
     int32_t solver;                /* 0 CG, 1 BiCGSTAB */
     int32_t threads;
     int32_t rcm;                   /* apply RCM + P^T K P per solve like neon's CPU CG */
+    int32_t record_trajectory;     /* keep u and the committed J2 history of every converged step */
 } orc_newton_options;
 
 typedef struct {
@@ -182,6 +183,17 @@ typedef struct {
 int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, const orc_dirichlet* bcs,
                      double* u /* in: ignored (zero start); out: final displacement */,
                      orc_newton_record* records, int32_t record_capacity, int32_t* n_records);
+
+/* Adjoint of the whole load history of a J2 model in consistent mode (no counterpart in the
+ * reference), after orc_newton_solve with record_trajectory: objective J(u_N) with gradient
+ * dJdu (3N) at the final converged displacement, constrained dofs `fixed`; grad (4 per
+ * element) = dJ/d(bulk, shear, yield, hardening).  Local derivatives of the radial return
+ * come from complex-step differentiation of the oracle's own return map; the linear solves
+ * are Jacobi-PCG to `tolerance` (relative).  The model's final state is restored. */
+int orc_adjoint_j2(orc_model* m, const double* dJdu, int32_t n_fixed, const int32_t* fixed, double tolerance,
+                   double* grad);
+/* number of recorded steps of the last trajectory */
+int orc_trajectory_steps(const orc_model* m);
 
 #ifdef __cplusplus
 }

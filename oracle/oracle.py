@@ -39,7 +39,7 @@ class NewtonOptions(C.Structure):
                 ("max_newton_iterations", C.c_int32), ("absolute_norm", C.c_int32), ("norm_mode", C.c_int32),
                 ("solver_tolerance", C.c_double), ("solver_max_iterations", C.c_int32),
                 ("solver_criterion", C.c_int32), ("solver", C.c_int32), ("threads", C.c_int32),
-                ("rcm", C.c_int32)]
+                ("rcm", C.c_int32), ("record_trajectory", C.c_int32)]
 
 
 class NewtonRecord(C.Structure):
@@ -59,6 +59,8 @@ _SIG = {
     "orc_get_qp_state": [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p],
     "orc_element_matrices": [C.c_void_p, C.c_int32, f64p, f64p],
     "orc_parameter_gradient": [C.c_void_p, f64p, f64p],
+    "orc_adjoint_j2": [C.c_void_p, f64p, C.c_int32, i32p, C.c_double, f64p],
+    "orc_trajectory_steps": [C.c_void_p],
     "orc_sparsity_fill": [C.c_void_p, i32p, i32p],
     "orc_assemble": [C.c_void_p, i32p, i32p, f64p, f64p, C.c_int32],
     "orc_enforce_dirichlet": [C.c_int32, i32p, i32p, f64p, f64p, C.c_int32, i32p],
@@ -183,6 +185,20 @@ class Model:
             raise RuntimeError(f"orc_parameter_gradient failed: {code}")
         return g
 
+    def adjoint_j2(self, dJdu, fixed, tolerance=1e-14):
+        """dJ/dparams (n_elem x 4) over the load history recorded by the last newton() with
+        record_trajectory (J2, consistent mode)."""
+        dJdu = np.ascontiguousarray(dJdu, dtype=np.float64)
+        fixed = np.ascontiguousarray(fixed, dtype=np.int32)
+        g = np.zeros((self.n_elem, 4))
+        code = lib().orc_adjoint_j2(self.h, _f(dJdu), len(fixed), _i(fixed), tolerance, _f(g))
+        if code != OK:
+            raise RuntimeError(f"orc_adjoint_j2 failed: {code}")
+        return g
+
+    def trajectory_steps(self):
+        return lib().orc_trajectory_steps(self.h)
+
     def pattern(self):
         nnz = lib().orc_sparsity_nnz(self.h)
         rp, ci = np.zeros(self.n_dofs + 1, np.int32), np.zeros(nnz, np.int32)
@@ -290,7 +306,7 @@ class TimeStep:
         return dict(step_time=st.value, last_step_time=lt.value, fully_applied=bool(fa.value), successful=su.value)
 
 
-def newton_options(sim: dict, criterion=STOP_RELATIVE, threads=0, rcm=0, solver=0):
+def newton_options(sim: dict, criterion=STOP_RELATIVE, threads=0, rcm=0, solver=0, record_trajectory=False):
     t = sim["time"]
     inc = t["increments"]
     nl = sim["nonlinear_options"]
@@ -299,4 +315,4 @@ def newton_options(sim: dict, criterion=STOP_RELATIVE, threads=0, rcm=0, solver=
         TimeOptions(t["period"], inc["initial"], inc["minimum"], inc["maximum"], int(bool(inc["adaptive"]))),
         nl["residual_tolerance"], nl["displacement_tolerance"], nl.get("linear_iterations", 10),
         int("absolute_tolerance" in nl), sim.get("norm_mode", 0), ls.get("tolerance", 1e-5),
-        ls.get("maximum_iterations", 2000), criterion, solver, threads, rcm)
+        ls.get("maximum_iterations", 2000), criterion, solver, threads, rcm, int(record_trajectory))

@@ -46,3 +46,45 @@ def test_oracle_adjoint_matches_finite_differences():
         fd = ah.oracle_fd(p, log_mu, u, d, target)
         adj = g @ d
         assert abs(fd - adj) <= 1e-6 * abs(adj), (fd, adj)
+
+
+# ------------------------------------------------------------- J2 load-history adjoint
+def _j2_case():
+    p = problems.notched_plate_j2(8, 4, 2, steps=5, hole_cells=1.0, residual_tolerance=1e-12,
+                                  displacement_tolerance=1e-12, norm_mode=1, solver_tolerance=1e-14,
+                                  max_iterations=100000)
+    w = np.random.default_rng(21).normal(size=p.n_dofs)
+    w[p.fixed_dofs()] = 0.0
+    return p, w
+
+
+def _j2_forward(p, params, record=False):
+    m = orc.Model(p.X, p.conn, params, model=orc.MODEL_J2, tangent=orc.TANGENT_CONSISTENT)
+    code, u, recs = m.newton(orc.newton_options(p.simulation, record_trajectory=record), p.boundaries)
+    assert code == orc.OK
+    return m, u, recs
+
+
+def test_oracle_j2_adjoint_matches_finite_differences():
+    # objective J = w . u_N (a fixed linear functional of the final displacement); gradient
+    # with respect to every element's yield stress, hardening, bulk and shear moduli
+    p, w = _j2_case()
+    params = p.params.copy()
+    m, u, recs = _j2_forward(p, params, record=True)
+    assert m.trajectory_steps() == 5
+    q = m.qp_state()
+    assert (q["ebar"] > 0).any() and (q["ebar"] == 0).any()  # part of the plate yielded
+    g = m.adjoint_j2(w, p.fixed_dofs())
+    rng = np.random.default_rng(5)
+    for k, rel in ((2, 1e-6), (3, 1e-5), (0, 1e-6), (1, 1e-6)):
+        d = rng.normal(size=p.n_elem)
+        h = rel * np.abs(params[:, k]).mean()
+        vals = []
+        for sgn in (1.0, -1.0):
+            pp = params.copy()
+            pp[:, k] += sgn * h * d
+            _, up, _ = _j2_forward(p, pp)
+            vals.append(w @ up)
+        fd = (vals[0] - vals[1]) / (2 * h)
+        adj = g[:, k] @ d
+        assert abs(fd - adj) <= 1e-6 * abs(adj) + 1e-9 * np.abs(g[:, k]).sum() * np.abs(d).max(), (k, fd, adj)

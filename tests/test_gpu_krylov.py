@@ -141,12 +141,13 @@ def test_mixed_newton_block(ctx):
 SPMV_SHAPES = [
     (32, 0), (16, 0), (32, neon_amd.SPMV_SWEEP), (32, neon_amd.SPMV_SWEEP | neon_amd.SPMV_NONTEMPORAL),
     (32, neon_amd.SPMV_PIPELINE), (32, neon_amd.SPMV_PIPELINE | neon_amd.SPMV_NONTEMPORAL),
+    (32, neon_amd.SPMV_LDS),
 ]
 
 
 @pytest.mark.parametrize("precision", [neon_amd.PRECISION_FP64, neon_amd.PRECISION_MIXED])
 def test_spmv_shapes_agree(ctx, precision):
-    # every SpMV shape (lanes, XCD sweep, nontemporal, pipelined) solves the device tangent to
+    # every SpMV shape (lanes, XCD sweep, nontemporal, pipelined, LDS-staged) solves the device tangent to
     # the same answer as the plain shape and with the same Krylov count up to rounding;
     # each solve's true residual meets the tolerance
     p = problems.block_rollers(12)
