@@ -27,6 +27,7 @@ from ._capi import (  # noqa: F401
     SPMV_LDS,
     SPMV_NONTEMPORAL,
     SPMV_PIPELINE,
+    SPMV_STREAM,
     SPMV_SWEEP,
     STOP_ABSOLUTE_PRECONDITIONED,
     STOP_RELATIVE,

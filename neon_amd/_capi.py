@@ -42,6 +42,9 @@ SPMV_NONTEMPORAL = 1
 SPMV_SWEEP = 2
 SPMV_PIPELINE = 4
 SPMV_LDS = 8
+SPMV_STREAM = 16
+SPMV_PROBE_DOT = 32
+SPMV_PROBE_DIRTY = 64
 
 STATUS_NAMES = {
     0: "OK",

@@ -328,10 +328,10 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        if ((lanes != 16 && lanes != 32) || (flags & ~15) || blocks_fp64 < 0 || blocks_fp32 < 0)
+        if ((lanes != 16 && lanes != 32) || (flags & ~31) || blocks_fp64 < 0 || blocks_fp32 < 0)
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
-        if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS)) && lanes != 32)
-            throw status_error(NEON_BAD_ARGUMENT, "the pipelined and LDS-staged SpMVs use 32 lanes");
+        if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS | NEON_SPMV_STREAM)) && lanes != 32)
+            throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs use 32 lanes");
         spmv_config& c = ctx->mesh->spmv;
         c.lanes = lanes;
         c.flags = flags;
@@ -355,10 +355,12 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
         A.nb_sweep = (flags & NEON_SPMV_SWEEP) != 0;
         A.nb_pipeline = (flags & NEON_SPMV_PIPELINE) != 0;
         A.nb_lds = (flags & NEON_SPMV_LDS) != 0;
-        if ((A.nb_pipeline || A.nb_lds) && lanes != 32)
-            throw status_error(NEON_BAD_ARGUMENT, "the pipelined and LDS-staged SpMVs use 32 lanes");
-        if ((A.nb_pipeline || A.nb_lds) && m.max_degree() > 32)
-            throw status_error(NEON_BAD_ARGUMENT, "the pipelined and LDS-staged SpMVs need rows of at most 32 blocks");
+        A.nb_stream = (flags & NEON_SPMV_STREAM) != 0;
+        bool const special = A.nb_pipeline || A.nb_lds || A.nb_stream;
+        if (special && lanes != 32)
+            throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs use 32 lanes");
+        if (special && m.max_degree() > 32)
+            throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs need rows of at most 32 blocks");
         A.blocks_override = blocks;
         int64_t const n = m.dofs();
         hipStream_t const s = ctx->stream;
@@ -366,15 +368,26 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
         NEON_CHECK_HIP(hipEventCreate(&e0));
         NEON_CHECK_HIP(hipEventCreate(&e1));
         float ms = 0.0f;
+        // NEON_SPMV_PROBE_DOT: the solvers' form with partials of x.y; NEON_SPMV_PROBE_DIRTY:
+        // each product follows a write of 4 vectors of n entries, as the Krylov updates do
+        device_array<double> part, dirty;
+        bool const dot = (flags & NEON_SPMV_PROBE_DOT) != 0;
+        bool const dirt = (flags & NEON_SPMV_PROBE_DIRTY) != 0;
+        if (dot) part.resize(kMaxPartials);
+        if (dirt) dirty.resize(4 * n);
         if (precision == NEON_PRECISION_FP64)
         {
             device_array<double> x, y;
             x.resize(n);
             y.resize(n);
             fill_ones(s, n, x.data());
-            A.apply64(s, x.data(), y.data(), nullptr, nullptr); // warm
+            A.apply64(s, x.data(), y.data(), part.data(), nullptr); // warm
             NEON_CHECK_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r) A.apply64(s, x.data(), y.data(), nullptr, nullptr);
+            for (int r = 0; r < reps; ++r)
+            {
+                if (dirt) fill_ones(s, 4 * n, dirty.data());
+                A.apply64(s, x.data(), y.data(), part.data(), nullptr);
+            }
             NEON_CHECK_HIP(hipEventRecord(e1, s));
             NEON_CHECK_HIP(hipEventSynchronize(e1));
         }
@@ -384,11 +397,35 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
             x.resize(n);
             y.resize(n);
             fill_ones(s, n, x.data());
-            A.apply32(s, x.data(), y.data(), nullptr, nullptr);
+            A.apply32(s, x.data(), y.data(), part.data(), nullptr);
             NEON_CHECK_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r) A.apply32(s, x.data(), y.data(), nullptr, nullptr);
+            for (int r = 0; r < reps; ++r)
+            {
+                if (dirt) fill_ones(s, 4 * n, dirty.data());
+                A.apply32(s, x.data(), y.data(), part.data(), nullptr);
+            }
             NEON_CHECK_HIP(hipEventRecord(e1, s));
             NEON_CHECK_HIP(hipEventSynchronize(e1));
+        }
+        if (dirt)
+        {
+            // subtract the fills, timed alone
+            hipEvent_t f0, f1;
+            NEON_CHECK_HIP(hipEventCreate(&f0));
+            NEON_CHECK_HIP(hipEventCreate(&f1));
+            NEON_CHECK_HIP(hipEventRecord(f0, s));
+            for (int r = 0; r < reps; ++r) fill_ones(s, 4 * n, dirty.data());
+            NEON_CHECK_HIP(hipEventRecord(f1, s));
+            NEON_CHECK_HIP(hipEventSynchronize(f1));
+            float fm = 0.0f, tm = 0.0f;
+            NEON_CHECK_HIP(hipEventElapsedTime(&fm, f0, f1));
+            NEON_CHECK_HIP(hipEventElapsedTime(&tm, e0, e1));
+            (void)hipEventDestroy(f0);
+            (void)hipEventDestroy(f1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            *avg_ms = double(tm - fm) / reps;
+            return NEON_OK;
         }
         NEON_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);

@@ -56,7 +56,9 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const X
         nb_matrix m{A.nodes, A.bptr, A.bcol};
 #define NEON_NB(L, NT, SW) \
     spmv_nb_kernel<L, 1, NT, SW, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate)
-        if (A.nb_lds)
+        if (A.nb_stream)
+            spmv_nb_stream_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
+        else if (A.nb_lds)
             spmv_nb_lds_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
         else if (A.nb_pipeline)
         {
@@ -509,11 +511,18 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             A.apply32(s_, p32, Ap32, part_pAp, cur);
             if (prof) prof->end(s_, tok);
             mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, part_rr, part_rz);
-            mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
-            mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, cur, 0);
-            A.residual64(s_, b, x, r64, cur, 1);
-            mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
-            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, nb, cur, nxt, max_it);
+            // the reliable-update / true-residual convergence test runs every refine_every
+            // iterations; in between, update_p takes r.r and r.z from the x/r update
+            int const decided = (launched + 1) % refine_every == 0 ? 1 : 0;
+            if (decided)
+            {
+                mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
+                mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, cur, 0);
+                A.residual64(s_, b, x, r64, cur, 1);
+                mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
+            }
+            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, nb,
+                                                         decided, cur, nxt, max_it);
         }
         NEON_CHECK_HIP(hipGetLastError());
     }
@@ -1022,6 +1031,7 @@ linear_operator device_mesh::op(int precision) const
     // the pipelined kernel holds one block per lane: rows of at most 32 blocks
     A.nb_pipeline = (spmv.flags & NEON_SPMV_PIPELINE) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
     A.nb_lds = (spmv.flags & NEON_SPMV_LDS) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
+    A.nb_stream = (spmv.flags & NEON_SPMV_STREAM) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
     A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
     return A;
 }

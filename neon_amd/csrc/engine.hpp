@@ -113,6 +113,7 @@ struct linear_operator
     bool nb_sweep = false;
     bool nb_pipeline = false;
     bool nb_lds = false;
+    bool nb_stream = false;
     int blocks_override = 0;
     // values
     const double* v64 = nullptr;
@@ -158,6 +159,9 @@ public:
     int poll_every = 16;
     // reliable-update trigger of the mixed-precision CG: ||r32|| < delta ||r at last update||
     double reliable_delta = 0.1;
+    // the mixed CG tests the trigger (and, through the true residual, convergence) every this
+    // many iterations
+    int refine_every = 4;
 
 private:
     void ensure(int64_t n);

@@ -75,19 +75,17 @@ __device__ inline T ld(const T* p)
 // block's column index, the three x entries of the column node and the 9 values (3 runs of 3
 // contiguous values, one per dof row), then the three row sums are reduced across the group.
 // NT streams the values with nontemporal loads (read once per product) so that x stays cached.
+// With DOT, also adds the lane's share of x . y: sum_i x_{n,i} (K_nm x_m)_i for its block, so
+// that the shares of a row sum to x_n . y_n without waiting for the row reduction.
+// b0, d: the rows' first block and block count (bptr[n], bptr[n+1] - bptr[n]; 0 for rows not ok).
 template <int LPR, int ROWS, bool NT, typename VT, typename XT, typename AT>
 __device__ inline void nb_rows(nb_matrix A, const VT* __restrict__ val, const XT* __restrict__ x,
-                               const int32_t* n, const bool* ok, int lane, AT (*y)[3])
+                               const int32_t* n, const bool* ok, const int32_t* b0, const int32_t* d, int lane,
+                               AT (*y)[3])
 {
-    int32_t b0[ROWS], d[ROWS];
     AT s[ROWS][3];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r)
-    {
-        b0[r] = ok[r] ? A.bptr[n[r]] : 0;
-        d[r] = ok[r] ? A.bptr[n[r] + 1] - b0[r] : 0;
-        s[r][0] = s[r][1] = s[r][2] = AT(0);
-    }
+    for (int r = 0; r < ROWS; ++r) s[r][0] = s[r][1] = s[r][2] = AT(0);
     int32_t dmax = d[0];
 #pragma unroll
     for (int r = 1; r < ROWS; ++r) dmax = d[r] > dmax ? d[r] : dmax;
@@ -165,30 +163,43 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
     double dot = 0.0;
     for (int64_t n0 = start; n0 < end; n0 += stride * ROWS)
     {
-        int32_t n[ROWS];
+        int32_t n[ROWS], b0[ROWS], d[ROWS];
         bool ok[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r)
         {
             ok[r] = n0 + r * stride < end;
             n[r] = ok[r] ? int32_t(n0 + r * stride) : 0;
+            b0[r] = ok[r] ? A.bptr[n[r]] : 0;
+            d[r] = ok[r] ? A.bptr[n[r] + 1] - b0[r] : 0;
+        }
+        // the row's own x (dot) and b (residual) entries, requested before the row's blocks so
+        // that their latency overlaps (lanes >= 3 read a valid neighbour and drop it); the dot
+        // uses the stored (rounded) y, consistent with the recurrence that uses it
+        double br[ROWS];
+        XT xr[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+        {
+            int64_t const row = 3ll * n[r] + (l < 3 ? l : 2);
+            br[r] = MODE == 1 ? bvec[row] : 0.0;
+            xr[r] = DOT ? x[row] : XT(0);
         }
         AT yy[ROWS][3];
-        nb_rows<LPR, ROWS, NT, VT, XT, AT>(A, val, x, n, ok, l, yy);
-        if (l < 3)
-        {
+        nb_rows<LPR, ROWS, NT, VT, XT, AT>(A, val, x, n, ok, b0, d, l, yy);
 #pragma unroll
-            for (int r = 0; r < ROWS; ++r)
-            {
-                if (!ok[r]) continue;
-                AT const yi = l == 0 ? yy[r][0] : (l == 1 ? yy[r][1] : yy[r][2]);
-                int64_t const row = 3ll * n[r] + l;
-                if constexpr (MODE == 1)
-                    y[row] = YT(bvec[row] - double(yi));
-                else
-                    y[row] = YT(yi);
-                if constexpr (DOT) dot += double(x[row]) * double(YT(yi));
-            }
+        for (int r = 0; r < ROWS; ++r)
+        {
+            // lane l < 3 takes row sum l (selects on named values: no indexed private array)
+            AT const y0 = yy[r][0], y1 = yy[r][1], y2 = yy[r][2];
+            AT yi = l == 1 ? y1 : y0;
+            yi = l == 2 ? y2 : yi;
+            YT const yo = MODE == 1 ? YT(br[r] - double(yi)) : YT(yi);
+            bool const mine = ok[r] && l < 3;
+            // every lane consumes its loads on every path (a load left pending on one path
+            // would force a wait before the next row's loads reuse its register)
+            if constexpr (DOT) dot += mine ? double(xr[r]) * double(yo) : 0.0;
+            if (mine) y[3ll * n[r] + l] = yo;
         }
     }
     if constexpr (DOT)
@@ -398,6 +409,152 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_lds_kernel(nb_matrix A, const 
             }
         }
         __syncthreads();
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<kBlock>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
+// Streaming node-block SpMV (32 lanes per block row, at most 32 blocks per row, XCD sweep
+// order).  Each iteration of a workgroup combines 8 consecutive node rows, whose values, block
+// columns and row pointers are contiguous segments of their arrays.  The workgroup keeps the
+// current rows' segments in LDS and, before combining them, requests the next rows' segments
+// into registers with 16-byte loads (and the row pointers of the rows after), so the value
+// stream never waits on the x gather or the reductions of the rows being combined.  The value
+// array must extend one 16-byte vector past nnz.
+template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(kBlock) void spmv_nb_stream_kernel(nb_matrix A, const VT* __restrict__ val,
+                                                                const XT* __restrict__ x, YT* __restrict__ y,
+                                                                const double* __restrict__ bvec,
+                                                                double* __restrict__ part,
+                                                                const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate && !st->refine))) return;
+    constexpr int LPR = 32;
+    constexpr int G = kBlock / LPR;                   // node rows per iteration
+    constexpr int VPV = 16 / int(sizeof(VT));         // values per 16-byte vector
+    constexpr int NV = (G * 9 * LPR + 2 * VPV) / VPV; // vectors of a segment incl. alignment
+    constexpr int PER = (NV + kBlock - 1) / kBlock;   // vectors per thread
+    typedef VT vec __attribute__((ext_vector_type(VPV)));
+    __shared__ __align__(16) VT sv[NV * VPV];
+    __shared__ int32_t sc[G * LPR];
+    __shared__ int32_t sp[2][G + 1];
+    int const t = threadIdx.x;
+    int const g = t / LPR, l = t % LPR;
+    int64_t beg, end, step, first;
+    if ((gridDim.x & 7) == 0)
+    {
+        int64_t const per = (int64_t(A.nodes) + 7) / 8;
+        beg = int64_t(blockIdx.x & 7) * per;
+        end = beg + per < A.nodes ? beg + per : A.nodes;
+        step = int64_t(gridDim.x >> 3) * G;
+        first = beg + int64_t(blockIdx.x >> 3) * G;
+    }
+    else
+    {
+        xcd_chunk(A.nodes, beg, end);
+        step = G;
+        first = beg;
+    }
+    int64_t const iters = first < end ? (end - first + step - 1) / step : 0;
+    double dot = 0.0;
+    // row pointers bptr[n0 + k], k <= G, of iteration it (clamped at the end of the range)
+    auto pointer = [&](int64_t it) -> int32_t {
+        int64_t const n0 = first + it * step;
+        int64_t const rows = end - n0 < G ? end - n0 : G;
+        return A.bptr[n0 + (t < rows ? t : rows)];
+    };
+    vec pv[PER];
+    int32_t pc = 0, pp = 0;
+    // requests of the segments of iteration it, whose row pointers are in sp[it & 1]
+    auto request = [&](int64_t it) {
+        int32_t const* q = sp[it & 1];
+        int64_t const s0 = 9ll * q[0], s1 = 9ll * q[G];
+        int64_t const a0 = s0 & ~int64_t(VPV - 1);
+        int const nv = int((s1 - a0 + VPV - 1) / VPV);
+        const vec* src = reinterpret_cast<const vec*>(val + a0);
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+        {
+            int const v = t + k * kBlock;
+            int const vc = v < nv ? v : 0;
+            pv[k] = __builtin_nontemporal_load(src + vc);
+        }
+        int const nc = q[G] - q[0];
+        pc = A.bcol[q[0] + (t < nc ? t : 0)];
+    };
+    auto store = [&](int64_t it) {
+        int32_t const* q = sp[it & 1];
+        int64_t const s0 = 9ll * q[0], s1 = 9ll * q[G];
+        int64_t const a0 = s0 & ~int64_t(VPV - 1);
+        int const nv = int((s1 - a0 + VPV - 1) / VPV);
+        vec* dst = reinterpret_cast<vec*>(sv);
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+        {
+            int const v = t + k * kBlock;
+            if (v < nv) dst[v] = pv[k];
+        }
+        if (t < q[G] - q[0]) sc[t] = pc;
+    };
+    if (iters > 0)
+    {
+        if (t <= G) sp[0][t] = pointer(0);
+        if (t <= G && iters > 1) sp[1][t] = pointer(1);
+        __syncthreads();
+        request(0);
+        store(0);
+        __syncthreads();
+        for (int64_t it = 0; it < iters; ++it)
+        {
+            bool const more = it + 1 < iters;
+            if (more) request(it + 1);
+            if (t <= G && it + 2 < iters) pp = pointer(it + 2);
+            // combine the current rows from LDS
+            int64_t const n0 = first + it * step;
+            int64_t const rows = end - n0 < G ? end - n0 : G;
+            int32_t const* q = sp[it & 1];
+            if (g < rows)
+            {
+                int64_t const n = n0 + g;
+                int32_t const b0 = q[g], d = q[g + 1] - b0;
+                bool const on = l < d;
+                int const qq = on ? l : 0;
+                int64_t const m = sc[b0 - q[0] + qq];
+                int64_t const row = 3 * n + (l < 3 ? l : 2);
+                double const xr = DOT ? double(x[row]) : 0.0;
+                double const br = MODE == 1 ? bvec[row] : 0.0;
+                AT xs[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) xs[j] = on ? AT(x[3 * m + j]) : AT(0);
+                int64_t const a0 = (9ll * q[0]) & ~int64_t(VPV - 1);
+                const VT* pvv = sv + (9ll * b0 - a0) + 3 * qq;
+                int const rs = 3 * d;
+                AT s0v = AT(pvv[0]) * xs[0] + AT(pvv[1]) * xs[1] + AT(pvv[2]) * xs[2];
+                AT s1v = AT(pvv[rs]) * xs[0] + AT(pvv[rs + 1]) * xs[1] + AT(pvv[rs + 2]) * xs[2];
+                AT s2v = AT(pvv[2 * rs]) * xs[0] + AT(pvv[2 * rs + 1]) * xs[1] + AT(pvv[2 * rs + 2]) * xs[2];
+#pragma unroll
+                for (int off = LPR / 2; off > 0; off >>= 1)
+                {
+                    s0v += __shfl_xor(s0v, off, LPR);
+                    s1v += __shfl_xor(s1v, off, LPR);
+                    s2v += __shfl_xor(s2v, off, LPR);
+                }
+                if (l < 3)
+                {
+                    AT const yi = l == 0 ? s0v : (l == 1 ? s1v : s2v);
+                    YT const yo = MODE == 1 ? YT(br - double(yi)) : YT(yi);
+                    y[3 * n + l] = yo;
+                    if constexpr (DOT) dot += xr * double(yo);
+                }
+            }
+            __syncthreads();
+            if (more) store(it + 1);
+            if (t <= G && it + 2 < iters) sp[it & 1][t] = pp;
+            __syncthreads();
+        }
     }
     if constexpr (DOT)
     {
@@ -879,12 +1036,16 @@ __global__ __launch_bounds__(kBlock) void mixed_refresh_kernel(int64_t n, const 
 }
 
 // convergence on the true residual, beta, p32 = z + beta p32; block 0 writes the next state
+// (decided = 0: no decision kernel ran this iteration, so r.r and r.z come from the partials
+// of the x/r update and no reliable update is due)
 __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float* __restrict__ p32,
                                                                 const float* __restrict__ r32,
                                                                 const float* __restrict__ minv,
                                                                 const double* __restrict__ part_rr_true,
-                                                                const double* __restrict__ part_rz_true, int npart,
-                                                                const krylov_state* __restrict__ st,
+                                                                const double* __restrict__ part_rz_true,
+                                                                const double* __restrict__ part_rr,
+                                                                const double* __restrict__ part_rz, int npart,
+                                                                int decided, const krylov_state* __restrict__ st,
                                                                 krylov_state* __restrict__ st_next, int max_it)
 {
     krylov_state s = *st;
@@ -894,6 +1055,12 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
         return;
     }
     double rr = s.rr, rz = s.aux1;
+    if (!decided)
+    {
+        rr = reduce_partials<kBlock>(part_rr, npart);
+        rz = reduce_partials<kBlock>(part_rz, npart);
+        s.refine = 0;
+    }
     bool conv = false;
     if (s.refine)
     {
