@@ -32,6 +32,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_nb_neohooke128_traffic.txt")
+
+
+def measured_traffic(n):
+    """HBM bytes per launch of the FP32 node-block SpMV at the 128^3 workload, from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in the profile file (None
+    for other sizes or without the file)."""
+    if n != 128 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch_fp32"):
+                return int(line.split("=")[1])
+    return None
+
+
 def spmv_bytes(nnz, nnzb, nodes, n, value_bytes):
     """Algorithmic bytes of one node-block SpMV: values once, one int32 block column per 9
     values, block row pointers, x read once, y written once."""
@@ -187,7 +203,8 @@ def main():
                                f"steps; Newton steps of load step 1; Jacobi-PCG 1e-10; Newton target 1e-8",
                    "elements": p.n_elem, "dofs": p.n_dofs, "nnz": nnz},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.n) if head == "mixed" else None,
                      "kernel": f"spmv_nb_kernel<{'float' if vb == 4 else 'double'}> (node-block CSR SpMV + p.Ap)",
                      "algorithmic_bytes": bytes_spmv, "avg_ms": spmv_avg_ms, "launches": r["spmv_count"]},
     }
