@@ -147,6 +147,7 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
 #define NEON_SPMV_STREAM 16     /* LDS-staged with the next rows' segments in flight (32 lanes, sweep order) */
 #define NEON_SPMV_PROBE_DOT 32   /* probe only: the solvers' form with block partials of x . y */
 #define NEON_SPMV_PROBE_DIRTY 64 /* probe only: each product follows a write of 4 vectors of n (timed apart) */
+#define NEON_SPMV_META 128       /* row pointers of 32 rows per load, shared by shuffles (32 lanes, sweep order) */
 int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
                          int32_t reps, double* avg_ms);
 /* Shape of the SpMV that the solvers on this mesh's tangent use from now on (lanes, flags as

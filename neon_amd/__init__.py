@@ -25,6 +25,7 @@ from ._capi import (  # noqa: F401
     SOLVER_BICGSTAB,
     SOLVER_CG,
     SPMV_LDS,
+    SPMV_META,
     SPMV_NONTEMPORAL,
     SPMV_PIPELINE,
     SPMV_STREAM,

@@ -45,6 +45,7 @@ SPMV_LDS = 8
 SPMV_STREAM = 16
 SPMV_PROBE_DOT = 32
 SPMV_PROBE_DIRTY = 64
+SPMV_META = 128
 
 STATUS_NAMES = {
     0: "OK",

@@ -328,9 +328,9 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        if ((lanes != 16 && lanes != 32) || (flags & ~31) || blocks_fp64 < 0 || blocks_fp32 < 0)
+        if ((lanes != 16 && lanes != 32) || (flags & ~(31 | NEON_SPMV_META)) || blocks_fp64 < 0 || blocks_fp32 < 0)
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
-        if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS | NEON_SPMV_STREAM)) && lanes != 32)
+        if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS | NEON_SPMV_STREAM | NEON_SPMV_META)) && lanes != 32)
             throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs use 32 lanes");
         spmv_config& c = ctx->mesh->spmv;
         c.lanes = lanes;
@@ -356,6 +356,8 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
         A.nb_pipeline = (flags & NEON_SPMV_PIPELINE) != 0;
         A.nb_lds = (flags & NEON_SPMV_LDS) != 0;
         A.nb_stream = (flags & NEON_SPMV_STREAM) != 0;
+        A.nb_meta = (flags & NEON_SPMV_META) != 0;
+        if (A.nb_meta && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "the row-pointer-chunk SpMV uses 32 lanes");
         bool const special = A.nb_pipeline || A.nb_lds || A.nb_stream;
         if (special && lanes != 32)
             throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs use 32 lanes");

@@ -56,7 +56,9 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const X
         nb_matrix m{A.nodes, A.bptr, A.bcol};
 #define NEON_NB(L, NT, SW) \
     spmv_nb_kernel<L, 1, NT, SW, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate)
-        if (A.nb_stream)
+        if (A.nb_meta)
+            spmv_nb_meta_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
+        else if (A.nb_stream)
             spmv_nb_stream_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
         else if (A.nb_lds)
             spmv_nb_lds_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
@@ -1032,6 +1034,7 @@ linear_operator device_mesh::op(int precision) const
     A.nb_pipeline = (spmv.flags & NEON_SPMV_PIPELINE) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
     A.nb_lds = (spmv.flags & NEON_SPMV_LDS) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
     A.nb_stream = (spmv.flags & NEON_SPMV_STREAM) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
+    A.nb_meta = (spmv.flags & NEON_SPMV_META) != 0 && spmv.lanes == 32;
     A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
     return A;
 }

@@ -114,6 +114,7 @@ struct linear_operator
     bool nb_pipeline = false;
     bool nb_lds = false;
     bool nb_stream = false;
+    bool nb_meta = false;
     int blocks_override = 0;
     // values
     const double* v64 = nullptr;

@@ -209,6 +209,83 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
     }
 }
 
+// Node-block SpMV with the row pointers of 32 rows per load (32 lanes per block row, XCD sweep
+// order as above, any row length).  Lane t of a lane group holds bptr of the group's row
+// 32 c + t (and of the row after it) for the current chunk c, the next chunk's being requested
+// a chunk ahead, and each row takes its first block and length by a shuffle instead of a
+// dependent load: a row then waits on two memory round trips (columns with values, then the
+// x gather) instead of three.
+template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(kBlock, sizeof(VT) == 8 || sizeof(XT) == 8 ? 6 : 8) void spmv_nb_meta_kernel(nb_matrix A, const VT* __restrict__ val,
+                                                              const XT* __restrict__ x, YT* __restrict__ y,
+                                                              const double* __restrict__ bvec,
+                                                              double* __restrict__ part,
+                                                              const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate && !st->refine))) return;
+    constexpr int LPR = 32;
+    constexpr int G = kBlock / LPR;
+    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    int64_t start, end, stride;
+    if ((gridDim.x & 7) == 0)
+    {
+        int64_t const per = (int64_t(A.nodes) + 7) / 8;
+        int64_t const beg = int64_t(blockIdx.x & 7) * per;
+        end = beg + per < A.nodes ? beg + per : A.nodes;
+        stride = int64_t(gridDim.x >> 3) * G;
+        start = beg + int64_t(blockIdx.x >> 3) * G + g;
+    }
+    else
+    {
+        int64_t beg;
+        xcd_chunk(A.nodes, beg, end);
+        stride = G;
+        start = beg + g;
+    }
+    int64_t const count = start < end ? (end - start + stride - 1) / stride : 0;
+    // row pointers of row 32 c + l (clamped to the last row; past the end the row is empty)
+    auto meta = [&](int64_t c, int32_t& lo, int32_t& hi) {
+        int64_t const t = c * LPR + l;
+        int64_t const tc = t < count ? t : (count > 0 ? count - 1 : 0);
+        int64_t const n = count > 0 ? start + tc * stride : 0;
+        lo = A.bptr[n];
+        hi = t < count ? A.bptr[n + 1] : lo;
+    };
+    double dot = 0.0;
+    int32_t mlo, mhi, nlo, nhi;
+    meta(0, mlo, mhi);
+    meta(1, nlo, nhi);
+    for (int64_t t = 0; t < count; ++t)
+    {
+        int const ln = int(t % LPR);
+        if (ln == 0 && t > 0)
+        {
+            mlo = nlo;
+            mhi = nhi;
+            meta(t / LPR + 1, nlo, nhi);
+        }
+        int32_t const b0 = __shfl(mlo, ln, LPR);
+        int32_t const d = __shfl(mhi, ln, LPR) - b0;
+        int32_t const n = int32_t(start + t * stride);
+        int64_t const row = 3ll * n + (l < 3 ? l : 2);
+        double const br = MODE == 1 ? bvec[row] : 0.0;
+        XT const xr = DOT ? x[row] : XT(0);
+        bool const ok = true;
+        AT yy[1][3];
+        nb_rows<LPR, 1, false, VT, XT, AT>(A, val, x, &n, &ok, &b0, &d, l, yy);
+        AT yi = l == 1 ? yy[0][1] : yy[0][0];
+        yi = l == 2 ? yy[0][2] : yi;
+        YT const yo = MODE == 1 ? YT(br - double(yi)) : YT(yi);
+        if constexpr (DOT) dot += l < 3 ? double(xr) * double(yo) : 0.0;
+        if (l < 3) y[3ll * n + l] = yo;
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<kBlock>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
 // Software-pipelined node-block SpMV (32 lanes per block row, at most 32 blocks per row,
 // XCD sweep order as above).  Iteration t of a lane group gathers x for row t, requests the
 // row pointers of row t+2 and the column indices and 9 values per lane of row t+1, and only

@@ -146,7 +146,7 @@ def test_mixed_newton_block(ctx):
 SPMV_SHAPES = [
     (32, 0), (16, 0), (32, neon_amd.SPMV_SWEEP), (32, neon_amd.SPMV_SWEEP | neon_amd.SPMV_NONTEMPORAL),
     (32, neon_amd.SPMV_PIPELINE), (32, neon_amd.SPMV_PIPELINE | neon_amd.SPMV_NONTEMPORAL),
-    (32, neon_amd.SPMV_LDS), (32, neon_amd.SPMV_STREAM),
+    (32, neon_amd.SPMV_LDS), (32, neon_amd.SPMV_STREAM), (32, neon_amd.SPMV_META),
 ]
 
 
