@@ -155,6 +155,11 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
  * meshes whose nodes have at most 32 neighbours and falls back to the plain one otherwise.
  * No counterpart in the reference: a tuning knob; results agree to rounding. */
 int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);
+/* Average milliseconds of one assembly (tangent + f_int, all colours) of the held state with
+ * `blocks_per_cu` one-wavefront workgroups per compute unit and colour (default 128).  No
+ * counterpart in the reference: a tuning probe. */
+int neon_mesh_assembly_bench(neon_context* ctx, int32_t precision, int32_t blocks_per_cu, int32_t reps,
+                             double* avg_ms);
 /* Average milliseconds of a streaming read and of a copy of `bytes` bytes (calibration of the
  * achievable HBM bandwidth and of the traffic counters). */
 int neon_stream_probe(neon_context* ctx, int64_t bytes, int32_t reps, double* read_ms, double* copy_ms);

@@ -150,6 +150,7 @@ SIGNATURES = {
     "neon_mesh_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int32, f64p, f64p, i32p, f64p]),
     "neon_mesh_spmv_bench": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p]),
     "neon_mesh_spmv_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "neon_mesh_assembly_bench": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, f64p]),
     "neon_stream_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f64p, f64p]),
     "neon_newton_solve": (
         C.c_int,

@@ -437,6 +437,29 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
     });
 }
 
+int neon_mesh_assembly_bench(neon_context* ctx, int32_t precision, int32_t blocks_per_cu, int32_t reps,
+                             double* avg_ms)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (blocks_per_cu <= 0 || blocks_per_cu > 1024 || reps <= 0 || !avg_ms)
+            throw status_error(NEON_BAD_ARGUMENT, "bad assembly probe arguments");
+        device_mesh& m = *ctx->mesh;
+        int const keep = m.assembly_blocks_per_cu;
+        m.assembly_blocks_per_cu = blocks_per_cu;
+        double total = 0.0;
+        m.assemble(m.u(), precision); // warm
+        for (int r = 0; r < reps; ++r)
+        {
+            m.assemble(m.u(), precision);
+            total += m.last_assemble_ms;
+        }
+        m.assembly_blocks_per_cu = keep;
+        *avg_ms = total / reps;
+        return NEON_OK;
+    });
+}
+
 int neon_stream_probe(neon_context* ctx, int64_t bytes, int32_t reps, double* read_ms, double* copy_ms)
 {
     return guarded(ctx, [&] {

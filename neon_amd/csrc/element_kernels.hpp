@@ -359,6 +359,9 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
         {
             int const a = hi, b = lo;
             double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            // not unrolled: unrolled, the 8 quadrature points' LDS operands are all held in
+            // registers at once and occupancy falls to one wavefront per SIMD
+#pragma unroll 1
             for (int l = 0; l < 8; ++l)
             {
                 double const ga0 = s_g[l][a][0], ga1 = s_g[l][a][1], ga2 = s_g[l][a][2];
@@ -419,6 +422,7 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
         {
             int const a = hi, i = lo;
             double f = 0.0;
+#pragma unroll 1
             for (int l = 0; l < 8; ++l)
             {
                 double const s0 = s_c[l][9], s1 = s_c[l][10], s2 = s_c[l][11], s3 = s_c[l][12], s4 = s_c[l][13],

@@ -872,7 +872,7 @@ void device_mesh::assemble(const double* u_dev, int precision)
         int32_t const off = colour_offsets_[c];
         int32_t const cnt = colour_offsets_[c + 1] - off;
         if (cnt == 0) continue;
-        int const blocks = int(std::min<int64_t>(cnt, int64_t(cus) * 16));
+        int const blocks = int(std::min<int64_t>(cnt, int64_t(cus) * assembly_blocks_per_cu));
         const int32_t* el = elems_.data() + off;
         if (precision == NEON_PRECISION_FP64)
         {

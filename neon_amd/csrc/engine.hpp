@@ -252,6 +252,8 @@ public:
     // last assemble time split (ms) measured with events
     float last_assemble_ms = 0.0f;
     spmv_config spmv;
+    // assembly grid: workgroups (one wavefront each) per compute unit and colour
+    int assembly_blocks_per_cu = 128;
     int32_t max_degree() const { return max_degree_; }
     kernel_profiler* prof = nullptr;
 
