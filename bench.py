@@ -32,11 +32,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_nb_neohooke128_traffic.txt")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_sell_neohooke128_traffic.txt")
 
 
 def measured_traffic(n):
-    """HBM bytes per launch of the FP32 node-block SpMV at the 128^3 workload, from the
+    """HBM bytes per launch of the FP32 sliced node-block SpMV at the 128^3 workload, from the
     calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in the profile file (None
     for other sizes or without the file)."""
     if n != 128 or not os.path.exists(TRAFFIC_PROFILE):
@@ -50,8 +50,9 @@ def measured_traffic(n):
 
 def spmv_bytes(nnz, nnzb, nodes, n, value_bytes):
     """Algorithmic bytes of one node-block SpMV: values once, one int32 block column per 9
-    values, block row pointers, x read once, y written once."""
-    return value_bytes * nnz + 4 * nnzb + 4 * (nodes + 1) + 2 * value_bytes * n
+    values, x read once, y written once (row pointers and the sliced copy's padding, which
+    depend on the storage format, are not counted)."""
+    return value_bytes * nnz + 4 * nnzb + 2 * value_bytes * n
 
 
 def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
@@ -209,7 +210,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.n) if head == "mixed" else None,
-                     "kernel": f"spmv_nb_kernel<{'float' if vb == 4 else 'double'}> (node-block CSR SpMV + p.Ap)",
+                     "kernel": f"spmv_sell_kernel<{'float' if vb == 4 else 'double'}> (sliced node-block SpMV + p.Ap)",
                      "algorithmic_bytes": bytes_spmv, "avg_ms": spmv_avg_ms, "launches": r["spmv_count"]},
     }
     for prec, rr in results.items():

@@ -148,10 +148,13 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
 #define NEON_SPMV_PROBE_DOT 32   /* probe only: the solvers' form with block partials of x . y */
 #define NEON_SPMV_PROBE_DIRTY 64 /* probe only: each product follows a write of 4 vectors of n (timed apart) */
 #define NEON_SPMV_META 128       /* row pointers of 32 rows per load, shared by shuffles (32 lanes, sweep order) */
+#define NEON_SPMV_SLICED 256     /* sliced copy of the values: 64 nodes per wavefront, one per lane */
+#define NEON_SPMV_SLICED_WIDE 512 /* with NEON_SPMV_SLICED: 1024-thread workgroups instead of 256 */
 int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
                          int32_t reps, double* avg_ms);
 /* Shape of the SpMV that the solvers on this mesh's tangent use from now on (lanes, flags as
- * above, workgroups for FP64 and for FP32 values; 0: by size).  The pipelined form applies to
+ * above, workgroups for FP64 and for FP32 values; 0: by size).  Default: NEON_SPMV_SLICED,
+ * workgroups by size.  The pipelined form applies to
  * meshes whose nodes have at most 32 neighbours and falls back to the plain one otherwise.
  * No counterpart in the reference: a tuning knob; results agree to rounding. */
 int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);

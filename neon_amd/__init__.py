@@ -26,6 +26,8 @@ from ._capi import (  # noqa: F401
     SOLVER_CG,
     SPMV_LDS,
     SPMV_META,
+    SPMV_SLICED,
+    SPMV_SLICED_WIDE,
     SPMV_NONTEMPORAL,
     SPMV_PIPELINE,
     SPMV_STREAM,
@@ -270,9 +272,10 @@ class DeviceMesh:
         self.ctx.check(_capi.lib().neon_mesh_enforce_dirichlet(self.ctx.handle, len(fixed), _capi.i32(fixed),
                                                                _capi.f64(b)))
 
-    def spmv_config(self, lanes: int = 32, flags: int = _capi.SPMV_SWEEP, blocks_fp64: int = 1024,
-                    blocks_fp32: int = 2048) -> None:
-        """Shape of the node-block SpMV used by the solvers on this mesh (tuning knob)."""
+    def spmv_config(self, lanes: int = 32, flags: int = _capi.SPMV_SLICED, blocks_fp64: int = 0,
+                    blocks_fp32: int = 0) -> None:
+        """Shape of the node-block SpMV used by the solvers on this mesh (tuning knob; the
+        defaults are the library's: the sliced copy, one wavefront per 64 nodes)."""
         self.ctx.check(_capi.lib().neon_mesh_spmv_config(self.ctx.handle, lanes, flags, blocks_fp64, blocks_fp32))
 
     def adjoint_gradient(self, dJdu: np.ndarray, fixed: np.ndarray, tol: float = 1e-12,

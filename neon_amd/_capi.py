@@ -46,6 +46,8 @@ SPMV_STREAM = 16
 SPMV_PROBE_DOT = 32
 SPMV_PROBE_DIRTY = 64
 SPMV_META = 128
+SPMV_SLICED = 256
+SPMV_SLICED_WIDE = 512
 
 STATUS_NAMES = {
     0: "OK",

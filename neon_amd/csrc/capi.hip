@@ -328,7 +328,8 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        if ((lanes != 16 && lanes != 32) || (flags & ~(31 | NEON_SPMV_META)) || blocks_fp64 < 0 || blocks_fp32 < 0)
+        if ((lanes != 16 && lanes != 32) || (flags & ~(31 | NEON_SPMV_META | NEON_SPMV_SLICED | NEON_SPMV_SLICED_WIDE))
+            || blocks_fp64 < 0 || blocks_fp32 < 0)
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
         if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS | NEON_SPMV_STREAM | NEON_SPMV_META)) && lanes != 32)
             throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs use 32 lanes");
@@ -364,6 +365,8 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
         if (special && m.max_degree() > 32)
             throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs need rows of at most 32 blocks");
         A.blocks_override = blocks;
+        A.sell = false;
+        if (flags & NEON_SPMV_SLICED) m.attach_sliced(A, precision, (flags & NEON_SPMV_SLICED_WIDE) != 0);
         int64_t const n = m.dofs();
         hipStream_t const s = ctx->stream;
         hipEvent_t e0, e1;

@@ -32,6 +32,17 @@ inline int xcd_blocks(int64_t want, int cap)
 // ================================================================ operator
 int linear_operator::spmv_blocks() const
 {
+    if (node_block && sell)
+    {
+        // one wavefront per slice in one pass, or fewer workgroups striding over the slices;
+        // XCD order: a multiple of 8 workgroups
+        int64_t const per = sell_block / 64;
+        int64_t needed = (int64_t(slices) + per - 1) / per;
+        if (nb_sweep) needed = (needed + 7) / 8 * 8;
+        int64_t const want = blocks_override > 0 ? std::min<int64_t>(blocks_override, needed) : needed;
+        if (nb_sweep) return xcd_blocks(want, kMaxPartials);
+        return int(std::max<int64_t>(1, std::min<int64_t>(want, kMaxPartials)));
+    }
     int64_t const rows = node_block ? nodes : n;
     int64_t const rows_per_pass = node_block ? kBlock / nb_lanes : kBlock / lanes_per_row;
     if (blocks_override > 0)
@@ -47,10 +58,30 @@ int linear_operator::spmv_blocks() const
 namespace
 {
 template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
-void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const XT* x, YT* y, const double* b,
-                 double* part, const krylov_state* st, int gate = 0)
+void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const VT* sval, const XT* x, YT* y,
+                 const double* b, double* part, const krylov_state* st, int gate = 0)
 {
     int const blocks = A.spmv_blocks();
+    if (A.node_block && A.sell)
+    {
+        if (!sval) throw status_error(NEON_BAD_ARGUMENT, "sliced operator without sliced values");
+        sell_matrix m{A.nodes, A.slices, A.sptr, A.scol};
+        // a lane sums its node's products in sequence (no tree across lanes), so the sliced
+        // form accumulates in FP64 also for FP32 values and vectors and rounds once at the store
+        if (A.sell_block == 1024)
+        {
+            if (A.nb_sweep)
+                spmv_sell_kernel<1024, true, VT, XT, YT, double, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
+            else
+                spmv_sell_kernel<1024, false, VT, XT, YT, double, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
+        }
+        else if (A.nb_sweep)
+            spmv_sell_kernel<256, true, VT, XT, YT, double, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
+        else
+            spmv_sell_kernel<256, false, VT, XT, YT, double, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
+        NEON_CHECK_HIP(hipGetLastError());
+        return;
+    }
     if (A.node_block)
     {
         nb_matrix m{A.nodes, A.bptr, A.bcol};
@@ -108,27 +139,27 @@ void linear_operator::apply64(hipStream_t s, const double* x, double* y, double*
 {
     if (!v64) throw status_error(NEON_BAD_ARGUMENT, "operator has no FP64 values");
     if (part)
-        launch_spmv<double, double, double, double, true, 0>(*this, s, v64, x, y, nullptr, part, st);
+        launch_spmv<double, double, double, double, true, 0>(*this, s, v64, s64, x, y, nullptr, part, st);
     else
-        launch_spmv<double, double, double, double, false, 0>(*this, s, v64, x, y, nullptr, nullptr, st);
+        launch_spmv<double, double, double, double, false, 0>(*this, s, v64, s64, x, y, nullptr, nullptr, st);
 }
 
 void linear_operator::apply32(hipStream_t s, const float* x, float* y, double* part, const krylov_state* st) const
 {
     if (!v32) throw status_error(NEON_BAD_ARGUMENT, "operator has no FP32 values");
     if (part)
-        launch_spmv<float, float, float, float, true, 0>(*this, s, v32, x, y, nullptr, part, st);
+        launch_spmv<float, float, float, float, true, 0>(*this, s, v32, s32, x, y, nullptr, part, st);
     else
-        launch_spmv<float, float, float, float, false, 0>(*this, s, v32, x, y, nullptr, nullptr, st);
+        launch_spmv<float, float, float, float, false, 0>(*this, s, v32, s32, x, y, nullptr, nullptr, st);
 }
 
 void linear_operator::residual64(hipStream_t s, const double* b, const double* x, double* r, const krylov_state* st,
                                  int gate) const
 {
     if (v64)
-        launch_spmv<double, double, double, double, false, 1>(*this, s, v64, x, r, b, nullptr, st, gate);
+        launch_spmv<double, double, double, double, false, 1>(*this, s, v64, s64, x, r, b, nullptr, st, gate);
     else
-        launch_spmv<float, double, double, double, false, 1>(*this, s, v32, x, r, b, nullptr, st, gate);
+        launch_spmv<float, double, double, double, false, 1>(*this, s, v32, s32, x, r, b, nullptr, st, gate);
 }
 
 // ================================================================ profiler
@@ -758,6 +789,40 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
     NEON_CHECK_HIP(hipMemcpyAsync(elems_.data(), elems.data(), size_t(E_) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(bptr_.data(), bptr_h_.data(), (size_t(N_) + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(bcol_.data(), bcol_h_.data(), size_t(nnzb) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    // sliced layout of the pattern (sell_matrix): 64 consecutive nodes per slice, as many steps
+    // as the slice's largest block count; padded steps point at the node itself (node 0 for
+    // lanes past the last node)
+    {
+        slices_ = int32_t((int64_t(N_) + 63) / 64);
+        std::vector<int32_t> sptr(size_t(slices_) + 1, 0);
+        for (int32_t sl = 0; sl < slices_; ++sl)
+        {
+            int32_t md = 0;
+            for (int64_t nd = 64ll * sl; nd < std::min<int64_t>(64ll * sl + 64, N_); ++nd)
+                md = std::max<int32_t>(md, bptr_h_[nd + 1] - bptr_h_[nd]);
+            if (int64_t(sptr[sl]) + md > int64_t(INT32_MAX) / (9 * 64))
+                throw status_error(NEON_BAD_ARGUMENT, "sliced tangent exceeds the index range");
+            sptr[sl + 1] = sptr[sl] + md;
+        }
+        sell_steps_ = sptr[slices_];
+        std::vector<int32_t> scol(size_t(sell_steps_) * 64);
+#pragma omp parallel for schedule(static)
+        for (int32_t sl = 0; sl < slices_; ++sl)
+            for (int lane = 0; lane < 64; ++lane)
+            {
+                int64_t const nd = 64ll * sl + lane;
+                bool const ok = nd < N_;
+                int32_t const b0 = ok ? bptr_h_[nd] : 0, d = ok ? bptr_h_[nd + 1] - b0 : 0;
+                for (int32_t kk = 0; kk < sptr[sl + 1] - sptr[sl]; ++kk)
+                    scol[size_t(sptr[sl] + kk) * 64 + lane] = kk < d ? bcol_h_[size_t(b0) + kk] : int32_t(ok ? nd : 0);
+            }
+        sptr_.resize(sptr.size());
+        scol_.resize(scol.size());
+        NEON_CHECK_HIP(hipMemcpyAsync(sptr_.data(), sptr.data(), sptr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+        NEON_CHECK_HIP(hipMemcpyAsync(scol_.data(), scol.data(), scol.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+        // the host vectors go out of scope: wait for the copies
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    }
     NEON_CHECK_HIP(hipMemcpyAsync(self_slot_.data(), self.data(), size_t(N_) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(slots_.data(), slots.data(), slots.size(), hipMemcpyHostToDevice, s_));
 
@@ -902,11 +967,13 @@ void device_mesh::assemble(const double* u_dev, int precision)
     (void)hipEventDestroy(e1);
     k64_valid_ = precision == NEON_PRECISION_FP64;
     k32_valid_ = precision != NEON_PRECISION_FP64;
+    s64_fresh_ = s32_fresh_ = false;
 }
 
 void device_mesh::enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision)
 {
     if (n_fixed <= 0) return;
+    s64_fresh_ = s32_fresh_ = false;
     if (precision == NEON_PRECISION_FP64)
         dirichlet_kernel<double><<<grid_for(n_fixed), kBlock, 0, s_>>>(n_fixed, fixed_dev, bptr_.data(), bcol_.data(),
                                                                          k64_.data(), b_dev);
@@ -1036,7 +1103,64 @@ linear_operator device_mesh::op(int precision) const
     A.nb_stream = (spmv.flags & NEON_SPMV_STREAM) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
     A.nb_meta = (spmv.flags & NEON_SPMV_META) != 0 && spmv.lanes == 32;
     A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
+    // nodes of more than 170 neighbours keep the CSR-order values (the sliced copy is packed
+    // through LDS rows of 3 * max degree values)
+    if ((spmv.flags & NEON_SPMV_SLICED) && sliced_fits())
+        attach_sliced(A, precision, (spmv.flags & NEON_SPMV_SLICED_WIDE) != 0);
     return A;
+}
+
+namespace
+{
+// sliced copy of CSR-order values (a slice's 64 rows of 3 * max degree values staged in LDS)
+template <typename VT>
+void pack_sliced(hipStream_t s, const sell_matrix& m, const int32_t* bptr, const VT* val, VT* sval, int32_t max_degree)
+{
+    int32_t const row_len = 3 * std::max<int32_t>(max_degree, 1);
+    size_t const lds = size_t(64) * row_len * sizeof(VT);
+    if (lds > 65536) throw status_error(NEON_BAD_ARGUMENT, "node degree too large for the sliced tangent copy");
+    sell_pack_lds_kernel<VT><<<m.slices, 256, lds, s>>>(m, bptr, val, sval, row_len);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+} // namespace
+
+bool device_mesh::sliced_fits() const
+{
+    return size_t(64) * 3 * std::max<int32_t>(max_degree_, 1) * sizeof(double) <= 65536;
+}
+
+void device_mesh::attach_sliced(linear_operator& A, int precision, bool wide) const
+{
+    if (!sliced_fits()) throw status_error(NEON_BAD_ARGUMENT, "node degree too large for the sliced tangent copy");
+    sell_matrix const m{N_, slices_, sptr_.data(), scol_.data()};
+    size_t const vals = size_t(sell_steps_) * 9 * 64;
+    if (precision == NEON_PRECISION_FP64)
+    {
+        if (!k64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP64 tangent assembled");
+        if (!s64_fresh_)
+        {
+            s64_.resize(vals);
+            pack_sliced<double>(s_, m, bptr_.data(), k64_.data(), s64_.data(), max_degree_);
+            s64_fresh_ = true;
+        }
+        A.s64 = s64_.data();
+    }
+    else
+    {
+        if (!k32_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP32 tangent assembled");
+        if (!s32_fresh_)
+        {
+            s32_.resize(vals);
+            pack_sliced<float>(s_, m, bptr_.data(), k32_.data(), s32_.data(), max_degree_);
+            s32_fresh_ = true;
+        }
+        A.s32 = s32_.data();
+    }
+    A.sell = true;
+    A.sell_block = wide ? 1024 : 256;
+    A.slices = slices_;
+    A.sptr = sptr_.data();
+    A.scol = scol_.data();
 }
 
 namespace

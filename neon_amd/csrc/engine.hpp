@@ -119,6 +119,14 @@ struct linear_operator
     // values
     const double* v64 = nullptr;
     const float* v32 = nullptr;
+    // sliced copy of the node-block values (sell_matrix layout); the SpMV uses it when set
+    bool sell = false;
+    int sell_block = 256; // threads per workgroup (256 or 1024), one wavefront per slice
+    int32_t slices = 0;
+    const int32_t* sptr = nullptr;
+    const int32_t* scol = nullptr;
+    const double* s64 = nullptr;
+    const float* s32 = nullptr;
 
     int spmv_blocks() const;
     // FP64: y = A x (values v64); with part, also block partials of x.y (or w.y if w).
@@ -193,9 +201,9 @@ constexpr int64_t kValuePad = 16;
 struct spmv_config
 {
     int lanes = 32;
-    int flags = NEON_SPMV_SWEEP;
-    int blocks64 = 1024;
-    int blocks32 = 2048;
+    int flags = NEON_SPMV_SLICED;
+    int blocks64 = 0;
+    int blocks32 = 0;
 };
 
 // Device-resident mesh: geometry, constitutive state, pattern, tangent and f_int.
@@ -224,6 +232,11 @@ public:
     void download_qp_state(double* sigma, double* D, double* detF, double* eps_p, double* ebar, double* vm);
     // operator over the assembled tangent (FP64 values, or FP32 values for mixed)
     linear_operator op(int precision) const;
+    // points A's SpMV at the sliced copy of the tangent (packed here when the tangent changed
+    // since the last copy); wide: 1024-thread workgroups instead of 256
+    void attach_sliced(linear_operator& A, int precision, bool wide) const;
+    // the sliced copy is packed through LDS: nodes of at most 170 neighbours
+    bool sliced_fits() const;
     // Jacobi inverse from the node-block diagonal
     void jacobi(double* minv, int eigen_rule, int precision);
     // adjoint: grad (4 per element) = -lambda^T d f_int / d params at displacement u
@@ -270,6 +283,13 @@ private:
     device_array<double> k64_;
     device_array<float> k32_;
     bool k64_valid_ = false, k32_valid_ = false;
+    // sliced layout of the pattern (sell_matrix) and the sliced copies of the tangent
+    int32_t slices_ = 0;
+    int64_t sell_steps_ = 0;
+    device_array<int32_t> sptr_, scol_;
+    mutable device_array<double> s64_;
+    mutable device_array<float> s32_;
+    mutable bool s64_fresh_ = false, s32_fresh_ = false;
     device_array<double> sigma_, coef_, detF_, eps_p_, ebar_, vm_, eps_p_old_, ebar_old_;
     device_array<uint32_t> status_;
     std::vector<std::unique_ptr<device_array<double>>> traj_u_, traj_ep_, traj_eb_;

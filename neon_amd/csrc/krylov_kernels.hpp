@@ -209,6 +209,150 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
     }
 }
 
+// Sliced node-block layout: a copy of the node-block values in which every wavefront load
+// reads 64 consecutive values.  Slice s holds the 64 consecutive nodes 64 s .. 64 s + 63
+// (lane = node) and sptr[s+1] - sptr[s] steps, the largest block count among them.  Step kk of
+// a node is its kk-th block (ascending column node, as in bcol); steps past the node's block
+// count, and lanes past the last node, hold zero values with the node itself (node 0 past the
+// end) as column.  Value (i, j) of step kk of slice s sits at
+//   sval[((sptr[s] + kk) * 9 + 3 i + j) * 64 + lane],  its column node at scol[(sptr[s] + kk) * 64 + lane].
+struct sell_matrix
+{
+    int32_t nodes;
+    int32_t slices;
+    const int32_t* sptr;
+    const int32_t* scol;
+};
+
+// y = A x on the sliced layout, one wavefront per slice, one node (three rows) per lane.  Each
+// lane accumulates its three row sums over its steps in order, so no cross-lane reduction is
+// needed.  With DOT, also writes the block partial of x . y (the stored, rounded y).
+// MODE 0: y = Ax.  MODE 1: y = b - Ax.
+// Slice order: XCD=false, wavefront w of workgroup b takes slices b W + w, stride gridDim W
+// (consecutive slices on different XCDs, so every XCD's L2 fetches all of x).  XCD=true (a
+// multiple of 8 workgroups): the workgroups b with b % 8 = e (one XCD) sweep the e-th eighth
+// of the slices together, so each L2 fetches x only around its own eighth.
+template <int BS, bool XCD, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* __restrict__ val,
+                                                       const XT* __restrict__ x, YT* __restrict__ y,
+                                                       const double* __restrict__ bvec, double* __restrict__ part,
+                                                       const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate && !st->refine))) return;
+    constexpr int W = BS / 64;
+    int const lane = threadIdx.x & 63;
+    int32_t first, stride, last;
+    if (XCD && (gridDim.x & 7) == 0)
+    {
+        int32_t const per = (A.slices + 7) / 8;
+        int32_t const beg = int32_t(blockIdx.x & 7) * per;
+        last = beg + per < A.slices ? beg + per : A.slices;
+        first = beg + int32_t(blockIdx.x >> 3) * W + int32_t(threadIdx.x >> 6);
+        stride = int32_t(gridDim.x >> 3) * W;
+    }
+    else
+    {
+        first = int32_t(blockIdx.x) * W + int32_t(threadIdx.x >> 6);
+        stride = int32_t(gridDim.x) * W;
+        last = A.slices;
+    }
+    double dot = 0.0;
+    for (int32_t s = first; s < last; s += stride)
+    {
+        int64_t const n = int64_t(s) * 64 + lane;
+        bool const ok = n < A.nodes;
+        int64_t const row = 3 * (ok ? n : int64_t(A.nodes) - 1);
+        // the node's own b (residual) and x (dot) entries, requested before its blocks
+        double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+        XT xr0 = XT(0), xr1 = XT(0), xr2 = XT(0);
+        if constexpr (MODE == 1)
+        {
+            b0 = bvec[row];
+            b1 = bvec[row + 1];
+            b2 = bvec[row + 2];
+        }
+        if constexpr (DOT)
+        {
+            xr0 = x[row];
+            xr1 = x[row + 1];
+            xr2 = x[row + 2];
+        }
+        int32_t const k0 = A.sptr[s], ks = A.sptr[s + 1] - k0;
+        AT a0 = AT(0), a1 = AT(0), a2 = AT(0);
+        for (int32_t kk = 0; kk < ks; ++kk)
+        {
+            int64_t const blk = int64_t(k0) + kk;
+            int64_t const m = A.scol[blk * 64 + lane];
+            VT v[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) v[t] = val[(blk * 9 + t) * 64 + lane];
+            AT const x0 = AT(x[3 * m]), x1 = AT(x[3 * m + 1]), x2 = AT(x[3 * m + 2]);
+            a0 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
+            a1 += AT(v[3]) * x0 + AT(v[4]) * x1 + AT(v[5]) * x2;
+            a2 += AT(v[6]) * x0 + AT(v[7]) * x1 + AT(v[8]) * x2;
+        }
+        YT const y0 = MODE == 1 ? YT(b0 - double(a0)) : YT(a0);
+        YT const y1 = MODE == 1 ? YT(b1 - double(a1)) : YT(a1);
+        YT const y2 = MODE == 1 ? YT(b2 - double(a2)) : YT(a2);
+        if (ok)
+        {
+            y[row] = y0;
+            y[row + 1] = y1;
+            y[row + 2] = y2;
+        }
+        if constexpr (DOT)
+            dot += ok ? double(xr0) * double(y0) + double(xr1) * double(y1) + double(xr2) * double(y2) : 0.0;
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<BS>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
+// Sliced copy of node-block values (layout above), with coalesced reads: one 256-thread
+// workgroup per slice; for each dof row i the slice's 64 row segments (3 deg contiguous values
+// each, CSR order val[9 bptr[n] + i (3 deg) + 3 kk + j]) are read into LDS (rows of `row_len`
+// = 3 * max degree values), then written out 64 consecutive values at a time.  Steps past a
+// node's block count get zeros.  Dynamic LDS: 64 * row_len values.
+template <typename VT>
+__global__ __launch_bounds__(256) void sell_pack_lds_kernel(sell_matrix A, const int32_t* __restrict__ bptr,
+                                                            const VT* __restrict__ val, VT* __restrict__ sval,
+                                                            int32_t row_len)
+{
+    extern __shared__ __align__(16) unsigned char pack_smem[];
+    VT* sm = reinterpret_cast<VT*>(pack_smem);
+    __shared__ int32_t sb0[64], sd[64];
+    int32_t const s = blockIdx.x;
+    int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid < 64)
+    {
+        int64_t const n = int64_t(s) * 64 + tid;
+        bool const ok = n < A.nodes;
+        sb0[tid] = ok ? bptr[n] : 0;
+        sd[tid] = ok ? bptr[n + 1] - bptr[n] : 0;
+    }
+    int32_t const k0 = A.sptr[s], ks = A.sptr[s + 1] - k0;
+    __syncthreads();
+    for (int i = 0; i < 3; ++i)
+    {
+        for (int q = w; q < 64; q += 4)
+        {
+            int32_t const d = sd[q];
+            const VT* src = val + 9ll * sb0[q] + int64_t(i) * 3 * d;
+            for (int e = lane; e < 3 * d; e += 64) sm[q * row_len + e] = src[e];
+        }
+        __syncthreads();
+        for (int c = w; c < 3 * ks; c += 4)
+        {
+            int const kk = c / 3, j = c - 3 * (c / 3);
+            VT const v = kk < sd[lane] ? sm[lane * row_len + 3 * kk + j] : VT(0);
+            sval[((int64_t(k0) + kk) * 9 + 3 * i + j) * 64 + lane] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // Node-block SpMV with the row pointers of 32 rows per load (32 lanes per block row, XCD sweep
 // order as above, any row length).  Lane t of a lane group holds bptr of the group's row
 // 32 c + t (and of the row after it) for the current chunk c, the next chunk's being requested

@@ -6,7 +6,7 @@ builds the n^3 block (the bench's mesh), assembles the tangent at u = 0 in the r
 precision, runs `reps` SpMVs with the solver's default shape (`neon_mesh_spmv_bench`) and a
 streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the calibration kernel for the
 FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the mean SpMV time, its
-algorithmic bytes (values, block columns, block row pointers, x read once, y written once)
+algorithmic bytes (values, block columns, x read once, y written once)
 and the achieved fraction of the 8 TB/s HBM peak.  Run it under `rocprofv3 --kernel-trace
 --stats` or `--pmc` to read the same kernels' durations and traffic counters.
 """
@@ -25,8 +25,9 @@ HBM_PEAK_GBS = 8000.0
 
 
 def spmv_bytes(nnz: int, nodes: int, value_bytes: int) -> int:
+    """Values once, one int32 block column per 9 values, x read once, y written once."""
     n = 3 * nodes
-    return value_bytes * nnz + 4 * (nnz // 9) + 4 * (nodes + 1) + 2 * value_bytes * n
+    return value_bytes * nnz + 4 * (nnz // 9) + 2 * value_bytes * n
 
 
 def main(argv=None) -> dict:
@@ -46,8 +47,8 @@ def main(argv=None) -> dict:
     mesh.assemble(prec)
     ms = C.c_double()
     lib = _capi.lib()
-    # the solvers' default shape (spmv_config defaults): 32 lanes, XCD sweep order
-    cfg = dict(lanes=32, flags=_capi.SPMV_SWEEP, blocks=1024 if prec == _capi.PRECISION_FP64 else 2048)
+    # the solvers' default shape (spmv_config defaults): the sliced copy, workgroups by size
+    cfg = dict(lanes=32, flags=_capi.SPMV_SLICED, blocks=0)
     ctx.check(lib.neon_mesh_spmv_bench(ctx.handle, prec, cfg["lanes"], cfg["flags"], cfg["blocks"], args.reps,
                                        C.byref(ms)))
     nnz = mesh.nnz()

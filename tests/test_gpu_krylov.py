@@ -47,11 +47,13 @@ def test_bicgstab_host_csr_parity(ctx, criterion):
     x = np.zeros_like(b)
     s.solve((rp, ci, val), x, b)
     # BiCGSTAB counts follow roundoff closely: the oracle's own count on this system moves
-    # between 89 and 100 (Eigen rule) with its thread count.  The CUDA rule's test value
-    # sqrt(r0 . r) is not a norm and its count moves more (39 to 50 between hosts), so only the
-    # attained residual is compared there.
+    # between 89 and 103 (Eigen rule) with its thread count (the order of its parallel dot
+    # products), so the count is held within 15 % of the oracle's range over several thread
+    # counts.  The CUDA rule's test value sqrt(r0 . r) is not a norm and its count moves more
+    # (39 to 50 between hosts), so only the attained residual is compared there.
     if criterion == neon_amd.STOP_RELATIVE:
-        assert abs(s.iterations - its0) <= max(3, 0.15 * its0), (s.iterations, its0)
+        counts = [its0] + [ref(rp, ci, val, b, tol, 20000, threads=t)[2] for t in (1, 2, 3, 4, 8)]
+        assert 0.85 * min(counts) <= s.iterations <= 1.15 * max(counts), (s.iterations, counts)
     A = sp.csr_matrix((val, ci, rp))
     res, res0 = np.linalg.norm(b - A @ x), np.linalg.norm(b - A @ x0)
     if criterion == neon_amd.STOP_RELATIVE:
@@ -147,12 +149,15 @@ SPMV_SHAPES = [
     (32, 0), (16, 0), (32, neon_amd.SPMV_SWEEP), (32, neon_amd.SPMV_SWEEP | neon_amd.SPMV_NONTEMPORAL),
     (32, neon_amd.SPMV_PIPELINE), (32, neon_amd.SPMV_PIPELINE | neon_amd.SPMV_NONTEMPORAL),
     (32, neon_amd.SPMV_LDS), (32, neon_amd.SPMV_STREAM), (32, neon_amd.SPMV_META),
+    (32, neon_amd.SPMV_SLICED), (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SLICED_WIDE),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP | neon_amd.SPMV_SLICED_WIDE),
 ]
 
 
 @pytest.mark.parametrize("precision", [neon_amd.PRECISION_FP64, neon_amd.PRECISION_MIXED])
 def test_spmv_shapes_agree(ctx, precision):
-    # every SpMV shape (lanes, XCD sweep, nontemporal, pipelined, LDS-staged, streaming) solves the device tangent to
+    # every SpMV shape (lanes, XCD sweep, nontemporal, pipelined, LDS-staged, streaming, sliced) solves the device tangent to
     # the same answer as the plain shape and with the same Krylov count up to rounding;
     # each solve's true residual meets the tolerance
     p = problems.block_rollers(12)
@@ -169,10 +174,12 @@ def test_spmv_shapes_agree(ctx, precision):
     A = sp.csr_matrix((val, ci, rp))
     results = []
     for lanes, flags in SPMV_SHAPES:
-        for blocks in (0, 64):
+        # sliced forms: also fewer workgroups than slices (the grid-stride loops)
+        for blocks in (0, 64, 3, 8) if flags & neon_amd.SPMV_SLICED else (0, 64):
             mg.spmv_config(lanes, flags, blocks, blocks)
             x, its, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
             assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b), (lanes, flags, blocks)
+            print("shape", lanes, flags, blocks, "iterations", its)
             results.append((x, its))
     x0, its0 = results[0]
     # the mixed solver tests convergence every 4 iterations, so its counts move in steps of 4

@@ -252,16 +252,20 @@ def _per_step(recs):
 def _check_newton(u, recs, u0, recs0, u_tol=1e-10, newton_slack=0):
     """Same load steps; Newton iterations per step equal (newton_slack = 1 where the last
     residual of a step lands within roundoff of the tolerance, which also happens between
-    thread counts of the CPU oracle); Krylov counts within 10 or 12 % (the oracle's own
-    counts move by up to 9 % between 1, 2 and 8 threads once the Newton residual is near
-    roundoff); displacement within u_tol relative to max|u|."""
+    thread counts of the CPU oracle); Krylov counts within 10 or 12 % while the Newton
+    residual is above 1e-6 of the step's first, within 25 % below that (the right-hand side
+    is then a few orders above the roundoff of f_int and the count follows that roundoff: the
+    oracle's own count at such an iterate of the 10^3 cube moves between 93 and 113 across
+    runs and thread counts); displacement within u_tol relative to max|u|."""
     a, b = _per_step(recs), _per_step(recs0)
     assert list(a) == list(b)
     for t in a:
         assert abs(len(a[t]) - len(b[t])) <= newton_slack, (t, len(a[t]), len(b[t]))
         assert all(r["status"] == 0 for r in a[t]) and all(r["status"] == 0 for r in b[t])
+        first = b[t][0]["force_norm"]
         for r, r0 in zip(a[t], b[t]):
-            assert abs(r["solver_iterations"] - r0["solver_iterations"]) <= max(10, 0.12 * r0["solver_iterations"]), \
+            rel = 0.12 if r0["force_norm"] > 1e-6 * first else 0.25
+            assert abs(r["solver_iterations"] - r0["solver_iterations"]) <= max(10, rel * r0["solver_iterations"]), \
                 (r, r0)
     err = np.abs(u - u0).max() / np.abs(u0).max()
     assert err < u_tol, err
