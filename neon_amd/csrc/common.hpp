@@ -83,4 +83,22 @@ __device__ inline void xcd_chunk(int64_t total, int64_t& begin, int64_t& end)
     begin = int64_t(logical) * per;
     end = begin + per < total ? begin + per : total;
 }
+// The same split with chunk boundaries at multiples of A items (the last chunk ends at
+// total), so that a chunk of an A-aligned array starts on an A-item boundary.
+template <int A>
+__device__ inline void xcd_chunk_aligned(int64_t total, int64_t& begin, int64_t& end)
+{
+    int const nb = gridDim.x;
+    int const b = blockIdx.x;
+    int logical = b;
+    if ((nb & 7) == 0) logical = (b & 7) * (nb >> 3) + (b >> 3);
+    int64_t const units = (total + A - 1) / A;
+    int64_t const per = (units + nb - 1) / nb;
+    int64_t const b0 = int64_t(logical) * per * A;
+    begin = b0 < total ? b0 : total;
+    end = begin + per * A < total ? begin + per * A : total;
+}
+
+typedef float float4v __attribute__((ext_vector_type(4)));
+typedef double double2v __attribute__((ext_vector_type(2)));
 } // namespace neon_amd

@@ -972,16 +972,34 @@ __global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double*
     double const pAp = reduce_partials<kBlock>(part_pAp, npart);
     double const alpha = st->rz / pAp;
     int64_t beg, end;
-    xcd_chunk(n, beg, end);
+    xcd_chunk_aligned<2>(n, beg, end);
     double rr = 0.0, rz = 0.0;
-    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
-    {
-        x[i] += alpha * p[i];
-        double const ri = r[i] - alpha * Ap[i];
-        r[i] = ri;
-        double const zi = minv[i] * ri;
+    auto body = [&](double& xi, double& ri, double pi, double api, double mi) {
+        xi += alpha * pi;
+        ri = ri - alpha * api;
+        double const zi = mi * ri;
         rr += ri * ri;
         rz += ri * zi;
+    };
+    // two entries per thread in 16-byte accesses, then the chunk's odd tail
+    int64_t const full = beg + (end - beg) / 2 * 2;
+    for (int64_t i = beg + 2 * threadIdx.x; i < full; i += 2 * kBlock)
+    {
+        double2v xv = *reinterpret_cast<const double2v*>(x + i), rv = *reinterpret_cast<const double2v*>(r + i);
+        double2v const pv = *reinterpret_cast<const double2v*>(p + i), av = *reinterpret_cast<const double2v*>(Ap + i),
+                       mv = *reinterpret_cast<const double2v*>(minv + i);
+        double x0 = xv.x, x1 = xv.y, r0 = rv.x, r1 = rv.y;
+        body(x0, r0, pv.x, av.x, mv.x);
+        body(x1, r1, pv.y, av.y, mv.y);
+        *reinterpret_cast<double2v*>(x + i) = double2v{x0, x1};
+        *reinterpret_cast<double2v*>(r + i) = double2v{r0, r1};
+    }
+    for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
+    {
+        double xi = x[i], ri = r[i];
+        body(xi, ri, p[i], Ap[i], minv[i]);
+        x[i] = xi;
+        r[i] = ri;
     }
     rr = block_sum<kBlock>(rr);
     rz = block_sum<kBlock>(rz);
@@ -1034,8 +1052,16 @@ __global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* 
     if (done == KRYLOV_RUNNING)
     {
         int64_t beg, end;
-        xcd_chunk(n, beg, end);
-        for (int64_t i = beg + threadIdx.x; i < end; i += kBlock) p[i] = minv[i] * r[i] + beta * p[i];
+        xcd_chunk_aligned<2>(n, beg, end);
+        int64_t const full = beg + (end - beg) / 2 * 2;
+        for (int64_t i = beg + 2 * threadIdx.x; i < full; i += 2 * kBlock)
+        {
+            double2v const mv = *reinterpret_cast<const double2v*>(minv + i),
+                           rv = *reinterpret_cast<const double2v*>(r + i),
+                           pv = *reinterpret_cast<const double2v*>(p + i);
+            *reinterpret_cast<double2v*>(p + i) = double2v{mv.x * rv.x + beta * pv.x, mv.y * rv.y + beta * pv.y};
+        }
+        for (int64_t i = full + threadIdx.x; i < end; i += kBlock) p[i] = minv[i] * r[i] + beta * p[i];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
@@ -1177,16 +1203,36 @@ __global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, floa
     double const pAp = reduce_partials<kBlock>(part_pAp, npart);
     float const alpha = float(st->rz / pAp);
     int64_t beg, end;
-    xcd_chunk(n, beg, end);
+    xcd_chunk_aligned<4>(n, beg, end);
     double rr = 0.0, rz = 0.0;
-    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
-    {
-        x32[i] += alpha * p32[i];
-        float const ri = r32[i] - alpha * Ap32[i];
-        r32[i] = ri;
-        float const zi = minv[i] * ri;
+    auto body = [&](float& xi, float& ri, float pi, float api, float mi) {
+        xi += alpha * pi;
+        ri = ri - alpha * api;
+        float const zi = mi * ri;
         rr += double(ri) * double(ri);
         rz += double(ri) * double(zi);
+    };
+    // four entries per thread in 16-byte accesses, then the chunk's tail
+    int64_t const full = beg + (end - beg) / 4 * 4;
+    for (int64_t i = beg + 4 * threadIdx.x; i < full; i += 4 * kBlock)
+    {
+        float4v xv = *reinterpret_cast<const float4v*>(x32 + i), rv = *reinterpret_cast<const float4v*>(r32 + i);
+        float4v const pv = *reinterpret_cast<const float4v*>(p32 + i), av = *reinterpret_cast<const float4v*>(Ap32 + i),
+                      mv = *reinterpret_cast<const float4v*>(minv + i);
+        float x0 = xv.x, x1 = xv.y, x2 = xv.z, x3 = xv.w, r0 = rv.x, r1 = rv.y, r2 = rv.z, r3 = rv.w;
+        body(x0, r0, pv.x, av.x, mv.x);
+        body(x1, r1, pv.y, av.y, mv.y);
+        body(x2, r2, pv.z, av.z, mv.z);
+        body(x3, r3, pv.w, av.w, mv.w);
+        *reinterpret_cast<float4v*>(x32 + i) = float4v{x0, x1, x2, x3};
+        *reinterpret_cast<float4v*>(r32 + i) = float4v{r0, r1, r2, r3};
+    }
+    for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
+    {
+        float xi = x32[i], ri = r32[i];
+        body(xi, ri, p32[i], Ap32[i], minv[i]);
+        x32[i] = xi;
+        r32[i] = ri;
     }
     rr = block_sum<kBlock>(rr);
     rz = block_sum<kBlock>(rz);
@@ -1298,8 +1344,17 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
     if (done == KRYLOV_RUNNING)
     {
         int64_t beg, end;
-        xcd_chunk(n, beg, end);
-        for (int64_t i = beg + threadIdx.x; i < end; i += kBlock) p32[i] = minv[i] * r32[i] + beta * p32[i];
+        xcd_chunk_aligned<4>(n, beg, end);
+        int64_t const full = beg + (end - beg) / 4 * 4;
+        for (int64_t i = beg + 4 * threadIdx.x; i < full; i += 4 * kBlock)
+        {
+            float4v const mv = *reinterpret_cast<const float4v*>(minv + i),
+                          rv = *reinterpret_cast<const float4v*>(r32 + i),
+                          pv = *reinterpret_cast<const float4v*>(p32 + i);
+            *reinterpret_cast<float4v*>(p32 + i) = float4v{mv.x * rv.x + beta * pv.x, mv.y * rv.y + beta * pv.y,
+                                                           mv.z * rv.z + beta * pv.z, mv.w * rv.w + beta * pv.w};
+        }
+        for (int64_t i = full + threadIdx.x; i < end; i += kBlock) p32[i] = minv[i] * r32[i] + beta * p32[i];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
