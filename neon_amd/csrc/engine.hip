@@ -66,19 +66,17 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
     {
         if (!sval) throw status_error(NEON_BAD_ARGUMENT, "sliced operator without sliced values");
         sell_matrix m{A.nodes, A.slices, A.sptr, A.scol};
-        // a lane sums its node's products in sequence (no tree across lanes), so the sliced
-        // form accumulates in FP64 also for FP32 values and vectors and rounds once at the store
         if (A.sell_block == 1024)
         {
             if (A.nb_sweep)
-                spmv_sell_kernel<1024, true, VT, XT, YT, double, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
+                spmv_sell_kernel<1024, true, VT, XT, YT, AT, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
             else
-                spmv_sell_kernel<1024, false, VT, XT, YT, double, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
+                spmv_sell_kernel<1024, false, VT, XT, YT, AT, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
         }
         else if (A.nb_sweep)
-            spmv_sell_kernel<256, true, VT, XT, YT, double, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
+            spmv_sell_kernel<256, true, VT, XT, YT, AT, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
         else
-            spmv_sell_kernel<256, false, VT, XT, YT, double, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
+            spmv_sell_kernel<256, false, VT, XT, YT, AT, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
         NEON_CHECK_HIP(hipGetLastError());
         return;
     }

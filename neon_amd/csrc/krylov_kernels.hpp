@@ -263,13 +263,13 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
         bool const ok = n < A.nodes;
         int64_t const row = 3 * (ok ? n : int64_t(A.nodes) - 1);
         // the node's own b (residual) and x (dot) entries, requested before its blocks
-        double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+        double br0 = 0.0, br1 = 0.0, br2 = 0.0;
         XT xr0 = XT(0), xr1 = XT(0), xr2 = XT(0);
         if constexpr (MODE == 1)
         {
-            b0 = bvec[row];
-            b1 = bvec[row + 1];
-            b2 = bvec[row + 2];
+            br0 = bvec[row];
+            br1 = bvec[row + 1];
+            br2 = bvec[row + 2];
         }
         if constexpr (DOT)
         {
@@ -278,7 +278,12 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
             xr2 = x[row + 2];
         }
         int32_t const k0 = A.sptr[s], ks = A.sptr[s + 1] - k0;
-        AT a0 = AT(0), a1 = AT(0), a2 = AT(0);
+        // FP32 accumulation: four accumulator sets take the steps in turn (steps kk, kk+4, ...
+        // in one set) and are added pairwise at the end, so a row's rounding error grows like
+        // that of a sum of ~7 terms instead of ~27 (the CSR-order kernels' lane tree gives
+        // about the same); FP64 accumulation uses the first set only
+        AT a0 = AT(0), a1 = AT(0), a2 = AT(0), b0 = AT(0), b1 = AT(0), b2 = AT(0);
+        AT c0 = AT(0), c1 = AT(0), c2 = AT(0), d0 = AT(0), d1 = AT(0), d2 = AT(0);
         for (int32_t kk = 0; kk < ks; ++kk)
         {
             int64_t const blk = int64_t(k0) + kk;
@@ -287,13 +292,26 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
 #pragma unroll
             for (int t = 0; t < 9; ++t) v[t] = val[(blk * 9 + t) * 64 + lane];
             AT const x0 = AT(x[3 * m]), x1 = AT(x[3 * m + 1]), x2 = AT(x[3 * m + 2]);
-            a0 += AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2;
-            a1 += AT(v[3]) * x0 + AT(v[4]) * x1 + AT(v[5]) * x2;
-            a2 += AT(v[6]) * x0 + AT(v[7]) * x1 + AT(v[8]) * x2;
+            AT const n0 = a0 + (AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2);
+            AT const n1 = a1 + (AT(v[3]) * x0 + AT(v[4]) * x1 + AT(v[5]) * x2);
+            AT const n2 = a2 + (AT(v[6]) * x0 + AT(v[7]) * x1 + AT(v[8]) * x2);
+            if constexpr (sizeof(AT) == 4)
+            {
+                a0 = b0, a1 = b1, a2 = b2;
+                b0 = c0, b1 = c1, b2 = c2;
+                c0 = d0, c1 = d1, c2 = d2;
+                d0 = n0, d1 = n1, d2 = n2;
+            }
+            else
+            {
+                // FP64 accumulation: one set (the others stay zero; fewer registers)
+                a0 = n0, a1 = n1, a2 = n2;
+            }
         }
-        YT const y0 = MODE == 1 ? YT(b0 - double(a0)) : YT(a0);
-        YT const y1 = MODE == 1 ? YT(b1 - double(a1)) : YT(a1);
-        YT const y2 = MODE == 1 ? YT(b2 - double(a2)) : YT(a2);
+        AT const r0 = (a0 + b0) + (c0 + d0), r1 = (a1 + b1) + (c1 + d1), r2 = (a2 + b2) + (c2 + d2);
+        YT const y0 = MODE == 1 ? YT(br0 - double(r0)) : YT(r0);
+        YT const y1 = MODE == 1 ? YT(br1 - double(r1)) : YT(r1);
+        YT const y2 = MODE == 1 ? YT(br2 - double(r2)) : YT(r2);
         if (ok)
         {
             y[row] = y0;
