@@ -60,10 +60,24 @@ bool adaptive_time_step::update(bool converged)
     }
     double const dt = std::max(minimum_, (current_ - last_converged_) * 0.5);
     current_ = last_converged_ + dt;
-    if (current_ > final_ || current_ < 0.0) return false;
+    if (current_ > final_)
+    {
+        failure_ = "minimum increment is not small enough to resolve the time step";
+        return false;
+    }
+    if (current_ < 0.0)
+    {
+        failure_ = "Step has reduced below the final time for the last load case";
+        return false;
+    }
     consecutive_unconverged_++;
     consecutive_converged_ = 0;
-    return consecutive_unconverged_ != increment_limit_;
+    if (consecutive_unconverged_ == increment_limit_)
+    {
+        failure_ = "Too many reductions.  Convergence not judged likely";
+        return false;
+    }
+    return true;
 }
 
 bool boundary_active(const neon_dirichlet& bc, double t) { return t >= bc.times[0] && t <= bc.times[bc.n_points - 1]; }
@@ -298,6 +312,7 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
         if (!ts.update(false))
         {
             result = NEON_FATAL;
+            ctx->error = ts.failure();
             break;
         }
         m.save_internal_variables(false);

@@ -22,8 +22,10 @@ class adaptive_time_step
 public:
     adaptive_time_step(double period, double initial, double minimum, double maximum, bool adaptive,
                        std::vector<double> history);
-    // false where neon throws std::domain_error
+    // false where neon throws std::domain_error (adaptive_time_step.cpp:68-91); failure()
+    // then holds neon's message
     bool update(bool converged);
+    const char* failure() const { return failure_; }
     bool fully_applied() const { return applied_; }
     double step_time() const { return current_; }
     double last_step_time() const { return last_converged_; }
@@ -35,6 +37,7 @@ private:
     int32_t increment_limit_ = 5, successful_ = 0, consecutive_converged_ = 0, consecutive_unconverged_ = 0;
     double final_ = 1, current_ = 1, last_converged_ = 0, last_size_ = 0, minimum_ = 0, maximum_ = 0;
     bool applied_ = false;
+    const char* failure_ = "";
 };
 
 bool boundary_active(const neon_dirichlet& bc, double t);

@@ -356,4 +356,5 @@ def test_newton_cutback_exhaustion_is_fatal(ctx):
     with pytest.raises(neon_amd.NeonError) as e:
         sm.solve()
     assert e.value.code == -1
+    assert "Too many reductions" in str(e.value)
     assert sum(r["status"] == 1 for r in sm.records) == 5
