@@ -210,10 +210,37 @@ typedef struct {
     int64_t refinement_count;  /* FP64 residual recomputations (mixed precision) */
 } neon_newton_timing;
 
-/* u_out: 3N final displacement (host).  records may be NULL. */
+/* u_out: 3N final displacement (host).  records may be NULL.  No nonfollower loads (f_ext = 0). */
 int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int32_t n_bc,
                       const neon_dirichlet* bcs, double* u_out, neon_newton_record* records,
                       int32_t record_capacity, int32_t* n_records, neon_newton_timing* timing);
+
+/* Nonfollower (Neumann) load of the Newton driver (static_matrix.hpp:205-241
+ * compute_external_force, nonfollower_load.cpp:12-122): at a step time t it adds
+ * load(t) * values[i] to f_ext[dofs[i]], where `values` is the load vector of a unit load
+ * (neon_surface_load_vector / neon_volume_load_vector for traction, pressure and body force;
+ * 1 at each dof for a nodal force) and load(t) interpolates (times, loads) piecewise
+ * linearly (boundary.cpp:50-85).  fixed_time >= 0 evaluates the history at that time in
+ * every step instead: neon applies nodal forces at value_view()'s default load factor 1
+ * (static_matrix.hpp:230-236, nodal_value.hpp:29); fixed_time < 0 uses the step time. */
+typedef struct {
+    int32_t n_entries;
+    const int32_t* dofs;   /* global dof = 3 node + component */
+    const double* values;  /* unit-load vector entries */
+    int32_t n_points;
+    const double* times;
+    const double* loads;
+    double fixed_time;
+} neon_neumann;
+
+/* neon_newton_solve with nonfollower loads: the residual is f_ext - f_int and the relative
+ * force norm divides by max(|r_0|, |f_ext|, |f_int|) (static_matrix.hpp:385-400); the load
+ * histories' times join the step-time history (mesh.cpp:146-172).  A step time outside a
+ * load's history is an error (neon interpolates past the end of its table there). */
+int neon_newton_solve_loaded(neon_context* ctx, const neon_newton_options* options, int32_t n_bc,
+                             const neon_dirichlet* bcs, int32_t n_loads, const neon_neumann* loads,
+                             double* u_out, neon_newton_record* records, int32_t record_capacity,
+                             int32_t* n_records, neon_newton_timing* timing);
 
 /* ---- adjoint (no counterpart in the reference) ----
  * Gradient of an objective J(u) of the final displacement with respect to the per-element
@@ -244,6 +271,25 @@ int neon_mesh_box_size(int32_t nx, int32_t ny, int32_t nz, double Lx, double Ly,
                        int32_t* n_nodes, int32_t* n_elem);
 int neon_mesh_box(int32_t nx, int32_t ny, int32_t nz, double Lx, double Ly, double Lz, double jitter,
                   uint64_t seed, double hole_radius, double* X, int32_t* conn, int32_t* node_ijk);
+/* ---- unit nonfollower load vectors (host only, no device use) ----
+ * Load vector of a unit load on the initial configuration X (3N), accumulated into `out`
+ * (3N, dense):
+ *  - neon_surface_load_vector: quadrilateral faces (4 nodes each in neon's quadrilateral4
+ *    order, quadrilateral.cpp:17-46) with 2x2 Gauss points (the centre when reduced,
+ *    quadrilateral_quadrature.cpp:10-24).  component 0..2: traction,
+ *    out[3a+c] += sum_l w N_a j with j = |x_xi x x_eta| (neumann.hpp:104-118);
+ *    component -1: pressure, out[3a+i] -= sum_l w N_a n_i j, n = (x_xi x x_eta) / j
+ *    (pressure.cpp:8-31).
+ *  - neon_volume_load_vector: hex8 elements (hexahedron.cpp:21-28 order), 2x2x2 Gauss points
+ *    (the centre when reduced); body force, out[3a+c] += sum_l w N_a det J
+ *    (neumann.hpp:159-173).
+ * Triangular faces are not supported: neon's one-point triangle rule weighs 1 instead of
+ * 1/2 (triangle_quadrature.cpp:12-17), doubling such loads. */
+int neon_surface_load_vector(int32_t n_nodes, const double* X, int32_t n_faces, const int32_t* faces,
+                             int32_t component, int32_t reduced, double* out);
+int neon_volume_load_vector(int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
+                            int32_t component, int32_t reduced, double* out);
+
 /* Uniform(lo, hi) per index from the counter-based SplitMix64 stream (n values). */
 int neon_random_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
 /* Standard normal per index (Box-Muller on two stream values). */

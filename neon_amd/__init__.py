@@ -316,6 +316,46 @@ def _bc_structs(boundaries):
     return arr, keep
 
 
+def _load_structs(loads):
+    """loads: list of dicts {dofs, values, times, loads[, fixed_time]} -> ctypes array (+ keepalive)."""
+    keep = []
+    arr = (_capi.Neumann * max(1, len(loads)))()
+    for k, ld in enumerate(loads):
+        d = np.ascontiguousarray(ld["dofs"], dtype=np.int32)
+        v = np.ascontiguousarray(ld["values"], dtype=np.float64)
+        t = np.ascontiguousarray(ld["times"], dtype=np.float64)
+        q = np.ascontiguousarray(ld["loads"], dtype=np.float64)
+        if len(d) != len(v) or len(t) != len(q):
+            raise ValueError("load dofs/values or times/loads differ in length")
+        keep += [d, v, t, q]
+        arr[k] = _capi.Neumann(len(d), _capi.i32(d), _capi.f64(v), len(t), _capi.f64(t), _capi.f64(q),
+                               float(ld.get("fixed_time", -1.0)))
+    return arr, keep
+
+
+def surface_load_vector(X: np.ndarray, faces: np.ndarray, component: int, reduced: bool = False) -> np.ndarray:
+    """Unit traction (component 0..2) or unit pressure (component -1) load vector (3N) of
+    quadrilateral faces on the initial configuration (``neon_surface_load_vector``)."""
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 4)
+    out = np.zeros(3 * len(X))
+    code = _capi.lib().neon_surface_load_vector(len(X), _capi.f64(X), len(faces), _capi.i32(faces), int(component),
+                                                int(bool(reduced)), _capi.f64(out))
+    _raise(code, None)
+    return out
+
+
+def volume_load_vector(X: np.ndarray, conn: np.ndarray, component: int, reduced: bool = False) -> np.ndarray:
+    """Unit body-force load vector (3N) of hex8 elements (``neon_volume_load_vector``)."""
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 8)
+    out = np.zeros(3 * len(X))
+    code = _capi.lib().neon_volume_load_vector(len(X), _capi.f64(X), len(conn), _capi.i32(conn), int(component),
+                                               int(bool(reduced)), _capi.f64(out))
+    _raise(code, None)
+    return out
+
+
 class static_matrix:
     """Newton driver (``static_matrix::solve``, src/assembler/mechanics/static_matrix.hpp:145-467)
     on a :class:`DeviceMesh`.  ``simulation`` uses neon's keys: ``time``
@@ -323,10 +363,12 @@ class static_matrix:
     {displacement_tolerance, residual_tolerance, linear_iterations, absolute_tolerance},
     ``linear_solver`` {tolerance, maximum_iterations, backend{precision}}; plus ``criterion``,
     ``norm_mode``, ``max_load_steps``, ``max_total_iterations`` and ``record_trajectory`` (keep the
-    load history on the device for the J2 adjoint) extensions."""
+    load history on the device for the J2 adjoint) extensions.  ``loads``: nonfollower loads
+    (``neon_neumann``), each {dofs, values (unit-load vector), times, loads[, fixed_time]}."""
 
-    def __init__(self, mesh: DeviceMesh, simulation: dict, boundaries: list):
+    def __init__(self, mesh: DeviceMesh, simulation: dict, boundaries: list, loads: list | None = None):
         self.mesh = mesh
+        self.loads = list(loads or [])
         nl = simulation["nonlinear_options"]
         if "displacement_tolerance" not in nl:
             raise ValueError("displacement_tolerance not specified in nonlinear_options")
@@ -367,8 +409,15 @@ class static_matrix:
         nrec = C.c_int32(0)
         tm = _capi.NewtonTiming()
         bcs, keep = _bc_structs(self.boundaries)
-        code = _capi.lib().neon_newton_solve(m.ctx.handle, C.byref(self.options), len(self.boundaries), bcs,
-                                             _capi.f64(u), recs, capacity, C.byref(nrec), C.byref(tm))
+        if self.loads:
+            lds, keep_l = _load_structs(self.loads)
+            code = _capi.lib().neon_newton_solve_loaded(m.ctx.handle, C.byref(self.options), len(self.boundaries),
+                                                        bcs, len(self.loads), lds, _capi.f64(u), recs, capacity,
+                                                        C.byref(nrec), C.byref(tm))
+            del keep_l
+        else:
+            code = _capi.lib().neon_newton_solve(m.ctx.handle, C.byref(self.options), len(self.boundaries), bcs,
+                                                 _capi.f64(u), recs, capacity, C.byref(nrec), C.byref(tm))
         del keep
         self.records = [
             dict(step=r.step, iteration=r.iteration, solver_iterations=r.solver_iterations, status=r.status,

@@ -78,6 +78,18 @@ class Dirichlet(C.Structure):
     ]
 
 
+class Neumann(C.Structure):
+    _fields_ = [
+        ("n_entries", C.c_int32),
+        ("dofs", i32p),
+        ("values", f64p),
+        ("n_points", C.c_int32),
+        ("times", f64p),
+        ("loads", f64p),
+        ("fixed_time", C.c_double),
+    ]
+
+
 class NewtonOptions(C.Structure):
     _fields_ = [
         ("period", C.c_double),
@@ -164,6 +176,13 @@ SIGNATURES = {
         C.c_int,
         [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, i32p, i32p],
     ),
+    "neon_newton_solve_loaded": (
+        C.c_int,
+        [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p, P(NewtonRecord),
+         C.c_int32, i32p, P(NewtonTiming)],
+    ),
+    "neon_surface_load_vector": (C.c_int, [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, C.c_int32, f64p]),
+    "neon_volume_load_vector": (C.c_int, [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, C.c_int32, f64p]),
     "neon_mesh_box": (
         C.c_int,
         [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_double,

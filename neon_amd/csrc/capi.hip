@@ -482,7 +482,23 @@ int neon_newton_solve(neon_context* ctx, const neon_newton_options* options, int
         if (!options || !u_out || n_bc < 0 || (n_bc > 0 && !bcs)) throw status_error(NEON_BAD_ARGUMENT, "bad arguments");
         if (options->precision != NEON_PRECISION_FP64 && options->precision != NEON_PRECISION_MIXED)
             throw status_error(NEON_BAD_ARGUMENT, "unknown precision");
-        return newton_solve(ctx, options, n_bc, bcs, u_out, records, record_capacity, n_records, timing);
+        return newton_solve(ctx, options, n_bc, bcs, 0, nullptr, u_out, records, record_capacity, n_records, timing);
+    });
+}
+
+int neon_newton_solve_loaded(neon_context* ctx, const neon_newton_options* options, int32_t n_bc,
+                             const neon_dirichlet* bcs, int32_t n_loads, const neon_neumann* loads, double* u_out,
+                             neon_newton_record* records, int32_t record_capacity, int32_t* n_records,
+                             neon_newton_timing* timing)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!options || !u_out || n_bc < 0 || (n_bc > 0 && !bcs) || n_loads < 0 || (n_loads > 0 && !loads))
+            throw status_error(NEON_BAD_ARGUMENT, "bad arguments");
+        if (options->precision != NEON_PRECISION_FP64 && options->precision != NEON_PRECISION_MIXED)
+            throw status_error(NEON_BAD_ARGUMENT, "unknown precision");
+        return newton_solve(ctx, options, n_bc, bcs, n_loads, loads, u_out, records, record_capacity, n_records,
+                            timing);
     });
 }
 

@@ -95,8 +95,30 @@ double boundary_value(const neon_dirichlet& bc, double t)
     throw status_error(NEON_FATAL, "step time outside the boundary time history");
 }
 
+double load_value(const neon_neumann& load, double t)
+{
+    // the same interpolation as the Dirichlet histories (boundary.cpp:50-85)
+    neon_dirichlet const h{0, nullptr, load.n_points, load.times, load.loads};
+    double const at = load.fixed_time >= 0.0 ? load.fixed_time : t;
+    for (int32_t i = 0; i < h.n_points; ++i)
+        if (is_approx(h.times[i], at)) return h.values[i];
+    for (int32_t i = 0; i + 1 < h.n_points; ++i)
+        if (h.times[i] < at && at < h.times[i + 1])
+        {
+            double const f = (at - h.times[i]) / (h.times[i + 1] - h.times[i]);
+            return (1.0 - f) * h.values[i] + f * h.values[i + 1];
+        }
+    throw status_error(NEON_FATAL, "step time outside a load's time history");
+}
+
 namespace
 {
+__global__ void sub_kernel(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ y)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) y[i] = a[i] - b[i];
+}
+
 __global__ void scale_copy_kernel(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y)
 {
     int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -137,7 +159,8 @@ solve_result device_solve(neon_context* ctx, int solver, int criterion, int prec
 }
 
 int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, const neon_dirichlet* bcs,
-                 double* u_out, neon_newton_record* records, int32_t cap, int32_t* n_records, neon_newton_timing* timing)
+                 int32_t n_loads, const neon_neumann* loads, double* u_out, neon_newton_record* records, int32_t cap,
+                 int32_t* n_records, neon_newton_timing* timing)
 {
     device_mesh& m = *ctx->mesh;
     krylov_engine& k = *ctx->engine;
@@ -155,7 +178,16 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
             if (bcs[b].dofs[i] < 0 || bcs[b].dofs[i] >= n) throw status_error(NEON_BAD_ARGUMENT, "boundary dof out of range");
         hist.insert(hist.end(), bcs[b].times, bcs[b].times + bcs[b].n_points);
     }
-    // mesh::time_history collects the boundary times into a std::set (mesh.cpp:146-172)
+    for (int32_t b = 0; b < n_loads; ++b)
+    {
+        const neon_neumann& L = loads[b];
+        if (L.n_points < 1 || !L.times || !L.loads || L.n_entries < 0 || (L.n_entries > 0 && (!L.dofs || !L.values)))
+            throw status_error(NEON_BAD_ARGUMENT, "malformed nonfollower load");
+        for (int32_t i = 0; i < L.n_entries; ++i)
+            if (L.dofs[i] < 0 || L.dofs[i] >= n) throw status_error(NEON_BAD_ARGUMENT, "load dof out of range");
+        hist.insert(hist.end(), L.times, L.times + L.n_points);
+    }
+    // mesh::time_history collects the boundary and load times into a std::set (mesh.cpp:146-172)
     std::sort(hist.begin(), hist.end());
     hist.erase(std::unique(hist.begin(), hist.end()), hist.end());
     adaptive_time_step ts(o->period, o->initial, o->minimum, o->maximum, o->adaptive != 0, hist);
@@ -178,6 +210,12 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
     int32_t nrec = 0;
     std::vector<double> inc_h(n);
     std::vector<int32_t> fixed_h;
+    std::vector<double> fext_h;
+    if (n_loads > 0)
+    {
+        fext.resize(n);
+        fext_h.resize(n);
+    }
 
     auto record = [&](int it, int sits, int status, double dn, double fn) {
         if (records && nrec < cap)
@@ -197,9 +235,24 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
         while (err == NEON_OK && !ts.fully_applied())
         {
             if (o->max_load_steps > 0 && steps_done >= o->max_load_steps) break;
-            // perform_equilibrium_iterations (static_matrix.hpp:402-467); f_ext = 0 (no Neumann loads)
+            // compute_external_force then perform_equilibrium_iterations (static_matrix.hpp:145-187,
+            // 205-241, 402-467)
             NEON_CHECK_HIP(hipMemcpyAsync(u, u_old.data(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
             double const step_time = ts.step_time(), last_time = ts.last_step_time();
+            double fe = 0.0; // |f_ext|
+            if (n_loads > 0)
+            {
+                std::fill(fext_h.begin(), fext_h.end(), 0.0);
+                for (int32_t b = 0; b < n_loads; ++b)
+                {
+                    double const v = load_value(loads[b], step_time);
+                    for (int32_t i = 0; i < loads[b].n_entries; ++i) fext_h[loads[b].dofs[i]] += v * loads[b].values[i];
+                }
+                double ss = 0.0;
+                for (double const x : fext_h) ss += x * x;
+                fe = std::sqrt(ss);
+                NEON_CHECK_HIP(hipMemcpyAsync(fext.data(), fext_h.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+            }
             t0 = std::chrono::steady_clock::now();
             err = m.update_internal_variables(u);
             tm.update_ms += ms_since(t0);
@@ -229,7 +282,10 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
                 tm.assemble_ms += ms_since(t0);
                 t0 = std::chrono::steady_clock::now();
                 // minus_residual = f_ext - f_int
-                scale_copy_kernel<<<blocks(n), kBlock, 0, s>>>(n, -1.0, m.f_int(), mres.data());
+                if (n_loads > 0)
+                    sub_kernel<<<blocks(n), kBlock, 0, s>>>(n, fext.data(), m.f_int(), mres.data());
+                else
+                    scale_copy_kernel<<<blocks(n), kBlock, 0, s>>>(n, -1.0, m.f_int(), mres.data());
                 linear_operator const A = m.op(precision);
                 if (it == 0)
                 {
@@ -265,10 +321,9 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
                     record(it, sr.iterations, err, 0, 0);
                     break;
                 }
-                // update_relative_norms (static_matrix.hpp:385-400); f_ext = 0
+                // update_relative_norms (static_matrix.hpp:385-400)
                 double const un = std::sqrt(k.dot(n, u, u));
                 double const fi = std::sqrt(k.dot(n, m.f_int(), m.f_int()));
-                double const fe = 0.0;
                 double const dn = std::sqrt(k.dot(n, delta.data(), delta.data()));
                 double const rn = std::sqrt(k.dot(n, mres.data(), mres.data()));
                 bool const rel = o->norm_mode == 1 ? true

@@ -42,13 +42,15 @@ private:
 
 bool boundary_active(const neon_dirichlet& bc, double t);
 double boundary_value(const neon_dirichlet& bc, double t);
+// a nonfollower load's history value at step time t (fixed_time >= 0: at that time)
+double load_value(const neon_neumann& load, double t);
 
 solve_result device_solve(neon_context* ctx, int solver, int criterion, int precision, double tol, int32_t max_it,
                           const double* b_dev, double* x_dev, double* minv_dev);
 
 int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, const neon_dirichlet* bcs,
-                 double* u_out, neon_newton_record* records, int32_t cap, int32_t* n_records,
-                 neon_newton_timing* timing);
+                 int32_t n_loads, const neon_neumann* loads, double* u_out, neon_newton_record* records,
+                 int32_t cap, int32_t* n_records, neon_newton_timing* timing);
 
 // Adjoint of the recorded J2 load history (device buffers): grad (4 per element) of an
 // objective of the final displacement with gradient dJdu; returns a status, PCG iterations

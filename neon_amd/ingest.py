@@ -1,0 +1,264 @@
+"""Reader of neon's input files for the device path.
+
+A neon simulation is a JSON file (src/simulation_parser.cpp:45-145) naming its parts, their
+materials and the load steps; each part's mesh is ``<part name>.mesh`` beside it, a JSON file
+with ``Nodes[0].Coordinates`` (nodal_coordinates.cpp:13-33) and ``Elements``: groups with a
+``Name``, a gmsh element ``Type`` and a ``NodalConnectivity`` list (basic_mesh.cpp:8-29,
+basic_submesh.cpp:12-52).  The part's volume elements are the groups named like the part;
+boundary groups carry the boundary names.  Hexahedron8 (gmsh 5) and quadrilateral4 (gmsh 3)
+keep gmsh's node order (node_ordering_adapter.cpp:46-136 reorders neither).
+
+``read(path)`` returns a :class:`Problem` for :class:`neon_amd.DeviceMesh` and
+:class:`neon_amd.static_matrix`:
+
+* materials (isotropic_elastic_property.cpp:10-43): bulk and shear moduli, or the elastic
+  modulus and Poisson's ratio (ratio < 0.5); J2 also ``yield_stress`` and the optional
+  ``isotropic_hardening_modulus`` (isotropic_elastic_plastic.cpp:10-36);
+* constitutive models (constitutive_model_factory.cpp:37-160): ``neohooke`` /
+  ``compressible_neohooke`` and ``J2_plasticity`` with ``finite_strain: false``;
+* boundaries (mesh.cpp:85-143, nonfollower_load.cpp:12-122): ``displacement`` per component
+  key x / y / z on the unique nodes of the named groups; ``traction`` (per component) and
+  ``pressure`` (``value``) on quadrilateral faces and ``body_force`` (per component) on hex8
+  groups, as unit-load vectors on the initial configuration times the history; and
+  ``nodal_force`` (per component, 1 at each unique node, applied at its t = 1 value in every
+  step as neon's compute_external_force does);
+* the ``linear_solver`` block: ``iterative`` keeps its tolerance and iteration limit (neon's
+  defaults 1e-5 and 2000, linear_solver.hpp:54-55); the direct solvers (``PaStiX``,
+  ``MUMPS``, ``direct``) have no device counterpart and become CG to a relative 1e-12.
+
+Everything else (multibody steps, other element types, generated sinusoidal histories,
+other constitutive models) raises ``ValueError`` naming what is not supported.
+"""
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+
+GMSH_QUAD4 = 3
+GMSH_HEX8 = 5
+DOF_TABLE = {"x": 0, "y": 1, "z": 2}
+DIRECT_SOLVERS = ("PaStiX", "MUMPS", "direct")
+DIRECT_TOLERANCE = 1.0e-12
+DIRECT_MAX_ITERATIONS = 100000
+
+
+@dataclass
+class Problem:
+    name: str
+    X: np.ndarray          # (N, 3) coordinates
+    conn: np.ndarray       # (E, 8) hex8 connectivity
+    params: np.ndarray     # (E, 4): bulk, shear, yield stress, hardening
+    model: int             # MODEL_NEOHOOKE or MODEL_J2
+    boundaries: list       # Dirichlet: {name, dofs, times, values}
+    loads: list            # nonfollower: {name, dofs, values, times, loads[, fixed_time]}
+    simulation: dict       # static_matrix options
+    groups: dict = field(default_factory=dict)  # name -> list of (gmsh type, connectivity array)
+
+    @property
+    def n_nodes(self) -> int:
+        return len(self.X)
+
+    @property
+    def n_elem(self) -> int:
+        return len(self.conn)
+
+    @property
+    def n_dofs(self) -> int:
+        return 3 * len(self.X)
+
+    def fixed_dofs(self) -> np.ndarray:
+        d = [np.asarray(b["dofs"]) for b in self.boundaries]
+        return np.unique(np.concatenate(d)) if d else np.zeros(0, dtype=np.int32)
+
+    def unique_nodes(self, name: str) -> np.ndarray:
+        return np.unique(np.concatenate([c.reshape(-1) for _, c in self.groups[name]]))
+
+
+def _material_params(material: dict, model: int) -> np.ndarray:
+    """bulk, shear, yield stress, hardening of one material (isotropic_elastic_property.cpp:10-43,
+    isotropic_elastic_plastic.cpp:10-36)."""
+    if "bulk_modulus" in material and "shear_modulus" in material:
+        K, G = float(material["bulk_modulus"]), float(material["shear_modulus"])
+    elif "elastic_modulus" in material and "poissons_ratio" in material:
+        E, nu = float(material["elastic_modulus"]), float(material["poissons_ratio"])
+        if nu >= 0.5:
+            raise ValueError('"poissons_ratio" must be less than or equal to 0.5')
+        K, G = E / (3.0 * (1.0 - 2.0 * nu)), E / (2.0 * (1.0 + nu))
+    else:
+        raise ValueError('"elastic_modulus" and "poissons_ratio" or "bulk_modulus" and "shear_modulus" need to be '
+                         "specified as material properties")
+    sy = H = 0.0
+    if model == _capi.MODEL_J2:
+        if "yield_stress" not in material:
+            raise ValueError('"yield_stress" was not specified in material properties')
+        sy = float(material["yield_stress"])
+        H = float(material.get("isotropic_hardening_modulus", 0.0))
+    return np.array([K, G, sy, H])
+
+
+def _model(constitutive: dict) -> int:
+    """constitutive_model_factory.cpp:37-160, the models of the device path."""
+    if "name" not in constitutive:
+        raise ValueError('"constitutive" needs a "name"')
+    name = constitutive["name"]
+    if name in ("neohooke", "compressible_neohooke"):
+        return _capi.MODEL_NEOHOOKE
+    if name == "J2_plasticity":
+        if "finite_strain" not in constitutive:
+            raise ValueError('"small_strain_J2_plasticity" must have a boolean value for "finite_strain"')
+        if constitutive["finite_strain"] or "damage" in constitutive:
+            raise ValueError("only the small-strain J2 model without damage runs on the device path")
+        return _capi.MODEL_J2
+    raise ValueError(f'constitutive model "{name}" does not run on the device path')
+
+
+def _read_mesh(path: str):
+    with open(path) as fh:
+        mesh = json.load(fh)
+    if not mesh.get("Nodes"):
+        raise ValueError('The mesh file is missing the "Nodes" field')
+    if "Elements" not in mesh:
+        raise ValueError('The mesh file is missing the "Elements" field')
+    X = np.asarray(mesh["Nodes"][0]["Coordinates"], dtype=np.float64).reshape(-1, 3)
+    groups: dict = {}
+    for g in mesh["Elements"]:
+        for key in ("Name", "Type", "NodalConnectivity"):
+            if key not in g:
+                raise ValueError(f'The element group in the mesh file is missing the "{key}" field')
+        conn = np.asarray(g["NodalConnectivity"], dtype=np.int64)
+        if conn.size == 0:
+            raise ValueError("The element group in the mesh file is empty")
+        if conn.min() < 0 or conn.max() >= len(X):
+            raise ValueError(f'element group "{g["Name"]}" refers to nodes outside the mesh')
+        groups.setdefault(g["Name"], []).append((int(g["Type"]), conn.astype(np.int32)))
+    return X, groups
+
+
+def _history(boundary: dict, key: str):
+    t = np.asarray(boundary["time"], dtype=np.float64)
+    v = np.asarray(boundary[key], dtype=np.float64)
+    if len(t) != len(v) or len(t) == 0:
+        raise ValueError(f'boundary "{boundary["name"]}": "time" and "{key}" differ in length')
+    return t, v
+
+
+def _faces(groups: dict, name: str) -> np.ndarray:
+    out = []
+    for gtype, conn in groups[name]:
+        if gtype != GMSH_QUAD4:
+            raise ValueError(f'boundary "{name}": surface loads need quadrilateral4 faces (gmsh type 3), got {gtype}')
+        out.append(conn)
+    return np.concatenate(out)
+
+
+def _sparse(vec: np.ndarray):
+    nz = np.nonzero(vec)[0]
+    return nz.astype(np.int32), vec[nz]
+
+
+def read(path: str, step: int = 0) -> Problem:
+    """Problem of load step `step` of the neon simulation file `path`."""
+    from . import surface_load_vector, volume_load_vector
+
+    with open(path) as fh:
+        root = json.load(fh)
+    for key in ("materials", "parts", "steps"):
+        if key not in root:
+            raise ValueError(f'"{key}" field does not exist in input file.')
+    if len(root["parts"]) != 1:
+        raise ValueError("the device path runs one part")
+    part = root["parts"][0]
+    materials = {m["name"]: m for m in root["materials"]}
+    if part["material"] not in materials:
+        raise ValueError(f'material "{part["material"]}" of part "{part["name"]}" is not defined')
+    sim = root["steps"][step]
+    for key in ("name", "time", "solution"):
+        if key not in sim:
+            raise ValueError(f'A simulation case needs a "{key}" field')
+    if len(sim["meshes"]) != 1:
+        raise ValueError("Multibody simulation are not yet supported")
+    mesh_data = sim["meshes"][0]
+    if mesh_data["name"] != part["name"]:
+        raise ValueError(f'Mesh name "{mesh_data["name"]}" was not found in the mesh store')
+    model = _model(mesh_data.get("constitutive", {}))
+    reduced = mesh_data.get("element_options", {}).get("quadrature", "full") == "reduced"
+    if reduced:
+        raise ValueError("the device path integrates hex8 elements with the full 8-point rule")
+
+    X, groups = _read_mesh(os.path.join(os.path.dirname(os.path.abspath(path)), part["name"] + ".mesh"))
+    if part["name"] not in groups:
+        raise ValueError(f'Mesh name {part["name"]} does not exist in the mesh store')
+    vols = groups[part["name"]]
+    for gtype, _ in vols:
+        if gtype != GMSH_HEX8:
+            raise ValueError(f"the device path runs hexahedron8 elements (gmsh type 5), got type {gtype}")
+    conn = np.concatenate([c for _, c in vols]).astype(np.int32)
+    params = np.tile(_material_params(materials[part["material"]], model), (len(conn), 1))
+
+    boundaries, loads = [], []
+    for b in mesh_data.get("boundaries", []):
+        for key in ("name", "type"):
+            if key not in b:
+                raise ValueError(f'"{key}" was not specified in "boundary".')
+        if "time" not in b:
+            raise ValueError('generated ("generate_type") boundary histories are not supported; give "time"')
+        name, btype = b["name"], b["type"]
+        if name not in groups:
+            raise ValueError(f"Mesh name {name} does not exist in the mesh store")
+        if btype == "displacement":
+            nodes = np.unique(np.concatenate([c.reshape(-1) for _, c in groups[name]]))
+            for key, off in DOF_TABLE.items():
+                if key in b:
+                    t, v = _history(b, key)
+                    boundaries.append({"name": name, "dofs": (3 * nodes + off).astype(np.int32), "times": t,
+                                       "values": v})
+        elif btype == "traction":
+            faces = _faces(groups, name)
+            for key, off in DOF_TABLE.items():
+                if key in b:
+                    t, v = _history(b, key)
+                    d, vals = _sparse(surface_load_vector(X, faces, off, reduced))
+                    loads.append({"name": name, "dofs": d, "values": vals, "times": t, "loads": v})
+        elif btype == "pressure":
+            if "value" not in b:
+                raise ValueError('Pressure boundary condition must specify a "value" array')
+            t, v = _history(b, "value")
+            d, vals = _sparse(surface_load_vector(X, _faces(groups, name), -1, reduced))
+            loads.append({"name": name, "dofs": d, "values": vals, "times": t, "loads": v})
+        elif btype == "body_force":
+            for gtype, _ in groups[name]:
+                if gtype != GMSH_HEX8:
+                    raise ValueError(f'boundary "{name}": body forces need hexahedron8 groups (gmsh type 5)')
+            econn = np.concatenate([c for _, c in groups[name]])
+            for key, off in DOF_TABLE.items():
+                if key in b:
+                    t, v = _history(b, key)
+                    d, vals = _sparse(volume_load_vector(X, econn, off, reduced))
+                    loads.append({"name": name, "dofs": d, "values": vals, "times": t, "loads": v})
+        elif btype == "nodal_force":
+            nodes = np.unique(np.concatenate([c.reshape(-1) for _, c in groups[name]]))
+            for key, off in DOF_TABLE.items():
+                if key in b:
+                    t, v = _history(b, key)
+                    loads.append({"name": name, "dofs": (3 * nodes + off).astype(np.int32),
+                                  "values": np.ones(len(nodes)), "times": t, "loads": v, "fixed_time": 1.0})
+        else:
+            raise ValueError(f'boundary "{btype}" is not recognised')
+
+    ls = dict(sim.get("linear_solver", {}))
+    if "type" not in ls:
+        raise ValueError('A "linear_solver" "type" must be specified')
+    if ls["type"] in DIRECT_SOLVERS:
+        solver = {"tolerance": DIRECT_TOLERANCE, "maximum_iterations": DIRECT_MAX_ITERATIONS}
+    elif ls["type"] == "iterative":
+        solver = {"tolerance": ls.get("tolerance", 1.0e-5), "maximum_iterations": ls.get("maximum_iterations", 2000)}
+    else:
+        raise ValueError(f'Linear solver {ls["type"]} is not recognised.  Please use "PaStiX", "MUMPS", '
+                         '"direct" or "iterative"')
+    simulation = {"time": sim["time"], "nonlinear_options": sim.get("nonlinear_options", {}), "linear_solver": solver}
+    return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, simulation, groups)

@@ -1302,11 +1302,26 @@ double orc_boundary_value(int32_t n_points, const double* times, const double* v
     return bc_value(bc, t);
 }
 
-// static_matrix::solve / perform_equilibrium_iterations (static_matrix.hpp:145-187, 402-467)
 int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, const orc_dirichlet* bcs,
                      double* u_out, orc_newton_record* records, int32_t cap, int32_t* n_records)
 {
-    if (!m || !opt || (n_bc > 0 && !bcs) || !u_out) return ORC_BAD_ARGUMENT;
+    return orc_newton_solve_loaded(m, opt, n_bc, bcs, 0, nullptr, u_out, records, cap, n_records);
+}
+
+// static_matrix::solve / perform_equilibrium_iterations (static_matrix.hpp:145-187, 402-467)
+int orc_newton_solve_loaded(orc_model* m, const orc_newton_options* opt, int32_t n_bc, const orc_dirichlet* bcs,
+                            int32_t n_loads, const orc_neumann* loads, double* u_out, orc_newton_record* records,
+                            int32_t cap, int32_t* n_records)
+{
+    if (!m || !opt || (n_bc > 0 && !bcs) || !u_out || n_loads < 0 || (n_loads > 0 && !loads)) return ORC_BAD_ARGUMENT;
+    for (int32_t k = 0; k < n_loads; ++k)
+    {
+        if (loads[k].n_points < 1 || !loads[k].times || !loads[k].loads || loads[k].n_entries < 0
+            || (loads[k].n_entries > 0 && (!loads[k].dofs || !loads[k].values)))
+            return ORC_BAD_ARGUMENT;
+        for (int32_t i = 0; i < loads[k].n_entries; ++i)
+            if (loads[k].dofs[i] < 0 || loads[k].dofs[i] >= 3 * m->N) return ORC_BAD_ARGUMENT;
+    }
     int32_t const threads = opt->threads;
     if (threads > 0) omp_set_num_threads(threads);
     int32_t const n = 3 * m->N;
@@ -1315,7 +1330,9 @@ int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, 
     vector<double> hist;
     for (int32_t k = 0; k < n_bc; ++k)
         for (int32_t i = 0; i < bcs[k].n_points; ++i) hist.push_back(bcs[k].times[i]);
-    // mesh::time_history collects the boundary times into a std::set (mesh.cpp:146-172)
+    for (int32_t k = 0; k < n_loads; ++k)
+        for (int32_t i = 0; i < loads[k].n_points; ++i) hist.push_back(loads[k].times[i]);
+    // mesh::time_history collects the boundary and load times into a std::set (mesh.cpp:146-172)
     std::sort(hist.begin(), hist.end());
     hist.erase(std::unique(hist.begin(), hist.end()), hist.end());
     orc_time* ts = nullptr;
@@ -1361,12 +1378,30 @@ int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, 
             int32_t applied;
             orc_time_query(ts, nullptr, nullptr, &applied, nullptr);
             if (applied) break;
-            std::fill(f_ext.begin(), f_ext.end(), 0.0); // compute_external_force: no Neumann loads
-
             // perform_equilibrium_iterations
             u = u_old;
             double step_time, last_time;
             orc_time_query(ts, &step_time, &last_time, nullptr, nullptr);
+            // compute_external_force (static_matrix.hpp:205-241) at the step time
+            std::fill(f_ext.begin(), f_ext.end(), 0.0);
+            bool load_outside = false;
+            for (int32_t k = 0; k < n_loads; ++k)
+            {
+                double const at = loads[k].fixed_time >= 0.0 ? loads[k].fixed_time : step_time;
+                orc_dirichlet const h{0, nullptr, loads[k].n_points, loads[k].times, loads[k].loads};
+                if (!(at >= h.times[0] && at <= h.times[h.n_points - 1]))
+                {
+                    load_outside = true;
+                    break;
+                }
+                double const v = bc_value(h, at);
+                for (int32_t i = 0; i < loads[k].n_entries; ++i) f_ext[loads[k].dofs[i]] += v * loads[k].values[i];
+            }
+            if (load_outside)
+            {
+                err = ORC_FATAL;
+                break;
+            }
             err = orc_update_internal_variables(m, u.data(), step_time - last_time);
             if (err != ORC_OK) break;
             int it = 0;

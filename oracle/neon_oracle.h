@@ -184,6 +184,25 @@ int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, 
                      double* u /* in: ignored (zero start); out: final displacement */,
                      orc_newton_record* records, int32_t record_capacity, int32_t* n_records);
 
+/* Nonfollower load: at step time t, f_ext[dofs[i]] += load(t) values[i], load(t) the
+ * piecewise-linear history (boundary.cpp:50-85) at t, or at fixed_time when >= 0 (neon's
+ * nodal forces use value_view()'s default load factor 1, static_matrix.hpp:230-236). */
+typedef struct {
+    int32_t n_entries;
+    const int32_t* dofs;
+    const double* values;
+    int32_t n_points;
+    const double* times;
+    const double* loads;
+    double fixed_time;
+} orc_neumann;
+
+/* orc_newton_solve with nonfollower loads (compute_external_force, static_matrix.hpp:205-241;
+ * the load times join the step-time history, mesh.cpp:146-172). */
+int orc_newton_solve_loaded(orc_model* m, const orc_newton_options* opt, int32_t n_bc, const orc_dirichlet* bcs,
+                            int32_t n_loads, const orc_neumann* loads, double* u, orc_newton_record* records,
+                            int32_t record_capacity, int32_t* n_records);
+
 /* Adjoint of the whole load history of a J2 model in consistent mode (no counterpart in the
  * reference), after orc_newton_solve with record_trajectory: objective J(u_N) with gradient
  * dJdu (3N) at the final converged displacement, constrained dofs `fixed`; grad (4 per

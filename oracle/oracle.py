@@ -34,6 +34,11 @@ class Dirichlet(C.Structure):
                 ("values", f64p)]
 
 
+class Neumann(C.Structure):
+    _fields_ = [("n_entries", C.c_int32), ("dofs", i32p), ("values", f64p), ("n_points", C.c_int32),
+                ("times", f64p), ("loads", f64p), ("fixed_time", C.c_double)]
+
+
 class NewtonOptions(C.Structure):
     _fields_ = [("time", TimeOptions), ("residual_tolerance", C.c_double), ("displacement_tolerance", C.c_double),
                 ("max_newton_iterations", C.c_int32), ("absolute_norm", C.c_int32), ("norm_mode", C.c_int32),
@@ -72,6 +77,8 @@ _SIG = {
     "orc_time_query": [C.c_void_p, f64p, f64p, i32p, i32p],
     "orc_newton_solve": [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), f64p, P(NewtonRecord), C.c_int32,
                          i32p],
+    "orc_newton_solve_loaded": [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p,
+                                P(NewtonRecord), C.c_int32, i32p],
 }
 for _k in ("orc_cg_eigen", "orc_cg_cuda", "orc_bicgstab_eigen", "orc_bicgstab_cuda"):
     _SIG[_k] = [C.c_int32, i32p, i32p, f64p, f64p, f64p, C.c_double, C.c_int32, C.c_int32, i32p, f64p]
@@ -210,7 +217,8 @@ class Model:
         lib().orc_assemble(self.h, _i(rp), _i(ci), _f(val), _f(f), threads)
         return val, f
 
-    def newton(self, options: NewtonOptions, boundaries, capacity=4096):
+    def newton(self, options: NewtonOptions, boundaries, capacity=4096, loads=None):
+        """Newton driver; ``loads``: nonfollower loads {dofs, values, times, loads[, fixed_time]}."""
         arr = (Dirichlet * max(1, len(boundaries)))()
         keep = []
         for k, bc in enumerate(boundaries):
@@ -219,15 +227,78 @@ class Model:
             v = np.ascontiguousarray(bc["values"], dtype=np.float64)
             keep += [d, t, v]
             arr[k] = Dirichlet(len(d), _i(d), len(t), _f(t), _f(v))
+        loads = list(loads or [])
+        larr = (Neumann * max(1, len(loads)))()
+        for k, ld in enumerate(loads):
+            d = np.ascontiguousarray(ld["dofs"], dtype=np.int32)
+            v = np.ascontiguousarray(ld["values"], dtype=np.float64)
+            t = np.ascontiguousarray(ld["times"], dtype=np.float64)
+            q = np.ascontiguousarray(ld["loads"], dtype=np.float64)
+            keep += [d, v, t, q]
+            larr[k] = Neumann(len(d), _i(d), _f(v), len(t), _f(t), _f(q), float(ld.get("fixed_time", -1.0)))
         u = np.zeros(self.n_dofs)
         recs = (NewtonRecord * capacity)()
         n = C.c_int32(0)
-        code = lib().orc_newton_solve(self.h, C.byref(options), len(boundaries), arr, _f(u), recs, capacity,
-                                      C.byref(n))
+        code = lib().orc_newton_solve_loaded(self.h, C.byref(options), len(boundaries), arr, len(loads), larr, _f(u),
+                                             recs, capacity, C.byref(n))
         records = [dict(step=r.step, iteration=r.iteration, solver_iterations=r.solver_iterations, status=r.status,
                         step_time=r.step_time, displacement_norm=r.displacement_norm, force_norm=r.force_norm)
                    for r in recs[: min(n.value, capacity)]]
         return code, u, records
+
+
+# ------------------------------------------------------------------ nonfollower load vectors
+_GAUSS = 1.0 / np.sqrt(3.0)
+_QUAD_NODES = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]])  # quadrilateral.cpp:21-22
+_HEX_NODES = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                       [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)  # hexahedron.cpp:21-28
+
+
+def surface_load_vector(X, faces, component, reduced=False):
+    """Unit-load vector (3N) of quadrilateral4 faces on the initial configuration X: traction in
+    `component` (neumann.hpp:104-118: sum_l w N_a |x_xi x x_eta|) or, for component -1, a unit
+    pressure (pressure.cpp:8-31: -sum_l w N_a n |x_xi x x_eta|).  2x2 Gauss points of weight 1,
+    or the centre with weight 4 when reduced (quadrilateral_quadrature.cpp:10-24)."""
+    X = np.asarray(X, dtype=np.float64).reshape(-1, 3)
+    faces = np.asarray(faces, dtype=np.int64).reshape(-1, 4)
+    pts = [(0.0, 0.0, 4.0)] if reduced else [(s * _GAUSS, t * _GAUSS, 1.0) for s, t in _QUAD_NODES]
+    out = np.zeros(3 * len(X))
+    xf = X[faces]  # (F, 4, 3)
+    for xi, eta, w in pts:
+        N = 0.25 * (1 + _QUAD_NODES[:, 0] * xi) * (1 + _QUAD_NODES[:, 1] * eta)
+        dxi = 0.25 * _QUAD_NODES[:, 0] * (1 + _QUAD_NODES[:, 1] * eta)
+        deta = 0.25 * _QUAD_NODES[:, 1] * (1 + _QUAD_NODES[:, 0] * xi)
+        t1 = np.einsum("fai,a->fi", xf, dxi)
+        t2 = np.einsum("fai,a->fi", xf, deta)
+        c = np.cross(t1, t2)
+        j = np.linalg.norm(c, axis=1)
+        for a in range(4):
+            if component >= 0:
+                np.add.at(out, 3 * faces[:, a] + component, w * N[a] * j)
+            else:
+                for i in range(3):
+                    np.add.at(out, 3 * faces[:, a] + i, -w * N[a] * c[:, i])
+    return out
+
+
+def volume_load_vector(X, conn, component, reduced=False):
+    """Unit body-force vector (3N) of hex8 elements (neumann.hpp:159-173: sum_l w N_a det J),
+    2x2x2 Gauss points or the centre with weight 8 when reduced."""
+    X = np.asarray(X, dtype=np.float64).reshape(-1, 3)
+    conn = np.asarray(conn, dtype=np.int64).reshape(-1, 8)
+    pts = [(0.0, 0.0, 0.0, 8.0)] if reduced else [(p[0] * _GAUSS, p[1] * _GAUSS, p[2] * _GAUSS, 1.0) for p in _HEX_NODES]
+    out = np.zeros(3 * len(X))
+    xe = X[conn]  # (E, 8, 3)
+    for xi, eta, zeta, w in pts:
+        f = 1 + _HEX_NODES * np.array([xi, eta, zeta])  # (8, 3)
+        N = 0.125 * f[:, 0] * f[:, 1] * f[:, 2]
+        dN = 0.125 * np.stack([_HEX_NODES[:, 0] * f[:, 1] * f[:, 2], _HEX_NODES[:, 1] * f[:, 0] * f[:, 2],
+                               _HEX_NODES[:, 2] * f[:, 0] * f[:, 1]], axis=1)  # (8, 3)
+        J = np.einsum("eai,ak->eik", xe, dN)
+        det = np.linalg.det(J)
+        for a in range(8):
+            np.add.at(out, 3 * conn[:, a] + component, w * N[a] * det)
+    return out
 
 
 def boundary_value(times, values, t):
