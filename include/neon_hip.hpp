@@ -284,6 +284,43 @@ struct dirichlet_boundary
     std::vector<double> times, values;
 };
 
+/// A nonfollower load (neon's traction, pressure, body_force or nodal_force boundary,
+/// nonfollower_load.cpp:12-122): f_ext += load(t) * values at dofs, where values is the load
+/// vector of a unit load (surface_load_vector / volume_load_vector, or 1 per dof for a nodal
+/// force) and load(t) interpolates (times, loads).  fixed_time >= 0 evaluates the history at
+/// that time in every step (neon applies nodal forces at t = 1, static_matrix.hpp:230-236).
+struct nonfollower_load
+{
+    std::vector<std::int32_t> dofs;
+    std::vector<double> values, times, loads;
+    double fixed_time = -1.0;
+};
+
+/// Unit-load vector (3N, dense) of quadrilateral faces on the initial configuration X:
+/// traction in component 0..2, or unit pressure for component -1 (neon_surface_load_vector).
+inline std::vector<double> surface_load_vector(std::vector<double> const& X, std::vector<std::int32_t> const& faces,
+                                               std::int32_t component, bool reduced = false)
+{
+    std::vector<double> out(X.size(), 0.0);
+    if (neon_surface_load_vector(std::int32_t(X.size() / 3), X.data(), std::int32_t(faces.size() / 4), faces.data(),
+                                 component, reduced ? 1 : 0, out.data())
+        != NEON_OK)
+        throw std::domain_error("invalid surface load (node index or component out of range)");
+    return out;
+}
+
+/// Unit body-force vector (3N) of hex8 elements (neon_volume_load_vector).
+inline std::vector<double> volume_load_vector(std::vector<double> const& X, std::vector<std::int32_t> const& conn,
+                                              std::int32_t component, bool reduced = false)
+{
+    std::vector<double> out(X.size(), 0.0);
+    if (neon_volume_load_vector(std::int32_t(X.size() / 3), X.data(), std::int32_t(conn.size() / 8), conn.data(),
+                                component, reduced ? 1 : 0, out.data())
+        != NEON_OK)
+        throw std::domain_error("invalid body force (node index or component out of range)");
+    return out;
+}
+
 /// The "time", "nonlinear_options" and "linear_solver" entries of a neon simulation.
 struct simulation_options
 {
@@ -305,8 +342,9 @@ struct simulation_options
 class static_matrix
 {
 public:
-    static_matrix(device_mesh& m, simulation_options const& o, std::vector<dirichlet_boundary> b)
-        : mesh_(m), options_(o), boundaries_(std::move(b))
+    static_matrix(device_mesh& m, simulation_options const& o, std::vector<dirichlet_boundary> b,
+                  std::vector<nonfollower_load> loads = {})
+        : mesh_(m), options_(o), boundaries_(std::move(b)), loads_(std::move(loads))
     {
     }
 
@@ -339,11 +377,20 @@ public:
         for (auto const& b : boundaries_)
             bcs.push_back(neon_dirichlet{std::int32_t(b.dofs.size()), b.dofs.data(), std::int32_t(b.times.size()),
                                          b.times.data(), b.values.data()});
+        std::vector<neon_neumann> lds;
+        for (auto const& f : loads_)
+        {
+            if (f.dofs.size() != f.values.size() || f.times.size() != f.loads.size())
+                throw std::domain_error("nonfollower load: dofs/values or times/loads differ in length");
+            lds.push_back(neon_neumann{std::int32_t(f.dofs.size()), f.dofs.data(), f.values.data(),
+                                       std::int32_t(f.times.size()), f.times.data(), f.loads.data(), f.fixed_time});
+        }
         std::vector<double> u(mesh_.dofs());
         records_.assign(record_capacity, neon_newton_record{});
         std::int32_t n = 0;
-        int const code = neon_newton_solve(mesh_.ctx().get(), &o, std::int32_t(bcs.size()), bcs.data(), u.data(),
-                                           records_.data(), record_capacity, &n, &timing_);
+        int const code = neon_newton_solve_loaded(mesh_.ctx().get(), &o, std::int32_t(bcs.size()), bcs.data(),
+                                                  std::int32_t(lds.size()), lds.data(), u.data(), records_.data(),
+                                                  record_capacity, &n, &timing_);
         records_.resize(std::min(n, record_capacity));
         check(mesh_.ctx().get(), code == NEON_OK || code < 0 ? code : NEON_FATAL);
         return u;
@@ -357,6 +404,7 @@ private:
     device_mesh& mesh_;
     simulation_options options_;
     std::vector<dirichlet_boundary> boundaries_;
+    std::vector<nonfollower_load> loads_;
     std::vector<neon_newton_record> records_;
     neon_newton_timing timing_{};
 };

@@ -162,11 +162,137 @@ void patch_test(neon_hip::context& ctx)
     expect(err <= 1e-9 * umax, "uniaxial neo-Hooke patch test matches the closed form");
     expect(sm.records().size() >= 2, "two load steps recorded");
 }
+// unit-load vectors of the box's top face (area 1) and volume (1): host-only calls
+void load_vectors()
+{
+    int const n = 4;
+    std::int32_t nn = 0, ne = 0;
+    neon_mesh_box_size(n, n, n, 1.0, 1.0, 1.0, 0.0, &nn, &ne);
+    std::vector<double> X(3 * std::size_t(nn));
+    std::vector<std::int32_t> conn(8 * std::size_t(ne)), ijk(3 * std::size_t(nn));
+    neon_mesh_box(n, n, n, 1.0, 1.0, 1.0, 0.1, 0x6E656F6E01ull, 0.0, X.data(), conn.data(), ijk.data());
+    std::vector<std::int32_t> top;
+    for (std::int32_t e = 0; e < ne; ++e)
+        if (ijk[3 * std::size_t(conn[8 * e + 4]) + 2] == n) top.insert(top.end(), &conn[8 * e + 4], &conn[8 * e + 8]);
+    std::vector<double> const t = neon_hip::surface_load_vector(X, top, 2);
+    std::vector<double> const p = neon_hip::surface_load_vector(X, top, -1);
+    std::vector<double> const b = neon_hip::volume_load_vector(X, conn, 1);
+    double tz = 0.0, pz = 0.0, by = 0.0, other = 0.0;
+    for (std::int32_t a = 0; a < nn; ++a)
+    {
+        tz += t[3 * a + 2];
+        pz += p[3 * a + 2];
+        by += b[3 * a + 1];
+        other += std::abs(t[3 * a]) + std::abs(t[3 * a + 1]) + std::abs(b[3 * a]) + std::abs(b[3 * a + 2]);
+    }
+    std::printf("      load vectors: traction %.15f, pressure %.15f, body force %.15f\n", tz, pz, by);
+    expect(std::abs(tz - 1.0) < 1e-13 && std::abs(pz + 1.0) < 1e-13 && std::abs(by - 1.0) < 1e-13 && other == 0.0,
+           "unit-load vectors integrate the face area and the volume");
+    bool threw = false;
+    try
+    {
+        neon_hip::volume_load_vector(X, conn, 3);
+    }
+    catch (std::domain_error const&)
+    {
+        threw = true;
+    }
+    expect(threw, "a body force component outside 0..2 is rejected");
+}
+
+// uniaxial neo-Hooke under a top traction t (first Piola stress P_zz = t, no lateral stress):
+// lambda ln J + mu (l^2 - 1) = 0 and (lambda ln J + mu (lz^2 - 1)) / lz = t, J = l^2 lz
+void traction_stretches(double bulk, double shear, double t, double& lz, double& l)
+{
+    double const lambda = bulk - 2.0 * shear / 3.0;
+    lz = 1.0;
+    l = 1.0;
+    for (int it = 0; it < 60; ++it)
+    {
+        double const lnJ = std::log(l * l * lz);
+        double const r1 = lambda * lnJ + shear * (l * l - 1.0);
+        double const r2 = (lambda * lnJ + shear * (lz * lz - 1.0)) / lz - t;
+        double const a11 = 2.0 * lambda / l + 2.0 * shear * l, a12 = lambda / lz;
+        double const a21 = 2.0 * lambda / (l * lz);
+        double const a22 = (lambda / lz + 2.0 * shear * lz) / lz - (lambda * lnJ + shear * (lz * lz - 1.0)) / (lz * lz);
+        double const det = a11 * a22 - a12 * a21;
+        l -= (r1 * a22 - a12 * r2) / det;
+        lz -= (a11 * r2 - a21 * r1) / det;
+    }
+}
+
+void traction_patch_test(neon_hip::context& ctx)
+{
+    int const n = 4;
+    double const t = 0.1, bulk = 2.0, shear = 0.4;
+    std::int32_t nn = 0, ne = 0;
+    neon_mesh_box_size(n, n, n, 1.0, 1.0, 1.0, 0.0, &nn, &ne);
+    std::vector<double> X(3 * std::size_t(nn));
+    std::vector<std::int32_t> conn(8 * std::size_t(ne)), ijk(3 * std::size_t(nn));
+    neon_mesh_box(n, n, n, 1.0, 1.0, 1.0, 0.1, 0x6E656F6E01ull, 0.0, X.data(), conn.data(), ijk.data());
+    std::vector<double> params(4 * std::size_t(ne));
+    for (std::int32_t e = 0; e < ne; ++e)
+    {
+        params[4 * e] = bulk;
+        params[4 * e + 1] = shear;
+    }
+    neon_hip::device_mesh mesh(ctx, X, conn, params, NEON_MODEL_NEOHOOKE);
+    std::vector<neon_hip::dirichlet_boundary> bcs(3);
+    for (std::int32_t a = 0; a < nn; ++a)
+        for (int i = 0; i < 3; ++i)
+            if (ijk[3 * a + i] == 0) bcs[i].dofs.push_back(3 * a + i);
+    for (auto& b : bcs)
+    {
+        b.times = {0.0, 1.0};
+        b.values = {0.0, 0.0};
+    }
+    std::vector<std::int32_t> top;
+    for (std::int32_t e = 0; e < ne; ++e)
+        if (ijk[3 * std::size_t(conn[8 * e + 4]) + 2] == n) top.insert(top.end(), &conn[8 * e + 4], &conn[8 * e + 8]);
+    std::vector<double> const unit = neon_hip::surface_load_vector(X, top, 2);
+    neon_hip::nonfollower_load load;
+    for (std::int32_t i = 0; i < std::int32_t(unit.size()); ++i)
+        if (unit[i] != 0.0)
+        {
+            load.dofs.push_back(i);
+            load.values.push_back(unit[i]);
+        }
+    load.times = {0.0, 1.0};
+    load.loads = {0.0, t};
+    neon_hip::simulation_options o;
+    o.initial = o.minimum = o.maximum = 0.5;
+    o.residual_tolerance = 1e-10;
+    o.displacement_tolerance = 1e-10;
+    o.always_relative = true;
+    o.linear.type = "iterative";
+    o.linear.device = "gpu";
+    o.linear.backend = "hip";
+    o.linear.tolerance = 1e-13;
+    o.linear.maximum_iterations = 5000;
+    neon_hip::static_matrix sm(mesh, o, bcs, {load});
+    std::vector<double> const u = sm.solve();
+    double lz = 1.0, l = 1.0;
+    traction_stretches(bulk, shear, t, lz, l);
+    double err = 0.0, umax = 0.0;
+    for (std::int32_t a = 0; a < nn; ++a)
+    {
+        double const ue[3] = {(l - 1.0) * X[3 * a], (l - 1.0) * X[3 * a + 1], (lz - 1.0) * X[3 * a + 2]};
+        for (int i = 0; i < 3; ++i)
+        {
+            err = std::max(err, std::abs(u[3 * a + i] - ue[i]));
+            umax = std::max(umax, std::abs(ue[i]));
+        }
+    }
+    std::printf("      traction patch test: stretches %.12f %.12f, max |u - (F - I) X| / max|u| = %.3e\n", lz, l,
+                err / umax);
+    expect(err <= 1e-9 * umax, "uniaxial neo-Hooke under a top traction matches the closed form");
+}
 } // namespace
 
 int main()
 {
     factory_errors();
+    load_vectors();
     int devices = 0;
     neon_device_count(&devices);
     if (devices == 0)
@@ -187,6 +313,7 @@ int main()
     neon_hip::context ctx(0);
     tridiagonal(ctx);
     patch_test(ctx);
+    traction_patch_test(ctx);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
