@@ -119,6 +119,8 @@ int neon_mesh_set_parameters(neon_context* ctx, const double* params);
 int neon_mesh_pattern(neon_context* ctx, int64_t* nnz, int32_t* row_ptr, int32_t* col);
 /* u: 3N displacements (host). */
 int neon_mesh_set_displacement(neon_context* ctx, const double* u);
+/* The displacement held by the mesh (3N, host). */
+int neon_mesh_displacement(neon_context* ctx, double* u);
 int neon_mesh_update_internal_variables(neon_context* ctx, double time_step);
 int neon_mesh_save_internal_variables(neon_context* ctx, int32_t converged);
 /* Per-QP state, index e*8+l.  Any pointer may be NULL.  sigma/eps_p 9 per QP (row-major
@@ -176,6 +178,15 @@ typedef struct {
     const double* values;
 } neon_dirichlet;
 
+/* Called by the Newton driver where neon's static_matrix::solve calls mesh.write
+ * (static_matrix.hpp:150-154, 459-465): at the start of every solve attempt (the initial state,
+ * and again after each cut-back, neon's solve() re-entering itself) and after every converged
+ * load step, with the converged increments so far and the last converged time.  The context
+ * then holds that state: the callback may read it through neon_mesh_displacement,
+ * neon_mesh_qp_state and neon_mesh_download (f_int of the last assembly, neon's reaction
+ * force is its negative; zero before the first), and must not change it. */
+typedef void (*neon_step_callback)(void* user, int32_t step, double time);
+
 typedef struct {
     double period, initial, minimum, maximum; /* "time" block */
     int32_t adaptive;
@@ -192,6 +203,8 @@ typedef struct {
     int32_t max_total_iterations; /* stop after this many Newton iterations in all (0: no limit) */
     int32_t record_trajectory;    /* keep u and the committed history of every converged step on the
                                      device, for neon_adjoint_gradient on J2 models (0: off) */
+    neon_step_callback on_step;   /* NULL: none */
+    void* callback_user;
 } neon_newton_options;
 
 typedef struct {

@@ -249,6 +249,12 @@ class DeviceMesh:
     def save_internal_variables(self, converged: bool) -> None:
         self.ctx.check(_capi.lib().neon_mesh_save_internal_variables(self.ctx.handle, int(converged)))
 
+    def displacement(self) -> np.ndarray:
+        """The displacement held by the mesh (``neon_mesh_displacement``)."""
+        u = np.zeros(self.n_dofs)
+        self.ctx.check(_capi.lib().neon_mesh_displacement(self.ctx.handle, _capi.f64(u)))
+        return u
+
     def qp_state(self):
         Q = 8 * self.n_elem
         sigma, D = np.zeros(9 * Q), np.zeros(36 * Q)
@@ -402,9 +408,25 @@ class static_matrix:
         self.records: list[dict] = []
         self.timing: dict = {}
 
-    def solve(self, capacity: int = 4096) -> np.ndarray:
+    def solve(self, capacity: int = 4096, on_step=None) -> np.ndarray:
+        """Runs every load step.  ``on_step(step, time)`` is called where neon writes its
+        output (the initial state, after each cut-back and after each converged step); it may
+        read the mesh (``DeviceMesh.displacement``, ``qp_state``, ``download``) but not change it."""
         m = self.mesh
         u = np.zeros(m.n_dofs)
+        failure = []
+
+        def _cb(user, step, time):
+            # exceptions cannot cross the C callback: keep the first and re-raise after the solve
+            if failure:
+                return
+            try:
+                on_step(int(step), float(time))
+            except BaseException as e:  # noqa: BLE001
+                failure.append(e)
+
+        cb = _capi.StepCallback(_cb) if on_step is not None else _capi.StepCallback()
+        self.options.on_step = cb
         recs = (_capi.NewtonRecord * capacity)()
         nrec = C.c_int32(0)
         tm = _capi.NewtonTiming()
@@ -425,5 +447,9 @@ class static_matrix:
             for r in recs[: min(nrec.value, capacity)]
         ]
         self.timing = {name: getattr(tm, name) for name, _ in _capi.NewtonTiming._fields_}
+        self.options.on_step = _capi.StepCallback()
+        del cb
+        if failure:
+            raise failure[0]
         m.ctx.check(code)
         return u

@@ -90,6 +90,10 @@ class Neumann(C.Structure):
     ]
 
 
+# void (*)(void* user, int32_t step, double time)
+StepCallback = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_double)
+
+
 class NewtonOptions(C.Structure):
     _fields_ = [
         ("period", C.c_double),
@@ -110,6 +114,8 @@ class NewtonOptions(C.Structure):
         ("max_load_steps", C.c_int32),
         ("max_total_iterations", C.c_int32),
         ("record_trajectory", C.c_int32),
+        ("on_step", StepCallback),
+        ("callback_user", C.c_void_p),
     ]
 
 
@@ -176,6 +182,7 @@ SIGNATURES = {
         C.c_int,
         [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, i32p, i32p],
     ),
+    "neon_mesh_displacement": (C.c_int, [C.c_void_p, f64p]),
     "neon_newton_solve_loaded": (
         C.c_int,
         [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p, P(NewtonRecord),

@@ -232,6 +232,18 @@ int neon_mesh_set_displacement(neon_context* ctx, const double* u)
     });
 }
 
+int neon_mesh_displacement(neon_context* ctx, double* u)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!u) throw status_error(NEON_BAD_ARGUMENT, "null displacement");
+        device_mesh& m = *ctx->mesh;
+        NEON_CHECK_HIP(hipMemcpyAsync(u, m.u(), m.dofs() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        NEON_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        return NEON_OK;
+    });
+}
+
 int neon_mesh_update_internal_variables(neon_context* ctx, double)
 {
     return guarded(ctx, [&] {

@@ -201,6 +201,8 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
     minv.resize(n);
     NEON_CHECK_HIP(hipMemsetAsync(u_old.data(), 0, n * sizeof(double), s));
     NEON_CHECK_HIP(hipMemsetAsync(m.u(), 0, n * sizeof(double), s));
+    // static_matrix's f_int starts at zero (the reaction force of the initial write)
+    NEON_CHECK_HIP(hipMemsetAsync(m.f_int(), 0, n * sizeof(double), s));
     if (o->record_trajectory) m.clear_trajectory();
     NEON_CHECK_HIP(hipMemsetAsync(delta.data(), 0, n * sizeof(double), s));
     double* u = m.u();
@@ -232,6 +234,11 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
         auto t0 = std::chrono::steady_clock::now();
         int err = m.update_internal_variables(u);
         tm.update_ms += ms_since(t0);
+        if (err == NEON_OK && o->on_step)
+        {
+            NEON_CHECK_HIP(hipStreamSynchronize(s));
+            o->on_step(o->callback_user, ts.successful(), ts.last_step_time());
+        }
         while (err == NEON_OK && !ts.fully_applied())
         {
             if (o->max_load_steps > 0 && steps_done >= o->max_load_steps) break;
@@ -361,6 +368,11 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
             ts.update(true);
             m.save_internal_variables(true);
             ++steps_done;
+            if (o->on_step)
+            {
+                NEON_CHECK_HIP(hipStreamSynchronize(s));
+                o->on_step(o->callback_user, ts.successful(), ts.last_step_time());
+            }
         }
         if (err == NEON_OK) break;
         // computational_error: cutback, revert the history, reset u (static_matrix.hpp:168-182)

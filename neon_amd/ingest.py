@@ -58,6 +58,8 @@ class Problem:
     loads: list            # nonfollower: {name, dofs, values, times, loads[, fixed_time]}
     simulation: dict       # static_matrix options
     groups: dict = field(default_factory=dict)  # name -> list of (gmsh type, connectivity array)
+    visualisation: dict | None = None  # the step mesh's "visualisation" block (output fields)
+    output_name: str = ""              # file name of the output (the step mesh's "name")
 
     @property
     def n_nodes(self) -> int:
@@ -261,4 +263,5 @@ def read(path: str, step: int = 0) -> Problem:
         raise ValueError(f'Linear solver {ls["type"]} is not recognised.  Please use "PaStiX", "MUMPS", '
                          '"direct" or "iterative"')
     simulation = {"time": sim["time"], "nonlinear_options": sim.get("nonlinear_options", {}), "linear_solver": solver}
-    return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, simulation, groups)
+    return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, simulation, groups,
+                   mesh_data.get("visualisation"), mesh_data["name"])
