@@ -152,6 +152,8 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
 #define NEON_SPMV_META 128       /* row pointers of 32 rows per load, shared by shuffles (32 lanes, sweep order) */
 #define NEON_SPMV_SLICED 256     /* sliced copy of the values: 64 nodes per wavefront, one per lane */
 #define NEON_SPMV_SLICED_WIDE 512 /* with NEON_SPMV_SLICED: 1024-thread workgroups instead of 256 */
+#define NEON_SPMV_INDEX32 1024   /* with NEON_SPMV_SLICED: 32-bit columns even where 16-bit offsets from the
+                                    row node fit (the default uses the 16-bit form when every offset fits) */
 int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
                          int32_t reps, double* avg_ms);
 /* Shape of the SpMV that the solvers on this mesh's tangent use from now on (lanes, flags as
@@ -160,6 +162,10 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
  * meshes whose nodes have at most 32 neighbours and falls back to the plain one otherwise.
  * No counterpart in the reference: a tuning knob; results agree to rounding. */
 int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);
+/* The sliced copy's shape: slices (64 nodes each), steps (block columns per lane, padding
+ * included) and whether its columns fit 16-bit offsets from the row node (1) or stay 32-bit
+ * (0).  No counterpart in the reference. */
+int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int32_t* index16);
 /* Average milliseconds of one assembly (tangent + f_int, all colours) of the held state with
  * `blocks_per_cu` one-wavefront workgroups per compute unit and colour (default 128).  No
  * counterpart in the reference: a tuning probe. */

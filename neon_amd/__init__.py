@@ -28,6 +28,7 @@ from ._capi import (  # noqa: F401
     SPMV_META,
     SPMV_SLICED,
     SPMV_SLICED_WIDE,
+    SPMV_INDEX32,
     SPMV_NONTEMPORAL,
     SPMV_PIPELINE,
     SPMV_STREAM,
@@ -248,6 +249,13 @@ class DeviceMesh:
 
     def save_internal_variables(self, converged: bool) -> None:
         self.ctx.check(_capi.lib().neon_mesh_save_internal_variables(self.ctx.handle, int(converged)))
+
+    def sliced_info(self) -> dict:
+        """Shape of the sliced tangent copy (``neon_mesh_sliced_info``)."""
+        sl, idx = C.c_int32(0), C.c_int32(0)
+        st = C.c_int64(0)
+        self.ctx.check(_capi.lib().neon_mesh_sliced_info(self.ctx.handle, C.byref(sl), C.byref(st), C.byref(idx)))
+        return {"slices": sl.value, "steps": st.value, "index16": bool(idx.value)}
 
     def displacement(self) -> np.ndarray:
         """The displacement held by the mesh (``neon_mesh_displacement``)."""

@@ -48,6 +48,7 @@ SPMV_PROBE_DIRTY = 64
 SPMV_META = 128
 SPMV_SLICED = 256
 SPMV_SLICED_WIDE = 512
+SPMV_INDEX32 = 1024
 
 STATUS_NAMES = {
     0: "OK",
@@ -183,6 +184,7 @@ SIGNATURES = {
         [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, i32p, i32p],
     ),
     "neon_mesh_displacement": (C.c_int, [C.c_void_p, f64p]),
+    "neon_mesh_sliced_info": (C.c_int, [C.c_void_p, i32p, P(C.c_int64), i32p]),
     "neon_newton_solve_loaded": (
         C.c_int,
         [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p, P(NewtonRecord),

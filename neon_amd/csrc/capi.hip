@@ -340,7 +340,8 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        if ((lanes != 16 && lanes != 32) || (flags & ~(31 | NEON_SPMV_META | NEON_SPMV_SLICED | NEON_SPMV_SLICED_WIDE))
+        if ((lanes != 16 && lanes != 32)
+            || (flags & ~(31 | NEON_SPMV_META | NEON_SPMV_SLICED | NEON_SPMV_SLICED_WIDE | NEON_SPMV_INDEX32))
             || blocks_fp64 < 0 || blocks_fp32 < 0)
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
         if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS | NEON_SPMV_STREAM | NEON_SPMV_META)) && lanes != 32)
@@ -350,6 +351,17 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
         c.flags = flags;
         c.blocks64 = blocks_fp64;
         c.blocks32 = blocks_fp32;
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int32_t* index16)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (slices) *slices = ctx->mesh->sliced_slices();
+        if (steps) *steps = ctx->mesh->sliced_steps();
+        if (index16) *index16 = ctx->mesh->sliced_index16() ? 1 : 0;
         return NEON_OK;
     });
 }
@@ -378,7 +390,8 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
             throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs need rows of at most 32 blocks");
         A.blocks_override = blocks;
         A.sell = false;
-        if (flags & NEON_SPMV_SLICED) m.attach_sliced(A, precision, (flags & NEON_SPMV_SLICED_WIDE) != 0);
+        if (flags & NEON_SPMV_SLICED)
+            m.attach_sliced(A, precision, (flags & NEON_SPMV_SLICED_WIDE) != 0, (flags & NEON_SPMV_INDEX32) != 0);
         int64_t const n = m.dofs();
         hipStream_t const s = ctx->stream;
         hipEvent_t e0, e1;

@@ -65,18 +65,22 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
     if (A.node_block && A.sell)
     {
         if (!sval) throw status_error(NEON_BAD_ARGUMENT, "sliced operator without sliced values");
-        sell_matrix m{A.nodes, A.slices, A.sptr, A.scol};
-        if (A.sell_block == 1024)
+        sell_matrix m{A.nodes, A.slices, A.sptr, A.scol, A.scol16};
+#define NEON_SELL(BS, XCD, I16) \
+    spmv_sell_kernel<BS, XCD, I16, VT, XT, YT, AT, DOT, MODE><<<blocks, BS, 0, s>>>(m, sval, x, y, b, part, st, gate)
+        int const key = (A.sell_block == 1024 ? 4 : 0) + (A.nb_sweep ? 2 : 0) + (A.scol16 ? 1 : 0);
+        switch (key)
         {
-            if (A.nb_sweep)
-                spmv_sell_kernel<1024, true, VT, XT, YT, AT, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
-            else
-                spmv_sell_kernel<1024, false, VT, XT, YT, AT, DOT, MODE><<<blocks, 1024, 0, s>>>(m, sval, x, y, b, part, st, gate);
+        case 0: NEON_SELL(256, false, false); break;
+        case 1: NEON_SELL(256, false, true); break;
+        case 2: NEON_SELL(256, true, false); break;
+        case 3: NEON_SELL(256, true, true); break;
+        case 4: NEON_SELL(1024, false, false); break;
+        case 5: NEON_SELL(1024, false, true); break;
+        case 6: NEON_SELL(1024, true, false); break;
+        default: NEON_SELL(1024, true, true); break;
         }
-        else if (A.nb_sweep)
-            spmv_sell_kernel<256, true, VT, XT, YT, AT, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
-        else
-            spmv_sell_kernel<256, false, VT, XT, YT, AT, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
+#undef NEON_SELL
         NEON_CHECK_HIP(hipGetLastError());
         return;
     }
@@ -818,6 +822,31 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
         scol_.resize(scol.size());
         NEON_CHECK_HIP(hipMemcpyAsync(sptr_.data(), sptr.data(), sptr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
         NEON_CHECK_HIP(hipMemcpyAsync(scol_.data(), scol.data(), scol.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+        // the same columns as 16-bit offsets from the lane's node, when every offset fits
+        std::vector<int16_t> scol16(scol.size());
+        bool fits = true;
+#pragma omp parallel for schedule(static) reduction(&& : fits)
+        for (int32_t sl = 0; sl < slices_; ++sl)
+            for (int lane = 0; lane < 64; ++lane)
+            {
+                int64_t const nd = 64ll * sl + lane;
+                for (int32_t kk = 0; kk < sptr[sl + 1] - sptr[sl]; ++kk)
+                {
+                    size_t const at = size_t(sptr[sl] + kk) * 64 + lane;
+                    // lanes past the last node gather from the last node
+                    int64_t const col = nd < N_ ? int64_t(scol[at]) : int64_t(N_) - 1;
+                    int64_t const off = col - nd;
+                    if (off < INT16_MIN || off > INT16_MAX) fits = false;
+                    scol16[at] = int16_t(off < INT16_MIN ? 0 : (off > INT16_MAX ? 0 : off));
+                }
+            }
+        scol16_ok_ = fits;
+        if (fits)
+        {
+            scol16_.resize(scol16.size());
+            NEON_CHECK_HIP(hipMemcpyAsync(scol16_.data(), scol16.data(), scol16.size() * sizeof(int16_t),
+                                          hipMemcpyHostToDevice, s_));
+        }
         // the host vectors go out of scope: wait for the copies
         NEON_CHECK_HIP(hipStreamSynchronize(s_));
     }
@@ -1104,7 +1133,7 @@ linear_operator device_mesh::op(int precision) const
     // nodes of more than 170 neighbours keep the CSR-order values (the sliced copy is packed
     // through LDS rows of 3 * max degree values)
     if ((spmv.flags & NEON_SPMV_SLICED) && sliced_fits())
-        attach_sliced(A, precision, (spmv.flags & NEON_SPMV_SLICED_WIDE) != 0);
+        attach_sliced(A, precision, (spmv.flags & NEON_SPMV_SLICED_WIDE) != 0, (spmv.flags & NEON_SPMV_INDEX32) != 0);
     return A;
 }
 
@@ -1127,10 +1156,10 @@ bool device_mesh::sliced_fits() const
     return size_t(64) * 3 * std::max<int32_t>(max_degree_, 1) * sizeof(double) <= 65536;
 }
 
-void device_mesh::attach_sliced(linear_operator& A, int precision, bool wide) const
+void device_mesh::attach_sliced(linear_operator& A, int precision, bool wide, bool index32) const
 {
     if (!sliced_fits()) throw status_error(NEON_BAD_ARGUMENT, "node degree too large for the sliced tangent copy");
-    sell_matrix const m{N_, slices_, sptr_.data(), scol_.data()};
+    sell_matrix const m{N_, slices_, sptr_.data(), scol_.data(), nullptr};
     size_t const vals = size_t(sell_steps_) * 9 * 64;
     if (precision == NEON_PRECISION_FP64)
     {
@@ -1159,6 +1188,7 @@ void device_mesh::attach_sliced(linear_operator& A, int precision, bool wide) co
     A.slices = slices_;
     A.sptr = sptr_.data();
     A.scol = scol_.data();
+    A.scol16 = index32 || !scol16_ok_ ? nullptr : scol16_.data();
 }
 
 namespace

@@ -125,6 +125,7 @@ struct linear_operator
     int32_t slices = 0;
     const int32_t* sptr = nullptr;
     const int32_t* scol = nullptr;
+    const int16_t* scol16 = nullptr; // 16-bit column offsets (used when set)
     const double* s64 = nullptr;
     const float* s32 = nullptr;
 
@@ -234,7 +235,11 @@ public:
     linear_operator op(int precision) const;
     // points A's SpMV at the sliced copy of the tangent (packed here when the tangent changed
     // since the last copy); wide: 1024-thread workgroups instead of 256
-    void attach_sliced(linear_operator& A, int precision, bool wide) const;
+    void attach_sliced(linear_operator& A, int precision, bool wide, bool index32 = false) const;
+    // whether every sliced column fits a 16-bit offset from its row node
+    bool sliced_index16() const { return scol16_ok_; }
+    int32_t sliced_slices() const { return slices_; }
+    int64_t sliced_steps() const { return sell_steps_; }
     // the sliced copy is packed through LDS: nodes of at most 170 neighbours
     bool sliced_fits() const;
     // Jacobi inverse from the node-block diagonal
@@ -287,6 +292,8 @@ private:
     int32_t slices_ = 0;
     int64_t sell_steps_ = 0;
     device_array<int32_t> sptr_, scol_;
+    device_array<int16_t> scol16_;
+    bool scol16_ok_ = false;
     mutable device_array<double> s64_;
     mutable device_array<float> s32_;
     mutable bool s64_fresh_ = false, s32_fresh_ = false;

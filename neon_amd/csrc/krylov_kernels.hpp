@@ -216,12 +216,15 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
 // count, and lanes past the last node, hold zero values with the node itself (node 0 past the
 // end) as column.  Value (i, j) of step kk of slice s sits at
 //   sval[((sptr[s] + kk) * 9 + 3 i + j) * 64 + lane],  its column node at scol[(sptr[s] + kk) * 64 + lane].
+// scol16, when set, holds the same columns as 16-bit offsets from the lane's node (64 s + lane;
+// lanes past the last node point at the last node), halving the column bytes.
 struct sell_matrix
 {
     int32_t nodes;
     int32_t slices;
     const int32_t* sptr;
     const int32_t* scol;
+    const int16_t* scol16;
 };
 
 // y = A x on the sliced layout, one wavefront per slice, one node (three rows) per lane.  Each
@@ -232,7 +235,7 @@ struct sell_matrix
 // (consecutive slices on different XCDs, so every XCD's L2 fetches all of x).  XCD=true (a
 // multiple of 8 workgroups): the workgroups b with b % 8 = e (one XCD) sweep the e-th eighth
 // of the slices together, so each L2 fetches x only around its own eighth.
-template <int BS, bool XCD, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+template <int BS, bool XCD, bool I16, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
 __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* __restrict__ val,
                                                        const XT* __restrict__ x, YT* __restrict__ y,
                                                        const double* __restrict__ bvec, double* __restrict__ part,
@@ -287,7 +290,7 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
         for (int32_t kk = 0; kk < ks; ++kk)
         {
             int64_t const blk = int64_t(k0) + kk;
-            int64_t const m = A.scol[blk * 64 + lane];
+            int64_t const m = I16 ? n + int64_t(A.scol16[blk * 64 + lane]) : int64_t(A.scol[blk * 64 + lane]);
             VT v[9];
 #pragma unroll
             for (int t = 0; t < 9; ++t) v[t] = val[(blk * 9 + t) * 64 + lane];

@@ -152,6 +152,8 @@ SPMV_SHAPES = [
     (32, neon_amd.SPMV_SLICED), (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SLICED_WIDE),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP | neon_amd.SPMV_SLICED_WIDE),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_INDEX32),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_INDEX32 | neon_amd.SPMV_SLICED_WIDE),
 ]
 
 
@@ -188,3 +190,37 @@ def test_spmv_shapes_agree(ctx, precision):
         assert abs(its - its0) <= slack, (its, its0)
         assert np.linalg.norm(x - x0) <= 1e-8 * np.linalg.norm(x0)
     mg.spmv_config()
+
+
+def test_sliced_columns_without_16bit_offsets(ctx):
+    # node numbers shuffled over a 32^3-cell block (35,937 nodes): neighbours lie up to
+    # 35,936 apart, beyond 16-bit offsets, so the sliced copy keeps 32-bit columns; the solve
+    # still meets the tolerance against the downloaded matrix, and agrees with the CSR-order
+    # shape; the unshuffled block uses 16-bit offsets
+    p = problems.block_rollers(32)
+    perm = np.random.default_rng(3).permutation(p.n_nodes)
+    inv = np.argsort(perm)
+    X = p.X[perm]
+    conn = inv[p.conn.reshape(-1, 8)].astype(np.int32)
+    plain = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    assert plain.sliced_info()["index16"]
+    mg = neon_amd.DeviceMesh(X, conn, p.params, context=ctx)
+    info = mg.sliced_info()
+    assert not info["index16"] and info["slices"] == (p.n_nodes + 63) // 64
+    u = np.zeros(p.n_dofs)
+    u.reshape(-1, 3)[:, 2] = 0.01 * X[:, 2]
+    mg.set_displacement(u)
+    mg.update_internal_variables()
+    mg.assemble(neon_amd.PRECISION_FP64)
+    _, f = mg.download(stiffness=False)
+    b = -f.copy()
+    fixed = np.nonzero(np.isin(np.arange(p.n_dofs) // 3, inv[np.nonzero(p.X[:, 2] == 0)[0]]))[0]
+    mg.enforce_dirichlet(fixed, b)
+    rp, ci = mg.pattern()
+    val, _ = mg.download(internal_force=False)
+    A = sp.csr_matrix((val, ci, rp))
+    x1, its1, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
+    assert np.linalg.norm(b - A @ x1) <= 1e-10 * np.linalg.norm(b)
+    mg.spmv_config(32, neon_amd.SPMV_SWEEP, 0, 0)
+    x2, its2, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
+    assert abs(its1 - its2) <= 2 and np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x2)
