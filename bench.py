@@ -48,11 +48,12 @@ def measured_traffic(n):
     return None
 
 
-def spmv_bytes(nnz, nnzb, nodes, n, value_bytes):
-    """Algorithmic bytes of one node-block SpMV: values once, one int32 block column per 9
-    values, x read once, y written once (row pointers and the sliced copy's padding, which
-    depend on the storage format, are not counted)."""
-    return value_bytes * nnz + 4 * nnzb + 2 * value_bytes * n
+def spmv_bytes(nnz, nnzb, nodes, n, value_bytes, index_bytes=4):
+    """Algorithmic bytes of one node-block SpMV: values once, one block column per 9 values
+    (`index_bytes`: 2 when the solvers' sliced copy holds 16-bit offsets from the row node, 4
+    for int32 columns), x read once, y written once (row pointers and the sliced copy's
+    padding, which depend on the storage format, are not counted)."""
+    return value_bytes * nnz + index_bytes * nnzb + 2 * value_bytes * n
 
 
 def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
@@ -178,6 +179,8 @@ def main():
     mesh = neon_amd.DeviceMesh(p.X, p.conn, p.params, model=p.model, context=ctx)
     nnz = mesh.nnz()
     nnzb = nnz // 9
+    # the column bytes the solvers' SpMV needs: 16-bit offsets when they fit
+    ib = 2 if mesh.sliced_info()["index16"] else 4
     log(f"setup {time.perf_counter() - t0:.1f} s: {p.n_elem} elements, {p.n_dofs} dofs, nnz {nnz}")
 
     results = {}
@@ -188,7 +191,7 @@ def main():
     r = results[head]
     value = args.steps / r["seconds"]
     vb = 4 if head == "mixed" else 8
-    bytes_spmv = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vb)
+    bytes_spmv = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vb, ib)
     spmv_avg_ms = r["spmv_ms"] / max(r["spmv_count"], 1)
     achieved = bytes_spmv / (spmv_avg_ms * 1e-3) / 1e9
     line = {
@@ -215,7 +218,7 @@ def main():
     }
     for prec, rr in results.items():
         vbp = 4 if prec == "mixed" else 8
-        bs = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vbp)
+        bs = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vbp, ib)
         avg = rr["spmv_ms"] / max(rr["spmv_count"], 1)
         asm_avg = rr["asm_ms"] / max(rr["asm_count"], 1)
         its = [x["solver_iterations"] for x in rr["records"]]

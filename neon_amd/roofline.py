@@ -6,7 +6,8 @@ builds the n^3 block (the bench's mesh), assembles the tangent at u = 0 in the r
 precision, runs `reps` SpMVs with the solver's default shape (`neon_mesh_spmv_bench`) and a
 streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the calibration kernel for the
 FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the mean SpMV time, its
-algorithmic bytes (values, block columns, x read once, y written once)
+algorithmic bytes (values, block columns (2 bytes each as 16-bit offsets), x read once, y
+written once)
 and the achieved fraction of the 8 TB/s HBM peak.  Run it under `rocprofv3 --kernel-trace
 --stats` or `--pmc` to read the same kernels' durations and traffic counters.
 """
@@ -24,10 +25,11 @@ from neon_amd import _capi, problems
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(nnz: int, nodes: int, value_bytes: int) -> int:
-    """Values once, one int32 block column per 9 values, x read once, y written once."""
+def spmv_bytes(nnz: int, nodes: int, value_bytes: int, index_bytes: int = 4) -> int:
+    """Values once, one block column per 9 values (`index_bytes` each: 2 for the sliced copy's
+    16-bit offsets, 4 for int32 columns), x read once, y written once."""
     n = 3 * nodes
-    return value_bytes * nnz + 4 * (nnz // 9) + 2 * value_bytes * n
+    return value_bytes * nnz + index_bytes * (nnz // 9) + 2 * value_bytes * n
 
 
 def main(argv=None) -> dict:
@@ -53,7 +55,7 @@ def main(argv=None) -> dict:
                                        C.byref(ms)))
     nnz = mesh.nnz()
     vb = 8 if prec == _capi.PRECISION_FP64 else 4
-    b = spmv_bytes(nnz, p.n_nodes, vb)
+    b = spmv_bytes(nnz, p.n_nodes, vb, 2 if mesh.sliced_info()["index16"] else 4)
     gbs = b / (ms.value * 1e-3) / 1e9
     read_ms, copy_ms = C.c_double(), C.c_double()
     copy_bytes = args.copy_mb << 20
