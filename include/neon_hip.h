@@ -57,11 +57,21 @@ extern "C" {
 #define NEON_NO_DEVICE -5
 
 /* ---- enumerations ---- */
-#define NEON_MODEL_NEOHOOKE 0 /* compressible_neohooke */
-#define NEON_MODEL_J2 1       /* small_strain_J2_plasticity */
+/* Constitutive models of make_constitutive_model (constitutive_model_factory.cpp:37-160).
+ * The microsphere models take their unit-sphere rule (unit_sphere_quadrature.cpp:10-90) as a
+ * flag: BO21 (default, 0) or NEON_SPHERE_BO33. */
+#define NEON_MODEL_NEOHOOKE 0             /* compressible_neohooke */
+#define NEON_MODEL_J2 1                   /* small_strain_J2_plasticity */
+#define NEON_MODEL_LINEAR_ELASTIC 2       /* isotropic_linear_elasticity */
+#define NEON_MODEL_MICROSPHERE_GAUSSIAN 3 /* gaussian_affine_microsphere */
+#define NEON_MODEL_MICROSPHERE_LANGEVIN 4 /* affine_microsphere (Pade inverse Langevin chains) */
+#define NEON_MODEL_MICROSPHERE_NONAFFINE 5 /* nonaffine_microsphere */
+#define NEON_SPHERE_BO33 0x100            /* 33-direction rule instead of the 21-direction one */
 
 #define NEON_TANGENT_REFERENCE 0  /* neon's tangents (neo-Hooke D without 1/J) */
-#define NEON_TANGENT_CONSISTENT 1 /* exact Jacobian of f_int (D/J for neo-Hooke) */
+#define NEON_TANGENT_CONSISTENT 1 /* exact Jacobian of f_int (D/J for neo-Hooke; the microsphere
+                                     D with the term its deviatoric projections drop, over J;
+                                     small-strain models integrated on X) */
 
 #define NEON_SOLVER_CG 0
 #define NEON_SOLVER_BICGSTAB 1
@@ -108,9 +118,11 @@ int neon_krylov_solve_csr(neon_context* ctx, int solver, int criterion, int prec
 
 /* ---- device-resident mesh, constitutive state and tangent ----
  * X: 3 per node, node-interleaved.  conn: 8 per element in hex8 order
- * (src/interpolations/hexahedron.cpp:21-28).  params: 4 per element
- * (bulk modulus, shear modulus, yield stress, hardening modulus); neo-Hooke uses the
- * first two.  One mesh per context. */
+ * (src/interpolations/hexahedron.cpp:21-28).  params: 4 per element: (bulk modulus, shear
+ * modulus, yield stress, hardening modulus) for J2, of which neo-Hooke and linear elasticity
+ * use the first two; (bulk modulus, shear modulus, segments per chain, non-affine stretch
+ * parameter) for the microsphere models (micromechanical_elastomer.cpp:12-20,
+ * nonaffine_microsphere.cpp:14-24).  One mesh per context. */
 int neon_mesh_create(neon_context* ctx, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
                      int32_t model, const double* params, int32_t tangent_mode);
 /* Replace the per-element parameters (4 per element) without rebuilding the pattern. */

@@ -19,7 +19,12 @@ import numpy as np
 from . import _capi
 from ._capi import (  # noqa: F401
     MODEL_J2,
+    MODEL_LINEAR_ELASTIC,
+    MODEL_MICROSPHERE_GAUSSIAN,
+    MODEL_MICROSPHERE_LANGEVIN,
+    MODEL_MICROSPHERE_NONAFFINE,
     MODEL_NEOHOOKE,
+    SPHERE_BO33,
     PRECISION_FP64,
     PRECISION_MIXED,
     SOLVER_BICGSTAB,

@@ -26,6 +26,11 @@ NEON_NO_DEVICE = -5
 
 MODEL_NEOHOOKE = 0
 MODEL_J2 = 1
+MODEL_LINEAR_ELASTIC = 2
+MODEL_MICROSPHERE_GAUSSIAN = 3
+MODEL_MICROSPHERE_LANGEVIN = 4
+MODEL_MICROSPHERE_NONAFFINE = 5
+SPHERE_BO33 = 0x100  # microsphere rule flag: 33 directions instead of 21
 TANGENT_REFERENCE = 0
 TANGENT_CONSISTENT = 1
 SOLVER_CG = 0

@@ -538,6 +538,8 @@ int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed
         if (m.tangent_mode() != NEON_TANGENT_CONSISTENT)
             throw status_error(NEON_BAD_ARGUMENT, "the adjoint needs the consistent tangent (exact dR/du)");
         int64_t const n = m.dofs();
+        if (m.model() != NEON_MODEL_NEOHOOKE && m.model() != NEON_MODEL_J2)
+            throw status_error(NEON_BAD_ARGUMENT, "the adjoint gradient supports the neo-Hooke and J2 models");
         for (int32_t i = 0; i < n_fixed; ++i)
             if (fixed[i] < 0 || fixed[i] >= n) throw status_error(NEON_BAD_ARGUMENT, "fixed dof out of range");
         hipStream_t const s = ctx->stream;

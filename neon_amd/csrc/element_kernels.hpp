@@ -6,6 +6,11 @@
 //   deformation measures   src/mesh/mechanics/solid/submesh.cpp:210-269
 //   neo-Hooke              src/constitutive/mechanics/solid/compressible_neohooke.cpp:17-90
 //   J2 radial return       src/constitutive/mechanics/solid/small_strain_J2_plasticity.cpp:30-163
+//   linear elasticity      src/constitutive/mechanics/solid/isotropic_linear_elasticity.cpp:27-67
+//   microsphere models     src/constitutive/mechanics/solid/affine_microsphere.cpp:23-111,
+//                          gaussian_affine_microsphere.cpp:23-83, nonaffine_microsphere.cpp:26-178,
+//                          src/constitutive/mechanics/detail/microsphere.hpp,
+//                          src/quadrature/sphere/unit_sphere_quadrature.cpp:10-90
 //   element K, f_int       src/mesh/mechanics/solid/submesh.cpp:66-160
 //   Dirichlet zeroing      src/assembler/mechanics/static_matrix.hpp:279-320
 #pragma once
@@ -22,9 +27,11 @@ namespace neon_amd
 //              n in kinetic Voigt order); neo-Hooke: A = lambda, Bt = mu - lambda ln J, C = 0;
 //              J2: A = lambda + c1/3, Bt = G - c1/2, C = c2 with c1, c2 of the algorithmic
 //              tangent (small_strain_J2_plasticity.cpp:95-106).
+//              Microsphere models (general symmetric D): 21 coefficients, the upper triangle
+//              of D row by row, D(i,j) at kUpper[i][j].
 //   detF  : 1
 //   J2 history: eps_p 6 (tensor components 11,22,33,23,13,12), ebar 1, vm 1; committed
-//   copies eps_p_old, ebar_old.
+//   copies eps_p_old, ebar_old.  Linear elasticity keeps vm.
 enum
 {
     COEF_A = 0,
@@ -32,6 +39,32 @@ enum
     COEF_C = 2,
     COEF_N = 3
 };
+
+constexpr int kFullCoefs = 21;
+__host__ __device__ constexpr int upper_index(int i, int j)
+{
+    return i <= j ? i * 6 - i * (i - 1) / 2 + (j - i) : j * 6 - j * (j - 1) / 2 + (i - j);
+}
+
+__host__ __device__ constexpr bool is_microsphere(int model)
+{
+    return (model & 0xff) >= NEON_MODEL_MICROSPHERE_GAUSSIAN && (model & 0xff) <= NEON_MODEL_MICROSPHERE_NONAFFINE;
+}
+// constitutive_model::is_finite_deformation
+__host__ __device__ constexpr bool is_finite_model(int model)
+{
+    return model == NEON_MODEL_NEOHOOKE || is_microsphere(model);
+}
+
+// Unit-sphere rules: [0] 21 directions (BO21), [1] 33 directions (BO33); w and r (x, y, z)
+struct sphere_tables
+{
+    int n[2];
+    double w[2][33];
+    double r[2][33][3];
+};
+
+__constant__ sphere_tables c_sphere;
 
 struct hex8_tables
 {
@@ -83,13 +116,208 @@ __device__ inline void inv3(const double* a, double* r)
     r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
 }
 
+// ------------------------------------------------------------ microsphere update
+// Kinetic Voigt components (11, 22, 33, 23, 13, 12) of a (x) b + symmetric
+__device__ inline void sym_dyad(const double* a, double* v)
+{
+    v[0] = a[0] * a[0];
+    v[1] = a[1] * a[1];
+    v[2] = a[2] * a[2];
+    v[3] = a[1] * a[2];
+    v[4] = a[0] * a[2];
+    v[5] = a[0] * a[1];
+}
+
+// c += s v v^T on the 21 stored entries
+__device__ inline void add_outer21(double* c, double s, const double* v)
+{
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) c[upper_index(i, j)] += s * v[i] * v[j];
+}
+
+// Per-QP microsphere update (all three models share the volumetric part and the
+// projections of affine_microsphere.cpp:50-71):
+//   macro stress tau_bar and macro moduli c_bar from sphere sums over the rule's directions r
+//   (t = F_bar r, l = |t|; non-affine also n = F_bar^-T r, nu = |n|):
+//     Gaussian   tau_bar = 3G sum w t t,                 c_bar = 0
+//     Langevin   tau_bar = G sum w pf(l) t t,            c_bar = G sum w (ps(l) - pf(l)) / l^2 tttt
+//     non-affine chain stretch L = (sum w l^p)^(1/p), h = sum w l^(p-2) t t,
+//                H = (p - 2) sum w l^(p-4) tttt, k = sum w nu^-1 n n,
+//                T = sum w (-nu^-3 nnnn + 2 nu^-1 o(n))  (tube exponent q = 1, U = 9):
+//                tau_bar = G pf(L) L^(2-p) h - G N U k,
+//                c_bar = [G ps(L) L^(2-2p) - (p-1) G pf(L) L^(2-2p)] h h + G pf(L) L^(2-p) H + G N U T
+//   with pf = (3N - l^2)/(N - l^2), ps = (l^4 + 3N^2)/(N - l^2)^2 (microsphere.hpp:37-65);
+//   p = J U'(J), kappa = J^2 U''(J) for U = K/4 (J^2 - 1) - K/2 ln J;
+//   sigma = (p I + dev tau_bar) / J;
+//   D = (kappa + p) m m^T - 2p I + 2/3 tr(tau_bar) (I - m m^T / 3) + P c_bar P,
+//   I = diag(1,1,1,1/2,1/2,1/2), m = (1,1,1,0,0,0), P the kinetic deviatoric projector: this is
+//   neon's P [c_bar + 2/3 tr(tau_bar) I - 2/3 (tau_bar m^T + m tau_bar^T)] P, whose last term the
+//   projections annihilate.  The consistent mode adds the exact tangent's
+//   -2/3 (dev tau_bar m^T + m dev tau_bar^T) and divides by J.
+template <int BASE>
+__device__ void microsphere_update(const double* F, double J, const double* prm, int rule, bool consistent,
+                                   double* sig, double* D)
+{
+    double const K = prm[0], G = prm[1], N = prm[2], p = prm[3];
+    double const sc = pow(J, -1.0 / 3.0);
+    double Fb[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fb[k] = sc * F[k];
+    double tau[6] = {0, 0, 0, 0, 0, 0};
+    double cb[kFullCoefs];
+#pragma unroll
+    for (int k = 0; k < kFullCoefs; ++k) cb[k] = 0.0;
+    int const nd = c_sphere.n[rule];
+    if constexpr (BASE == NEON_MODEL_MICROSPHERE_NONAFFINE)
+    {
+        double Fi[9];
+        inv3(Fb, Fi);
+        double sum_p = 0.0, h[6] = {0, 0, 0, 0, 0, 0}, kk[6] = {0, 0, 0, 0, 0, 0}, H[kFullCoefs];
+#pragma unroll
+        for (int k = 0; k < kFullCoefs; ++k) H[k] = 0.0;
+        for (int i = 0; i < nd; ++i)
+        {
+            double const w = c_sphere.w[rule][i];
+            double const r0 = c_sphere.r[rule][i][0], r1 = c_sphere.r[rule][i][1], r2 = c_sphere.r[rule][i][2];
+            double t[3], n[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+            {
+                t[a] = Fb[a * 3] * r0 + Fb[a * 3 + 1] * r1 + Fb[a * 3 + 2] * r2;
+                n[a] = Fi[a] * r0 + Fi[3 + a] * r1 + Fi[6 + a] * r2; // F_bar^-T r
+            }
+            double const l2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+            double const nu2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+            double const nu = sqrt(nu2);
+            double const lp4 = pow(l2, 0.5 * (p - 4.0)); // l^(p-4)
+            double vt[6], vn[6];
+            sym_dyad(t, vt);
+            sym_dyad(n, vn);
+            sum_p += lp4 * l2 * l2 * w;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+            {
+                h[k] += lp4 * l2 * vt[k] * w;
+                kk[k] += vn[k] * (w / nu);
+            }
+            add_outer21(H, lp4 * w, vt);
+            // tube moduli q (q - 2) nu^(q-4) nnnn + 2 q nu^(q-2) o(n), q = 1
+            add_outer21(cb, -w / (nu2 * nu), vn);
+            double const o = 2.0 * w / nu;
+            double const n00 = n[0] * n[0], n11 = n[1] * n[1], n22 = n[2] * n[2];
+            double const n01 = n[0] * n[1], n02 = n[0] * n[2], n12 = n[1] * n[2];
+            cb[upper_index(0, 0)] += o * 2.0 * n00;
+            cb[upper_index(0, 4)] += o * n02;
+            cb[upper_index(0, 5)] += o * n01;
+            cb[upper_index(1, 1)] += o * 2.0 * n11;
+            cb[upper_index(1, 3)] += o * n12;
+            cb[upper_index(1, 5)] += o * n01;
+            cb[upper_index(2, 2)] += o * 2.0 * n22;
+            cb[upper_index(2, 3)] += o * n12;
+            cb[upper_index(2, 4)] += o * n02;
+            cb[upper_index(3, 3)] += o * 0.5 * (n11 + n22);
+            cb[upper_index(3, 4)] += o * 0.5 * n01;
+            cb[upper_index(3, 5)] += o * 0.5 * n02;
+            cb[upper_index(4, 4)] += o * 0.5 * (n00 + n22);
+            cb[upper_index(4, 5)] += o * 0.5 * n12;
+            cb[upper_index(5, 5)] += o * 0.5 * (n00 + n11);
+        }
+        double const L = pow(sum_p, 1.0 / p);
+        double const L2 = L * L;
+        double const pf = (3.0 * N - L2) / (N - L2);
+        double const ps = (L2 * L2 + 3.0 * N * N) / ((N - L2) * (N - L2));
+        double const kirchhoff = G * pf * L; // micro Kirchhoff force
+        double const c_f = kirchhoff * pow(L, 1.0 - p);
+        double const c_hh = G * ps * pow(L, 2.0 - 2.0 * p) - (p - 1.0) * kirchhoff * pow(L, 1.0 - 2.0 * p);
+        double const tube = G * N * 9.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tau[k] = c_f * h[k] - tube * kk[k];
+#pragma unroll
+        for (int k = 0; k < kFullCoefs; ++k) cb[k] = tube * cb[k] + c_f * (p - 2.0) * H[k];
+        add_outer21(cb, c_hh, h);
+    }
+    else
+    {
+        for (int i = 0; i < nd; ++i)
+        {
+            double const w = c_sphere.w[rule][i];
+            double const r0 = c_sphere.r[rule][i][0], r1 = c_sphere.r[rule][i][1], r2 = c_sphere.r[rule][i][2];
+            double t[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) t[a] = Fb[a * 3] * r0 + Fb[a * 3 + 1] * r1 + Fb[a * 3 + 2] * r2;
+            double vt[6];
+            sym_dyad(t, vt);
+            if constexpr (BASE == NEON_MODEL_MICROSPHERE_GAUSSIAN)
+            {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) tau[k] += vt[k] * w;
+            }
+            else
+            {
+                double const l2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+                double const pf = (3.0 * N - l2) / (N - l2);
+                double const ps = (l2 * l2 + 3.0 * N * N) / ((N - l2) * (N - l2));
+#pragma unroll
+                for (int k = 0; k < 6; ++k) tau[k] += pf * vt[k] * w;
+                add_outer21(cb, (ps - pf) / l2 * w, vt);
+            }
+        }
+        double const s = BASE == NEON_MODEL_MICROSPHERE_GAUSSIAN ? 3.0 * G : G;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tau[k] *= s;
+#pragma unroll
+        for (int k = 0; k < kFullCoefs; ++k) cb[k] *= G;
+    }
+    double const pr = J * (K / 2.0 * (J - 1.0 / J));
+    double const kappa = J * J * (K / 2.0 * (1.0 + 1.0 / (J * J)));
+    double const tr = tau[0] + tau[1] + tau[2];
+    double dev[6] = {tau[0] - tr / 3.0, tau[1] - tr / 3.0, tau[2] - tr / 3.0, tau[3], tau[4], tau[5]};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sig[k] = ((k < 3 ? pr : 0.0) + dev[k]) / J;
+    // P c_bar P: P = blockdiag(I - 1 1^T / 3, I); subtract the normal block's row / column means
+    double c[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) c[i][j] = cb[upper_index(i, j)];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+    {
+        double const mean = (c[0][j] + c[1][j] + c[2][j]) / 3.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) c[i][j] -= mean;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+    {
+        double const mean = (c[i][0] + c[i][1] + c[i][2]) / 3.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) c[i][j] -= mean;
+    }
+    double const a2 = 2.0 / 3.0 * tr;
+    double const inv_j = consistent ? 1.0 / J : 1.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j)
+        {
+            bool const nn = i < 3 && j < 3;
+            double const Ik = i == j ? (i < 3 ? 1.0 : 0.5) : 0.0;
+            double v = c[i][j] + (nn ? kappa + pr - a2 / 3.0 : 0.0) + (a2 - 2.0 * pr) * Ik;
+            if (consistent) v -= 2.0 / 3.0 * ((j < 3 ? dev[i] : 0.0) + (i < 3 ? dev[j] : 0.0));
+            D[upper_index(i, j)] = v * inv_j;
+        }
+}
+
 // ---------------------------------------------------- constitutive update per QP
 // One thread per quadrature point (qp = 8 e + l).
 template <int MODEL>
 __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const int32_t* __restrict__ conn,
                                                               const double* __restrict__ X,
                                                               const double* __restrict__ u, element_params prm,
-                                                              qp_state st, int tangent_mode,
+                                                              qp_state st, int tangent_mode, int rule,
                                                               uint32_t* __restrict__ status)
 {
     int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -135,11 +363,47 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
     st.detF[qp] = J;
     if (signbit(J)) atomicOr(status, STATUS_NEGATIVE_JACOBIAN);
 
+    if constexpr (is_microsphere(MODEL))
+    {
+        double const pe[4] = {prm.p[4 * e], prm.p[4 * e + 1], prm.p[4 * e + 2], prm.p[4 * e + 3]};
+        double sig[6], D[kFullCoefs];
+        microsphere_update<MODEL>(F, J, pe, rule, tangent_mode == NEON_TANGENT_CONSISTENT, sig, D);
+        bool finite = true;
+        for (int k = 0; k < 6; ++k)
+        {
+            st.sigma[k * Q + qp] = sig[k];
+            finite = finite && isfinite(sig[k]);
+        }
+        for (int k = 0; k < kFullCoefs; ++k)
+        {
+            st.coef[k * Q + qp] = D[k];
+            finite = finite && isfinite(D[k]);
+        }
+        if (!finite) atomicOr(status, STATUS_NAN);
+        return;
+    }
     double const bulk = prm.p[4 * e], shear = prm.p[4 * e + 1];
     double const lambda = bulk - 2.0 * shear / 3.0;
     double sig[6];
     double cA, cB, cC = 0.0, nv[6] = {0, 0, 0, 0, 0, 0};
-    if constexpr (MODEL == NEON_MODEL_NEOHOOKE)
+    auto vonmises = [](const double* s) {
+        double const t = (s[0] + s[1] + s[2]) / 3.0;
+        double const d0 = s[0] - t, d1 = s[1] - t, d2 = s[2] - t;
+        double const nrm = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+        return sqrt(3.0 / 2.0) * sqrt(nrm);
+    };
+    if constexpr (MODEL == NEON_MODEL_LINEAR_ELASTIC)
+    {
+        // sigma = lambda tr(eps) I + 2 G eps, eps = sym(H); constant moduli
+        double const eps[6] = {H[0], H[4], H[8], 0.5 * (H[5] + H[7]), 0.5 * (H[2] + H[6]), 0.5 * (H[1] + H[3])};
+        double const tr = eps[0] + eps[1] + eps[2];
+        for (int k = 0; k < 6; ++k) sig[k] = 2.0 * shear * eps[k];
+        for (int k = 0; k < 3; ++k) sig[k] += lambda * tr;
+        st.vm[qp] = vonmises(sig);
+        cA = lambda;
+        cB = shear;
+    }
+    else if constexpr (MODEL == NEON_MODEL_NEOHOOKE)
     {
         // sigma = (lambda ln J I + mu (F F^T - I)) / J ; D: lambda, mu' = mu - lambda ln J
         double const lnJ = log(J);
@@ -183,12 +447,6 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
         double const tr = ee[0] + ee[1] + ee[2];
         for (int k = 0; k < 6; ++k) sig[k] = 2.0 * G * ee[k];
         for (int k = 0; k < 3; ++k) sig[k] += lambda * tr;
-        auto vonmises = [](const double* s) {
-            double const t = (s[0] + s[1] + s[2]) / 3.0;
-            double const d0 = s[0] - t, d1 = s[1] - t, d2 = s[2] - t;
-            double const nrm = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
-            return sqrt(3.0 / 2.0) * sqrt(nrm);
-        };
         double vm = vonmises(sig);
         cA = lambda;
         cB = G;
@@ -249,14 +507,18 @@ __global__ __launch_bounds__(kBlock) void history_copy_kernel(int64_t n, const d
 // Expand the stored coefficients to the 6x6 Voigt tangent (test download).
 __global__ __launch_bounds__(kBlock) void expand_tangent_kernel(int64_t Q, const double* __restrict__ coef,
                                                                 const double* __restrict__ sigma,
-                                                                double* __restrict__ D, double* __restrict__ s9)
+                                                                double* __restrict__ D, double* __restrict__ s9,
+                                                                bool full)
 {
     int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (qp >= Q) return;
+    if (D && full)
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) D[36 * qp + 6 * i + j] = coef[int64_t(upper_index(i, j)) * Q + qp];
     double const A = coef[COEF_A * Q + qp], B = coef[COEF_B * Q + qp], C = coef[COEF_C * Q + qp];
     double n[6];
     for (int k = 0; k < 6; ++k) n[k] = coef[(COEF_N + k) * Q + qp];
-    if (D)
+    if (D && !full)
         for (int i = 0; i < 6; ++i)
             for (int j = 0; j < 6; ++j)
             {
@@ -295,7 +557,10 @@ __global__ __launch_bounds__(kBlock) void expand_tangent_kernel(int64_t Q, const
 //     (submesh.cpp:136-160, 110-134); the 9 values are added into CSR through the
 //     element's slot table.
 //   phase C, lanes (a, i<3): f_a,i = sum_l (sigma g_a)_i w det J (submesh.cpp:82-108).
-template <typename KT, bool FINITE>
+// FULL: a general symmetric D (21 stored coefficients, microsphere models); phase B then forms
+//   K_ab = sum_l B_a^T D B_b (+ the geometric term) from the engineering-shear columns of B:
+//   column j of B_b has g_b,j in row j and g_b in two shear rows (kShearRow / kShearComp).
+template <typename KT, bool FINITE, bool FULL = false>
 __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict__ elems, int32_t count,
                                                       const int32_t* __restrict__ conn,
                                                       const double* __restrict__ X,
@@ -304,9 +569,14 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
                                                       const int32_t* __restrict__ bptr, KT* __restrict__ K,
                                                       double* __restrict__ f_int)
 {
+    // coefficients per QP: A, Bt, C, n(6) (n unscaled) or the 21 of a general D; then sigma(6),
+    // all scaled by w detJ
+    constexpr int NCOEF = FULL ? kFullCoefs : 9;
+    constexpr int SIG = NCOEF;
+    constexpr int NC = NCOEF + 6;
     __shared__ double s_x[8][3];
     __shared__ double s_g[8][8][3]; // [l][a][k]
-    __shared__ double s_c[8][16];   // [l]: A, Bt, C, n(6), sigma(6) scaled by w detJ (n unscaled)
+    __shared__ double s_c[8][NC + 1];
     __shared__ int32_t s_node[8];
 
     int const lane = threadIdx.x;
@@ -342,15 +612,15 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
                 s_g[l][a][k] = c_hex8.dN[l][a][0] * Ji[0 * 3 + k] + c_hex8.dN[l][a][1] * Ji[1 * 3 + k]
                                + c_hex8.dN[l][a][2] * Ji[2 * 3 + k];
             int64_t const qp = 8 * e + l;
-            // coefficient slots: a -> 0..7, a+8 -> 8..14
-            for (int c = a; c < 15; c += 8)
+            // coefficient slots c = a, a + 8, ...
+            for (int c = a; c < NC; c += 8)
             {
                 double v;
-                if (c < 9)
+                if (c < NCOEF)
                     v = st.coef[int64_t(c) * Q + qp];
                 else
-                    v = st.sigma[int64_t(c - 9) * Q + qp];
-                if (c < 3 || c >= 9) v *= wdet;
+                    v = st.sigma[int64_t(c - SIG) * Q + qp];
+                if (FULL || c < 3 || c >= SIG) v *= wdet;
                 s_c[l][c] = v;
             }
         }
@@ -366,19 +636,50 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
             {
                 double const ga0 = s_g[l][a][0], ga1 = s_g[l][a][1], ga2 = s_g[l][a][2];
                 double const gb0 = s_g[l][b][0], gb1 = s_g[l][b][1], gb2 = s_g[l][b][2];
-                double const A = s_c[l][0], Bt = s_c[l][1], C = s_c[l][2];
-                double const gab = ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
-                double diag = Bt * gab;
+                double geo = 0.0;
                 if constexpr (FINITE)
                 {
                     // g_a^T sigma g_b, sigma (11,22,33,23,13,12) pre-scaled
-                    double const s0 = s_c[l][9], s1 = s_c[l][10], s2 = s_c[l][11], s3 = s_c[l][12],
-                                 s4 = s_c[l][13], s5 = s_c[l][14];
+                    double const s0 = s_c[l][SIG], s1 = s_c[l][SIG + 1], s2 = s_c[l][SIG + 2],
+                                 s3 = s_c[l][SIG + 3], s4 = s_c[l][SIG + 4], s5 = s_c[l][SIG + 5];
                     double const t0 = s0 * gb0 + s5 * gb1 + s4 * gb2;
                     double const t1 = s5 * gb0 + s1 * gb1 + s3 * gb2;
                     double const t2 = s4 * gb0 + s3 * gb1 + s2 * gb2;
-                    diag += ga0 * t0 + ga1 * t1 + ga2 * t2;
+                    geo = ga0 * t0 + ga1 * t1 + ga2 * t2;
                 }
+                if constexpr (FULL)
+                {
+                    // rows of B's column j: j (normal) and the two shear rows kRow[j][0..1], whose
+                    // entries are g components kComp[j][0..1]
+                    constexpr int kRow[3][2] = {{4, 5}, {3, 5}, {3, 4}};
+                    constexpr int kComp[3][2] = {{2, 1}, {2, 0}, {1, 0}};
+                    double const ga[3] = {ga0, ga1, ga2}, gb[3] = {gb0, gb1, gb2};
+                    double Dm[6][6];
+#pragma unroll
+                    for (int i = 0; i < 6; ++i)
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) Dm[i][j] = s_c[l][upper_index(i, j)];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                    {
+                        double db[6]; // D B_b[:, j]
+#pragma unroll
+                        for (int I = 0; I < 6; ++I)
+                            db[I] = Dm[I][j] * gb[j] + Dm[I][kRow[j][0]] * gb[kComp[j][0]]
+                                    + Dm[I][kRow[j][1]] * gb[kComp[j][1]];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+                        {
+                            double v = ga[i] * db[i] + ga[kComp[i][0]] * db[kRow[i][0]] + ga[kComp[i][1]] * db[kRow[i][1]];
+                            if (i == j) v += geo;
+                            k[i * 3 + j] += v;
+                        }
+                    }
+                    continue;
+                }
+                double const A = s_c[l][0], Bt = s_c[l][1], C = s_c[l][2];
+                double const gab = ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
+                double const diag = Bt * gab + geo;
                 double const ga[3] = {ga0, ga1, ga2}, gb[3] = {gb0, gb1, gb2};
                 double na[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
                 if (C != 0.0)
@@ -425,8 +726,8 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
 #pragma unroll 1
             for (int l = 0; l < 8; ++l)
             {
-                double const s0 = s_c[l][9], s1 = s_c[l][10], s2 = s_c[l][11], s3 = s_c[l][12], s4 = s_c[l][13],
-                             s5 = s_c[l][14];
+                double const s0 = s_c[l][SIG], s1 = s_c[l][SIG + 1], s2 = s_c[l][SIG + 2], s3 = s_c[l][SIG + 3],
+                             s4 = s_c[l][SIG + 4], s5 = s_c[l][SIG + 5];
                 double const g0 = s_g[l][a][0], g1 = s_g[l][a][1], g2 = s_g[l][a][2];
                 double v;
                 if (i == 0)

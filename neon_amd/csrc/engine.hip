@@ -661,18 +661,102 @@ void upload_hex8_tables()
     done |= bit;
 }
 
+// The Bazant-Oh (1986) unit-sphere rules of unit_sphere_quadrature.cpp:10-90 with their published
+// 12-digit direction cosines and weights, generated from the symmetry orbits in the reference's
+// order: axes, (v, +-v, 0) pairs, (a, b, 0) and (a, a, b) families.
+static void sphere_rules(sphere_tables& t)
+{
+    auto build = [&](int k, auto&& orbits) {
+        int& n = t.n[k];
+        n = 0;
+        auto add = [&](double w, double x, double y, double z) {
+            t.w[k][n] = w;
+            t.r[k][n][0] = x;
+            t.r[k][n][1] = y;
+            t.r[k][n][2] = z;
+            ++n;
+        };
+        orbits(add);
+    };
+    static const int pair[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    auto axes = [](auto& add, double w) {
+        add(w, 1.0, 0.0, 0.0);
+        add(w, 0.0, 1.0, 0.0);
+        add(w, 0.0, 0.0, 1.0);
+    };
+    auto pairs = [](auto& add, double w, double v) {
+        for (auto const& p : pair)
+            for (double sg : {1.0, -1.0})
+            {
+                double r[3] = {0, 0, 0};
+                r[p[0]] = v;
+                r[p[1]] = sg * v;
+                add(w, r[0], r[1], r[2]);
+            }
+    };
+    auto ab0 = [](auto& add, double w, double a, double b) {
+        for (auto const& p : pair)
+            for (int k = 0; k < 4; ++k)
+            {
+                double r[3] = {0, 0, 0};
+                r[p[0]] = k < 2 ? a : b;
+                r[p[1]] = (k < 2 ? b : a) * (k % 2 ? -1.0 : 1.0);
+                add(w, r[0], r[1], r[2]);
+            }
+    };
+    auto aab = [](auto& add, double w, double a, double b) {
+        for (int pos : {2, 1, 0})
+            for (int k = 0; k < 4; ++k)
+            {
+                double r[3] = {a, a, a};
+                r[pos] = b;
+                if (k >= 2) r[1] = -r[1];
+                if (k % 2) r[2] = -r[2];
+                add(w, r[0], r[1], r[2]);
+            }
+    };
+    build(0, [&](auto& add) { // BO21
+        axes(add, 0.0265214244093 * 2.0);
+        pairs(add, 0.0199301476312 * 2.0, 0.707106781187);
+        aab(add, 0.0250712367487 * 2.0, 0.387907304067, 0.836095596749);
+    });
+    build(1, [&](auto& add) { // BO33
+        axes(add, 2.0 * 0.00985353993433);
+        pairs(add, 2.0 * 0.0162969685886, 0.707106781187);
+        ab0(add, 2.0 * 0.0134788844008, 0.933898956394, 0.357537045978);
+        aab(add, 2.0 * 0.0175759129880, 0.437263676092, 0.785875915868);
+    });
+}
+
+static void upload_sphere_tables()
+{
+    static uint64_t done = 0;
+    int dev = 0;
+    NEON_CHECK_HIP(hipGetDevice(&dev));
+    uint64_t const bit = dev < 64 ? uint64_t(1) << dev : 0;
+    if (done & bit) return;
+    sphere_tables t{};
+    sphere_rules(t);
+    NEON_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_sphere), &t, sizeof(t)));
+    done |= bit;
+}
+
 device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
                          int32_t model, const double* params, int32_t tangent_mode)
     : s_(s), N_(n_nodes), E_(n_elem), model_(model), tangent_mode_(tangent_mode)
 {
     if (n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params)
         throw status_error(NEON_BAD_ARGUMENT, "empty mesh or null pointer");
-    if (model != NEON_MODEL_NEOHOOKE && model != NEON_MODEL_J2) throw status_error(NEON_BAD_ARGUMENT, "unknown model");
+    if ((model & ~(0xff | NEON_SPHERE_BO33)) != 0 || (model & 0xff) > NEON_MODEL_MICROSPHERE_NONAFFINE)
+        throw status_error(NEON_BAD_ARGUMENT, "unknown model");
+    if ((model & NEON_SPHERE_BO33) && !is_microsphere(model))
+        throw status_error(NEON_BAD_ARGUMENT, "a unit-sphere rule is only for the microsphere models");
     if (tangent_mode != NEON_TANGENT_REFERENCE && tangent_mode != NEON_TANGENT_CONSISTENT)
         throw status_error(NEON_BAD_ARGUMENT, "unknown tangent mode");
     for (int64_t i = 0; i < 8ll * n_elem; ++i)
         if (conn[i] < 0 || conn[i] >= n_nodes) throw status_error(NEON_BAD_ARGUMENT, "connectivity out of range");
     upload_hex8_tables();
+    if (is_microsphere(model)) upload_sphere_tables();
 
     // node -> element incidence
     std::vector<int32_t> inc_ptr(size_t(N_) + 1, 0), inc;
@@ -854,21 +938,25 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
     NEON_CHECK_HIP(hipMemcpyAsync(slots_.data(), slots.data(), slots.size(), hipMemcpyHostToDevice, s_));
 
     int64_t const Q = 8ll * E_;
+    int64_t const ncoef = is_microsphere(model_) ? kFullCoefs : 9;
     sigma_.resize(6 * Q);
-    coef_.resize(9 * Q);
+    coef_.resize(ncoef * Q);
     detF_.resize(Q);
     NEON_CHECK_HIP(hipMemsetAsync(sigma_.data(), 0, 6 * Q * sizeof(double), s_));
-    NEON_CHECK_HIP(hipMemsetAsync(coef_.data(), 0, 9 * Q * sizeof(double), s_));
+    NEON_CHECK_HIP(hipMemsetAsync(coef_.data(), 0, ncoef * Q * sizeof(double), s_));
+    if (model_ == NEON_MODEL_J2 || model_ == NEON_MODEL_LINEAR_ELASTIC)
+    {
+        vm_.resize(Q);
+        NEON_CHECK_HIP(hipMemsetAsync(vm_.data(), 0, Q * sizeof(double), s_));
+    }
     if (model_ == NEON_MODEL_J2)
     {
         eps_p_.resize(6 * Q);
         ebar_.resize(Q);
-        vm_.resize(Q);
         eps_p_old_.resize(6 * Q);
         ebar_old_.resize(Q);
         NEON_CHECK_HIP(hipMemsetAsync(eps_p_.data(), 0, 6 * Q * sizeof(double), s_));
         NEON_CHECK_HIP(hipMemsetAsync(ebar_.data(), 0, Q * sizeof(double), s_));
-        NEON_CHECK_HIP(hipMemsetAsync(vm_.data(), 0, Q * sizeof(double), s_));
         NEON_CHECK_HIP(hipMemsetAsync(eps_p_old_.data(), 0, 6 * Q * sizeof(double), s_));
         NEON_CHECK_HIP(hipMemsetAsync(ebar_old_.data(), 0, Q * sizeof(double), s_));
     }
@@ -896,12 +984,20 @@ int device_mesh::update_internal_variables(const double* u_dev)
                 eps_p_old_.data(), ebar_old_.data()};
     NEON_CHECK_HIP(hipMemsetAsync(status_.data(), 0, sizeof(uint32_t), s_));
     element_params prm{params_.data()};
-    if (model_ == NEON_MODEL_NEOHOOKE)
-        update_state_kernel<NEON_MODEL_NEOHOOKE><<<grid_for(Q), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_dev, prm, st,
-                                                                                  tangent_mode_, status_.data());
-    else
-        update_state_kernel<NEON_MODEL_J2><<<grid_for(Q), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_dev, prm, st,
-                                                                            tangent_mode_, status_.data());
+    int const rule = (model_ & NEON_SPHERE_BO33) ? 1 : 0;
+#define NEON_UPDATE(M)                                                                                             \
+    update_state_kernel<M><<<grid_for(Q), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_dev, prm, st, tangent_mode_, \
+                                                          rule, status_.data())
+    switch (model_ & 0xff)
+    {
+    case NEON_MODEL_NEOHOOKE: NEON_UPDATE(NEON_MODEL_NEOHOOKE); break;
+    case NEON_MODEL_J2: NEON_UPDATE(NEON_MODEL_J2); break;
+    case NEON_MODEL_LINEAR_ELASTIC: NEON_UPDATE(NEON_MODEL_LINEAR_ELASTIC); break;
+    case NEON_MODEL_MICROSPHERE_GAUSSIAN: NEON_UPDATE(NEON_MODEL_MICROSPHERE_GAUSSIAN); break;
+    case NEON_MODEL_MICROSPHERE_LANGEVIN: NEON_UPDATE(NEON_MODEL_MICROSPHERE_LANGEVIN); break;
+    default: NEON_UPDATE(NEON_MODEL_MICROSPHERE_NONAFFINE); break;
+    }
+#undef NEON_UPDATE
     NEON_CHECK_HIP(hipGetLastError());
     NEON_CHECK_HIP(hipMemcpyAsync(status_host_, status_.data(), sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
@@ -937,10 +1033,11 @@ void device_mesh::assemble(const double* u_dev, int precision)
     int64_t const Q = 8ll * E_;
     qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q,
                 eps_p_old_.data(), ebar_old_.data()};
-    bool const finite = model_ == NEON_MODEL_NEOHOOKE; // is_finite_deformation()
-    // neon integrates the small-strain model on the current configuration (submesh.cpp:68,84);
-    // the consistent mode integrates it on X
-    if (model_ == NEON_MODEL_J2 && tangent_mode_ == NEON_TANGENT_CONSISTENT) u_dev = nullptr;
+    bool const finite = is_finite_model(model_); // is_finite_deformation()
+    bool const full = is_microsphere(model_);
+    // neon integrates the small-strain models on the current configuration (submesh.cpp:68,84);
+    // the consistent mode integrates them on X
+    if (!finite && tangent_mode_ == NEON_TANGENT_CONSISTENT) u_dev = nullptr;
     hipEvent_t e0, e1;
     NEON_CHECK_HIP(hipEventCreate(&e0));
     NEON_CHECK_HIP(hipEventCreate(&e1));
@@ -966,24 +1063,28 @@ void device_mesh::assemble(const double* u_dev, int precision)
         if (cnt == 0) continue;
         int const blocks = int(std::min<int64_t>(cnt, int64_t(cus) * assembly_blocks_per_cu));
         const int32_t* el = elems_.data() + off;
+#define NEON_ASSEMBLE(KT, FIN, FULL, K)                                                                       \
+    assemble_kernel<KT, FIN, FULL><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st, slots_.data(), \
+                                                          bptr_.data(), K, fint_.data())
         if (precision == NEON_PRECISION_FP64)
         {
-            if (finite)
-                assemble_kernel<double, true><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
-                                                                      slots_.data(), bptr_.data(), k64_.data(), fint_.data());
+            if (full)
+                NEON_ASSEMBLE(double, true, true, k64_.data());
+            else if (finite)
+                NEON_ASSEMBLE(double, true, false, k64_.data());
             else
-                assemble_kernel<double, false><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
-                                                                       slots_.data(), bptr_.data(), k64_.data(), fint_.data());
+                NEON_ASSEMBLE(double, false, false, k64_.data());
         }
         else
         {
-            if (finite)
-                assemble_kernel<float, true><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
-                                                                     slots_.data(), bptr_.data(), k32_.data(), fint_.data());
+            if (full)
+                NEON_ASSEMBLE(float, true, true, k32_.data());
+            else if (finite)
+                NEON_ASSEMBLE(float, true, false, k32_.data());
             else
-                assemble_kernel<float, false><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st,
-                                                                      slots_.data(), bptr_.data(), k32_.data(), fint_.data());
+                NEON_ASSEMBLE(float, false, false, k32_.data());
         }
+#undef NEON_ASSEMBLE
     }
     NEON_CHECK_HIP(hipGetLastError());
     NEON_CHECK_HIP(hipEventRecord(e1, s_));
@@ -1066,7 +1167,7 @@ void device_mesh::download_qp_state(double* sigma, double* D, double* detF, doub
         if (D) d36.resize(36 * Q);
         if (sigma) s9.resize(9 * Q);
         expand_tangent_kernel<<<grid_for(Q), kBlock, 0, s_>>>(Q, coef_.data(), sigma_.data(), D ? d36.data() : nullptr,
-                                                               sigma ? s9.data() : nullptr);
+                                                               sigma ? s9.data() : nullptr, is_microsphere(model_));
         NEON_CHECK_HIP(hipGetLastError());
         if (D) NEON_CHECK_HIP(hipMemcpyAsync(D, d36.data(), 36 * Q * sizeof(double), hipMemcpyDeviceToHost, s_));
         if (sigma) NEON_CHECK_HIP(hipMemcpyAsync(sigma, s9.data(), 9 * Q * sizeof(double), hipMemcpyDeviceToHost, s_));
@@ -1098,7 +1199,13 @@ void device_mesh::download_qp_state(double* sigma, double* D, double* detF, doub
     {
         if (eps_p) std::fill(eps_p, eps_p + 9 * Q, 0.0);
         if (ebar) std::fill(ebar, ebar + Q, 0.0);
-        if (vm) std::fill(vm, vm + Q, 0.0);
+        if (vm)
+        {
+            if (model_ == NEON_MODEL_LINEAR_ELASTIC)
+                NEON_CHECK_HIP(hipMemcpyAsync(vm, vm_.data(), Q * sizeof(double), hipMemcpyDeviceToHost, s_));
+            else
+                std::fill(vm, vm + Q, 0.0);
+        }
     }
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
 }

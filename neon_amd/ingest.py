@@ -15,7 +15,10 @@ keep gmsh's node order (node_ordering_adapter.cpp:46-136 reorders neither).
   modulus and Poisson's ratio (ratio < 0.5); J2 also ``yield_stress`` and the optional
   ``isotropic_hardening_modulus`` (isotropic_elastic_plastic.cpp:10-36);
 * constitutive models (constitutive_model_factory.cpp:37-160): ``neohooke`` /
-  ``compressible_neohooke`` and ``J2_plasticity`` with ``finite_strain: false``;
+  ``compressible_neohooke``, ``isotropic_linear_elasticity``, ``J2_plasticity`` with
+  ``finite_strain: false``, and ``microsphere`` (affine with gaussian or langevin statistics,
+  nonaffine; quadrature BO21 or BO33) with ``segments_per_chain`` (and
+  ``nonaffine_stretch_parameter``) in the material (micromechanical_elastomer.cpp:12-20);
 * boundaries (mesh.cpp:85-143, nonfollower_load.cpp:12-122): ``displacement`` per component
   key x / y / z on the unique nodes of the named groups; ``traction`` (per component) and
   ``pressure`` (``value``) on quadrilateral faces and ``body_force`` (per component) on hex8
@@ -53,7 +56,7 @@ class Problem:
     X: np.ndarray          # (N, 3) coordinates
     conn: np.ndarray       # (E, 8) hex8 connectivity
     params: np.ndarray     # (E, 4): bulk, shear, yield stress, hardening
-    model: int             # MODEL_NEOHOOKE or MODEL_J2
+    model: int             # MODEL_* (with SPHERE_BO33 for the 33-direction microsphere rule)
     boundaries: list       # Dirichlet: {name, dofs, times, values}
     loads: list            # nonfollower: {name, dofs, values, times, loads[, fixed_time]}
     simulation: dict       # static_matrix options
@@ -100,23 +103,74 @@ def _material_params(material: dict, model: int) -> np.ndarray:
             raise ValueError('"yield_stress" was not specified in material properties')
         sy = float(material["yield_stress"])
         H = float(material.get("isotropic_hardening_modulus", 0.0))
+    elif (model & 0xFF) >= _capi.MODEL_MICROSPHERE_GAUSSIAN:
+        # micromechanical_elastomer.cpp:12-20, nonaffine_microsphere.cpp:14-24
+        if "segments_per_chain" not in material:
+            raise ValueError("segments_per_chain not specified in material data\n")
+        sy = float(material["segments_per_chain"])
+        if (model & 0xFF) == _capi.MODEL_MICROSPHERE_NONAFFINE:
+            if "nonaffine_stretch_parameter" not in material:
+                raise ValueError('"nonaffine_stretch_parameter" not specified in material data\n')
+            H = float(material["nonaffine_stretch_parameter"])
     return np.array([K, G, sy, H])
+
+
+def _microsphere(c: dict) -> int:
+    """The microsphere branch of constitutive_model_factory.cpp:48-121, with its messages."""
+    if "type" not in c:
+        raise ValueError('Missing "type" as "affine" or "nonaffine" in microsphere model')
+    if "quadrature" not in c:
+        raise ValueError('Missing "quadrature" as "BO21", "BO33" or "FM900" in microsphere model')
+    rules = {"BO21": 0, "BO33": _capi.SPHERE_BO33, "FM900": None}
+    if c["quadrature"] not in rules:
+        raise ValueError('"quadrature" field must be "BO21", "BO33" or "FM900"')
+    rule = rules[c["quadrature"]]
+    if c["type"] == "affine":
+        if "statistics" not in c:
+            raise ValueError('Missing "statistics" as "gaussian" or "langevin" in the microsphere model')
+        if c["statistics"] not in ("gaussian", "langevin"):
+            raise ValueError('"statistics" for the microsphere model must be either "gaussian" or "langevin"')
+        if c["statistics"] == "gaussian" and "ageing" in c:
+            if c["ageing"] != "BAND":
+                raise ValueError('The only microsphere ageing model supported is "BAND"')
+            raise ValueError("the ageing microsphere model does not run on the device path")
+        model = _capi.MODEL_MICROSPHERE_GAUSSIAN if c["statistics"] == "gaussian" else _capi.MODEL_MICROSPHERE_LANGEVIN
+    elif c["type"] == "nonaffine":
+        model = _capi.MODEL_MICROSPHERE_NONAFFINE
+    else:
+        raise ValueError('microsphere model options are "affine" or "nonaffine"')
+    if rule is None:
+        raise ValueError("the FM900 unit-sphere rule does not run on the device path (BO21 and BO33 do)")
+    return model | rule
 
 
 def _model(constitutive: dict) -> int:
     """constitutive_model_factory.cpp:37-160, the models of the device path."""
     if "name" not in constitutive:
-        raise ValueError('"constitutive" needs a "name"')
+        raise ValueError('Missing "name" in "constitutive"')
     name = constitutive["name"]
     if name in ("neohooke", "compressible_neohooke"):
         return _capi.MODEL_NEOHOOKE
+    if name == "microsphere":
+        return _microsphere(constitutive)
+    if name == "isotropic_linear_elasticity":
+        return _capi.MODEL_LINEAR_ELASTIC
     if name == "J2_plasticity":
         if "finite_strain" not in constitutive:
             raise ValueError('"small_strain_J2_plasticity" must have a boolean value for "finite_strain"')
-        if constitutive["finite_strain"] or "damage" in constitutive:
-            raise ValueError("only the small-strain J2 model without damage runs on the device path")
+        if "damage" in constitutive:
+            if constitutive["finite_strain"]:
+                raise ValueError('"damage" is not yet implemented for "finite_strain"')
+            raise ValueError("the J2 damage model does not run on the device path")
+        if constitutive["finite_strain"]:
+            # finite_strain_J2_plasticity::consistent_tangent receives the logarithmic strain
+            # where it expects the trial left Cauchy-Green tensor (finite_strain_J2_plasticity.cpp:
+            # 101,139), so its tangent vanishes at the undeformed state
+            raise ValueError("the finite-strain J2 model does not run on the device path")
         return _capi.MODEL_J2
-    raise ValueError(f'constitutive model "{name}" does not run on the device path')
+    raise ValueError(f"The model name {name} is not recognised\n"
+                     'Supported models are "neohooke", "microsphere" "isotropic_linear_elasticity" and '
+                     '"J2_plasticity"\n')
 
 
 def _read_mesh(path: str):

@@ -11,7 +11,8 @@ variables extrapolated from the 8 quadrature points to the element's nodes with 
 the elements sharing a node (node_averaged_variables.hpp:15-40, submesh.cpp:271-344):
 ``cauchy_stress``, ``deformation_gradient``, ``displacement_gradient`` (9 components, row
 major), ``DetF`` for every model; ``linearised_strain``, ``linearised_plastic_strain``,
-``von_mises_stress``, ``effective_plastic_strain`` for small-strain J2.  The deformation
+``von_mises_stress``, ``effective_plastic_strain`` for small-strain J2; ``linearised_strain``
+and ``von_mises_stress`` for linear elasticity.  The deformation
 measures are recomputed on the host from u as submesh::update_deformation_measures does
 (submesh.cpp:210-237: H = (x - X) dN (X dN)^-1, F = (x dN)(X dN)^-1).
 
@@ -38,6 +39,11 @@ _POINTS = _GAUSS * _NODES
 NODAL = ("displacement", "reaction_force")
 COMMON = ("cauchy_stress", "deformation_gradient", "displacement_gradient", "DetF")
 J2_ONLY = ("linearised_strain", "linearised_plastic_strain", "von_mises_stress", "effective_plastic_strain")
+# the variable names each model adds (constitutive_model::variable_names): small-strain J2
+# (small_strain_J2_plasticity.cpp:14-26 plus those of isotropic_linear_elasticity.cpp:15-19), linear
+# elasticity (isotropic_linear_elasticity.cpp:15-19);
+# the microsphere models add none
+MODEL_VARIABLES = {_capi.MODEL_J2: J2_ONLY, _capi.MODEL_LINEAR_ELASTIC: ("linearised_strain", "von_mises_stress")}
 
 
 def shape_at_points():
@@ -102,7 +108,7 @@ class vtk_file_output:
         if not file_name:
             raise ValueError('"name" field for file IO must be specified')
         self.fields = list(dict.fromkeys(visualisation.get("fields", [])))
-        allowed = NODAL + COMMON + (J2_ONLY if model == _capi.MODEL_J2 else ())
+        allowed = NODAL + COMMON + MODEL_VARIABLES.get(model & 0xFF, ())
         for f in self.fields:
             if f not in NODAL + COMMON + J2_ONLY:
                 raise ValueError(f"Name {f} is not a valid output variable name")

@@ -224,6 +224,306 @@ int j2(const double* H, double bulk, double shear, double sy, double Hh, double*
         for (int j = 0; j < 6; ++j) D[i * 6 + j] = C[i * 6 + j] - c1 * Idev[i * 6 + j] + c2 * nv[i] * nv[j];
     return ORC_OK;
 }
+// isotropic_linear_elasticity::update_internal_variables
+// (src/constitutive/mechanics/solid/isotropic_linear_elasticity.cpp:27-53): sigma = C : sym(H)
+// with the constant moduli of elastic_moduli (:55-67), and the von Mises stress.
+void linear_elastic(const double* H, double bulk, double shear, double* sigma, double* D, double* vm)
+{
+    double const lambda = bulk - 2.0 * shear / 3.0;
+    double e[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) e[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i]);
+    double const tr = e[0] + e[4] + e[8];
+    for (int i = 0; i < 9; ++i) sigma[i] = 2.0 * shear * e[i];
+    sigma[0] += lambda * tr;
+    sigma[4] += lambda * tr;
+    sigma[8] += lambda * tr;
+    *vm = von_mises(sigma);
+    elastic_moduli(bulk, shear, D);
+}
+
+// ------------------------------------------------------------ microsphere models
+// unit_sphere_quadrature (src/quadrature/sphere/unit_sphere_quadrature.cpp:10-90): the
+// Bazant-Oh (1986) rules with 21 and 33 directions on the half sphere (antipodal symmetry),
+// weights summing to one.  The published direction cosines and weights (12 digits) are
+// used as published, generated here from their symmetry orbits in the same order.
+struct sphere_rule
+{
+    int n = 0;
+    double w[33];
+    double r[33][3];
+    void add(double wt, double a, double b, double c)
+    {
+        w[n] = wt;
+        r[n][0] = a;
+        r[n][1] = b;
+        r[n][2] = c;
+        ++n;
+    }
+    void axes(double wt)
+    {
+        add(wt, 1.0, 0.0, 0.0);
+        add(wt, 0.0, 1.0, 0.0);
+        add(wt, 0.0, 0.0, 1.0);
+    }
+    // (v, +-v, 0) over the coordinate pairs (x,y), (x,z), (y,z)
+    void pairs(double wt, double v)
+    {
+        static const int ij[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (auto const& p : ij)
+            for (double s : {1.0, -1.0})
+            {
+                double t[3] = {0, 0, 0};
+                t[p[0]] = v;
+                t[p[1]] = s * v;
+                add(wt, t[0], t[1], t[2]);
+            }
+    }
+    // (a, b, 0) type: for the zero in z, y, x: (a, b), (a, -b), (b, a), (b, -a)
+    void ab0(double wt, double a, double b)
+    {
+        static const int ij[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (auto const& p : ij)
+            for (int k = 0; k < 4; ++k)
+            {
+                double t[3] = {0, 0, 0};
+                t[p[0]] = k < 2 ? a : b;
+                t[p[1]] = (k < 2 ? b : a) * (k % 2 ? -1.0 : 1.0);
+                add(wt, t[0], t[1], t[2]);
+            }
+    }
+    // (a, a, b) type: b in the z, y, x slot; signs (+,+), (+,-), (-,+), (-,-) on y and z
+    void aab(double wt, double a, double b)
+    {
+        for (int pos : {2, 1, 0})
+            for (int k = 0; k < 4; ++k)
+            {
+                double t[3] = {a, a, a};
+                t[pos] = b;
+                if (k >= 2) t[1] = -t[1];
+                if (k % 2) t[2] = -t[2];
+                add(wt, t[0], t[1], t[2]);
+            }
+    }
+};
+
+const sphere_rule& sphere(int which)
+{
+    static const sphere_rule bo21 = [] {
+        sphere_rule q;
+        q.axes(0.0265214244093 * 2.0);
+        q.pairs(0.0199301476312 * 2.0, 0.707106781187);
+        q.aab(0.0250712367487 * 2.0, 0.387907304067, 0.836095596749);
+        return q;
+    }();
+    static const sphere_rule bo33 = [] {
+        sphere_rule q;
+        q.axes(2.0 * 0.00985353993433);
+        q.pairs(2.0 * 0.0162969685886, 0.707106781187);
+        q.ab0(2.0 * 0.0134788844008, 0.933898956394, 0.357537045978);
+        q.aab(2.0 * 0.0175759129880, 0.437263676092, 0.785875915868);
+        return q;
+    }();
+    return which == ORC_SPHERE_BO33 ? bo33 : bo21;
+}
+
+// kinetic Voigt vector (11, 22, 33, 23, 13, 12) of a 3x3 tensor (tensor_operations.hpp:373-376)
+inline void kinetic6(const double* a, double* v)
+{
+    v[0] = a[0];
+    v[1] = a[4];
+    v[2] = a[8];
+    v[3] = a[5];
+    v[4] = a[2];
+    v[5] = a[1];
+}
+// outer_product(matrix3 a, matrix3 b) = kinetic(a) kinetic(b)^T (tensor_operations.hpp:480-483)
+inline void outer66(const double* a, const double* b, double* M)
+{
+    double va[6], vb[6];
+    kinetic6(a, va);
+    kinetic6(b, vb);
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) M[i * 6 + j] = va[i] * vb[j];
+}
+inline void matmul6(const double* A, const double* B, double* C)
+{
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+        {
+            double s = 0.0;
+            for (int k = 0; k < 6; ++k) s += A[i * 6 + k] * B[k * 6 + j];
+            C[i * 6 + j] = s;
+        }
+}
+inline void outer3(const double* t, const double* u, double* M)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[i * 3 + j] = t[i] * u[j];
+}
+// microsphere.hpp:37-65 (Pade approximations of the inverse Langevin function)
+inline double pade_first(double l, double N) { return (3.0 * N - l * l) / (N - l * l); }
+inline double pade_second(double l, double N) { return (std::pow(l, 4) + 3.0 * N * N) / std::pow(N - l * l, 2); }
+
+// Macro stress tau_bar (3x3) and macro moduli c_bar (6x6) of the affine and non-affine
+// microsphere models from the unimodular F_bar:
+//   gaussian_affine_microsphere::compute_macro_stress (gaussian_affine_microsphere.cpp:71-83),
+//     zero macro moduli (:41)
+//   affine_microsphere::compute_macro_stress / compute_macro_moduli
+//     (affine_microsphere.cpp:73-111)
+//   nonaffine_microsphere::update_internal_variables (nonaffine_microsphere.cpp:26-88) with
+//     compute_nonaffine_stretch, compute_h/H/k/K/G_tensor (:90-178), compute_o_dot_product
+//     (nonaffine_microsphere.hpp:111-121); tube parameters U = 9, q = 1 (hpp:104-106)
+// Each sphere sum adds f(r_i) * w_i in rule order (numerical_quadrature.hpp:47-55).
+void microsphere_macro(int base, const sphere_rule& q, const double* Fb, double G, double N, double p, double* tau,
+                       double* cb)
+{
+    std::fill(tau, tau + 9, 0.0);
+    std::fill(cb, cb + 36, 0.0);
+    auto tangent = [&](int i, double* t) { // deformed_tangent = F_bar r (microsphere.hpp:17-21)
+        for (int a = 0; a < 3; ++a) t[a] = Fb[a * 3] * q.r[i][0] + Fb[a * 3 + 1] * q.r[i][1] + Fb[a * 3 + 2] * q.r[i][2];
+    };
+    auto norm = [](const double* t) { return std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]); };
+    if (base == ORC_MODEL_MICROSPHERE_GAUSSIAN)
+    {
+        for (int i = 0; i < q.n; ++i)
+        {
+            double t[3], tt[9];
+            tangent(i, t);
+            outer3(t, t, tt);
+            for (int k = 0; k < 9; ++k) tau[k] += tt[k] * q.w[i];
+        }
+        for (int k = 0; k < 9; ++k) tau[k] *= 3.0 * G;
+        return;
+    }
+    if (base == ORC_MODEL_MICROSPHERE_LANGEVIN)
+    {
+        for (int i = 0; i < q.n; ++i)
+        {
+            double t[3], tt[9], tttt[36];
+            tangent(i, t);
+            double const l = norm(t);
+            outer3(t, t, tt);
+            outer66(tt, tt, tttt);
+            double const pf = pade_first(l, N);
+            double const a = std::pow(l, -2) * (pade_second(l, N) - pf);
+            for (int k = 0; k < 9; ++k) tau[k] += pf * tt[k] * q.w[i];
+            for (int k = 0; k < 36; ++k) cb[k] += a * tttt[k] * q.w[i];
+        }
+        for (int k = 0; k < 9; ++k) tau[k] *= G;
+        for (int k = 0; k < 36; ++k) cb[k] *= G;
+        return;
+    }
+    // non-affine: chain stretch lambda = [sum lambda_i^p w_i]^(1/p), tube area stretches nu_i
+    double Fbi[9], FbiT[9];
+    inv3(Fb, Fbi);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) FbiT[a * 3 + b] = Fbi[b * 3 + a];
+    double sum_p = 0.0, h[9] = {0}, H[36] = {0}, k3[9] = {0}, K6[36] = {0}, G6[36] = {0};
+    double const qt = 1.0, U = 9.0;
+    for (int i = 0; i < q.n; ++i)
+    {
+        double t[3], n[3], tt[9], nn[9], tttt[36], nnnn[36], o[36];
+        tangent(i, t);
+        for (int a = 0; a < 3; ++a)
+            n[a] = FbiT[a * 3] * q.r[i][0] + FbiT[a * 3 + 1] * q.r[i][1] + FbiT[a * 3 + 2] * q.r[i][2];
+        double const l = norm(t), nu = norm(n);
+        outer3(t, t, tt);
+        outer3(n, n, nn);
+        outer66(tt, tt, tttt);
+        outer66(nn, nn, nnnn);
+        sum_p += std::pow(l, p) * q.w[i];
+        for (int k = 0; k < 9; ++k) h[k] += std::pow(l, p - 2.0) * tt[k] * q.w[i];
+        for (int k = 0; k < 36; ++k) H[k] += std::pow(l, p - 4.0) * tttt[k] * q.w[i];
+        for (int k = 0; k < 9; ++k) k3[k] += std::pow(nu, qt - 2.0) * nn[k] * q.w[i];
+        for (int k = 0; k < 36; ++k) K6[k] += std::pow(nu, qt - 4.0) * nnnn[k] * q.w[i];
+        double const o_[36] = {2 * n[0] * n[0], 0, 0, 0, n[0] * n[2], n[0] * n[1],
+                               0, 2 * n[1] * n[1], 0, n[1] * n[2], 0, n[0] * n[1],
+                               0, 0, 2 * n[2] * n[2], n[1] * n[2], n[0] * n[2], 0,
+                               0, n[1] * n[2], n[1] * n[2], 0.5 * (n[1] * n[1] + n[2] * n[2]), 0.5 * n[0] * n[1], 0.5 * n[0] * n[2],
+                               n[0] * n[2], 0, n[0] * n[2], 0.5 * n[0] * n[1], 0.5 * (n[0] * n[0] + n[2] * n[2]), 0.5 * n[2] * n[1],
+                               n[0] * n[1], n[0] * n[1], 0, 0.5 * n[0] * n[2], 0.5 * n[2] * n[1], 0.5 * (n[0] * n[0] + n[1] * n[1])};
+        std::memcpy(o, o_, sizeof(o));
+        for (int k = 0; k < 36; ++k) G6[k] += std::pow(nu, qt - 2.0) * o[k] * q.w[i];
+    }
+    double const lna = std::pow(sum_p, 1.0 / p);
+    for (int k = 0; k < 36; ++k) H[k] *= p - 2.0;
+    for (int k = 0; k < 9; ++k) k3[k] *= qt;
+    for (int k = 0; k < 36; ++k) K6[k] *= qt * (qt - 2.0);
+    for (int k = 0; k < 36; ++k) G6[k] *= 2.0 * qt;
+    double const micro_kirchhoff = G * pade_first(lna, N) * lna;
+    double const micro_moduli = G * pade_second(lna, N);
+    double hh[36];
+    outer66(h, h, hh);
+    double const c_hh = micro_moduli * std::pow(lna, 2.0 - 2.0 * p) - (p - 1.0) * micro_kirchhoff * std::pow(lna, 1.0 - 2.0 * p);
+    double const c_f = micro_kirchhoff * std::pow(lna, 1.0 - p);
+    for (int k = 0; k < 9; ++k) tau[k] = c_f * h[k] - G * N * U * k3[k];
+    for (int k = 0; k < 36; ++k) cb[k] = (c_hh * hh[k] + c_f * H[k]) + G * N * U * (K6[k] + G6[k]);
+}
+
+// affine_microsphere::update_internal_variables / compute_kirchhoff_stress /
+// compute_material_tangent (affine_microsphere.cpp:23-71), shared by all three models:
+//   p = J U'(J), kappa = J^2 U''(J) with U = K/4 (J^2 - 1) - K/2 ln J
+//     (volumetric_free_energy.hpp:17-33);
+//   sigma = (p I + dev tau_bar) / J;
+//   D = (kappa + p) 1(x)1 - 2 p I + P [c_bar + 2/3 tr(tau_bar) I - 2/3 (tau_bar(x)1 + 1(x)tau_bar)] P
+//   with I = diag(1,1,1,1/2,1/2,1/2) and P the kinetic deviatoric projector.
+// The projections annihilate the last bracket term, so D lacks the -2/3 (dev tau_bar (x) 1 +
+// 1 (x) dev tau_bar) of the exact Kirchhoff tangent; the consistent mode adds it and divides
+// by J, which makes K = d f_int / d u.
+void microsphere(int model, const double* F, double J, const double* prm, int consistent, double* sigma, double* D)
+{
+    int const base = model & 0xff;
+    sphere_rule const& q = sphere(model & ORC_SPHERE_BO33);
+    double const K = prm[0], G = prm[1], N = prm[2], p = prm[3];
+    double Fb[9];
+    double const s = std::pow(det3(F), -1.0 / 3.0); // unimodular (tensor_operations.hpp:70-73)
+    for (int k = 0; k < 9; ++k) Fb[k] = s * F[k];
+    double tau[9], cb[36];
+    microsphere_macro(base, q, Fb, G, N, p, tau, cb);
+    double const pressure = J * (K / 2.0 * (J - 1.0 / J));
+    double const kappa = std::pow(J, 2) * (K / 2.0 * (1.0 + 1.0 / std::pow(J, 2)));
+    double dtau[9];
+    deviatoric3(tau, dtau);
+    for (int k = 0; k < 9; ++k) sigma[k] = ((k % 4 == 0 ? pressure : 0.0) + dtau[k]) / J;
+    static const double Ikin[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+    static const double Pdev[36] = {2.0 / 3.0, -1.0 / 3.0, -1.0 / 3.0, 0, 0, 0,
+                                    -1.0 / 3.0, 2.0 / 3.0, -1.0 / 3.0, 0, 0, 0,
+                                    -1.0 / 3.0, -1.0 / 3.0, 2.0 / 3.0, 0, 0, 0,
+                                    0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 1};
+    static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double tI[36], It[36], Dm[36], PD[36], PDP[36];
+    outer66(tau, I3, tI);
+    outer66(I3, tau, It);
+    double const trt = tau[0] + tau[4] + tau[8];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+            Dm[i * 6 + j] = cb[i * 6 + j] + (i == j ? 2.0 / 3.0 * trt * Ikin[i] : 0.0)
+                            - 2.0 / 3.0 * (tI[i * 6 + j] + It[i * 6 + j]);
+    matmul6(Pdev, Dm, PD);
+    matmul6(PD, Pdev, PDP);
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+            D[i * 6 + j] = (kappa + pressure) * (i < 3 && j < 3 ? 1.0 : 0.0) - 2.0 * pressure * (i == j ? Ikin[i] : 0.0)
+                           + PDP[i * 6 + j];
+    if (consistent)
+    {
+        double dI[36], Id[36];
+        outer66(dtau, I3, dI);
+        outer66(I3, dtau, Id);
+        for (int k = 0; k < 36; ++k) D[k] = (D[k] - 2.0 / 3.0 * (dI[k] + Id[k])) / J;
+    }
+}
+
+inline bool is_microsphere(int model)
+{
+    int const b = model & 0xff;
+    return b == ORC_MODEL_MICROSPHERE_GAUSSIAN || b == ORC_MODEL_MICROSPHERE_LANGEVIN
+           || b == ORC_MODEL_MICROSPHERE_NONAFFINE;
+}
+// constitutive_model::is_finite_deformation
+inline bool is_finite(int model) { return model == ORC_MODEL_NEOHOOKE || is_microsphere(model); }
+
 // The return map of j2() in consistent mode as a function of its inputs, templated on the
 // scalar so that complex-step differentiation gives its exact partial derivatives (adjoint
 // only).  Full 3x3 tensors; comparisons use real parts, so the complex evaluation takes the
@@ -372,6 +672,26 @@ int orc_neohooke_qp(const double* F, double bulk, double shear, double* sigma, d
     return ORC_OK;
 }
 
+int orc_microsphere_qp(int32_t model, const double* F, const double* params, int32_t consistent, double* sigma,
+                       double* D)
+{
+    if (!F || !params || !sigma || !D || !is_microsphere(model)) return ORC_BAD_ARGUMENT;
+    microsphere(model, F, det3(F), params, consistent, sigma, D);
+    return ORC_OK;
+}
+
+int orc_sphere_rule(int32_t model, double* w, double* r)
+{
+    sphere_rule const& q = sphere(model & ORC_SPHERE_BO33);
+    for (int i = 0; i < q.n; ++i)
+    {
+        if (w) w[i] = q.w[i];
+        if (r)
+            for (int a = 0; a < 3; ++a) r[3 * i + a] = q.r[i][a];
+    }
+    return q.n;
+}
+
 int orc_j2_qp(const double* H, double bulk, double shear, double yield, double hardening, double* eps_p,
               double* ebar, double* sigma, double* D, double* von_mises_out)
 {
@@ -382,7 +702,9 @@ int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int
                      const double* params, int32_t tangent_mode, orc_model** out)
 {
     if (!out || n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params) return ORC_BAD_ARGUMENT;
-    if (model != ORC_MODEL_NEOHOOKE && model != ORC_MODEL_J2) return ORC_BAD_ARGUMENT;
+    int const base = model & 0xff;
+    if ((model & ~(0xff | ORC_SPHERE_BO33)) != 0 || base > ORC_MODEL_MICROSPHERE_NONAFFINE) return ORC_BAD_ARGUMENT;
+    if ((model & ORC_SPHERE_BO33) && !is_microsphere(model)) return ORC_BAD_ARGUMENT;
     for (int64_t i = 0; i < int64_t(n_elem) * 8; ++i)
         if (conn[i] < 0 || conn[i] >= n_nodes) return ORC_BAD_ARGUMENT;
     auto* m = new orc_model;
@@ -404,7 +726,7 @@ int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int
     m->ebar.assign(Q, 0.0);
     m->vm.assign(Q, 0.0);
     m->D.assign(36 * Q, 0.0);
-    if (model == ORC_MODEL_J2) // isotropic_linear_elasticity.cpp:22-23
+    if (model == ORC_MODEL_J2 || model == ORC_MODEL_LINEAR_ELASTIC) // isotropic_linear_elasticity.cpp:22-23
         for (size_t q = 0; q < Q; ++q)
             elastic_moduli(m->params[4 * (q / 8)], m->params[4 * (q / 8) + 1], &m->D[36 * q]);
     m->Hg_o = m->Hg;
@@ -488,6 +810,22 @@ int orc_update_internal_variables(orc_model* m, const double* u, double /*dt*/)
                 for (int k = 0; k < 36; ++k) m->D[36 * q + k] /= m->detF[q];
         }
     }
+    else if (m->model == ORC_MODEL_LINEAR_ELASTIC)
+    {
+        for (size_t q = 0; q < Q; ++q)
+        {
+            double const* p = &m->params[4 * (q / 8)];
+            linear_elastic(&m->Hg[9 * q], p[0], p[1], &m->sigma[9 * q], &m->D[36 * q], &m->vm[q]);
+        }
+    }
+    else if (is_microsphere(m->model))
+    {
+        int const consistent = m->tangent_mode == ORC_TANGENT_CONSISTENT;
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < int64_t(Q); ++q)
+            microsphere(m->model, &m->F[9 * q], m->detF[q], &m->params[4 * (q / 8)], consistent, &m->sigma[9 * q],
+                        &m->D[36 * q]);
+    }
     else
     {
         int failed = 0;
@@ -570,7 +908,8 @@ void element_matrices(const orc_model& m, int64_t e, double* ke, double* fe, con
     double xe[24];
     // neon integrates on the current configuration x (submesh.cpp:68,84), also for the
     // small-strain J2 model; consistent mode integrates small strain on X.
-    bool const material_frame = m.model == ORC_MODEL_J2 && m.tangent_mode == ORC_TANGENT_CONSISTENT;
+    bool const material_frame = (m.model == ORC_MODEL_J2 || m.model == ORC_MODEL_LINEAR_ELASTIC)
+                                && m.tangent_mode == ORC_TANGENT_CONSISTENT;
     const std::vector<double>& cfg = material_frame ? m.X : m.x;
     for (int a = 0; a < 8; ++a)
         for (int i = 0; i < 3; ++i) xe[a * 3 + i] = cfg[3 * size_t(m.conn[8 * e + a]) + i];
@@ -579,7 +918,7 @@ void element_matrices(const orc_model& m, int64_t e, double* ke, double* fe, con
     std::fill(fe, fe + 24, 0.0);
     double kgeo[64];
     std::fill(kgeo, kgeo + 64, 0.0);
-    bool const finite = m.model == ORC_MODEL_NEOHOOKE; // is_finite_deformation()
+    bool const finite = is_finite(m.model); // is_finite_deformation()
     for (int l = 0; l < 8; ++l)
     {
         double J[9], Ji[9];

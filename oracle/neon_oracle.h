@@ -48,7 +48,17 @@ enum {
     ORC_BAD_ARGUMENT = -2
 };
 
-enum { ORC_MODEL_NEOHOOKE = 0, ORC_MODEL_J2 = 1 };
+/* Constitutive models (constitutive_model_factory.cpp:37-160).  The microsphere models take
+ * their unit-sphere rule as a flag: BO21 (0) or ORC_SPHERE_BO33. */
+enum {
+    ORC_MODEL_NEOHOOKE = 0,             /* compressible_neohooke */
+    ORC_MODEL_J2 = 1,                   /* small_strain_J2_plasticity */
+    ORC_MODEL_LINEAR_ELASTIC = 2,       /* isotropic_linear_elasticity */
+    ORC_MODEL_MICROSPHERE_GAUSSIAN = 3, /* gaussian_affine_microsphere */
+    ORC_MODEL_MICROSPHERE_LANGEVIN = 4, /* affine_microsphere */
+    ORC_MODEL_MICROSPHERE_NONAFFINE = 5,/* nonaffine_microsphere */
+    ORC_SPHERE_BO33 = 0x100
+};
 
 /* Krylov stopping rule: Eigen's relative ||r|| < tol ||b|| (CPU solver) or the
  * CUDA solver's absolute sqrt(r.z) <= tol. */
@@ -65,6 +75,14 @@ int orc_hex8_tables(double* N /*[8 qp][8 node]*/, double* dN /*[8 qp][8 node][3]
 /* neo-Hooke at one QP: F (row-major 3x3), J = det F. Outputs sigma (3x3) and D (6x6). */
 int orc_neohooke_qp(const double* F, double bulk, double shear, double* sigma, double* D);
 
+/* Microsphere model (model id with its rule flag) at one QP: F (3x3), params (bulk K, shear G,
+ * segments per chain N, non-affine stretch parameter p).  consistent != 0 returns the exact
+ * Cauchy-based tangent instead of neon's.  Outputs sigma (3x3) and D (6x6). */
+int orc_microsphere_qp(int32_t model, const double* F, const double* params, int32_t consistent,
+                       double* sigma, double* D);
+/* The unit-sphere rule of a microsphere model id: count, weights w[n], directions r[n][3]. */
+int orc_sphere_rule(int32_t model, double* w /*33*/, double* r /*99*/);
+
 /* J2 at one QP: H (3x3 displacement gradient); eps_p (3x3) and ebar are in/out history. */
 int orc_j2_qp(const double* H, double bulk, double shear, double yield, double hardening,
               double* eps_p, double* ebar, double* sigma, double* D, double* von_mises);
@@ -72,7 +90,9 @@ int orc_j2_qp(const double* H, double bulk, double shear, double yield, double h
 /* ---- mesh-level model ---- */
 typedef struct orc_model orc_model;
 
-/* params: per element (bulk K, shear G, yield stress, hardening modulus). */
+/* params: 4 per element, (bulk K, shear G, yield stress, hardening modulus) for J2 and
+ * neo-Hooke / linear elasticity, (K, G, segments per chain, non-affine stretch parameter) for
+ * the microsphere models. */
 int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
                      int32_t model, const double* params, int32_t tangent_mode,
                      orc_model** out);

@@ -19,7 +19,9 @@ i32p, f64p = P(C.c_int32), P(C.c_double)
 
 OK, NONCONVERGED, NAN_OR_INF, NEGATIVE_JACOBIAN, RETURN_MAP_FAILED, BREAKDOWN = 0, 1, 2, 3, 4, 5
 FATAL, BAD_ARGUMENT = -1, -2
-MODEL_NEOHOOKE, MODEL_J2 = 0, 1
+MODEL_NEOHOOKE, MODEL_J2, MODEL_LINEAR_ELASTIC = 0, 1, 2
+MODEL_MICROSPHERE_GAUSSIAN, MODEL_MICROSPHERE_LANGEVIN, MODEL_MICROSPHERE_NONAFFINE = 3, 4, 5
+SPHERE_BO33 = 0x100
 STOP_RELATIVE, STOP_ABSOLUTE_PRECONDITIONED = 0, 1
 TANGENT_REFERENCE, TANGENT_CONSISTENT = 0, 1
 
@@ -56,6 +58,8 @@ class NewtonRecord(C.Structure):
 _SIG = {
     "orc_hex8_tables": [f64p, f64p, f64p],
     "orc_neohooke_qp": [f64p, C.c_double, C.c_double, f64p, f64p],
+    "orc_microsphere_qp": [C.c_int32, f64p, f64p, C.c_int32, f64p, f64p],
+    "orc_sphere_rule": [C.c_int32, f64p, f64p],
     "orc_j2_qp": [f64p, C.c_double, C.c_double, C.c_double, C.c_double, f64p, f64p, f64p, f64p, f64p],
     "orc_model_create": [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, f64p, C.c_int32, P(C.c_void_p)],
     "orc_model_destroy": [C.c_void_p],
@@ -128,6 +132,25 @@ def neohooke_qp(F, bulk, shear):
     s, D = np.zeros((3, 3)), np.zeros((6, 6))
     lib().orc_neohooke_qp(_f(F), bulk, shear, _f(s), _f(D))
     return s, D
+
+
+def microsphere_qp(model, F, params, consistent=False):
+    """sigma (3x3) and D (6x6) of a microsphere model (id | rule flag) at one QP;
+    params = (K, G, segments per chain, non-affine stretch parameter)."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    s, D = np.zeros((3, 3)), np.zeros((6, 6))
+    code = lib().orc_microsphere_qp(model, _f(F), _f(prm), int(consistent), _f(s), _f(D))
+    if code != OK:
+        raise RuntimeError(f"orc_microsphere_qp failed: {code}")
+    return s, D
+
+
+def sphere_rule(model):
+    """(weights, directions) of the unit-sphere rule of a microsphere model id."""
+    w, r = np.zeros(33), np.zeros((33, 3))
+    n = lib().orc_sphere_rule(model, _f(w), _f(r))
+    return w[:n], r[:n]
 
 
 def j2_qp(H, bulk, shear, yield_stress, hardening, eps_p=None, ebar=0.0):

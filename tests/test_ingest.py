@@ -115,9 +115,26 @@ def test_ingest_rejects_what_the_device_path_does_not_run(tmp_path):
     mat = {"elastic_modulus": 10.0, "poissons_ratio": 0.3}
     nh = {"name": "neohooke"}
     bad = [
-        (mat, {"name": "microsphere"}, [], "constitutive model"),
+        (mat, {"name": "microsphere"}, [], 'Missing "type"'),
+        (mat, {"name": "microsphere", "type": "affine"}, [], 'Missing "quadrature"'),
+        (mat, {"name": "microsphere", "type": "affine", "quadrature": "BO61"}, [], '"quadrature" field must be'),
+        (mat, {"name": "microsphere", "type": "affine", "quadrature": "BO21"}, [], 'Missing "statistics"'),
+        (mat, {"name": "microsphere", "type": "affine", "quadrature": "BO21", "statistics": "x"}, [],
+         '"statistics" for the microsphere'),
+        (mat, {"name": "microsphere", "type": "affine", "quadrature": "BO21", "statistics": "gaussian",
+               "ageing": "other"}, [], "BAND"),
+        (mat, {"name": "microsphere", "type": "affine", "quadrature": "FM900", "statistics": "gaussian"}, [],
+         "FM900"),
+        (mat, {"name": "microsphere", "type": "cubic", "quadrature": "BO21"}, [], '"affine" or "nonaffine"'),
+        (mat, {"name": "microsphere", "type": "affine", "quadrature": "BO21", "statistics": "gaussian"}, [],
+         "segments_per_chain"),
+        (dict(mat, segments_per_chain=20), {"name": "microsphere", "type": "nonaffine", "quadrature": "BO21"}, [],
+         "nonaffine_stretch_parameter"),
+        (mat, {"name": "mooney_rivlin"}, [], "is not recognised"),
         (mat, {"name": "J2_plasticity"}, [], "finite_strain"),
-        (mat, {"name": "J2_plasticity", "finite_strain": True}, [], "small-strain"),
+        (mat, {"name": "J2_plasticity", "finite_strain": True}, [], "finite-strain J2"),
+        (mat, {"name": "J2_plasticity", "finite_strain": True, "damage": "isotropic_chaboche"}, [],
+         "not yet implemented"),
         ({"elastic_modulus": 10.0, "poissons_ratio": 0.5}, nh, [], "poissons_ratio"),
         ({"elastic_modulus": 10.0}, nh, [], "bulk_modulus"),
         (mat, nh, [{"name": "z1", "type": "heat_flux", "time": [0, 1], "z": [0, 1]}], "not recognised"),
@@ -135,6 +152,25 @@ def test_ingest_rejects_what_the_device_path_does_not_run(tmp_path):
         path = neon_files.write_cube_simulation(tmp_path / f"t{k}", 2, mat, nh, [bc])
         with pytest.raises(ValueError, match="need"):
             ingest.read(path)
+
+
+def test_ingest_constitutive_models(tmp_path):
+    mat = {"elastic_modulus": 10.0, "poissons_ratio": 0.3, "segments_per_chain": 25,
+           "nonaffine_stretch_parameter": 1.5}
+    K, G = 10.0 / (3 * 0.4), 10.0 / 2.6
+    cases = [
+        ({"name": "isotropic_linear_elasticity"}, _capi.MODEL_LINEAR_ELASTIC, [K, G, 0, 0]),
+        ({"name": "microsphere", "type": "affine", "statistics": "gaussian", "quadrature": "BO21"},
+         _capi.MODEL_MICROSPHERE_GAUSSIAN, [K, G, 25, 0]),
+        ({"name": "microsphere", "type": "affine", "statistics": "langevin", "quadrature": "BO33"},
+         _capi.MODEL_MICROSPHERE_LANGEVIN | _capi.SPHERE_BO33, [K, G, 25, 0]),
+        ({"name": "microsphere", "type": "nonaffine", "quadrature": "BO33"},
+         _capi.MODEL_MICROSPHERE_NONAFFINE | _capi.SPHERE_BO33, [K, G, 25, 1.5]),
+    ]
+    for k, (constitutive, model, prm) in enumerate(cases):
+        p = ingest.read(neon_files.write_cube_simulation(tmp_path / str(k), 2, mat, constitutive, []))
+        assert p.model == model
+        np.testing.assert_allclose(p.params, np.tile(prm, (8, 1)), rtol=1e-15)
 
 
 # ------------------------------------------------------------------ unit-load vectors

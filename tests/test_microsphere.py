@@ -1,0 +1,338 @@
+"""Microsphere models and isotropic linear elasticity in the oracle, on the CPU.
+
+What pins each part:
+
+* the unit-sphere rules (unit_sphere_quadrature.cpp:10-90, Bazant and Oh 1986): 21 and 33
+  unit directions whose weights sum to one and which integrate the sphere's moments of
+  degree 2, 4 and 6 exactly (to the 12 published digits);
+* the reference's own checks (test/hyperelastic_constitutive.cpp:132-300; E = 10e6,
+  nu = 0.45, N = 50, BO21): zero stress and a symmetric positive definite tangent at F = I,
+  and under the isochoric stretch F = diag(1.1, 1.1^-1/2, 1.1^-1/2);
+* the stresses against the free energies the models derive from, by complex-step
+  differentiation of W(F) = U(J) + W_bar(F_bar): U = K/4 (J^2 - 1) - K/2 ln J, and
+  Gaussian W_bar = 3G/2 sum w l^2, Langevin W_bar = G sum w (l^2/2 - N ln(N - l^2)), non-affine
+  W_bar = G (L^2/2 - N ln(N - L^2)) + G N U sum w nu with L = (sum w l^p)^(1/p), l = |F_bar r|,
+  nu = |F_bar^-T r|, U = 9;
+* the Gaussian stress against its closed form (p I + G dev B_bar) / J, since the rules integrate
+  r (x) r exactly;
+* the tangent against the derivative of the stress: J c = F F dP/dF - delta tau by central
+  differences, which the consistent mode must equal (times J) and the reference mode must equal
+  after adding the -2/3 (dev tau (x) 1 + 1 (x) dev tau) that its deviatoric projections drop;
+* at F = I the Gaussian tangent equals the small-strain moduli; at N -> infinity the Langevin
+  model equals the Gaussian one;
+* element level: the consistent-mode K against central differences of f_int on a jittered
+  mesh, and homogeneous uniaxial Newton solves against the model's own uniaxial stress state.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import brentq
+
+from oracle import oracle as orc
+
+import neon_files
+
+E_REF, NU_REF, N_REF = 10.0e6, 0.45, 50.0
+K_REF = E_REF / (3.0 * (1.0 - 2.0 * NU_REF))
+G_REF = E_REF / (2.0 * (1.0 + NU_REF))
+MICRO = (orc.MODEL_MICROSPHERE_GAUSSIAN, orc.MODEL_MICROSPHERE_LANGEVIN, orc.MODEL_MICROSPHERE_NONAFFINE)
+RULES = (0, orc.SPHERE_BO33)
+P_NA = 1.5  # non-affine stretch parameter
+
+
+def _params(model, K=K_REF, G=G_REF, N=N_REF, p=P_NA):
+    return np.array([K, G, N, p if (model & 0xFF) == orc.MODEL_MICROSPHERE_NONAFFINE else 0.0])
+
+
+def _random_F(rng, scale=0.15):
+    F = np.eye(3) + rng.uniform(-scale, scale, (3, 3))
+    assert np.linalg.det(F) > 0
+    return F
+
+
+# ------------------------------------------------------------------------ rules
+@pytest.mark.parametrize("rule, count", [(0, 21), (orc.SPHERE_BO33, 33)])
+def test_sphere_rules_integrate_moments(rule, count):
+    w, r = orc.sphere_rule(orc.MODEL_MICROSPHERE_GAUSSIAN | rule)
+    assert len(w) == count
+    assert w.sum() == pytest.approx(1.0, abs=1e-11)
+    np.testing.assert_allclose(np.linalg.norm(r, axis=1), 1.0, atol=1e-12)
+    d = np.eye(3)
+    m2 = np.einsum("i,ia,ib->ab", w, r, r)
+    np.testing.assert_allclose(m2, d / 3.0, atol=1e-12)
+    m4 = np.einsum("i,ia,ib,ic,id->abcd", w, r, r, r, r)
+    exact4 = (np.einsum("ab,cd->abcd", d, d) + np.einsum("ac,bd->abcd", d, d) + np.einsum("ad,bc->abcd", d, d)) / 15
+    np.testing.assert_allclose(m4, exact4, atol=1e-12)
+    # degree 6: <x^6> = 1/7, <x^4 y^2> = 1/35, <x^2 y^2 z^2> = 1/105 on the unit sphere
+    x, y, z = r.T
+    assert (w * x ** 6).sum() == pytest.approx(1 / 7, abs=1e-11)
+    assert (w * x ** 4 * y ** 2).sum() == pytest.approx(1 / 35, abs=1e-11)
+    assert (w * x ** 2 * y ** 2 * z ** 2).sum() == pytest.approx(1 / 105, abs=1e-11)
+    # antipodal symmetry: no direction appears with its opposite (half-sphere rule)
+    for i in range(count):
+        assert np.min(np.linalg.norm(r + r[i], axis=1)) > 0.1
+
+
+# --------------------------------------------------------- the reference's own checks
+@pytest.mark.parametrize("rule", RULES)
+@pytest.mark.parametrize("model", MICRO)
+def test_reference_checks_no_load_and_uniaxial(model, rule):
+    prm = _params(model | rule)
+    s, D = orc.microsphere_qp(model | rule, np.eye(3), prm)
+    assert np.abs(s).max() < 1e-5 * G_REF
+    np.testing.assert_allclose(D, D.T, atol=1e-9 * np.abs(D).max())
+    assert np.linalg.eigvalsh(D).min() > 0.0
+    F = np.diag([1.1, 1.0 / np.sqrt(1.1), 1.0 / np.sqrt(1.1)])
+    s, D = orc.microsphere_qp(model | rule, F, prm)
+    np.testing.assert_allclose(D, D.T, atol=1e-9 * np.abs(D).max())
+    assert np.linalg.eigvalsh(D).min() > 0.0
+    assert np.linalg.norm(s) > 0.0
+
+
+# ------------------------------------------------------------------- energies
+def _energy(model, F, prm):
+    """W(F) for complex-step differentiation (analytic operations only)."""
+    K, G, N, p = prm
+    base, rule = model & 0xFF, model & orc.SPHERE_BO33
+    w, r = orc.sphere_rule(orc.MODEL_MICROSPHERE_GAUSSIAN | rule)
+    J = (F[0, 0] * (F[1, 1] * F[2, 2] - F[1, 2] * F[2, 1]) - F[0, 1] * (F[1, 0] * F[2, 2] - F[1, 2] * F[2, 0])
+         + F[0, 2] * (F[1, 0] * F[2, 1] - F[1, 1] * F[2, 0]))
+    Fb = F * J ** (-1.0 / 3.0)
+    t = r @ Fb.T
+    l2 = (t * t).sum(axis=1)
+    vol = K / 4.0 * (J * J - 1.0) - K / 2.0 * np.log(J)
+    if base == orc.MODEL_MICROSPHERE_GAUSSIAN:
+        return vol + 1.5 * G * (w * l2).sum()
+    if base == orc.MODEL_MICROSPHERE_LANGEVIN:
+        return vol + G * (w * (l2 / 2.0 - N * np.log(N - l2))).sum()
+    L2 = ((w * l2 ** (p / 2.0)).sum()) ** (2.0 / p)
+    # F_bar^-T r by cofactors of F_bar (det F_bar = 1)
+    cof = np.array([[Fb[1, 1] * Fb[2, 2] - Fb[1, 2] * Fb[2, 1], Fb[1, 2] * Fb[2, 0] - Fb[1, 0] * Fb[2, 2],
+                     Fb[1, 0] * Fb[2, 1] - Fb[1, 1] * Fb[2, 0]],
+                    [Fb[0, 2] * Fb[2, 1] - Fb[0, 1] * Fb[2, 2], Fb[0, 0] * Fb[2, 2] - Fb[0, 2] * Fb[2, 0],
+                     Fb[0, 1] * Fb[2, 0] - Fb[0, 0] * Fb[2, 1]],
+                    [Fb[0, 1] * Fb[1, 2] - Fb[0, 2] * Fb[1, 1], Fb[0, 2] * Fb[1, 0] - Fb[0, 0] * Fb[1, 2],
+                     Fb[0, 0] * Fb[1, 1] - Fb[0, 1] * Fb[1, 0]]])
+    n = r @ cof.T
+    nu = np.sqrt((n * n).sum(axis=1))
+    return vol + G * (L2 / 2.0 - N * np.log(N - L2)) + G * N * 9.0 * (w * nu).sum()
+
+
+def _piola_complex_step(model, F, prm):
+    P = np.zeros((3, 3))
+    h = 1e-30
+    for i in range(3):
+        for j in range(3):
+            Fc = F.astype(complex)
+            Fc[i, j] += 1j * h
+            P[i, j] = _energy(model, Fc, prm).imag / h
+    return P
+
+
+@pytest.mark.parametrize("rule", RULES)
+@pytest.mark.parametrize("model", MICRO)
+def test_stress_is_the_derivative_of_the_energy(model, rule):
+    rng = np.random.default_rng(11 + model + rule)
+    prm = _params(model | rule)
+    for _ in range(4):
+        F = _random_F(rng)
+        s, _ = orc.microsphere_qp(model | rule, F, prm)
+        P = np.linalg.det(F) * s @ np.linalg.inv(F).T
+        Pw = _piola_complex_step(model | rule, F, prm)
+        np.testing.assert_allclose(P, Pw, rtol=0, atol=1e-10 * np.abs(Pw).max())
+        np.testing.assert_allclose(s, s.T, rtol=0, atol=1e-12 * np.abs(s).max())
+
+
+@pytest.mark.parametrize("rule", RULES)
+def test_gaussian_stress_closed_form(rule):
+    rng = np.random.default_rng(5)
+    K, G = 3.0e3, 1.0e3
+    for _ in range(5):
+        F = _random_F(rng, 0.3)
+        J = np.linalg.det(F)
+        Bb = J ** (-2.0 / 3.0) * F @ F.T
+        exact = (K / 2.0 * (J * J - 1.0) * np.eye(3) + G * (Bb - np.trace(Bb) / 3.0 * np.eye(3))) / J
+        s, _ = orc.microsphere_qp(orc.MODEL_MICROSPHERE_GAUSSIAN | rule, F, [K, G, 10.0, 0.0])
+        np.testing.assert_allclose(s, exact, rtol=0, atol=1e-10 * np.abs(exact).max())
+
+
+# ------------------------------------------------------------------- tangents
+_PAIRS = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+
+
+def _voigt(c):
+    return np.array([[c[i, j, k, l] for (k, l) in _PAIRS] for (i, j) in _PAIRS])
+
+
+def _kirchhoff_tangent_fd(model, F, prm, h=1e-6):
+    """J c_ijkl = F_jB F_lD dP_iB/dF_kD - delta_ik tau_jl, central differences of the oracle."""
+    def piola(Fx):
+        s, _ = orc.microsphere_qp(model, Fx, prm)
+        return np.linalg.det(Fx) * s @ np.linalg.inv(Fx).T
+
+    A = np.zeros((3, 3, 3, 3))
+    for k in range(3):
+        for d in range(3):
+            Fp, Fm = F.copy(), F.copy()
+            Fp[k, d] += h
+            Fm[k, d] -= h
+            A[:, :, k, d] = (piola(Fp) - piola(Fm)) / (2 * h)
+    s, _ = orc.microsphere_qp(model, F, prm)
+    tau = np.linalg.det(F) * s
+    c = np.einsum("jb,ld,ibkd->ijkl", F, F, A) - np.einsum("ik,jl->ijkl", np.eye(3), tau)
+    return _voigt(c), tau
+
+
+@pytest.mark.parametrize("rule", RULES)
+@pytest.mark.parametrize("model", MICRO)
+def test_tangent_against_the_stress_derivative(model, rule):
+    rng = np.random.default_rng(3 + model)
+    # moduli of order one keep the differences well conditioned
+    prm = _params(model | rule, K=3.0, G=1.0, N=8.0)
+    for _ in range(3):
+        F = _random_F(rng, 0.12)
+        J = np.linalg.det(F)
+        Jc, tau = _kirchhoff_tangent_fd(model | rule, F, prm)
+        _, Dc = orc.microsphere_qp(model | rule, F, prm, consistent=True)
+        np.testing.assert_allclose(J * Dc, Jc, rtol=0, atol=2e-7 * np.abs(Jc).max())
+        # neon's tangent: the exact one plus the term its projections drop
+        _, Dr = orc.microsphere_qp(model | rule, F, prm)
+        dev = tau - np.trace(tau) / 3.0 * np.eye(3)
+        v = np.array([dev[i, j] for (i, j) in _PAIRS])
+        m = np.array([1.0, 1, 1, 0, 0, 0])
+        np.testing.assert_allclose(Dr, Jc + 2.0 / 3.0 * (np.outer(v, m) + np.outer(m, v)), rtol=0,
+                                   atol=2e-7 * np.abs(Jc).max())
+
+
+@pytest.mark.parametrize("rule", RULES)
+def test_gaussian_tangent_at_identity_is_the_small_strain_moduli(rule):
+    K, G = 7.0e3, 2.0e3
+    _, D = orc.microsphere_qp(orc.MODEL_MICROSPHERE_GAUSSIAN | rule, np.eye(3), [K, G, 10.0, 0.0])
+    lam = K - 2.0 * G / 3.0
+    Ce = np.zeros((6, 6))
+    Ce[:3, :3] = lam
+    Ce[np.arange(3), np.arange(3)] += 2.0 * G
+    Ce[np.arange(3, 6), np.arange(3, 6)] = G
+    np.testing.assert_allclose(D, Ce, rtol=0, atol=1e-10 * K)
+
+
+def test_langevin_tends_to_gaussian_for_long_chains():
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        F = _random_F(rng)
+        prm = [K_REF, G_REF, 1e12, 0.0]
+        sl, Dl = orc.microsphere_qp(orc.MODEL_MICROSPHERE_LANGEVIN, F, prm)
+        sg, Dg = orc.microsphere_qp(orc.MODEL_MICROSPHERE_GAUSSIAN, F, prm)
+        np.testing.assert_allclose(sl, sg, rtol=0, atol=1e-9 * np.abs(sg).max())
+        np.testing.assert_allclose(Dl, Dg, rtol=0, atol=1e-9 * np.abs(Dg).max())
+
+
+# --------------------------------------------------------------- element level
+def _jittered(n, seed):
+    X, conn, faces = neon_files.cube_mesh(n)
+    rng = np.random.default_rng(seed)
+    inner = np.all((X > 0) & (X < 1), axis=1)
+    X = X.copy()
+    X[inner] += rng.uniform(-0.12 / n, 0.12 / n, (inner.sum(), 3))
+    return X, conn, faces
+
+
+def _assembled(m, u):
+    rp, ci = m.pattern()
+    assert m.update(u) == orc.OK
+    val, f = m.assemble(rp, ci)
+    n = len(rp) - 1
+    K = np.zeros((n, n))
+    for r in range(n):
+        K[r, ci[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+    return K, f
+
+
+@pytest.mark.parametrize("model", MICRO + (orc.MODEL_LINEAR_ELASTIC,))
+def test_consistent_stiffness_is_the_derivative_of_the_internal_force(model):
+    X, conn, _ = _jittered(2, 4)
+    params = np.tile(_params(model, K=3.0, G=1.0, N=8.0), (len(conn), 1))
+    m = orc.Model(X, conn, params, model=model, tangent=orc.TANGENT_CONSISTENT)
+    rng = np.random.default_rng(2)
+    u = rng.uniform(-0.03, 0.03, m.n_dofs)
+    K, _ = _assembled(m, u)
+    h = 1e-6
+    Kfd = np.zeros_like(K)
+    for j in range(m.n_dofs):
+        up, um = u.copy(), u.copy()
+        up[j] += h
+        um[j] -= h
+        Kfd[:, j] = (_assembled(m, up)[1] - _assembled(m, um)[1]) / (2 * h)
+    np.testing.assert_allclose(K, Kfd, rtol=0, atol=1e-7 * np.abs(K).max())
+    np.testing.assert_allclose(K, K.T, rtol=0, atol=1e-12 * np.abs(K).max())
+
+
+def _uniaxial_bcs(X, ez):
+    n = len(X)
+    dofs = lambda mask, c: (3 * np.nonzero(mask)[0] + c).astype(np.int32)  # noqa: E731
+    bcs = []
+    for c, mask in ((0, X[:, 0] == 0), (1, X[:, 1] == 0), (2, X[:, 2] == 0)):
+        d = dofs(mask, c)
+        bcs.append(dict(dofs=d, times=[0.0, 1.0], values=[0.0, 0.0]))
+    bcs.append(dict(dofs=dofs(X[:, 2] == 1, 2), times=[0.0, 1.0], values=[0.0, ez]))
+    assert n > 0
+    return bcs
+
+
+def _options(tangent):
+    sim = {"time": {"period": 1.0, "increments": {"initial": 0.5, "minimum": 0.01, "maximum": 0.5, "adaptive": False}},
+           "nonlinear_options": {"residual_tolerance": 1e-9, "displacement_tolerance": 1e-9},
+           "linear_solver": {"tolerance": 1e-13, "maximum_iterations": 5000}}
+    return orc.newton_options(sim)
+
+
+@pytest.mark.parametrize("tangent", [orc.TANGENT_REFERENCE, orc.TANGENT_CONSISTENT])
+@pytest.mark.parametrize("model", MICRO + (orc.MODEL_LINEAR_ELASTIC,))
+def test_uniaxial_newton_is_homogeneous(model, tangent):
+    """Rollers on x0, y0, z0 and u_z = 0.1 on z1: the solution is u = (F - I) X with F =
+    diag(l, l, 1.1) where the lateral stress of the model vanishes."""
+    X, conn, _ = _jittered(3, 9)
+    K, G = 3.0, 1.0
+    prm = _params(model, K=K, G=G, N=8.0)
+    m = orc.Model(X, conn, np.tile(prm, (len(conn), 1)), model=model, tangent=tangent)
+    code, u, recs = m.newton(_options(tangent), _uniaxial_bcs(X, 0.1))
+    assert code == orc.OK, recs[-3:]
+    if model == orc.MODEL_LINEAR_ELASTIC:
+        nu = (3 * K - 2 * G) / (2 * (3 * K + G))
+        lat = 1.0 - nu * 0.1
+    else:
+        lat = brentq(lambda l: orc.microsphere_qp(model, np.diag([l, l, 1.1]), prm)[0][0, 0], 0.7, 1.2, xtol=1e-15)
+    U = u.reshape(-1, 3)
+    np.testing.assert_allclose(U[:, 2], 0.1 * X[:, 2], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(U[:, 0], (lat - 1.0) * X[:, 0], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(U[:, 1], (lat - 1.0) * X[:, 1], rtol=0, atol=1e-9)
+    if tangent == orc.TANGENT_CONSISTENT:
+        # the exact Jacobian converges quadratically: at most 4 iterations per step
+        assert max(r["iteration"] for r in recs) <= 4
+
+
+def test_linear_elastic_qp_state():
+    X, conn, _ = _jittered(2, 1)
+    K, G = 5.0, 2.0
+    m = orc.Model(X, conn, np.tile([K, G, 0.0, 0.0], (len(conn), 1)), model=orc.MODEL_LINEAR_ELASTIC)
+    A = np.array([[1e-3, 2e-4, 0.0], [-1e-4, 3e-4, 5e-4], [2e-4, 0.0, -2e-3]])
+    assert m.update((X @ A.T).reshape(-1)) == orc.OK
+    st = m.qp_state()
+    eps = 0.5 * (A + A.T)
+    lam = K - 2 * G / 3
+    sig = lam * np.trace(eps) * np.eye(3) + 2 * G * eps
+    np.testing.assert_allclose(st["sigma"], np.broadcast_to(sig, st["sigma"].shape), rtol=0, atol=1e-14)
+    dev = sig - np.trace(sig) / 3 * np.eye(3)
+    np.testing.assert_allclose(st["vm"], np.sqrt(1.5 * (dev * dev).sum()), rtol=1e-12)
+    Ce = np.zeros((6, 6))
+    Ce[:3, :3] = lam
+    Ce[np.arange(3), np.arange(3)] += 2 * G
+    Ce[np.arange(3, 6), np.arange(3, 6)] = G
+    np.testing.assert_allclose(st["D"], np.broadcast_to(Ce, st["D"].shape), rtol=0, atol=1e-14)
+
+
+def test_model_ids_are_checked():
+    X, conn, _ = neon_files.cube_mesh(1)
+    with pytest.raises(RuntimeError):
+        orc.Model(X, conn, [[1.0, 1.0, 0.0, 0.0]], model=orc.MODEL_J2 | orc.SPHERE_BO33)
+    with pytest.raises(RuntimeError):
+        orc.Model(X, conn, [[1.0, 1.0, 0.0, 0.0]], model=6)
