@@ -29,6 +29,8 @@
  *   neon_mesh_enforce_dirichlet static_matrix::enforce_dirichlet_conditions (static_matrix.hpp:279-320)
  *   neon_mesh_solve            linear_solver::solve on the device-resident tangent (static_matrix.hpp:432)
  *   neon_newton_solve          static_matrix::solve (static_matrix.hpp:145-187, 402-467)
+ *   neon_natural_frequency     natural_frequency_matrix::solve (natural_frequency_matrix.hpp:54-68)
+ *   neon_mesh_mass             natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:100-129)
  *   neon_adjoint_gradient      no counterpart in the reference (new: adjoint of the converged state)
  *
  * Threading: one host thread per context; a context owns one HIP stream and every
@@ -72,6 +74,9 @@ extern "C" {
 #define NEON_TANGENT_CONSISTENT 1 /* exact Jacobian of f_int (D/J for neo-Hooke; the microsphere
                                      D with the term its deviatoric projections drop, over J;
                                      small-strain models integrated on X) */
+
+#define NEON_SPECTRUM_LOWER 0 /* eigen_solver::eigen_spectrum::lower ("SM") */
+#define NEON_SPECTRUM_UPPER 1 /* eigen_solver::eigen_spectrum::upper ("LM") */
 
 #define NEON_SOLVER_CG 0
 #define NEON_SOLVER_BICGSTAB 1
@@ -289,6 +294,27 @@ int neon_newton_solve_loaded(neon_context* ctx, const neon_newton_options* optio
  *    mu_{n-1} = (dh_n/dh_{n-1})^T mu_n - (d sigma_n/dh_{n-1})^T (w detJ B lambda_n), and
  *    dJ/dp += (dh_n/dp)^T mu_n - (d sigma_n/dp)^T (w detJ B lambda_n) for p = bulk, shear,
  *    yield stress, hardening.  The context's final state is restored afterwards. */
+/* ---- natural frequencies (reuse of the device tangent and SpMV) ----
+ * natural_frequency_matrix::solve (src/assembler/mechanics/natural_frequency_matrix.hpp:54-68)
+ * with the arpack eigen_solver (src/solver/eigen/arpack.cpp:40-62): K at the mesh's held state
+ * and the consistent mass (submesh::consistent_mass, submesh.cpp:162-183; density per element),
+ * both with the fixed dofs' rows and columns zeroed and diagonals kept
+ * (homogeneous_dirichlet.hpp:84-123), then the n_values eigenvalues of K x = lambda M x at the
+ * lower or upper end of the spectrum (NEON_SPECTRUM_*), in increasing order.  Block Krylov with
+ * full M-orthogonalisation and Rayleigh-Ritz over the device PCG (lower end: shift-invert,
+ * OP = K^-1 M; upper end: OP = M^-1 K), stopped when every wanted Ritz pair's residual
+ * |OP y - theta y|_M <= tolerance |theta|.  max_basis (0: max(4 n_values, n_values + 40)) caps
+ * the basis, block (0: 4) is the block width, which must reach the largest eigenvalue
+ * multiplicity wanted.  eigenvectors (3N x n_values, column-major, M-orthonormal) may be NULL;
+ * solves (may be NULL) receives the number of inner PCG solves.  NEON_NONCONVERGED: "Eigenvalues
+ * did not converge" within the basis cap. */
+int neon_natural_frequency(neon_context* ctx, const double* density, int32_t n_fixed, const int32_t* fixed,
+                           int32_t n_values, int32_t spectrum, double tolerance, int32_t max_basis, int32_t block,
+                           double* eigenvalues, double* eigenvectors, int32_t* solves);
+/* The consistent mass (density per element) in the pattern's CSR order (nnz values), FP64.
+ * submesh::consistent_mass assembled as natural_frequency_matrix::assemble_mass (:100-129). */
+int neon_mesh_mass(neon_context* ctx, const double* density, double* val);
+
 int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed, const int32_t* fixed,
                           double tolerance, int32_t max_iterations, double* grad, int32_t* iterations);
 

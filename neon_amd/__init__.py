@@ -29,6 +29,8 @@ from ._capi import (  # noqa: F401
     PRECISION_MIXED,
     SOLVER_BICGSTAB,
     SOLVER_CG,
+    SPECTRUM_LOWER,
+    SPECTRUM_UPPER,
     SPMV_LDS,
     SPMV_META,
     SPMV_SLICED,
@@ -52,6 +54,8 @@ __all__ = [
     "biconjugate_gradient_stabilised_hip",
     "make_linear_solver",
     "static_matrix",
+    "natural_frequency_matrix",
+    "make_eigen_solver",
     "computational_error",
 ]
 
@@ -280,6 +284,14 @@ class DeviceMesh:
     def assemble(self, precision: int = PRECISION_FP64) -> None:
         self.ctx.check(_capi.lib().neon_mesh_assemble(self.ctx.handle, precision))
 
+    def mass(self, density) -> np.ndarray:
+        """The consistent mass (``submesh::consistent_mass``) in the pattern's CSR order; density
+        per element or one value."""
+        rho = np.ascontiguousarray(np.broadcast_to(np.asarray(density, dtype=np.float64), (self.n_elem,)))
+        val = np.zeros(self.nnz())
+        self.ctx.check(_capi.lib().neon_mesh_mass(self.ctx.handle, _capi.f64(rho), _capi.f64(val)))
+        return val
+
     def download(self, stiffness: bool = True, internal_force: bool = True):
         val = np.zeros(self.nnz()) if stiffness else None
         f = np.zeros(self.n_dofs) if internal_force else None
@@ -466,3 +478,77 @@ class static_matrix:
             raise failure[0]
         m.ctx.check(code)
         return u
+
+
+def make_eigen_solver(data: dict) -> dict:
+    """``make_eigen_solver`` (src/solver/eigen/eigen_solver_factory.cpp:12-77): the number of
+    eigenvalues (``"eigenvalues"``, default 10) and the end of the spectrum.  ``"arpack"`` maps to
+    the device solver; neon's ``"lanczos"`` and ``"power_iteration"`` need its OpenCL build and
+    solve no generalised problem (power_iteration.cpp:45-48, lanczos_ocl.cpp:49), so they are
+    rejected as neon's default build rejects them.  Without ``"spectrum"`` neon leaves the choice
+    uninitialised; the lower end is taken here."""
+    if "type" not in data:
+        raise ValueError('Eigen solver type was not provided.  Please use "power_iteration", "arpack" or "lanczos"')
+    values = 10
+    if "eigenvalues" in data:
+        values = int(data["eigenvalues"])
+        if values <= 0:
+            raise ValueError("The number of requested eigenvalues must be positive")
+    spectrum = SPECTRUM_LOWER
+    if "spectrum" in data:
+        if data["spectrum"] == "lower":
+            spectrum = SPECTRUM_LOWER
+        elif data["spectrum"] == "upper":
+            spectrum = SPECTRUM_UPPER
+        else:
+            raise ValueError('A spectrum was requested but only "lower" and "upper" options are supported')
+    kind = data["type"]
+    if kind in ("power_iteration", "lanczos"):
+        raise ValueError(f'The "{kind}" option requires OpenCL support.  Recompile with -DENABLE_OPENCL=1')
+    if kind != "arpack":
+        raise ValueError(f'Eigen solver type "{kind}" is not recognised.  Please use "arpack"')
+    return {"type": kind, "values": values, "spectrum": spectrum}
+
+
+class natural_frequency_matrix:
+    """``natural_frequency_matrix`` (src/assembler/mechanics/natural_frequency_matrix.hpp:17-161) on
+    a :class:`DeviceMesh`: K at the mesh's held state and the consistent mass, neon's homogeneous
+    Dirichlet treatment of both (the dofs of ``boundaries``), and the eigenvalues of
+    K x = lambda M x on the device (``neon_natural_frequency``).  ``solve`` returns the eigenvalues
+    (increasing) and the M-orthonormal mode shapes (3N x values); ``frequencies`` are their square
+    roots (rad / time), as neon writes them."""
+
+    def __init__(self, mesh: DeviceMesh, eigen_solver: dict, boundaries: list, density):
+        self.mesh = mesh
+        self.solver = make_eigen_solver(eigen_solver)
+        d = [np.asarray(b["dofs"]) for b in boundaries]
+        self.fixed = np.ascontiguousarray(np.unique(np.concatenate(d)) if d else np.zeros(0), dtype=np.int32)
+        self.density = np.ascontiguousarray(np.broadcast_to(np.asarray(density, dtype=np.float64), (mesh.n_elem,)))
+        self.eigenvalues = None
+        self.eigenvectors = None
+        self.solves = 0
+
+    def solve(self, tolerance: float = 1.0e-8, max_basis: int = 0, block: int = 0):
+        """tolerance: on every wanted Ritz pair's residual |OP y - theta y|_M relative to theta (the
+        eigenvalues' error goes with its square)."""
+        k = self.solver["values"]
+        vals = np.zeros(k)
+        vecs = np.zeros((k, self.mesh.n_dofs))
+        solves = C.c_int32(0)
+        self.mesh.ctx.check(_capi.lib().neon_natural_frequency(
+            self.mesh.ctx.handle, _capi.f64(self.density), len(self.fixed), _capi.i32(self.fixed), k,
+            self.solver["spectrum"], tolerance, max_basis, block, _capi.f64(vals), _capi.f64(vecs), C.byref(solves)))
+        self.eigenvalues, self.eigenvectors, self.solves = vals, vecs.T.copy(), solves.value
+        return self.eigenvalues, self.eigenvectors
+
+    @property
+    def frequencies(self) -> np.ndarray:
+        return np.sqrt(self.eigenvalues)
+
+    def eigenvalue_table(self) -> str:
+        """The table of natural_frequency_matrix::print_eigenvalue_table (:131-160)."""
+        lines = ["      Eigenvalue | radians / time | cycles / time", "      " + "-" * 43]
+        for i, lam in enumerate(self.eigenvalues):
+            w = np.sqrt(lam)
+            lines.append(f"      {i + 1:>10}{w:>17.6g}{w / (2.0 * np.pi):>16.6g}")
+        return "\n".join(lines)

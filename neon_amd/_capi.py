@@ -33,6 +33,8 @@ MODEL_MICROSPHERE_NONAFFINE = 5
 SPHERE_BO33 = 0x100  # microsphere rule flag: 33 directions instead of 21
 TANGENT_REFERENCE = 0
 TANGENT_CONSISTENT = 1
+SPECTRUM_LOWER = 0
+SPECTRUM_UPPER = 1
 SOLVER_CG = 0
 SOLVER_BICGSTAB = 1
 STOP_RELATIVE = 0
@@ -184,6 +186,9 @@ SIGNATURES = {
          P(NewtonTiming)],
     ),
     "neon_adjoint_gradient": (C.c_int, [C.c_void_p, f64p, C.c_int32, i32p, C.c_double, C.c_int32, f64p, i32p]),
+    "neon_natural_frequency": (C.c_int, [C.c_void_p, f64p, C.c_int32, i32p, C.c_int32, C.c_int32, C.c_double,
+                                         C.c_int32, C.c_int32, f64p, f64p, i32p]),
+    "neon_mesh_mass": (C.c_int, [C.c_void_p, f64p, f64p]),
     "neon_mesh_box_size": (
         C.c_int,
         [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, i32p, i32p],

@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libneon_hip.so"
 
-SOURCES = ["engine.hip", "newton.hip", "capi.hip", "meshgen.cpp", "loads.cpp"]
+SOURCES = ["engine.hip", "newton.hip", "eigen.hip", "capi.hip", "meshgen.cpp", "loads.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

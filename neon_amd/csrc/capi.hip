@@ -580,4 +580,33 @@ int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed
     });
 }
 
+int neon_natural_frequency(neon_context* ctx, const double* density, int32_t n_fixed, const int32_t* fixed,
+                           int32_t n_values, int32_t spectrum, double tolerance, int32_t max_basis, int32_t block,
+                           double* eigenvalues, double* eigenvectors, int32_t* solves)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        device_mesh& m = *ctx->mesh;
+        if (!density || !eigenvalues || (n_fixed > 0 && !fixed) || n_fixed < 0 || max_basis < 0 || block < 0)
+            throw status_error(NEON_BAD_ARGUMENT, "null pointer or negative count");
+        for (int64_t e = 0; e < m.elements(); ++e)
+            if (!(density[e] > 0.0)) throw status_error(NEON_BAD_ARGUMENT, "density must be positive");
+        for (int32_t i = 0; i < n_fixed; ++i)
+            if (fixed[i] < 0 || fixed[i] >= m.dofs()) throw status_error(NEON_BAD_ARGUMENT, "fixed dof out of range");
+        return natural_frequency(ctx, density, n_fixed, fixed, n_values, spectrum, tolerance, max_basis, block,
+                                 eigenvalues, eigenvectors, solves);
+    });
+}
+
+int neon_mesh_mass(neon_context* ctx, const double* density, double* val)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!density || !val) throw status_error(NEON_BAD_ARGUMENT, "null pointer");
+        ctx->mesh->assemble_mass(density);
+        ctx->mesh->download_mass(val);
+        return NEON_OK;
+    });
+}
+
 } // extern "C"

@@ -70,6 +70,7 @@ struct hex8_tables
 {
     double dN[8][8][3]; // [qp][node][xi]
     double w[8];
+    double N[8][8]; // [qp][node]
 };
 
 __constant__ hex8_tables c_hex8;
@@ -741,6 +742,59 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
             int64_t const pos = 3ll * s_node[a] + i;
             f_int[pos] += f;
         }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------- consistent mass
+// M_ab = rho_e sum_l w_l N_a(l) N_b(l) det(X dN(l)) I_3 on the initial configuration
+// (submesh::consistent_mass, submesh.cpp:162-183), added into the diagonal entries of the node
+// blocks (a, b) through the slot table; one wavefront per element of one colour as in
+// assemble_kernel (lanes 0..7 compute the 8 Jacobian determinants, lane (a, b) one block).
+__global__ __launch_bounds__(64) void assemble_mass_kernel(const int32_t* __restrict__ elems, int32_t count,
+                                                           const int32_t* __restrict__ conn,
+                                                           const double* __restrict__ X,
+                                                           const double* __restrict__ density,
+                                                           const uint8_t* __restrict__ slots,
+                                                           const int32_t* __restrict__ bptr, double* __restrict__ M)
+{
+    __shared__ double s_x[8][3];
+    __shared__ double s_w[8]; // w_l det J_l
+    __shared__ int32_t s_node[8];
+    int const lane = threadIdx.x;
+    int const a = lane >> 3, b = lane & 7;
+    for (int32_t idx = blockIdx.x; idx < count; idx += gridDim.x)
+    {
+        int64_t const e = elems[idx];
+        if (lane < 8)
+        {
+            int64_t const nd = conn[8 * e + lane];
+            s_node[lane] = int32_t(nd);
+            for (int i = 0; i < 3; ++i) s_x[lane][i] = X[3 * nd + i];
+        }
+        __syncthreads();
+        if (lane < 8)
+        {
+            int const l = lane;
+            double J[9];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                {
+                    double acc = 0.0;
+                    for (int c = 0; c < 8; ++c) acc += s_x[c][i] * c_hex8.dN[l][c][j];
+                    J[i * 3 + j] = acc;
+                }
+            s_w[l] = det3(J) * c_hex8.w[l];
+        }
+        __syncthreads();
+        double mab = 0.0;
+        for (int l = 0; l < 8; ++l) mab += c_hex8.N[l][a] * c_hex8.N[l][b] * s_w[l];
+        mab *= density[e];
+        int32_t const na = s_node[a];
+        int32_t const b0 = bptr[na];
+        int64_t const deg = bptr[na + 1] - b0;
+        int64_t const base = 9ll * b0 + 3ll * slots[64 * e + 8 * a + b];
+        for (int i = 0; i < 3; ++i) M[base + i * 3 * deg + i] += mab;
         __syncthreads();
     }
 }

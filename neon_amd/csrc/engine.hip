@@ -655,6 +655,7 @@ void upload_hex8_tables()
             t.dN[l][a][0] = 1.0 / 8.0 * xa * (1.0 + ya * eta) * (1.0 + za * zeta);
             t.dN[l][a][1] = 1.0 / 8.0 * (1.0 + xa * xi) * ya * (1.0 + za * zeta);
             t.dN[l][a][2] = 1.0 / 8.0 * (1.0 + xa * xi) * (1.0 + ya * eta) * za;
+            t.N[l][a] = 1.0 / 8.0 * (1.0 + xa * xi) * (1.0 + ya * eta) * (1.0 + za * zeta);
         }
     }
     NEON_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_hex8), &t, sizeof(t)));
@@ -1098,6 +1099,60 @@ void device_mesh::assemble(const double* u_dev, int precision)
     s64_fresh_ = s32_fresh_ = false;
 }
 
+void device_mesh::assemble_mass(const double* density)
+{
+    device_array<double> rho;
+    rho.resize(E_);
+    NEON_CHECK_HIP(hipMemcpyAsync(rho.data(), density, size_t(E_) * sizeof(double), hipMemcpyHostToDevice, s_));
+    m64_.resize(nnz_ + kValuePad);
+    NEON_CHECK_HIP(hipMemsetAsync(m64_.data(), 0, (nnz_ + kValuePad) * sizeof(double), s_));
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    for (int c = 0; c < colours(); ++c)
+    {
+        int32_t const off = colour_offsets_[c];
+        int32_t const cnt = colour_offsets_[c + 1] - off;
+        if (cnt == 0) continue;
+        int const blocks = int(std::min<int64_t>(cnt, int64_t(cus) * assembly_blocks_per_cu));
+        assemble_mass_kernel<<<blocks, 64, 0, s_>>>(elems_.data() + off, cnt, conn_.data(), X_.data(), rho.data(),
+                                                    slots_.data(), bptr_.data(), m64_.data());
+    }
+    NEON_CHECK_HIP(hipGetLastError());
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    m64_valid_ = true;
+}
+
+void device_mesh::enforce_dirichlet_mass(int32_t n_fixed, const int32_t* fixed_dev)
+{
+    if (n_fixed <= 0) return;
+    if (!m64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no mass matrix assembled");
+    dirichlet_kernel<double><<<grid_for(n_fixed), kBlock, 0, s_>>>(n_fixed, fixed_dev, bptr_.data(), bcol_.data(),
+                                                                     m64_.data(), nullptr);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+linear_operator device_mesh::op_mass() const
+{
+    if (!m64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no mass matrix assembled");
+    linear_operator A;
+    A.n = 3ll * N_;
+    A.node_block = true;
+    A.nodes = N_;
+    A.bptr = bptr_.data();
+    A.bcol = bcol_.data();
+    A.v64 = m64_.data();
+    A.nb_lanes = 32;
+    return A;
+}
+
+void device_mesh::download_mass(double* val)
+{
+    if (!m64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no mass matrix assembled");
+    NEON_CHECK_HIP(hipMemcpyAsync(val, m64_.data(), nnz_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
 void device_mesh::enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision)
 {
     if (n_fixed <= 0) return;
@@ -1326,6 +1381,14 @@ void device_mesh::jacobi(double* minv, int eigen_rule, int precision)
     else
         nb_jacobi_kernel<float, double><<<grid_for(3ll * N_), kBlock, 0, s_>>>(N_, bptr_.data(), self_slot_.data(),
                                                                                  k32_.data(), minv, eigen_rule);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+void device_mesh::jacobi_mass(double* minv)
+{
+    if (!m64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no mass matrix assembled");
+    nb_jacobi_kernel<double, double><<<grid_for(3ll * N_), kBlock, 0, s_>>>(N_, bptr_.data(), self_slot_.data(),
+                                                                              m64_.data(), minv, 1);
     NEON_CHECK_HIP(hipGetLastError());
 }
 

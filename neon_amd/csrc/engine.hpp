@@ -244,6 +244,13 @@ public:
     bool sliced_fits() const;
     // Jacobi inverse from the node-block diagonal
     void jacobi(double* minv, int eigen_rule, int precision);
+    // consistent mass (density per element, host) in the pattern of K, FP64; Dirichlet rows and
+    // columns zeroed with the diagonal kept; operator, Jacobi inverse and download
+    void assemble_mass(const double* density);
+    void enforce_dirichlet_mass(int32_t n_fixed, const int32_t* fixed_dev);
+    linear_operator op_mass() const;
+    void jacobi_mass(double* minv);
+    void download_mass(double* val);
     // adjoint: grad (4 per element) = -lambda^T d f_int / d params at displacement u
     void parameter_gradient(const double* u_dev, const double* lambda_dev, double* grad_dev);
 
@@ -285,8 +292,9 @@ private:
     device_array<double> X_, u_, params_, fint_;
     device_array<int32_t> conn_, elems_, bptr_, bcol_, self_slot_;
     device_array<uint8_t> slots_;
-    device_array<double> k64_;
+    device_array<double> k64_, m64_;
     device_array<float> k32_;
+    bool m64_valid_ = false;
     bool k64_valid_ = false, k32_valid_ = false;
     // sliced layout of the pattern (sell_matrix) and the sliced copies of the tangent
     int32_t slices_ = 0;

@@ -57,4 +57,10 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
 // summed into iterations.  The mesh's final converged state is restored.
 int j2_adjoint(neon_context* ctx, const double* dJdu_dev, int32_t n_fixed, const int32_t* fixed_dev, double tol,
                int32_t max_it, double* grad_dev, int32_t* iterations);
+// Natural frequencies (eigen.hip): the nev eigenvalues of K x = lambda M x at the lower or upper
+// end of the spectrum, ascending, with M-orthonormal eigenvectors (n x nev, column-major, when
+// eigenvectors is not null); solves receives the number of inner PCG solves.
+int natural_frequency(neon_context* ctx, const double* density, int32_t n_fixed, const int32_t* fixed, int32_t nev,
+                      int32_t spectrum, double tol, int32_t max_basis, int32_t block, double* eigenvalues,
+                      double* eigenvectors, int32_t* solves);
 } // namespace neon_amd

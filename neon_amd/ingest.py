@@ -63,6 +63,9 @@ class Problem:
     groups: dict = field(default_factory=dict)  # name -> list of (gmsh type, connectivity array)
     visualisation: dict | None = None  # the step mesh's "visualisation" block (output fields)
     output_name: str = ""              # file name of the output (the step mesh's "name")
+    solution: str = "equilibrium"      # the step's "solution": "equilibrium" or "natural_frequency"
+    eigen_solver: dict | None = None   # the step's "eigen_solver" block (natural frequency)
+    density: np.ndarray | None = None  # (E,) the material's "density", when given
 
     @property
     def n_nodes(self) -> int:
@@ -236,6 +239,10 @@ def read(path: str, step: int = 0) -> Problem:
     for key in ("name", "time", "solution"):
         if key not in sim:
             raise ValueError(f'A simulation case needs a "{key}" field')
+    if sim["solution"] not in ("equilibrium", "natural_frequency"):
+        raise ValueError('"solution" is not valid.  Please use "equilibrium" or "natural_frequency"')
+    if sim["solution"] == "natural_frequency" and "eigen_solver" not in sim:
+        raise ValueError('A natural frequency step needs an "eigen_solver" block')
     if len(sim["meshes"]) != 1:
         raise ValueError("Multibody simulation are not yet supported")
     mesh_data = sim["meshes"][0]
@@ -306,6 +313,15 @@ def read(path: str, step: int = 0) -> Problem:
         else:
             raise ValueError(f'boundary "{btype}" is not recognised')
 
+    material = materials[part["material"]]
+    density = np.full(len(conn), float(material["density"])) if "density" in material else None
+    if sim["solution"] == "natural_frequency":
+        if density is None:
+            # material_property::initial_density (material_property.cpp:34-42)
+            raise ValueError("density was requested, but not specified in the input file\n")
+        return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, {}, groups,
+                       mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], dict(sim["eigen_solver"]),
+                       density)
     ls = dict(sim.get("linear_solver", {}))
     if "type" not in ls:
         raise ValueError('A "linear_solver" "type" must be specified')
@@ -318,4 +334,4 @@ def read(path: str, step: int = 0) -> Problem:
                          '"direct" or "iterative"')
     simulation = {"time": sim["time"], "nonlinear_options": sim.get("nonlinear_options", {}), "linear_solver": solver}
     return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, simulation, groups,
-                   mesh_data.get("visualisation"), mesh_data["name"])
+                   mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], None, density)

@@ -1080,6 +1080,56 @@ int orc_sparsity_fill(const orc_model* m, int32_t* row_ptr, int32_t* col)
 
 // assemble_stiffness (static_matrix.hpp:243-277) with add_to's binary search and atomic
 // add (thread_safe_plugin.hpp:7-23); compute_internal_force serial loop (static_matrix.hpp:189-203).
+// submesh::consistent_mass (submesh.cpp:162-183): rho sum_l w_l N(l) N(l)^T det(X dN(l)) on the
+// initial configuration, expanded to 3x3 identity blocks (identity_expansion_inplace<3>).
+int orc_element_mass(const orc_model* m, int32_t element, double density, double* me)
+{
+    if (!m || !me || element < 0 || element >= m->E) return ORC_BAD_ARGUMENT;
+    hex8 const& h = shape();
+    double Xe[24];
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) Xe[a * 3 + i] = m->X[3 * size_t(m->conn[8 * int64_t(element) + a]) + i];
+    double mm[64] = {0};
+    for (int l = 0; l < 8; ++l)
+    {
+        double J[9];
+        local_F(Xe, h.dN[l], J);
+        double const d = det3(J);
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) mm[a * 8 + b] += density * h.N[l][a] * h.N[l][b] * d * h.w[l];
+    }
+    std::fill(me, me + 576, 0.0);
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b)
+            for (int i = 0; i < 3; ++i) me[(a * 3 + i) * 24 + b * 3 + i] = mm[a * 8 + b];
+    return ORC_OK;
+}
+
+// natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:100-129)
+int orc_assemble_mass(const orc_model* m, const int32_t* row_ptr, const int32_t* col, const double* density,
+                      double* val)
+{
+    if (!m || !row_ptr || !col || !density || !val) return ORC_BAD_ARGUMENT;
+    int64_t const n = 3 * int64_t(m->N);
+    std::fill(val, val + row_ptr[n], 0.0);
+    double me[576];
+    for (int64_t e = 0; e < m->E; ++e)
+    {
+        orc_element_mass(m, int32_t(e), density[e], me);
+        int32_t dofs[24];
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i) dofs[a * 3 + i] = 3 * m->conn[8 * e + a] + i;
+        for (int a = 0; a < 24; ++a)
+            for (int b = 0; b < 24; ++b)
+            {
+                int32_t const r = dofs[a], c = dofs[b];
+                const int32_t* lo = std::lower_bound(col + row_ptr[r], col + row_ptr[r + 1], c);
+                val[lo - col] += me[a * 24 + b];
+            }
+    }
+    return ORC_OK;
+}
+
 int orc_assemble(const orc_model* m, const int32_t* row_ptr, const int32_t* col, double* val, double* f_int,
                  int32_t threads)
 {

@@ -123,6 +123,12 @@ int orc_sparsity_fill(const orc_model* m, int32_t* row_ptr, int32_t* col);
 int orc_assemble(const orc_model* m, const int32_t* row_ptr, const int32_t* col, double* val,
                  double* f_int, int32_t threads);
 
+/* Consistent mass (submesh::consistent_mass, submesh.cpp:162-183) of one element (576 values,
+ * row-major 24x24) and assembled into the CSR pattern (density per element). */
+int orc_element_mass(const orc_model* m, int32_t element, double density, double* me);
+int orc_assemble_mass(const orc_model* m, const int32_t* row_ptr, const int32_t* col, const double* density,
+                      double* val);
+
 /* enforce_dirichlet_conditions on a CSR matrix and rhs. */
 int orc_enforce_dirichlet(int32_t n, const int32_t* row_ptr, const int32_t* col, double* val,
                           double* b, int32_t n_fixed, const int32_t* fixed);

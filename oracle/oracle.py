@@ -72,6 +72,8 @@ _SIG = {
     "orc_trajectory_steps": [C.c_void_p],
     "orc_sparsity_fill": [C.c_void_p, i32p, i32p],
     "orc_assemble": [C.c_void_p, i32p, i32p, f64p, f64p, C.c_int32],
+    "orc_element_mass": [C.c_void_p, C.c_int32, C.c_double, f64p],
+    "orc_assemble_mass": [C.c_void_p, i32p, i32p, f64p, f64p],
     "orc_enforce_dirichlet": [C.c_int32, i32p, i32p, f64p, f64p, C.c_int32, i32p],
     "orc_spmv": [C.c_int32, i32p, i32p, f64p, f64p, f64p, C.c_int32],
     "orc_rcm": [C.c_int32, i32p, i32p, i32p],
@@ -239,6 +241,18 @@ class Model:
         val, f = np.zeros(len(ci)), np.zeros(self.n_dofs)
         lib().orc_assemble(self.h, _i(rp), _i(ci), _f(val), _f(f), threads)
         return val, f
+
+    def element_mass(self, e, density):
+        me = np.zeros((24, 24))
+        lib().orc_element_mass(self.h, e, density, _f(me))
+        return me
+
+    def assemble_mass(self, rp, ci, density):
+        """Consistent mass in the CSR pattern; density per element (or one value)."""
+        rho = np.ascontiguousarray(np.broadcast_to(np.asarray(density, dtype=np.float64), (self.n_elem,)))
+        val = np.zeros(len(ci))
+        lib().orc_assemble_mass(self.h, _i(rp), _i(ci), _f(rho), _f(val))
+        return val
 
     def newton(self, options: NewtonOptions, boundaries, capacity=4096, loads=None):
         """Newton driver; ``loads``: nonfollower loads {dofs, values, times, loads[, fixed_time]}."""
