@@ -30,7 +30,7 @@
  *   neon_mesh_solve            linear_solver::solve on the device-resident tangent (static_matrix.hpp:432)
  *   neon_newton_solve          static_matrix::solve (static_matrix.hpp:145-187, 402-467)
  *   neon_natural_frequency     natural_frequency_matrix::solve (natural_frequency_matrix.hpp:54-68)
- *   neon_mesh_mass             natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:100-129)
+ *   neon_mesh_mass             natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:106-137)
  *   neon_adjoint_gradient      no counterpart in the reference (new: adjoint of the converged state)
  *
  * Threading: one host thread per context; a context owns one HIP stream and every
@@ -312,7 +312,7 @@ int neon_natural_frequency(neon_context* ctx, const double* density, int32_t n_f
                            int32_t n_values, int32_t spectrum, double tolerance, int32_t max_basis, int32_t block,
                            double* eigenvalues, double* eigenvectors, int32_t* solves);
 /* The consistent mass (density per element) in the pattern's CSR order (nnz values), FP64.
- * submesh::consistent_mass assembled as natural_frequency_matrix::assemble_mass (:100-129). */
+ * submesh::consistent_mass assembled as natural_frequency_matrix::assemble_mass (:106-137). */
 int neon_mesh_mass(neon_context* ctx, const double* density, double* val);
 
 int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed, const int32_t* fixed,

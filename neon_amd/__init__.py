@@ -546,7 +546,7 @@ class natural_frequency_matrix:
         return np.sqrt(self.eigenvalues)
 
     def eigenvalue_table(self) -> str:
-        """The table of natural_frequency_matrix::print_eigenvalue_table (:131-160)."""
+        """The table of natural_frequency_matrix::print_eigenvalue_table (:139-160)."""
         lines = ["      Eigenvalue | radians / time | cycles / time", "      " + "-" * 43]
         for i, lam in enumerate(self.eigenvalues):
             w = np.sqrt(lam)

@@ -1105,7 +1105,7 @@ int orc_element_mass(const orc_model* m, int32_t element, double density, double
     return ORC_OK;
 }
 
-// natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:100-129)
+// natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:106-137)
 int orc_assemble_mass(const orc_model* m, const int32_t* row_ptr, const int32_t* col, const double* density,
                       double* val)
 {
