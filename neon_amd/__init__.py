@@ -303,10 +303,11 @@ class DeviceMesh:
         self.ctx.check(_capi.lib().neon_mesh_enforce_dirichlet(self.ctx.handle, len(fixed), _capi.i32(fixed),
                                                                _capi.f64(b)))
 
-    def spmv_config(self, lanes: int = 32, flags: int = _capi.SPMV_SLICED, blocks_fp64: int = 0,
-                    blocks_fp32: int = 0) -> None:
+    def spmv_config(self, lanes: int = 32, flags: int = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL,
+                    blocks_fp64: int = 0, blocks_fp32: int = 0) -> None:
         """Shape of the node-block SpMV used by the solvers on this mesh (tuning knob; the
-        defaults are the library's: the sliced copy, one wavefront per 64 nodes)."""
+        defaults are the library's: the sliced copy, one wavefront per 64 nodes, values read with
+        nontemporal loads)."""
         self.ctx.check(_capi.lib().neon_mesh_spmv_config(self.ctx.handle, lanes, flags, blocks_fp64, blocks_fp32))
 
     def adjoint_gradient(self, dJdu: np.ndarray, fixed: np.ndarray, tol: float = 1e-12,

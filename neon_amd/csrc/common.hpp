@@ -35,7 +35,7 @@ enum : uint32_t
 };
 
 constexpr int kBlock = 256;        // threads per block for vector/SpMV kernels
-constexpr int kMaxPartials = 4096; // partial-sum slots for grid reductions
+constexpr int kMaxPartials = 16384; // partial-sum slots for grid reductions
 
 __device__ inline double wave_sum(double v)
 {

@@ -66,19 +66,24 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
     {
         if (!sval) throw status_error(NEON_BAD_ARGUMENT, "sliced operator without sliced values");
         sell_matrix m{A.nodes, A.slices, A.sptr, A.scol, A.scol16};
-#define NEON_SELL(BS, XCD, I16) \
-    spmv_sell_kernel<BS, XCD, I16, VT, XT, YT, AT, DOT, MODE><<<blocks, BS, 0, s>>>(m, sval, x, y, b, part, st, gate)
-        int const key = (A.sell_block == 1024 ? 4 : 0) + (A.nb_sweep ? 2 : 0) + (A.scol16 ? 1 : 0);
+#define NEON_SELL(BS, XCD, I16, NT) \
+    spmv_sell_kernel<BS, XCD, I16, NT, VT, XT, YT, AT, DOT, MODE><<<blocks, BS, 0, s>>>(m, sval, x, y, b, part, st, gate)
+        // nontemporal value loads with 256-thread workgroups only
+        int const key = (A.sell_block == 1024 ? 4 : (A.nb_nt ? 8 : 0)) + (A.nb_sweep ? 2 : 0) + (A.scol16 ? 1 : 0);
         switch (key)
         {
-        case 0: NEON_SELL(256, false, false); break;
-        case 1: NEON_SELL(256, false, true); break;
-        case 2: NEON_SELL(256, true, false); break;
-        case 3: NEON_SELL(256, true, true); break;
-        case 4: NEON_SELL(1024, false, false); break;
-        case 5: NEON_SELL(1024, false, true); break;
-        case 6: NEON_SELL(1024, true, false); break;
-        default: NEON_SELL(1024, true, true); break;
+        case 0: NEON_SELL(256, false, false, false); break;
+        case 1: NEON_SELL(256, false, true, false); break;
+        case 2: NEON_SELL(256, true, false, false); break;
+        case 3: NEON_SELL(256, true, true, false); break;
+        case 4: NEON_SELL(1024, false, false, false); break;
+        case 5: NEON_SELL(1024, false, true, false); break;
+        case 6: NEON_SELL(1024, true, false, false); break;
+        case 7: NEON_SELL(1024, true, true, false); break;
+        case 8: NEON_SELL(256, false, false, true); break;
+        case 9: NEON_SELL(256, false, true, true); break;
+        case 10: NEON_SELL(256, true, false, true); break;
+        default: NEON_SELL(256, true, true, true); break;
         }
 #undef NEON_SELL
         NEON_CHECK_HIP(hipGetLastError());

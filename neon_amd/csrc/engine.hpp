@@ -202,7 +202,7 @@ constexpr int64_t kValuePad = 16;
 struct spmv_config
 {
     int lanes = 32;
-    int flags = NEON_SPMV_SLICED;
+    int flags = NEON_SPMV_SLICED | NEON_SPMV_NONTEMPORAL;
     int blocks64 = 0;
     int blocks32 = 0;
 };

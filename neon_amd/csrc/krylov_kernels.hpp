@@ -235,7 +235,8 @@ struct sell_matrix
 // (consecutive slices on different XCDs, so every XCD's L2 fetches all of x).  XCD=true (a
 // multiple of 8 workgroups): the workgroups b with b % 8 = e (one XCD) sweep the e-th eighth
 // of the slices together, so each L2 fetches x only around its own eighth.
-template <int BS, bool XCD, bool I16, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+// NT: the values are read with nontemporal loads (streamed past the caches, which keep x).
+template <int BS, bool XCD, bool I16, bool NT, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
 __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* __restrict__ val,
                                                        const XT* __restrict__ x, YT* __restrict__ y,
                                                        const double* __restrict__ bvec, double* __restrict__ part,
@@ -293,7 +294,14 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
             int64_t const m = I16 ? n + int64_t(A.scol16[blk * 64 + lane]) : int64_t(A.scol[blk * 64 + lane]);
             VT v[9];
 #pragma unroll
-            for (int t = 0; t < 9; ++t) v[t] = val[(blk * 9 + t) * 64 + lane];
+            for (int t = 0; t < 9; ++t)
+            {
+                const VT* src = val + (blk * 9 + t) * 64 + lane;
+                if constexpr (NT)
+                    v[t] = __builtin_nontemporal_load(src);
+                else
+                    v[t] = *src;
+            }
             AT const x0 = AT(x[3 * m]), x1 = AT(x[3 * m + 1]), x2 = AT(x[3 * m + 2]);
             AT const n0 = a0 + (AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2);
             AT const n1 = a1 + (AT(v[3]) * x0 + AT(v[4]) * x1 + AT(v[5]) * x2);

@@ -150,6 +150,9 @@ SPMV_SHAPES = [
     (32, neon_amd.SPMV_PIPELINE), (32, neon_amd.SPMV_PIPELINE | neon_amd.SPMV_NONTEMPORAL),
     (32, neon_amd.SPMV_LDS), (32, neon_amd.SPMV_STREAM), (32, neon_amd.SPMV_META),
     (32, neon_amd.SPMV_SLICED), (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SLICED_WIDE),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL | neon_amd.SPMV_SWEEP),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL | neon_amd.SPMV_INDEX32),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP | neon_amd.SPMV_SLICED_WIDE),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_INDEX32),
