@@ -159,7 +159,7 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
  * `lanes` (16, 32) lanes per block row, the variant `flags` below and `blocks` workgroups
  * (0: default); average milliseconds per SpMV.  No counterpart in the reference: a tuning and
  * roofline probe. */
-#define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads */
+#define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads (CSR-order and sliced forms) */
 #define NEON_SPMV_SWEEP 2       /* workgroups of one XCD sweep its node range together */
 #define NEON_SPMV_PIPELINE 4    /* software-pipelined rows (32 lanes, sweep order) */
 #define NEON_SPMV_LDS 8         /* values staged through LDS in 16-byte loads (32 lanes, sweep order) */
@@ -174,8 +174,8 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
 int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
                          int32_t reps, double* avg_ms);
 /* Shape of the SpMV that the solvers on this mesh's tangent use from now on (lanes, flags as
- * above, workgroups for FP64 and for FP32 values; 0: by size).  Default: NEON_SPMV_SLICED,
- * workgroups by size.  The pipelined form applies to
+ * above, workgroups for FP64 and for FP32 values; 0: by size).  Default: NEON_SPMV_SLICED |
+ * NEON_SPMV_NONTEMPORAL, workgroups by size (one pass, one slice per wavefront).  The pipelined form applies to
  * meshes whose nodes have at most 32 neighbours and falls back to the plain one otherwise.
  * No counterpart in the reference: a tuning knob; results agree to rounding. */
 int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);
