@@ -35,7 +35,10 @@ enum : uint32_t
 };
 
 constexpr int kBlock = 256;        // threads per block for vector/SpMV kernels
-constexpr int kMaxPartials = 16384; // partial-sum slots for grid reductions
+#ifndef NEON_MAX_PARTIALS
+#define NEON_MAX_PARTIALS 16384
+#endif
+constexpr int kMaxPartials = NEON_MAX_PARTIALS; // partial-sum slots for grid reductions
 
 __device__ inline double wave_sum(double v)
 {

@@ -238,7 +238,13 @@ krylov_engine::~krylov_engine()
     if (host_scalar_) (void)hipHostFree(host_scalar_);
 }
 
-int krylov_engine::vec_blocks(int64_t n) const { return xcd_blocks((n + 4 * kBlock - 1) / (4 * kBlock), 2048); }
+#ifndef NEON_VEC_BLOCKS_MAX
+#define NEON_VEC_BLOCKS_MAX 1024
+#endif
+int krylov_engine::vec_blocks(int64_t n) const
+{
+    return xcd_blocks((n + 4 * kBlock - 1) / (4 * kBlock), NEON_VEC_BLOCKS_MAX);
+}
 
 void krylov_engine::ensure(int64_t n)
 {
@@ -307,6 +313,10 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
     NEON_CHECK_HIP(hipGetLastError());
 
     solve_result res;
+    // the gated true-residual product is a no-op in all but a few of its launches: a grid of at
+    // most 2,048 workgroups striding over the slices keeps those launches short
+    linear_operator Ar = A;
+    if (Ar.node_block && Ar.sell) Ar.blocks_override = 2048;
     int64_t launched = 0;
     for (;;)
     {
@@ -537,6 +547,10 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
     mixed_init_kernel<<<nb, kBlock, 0, s_>>>(n, b, x, r32, x32, p32, minv, part_rr, part_rz);
     mixed_init_state_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, st, tol, max_it);
     NEON_CHECK_HIP(hipGetLastError());
+    // the gated true-residual product is a no-op in all but a few of its launches: a grid of at
+    // most 2,048 workgroups striding over the slices keeps those launches short
+    linear_operator Ar = A;
+    if (Ar.node_block && Ar.sell) Ar.blocks_override = 2048;
     int64_t launched = 0;
     for (;;)
     {
@@ -558,7 +572,7 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             {
                 mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
                 mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, cur, 0);
-                A.residual64(s_, b, x, r64, cur, 1);
+                Ar.residual64(s_, b, x, r64, cur, 1);
                 mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
             }
             mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, nb,

@@ -168,10 +168,16 @@ public:
     // iterations between convergence polls from the host
     int poll_every = 16;
     // reliable-update trigger of the mixed-precision CG: ||r32|| < delta ||r at last update||
-    double reliable_delta = 0.1;
+#ifndef NEON_RELIABLE_DELTA
+#define NEON_RELIABLE_DELTA 0.1
+#endif
+#ifndef NEON_REFINE_EVERY
+#define NEON_REFINE_EVERY 8
+#endif
+    double reliable_delta = NEON_RELIABLE_DELTA;
     // the mixed CG tests the trigger (and, through the true residual, convergence) every this
     // many iterations
-    int refine_every = 4;
+    int refine_every = NEON_REFINE_EVERY;
 
 private:
     void ensure(int64_t n);

@@ -61,8 +61,11 @@ def test_bicgstab_host_csr_parity(ctx, criterion):
         np.testing.assert_allclose(x, x0, rtol=0, atol=1e-7 * np.abs(x0).max())
     else:
         # the CUDA rule stops on sqrt(r0 . r), which is not a residual norm (it can cancel
-        # early), so only the attained residuals are compared
-        assert res <= 10 * res0 + 1e-14 * np.linalg.norm(b)
+        # early), so only the attained residuals are compared, against the range the oracle's
+        # own residual spans over its thread counts (8.5e-8 to 1.2e-5 on this system: the
+        # cancellation follows the summation order of the dot products)
+        res_t = [np.linalg.norm(b - A @ ref(rp, ci, val, b, tol, 20000, threads=t)[1]) for t in (1, 2, 3, 4, 8)]
+        assert res <= 10 * max([res0] + res_t) + 1e-14 * np.linalg.norm(b), (res, res0, res_t)
 
 
 def test_bicgstab_unsymmetric_system(ctx):
@@ -187,8 +190,8 @@ def test_spmv_shapes_agree(ctx, precision):
             print("shape", lanes, flags, blocks, "iterations", its)
             results.append((x, its))
     x0, its0 = results[0]
-    # the mixed solver tests convergence every 4 iterations, so its counts move in steps of 4
-    slack = 2 if precision == neon_amd.PRECISION_FP64 else 6
+    # the mixed solver tests convergence every 8 iterations, so its counts move in steps of 8
+    slack = 2 if precision == neon_amd.PRECISION_FP64 else 10
     for x, its in results[1:]:
         assert abs(its - its0) <= slack, (its, its0)
         assert np.linalg.norm(x - x0) <= 1e-8 * np.linalg.norm(x0)
