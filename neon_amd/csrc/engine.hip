@@ -313,10 +313,6 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
     NEON_CHECK_HIP(hipGetLastError());
 
     solve_result res;
-    // the gated true-residual product is a no-op in all but a few of its launches: a grid of at
-    // most 2,048 workgroups striding over the slices keeps those launches short
-    linear_operator Ar = A;
-    if (Ar.node_block && Ar.sell) Ar.blocks_override = 2048;
     int64_t launched = 0;
     for (;;)
     {
