@@ -23,6 +23,7 @@
 #include "neon_hip.h"
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -373,6 +374,14 @@ public:
         o.precision = l.precision == "mixed" ? NEON_PRECISION_MIXED : NEON_PRECISION_FP64;
         if (o.precision == NEON_PRECISION_MIXED) o.solver_criterion = NEON_STOP_RELATIVE;
         o.record_trajectory = options_.record_trajectory ? 1 : 0;
+        if (on_step)
+        {
+            // neon calls mesh.write(step, time) here (static_matrix.hpp:159-186)
+            o.on_step = [](void* user, std::int32_t step, double time) {
+                (*static_cast<std::function<void(std::int32_t, double)>*>(user))(step, time);
+            };
+            o.callback_user = &on_step;
+        }
         std::vector<neon_dirichlet> bcs;
         for (auto const& b : boundaries_)
             bcs.push_back(neon_dirichlet{std::int32_t(b.dofs.size()), b.dofs.data(), std::int32_t(b.times.size()),
@@ -396,6 +405,11 @@ public:
         return u;
     }
 
+    /// called with (step, time) where neon writes its output: the initial state, after a
+    /// cut-back and after each converged step; the device state (neon_mesh_displacement,
+    /// neon_mesh_qp_state, neon_mesh_download) is current during the call
+    std::function<void(std::int32_t, double)> on_step;
+
     /// one record per Newton iteration (load step, iteration, Krylov count, norms)
     std::vector<neon_newton_record> const& records() const { return records_; }
     neon_newton_timing const& timing() const { return timing_; }
@@ -407,5 +421,68 @@ private:
     std::vector<nonfollower_load> loads_;
     std::vector<neon_newton_record> records_;
     neon_newton_timing timing_{};
+};
+/// The "eigen_solver" block of a natural-frequency step (eigen_solver_factory.cpp:12-77).
+struct eigen_solver_options
+{
+    std::string type = "arpack";
+    std::int64_t eigenvalues = 10;
+    std::string spectrum = "lower";
+};
+
+/// make_eigen_solver's checks: "arpack" maps to the device solver; "lanczos" and
+/// "power_iteration" need neon's OpenCL build and solve no generalised problem.
+inline void check_eigen_solver(eigen_solver_options const& o)
+{
+    if (o.eigenvalues <= 0) throw std::domain_error("The number of requested eigenvalues must be positive");
+    if (o.spectrum != "lower" && o.spectrum != "upper")
+        throw std::domain_error("A spectrum was requested but only \"lower\" and \"upper\" options are supported");
+    if (o.type == "power_iteration" || o.type == "lanczos")
+        throw std::domain_error("The \"" + o.type + "\" option requires OpenCL support.  Recompile with -DENABLE_OPENCL=1");
+    if (o.type.empty())
+        throw std::domain_error("Eigen solver type was not provided.  Please use \"power_iteration\", \"arpack\" or "
+                                "\"lanczos\"");
+    if (o.type != "arpack")
+        throw std::domain_error("Eigen solver type \"" + o.type + "\" is not recognised.  Please use \"arpack\"");
+}
+
+/// fem::natural_frequency_matrix (natural_frequency_matrix.hpp:17-161) on a device mesh: K at the
+/// mesh's held state and the consistent mass with neon's homogeneous Dirichlet treatment, then the
+/// eigenvalues of K x = lambda M x (increasing) and M-orthonormal mode shapes (3N per mode).
+class natural_frequency_matrix
+{
+public:
+    natural_frequency_matrix(device_mesh& m, eigen_solver_options o, std::vector<std::int32_t> fixed,
+                             std::vector<double> density)
+        : mesh_(m), options_(std::move(o)), fixed_(std::move(fixed)), density_(std::move(density))
+    {
+        check_eigen_solver(options_);
+        if (density_.size() == 1) density_.assign(std::size_t(m.elements()), density_[0]);
+        if (density_.size() != std::size_t(m.elements())) throw hip_error(NEON_BAD_ARGUMENT, "one density per element");
+    }
+
+    void solve(double const tolerance = 1.0e-8, std::int32_t const max_basis = 0, std::int32_t const block = 0)
+    {
+        std::int32_t const k = std::int32_t(options_.eigenvalues);
+        eigenvalues_.assign(k, 0.0);
+        eigenvectors_.assign(std::size_t(k) * mesh_.dofs(), 0.0);
+        check(mesh_.ctx().get(),
+              neon_natural_frequency(mesh_.ctx().get(), density_.data(), std::int32_t(fixed_.size()), fixed_.data(), k,
+                                     options_.spectrum == "upper" ? NEON_SPECTRUM_UPPER : NEON_SPECTRUM_LOWER,
+                                     tolerance, max_basis, block, eigenvalues_.data(), eigenvectors_.data(), &solves_));
+    }
+
+    std::vector<double> const& eigenvalues() const { return eigenvalues_; }
+    /// mode e at [e * 3N, (e + 1) * 3N)
+    std::vector<double> const& eigenvectors() const { return eigenvectors_; }
+    std::int32_t inner_solves() const { return solves_; }
+
+private:
+    device_mesh& mesh_;
+    eigen_solver_options options_;
+    std::vector<std::int32_t> fixed_;
+    std::vector<double> density_;
+    std::vector<double> eigenvalues_, eigenvectors_;
+    std::int32_t solves_ = 0;
 };
 } // namespace neon_hip

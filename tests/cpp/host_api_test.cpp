@@ -270,7 +270,11 @@ void traction_patch_test(neon_hip::context& ctx)
     o.linear.tolerance = 1e-13;
     o.linear.maximum_iterations = 5000;
     neon_hip::static_matrix sm(mesh, o, bcs, {load});
+    std::vector<double> hook_times;
+    sm.on_step = [&](std::int32_t, double time) { hook_times.push_back(time); };
     std::vector<double> const u = sm.solve();
+    expect(hook_times.size() == 3 && hook_times.front() == 0.0 && hook_times.back() == 1.0,
+           "the step hook sees the initial state and both converged steps");
     double lz = 1.0, l = 1.0;
     traction_stretches(bulk, shear, t, lz, l);
     double err = 0.0, umax = 0.0;
@@ -287,11 +291,77 @@ void traction_patch_test(neon_hip::context& ctx)
                 err / umax);
     expect(err <= 1e-9 * umax, "uniaxial neo-Hooke under a top traction matches the closed form");
 }
+// the axial modes of a bar whose nodes move only along it, base fixed: the 1-D linear chain
+// with consistent mass, lambda_k = 6 M_c / (rho h^2) (1 - cos t_k) / (2 + cos t_k),
+// t_k = (2k - 1) pi / (2N), M_c = K + 4G/3
+void bar_frequencies(neon_hip::context& ctx)
+{
+    int const N = 20;
+    double const s = 0.1, bulk = 1.0 / (3 * (1 - 2 * 0.3)), shear = 1.0 / (2 * 1.3), rho = 1.0;
+    std::int32_t nn = 0, ne = 0;
+    neon_mesh_box_size(1, 1, N, s, s, double(N), 0.0, &nn, &ne);
+    std::vector<double> X(3 * std::size_t(nn));
+    std::vector<std::int32_t> conn(8 * std::size_t(ne)), ijk(3 * std::size_t(nn));
+    neon_mesh_box(1, 1, N, s, s, double(N), 0.0, 1ull, 0.0, X.data(), conn.data(), ijk.data());
+    std::vector<double> params(4 * std::size_t(ne), 0.0);
+    for (std::int32_t e = 0; e < ne; ++e)
+    {
+        params[4 * e] = bulk;
+        params[4 * e + 1] = shear;
+    }
+    neon_hip::device_mesh mesh(ctx, X, conn, params, NEON_MODEL_LINEAR_ELASTIC);
+    std::vector<std::int32_t> fixed;
+    for (std::int32_t a = 0; a < nn; ++a)
+    {
+        fixed.push_back(3 * a);
+        fixed.push_back(3 * a + 1);
+        if (ijk[3 * a + 2] == 0) fixed.push_back(3 * a + 2);
+    }
+    neon_hip::eigen_solver_options eo;
+    eo.eigenvalues = 6;
+    neon_hip::natural_frequency_matrix fm(mesh, eo, fixed, {rho});
+    fm.solve();
+    double err = 0.0;
+    for (int k = 1; k <= 6; ++k)
+    {
+        double const t = (2 * k - 1) * M_PI / (2 * N);
+        double const exact = 6 * (bulk + 4 * shear / 3) / rho * (1 - std::cos(t)) / (2 + std::cos(t));
+        err = std::max(err, std::abs(fm.eigenvalues()[k - 1] - exact) / exact);
+    }
+    std::printf("      bar frequencies: %d inner solves, max relative error %.3e\n", fm.inner_solves(), err);
+    expect(err < 1e-8, "natural frequencies of the bar match the 1-D consistent-mass chain");
+}
+
+void eigen_factory_errors()
+{
+    auto throws = [](neon_hip::eigen_solver_options const& o, char const* what) {
+        try
+        {
+            neon_hip::check_eigen_solver(o);
+        }
+        catch (std::domain_error const& e)
+        {
+            return std::string(e.what()).find(what) != std::string::npos;
+        }
+        return false;
+    };
+    neon_hip::eigen_solver_options o;
+    neon_hip::check_eigen_solver(o);
+    o.eigenvalues = 0;
+    expect(throws(o, "must be positive"), "zero eigenvalues are rejected");
+    o.eigenvalues = 3;
+    o.spectrum = "middle";
+    expect(throws(o, "\"lower\" and \"upper\""), "an unknown spectrum is rejected");
+    o.spectrum = "upper";
+    o.type = "lanczos";
+    expect(throws(o, "OpenCL"), "lanczos needs neon's OpenCL build");
+}
 } // namespace
 
 int main()
 {
     factory_errors();
+    eigen_factory_errors();
     load_vectors();
     int devices = 0;
     neon_device_count(&devices);
@@ -314,6 +384,7 @@ int main()
     tridiagonal(ctx);
     patch_test(ctx);
     traction_patch_test(ctx);
+    bar_frequencies(ctx);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }

@@ -3,7 +3,8 @@ neon_amd/build.py with g++ against libneon_hip.so).
 
 Without a GPU the program checks the factory's errors and that creating a context fails
 loudly with NO_DEVICE (exit 77); on the GPU it also runs the reference's tridiagonal golden
-system through both solvers and a uniaxial neo-Hooke patch test against its closed form."""
+system through both solvers, uniaxial neo-Hooke patch tests against their closed forms (with the
+step hook) and the natural frequencies of a bar against the 1-D consistent-mass chain."""
 import pathlib
 import subprocess
 
