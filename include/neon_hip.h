@@ -23,6 +23,9 @@
  *                              (src/constitutive/constitutive_model.hpp:36)
  *   neon_mesh_save_internal_variables
  *                              mesh::save_internal_variables (mesh.cpp:73-77)
+ *   neon_mesh_set_ageing       ageing_micromechanical_elastomer (micromechanical_elastomer.cpp:34-53)
+ *   neon_mesh_ageing_state     the ageing model's internal variables
+ *                              (gaussian_ageing_affine_microsphere.cpp:17-50)
  *   neon_mesh_assemble         static_matrix::assemble_stiffness + compute_internal_force
  *                              (src/assembler/mechanics/static_matrix.hpp:189-203, 243-277) over
  *                              submesh::tangent_stiffness / internal_force (submesh.cpp:66-160)
@@ -68,6 +71,7 @@ extern "C" {
 #define NEON_MODEL_MICROSPHERE_GAUSSIAN 3 /* gaussian_affine_microsphere */
 #define NEON_MODEL_MICROSPHERE_LANGEVIN 4 /* affine_microsphere (Pade inverse Langevin chains) */
 #define NEON_MODEL_MICROSPHERE_NONAFFINE 5 /* nonaffine_microsphere */
+#define NEON_MODEL_MICROSPHERE_AGEING 6   /* gaussian_ageing_affine_microsphere (BAND network) */
 #define NEON_SPHERE_BO33 0x100            /* 33-direction rule instead of the 21-direction one */
 
 #define NEON_TANGENT_REFERENCE 0  /* neon's tangents (neo-Hooke D without 1/J) */
@@ -138,8 +142,20 @@ int neon_mesh_pattern(neon_context* ctx, int64_t* nnz, int32_t* row_ptr, int32_t
 int neon_mesh_set_displacement(neon_context* ctx, const double* u);
 /* The displacement held by the mesh (3N, host). */
 int neon_mesh_displacement(neon_context* ctx, double* u);
+/* time_step is the rate models' step (the ageing microsphere integrates its network over it
+ * when non-zero); neon passes the load increment at a step's first update and 0 otherwise
+ * (static_matrix.hpp:151, 407, 436).  The other models ignore it. */
 int neon_mesh_update_internal_variables(neon_context* ctx, double time_step);
 int neon_mesh_save_internal_variables(neon_context* ctx, int32_t converged);
+/* Ageing microsphere: the scission and recombination probabilities of the material
+ * (micromechanical_elastomer.cpp:34-53, both >= 0; 0 before the first call).  Its per-element
+ * params are (K, G, N, (1 - cure_time / 100) G): bulk modulus, shear modulus, segments per
+ * chain, initial inactive shear modulus (gaussian_ageing_affine_microsphere.cpp:17-50). */
+int neon_mesh_set_ageing(neon_context* ctx, double scission, double recombination);
+/* Ageing microsphere: network state per QP (index e*8+l, 11 values each): active and inactive
+ * shear modulus, reduction factor, active and inactive segments, secondary network's
+ * intermediate stress (11, 22, 33, 23, 13, 12). */
+int neon_mesh_ageing_state(neon_context* ctx, double* state);
 /* Per-QP state, index e*8+l.  Any pointer may be NULL.  sigma/eps_p 9 per QP (row-major
  * 3x3), D 36 per QP (6x6 Voigt, engineering shear in strain), detF/ebar/vm 1 per QP. */
 int neon_mesh_qp_state(neon_context* ctx, double* sigma, double* D, double* detF, double* eps_p, double* ebar,

@@ -261,6 +261,19 @@ public:
     {
         check(ctx_.get(), neon_mesh_save_internal_variables(ctx_.get(), have_converged ? 1 : 0));
     }
+    /// the ageing microsphere's scission and recombination probabilities
+    /// (micromechanical_elastomer.cpp:22-53)
+    void set_ageing(double const scission, double const recombination)
+    {
+        check(ctx_.get(), neon_mesh_set_ageing(ctx_.get(), scission, recombination));
+    }
+    /// the ageing microsphere's network state, 11 values per quadrature point
+    std::vector<double> ageing_state() const
+    {
+        std::vector<double> z(88 * std::size_t(elements_));
+        check(ctx_.get(), neon_mesh_ageing_state(ctx_.get(), z.data()));
+        return z;
+    }
     /// gradient of an objective with respect to the per-element parameters (4 per element) at
     /// the held state, through the adjoint of the consistent tangent
     std::vector<double> adjoint_gradient(std::vector<double> const& dJdu, std::vector<std::int32_t> const& fixed,

@@ -22,6 +22,7 @@ from ._capi import (  # noqa: F401
     MODEL_LINEAR_ELASTIC,
     MODEL_MICROSPHERE_GAUSSIAN,
     MODEL_MICROSPHERE_LANGEVIN,
+    MODEL_MICROSPHERE_AGEING,
     MODEL_MICROSPHERE_NONAFFINE,
     MODEL_NEOHOOKE,
     SPHERE_BO33,
@@ -216,7 +217,8 @@ class DeviceMesh:
     """Device-resident hex8 mesh, constitutive state and tangent (one per context)."""
 
     def __init__(self, X: np.ndarray, conn: np.ndarray, params: np.ndarray, model: int = MODEL_NEOHOOKE,
-                 tangent: int = TANGENT_REFERENCE, context: Context | None = None):
+                 tangent: int = TANGENT_REFERENCE, context: Context | None = None, ageing=None):
+        """`ageing` = (scission, recombination) probabilities of the ageing microsphere model."""
         self.ctx = context or Context(0)
         self.X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
         self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1)
@@ -227,6 +229,21 @@ class DeviceMesh:
         self.ctx.check(_capi.lib().neon_mesh_create(
             self.ctx.handle, self.n_nodes, _capi.f64(self.X), self.n_elem, _capi.i32(self.conn), model,
             _capi.f64(self.params), tangent))
+        if ageing is not None:
+            self.set_ageing(*ageing)
+
+    def set_ageing(self, scission: float, recombination: float) -> None:
+        """The ageing microsphere's scission and recombination probabilities
+        (``neon_mesh_set_ageing``)."""
+        self.ctx.check(_capi.lib().neon_mesh_set_ageing(self.ctx.handle, float(scission), float(recombination)))
+
+    def ageing_state(self) -> np.ndarray:
+        """(E*8, 11) network state of the ageing microsphere: active / inactive shear modulus,
+        reduction factor, active / inactive segments, secondary intermediate stress (11, 22, 33,
+        23, 13, 12)."""
+        out = np.zeros((8 * self.n_elem, 11))
+        self.ctx.check(_capi.lib().neon_mesh_ageing_state(self.ctx.handle, _capi.f64(out)))
+        return out
 
     @property
     def n_dofs(self) -> int:

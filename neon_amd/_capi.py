@@ -30,6 +30,7 @@ MODEL_LINEAR_ELASTIC = 2
 MODEL_MICROSPHERE_GAUSSIAN = 3
 MODEL_MICROSPHERE_LANGEVIN = 4
 MODEL_MICROSPHERE_NONAFFINE = 5
+MODEL_MICROSPHERE_AGEING = 6
 SPHERE_BO33 = 0x100  # microsphere rule flag: 33 directions instead of 21
 TANGENT_REFERENCE = 0
 TANGENT_CONSISTENT = 1
@@ -170,6 +171,8 @@ SIGNATURES = {
     "neon_mesh_pattern": (C.c_int, [C.c_void_p, P(C.c_int64), i32p, i32p]),
     "neon_mesh_set_displacement": (C.c_int, [C.c_void_p, f64p]),
     "neon_mesh_update_internal_variables": (C.c_int, [C.c_void_p, C.c_double]),
+    "neon_mesh_set_ageing": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "neon_mesh_ageing_state": (C.c_int, [C.c_void_p, C.c_void_p]),
     "neon_mesh_save_internal_variables": (C.c_int, [C.c_void_p, C.c_int32]),
     "neon_mesh_qp_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p]),
     "neon_mesh_assemble": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -244,11 +244,30 @@ int neon_mesh_displacement(neon_context* ctx, double* u)
     });
 }
 
-int neon_mesh_update_internal_variables(neon_context* ctx, double)
+int neon_mesh_update_internal_variables(neon_context* ctx, double time_step)
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        return ctx->mesh->update_internal_variables(ctx->mesh->u());
+        return ctx->mesh->update_internal_variables(ctx->mesh->u(), time_step);
+    });
+}
+
+int neon_mesh_set_ageing(neon_context* ctx, double scission, double recombination)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        ctx->mesh->set_ageing(scission, recombination);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_ageing_state(neon_context* ctx, double* state)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!state) throw status_error(NEON_BAD_ARGUMENT, "null pointer");
+        ctx->mesh->download_ageing_state(state);
+        return NEON_OK;
     });
 }
 

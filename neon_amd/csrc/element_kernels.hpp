@@ -48,7 +48,7 @@ __host__ __device__ constexpr int upper_index(int i, int j)
 
 __host__ __device__ constexpr bool is_microsphere(int model)
 {
-    return (model & 0xff) >= NEON_MODEL_MICROSPHERE_GAUSSIAN && (model & 0xff) <= NEON_MODEL_MICROSPHERE_NONAFFINE;
+    return (model & 0xff) >= NEON_MODEL_MICROSPHERE_GAUSSIAN && (model & 0xff) <= NEON_MODEL_MICROSPHERE_AGEING;
 }
 // constitutive_model::is_finite_deformation
 __host__ __device__ constexpr bool is_finite_model(int model)
@@ -86,6 +86,15 @@ struct qp_state
     int64_t Q;
     const double* eps_p_old; // committed history (consistent J2 returns from it)
     const double* ebar_old;
+    double* age;             // ageing microsphere: network state, 11 per QP (index k Q + qp)
+    const double* age_old;   // its committed copy
+};
+
+// Ageing microsphere: the material's scission and recombination probabilities, this update's
+// time step (0 inside a load step) and the last non-zero one (a model-wide value in neon).
+struct ageing_args
+{
+    double scission, recombination, dt, last_dt;
 };
 
 struct element_params
@@ -157,6 +166,9 @@ __device__ inline void add_outer21(double* c, double s, const double* v)
 //   neon's P [c_bar + 2/3 tr(tau_bar) I - 2/3 (tau_bar m^T + m tau_bar^T)] P, whose last term the
 //   projections annihilate.  The consistent mode adds the exact tangent's
 //   -2/3 (dev tau_bar m^T + m dev tau_bar^T) and divides by J.
+__device__ inline void microsphere_tangent(const double* tau, const double* cb, double J, double K, bool consistent,
+                                           double* sig, double* D);
+
 template <int BASE>
 __device__ void microsphere_update(const double* F, double J, const double* prm, int rule, bool consistent,
                                    double* sig, double* D)
@@ -271,6 +283,14 @@ __device__ void microsphere_update(const double* F, double J, const double* prm,
 #pragma unroll
         for (int k = 0; k < kFullCoefs; ++k) cb[k] *= G;
     }
+    microsphere_tangent(tau, cb, J, K, consistent, sig, D);
+}
+
+// sigma and D of the microsphere models from the macro stress tau_bar (kinetic Voigt) and the
+// macro moduli c_bar (21 upper-triangle entries), as described above microsphere_update.
+__device__ inline void microsphere_tangent(const double* tau, const double* cb, double J, double K, bool consistent,
+                                           double* sig, double* D)
+{
     double const pr = J * (K / 2.0 * (J - 1.0 / J));
     double const kappa = J * J * (K / 2.0 * (1.0 + 1.0 / (J * J)));
     double const tr = tau[0] + tau[1] + tau[2];
@@ -312,6 +332,120 @@ __device__ void microsphere_update(const double* F, double J, const double* prm,
         }
 }
 
+// Ageing microsphere (gaussian_ageing_affine_microsphere.cpp:52-134 with the BAND network of
+// micromechanical_elastomer.cpp:55-122): z = (active shear modulus G_a, inactive G_i, reduction
+// r, active segments N_a, inactive N_i) advances by one fourth-order Runge-Kutta step over dt when
+// dt != 0, with
+//   alpha = 1 - (1 - p_r)^(N_i + 1), beta = 1 - (1 - p_s)^N_a, eta = 1 - (1 - p_r)^(N_a + 1),
+//   nu = 1 - (1 - p_s)^N_i,
+//   G_a' = alpha G_i + G_a (2 eta - beta), G_i' = 2 beta G_a + G_i (nu - 2 alpha),
+//   r' = -r (beta + 2 eta), N_a' = (2 alpha G_i N_i - beta N_a G_a - N_a G_a') / G_a,
+//   N_i' = (beta N_a G_a - 2 alpha N_i G_i - N_i G_i') / G_i (0 while G_i <= 1e-8);
+// the secondary network's intermediate stress S = last_dt F_bar^-1 (c / r) F_bar^-T + S_old with
+// the creation rate c = alpha G_i + 4 eta G_a; macro stress r (3 G sum w t t + F_bar S F_bar^T);
+// then the Gaussian model's stress and tangent (zero macro moduli).
+__device__ inline void band_rates(const double* y, double ps, double pr, double* f)
+{
+    double const Ga = y[0], Gi = y[1], r = y[2], Na = y[3], Ni = y[4];
+    double const alpha = 1.0 - pow(1.0 - pr, Ni + 1.0);
+    double const beta = 1.0 - pow(1.0 - ps, Na);
+    double const eta = 1.0 - pow(1.0 - pr, Na + 1.0);
+    double const nu = 1.0 - pow(1.0 - ps, Ni);
+    f[0] = alpha * Gi + Ga * (2.0 * eta - beta);
+    f[1] = 2.0 * beta * Ga + Gi * (nu - 2.0 * alpha);
+    f[2] = -r * (beta + 2.0 * eta);
+    f[3] = (2.0 * alpha * Gi * Ni - beta * Na * Ga - Na * f[0]) / Ga;
+    f[4] = Gi > 1.0e-8 ? (beta * Na * Ga - 2.0 * alpha * Ni * Gi - Ni * f[1]) / Gi : 0.0;
+}
+
+__device__ void ageing_update(const double* F, double J, const double* prm, int rule, ageing_args ag, double* z,
+                              const double* S_old, double* S, double* sig, double* D)
+{
+    double const K = prm[0], G = prm[1];
+    if (ag.dt != 0.0)
+    {
+        double k1[5], k2[5], k3[5], k4[5], y[5];
+        band_rates(z, ag.scission, ag.recombination, k1);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+        {
+            k1[i] *= ag.dt;
+            y[i] = z[i] + k1[i] / 2.0;
+        }
+        band_rates(y, ag.scission, ag.recombination, k2);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+        {
+            k2[i] *= ag.dt;
+            y[i] = z[i] + k2[i] / 2.0;
+        }
+        band_rates(y, ag.scission, ag.recombination, k3);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+        {
+            k3[i] *= ag.dt;
+            y[i] = z[i] + k3[i];
+        }
+        band_rates(y, ag.scission, ag.recombination, k4);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+        {
+            k4[i] *= ag.dt;
+            z[i] = z[i] + (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0;
+        }
+    }
+    double const pr = ag.recombination;
+    double const alpha = 1.0 - pow(1.0 - pr, z[4] + 1.0);
+    double const eta = 1.0 - pow(1.0 - pr, z[3] + 1.0);
+    double const creation = alpha * z[1] + 4.0 * eta * z[0];
+    double const sc = pow(J, -1.0 / 3.0);
+    double Fb[9], Fi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fb[k] = sc * F[k];
+    inv3(Fb, Fi);
+    // primary network: 3 G sum w t t
+    double tau[6] = {0, 0, 0, 0, 0, 0};
+    int const nd = c_sphere.n[rule];
+    for (int i = 0; i < nd; ++i)
+    {
+        double const w = c_sphere.w[rule][i];
+        double const r0 = c_sphere.r[rule][i][0], r1 = c_sphere.r[rule][i][1], r2 = c_sphere.r[rule][i][2];
+        double t[3], vt[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) t[a] = Fb[a * 3] * r0 + Fb[a * 3 + 1] * r1 + Fb[a * 3 + 2] * r2;
+        sym_dyad(t, vt);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tau[k] += vt[k] * w;
+    }
+    // S = a F_bar^-1 F_bar^-T + S_old, a = last_dt c / r; components (11, 22, 33, 23, 13, 12)
+    double const a = ag.last_dt * creation / z[2];
+    constexpr int pi[6] = {0, 1, 2, 1, 0, 0}, pj[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        S[k] = a * (Fi[pi[k] * 3] * Fi[pj[k] * 3] + Fi[pi[k] * 3 + 1] * Fi[pj[k] * 3 + 1]
+                    + Fi[pi[k] * 3 + 2] * Fi[pj[k] * 3 + 2])
+               + S_old[k];
+    // macro = r (3 G primary + F_bar S F_bar^T)
+    double const Sm[9] = {S[0], S[5], S[4], S[5], S[1], S[3], S[4], S[3], S[2]};
+    double FS[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) FS[i * 3 + j] = Fb[i * 3] * Sm[j] + Fb[i * 3 + 1] * Sm[3 + j] + Fb[i * 3 + 2] * Sm[6 + j];
+    double macro[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+    {
+        int const i = pi[k], j = pj[k];
+        double const push = FS[i * 3] * Fb[j * 3] + FS[i * 3 + 1] * Fb[j * 3 + 1] + FS[i * 3 + 2] * Fb[j * 3 + 2];
+        macro[k] = z[2] * (3.0 * G * tau[k] + push);
+    }
+    double cb[kFullCoefs];
+#pragma unroll
+    for (int k = 0; k < kFullCoefs; ++k) cb[k] = 0.0;
+    microsphere_tangent(macro, cb, J, K, false, sig, D);
+}
+
 // ---------------------------------------------------- constitutive update per QP
 // One thread per quadrature point (qp = 8 e + l).
 template <int MODEL>
@@ -319,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
                                                               const double* __restrict__ X,
                                                               const double* __restrict__ u, element_params prm,
                                                               qp_state st, int tangent_mode, int rule,
-                                                              uint32_t* __restrict__ status)
+                                                              ageing_args ag, uint32_t* __restrict__ status)
 {
     int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (qp >= 8ll * E) return;
@@ -368,7 +502,19 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
     {
         double const pe[4] = {prm.p[4 * e], prm.p[4 * e + 1], prm.p[4 * e + 2], prm.p[4 * e + 3]};
         double sig[6], D[kFullCoefs];
-        microsphere_update<MODEL>(F, J, pe, rule, tangent_mode == NEON_TANGENT_CONSISTENT, sig, D);
+        if constexpr (MODEL == NEON_MODEL_MICROSPHERE_AGEING)
+        {
+            // network state advanced in place (neon's internal variables), the secondary
+            // network's stress from its committed value
+            double z[5], S_old[6], S[6];
+            for (int k = 0; k < 5; ++k) z[k] = st.age[k * Q + qp];
+            for (int k = 0; k < 6; ++k) S_old[k] = st.age_old[(5 + k) * Q + qp];
+            ageing_update(F, J, pe, rule, ag, z, S_old, S, sig, D);
+            for (int k = 0; k < 5; ++k) st.age[k * Q + qp] = z[k];
+            for (int k = 0; k < 6; ++k) st.age[(5 + k) * Q + qp] = S[k];
+        }
+        else
+            microsphere_update<MODEL>(F, J, pe, rule, tangent_mode == NEON_TANGENT_CONSISTENT, sig, D);
         bool finite = true;
         for (int k = 0; k < 6; ++k)
         {

@@ -763,8 +763,10 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
 {
     if (n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params)
         throw status_error(NEON_BAD_ARGUMENT, "empty mesh or null pointer");
-    if ((model & ~(0xff | NEON_SPHERE_BO33)) != 0 || (model & 0xff) > NEON_MODEL_MICROSPHERE_NONAFFINE)
+    if ((model & ~(0xff | NEON_SPHERE_BO33)) != 0 || (model & 0xff) > NEON_MODEL_MICROSPHERE_AGEING)
         throw status_error(NEON_BAD_ARGUMENT, "unknown model");
+    if ((model & 0xff) == NEON_MODEL_MICROSPHERE_AGEING && tangent_mode == NEON_TANGENT_CONSISTENT)
+        throw status_error(NEON_BAD_ARGUMENT, "the ageing microsphere model has neon's tangent only");
     if ((model & NEON_SPHERE_BO33) && !is_microsphere(model))
         throw status_error(NEON_BAD_ARGUMENT, "a unit-sphere rule is only for the microsphere models");
     if (tangent_mode != NEON_TANGENT_REFERENCE && tangent_mode != NEON_TANGENT_CONSISTENT)
@@ -976,6 +978,26 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
         NEON_CHECK_HIP(hipMemsetAsync(eps_p_old_.data(), 0, 6 * Q * sizeof(double), s_));
         NEON_CHECK_HIP(hipMemsetAsync(ebar_old_.data(), 0, Q * sizeof(double), s_));
     }
+    if ((model_ & 0xff) == NEON_MODEL_MICROSPHERE_AGEING)
+    {
+        // gaussian_ageing_affine_microsphere.cpp:17-50: active shear modulus G, inactive
+        // modulus (params[3]), reduction factor 1, active segments N, inactive segments 0,
+        // secondary stress 0
+        std::vector<double> a(11 * size_t(Q), 0.0);
+        for (int64_t q = 0; q < Q; ++q)
+        {
+            double const* p = params + 4 * (q / 8);
+            a[q] = p[1];
+            a[Q + q] = p[3];
+            a[2 * Q + q] = 1.0;
+            a[3 * Q + q] = p[2];
+        }
+        age_.resize(11 * Q);
+        age_old_.resize(11 * Q);
+        NEON_CHECK_HIP(hipMemcpyAsync(age_.data(), a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+        NEON_CHECK_HIP(hipMemcpyAsync(age_old_.data(), a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    }
     status_.resize(1);
     NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&status_host_), sizeof(uint32_t)));
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
@@ -993,17 +1015,43 @@ void device_mesh::set_displacement_host(const double* u)
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
 }
 
-int device_mesh::update_internal_variables(const double* u_dev)
+void device_mesh::set_ageing(double scission, double recombination)
+{
+    if ((model_ & 0xff) != NEON_MODEL_MICROSPHERE_AGEING)
+        throw status_error(NEON_BAD_ARGUMENT, "the mesh's model is not the ageing microsphere");
+    if (!(scission >= 0.0) || !(recombination >= 0.0))
+        throw status_error(NEON_BAD_ARGUMENT, "scission and recombination probabilities must be non-negative");
+    scission_ = scission;
+    recombination_ = recombination;
+}
+
+void device_mesh::download_ageing_state(double* state)
+{
+    if ((model_ & 0xff) != NEON_MODEL_MICROSPHERE_AGEING)
+        throw status_error(NEON_BAD_ARGUMENT, "the mesh's model is not the ageing microsphere");
+    int64_t const Q = 8ll * E_;
+    std::vector<double> a(11 * size_t(Q));
+    NEON_CHECK_HIP(hipMemcpyAsync(a.data(), age_.data(), a.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    for (int64_t q = 0; q < Q; ++q)
+        for (int k = 0; k < 11; ++k) state[11 * q + k] = a[k * Q + q];
+}
+
+int device_mesh::update_internal_variables(const double* u_dev, double dt)
 {
     int64_t const Q = 8ll * E_;
     qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q,
-                eps_p_old_.data(), ebar_old_.data()};
+                eps_p_old_.data(), ebar_old_.data(), age_.data(), age_old_.data()};
     NEON_CHECK_HIP(hipMemsetAsync(status_.data(), 0, sizeof(uint32_t), s_));
     element_params prm{params_.data()};
     int const rule = (model_ & NEON_SPHERE_BO33) ? 1 : 0;
+    // the ageing model keeps the last non-zero step for its secondary network
+    // (gaussian_ageing_affine_microsphere.cpp:84-103)
+    if (dt != 0.0) last_dt_ = dt;
+    ageing_args const ag{scission_, recombination_, dt, last_dt_};
 #define NEON_UPDATE(M)                                                                                             \
     update_state_kernel<M><<<grid_for(Q), kBlock, 0, s_>>>(E_, conn_.data(), X_.data(), u_dev, prm, st, tangent_mode_, \
-                                                          rule, status_.data())
+                                                          rule, ag, status_.data())
     switch (model_ & 0xff)
     {
     case NEON_MODEL_NEOHOOKE: NEON_UPDATE(NEON_MODEL_NEOHOOKE); break;
@@ -1011,7 +1059,8 @@ int device_mesh::update_internal_variables(const double* u_dev)
     case NEON_MODEL_LINEAR_ELASTIC: NEON_UPDATE(NEON_MODEL_LINEAR_ELASTIC); break;
     case NEON_MODEL_MICROSPHERE_GAUSSIAN: NEON_UPDATE(NEON_MODEL_MICROSPHERE_GAUSSIAN); break;
     case NEON_MODEL_MICROSPHERE_LANGEVIN: NEON_UPDATE(NEON_MODEL_MICROSPHERE_LANGEVIN); break;
-    default: NEON_UPDATE(NEON_MODEL_MICROSPHERE_NONAFFINE); break;
+    case NEON_MODEL_MICROSPHERE_NONAFFINE: NEON_UPDATE(NEON_MODEL_MICROSPHERE_NONAFFINE); break;
+    default: NEON_UPDATE(NEON_MODEL_MICROSPHERE_AGEING); break;
     }
 #undef NEON_UPDATE
     NEON_CHECK_HIP(hipGetLastError());
@@ -1026,11 +1075,21 @@ int device_mesh::update_internal_variables(const double* u_dev)
 }
 
 // internal_variables::commit / revert (src/constitutive/internal_variables.hpp:203-214).  Only
-// the J2 history is path dependent; every other variable is recomputed from u before use.
+// the J2 history and the ageing network are path dependent; every other variable is recomputed
+// from u before use.
 void device_mesh::save_internal_variables(bool converged)
 {
-    if (model_ != NEON_MODEL_J2) return;
     int64_t const Q = 8ll * E_;
+    if ((model_ & 0xff) == NEON_MODEL_MICROSPHERE_AGEING)
+    {
+        if (converged)
+            history_copy_kernel<<<grid_for(11 * Q), kBlock, 0, s_>>>(11 * Q, age_.data(), age_old_.data());
+        else
+            history_copy_kernel<<<grid_for(11 * Q), kBlock, 0, s_>>>(11 * Q, age_old_.data(), age_.data());
+        NEON_CHECK_HIP(hipGetLastError());
+        return;
+    }
+    if (model_ != NEON_MODEL_J2) return;
     if (converged)
     {
         history_copy_kernel<<<grid_for(6 * Q), kBlock, 0, s_>>>(6 * Q, eps_p_.data(), eps_p_old_.data());
@@ -1048,7 +1107,7 @@ void device_mesh::assemble(const double* u_dev, int precision)
 {
     int64_t const Q = 8ll * E_;
     qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q,
-                eps_p_old_.data(), ebar_old_.data()};
+                eps_p_old_.data(), ebar_old_.data(), age_.data(), age_old_.data()};
     bool const finite = is_finite_model(model_); // is_finite_deformation()
     bool const full = is_microsphere(model_);
     // neon integrates the small-strain models on the current configuration (submesh.cpp:68,84);

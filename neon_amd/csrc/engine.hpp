@@ -229,8 +229,14 @@ public:
 
     void set_parameters(const double* params);
     void set_displacement_host(const double* u);
-    // u (device) -> state; returns NEON_OK or the recoverable code
-    int update_internal_variables(const double* u_dev);
+    // u (device) -> state; returns NEON_OK or the recoverable code.  dt is the time step of
+    // the path-dependent models' rate equations: the step size at a load step's first update,
+    // 0 at its Newton iterates
+    int update_internal_variables(const double* u_dev, double dt = 0.0);
+    // ageing microsphere: the material's scission and recombination probabilities
+    void set_ageing(double scission, double recombination);
+    // ageing microsphere: network state, 11 per QP in QP order (host)
+    void download_ageing_state(double* state);
     void save_internal_variables(bool converged);
     void assemble(const double* u_dev, int precision);
     void enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision);
@@ -312,6 +318,8 @@ private:
     mutable device_array<float> s32_;
     mutable bool s64_fresh_ = false, s32_fresh_ = false;
     device_array<double> sigma_, coef_, detF_, eps_p_, ebar_, vm_, eps_p_old_, ebar_old_;
+    device_array<double> age_, age_old_; // ageing microsphere: 11 x Q network state (SoA)
+    double scission_ = 0.0, recombination_ = 0.0, last_dt_ = 0.0;
     device_array<uint32_t> status_;
     std::vector<std::unique_ptr<device_array<double>>> traj_u_, traj_ep_, traj_eb_;
     device_array<double> mu_ep_, mu_eb_;

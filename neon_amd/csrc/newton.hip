@@ -261,7 +261,8 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
                 NEON_CHECK_HIP(hipMemcpyAsync(fext.data(), fext_h.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
             }
             t0 = std::chrono::steady_clock::now();
-            err = m.update_internal_variables(u);
+            // the step's first update carries its increment to the rate models (:407)
+            err = m.update_internal_variables(u, step_time - last_time);
             tm.update_ms += ms_since(t0);
             if (err != NEON_OK) break;
 

@@ -66,6 +66,7 @@ class Problem:
     solution: str = "equilibrium"      # the step's "solution": "equilibrium" or "natural_frequency"
     eigen_solver: dict | None = None   # the step's "eigen_solver" block (natural frequency)
     density: np.ndarray | None = None  # (E,) the material's "density", when given
+    ageing: tuple | None = None        # ageing microsphere: (scission, recombination) probabilities
 
     @property
     def n_nodes(self) -> int:
@@ -111,11 +112,31 @@ def _material_params(material: dict, model: int) -> np.ndarray:
         if "segments_per_chain" not in material:
             raise ValueError("segments_per_chain not specified in material data\n")
         sy = float(material["segments_per_chain"])
+        if (model & 0xFF) == _capi.MODEL_MICROSPHERE_AGEING:
+            # ageing_micromechanical_elastomer (micromechanical_elastomer.cpp:22-53): the
+            # initial inactive shear modulus (1 - cure_time / 100) G
+            H = (1.0 - _ageing_material(material)[2] / 100.0) * G
         if (model & 0xFF) == _capi.MODEL_MICROSPHERE_NONAFFINE:
             if "nonaffine_stretch_parameter" not in material:
                 raise ValueError('"nonaffine_stretch_parameter" not specified in material data\n')
             H = float(material["nonaffine_stretch_parameter"])
     return np.array([K, G, sy, H])
+
+
+def _ageing_material(material: dict) -> tuple:
+    """(scission, recombination, cure_time) with the checks and messages of
+    micromechanical_elastomer.cpp:22-53."""
+    for key in ("scission_probability", "recombination_probability", "cure_time"):
+        if key not in material:
+            raise ValueError(f'"{key}" is not specified in "material" data')
+    cure = float(material["cure_time"])
+    if cure <= 10:
+        raise ValueError('The "cure_time" is less than 10.  This is likely a mistake.  '
+                         "Please provide a value between 10 and 100")
+    ps, pr = float(material["scission_probability"]), float(material["recombination_probability"])
+    if ps < 0.0 or pr < 0.0:
+        raise ValueError("Material properties (probabilities) must be positive")
+    return ps, pr, cure
 
 
 def _microsphere(c: dict) -> int:
@@ -136,7 +157,9 @@ def _microsphere(c: dict) -> int:
         if c["statistics"] == "gaussian" and "ageing" in c:
             if c["ageing"] != "BAND":
                 raise ValueError('The only microsphere ageing model supported is "BAND"')
-            raise ValueError("the ageing microsphere model does not run on the device path")
+            if rule is None:
+                raise ValueError("the FM900 unit-sphere rule does not run on the device path (BO21 and BO33 do)")
+            return _capi.MODEL_MICROSPHERE_AGEING | rule
         model = _capi.MODEL_MICROSPHERE_GAUSSIAN if c["statistics"] == "gaussian" else _capi.MODEL_MICROSPHERE_LANGEVIN
     elif c["type"] == "nonaffine":
         model = _capi.MODEL_MICROSPHERE_NONAFFINE
@@ -315,13 +338,14 @@ def read(path: str, step: int = 0) -> Problem:
 
     material = materials[part["material"]]
     density = np.full(len(conn), float(material["density"])) if "density" in material else None
+    ageing = _ageing_material(material)[:2] if (model & 0xFF) == _capi.MODEL_MICROSPHERE_AGEING else None
     if sim["solution"] == "natural_frequency":
         if density is None:
             # material_property::initial_density (material_property.cpp:34-42)
             raise ValueError("density was requested, but not specified in the input file\n")
         return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, {}, groups,
                        mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], dict(sim["eigen_solver"]),
-                       density)
+                       density, ageing)
     ls = dict(sim.get("linear_solver", {}))
     if "type" not in ls:
         raise ValueError('A "linear_solver" "type" must be specified')
@@ -334,4 +358,4 @@ def read(path: str, step: int = 0) -> Problem:
                          '"direct" or "iterative"')
     simulation = {"time": sim["time"], "nonlinear_options": sim.get("nonlinear_options", {}), "linear_solver": solver}
     return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, simulation, groups,
-                   mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], None, density)
+                   mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], None, density, ageing)

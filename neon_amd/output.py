@@ -193,7 +193,7 @@ def solve_with_output(problem, ctx, root: str = ".", precision: str = "fp64", ta
 
     tangent = neon_amd.TANGENT_REFERENCE if tangent is None else tangent
     mesh = neon_amd.DeviceMesh(problem.X, problem.conn, problem.params, model=problem.model, tangent=tangent,
-                               context=ctx)
+                               context=ctx, ageing=problem.ageing)
     sim = dict(problem.simulation)
     sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": precision})
     sm = neon_amd.static_matrix(mesh, sim, problem.boundaries, loads=problem.loads)

@@ -471,16 +471,11 @@ void microsphere_macro(int base, const sphere_rule& q, const double* Fb, double 
 // The projections annihilate the last bracket term, so D lacks the -2/3 (dev tau_bar (x) 1 +
 // 1 (x) dev tau_bar) of the exact Kirchhoff tangent; the consistent mode adds it and divides
 // by J, which makes K = d f_int / d u.
-void microsphere(int model, const double* F, double J, const double* prm, int consistent, double* sigma, double* D)
+// The volumetric part and the projections shared by the microsphere models, from the macro
+// stress tau_bar (3x3) and macro moduli c_bar (6x6, NULL for zero).
+void microsphere_finish(const double* tau, const double* cb, double J, double K, int consistent, double* sigma,
+                        double* D)
 {
-    int const base = model & 0xff;
-    sphere_rule const& q = sphere(model & ORC_SPHERE_BO33);
-    double const K = prm[0], G = prm[1], N = prm[2], p = prm[3];
-    double Fb[9];
-    double const s = std::pow(det3(F), -1.0 / 3.0); // unimodular (tensor_operations.hpp:70-73)
-    for (int k = 0; k < 9; ++k) Fb[k] = s * F[k];
-    double tau[9], cb[36];
-    microsphere_macro(base, q, Fb, G, N, p, tau, cb);
     double const pressure = J * (K / 2.0 * (J - 1.0 / J));
     double const kappa = std::pow(J, 2) * (K / 2.0 * (1.0 + 1.0 / std::pow(J, 2)));
     double dtau[9];
@@ -498,7 +493,7 @@ void microsphere(int model, const double* F, double J, const double* prm, int co
     double const trt = tau[0] + tau[4] + tau[8];
     for (int i = 0; i < 6; ++i)
         for (int j = 0; j < 6; ++j)
-            Dm[i * 6 + j] = cb[i * 6 + j] + (i == j ? 2.0 / 3.0 * trt * Ikin[i] : 0.0)
+            Dm[i * 6 + j] = (cb ? cb[i * 6 + j] : 0.0) + (i == j ? 2.0 / 3.0 * trt * Ikin[i] : 0.0)
                             - 2.0 / 3.0 * (tI[i * 6 + j] + It[i * 6 + j]);
     matmul6(Pdev, Dm, PD);
     matmul6(PD, Pdev, PDP);
@@ -515,11 +510,108 @@ void microsphere(int model, const double* F, double J, const double* prm, int co
     }
 }
 
+void microsphere(int model, const double* F, double J, const double* prm, int consistent, double* sigma, double* D)
+{
+    int const base = model & 0xff;
+    sphere_rule const& q = sphere(model & ORC_SPHERE_BO33);
+    double const K = prm[0], G = prm[1], N = prm[2], p = prm[3];
+    double Fb[9];
+    double const s = std::pow(det3(F), -1.0 / 3.0); // unimodular (tensor_operations.hpp:70-73)
+    for (int k = 0; k < 9; ++k) Fb[k] = s * F[k];
+    double tau[9], cb[36];
+    microsphere_macro(base, q, Fb, G, N, p, tau, cb);
+    microsphere_finish(tau, cb, J, K, consistent, sigma, D);
+}
+
 inline bool is_microsphere(int model)
 {
     int const b = model & 0xff;
     return b == ORC_MODEL_MICROSPHERE_GAUSSIAN || b == ORC_MODEL_MICROSPHERE_LANGEVIN
-           || b == ORC_MODEL_MICROSPHERE_NONAFFINE;
+           || b == ORC_MODEL_MICROSPHERE_NONAFFINE || b == ORC_MODEL_MICROSPHERE_AGEING;
+}
+
+// ageing_micromechanical_elastomer::integrate (micromechanical_elastomer.cpp:70-122): one
+// fourth-order Runge-Kutta step (runge_kutta_integration.hpp:23-32) of the network evolution
+// z = (active shear modulus, inactive shear modulus, reduction factor, active segments,
+// inactive segments) over dt.
+void band_rates(const double* y, double ps, double pr, double* f)
+{
+    double const Ga = y[0], Gi = y[1], r = y[2], Na = y[3], Ni = y[4];
+    double const alpha = 1.0 - std::pow(1.0 - pr, Ni + 1.0); // inactive set recombination
+    double const beta = 1.0 - std::pow(1.0 - ps, Na);        // active set scission
+    double const eta = 1.0 - std::pow(1.0 - pr, Na + 1.0);   // active set generation
+    double const nu = 1.0 - std::pow(1.0 - ps, Ni);          // inactive set generation
+    f[0] = alpha * Gi + Ga * (2.0 * eta - beta);
+    f[1] = 2.0 * beta * Ga + Gi * (nu - 2.0 * alpha);
+    f[2] = -r * (beta + 2.0 * eta);
+    f[3] = (2.0 * alpha * Gi * Ni - beta * Na * Ga - Na * f[0]) / Ga;
+    f[4] = Gi > 1.0e-8 ? (beta * Na * Ga - 2.0 * alpha * Ni * Gi - Ni * f[1]) / Gi : 0.0;
+}
+
+void band_integrate(double* z, double dt, double ps, double pr)
+{
+    double k1[5], k2[5], k3[5], k4[5], y[5];
+    band_rates(z, ps, pr, k1);
+    for (int i = 0; i < 5; ++i) k1[i] *= dt;
+    for (int i = 0; i < 5; ++i) y[i] = z[i] + k1[i] / 2.0;
+    band_rates(y, ps, pr, k2);
+    for (int i = 0; i < 5; ++i) k2[i] *= dt;
+    for (int i = 0; i < 5; ++i) y[i] = z[i] + k2[i] / 2.0;
+    band_rates(y, ps, pr, k3);
+    for (int i = 0; i < 5; ++i) k3[i] *= dt;
+    for (int i = 0; i < 5; ++i) y[i] = z[i] + k3[i];
+    band_rates(y, ps, pr, k4);
+    for (int i = 0; i < 5; ++i) k4[i] *= dt;
+    for (int i = 0; i < 5; ++i) z[i] = z[i] + (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0;
+}
+
+// gaussian_ageing_affine_microsphere::update_internal_variables for one QP
+// (gaussian_ageing_affine_microsphere.cpp:52-134): the network ODE over dt when dt != 0, the
+// creation rate (micromechanical_elastomer.cpp:55-67), the primary network's Gaussian macro
+// stress 3 G sum w t t, the secondary network's intermediate stress by an implicit Euler step
+// S = last_dt F_bar^-1 (c / r I) F_bar^-T + S_old, macro stress r (primary + F_bar S F_bar^T),
+// and the Gaussian model's stress and tangent (zero macro moduli).
+void ageing(int model, const double* F, double J, const double* prm, double ps, double pr, double dt,
+            double last_dt, double* z, const double* S_old, double* S, double* sigma, double* D)
+{
+    sphere_rule const& q = sphere(model & ORC_SPHERE_BO33);
+    double const K = prm[0], G = prm[1];
+    if (dt != 0.0) band_integrate(z, dt, ps, pr);
+    double const alpha = 1.0 - std::pow(1.0 - pr, z[4] + 1.0);
+    double const eta = 1.0 - std::pow(1.0 - pr, z[3] + 1.0);
+    double const creation = alpha * z[1] + 4.0 * eta * z[0];
+    double Fb[9], Fbi[9];
+    double const sc = std::pow(det3(F), -1.0 / 3.0);
+    for (int k = 0; k < 9; ++k) Fb[k] = sc * F[k];
+    inv3(Fb, Fbi);
+    double primary[9], cb[36];
+    microsphere_macro(ORC_MODEL_MICROSPHERE_GAUSSIAN, q, Fb, G, 0.0, 0.0, primary, cb);
+    // S (tensor order 11, 22, 33, 23, 13, 12) = last_dt F_bar^-1 (c / r) F_bar^-T + S_old
+    double const a = last_dt * creation / z[2];
+    double St[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += Fbi[i * 3 + k] * Fbi[j * 3 + k];
+            St[i * 3 + j] = a * v;
+        }
+    static const int pair[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+    for (int k = 0; k < 6; ++k)
+    {
+        S[k] = St[pair[k][0] * 3 + pair[k][1]] + S_old[k];
+        St[pair[k][0] * 3 + pair[k][1]] = St[pair[k][1] * 3 + pair[k][0]] = S[k];
+    }
+    double FS[9], macro[9];
+    matmul3(Fb, St, FS);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += FS[i * 3 + k] * Fb[j * 3 + k];
+            macro[i * 3 + j] = z[2] * (primary[i * 3 + j] + v);
+        }
+    microsphere_finish(macro, nullptr, J, K, 0, sigma, D);
 }
 // constitutive_model::is_finite_deformation
 inline bool is_finite(int model) { return model == ORC_MODEL_NEOHOOKE || is_microsphere(model); }
@@ -651,6 +743,10 @@ struct orc_model
     vector<double> Hg, F, sigma, detF, eps_p, ebar, vm;
     vector<double> Hg_o, F_o, sigma_o, detF_o, eps_p_o, ebar_o, vm_o;
     vector<double> D; // fourth-order tangent, not saved
+    // ageing microsphere: network state (11 per QP) and its committed copy, the material's
+    // probabilities and the model's last non-zero time step (a model-wide member in neon)
+    vector<double> age, age_o;
+    double scission = 0.0, recombination = 0.0, last_dt = 0.0;
     // recorded load history: u_n and the committed history h_{n-1} of each converged step
     vector<vector<double>> traj_u, traj_ep, traj_eb;
 };
@@ -675,7 +771,8 @@ int orc_neohooke_qp(const double* F, double bulk, double shear, double* sigma, d
 int orc_microsphere_qp(int32_t model, const double* F, const double* params, int32_t consistent, double* sigma,
                        double* D)
 {
-    if (!F || !params || !sigma || !D || !is_microsphere(model)) return ORC_BAD_ARGUMENT;
+    if (!F || !params || !sigma || !D || !is_microsphere(model) || (model & 0xff) == ORC_MODEL_MICROSPHERE_AGEING)
+        return ORC_BAD_ARGUMENT;
     microsphere(model, F, det3(F), params, consistent, sigma, D);
     return ORC_OK;
 }
@@ -703,7 +800,8 @@ int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int
 {
     if (!out || n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params) return ORC_BAD_ARGUMENT;
     int const base = model & 0xff;
-    if ((model & ~(0xff | ORC_SPHERE_BO33)) != 0 || base > ORC_MODEL_MICROSPHERE_NONAFFINE) return ORC_BAD_ARGUMENT;
+    if ((model & ~(0xff | ORC_SPHERE_BO33)) != 0 || base > ORC_MODEL_MICROSPHERE_AGEING) return ORC_BAD_ARGUMENT;
+    if (base == ORC_MODEL_MICROSPHERE_AGEING && tangent_mode == ORC_TANGENT_CONSISTENT) return ORC_BAD_ARGUMENT;
     if ((model & ORC_SPHERE_BO33) && !is_microsphere(model)) return ORC_BAD_ARGUMENT;
     for (int64_t i = 0; i < int64_t(n_elem) * 8; ++i)
         if (conn[i] < 0 || conn[i] >= n_nodes) return ORC_BAD_ARGUMENT;
@@ -729,6 +827,22 @@ int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int
     if (model == ORC_MODEL_J2 || model == ORC_MODEL_LINEAR_ELASTIC) // isotropic_linear_elasticity.cpp:22-23
         for (size_t q = 0; q < Q; ++q)
             elastic_moduli(m->params[4 * (q / 8)], m->params[4 * (q / 8) + 1], &m->D[36 * q]);
+    if (base == ORC_MODEL_MICROSPHERE_AGEING)
+    {
+        // gaussian_ageing_affine_microsphere.cpp:17-50: active modulus G, inactive modulus
+        // (1 - cure_time / 100) G, reduction 1, active segments N, inactive segments 0
+        m->age.assign(11 * Q, 0.0);
+        for (size_t q = 0; q < Q; ++q)
+        {
+            double const* p = &m->params[4 * (q / 8)];
+            double* a = &m->age[11 * q];
+            a[0] = p[1];
+            a[1] = p[3];
+            a[2] = 1.0;
+            a[3] = p[2];
+        }
+        m->age_o = m->age;
+    }
     m->Hg_o = m->Hg;
     m->F_o = m->F;
     m->sigma_o = m->sigma;
@@ -751,7 +865,7 @@ int orc_model_n_dofs(const orc_model* m) { return 3 * m->N; }
 // mesh::update_internal_variables (src/mesh/mechanics/solid/mesh.cpp:57-71) ->
 // material_coordinates::update_current_configuration (src/mesh/material_coordinates.cpp:23-27)
 // -> submesh::update_internal_variables (submesh.cpp:194-208).
-int orc_update_internal_variables(orc_model* m, const double* u, double /*dt*/)
+int orc_update_internal_variables(orc_model* m, const double* u, double dt)
 {
     std::feclearexcept(FE_ALL_EXCEPT);
     for (int64_t i = 0; i < 3 * int64_t(m->N); ++i) m->x[i] = m->X[i] + u[i];
@@ -818,6 +932,15 @@ int orc_update_internal_variables(orc_model* m, const double* u, double /*dt*/)
             linear_elastic(&m->Hg[9 * q], p[0], p[1], &m->sigma[9 * q], &m->D[36 * q], &m->vm[q]);
         }
     }
+    else if ((m->model & 0xff) == ORC_MODEL_MICROSPHERE_AGEING)
+    {
+        if (dt != 0.0) m->last_dt = dt; // gaussian_ageing_affine_microsphere.cpp:84-103
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < int64_t(Q); ++q)
+            ageing(m->model, &m->F[9 * q], m->detF[q], &m->params[4 * (q / 8)], m->scission, m->recombination, dt,
+                   m->last_dt, &m->age[11 * q], &m->age_o[11 * q + 5], &m->age[11 * q + 5], &m->sigma[9 * q],
+                   &m->D[36 * q]);
+    }
     else if (is_microsphere(m->model))
     {
         int const consistent = m->tangent_mode == ORC_TANGENT_CONSISTENT;
@@ -867,6 +990,7 @@ int orc_save_internal_variables(orc_model* m, int32_t converged)
         m->eps_p_o = m->eps_p;
         m->ebar_o = m->ebar;
         m->vm_o = m->vm;
+        m->age_o = m->age;
     }
     else
     {
@@ -877,7 +1001,23 @@ int orc_save_internal_variables(orc_model* m, int32_t converged)
         m->eps_p = m->eps_p_o;
         m->ebar = m->ebar_o;
         m->vm = m->vm_o;
+        m->age = m->age_o;
     }
+    return ORC_OK;
+}
+
+int orc_model_set_ageing(orc_model* m, double scission, double recombination)
+{
+    if (!m || scission < 0.0 || recombination < 0.0) return ORC_BAD_ARGUMENT;
+    m->scission = scission;
+    m->recombination = recombination;
+    return ORC_OK;
+}
+
+int orc_get_ageing_state(const orc_model* m, double* state)
+{
+    if (!m || !state || m->age.empty()) return ORC_BAD_ARGUMENT;
+    std::memcpy(state, m->age.data(), m->age.size() * sizeof(double));
     return ORC_OK;
 }
 

@@ -57,6 +57,7 @@ enum {
     ORC_MODEL_MICROSPHERE_GAUSSIAN = 3, /* gaussian_affine_microsphere */
     ORC_MODEL_MICROSPHERE_LANGEVIN = 4, /* affine_microsphere */
     ORC_MODEL_MICROSPHERE_NONAFFINE = 5,/* nonaffine_microsphere */
+    ORC_MODEL_MICROSPHERE_AGEING = 6,   /* gaussian_ageing_affine_microsphere (BAND) */
     ORC_SPHERE_BO33 = 0x100
 };
 
@@ -103,6 +104,14 @@ int orc_model_n_dofs(const orc_model* m);
  * ORC_NAN_OR_INF or ORC_RETURN_MAP_FAILED like the reference's computational_error. */
 int orc_update_internal_variables(orc_model* m, const double* u, double dt);
 int orc_save_internal_variables(orc_model* m, int32_t converged);
+
+/* Ageing microsphere: scission and recombination probabilities (the mesh's material); the
+ * per-element params are (K, G, segments per chain, initial inactive shear modulus
+ * (1 - cure_time / 100) G).  Network state per QP (11 values): active and inactive shear
+ * moduli, reduction factor, active and inactive segments, and the secondary network's
+ * intermediate Kirchhoff stress (11, 22, 33, 23, 13, 12). */
+int orc_model_set_ageing(orc_model* m, double scission, double recombination);
+int orc_get_ageing_state(const orc_model* m, double* state /*11/qp*/);
 
 /* Per-QP state downloads (QP index e*8 + l). */
 int orc_get_qp_state(const orc_model* m, double* sigma /*9/qp or NULL*/, double* D /*36/qp or NULL*/,

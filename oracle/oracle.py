@@ -21,6 +21,7 @@ OK, NONCONVERGED, NAN_OR_INF, NEGATIVE_JACOBIAN, RETURN_MAP_FAILED, BREAKDOWN = 
 FATAL, BAD_ARGUMENT = -1, -2
 MODEL_NEOHOOKE, MODEL_J2, MODEL_LINEAR_ELASTIC = 0, 1, 2
 MODEL_MICROSPHERE_GAUSSIAN, MODEL_MICROSPHERE_LANGEVIN, MODEL_MICROSPHERE_NONAFFINE = 3, 4, 5
+MODEL_MICROSPHERE_AGEING = 6
 SPHERE_BO33 = 0x100
 STOP_RELATIVE, STOP_ABSOLUTE_PRECONDITIONED = 0, 1
 TANGENT_REFERENCE, TANGENT_CONSISTENT = 0, 1
@@ -60,6 +61,8 @@ _SIG = {
     "orc_neohooke_qp": [f64p, C.c_double, C.c_double, f64p, f64p],
     "orc_microsphere_qp": [C.c_int32, f64p, f64p, C.c_int32, f64p, f64p],
     "orc_sphere_rule": [C.c_int32, f64p, f64p],
+    "orc_model_set_ageing": [C.c_void_p, C.c_double, C.c_double],
+    "orc_get_ageing_state": [C.c_void_p, f64p],
     "orc_j2_qp": [f64p, C.c_double, C.c_double, C.c_double, C.c_double, f64p, f64p, f64p, f64p, f64p],
     "orc_model_create": [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, f64p, C.c_int32, P(C.c_void_p)],
     "orc_model_destroy": [C.c_void_p],
@@ -241,6 +244,18 @@ class Model:
         val, f = np.zeros(len(ci)), np.zeros(self.n_dofs)
         lib().orc_assemble(self.h, _i(rp), _i(ci), _f(val), _f(f), threads)
         return val, f
+
+    def set_ageing(self, scission, recombination):
+        if lib().orc_model_set_ageing(self.h, scission, recombination) != OK:
+            raise RuntimeError("orc_model_set_ageing failed")
+
+    def ageing_state(self):
+        """(E*8, 11): active / inactive shear modulus, reduction factor, active / inactive
+        segments, secondary intermediate stress (11, 22, 33, 23, 13, 12)."""
+        out = np.zeros((8 * self.n_elem, 11))
+        if lib().orc_get_ageing_state(self.h, _f(out)) != OK:
+            raise RuntimeError("orc_get_ageing_state failed")
+        return out
 
     def element_mass(self, e, density):
         me = np.zeros((24, 24))

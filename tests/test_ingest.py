@@ -125,6 +125,15 @@ def test_ingest_rejects_what_the_device_path_does_not_run(tmp_path):
                "ageing": "other"}, [], "BAND"),
         (mat, {"name": "microsphere", "type": "affine", "quadrature": "FM900", "statistics": "gaussian"}, [],
          "FM900"),
+        (dict(mat, segments_per_chain=20, recombination_probability=1e-5, cure_time=80),
+         {"name": "microsphere", "type": "affine", "quadrature": "BO21", "statistics": "gaussian", "ageing": "BAND"},
+         [], '"scission_probability" is not specified'),
+        (dict(mat, segments_per_chain=20, scission_probability=1e-5, recombination_probability=1e-5, cure_time=10),
+         {"name": "microsphere", "type": "affine", "quadrature": "BO21", "statistics": "gaussian", "ageing": "BAND"},
+         [], "less than 10"),
+        (dict(mat, segments_per_chain=20, scission_probability=-1e-5, recombination_probability=1e-5, cure_time=50),
+         {"name": "microsphere", "type": "affine", "quadrature": "BO21", "statistics": "gaussian", "ageing": "BAND"},
+         [], "must be positive"),
         (mat, {"name": "microsphere", "type": "cubic", "quadrature": "BO21"}, [], '"affine" or "nonaffine"'),
         (mat, {"name": "microsphere", "type": "affine", "quadrature": "BO21", "statistics": "gaussian"}, [],
          "segments_per_chain"),
