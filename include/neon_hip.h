@@ -32,6 +32,9 @@
  *   neon_mesh_enforce_dirichlet static_matrix::enforce_dirichlet_conditions (static_matrix.hpp:279-320)
  *   neon_mesh_solve            linear_solver::solve on the device-resident tangent (static_matrix.hpp:432)
  *   neon_newton_solve          static_matrix::solve (static_matrix.hpp:145-187, 402-467)
+ *   neon_latin_solve           latin_matrix::solve (src/assembler/mechanics/latin_matrix.hpp:150-244)
+ *                              with latin_submesh::incremental_latin_internal_force
+ *                              (src/mesh/mechanics/solid/latin_submesh.hpp:62-111)
  *   neon_natural_frequency     natural_frequency_matrix::solve (natural_frequency_matrix.hpp:54-68)
  *   neon_mesh_mass             natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:106-137)
  *   neon_adjoint_gradient      no counterpart in the reference (new: adjoint of the converged state)
@@ -293,6 +296,17 @@ int neon_newton_solve_loaded(neon_context* ctx, const neon_newton_options* optio
                              const neon_dirichlet* bcs, int32_t n_loads, const neon_neumann* loads,
                              double* u_out, neon_newton_record* records, int32_t record_capacity,
                              int32_t* n_records, neon_newton_timing* timing);
+/* latin_matrix::solve (latin_matrix.hpp:150-244): the iterations of neon_newton_solve_loaded,
+ * except that after each load step's first iteration the right-hand side of K delta = r is the
+ * LATIN force f = sum_e int B^T (sigma_old + s D (eps - eps_old) - sigma) dV on the current
+ * configuration (latin_submesh.hpp:62-111; sigma_old and eps_old the committed stress and
+ * strain, D the QP tangent, s = search_direction, neon's "latin_search_direction"), with the
+ * constrained entries zeroed; convergence is judged on f_ext - f_int as for Newton.  Needs
+ * NEON_TANGENT_REFERENCE. */
+int neon_latin_solve(neon_context* ctx, const neon_newton_options* options, double search_direction, int32_t n_bc,
+                     const neon_dirichlet* bcs, int32_t n_loads, const neon_neumann* loads, double* u_out,
+                     neon_newton_record* records, int32_t record_capacity, int32_t* n_records,
+                     neon_newton_timing* timing);
 
 /* ---- adjoint (no counterpart in the reference) ----
  * Gradient of an objective J(u) of the final displacement with respect to the per-element

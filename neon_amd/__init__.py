@@ -55,6 +55,7 @@ __all__ = [
     "biconjugate_gradient_stabilised_hip",
     "make_linear_solver",
     "static_matrix",
+    "latin_matrix",
     "natural_frequency_matrix",
     "make_eigen_solver",
     "computational_error",
@@ -450,6 +451,7 @@ class static_matrix:
         self.boundaries = boundaries
         self.records: list[dict] = []
         self.timing: dict = {}
+        self.latin_search_direction = None
 
     def solve(self, capacity: int = 4096, on_step=None) -> np.ndarray:
         """Runs every load step.  ``on_step(step, time)`` is called where neon writes its
@@ -474,7 +476,13 @@ class static_matrix:
         nrec = C.c_int32(0)
         tm = _capi.NewtonTiming()
         bcs, keep = _bc_structs(self.boundaries)
-        if self.loads:
+        if self.latin_search_direction is not None:
+            lds, keep_l = _load_structs(self.loads) if self.loads else (None, None)
+            code = _capi.lib().neon_latin_solve(m.ctx.handle, C.byref(self.options), self.latin_search_direction,
+                                                len(self.boundaries), bcs, len(self.loads), lds, _capi.f64(u), recs,
+                                                capacity, C.byref(nrec), C.byref(tm))
+            del keep_l
+        elif self.loads:
             lds, keep_l = _load_structs(self.loads)
             code = _capi.lib().neon_newton_solve_loaded(m.ctx.handle, C.byref(self.options), len(self.boundaries),
                                                         bcs, len(self.loads), lds, _capi.f64(u), recs, capacity,
@@ -496,6 +504,20 @@ class static_matrix:
             raise failure[0]
         m.ctx.check(code)
         return u
+
+
+class latin_matrix(static_matrix):
+    """LATIN driver (``latin_matrix``, src/assembler/mechanics/latin_matrix.hpp:150-244): the
+    load stepping and norms of :class:`static_matrix`; after each step's first iteration the
+    right-hand side is the LATIN force of ``latin_submesh::incremental_latin_internal_force``
+    (latin_submesh.hpp:62-111) with ``nonlinear_options["latin_search_direction"]``."""
+
+    def __init__(self, mesh: DeviceMesh, simulation: dict, boundaries: list, loads: list | None = None):
+        super().__init__(mesh, simulation, boundaries, loads)
+        nl = simulation["nonlinear_options"]
+        if "latin_search_direction" not in nl:
+            raise ValueError("latin_search_direction not specified in nonlinear_options")
+        self.latin_search_direction = float(nl["latin_search_direction"])
 
 
 def make_eigen_solver(data: dict) -> dict:

@@ -203,6 +203,11 @@ SIGNATURES = {
         [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p, P(NewtonRecord),
          C.c_int32, i32p, P(NewtonTiming)],
     ),
+    "neon_latin_solve": (
+        C.c_int,
+        [C.c_void_p, P(NewtonOptions), C.c_double, C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p,
+         P(NewtonRecord), C.c_int32, i32p, P(NewtonTiming)],
+    ),
     "neon_surface_load_vector": (C.c_int, [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, C.c_int32, f64p]),
     "neon_volume_load_vector": (C.c_int, [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, C.c_int32, f64p]),
     "neon_mesh_box": (

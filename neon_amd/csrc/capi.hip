@@ -546,6 +546,25 @@ int neon_newton_solve_loaded(neon_context* ctx, const neon_newton_options* optio
     });
 }
 
+int neon_latin_solve(neon_context* ctx, const neon_newton_options* options, double search_direction, int32_t n_bc,
+                     const neon_dirichlet* bcs, int32_t n_loads, const neon_neumann* loads, double* u_out,
+                     neon_newton_record* records, int32_t record_capacity, int32_t* n_records,
+                     neon_newton_timing* timing)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!options || !u_out || n_bc < 0 || (n_bc > 0 && !bcs) || n_loads < 0 || (n_loads > 0 && !loads))
+            throw status_error(NEON_BAD_ARGUMENT, "bad arguments");
+        if (options->precision != NEON_PRECISION_FP64 && options->precision != NEON_PRECISION_MIXED)
+            throw status_error(NEON_BAD_ARGUMENT, "unknown precision");
+        if (!std::isfinite(search_direction)) throw status_error(NEON_BAD_ARGUMENT, "search direction not finite");
+        if (ctx->mesh->tangent_mode() != NEON_TANGENT_REFERENCE)
+            throw status_error(NEON_BAD_ARGUMENT, "the LATIN iterations use neon's tangent (NEON_TANGENT_REFERENCE)");
+        return newton_solve(ctx, options, n_bc, bcs, n_loads, loads, u_out, records, record_capacity, n_records,
+                            timing, true, search_direction);
+    });
+}
+
 int neon_adjoint_gradient(neon_context* ctx, const double* dJdu, int32_t n_fixed, const int32_t* fixed,
                           double tolerance, int32_t max_iterations, double* grad, int32_t* iterations)
 {

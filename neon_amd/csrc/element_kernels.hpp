@@ -643,6 +643,106 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
     for (int k = 0; k < 6; ++k) st.coef[(COEF_N + k) * Q + qp] = nv[k];
 }
 
+// LATIN force (latin_submesh.hpp:62-111): one thread per element of a colour,
+//   f_a += int B_a^T (sigma_old + s D (eps - eps_old) - sigma) dV
+// on the current configuration x = X + u, B_a = dN_a J_x^-1; eps - eps_old is the symmetric
+// part of the displacement gradient's change, sum_a (u_a - u_old_a) (x) dN_a J_X^-1, as a
+// kinematic Voigt vector (engineering shears); D from the QP's tangent coefficients (the
+// structured form, or the 21 stored entries when FULL); sigma_old the committed stress.
+// Elements of one colour share no node, so the scatter needs no atomics.
+template <bool FULL>
+__global__ __launch_bounds__(64) void latin_force_kernel(const int32_t* __restrict__ elems, int32_t count,
+                                                         const int32_t* __restrict__ conn,
+                                                         const double* __restrict__ X, const double* __restrict__ u,
+                                                         const double* __restrict__ u_old,
+                                                         const double* __restrict__ sigma,
+                                                         const double* __restrict__ sigma_old,
+                                                         const double* __restrict__ coef, int64_t Q, double s,
+                                                         double* __restrict__ f)
+{
+    for (int32_t t = blockIdx.x * 64 + threadIdx.x; t < count; t += gridDim.x * 64)
+    {
+        int64_t const e = elems[t];
+        int32_t nd[8];
+        double Xe[8][3], xe[8][3], du[8][3];
+        for (int a = 0; a < 8; ++a)
+        {
+            nd[a] = conn[8 * e + a];
+            for (int i = 0; i < 3; ++i)
+            {
+                int64_t const g = 3ll * nd[a] + i;
+                Xe[a][i] = X[g];
+                xe[a][i] = X[g] + u[g];
+                du[a][i] = u[g] - u_old[g];
+            }
+        }
+        double fe[8][3];
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i) fe[a][i] = 0.0;
+        for (int l = 0; l < 8; ++l)
+        {
+            double Jx[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, J0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int a = 0; a < 8; ++a)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                    {
+                        Jx[i * 3 + j] += xe[a][i] * c_hex8.dN[l][a][j];
+                        J0[i * 3 + j] += Xe[a][i] * c_hex8.dN[l][a][j];
+                    }
+            double Jxi[9], J0i[9];
+            inv3(Jx, Jxi);
+            inv3(J0, J0i);
+            double const detJ = det3(Jx);
+            double Hd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int a = 0; a < 8; ++a)
+            {
+                double B0[3];
+                for (int j = 0; j < 3; ++j)
+                    B0[j] = c_hex8.dN[l][a][0] * J0i[j] + c_hex8.dN[l][a][1] * J0i[3 + j] + c_hex8.dN[l][a][2] * J0i[6 + j];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) Hd[i * 3 + j] += du[a][i] * B0[j];
+            }
+            double const ek[6] = {Hd[0], Hd[4], Hd[8], Hd[5] + Hd[7], Hd[2] + Hd[6], Hd[1] + Hd[3]};
+            int64_t const qp = 8 * e + l;
+            double De[6];
+            if constexpr (FULL)
+            {
+                for (int i = 0; i < 6; ++i)
+                {
+                    double v = 0.0;
+                    for (int j = 0; j < 6; ++j) v += coef[int64_t(upper_index(i, j)) * Q + qp] * ek[j];
+                    De[i] = v;
+                }
+            }
+            else
+            {
+                double const A = coef[COEF_A * Q + qp], Bt = coef[COEF_B * Q + qp], C = coef[COEF_C * Q + qp];
+                double n[6], ne = 0.0;
+                for (int k = 0; k < 6; ++k)
+                {
+                    n[k] = coef[(COEF_N + k) * Q + qp];
+                    ne += n[k] * ek[k];
+                }
+                double const tr = ek[0] + ek[1] + ek[2];
+                for (int k = 0; k < 6; ++k) De[k] = (k < 3 ? A * tr + 2.0 * Bt * ek[k] : Bt * ek[k]) + C * ne * n[k];
+            }
+            double d[6];
+            for (int k = 0; k < 6; ++k) d[k] = sigma_old[k * Q + qp] + s * De[k] - sigma[k * Q + qp];
+            double const S[9] = {d[0], d[5], d[4], d[5], d[1], d[3], d[4], d[3], d[2]};
+            double const wd = detJ * c_hex8.w[l];
+            for (int a = 0; a < 8; ++a)
+            {
+                double B[3];
+                for (int j = 0; j < 3; ++j)
+                    B[j] = c_hex8.dN[l][a][0] * Jxi[j] + c_hex8.dN[l][a][1] * Jxi[3 + j] + c_hex8.dN[l][a][2] * Jxi[6 + j];
+                for (int i = 0; i < 3; ++i) fe[a][i] += (B[0] * S[i] + B[1] * S[3 + i] + B[2] * S[6 + i]) * wd;
+            }
+        }
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i) f[3ll * nd[a] + i] += fe[a][i];
+    }
+}
+
 // commit (old <- current) or revert (current <- old) of the J2 history
 __global__ __launch_bounds__(kBlock) void history_copy_kernel(int64_t n, const double* __restrict__ src,
                                                               double* __restrict__ dst)

@@ -1015,6 +1015,43 @@ void device_mesh::set_displacement_host(const double* u)
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
 }
 
+void device_mesh::latin_reset()
+{
+    int64_t const Q = 8ll * E_;
+    sigma_old_.resize(6 * Q);
+    NEON_CHECK_HIP(hipMemsetAsync(sigma_old_.data(), 0, 6 * Q * sizeof(double), s_));
+}
+
+void device_mesh::latin_commit()
+{
+    int64_t const Q = 8ll * E_;
+    history_copy_kernel<<<grid_for(6 * Q), kBlock, 0, s_>>>(6 * Q, sigma_.data(), sigma_old_.data());
+    NEON_CHECK_HIP(hipGetLastError());
+}
+
+void device_mesh::latin_force(const double* u_dev, const double* u_old_dev, double s, double* f_dev)
+{
+    int64_t const Q = 8ll * E_;
+    if (sigma_old_.size() != size_t(6 * Q)) throw status_error(NEON_BAD_ARGUMENT, "latin_reset was not called");
+    NEON_CHECK_HIP(hipMemsetAsync(f_dev, 0, 3ull * N_ * sizeof(double), s_));
+    bool const full = is_microsphere(model_);
+    for (int c = 0; c < colours(); ++c)
+    {
+        int32_t const off = colour_offsets_[c];
+        int32_t const cnt = colour_offsets_[c + 1] - off;
+        if (cnt == 0) continue;
+        int const blocks = int(std::min<int64_t>((cnt + 63) / 64, 4096));
+        const int32_t* el = elems_.data() + off;
+        if (full)
+            latin_force_kernel<true><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, u_old_dev,
+                                                            sigma_.data(), sigma_old_.data(), coef_.data(), Q, s, f_dev);
+        else
+            latin_force_kernel<false><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, u_old_dev,
+                                                             sigma_.data(), sigma_old_.data(), coef_.data(), Q, s, f_dev);
+        NEON_CHECK_HIP(hipGetLastError());
+    }
+}
+
 void device_mesh::set_ageing(double scission, double recombination)
 {
     if ((model_ & 0xff) != NEON_MODEL_MICROSPHERE_AGEING)

@@ -233,6 +233,12 @@ public:
     // the path-dependent models' rate equations: the step size at a load step's first update,
     // 0 at its Newton iterates
     int update_internal_variables(const double* u_dev, double dt = 0.0);
+    // LATIN (latin_matrix.hpp): committed stress <- 0 (latin_reset) or <- current stress
+    // (latin_commit); f <- the LATIN force at the current state with the committed
+    // displacement u_old (device) and search direction s
+    void latin_reset();
+    void latin_commit();
+    void latin_force(const double* u_dev, const double* u_old_dev, double s, double* f_dev);
     // ageing microsphere: the material's scission and recombination probabilities
     void set_ageing(double scission, double recombination);
     // ageing microsphere: network state, 11 per QP in QP order (host)
@@ -319,6 +325,7 @@ private:
     mutable bool s64_fresh_ = false, s32_fresh_ = false;
     device_array<double> sigma_, coef_, detF_, eps_p_, ebar_, vm_, eps_p_old_, ebar_old_;
     device_array<double> age_, age_old_; // ageing microsphere: 11 x Q network state (SoA)
+    device_array<double> sigma_old_;     // LATIN: committed stress (6 x Q)
     double scission_ = 0.0, recombination_ = 0.0, last_dt_ = 0.0;
     device_array<uint32_t> status_;
     std::vector<std::unique_ptr<device_array<double>>> traj_u_, traj_ep_, traj_eb_;

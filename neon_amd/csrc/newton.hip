@@ -158,9 +158,16 @@ solve_result device_solve(neon_context* ctx, int solver, int criterion, int prec
     return k.cg_mixed_nb(A, b_dev, x_dev, minv_dev, tol, max_it);
 }
 
+__global__ __launch_bounds__(kBlock) void zero_entries_kernel(int32_t count, const int32_t* __restrict__ idx,
+                                                              double* __restrict__ v)
+{
+    int32_t const i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) v[idx[i]] = 0.0;
+}
+
 int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, const neon_dirichlet* bcs,
                  int32_t n_loads, const neon_neumann* loads, double* u_out, neon_newton_record* records, int32_t cap,
-                 int32_t* n_records, neon_newton_timing* timing)
+                 int32_t* n_records, neon_newton_timing* timing, bool latin, double latin_s)
 {
     device_mesh& m = *ctx->mesh;
     krylov_engine& k = *ctx->engine;
@@ -205,6 +212,13 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
     NEON_CHECK_HIP(hipMemsetAsync(m.f_int(), 0, n * sizeof(double), s));
     if (o->record_trajectory) m.clear_trajectory();
     NEON_CHECK_HIP(hipMemsetAsync(delta.data(), 0, n * sizeof(double), s));
+    device_array<double> lres;
+    if (latin)
+    {
+        // the committed stress starts at its initial zero (internal_variables), as u_old does
+        lres.resize(n);
+        m.latin_reset();
+    }
     double* u = m.u();
     int const max_it = o->max_newton_iterations > 0 ? o->max_newton_iterations : 10;
     int const precision = o->precision;
@@ -304,12 +318,26 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
                     norm_initial_residual = std::sqrt(k.dot(n, mres.data(), mres.data()));
                 }
                 m.enforce_dirichlet(int32_t(fixed_h.size()), fixed_dev.data(), mres.data(), precision);
+                // latin_matrix::perform_equilibrium_iterations (latin_matrix.hpp:209-226): after
+                // the first iteration the right-hand side is the LATIN force at the current
+                // state, its fixed entries zeroed with the residual's; the norms still use
+                // f_ext - f_int
+                const double* rhs = mres.data();
+                if (latin && it > 0)
+                {
+                    m.latin_force(u, u_old.data(), latin_s, lres.data());
+                    if (!fixed_h.empty())
+                        zero_entries_kernel<<<blocks(int64_t(fixed_h.size())), kBlock, 0, s>>>(
+                            int32_t(fixed_h.size()), fixed_dev.data(), lres.data());
+                    NEON_CHECK_HIP(hipGetLastError());
+                    rhs = lres.data();
+                }
                 NEON_CHECK_HIP(hipStreamSynchronize(s));
                 tm.dirichlet_ms += ms_since(t0);
 
                 t0 = std::chrono::steady_clock::now();
                 solve_result const sr = device_solve(ctx, o->solver, o->solver_criterion, precision, o->solver_tolerance,
-                                                     o->solver_max_iterations, mres.data(), delta.data(), minv.data());
+                                                     o->solver_max_iterations, rhs, delta.data(), minv.data());
                 tm.solve_ms += ms_since(t0);
                 tm.spmv_count += sr.spmv;
                 tm.refinement_count += sr.refinements;
@@ -368,6 +396,7 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
             if (o->record_trajectory) m.record_step();
             ts.update(true);
             m.save_internal_variables(true);
+            if (latin) m.latin_commit();
             ++steps_done;
             if (o->on_step)
             {

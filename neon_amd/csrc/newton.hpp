@@ -48,9 +48,12 @@ double load_value(const neon_neumann& load, double t);
 solve_result device_solve(neon_context* ctx, int solver, int criterion, int precision, double tol, int32_t max_it,
                           const double* b_dev, double* x_dev, double* minv_dev);
 
+// static_matrix::solve; with latin set, latin_matrix::solve (latin_matrix.hpp:150-244) with
+// the search direction latin_s
 int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, const neon_dirichlet* bcs,
                  int32_t n_loads, const neon_neumann* loads, double* u_out, neon_newton_record* records,
-                 int32_t cap, int32_t* n_records, neon_newton_timing* timing);
+                 int32_t cap, int32_t* n_records, neon_newton_timing* timing, bool latin = false,
+                 double latin_s = 0.0);
 
 // Adjoint of the recorded J2 load history (device buffers): grad (4 per element) of an
 // objective of the final displacement with gradient dJdu; returns a status, PCG iterations

@@ -63,7 +63,7 @@ class Problem:
     groups: dict = field(default_factory=dict)  # name -> list of (gmsh type, connectivity array)
     visualisation: dict | None = None  # the step mesh's "visualisation" block (output fields)
     output_name: str = ""              # file name of the output (the step mesh's "name")
-    solution: str = "equilibrium"      # the step's "solution": "equilibrium" or "natural_frequency"
+    solution: str = "equilibrium"      # the step's "solution": "equilibrium", "latin" or "natural_frequency"
     eigen_solver: dict | None = None   # the step's "eigen_solver" block (natural frequency)
     density: np.ndarray | None = None  # (E,) the material's "density", when given
     ageing: tuple | None = None        # ageing microsphere: (scission, recombination) probabilities
@@ -262,8 +262,17 @@ def read(path: str, step: int = 0) -> Problem:
     for key in ("name", "time", "solution"):
         if key not in sim:
             raise ValueError(f'A simulation case needs a "{key}" field')
-    if sim["solution"] not in ("equilibrium", "natural_frequency"):
+    if sim["solution"] not in ("equilibrium", "natural_frequency", "latin"):
         raise ValueError('"solution" is not valid.  Please use "equilibrium" or "natural_frequency"')
+    if sim["solution"] == "latin":
+        # module_factory.cpp:64-85 and latin_matrix.hpp:95-104
+        if "nonlinear_options" not in sim:
+            raise ValueError('"nonlinear_options" needs to be present for a solid_mechanics simulation')
+        nl = sim["nonlinear_options"]
+        if nl.get("latin_scheme") != "incremental" or nl.get("latin_local_stage") != "implicit":
+            raise ValueError("Only incremental and implicit LATIN formulation has been implemented so far")
+        if "latin_search_direction" not in nl:
+            raise ValueError("latin_search_direction not specified in nonlinear_options")
     if sim["solution"] == "natural_frequency" and "eigen_solver" not in sim:
         raise ValueError('A natural frequency step needs an "eigen_solver" block')
     if len(sim["meshes"]) != 1:

@@ -196,7 +196,8 @@ def solve_with_output(problem, ctx, root: str = ".", precision: str = "fp64", ta
                                context=ctx, ageing=problem.ageing)
     sim = dict(problem.simulation)
     sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": precision})
-    sm = neon_amd.static_matrix(mesh, sim, problem.boundaries, loads=problem.loads)
+    driver = neon_amd.latin_matrix if problem.solution == "latin" else neon_amd.static_matrix
+    sm = driver(mesh, sim, problem.boundaries, loads=problem.loads)
     writer = None
     if problem.visualisation is not None:
         writer = vtk_file_output(problem.output_name, problem.visualisation, problem.X, problem.conn, problem.model,

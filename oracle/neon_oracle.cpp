@@ -1314,6 +1314,54 @@ int orc_assemble(const orc_model* m, const int32_t* row_ptr, const int32_t* col,
     return ORC_OK;
 }
 
+// latin_submesh::incremental_latin_internal_force (latin_submesh.hpp:62-111) per element,
+// summed as latin_matrix::compute_incremental_latin_internal_force (latin_matrix.hpp:110-126):
+// at each QP sigma_global = sigma_old + s D (eps - eps_old) with D applied to the kinematic
+// Voigt strain and read back as a kinetic Voigt stress, then int B^T (sigma_global - sigma)
+// on the current configuration.
+int orc_latin_force(const orc_model* m, double search_direction, double* f)
+{
+    if (!m || !f) return ORC_BAD_ARGUMENT;
+    std::fill(f, f + 3 * int64_t(m->N), 0.0);
+    static const int pair[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+    for (int64_t e = 0; e < m->E; ++e)
+    {
+        double diff[72], fe[24];
+        for (int l = 0; l < 8; ++l)
+        {
+            size_t const q = 8 * size_t(e) + l;
+            const double* H = &m->Hg[9 * q];
+            const double* Ho = &m->Hg_o[9 * q];
+            double eps[6]; // kinematic Voigt: engineering shears
+            for (int k = 0; k < 6; ++k)
+            {
+                int const i = pair[k][0], j = pair[k][1];
+                double const cur = 0.5 * (H[i * 3 + j] + H[j * 3 + i]);
+                double const old = 0.5 * (Ho[i * 3 + j] + Ho[j * 3 + i]);
+                eps[k] = (k < 3 ? 1.0 : 2.0) * (cur - old);
+            }
+            double sg[6];
+            for (int r = 0; r < 6; ++r)
+            {
+                double v = 0.0;
+                for (int c = 0; c < 6; ++c) v += m->D[36 * q + r * 6 + c] * eps[c];
+                sg[r] = v;
+            }
+            double* d = diff + 9 * l;
+            for (int k = 0; k < 6; ++k)
+            {
+                int const i = pair[k][0], j = pair[k][1];
+                double const v = m->sigma_o[9 * q + i * 3 + j] + search_direction * sg[k] - m->sigma[9 * q + i * 3 + j];
+                d[i * 3 + j] = d[j * 3 + i] = v;
+            }
+        }
+        element_matrices(*m, e, nullptr, fe, diff);
+        for (int a = 0; a < 8; ++a)
+            for (int i = 0; i < 3; ++i) f[3 * int64_t(m->conn[8 * e + a]) + i] += fe[a * 3 + i];
+    }
+    return ORC_OK;
+}
+
 // enforce_dirichlet_conditions (static_matrix.hpp:279-320)
 int orc_enforce_dirichlet(int32_t n, const int32_t* row_ptr, const int32_t* col, double* val, double* b,
                           int32_t n_fixed, const int32_t* fixed)
@@ -1871,7 +1919,7 @@ int orc_newton_solve_loaded(orc_model* m, const orc_newton_options* opt, int32_t
     vector<int32_t> rp(n + 1), ci(nnz);
     orc_sparsity_fill(m, rp.data(), ci.data());
     vector<double> K(nnz);
-    vector<double> f_int(n, 0.0), f_ext(n, 0.0), u(n, 0.0), u_old(n, 0.0), delta(n, 0.0), mres(n, 0.0);
+    vector<double> f_int(n, 0.0), f_ext(n, 0.0), u(n, 0.0), u_old(n, 0.0), delta(n, 0.0), mres(n, 0.0), lres;
     if (opt->record_trajectory)
     {
         m->traj_u.clear();
@@ -1966,6 +2014,17 @@ int orc_newton_solve_loaded(orc_model* m, const orc_newton_options* opt, int32_t
                     if (bc_active(bcs[k], step_time)) fixed.insert(fixed.end(), bcs[k].dofs, bcs[k].dofs + bcs[k].n_dofs);
                 orc_enforce_dirichlet(n, rp.data(), ci.data(), K.data(), mres.data(), int32_t(fixed.size()),
                                       fixed.data());
+                // latin_matrix::perform_equilibrium_iterations (latin_matrix.hpp:209-226): after
+                // the first iteration the right-hand side is the LATIN force, its fixed entries
+                // zeroed with the residual's
+                const double* rhs = mres.data();
+                if (opt->latin && it > 0)
+                {
+                    lres.resize(n);
+                    orc_latin_force(m, opt->latin_search_direction, lres.data());
+                    for (int32_t d : fixed) lres[d] = 0.0;
+                    rhs = lres.data();
+                }
 
                 int32_t its = 0;
                 double e = 0;
@@ -1982,12 +2041,12 @@ int orc_newton_solve_loaded(orc_model* m, const orc_newton_options* opt, int32_t
                     permute(n, rp.data(), ci.data(), K.data(), perm, inv, prp, pci, pv);
                     pb.resize(n);
                     px.resize(n);
-                    for (int32_t i = 0; i < n; ++i) pb[i] = mres[perm[i]];
+                    for (int32_t i = 0; i < n; ++i) pb[i] = rhs[perm[i]];
                 }
                 const int32_t* srp = opt->rcm ? prp.data() : rp.data();
                 const int32_t* sci = opt->rcm ? pci.data() : ci.data();
                 const double* sv = opt->rcm ? pv.data() : K.data();
-                const double* sb = opt->rcm ? pb.data() : mres.data();
+                const double* sb = opt->rcm ? pb.data() : rhs;
                 double* sx = opt->rcm ? px.data() : delta.data();
                 if (opt->solver == 0)
                     s = opt->solver_criterion == ORC_STOP_RELATIVE

@@ -203,6 +203,8 @@ typedef struct {
     int32_t threads;
     int32_t rcm;                   /* apply RCM + P^T K P per solve like neon's CPU CG */
     int32_t record_trajectory;     /* keep u and the committed J2 history of every converged step */
+    int32_t latin;                 /* 1: latin_matrix's iterations (latin_matrix.hpp:172-244) */
+    double latin_search_direction; /* "latin_search_direction" */
 } orc_newton_options;
 
 typedef struct {
@@ -214,6 +216,12 @@ typedef struct {
     double displacement_norm;
     double force_norm;
 } orc_newton_record;
+
+/* latin_submesh::incremental_latin_internal_force assembled (latin_matrix.hpp:110-126) at the
+ * model's state: f = sum_e int B^T (sigma_old + s D (eps - eps_old) - sigma) dV on the current
+ * configuration, eps and eps_old the symmetric parts of the current and committed displacement
+ * gradients, sigma_old the committed stress.  f has 3N entries. */
+int orc_latin_force(const orc_model* m, double search_direction, double* f);
 
 int orc_newton_solve(orc_model* m, const orc_newton_options* opt, int32_t n_bc, const orc_dirichlet* bcs,
                      double* u /* in: ignored (zero start); out: final displacement */,

@@ -47,7 +47,8 @@ class NewtonOptions(C.Structure):
                 ("max_newton_iterations", C.c_int32), ("absolute_norm", C.c_int32), ("norm_mode", C.c_int32),
                 ("solver_tolerance", C.c_double), ("solver_max_iterations", C.c_int32),
                 ("solver_criterion", C.c_int32), ("solver", C.c_int32), ("threads", C.c_int32),
-                ("rcm", C.c_int32), ("record_trajectory", C.c_int32)]
+                ("rcm", C.c_int32), ("record_trajectory", C.c_int32), ("latin", C.c_int32),
+                ("latin_search_direction", C.c_double)]
 
 
 class NewtonRecord(C.Structure):
@@ -78,6 +79,7 @@ _SIG = {
     "orc_element_mass": [C.c_void_p, C.c_int32, C.c_double, f64p],
     "orc_assemble_mass": [C.c_void_p, i32p, i32p, f64p, f64p],
     "orc_enforce_dirichlet": [C.c_int32, i32p, i32p, f64p, f64p, C.c_int32, i32p],
+    "orc_latin_force": [C.c_void_p, C.c_double, f64p],
     "orc_spmv": [C.c_int32, i32p, i32p, f64p, f64p, f64p, C.c_int32],
     "orc_rcm": [C.c_int32, i32p, i32p, i32p],
     "orc_time_create": [P(TimeOptions), C.c_int32, f64p, P(C.c_void_p)],
@@ -257,6 +259,13 @@ class Model:
             raise RuntimeError("orc_get_ageing_state failed")
         return out
 
+    def latin_force(self, search_direction):
+        """orc_latin_force at the model's state (3N)."""
+        f = np.zeros(3 * self.n_nodes)
+        if lib().orc_latin_force(self.h, float(search_direction), _f(f)) != OK:
+            raise RuntimeError("orc_latin_force failed")
+        return f
+
     def element_mass(self, e, density):
         me = np.zeros((24, 24))
         lib().orc_element_mass(self.h, e, density, _f(me))
@@ -429,7 +438,9 @@ class TimeStep:
         return dict(step_time=st.value, last_step_time=lt.value, fully_applied=bool(fa.value), successful=su.value)
 
 
-def newton_options(sim: dict, criterion=STOP_RELATIVE, threads=0, rcm=0, solver=0, record_trajectory=False):
+def newton_options(sim: dict, criterion=STOP_RELATIVE, threads=0, rcm=0, solver=0, record_trajectory=False,
+                   latin=None):
+    """latin: the search direction of latin_matrix's iterations, or None for static_matrix's."""
     t = sim["time"]
     inc = t["increments"]
     nl = sim["nonlinear_options"]
@@ -438,4 +449,5 @@ def newton_options(sim: dict, criterion=STOP_RELATIVE, threads=0, rcm=0, solver=
         TimeOptions(t["period"], inc["initial"], inc["minimum"], inc["maximum"], int(bool(inc["adaptive"]))),
         nl["residual_tolerance"], nl["displacement_tolerance"], nl.get("linear_iterations", 10),
         int("absolute_tolerance" in nl), sim.get("norm_mode", 0), ls.get("tolerance", 1e-5),
-        ls.get("maximum_iterations", 2000), criterion, solver, threads, rcm, int(record_trajectory))
+        ls.get("maximum_iterations", 2000), criterion, solver, threads, rcm, int(record_trajectory),
+        int(latin is not None), 0.0 if latin is None else float(latin))
