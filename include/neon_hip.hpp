@@ -350,6 +350,8 @@ struct simulation_options
     bool record_trajectory = false;
     solver_options linear;
     bool symmetric = true;
+    /// "latin_search_direction" of a LATIN step (latin_matrix.hpp:95-104)
+    std::optional<double> latin_search_direction;
 };
 
 /// Newton-Raphson with adaptive load stepping on the device (fem::static_matrix::solve).
@@ -410,9 +412,14 @@ public:
         std::vector<double> u(mesh_.dofs());
         records_.assign(record_capacity, neon_newton_record{});
         std::int32_t n = 0;
-        int const code = neon_newton_solve_loaded(mesh_.ctx().get(), &o, std::int32_t(bcs.size()), bcs.data(),
-                                                  std::int32_t(lds.size()), lds.data(), u.data(), records_.data(),
-                                                  record_capacity, &n, &timing_);
+        int const code =
+            options_.latin_search_direction
+                ? neon_latin_solve(mesh_.ctx().get(), &o, *options_.latin_search_direction, std::int32_t(bcs.size()),
+                                   bcs.data(), std::int32_t(lds.size()), lds.data(), u.data(), records_.data(),
+                                   record_capacity, &n, &timing_)
+                : neon_newton_solve_loaded(mesh_.ctx().get(), &o, std::int32_t(bcs.size()), bcs.data(),
+                                           std::int32_t(lds.size()), lds.data(), u.data(), records_.data(),
+                                           record_capacity, &n, &timing_);
         records_.resize(std::min(n, record_capacity));
         check(mesh_.ctx().get(), code == NEON_OK || code < 0 ? code : NEON_FATAL);
         return u;
@@ -435,6 +442,26 @@ private:
     std::vector<neon_newton_record> records_;
     neon_newton_timing timing_{};
 };
+/// LATIN iterations on the device (latin_matrix.hpp:150-244): static_matrix's load stepping
+/// with the LATIN force as the right-hand side after each step's first iteration.
+class latin_matrix : public static_matrix
+{
+public:
+    latin_matrix(device_mesh& m, simulation_options const& o, std::vector<dirichlet_boundary> b,
+                 std::vector<nonfollower_load> loads = {})
+        : static_matrix(m, checked(o), std::move(b), std::move(loads))
+    {
+    }
+
+private:
+    static simulation_options const& checked(simulation_options const& o)
+    {
+        if (!o.latin_search_direction)
+            throw std::domain_error("latin_search_direction not specified in nonlinear_options");
+        return o;
+    }
+};
+
 /// The "eigen_solver" block of a natural-frequency step (eigen_solver_factory.cpp:12-77).
 struct eigen_solver_options
 {

@@ -161,6 +161,29 @@ void patch_test(neon_hip::context& ctx)
                 err / umax, sm.records().size());
     expect(err <= 1e-9 * umax, "uniaxial neo-Hooke patch test matches the closed form");
     expect(sm.records().size() >= 2, "two load steps recorded");
+
+    // LATIN with the zero search direction on a fresh mesh: the displacement-driven stretch
+    // reaches the same state; without a direction latin_matrix refuses, as neon's constructor
+    bool refused = false;
+    try
+    {
+        neon_hip::latin_matrix bad(mesh, o, bcs);
+    }
+    catch (std::domain_error const& e)
+    {
+        refused = std::string(e.what()).find("latin_search_direction") != std::string::npos;
+    }
+    expect(refused, "latin_matrix needs latin_search_direction");
+    neon_hip::device_mesh mesh2(ctx, X, conn, params, NEON_MODEL_NEOHOOKE);
+    neon_hip::simulation_options ol = o;
+    ol.latin_search_direction = 0.0;
+    neon_hip::latin_matrix lm(mesh2, ol, bcs);
+    std::vector<double> const ul = lm.solve();
+    double dl = 0.0;
+    for (std::size_t i = 0; i < u.size(); ++i) dl = std::max(dl, std::abs(ul[i] - u[i]));
+    std::printf("      LATIN (s = 0): max |u_latin - u_newton| / max|u| = %.3e, %zu records\n", dl / umax,
+                lm.records().size());
+    expect(dl <= 1e-9 * umax, "LATIN with s = 0 reaches the Newton solution of the patch test");
 }
 // unit-load vectors of the box's top face (area 1) and volume (1): host-only calls
 void load_vectors()

@@ -144,5 +144,5 @@ def test_ingest_natural_frequency_step(tmp_path):
     with pytest.raises(ValueError, match="eigen_solver"):
         ingest.read(_frequency_json(tmp_path / "c", mat))
     with pytest.raises(ValueError, match='"solution" is not valid'):
-        ingest.read(_frequency_json(tmp_path / "d", mat, {"type": "arpack"}, solution="latin"))
+        ingest.read(_frequency_json(tmp_path / "d", mat, {"type": "arpack"}, solution="explicit_dynamics"))
     assert os.path.exists(tmp_path / "a" / "cube.mesh")
