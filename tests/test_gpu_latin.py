@@ -21,10 +21,10 @@ import test_ingest as ti
 pytestmark = pytest.mark.gpu
 
 
-def _run(p, ctx, s, sim=None):
+def _run(p, ctx, s, sim=None, precision="fp64"):
     sim = copy.deepcopy(sim or p.simulation)
     sim["nonlinear_options"]["latin_search_direction"] = s
-    sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": "fp64"})
+    sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": precision})
     mesh = neon_amd.DeviceMesh(p.X, p.conn, p.params, model=p.model, tangent=neon_amd.TANGENT_REFERENCE, context=ctx)
     lm = neon_amd.latin_matrix(mesh, sim, p.boundaries, loads=p.loads)
     try:
@@ -67,3 +67,15 @@ def test_latin_stalling_direction_cut_backs(ctx):
     _same(recs, recs0)
     # the last iterate of the failed attempt (neither driver resets u on the fatal exit)
     np.testing.assert_allclose(mesh.displacement(), u0, rtol=0, atol=1e-8 * max(np.abs(u0).max(), 1e-300))
+
+
+def test_latin_mixed_precision(ctx, tmp_path):
+    """The LATIN right-hand side through the mixed-precision PCG: the converged displacement of
+    the FP64 oracle to 1e-9 of max |u| and the same load steps."""
+    p = tgm._simulation(tmp_path, orc.MODEL_MICROSPHERE_GAUSSIAN, neon_files.rollers(
+        [{"name": "z1", "type": "displacement", "time": [0.0, 1.0], "z": [0.0, 0.2]}]), n=4)
+    u, err, recs, mesh, code, u0, recs0 = _run(p, ctx, 0.0, precision="mixed")
+    assert err is None and code == orc.OK
+    assert [round(r["step_time"], 12) for r in recs if r["iteration"] == 0] == \
+        [round(r["step_time"], 12) for r in recs0 if r["iteration"] == 0]
+    np.testing.assert_allclose(u, u0, rtol=0, atol=1e-9 * np.abs(u0).max())
