@@ -25,6 +25,18 @@
  *   CUDA CG / BiCGSTAB        src/solver/linear/conjugate_gradient_cuda.cpp:72-192,
  *                             src/solver/linear/biconjugate_gradient_stabilised_cuda.cpp:31-200
  *   RCM reordering            src/graph/cuthill_mckee.hpp:45-97
+ *   linear elasticity         src/constitutive/mechanics/solid/isotropic_linear_elasticity.cpp:27-67
+ *   microsphere models        src/constitutive/mechanics/solid/{gaussian_,}affine_microsphere.cpp,
+ *                             nonaffine_microsphere.cpp:26-178, unit_sphere_quadrature.cpp:10-90
+ *   ageing microsphere        gaussian_ageing_affine_microsphere.cpp:17-134,
+ *                             src/material/micromechanical_elastomer.cpp:22-122
+ *   consistent mass           src/mesh/mechanics/solid/submesh.cpp:162-183
+ *   LATIN force and driver    src/mesh/mechanics/solid/latin_submesh.hpp:62-111,
+ *                             src/assembler/mechanics/latin_matrix.hpp:150-244
+ *
+ * What pins each restatement (closed forms, the reference's own checks and vectors, invariants)
+ * is stated in the test module that checks it (tests/test_oracle.py, test_microsphere.py,
+ * test_natural_frequency.py, test_latin.py, test_adjoint_oracle.py, test_ingest.py).
  *
  * Every entry point returns an int status (ORC_* codes below).
  */
