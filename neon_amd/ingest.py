@@ -221,7 +221,37 @@ def _read_mesh(path: str):
     return X, groups
 
 
-def _history(boundary: dict, key: str):
+def _sinusoidal(boundary: dict, key: str, time_step: float):
+    """boundary::generate_sinusoidal (src/mesh/boundary/boundary.cpp:87-143): from time 0
+    (value 0), block i appends floor(cycles_i period_i / time_step) equal steps spanning
+    cycles_i period_i, each valued amplitude_i sin(2 pi / period_i t + phase_i) at its absolute
+    time t."""
+    for field in ("period", "phase", "number_of_cycles"):
+        if field not in boundary:
+            raise ValueError(f'"{field}" was not specified in "boundaries".')
+    amplitude, period = list(boundary[key]), list(boundary["period"])
+    cycles, phase = list(boundary["number_of_cycles"]), list(boundary["phase"])
+    if not (len(amplitude) == len(period) == len(phase) == len(cycles)):
+        raise ValueError("The amplitude, period, number_of_cycles, time_step and phase vectors must be of the same "
+                         "size.")
+    times, loads = [0.0], [0.0]
+    for a, T, c, ph in zip(amplitude, period, cycles, phase):
+        omega = 2.0 * np.pi / T
+        steps = int(c * T / time_step)  # the vector length truncates the quotient
+        if steps < 1:
+            raise ValueError(f'boundary "{boundary["name"]}": a sinusoidal block shorter than the time step')
+        scale, start = c * T / steps, times[-1]
+        block = [k * scale + start for k in range(1, steps + 1)]
+        times += block
+        loads += [a * np.sin(omega * t + ph) for t in block]
+    return np.array(times), np.array(loads)
+
+
+def _history(boundary: dict, key: str, time_step: float = 0.0):
+    if "generate_type" in boundary:
+        if boundary["generate_type"] != "sinusoidal":
+            raise ValueError(f'"generate_type" {boundary["generate_type"]} is not recognised; use "sinusoidal"')
+        return _sinusoidal(boundary, key, time_step)
     t = np.asarray(boundary["time"], dtype=np.float64)
     v = np.asarray(boundary[key], dtype=np.float64)
     if len(t) != len(v) or len(t) == 0:
@@ -296,12 +326,14 @@ def read(path: str, step: int = 0) -> Problem:
     params = np.tile(_material_params(materials[part["material"]], model), (len(conn), 1))
 
     boundaries, loads = [], []
+    # generated histories step with the first increment (solid_mechanics_module.cpp:15)
+    dt0 = float(sim["time"]["increments"]["initial"]) if "increments" in sim.get("time", {}) else 0.0
     for b in mesh_data.get("boundaries", []):
         for key in ("name", "type"):
             if key not in b:
                 raise ValueError(f'"{key}" was not specified in "boundary".')
-        if "time" not in b:
-            raise ValueError('generated ("generate_type") boundary histories are not supported; give "time"')
+        if "time" not in b and "generate_type" not in b:
+            raise ValueError('Neither "time" nor "generate_type" was specified in "boundary".')
         name, btype = b["name"], b["type"]
         if name not in groups:
             raise ValueError(f"Mesh name {name} does not exist in the mesh store")
@@ -309,20 +341,20 @@ def read(path: str, step: int = 0) -> Problem:
             nodes = np.unique(np.concatenate([c.reshape(-1) for _, c in groups[name]]))
             for key, off in DOF_TABLE.items():
                 if key in b:
-                    t, v = _history(b, key)
+                    t, v = _history(b, key, dt0)
                     boundaries.append({"name": name, "dofs": (3 * nodes + off).astype(np.int32), "times": t,
                                        "values": v})
         elif btype == "traction":
             faces = _faces(groups, name)
             for key, off in DOF_TABLE.items():
                 if key in b:
-                    t, v = _history(b, key)
+                    t, v = _history(b, key, dt0)
                     d, vals = _sparse(surface_load_vector(X, faces, off, reduced))
                     loads.append({"name": name, "dofs": d, "values": vals, "times": t, "loads": v})
         elif btype == "pressure":
             if "value" not in b:
                 raise ValueError('Pressure boundary condition must specify a "value" array')
-            t, v = _history(b, "value")
+            t, v = _history(b, "value", dt0)
             d, vals = _sparse(surface_load_vector(X, _faces(groups, name), -1, reduced))
             loads.append({"name": name, "dofs": d, "values": vals, "times": t, "loads": v})
         elif btype == "body_force":
@@ -332,14 +364,14 @@ def read(path: str, step: int = 0) -> Problem:
             econn = np.concatenate([c for _, c in groups[name]])
             for key, off in DOF_TABLE.items():
                 if key in b:
-                    t, v = _history(b, key)
+                    t, v = _history(b, key, dt0)
                     d, vals = _sparse(volume_load_vector(X, econn, off, reduced))
                     loads.append({"name": name, "dofs": d, "values": vals, "times": t, "loads": v})
         elif btype == "nodal_force":
             nodes = np.unique(np.concatenate([c.reshape(-1) for _, c in groups[name]]))
             for key, off in DOF_TABLE.items():
                 if key in b:
-                    t, v = _history(b, key)
+                    t, v = _history(b, key, dt0)
                     loads.append({"name": name, "dofs": (3 * nodes + off).astype(np.int32),
                                   "values": np.ones(len(nodes)), "times": t, "loads": v, "fixed_time": 1.0})
         else:

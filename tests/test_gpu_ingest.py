@@ -103,3 +103,14 @@ def test_load_history_must_cover_the_steps(ctx, tmp_path):
     p = ingest.read(neon_files.write_cube_simulation(tmp_path, 2, ti.NEOHOOKE, {"name": "neohooke"}, bcs))
     with pytest.raises(neon_amd.NeonError, match="load's time history"):
         gpu_solve(p, ctx)
+
+
+def test_generated_sinusoidal_history(ctx, tmp_path):
+    """A generated (sinusoidal) boundary history drives the device Newton solve through the
+    same steps and iterations as the oracle to the same displacement."""
+    p = ti.sinusoid_problem(tmp_path, n=3)
+    u, recs = gpu_solve(p, ctx)
+    code, u0, recs0, _ = ti.oracle_solve(p)
+    assert code == orc.OK
+    _same_newton(recs, recs0)
+    np.testing.assert_allclose(u, u0, rtol=0, atol=1e-10 * np.abs(u0).max())

@@ -304,3 +304,56 @@ def test_oracle_body_and_nodal_force_equilibrium(tmp_path):
     z0 = np.asarray(p.boundaries[2]["dofs"])
     assert (f[z0] - fext[z0]).sum() == pytest.approx(-fext[2::3].sum(), rel=1e-9)
     assert fext[2::3].sum() == pytest.approx(-0.3 + 16 * 0.01, rel=1e-12)
+
+
+def _sinusoid_bc(**kw):
+    b = {"name": "z1", "type": "displacement", "generate_type": "sinusoidal"}
+    b.update(kw)
+    return b
+
+
+def test_generated_sinusoidal_history(tmp_path):
+    """boundary::generate_sinusoidal (boundary.cpp:87-143) with the first increment as its step:
+    two blocks, the second shifted to start where the first ends, each value
+    a_i sin(2 pi t / T_i + phase_i) at its absolute time; neon's messages for missing fields."""
+    time = {"period": 2.0, "increments": {"initial": 0.25, "minimum": 0.01, "maximum": 0.25, "adaptive": False}}
+    bc = _sinusoid_bc(z=[0.01, 0.02], period=[1.0, 0.5], number_of_cycles=[1.0, 2.0], phase=[0.0, np.pi / 2])
+    path = neon_files.write_cube_simulation(tmp_path / "a", 2, NEOHOOKE, {"name": "neohooke"},
+                                            neon_files.rollers([bc]), time=time)
+    p = ingest.read(path)
+    (b,) = [b for b in p.boundaries if b["name"] == "z1"]
+    t = np.array([0.0, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 1.75, 2.0])
+    np.testing.assert_allclose(b["times"], t, rtol=0, atol=1e-15)
+    want = np.concatenate([[0.0], 0.01 * np.sin(2 * np.pi * t[1:5]), 0.02 * np.sin(4 * np.pi * t[5:] + np.pi / 2)])
+    np.testing.assert_allclose(b["values"], want, rtol=0, atol=1e-17)
+    for k, (bad, msg) in enumerate([({"phase": None}, '"phase" was not specified'),
+                                    ({"period": [1.0]}, "must be of the same size"),
+                                    ({"generate_type": "square"}, "not recognised")]):
+        bb = dict(bc, **bad)
+        bb = {kk: v for kk, v in bb.items() if v is not None}
+        path = neon_files.write_cube_simulation(tmp_path / f"b{k}", 2, NEOHOOKE, {"name": "neohooke"},
+                                                neon_files.rollers([bb]), time=time)
+        with pytest.raises(ValueError, match=msg):
+            ingest.read(path)
+    bb = {k: v for k, v in bc.items() if k != "generate_type"}
+    path = neon_files.write_cube_simulation(tmp_path / "c", 2, NEOHOOKE, {"name": "neohooke"},
+                                            neon_files.rollers([bb]), time=time)
+    with pytest.raises(ValueError, match='Neither "time" nor "generate_type"'):
+        ingest.read(path)
+
+
+def sinusoid_problem(tmp_path, n=2):
+    """Three quarters of a compression-first cycle of u_z on z1 (generated history, step 0.25)."""
+    time = {"period": 0.75, "increments": {"initial": 0.25, "minimum": 0.01, "maximum": 0.25, "adaptive": False}}
+    bcs = [{"name": f"{a}0", "type": "displacement", "time": [0.0, 0.75], a: [0.0, 0.0]} for a in "xyz"]
+    bcs.append(_sinusoid_bc(z=[-0.05], period=[1.0], number_of_cycles=[0.75], phase=[0.0]))
+    return ingest.read(neon_files.write_cube_simulation(tmp_path, n, NEOHOOKE, {"name": "neohooke"}, bcs, time=time))
+
+
+def test_generated_history_drives_the_newton_solve(tmp_path):
+    p = sinusoid_problem(tmp_path)
+    code, u, recs, _ = oracle_solve(p)
+    assert code == orc.OK
+    assert sorted({round(r["step_time"], 12) for r in recs}) == [0.25, 0.5, 0.75]
+    # u_z = -0.05 sin(3 pi / 2) X_z = 0.05 X_z at the end, homogeneous
+    np.testing.assert_allclose(u.reshape(-1, 3)[:, 2], 0.05 * p.X[:, 2], rtol=0, atol=1e-12)
