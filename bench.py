@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import subprocess
 import os
 import sys
 import time
@@ -78,7 +79,8 @@ def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
     asm_count, asm_ms = ctx.profile_read(_capi.KERNEL_ASSEMBLY)
     ctx.profile(False)
     timed = dict(seconds=t1 - t0, records=sm.records, timing=sm.timing, spmv_count=spmv_count, spmv_ms=spmv_ms,
-                 asm_count=asm_count, asm_ms=asm_ms)
+                 asm_count=asm_count, asm_ms=asm_ms,
+                 load_steps=sorted({r["step"] + 1 for r in sm.records}))
     log(f"[{precision}] {steps} Newton steps in {t1 - t0:.3f} s; CG its "
         f"{[r['solver_iterations'] for r in sm.records]}; spmv {spmv_count} x {spmv_ms / max(spmv_count, 1):.4f} ms")
     # first load step to the 1e-8 relative residual
@@ -86,10 +88,13 @@ def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
     sm = neon_amd.static_matrix(mesh, dict(sim, max_load_steps=1), p.boundaries)
     sm.solve()
     t1 = time.perf_counter()
+    u1 = mesh.displacement()
     timed["to_tolerance"] = dict(seconds=t1 - t0, newton_iterations=len(sm.records),
                                  final_force_norm=sm.records[-1]["force_norm"],
                                  final_displacement_norm=sm.records[-1]["displacement_norm"],
-                                 cg_iterations=[r["solver_iterations"] for r in sm.records])
+                                 cg_iterations=[r["solver_iterations"] for r in sm.records],
+                                 reliable_updates=sm.timing.get("refinement_count", 0))
+    timed["u_step1"] = u1
     log(f"[{precision}] load step 1 to 1e-8: {len(sm.records)} Newton its, {t1 - t0:.3f} s, "
         f"force norms {['%.2e' % r['force_norm'] for r in sm.records]}")
     return timed
@@ -103,7 +108,7 @@ def cpu_baseline(problems, n_gpu, cg_its_per_step, nnz_gpu, n_elem_gpu):
     GPU FP64 run's mean CG iterations per Newton step."""
     from oracle import oracle as orc
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads = cpu_threads()
     ns = 64
     p = problems.block_rollers(ns)
     m = orc.Model(p.X, p.conn, p.params, model=p.model)
@@ -139,19 +144,38 @@ def cpu_baseline(problems, n_gpu, cg_its_per_step, nnz_gpu, n_elem_gpu):
                    f"{n_elem_gpu} elements / {nnz_gpu} nnz and {cg_its_per_step:.0f} CG iterations per Newton step"),
         "cg_iteration_ms_full_mesh": 1e3 * per_it * scale_nnz,
         "spmv_gbs": spmv_b / per_it / 1e9,
-        "cpu_model": _cpu_model(),
+        "host": host_info(),
     }
 
 
-def _cpu_model():
+def _span(steps):
+    return f"{steps[0]}-{steps[-1]}" if len(steps) > 1 else str(steps[0]) if steps else "none"
+
+
+def cpu_threads():
+    """Every core of this process's affinity set, bounded by the job's CPU share when one is
+    set (OMP_NUM_THREADS: the host grants a one-GPU job that many cores)."""
+    cores = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        cores = min(cores, int(share))
+    return cores
+
+
+def host_info():
+    """nproc, the affinity set, the job's CPU share and the lscpu model name of this host."""
+    model = "unknown"
     try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "threads_used": cpu_threads(),
+            "lscpu_model": model}
 
 
 def main():
@@ -208,8 +232,10 @@ def main():
         "dtype": "f32 inner PCG + f64 refinement/residuals" if head == "mixed" else "f64",
         "data": "synthetic (seeded jittered structured hex8 mesh, per-element mu ~ U[0.5,1.5] MPa)",
         "config": {"workload": f"neo-Hooke block {args.n}^3 hex8, rollers x=0/y=0/z=0, top u_z = 20% in 10 "
-                               f"steps; Newton steps of load step 1; Jacobi-PCG 1e-10; Newton target 1e-8",
-                   "elements": p.n_elem, "dofs": p.n_dofs, "nnz": nnz},
+                               f"steps; the timed Newton iterations are the first {args.steps} of the run from u = 0 "
+                               f"(after {args.warmup} untimed), covering load steps {_span(r['load_steps'])}; "
+                               f"Jacobi-PCG 1e-10; Newton target 1e-8",
+                   "elements": p.n_elem, "dofs": p.n_dofs, "nnz": nnz, "load_steps_timed": r["load_steps"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.n) if head == "mixed" else None,
@@ -228,7 +254,6 @@ def main():
             "spmv_gbs": bs / (avg * 1e-3) / 1e9,
             "spmv_avg_ms": avg,
             "spmv_hbm_frac": bs / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "spmv_csr_equivalent_gbs": (12 if vbp == 8 else 8) * nnz / (avg * 1e-3) / 1e9,
             "cg_iterations_per_newton_step": its,
             "assembly_avg_ms": asm_avg,
             "assembly_gflops": ASSEMBLY_FLOP_PER_ELEMENT * p.n_elem / (asm_avg * 1e-3) / 1e9 if asm_avg > 0 else None,
@@ -237,11 +262,19 @@ def main():
             "phase_ms": {k: rr["timing"][k] for k in ("assemble_ms", "dirichlet_ms", "solve_ms", "update_ms",
                                                       "total_ms")},
             "to_1e-8": rr["to_tolerance"],
+            "reliable_updates": rr["timing"].get("refinement_count", 0),
         }
     if "mixed" in results and "fp64" in results:
         line["mixed_vs_fp64_speedup"] = {
             "newton_steps_per_s": line["mixed"]["newton_steps_per_s"] / line["fp64"]["newton_steps_per_s"],
             "time_to_1e-8": results["fp64"]["to_tolerance"]["seconds"] / results["mixed"]["to_tolerance"]["seconds"],
+        }
+        # the mixed path's error against the FP64 path after load step 1 (both to the 1e-8 residual)
+        u64, u32 = results["fp64"]["u_step1"], results["mixed"]["u_step1"]
+        line["mixed_vs_fp64_displacement_error"] = {
+            "max_abs_rel": float(np.abs(u32 - u64).max() / np.abs(u64).max()),
+            "l2_rel": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+            "load_step": 1,
         }
     if not args.no_cpu_baseline:
         its64 = [x["solver_iterations"] for x in results.get("fp64", r)["records"]]

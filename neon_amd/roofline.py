@@ -3,7 +3,8 @@
     python -m neon_amd.roofline --n 128 --precision mixed --reps 20
 
 builds the n^3 block (the bench's mesh), assembles the tangent at u = 0 in the requested
-precision, runs `reps` SpMVs with the solver's default shape (`neon_mesh_spmv_bench`) and a
+precision, runs `reps` SpMVs in the solvers' form (`neon_mesh_spmv_bench`: the sliced copy,
+nontemporal value loads, x . y block partials; the CG loop's kernel instance) and a
 streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the calibration kernel for the
 FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the mean SpMV time, its
 algorithmic bytes (values, block columns (2 bytes each as 16-bit offsets), x read once, y
@@ -38,6 +39,7 @@ def main(argv=None) -> dict:
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mb", type=int, default=1024)
+    ap.add_argument("--dirty", action="store_true")
     args = ap.parse_args(argv)
 
     ctx = neon_amd.Context(0)
@@ -49,8 +51,13 @@ def main(argv=None) -> dict:
     mesh.assemble(prec)
     ms = C.c_double()
     lib = _capi.lib()
-    # the solvers' default shape (spmv_config defaults): the sliced copy, workgroups by size
-    cfg = dict(lanes=32, flags=_capi.SPMV_SLICED, blocks=0)
+    # the solvers' form exactly: the sliced copy with nontemporal value loads and the x . y block
+    # partials (the same kernel instance as inside the CG loop), workgroups by size; --dirty puts
+    # a write of 4 vectors of n before every product, as the CG vector updates do (timed apart)
+    flags = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL | _capi.SPMV_PROBE_DOT
+    if args.dirty:
+        flags |= _capi.SPMV_PROBE_DIRTY
+    cfg = dict(lanes=32, flags=flags, blocks=0)
     ctx.check(lib.neon_mesh_spmv_bench(ctx.handle, prec, cfg["lanes"], cfg["flags"], cfg["blocks"], args.reps,
                                        C.byref(ms)))
     nnz = mesh.nnz()
