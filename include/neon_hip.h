@@ -14,6 +14,9 @@
  *                              (src/mesh/mechanics/solid/mesh.cpp:25-46, submesh.cpp:26-52) with
  *                              the constitutive model from make_constitutive_model
  *                              (src/constitutive/constitutive_model_factory.cpp:26-164)
+ *   neon_model_traits,         constitutive_model::is_finite_deformation / is_symmetric / variable_names
+ *   neon_model_variable_names  (src/constitutive/constitutive_model.hpp:36-48); neon_mesh_model the
+ *   neon_mesh_model            model behind mesh::is_symmetric (src/mesh/mechanics/solid/mesh.cpp:47-53)
  *   neon_mesh_pattern          compute_sparsity_pattern (src/assembler/sparsity_pattern.hpp:18-48)
  *   neon_mesh_set_displacement material_coordinates::update_current_configuration
  *                              (src/mesh/material_coordinates.cpp:23-27)
@@ -137,6 +140,18 @@ int neon_krylov_solve_csr(neon_context* ctx, int solver, int criterion, int prec
  * nonaffine_microsphere.cpp:14-24).  One mesh per context. */
 int neon_mesh_create(neon_context* ctx, int32_t n_nodes, const double* X, int32_t n_elem, const int32_t* conn,
                      int32_t model, const double* params, int32_t tangent_mode);
+/* ---- constitutive model traits (constitutive_model::is_finite_deformation, is_symmetric and
+ * variable_names, src/constitutive/constitutive_model.hpp:36-48; host only, no device use) ----
+ * finite_deformation: the tangent carries the geometric term and is integrated on the current
+ * configuration (submesh.cpp:75-78); symmetric: the tangent is symmetric, so neon solves with
+ * conjugate gradient, otherwise with BiCGSTAB (mesh.cpp:47-53, linear_solver_factory.cpp:150-161).
+ * NEON_BAD_ARGUMENT for an unknown model. */
+int neon_model_traits(int32_t model, int32_t* finite_deformation, int32_t* symmetric);
+/* The internal variable names the model adds to its `names` set, sorted and ';'-separated (as
+ * neon's std::set iterates them); "" when it adds none; NULL for an unknown model. */
+const char* neon_model_variable_names(int32_t model);
+/* The model the mesh was created with (NEON_MODEL_* with its flags). */
+int neon_mesh_model(neon_context* ctx, int32_t* model);
 /* Replace the per-element parameters (4 per element) without rebuilding the pattern. */
 int neon_mesh_set_parameters(neon_context* ctx, const double* params);
 /* Sparsity pattern: nnz always; row_ptr (3N+1) and col (nnz) when non-NULL. */

@@ -11,6 +11,8 @@
  *   computational_error, hip_error          computational_error, cuda_error (src/exceptions.hpp:14-23)
  *   device_mesh                             mechanics::solid::mesh + its constitutive models
  *                                           (src/mesh/mechanics/solid/mesh.hpp:37-80)
+ *   constitutive_model, material_property   src/constitutive/constitutive_model.hpp:19-56,
+ *                                           src/material/material_property.hpp:13-37
  *   static_matrix                           fem::static_matrix (src/assembler/mechanics/static_matrix.hpp:145-467)
  *
  * Recoverable failures (Krylov or Newton limits, non-finite values, det F < 0, radial return)
@@ -26,6 +28,7 @@
 #include <functional>
 #include <memory>
 #include <optional>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -225,6 +228,63 @@ inline std::unique_ptr<linear_solver> make_linear_solver(solver_options const& o
     return s;
 }
 
+/// The parameters of a device mesh's material (material_property and its isotropic subclasses,
+/// src/material/material_property.hpp:13-37, isotropic_elastic_property.hpp): the name and, per
+/// element (the reference holds one value per part), the model's four parameters in the order of
+/// neon_mesh_create.
+class material_property
+{
+public:
+    material_property(std::string name, std::vector<double> params, std::optional<double> density = std::nullopt)
+        : name_(std::move(name)), params_(std::move(params)), density_(density)
+    {
+    }
+    std::string const& name() const { return name_; }
+    double initial_density() const
+    {
+        if (!density_) throw std::domain_error("Density was requested, but not specified in the input file");
+        return *density_;
+    }
+    /// parameter k (0..3) of element e
+    double parameter(std::int32_t e, int k) const { return params_[4 * std::size_t(e) + k]; }
+    double bulk_modulus(std::int32_t e) const { return parameter(e, 0); }
+    double shear_modulus(std::int32_t e) const { return parameter(e, 1); }
+    double lambda(std::int32_t e) const { return parameter(e, 0) - 2.0 * parameter(e, 1) / 3.0; }
+    std::vector<double> const& parameters() const { return params_; }
+
+private:
+    std::string name_;
+    std::vector<double> params_;
+    std::optional<double> density_;
+};
+
+class device_mesh;
+
+/// constitutive_model (src/constitutive/constitutive_model.hpp:19-56) of a device mesh: its
+/// update launches the device kernels on every quadrature point of the mesh, whose internal
+/// variables stay on the device.
+class constitutive_model
+{
+public:
+    constitutive_model(device_mesh& m, int model, material_property material);
+
+    /// Update the internal variables and tangents at the quadrature points for the displacement
+    /// the mesh holds (time_step_size: the rate models' step; neon passes the load increment at a
+    /// step's first update and 0 otherwise)
+    void update_internal_variables(double const time_step_size);
+
+    material_property const& intrinsic_material() const { return material_; }
+    bool is_finite_deformation() const { return finite_; }
+    bool is_symmetric() const { return symmetric_; }
+    std::set<std::string> const& variable_names() const noexcept { return names_; }
+
+private:
+    device_mesh& mesh_;
+    material_property material_;
+    bool finite_ = false, symmetric_ = true;
+    std::set<std::string> names_;
+};
+
 /// Device-resident hex8 mesh with its constitutive model, sparsity pattern and tangent.
 class device_mesh
 {
@@ -232,14 +292,23 @@ public:
     /// X: 3 per node; conn: 8 per element (hexahedron.cpp:21-28 order); params: 4 per element
     /// (bulk, shear, yield stress, hardening modulus); model: NEON_MODEL_*; tangent: NEON_TANGENT_*
     device_mesh(context& c, std::vector<double> const& X, std::vector<std::int32_t> const& conn,
-                std::vector<double> const& params, int model, int tangent = NEON_TANGENT_REFERENCE)
+                std::vector<double> const& params, int model, int tangent = NEON_TANGENT_REFERENCE,
+                std::string material_name = "")
         : ctx_(c), nodes_(std::int32_t(X.size() / 3)), elements_(std::int32_t(conn.size() / 8))
     {
         if (X.size() % 3 || conn.size() % 8 || params.size() != 4 * conn.size() / 8)
             throw hip_error(NEON_BAD_ARGUMENT, "mesh array sizes disagree");
         check(c.get(), neon_mesh_create(c.get(), nodes_, X.data(), elements_, conn.data(), model, params.data(),
                                         tangent));
+        model_ = std::make_unique<constitutive_model>(*this, model, material_property(std::move(material_name), params));
     }
+
+    /// the mesh's constitutive model (submesh::constitutive, submesh.hpp:62)
+    constitutive_model& constitutive() { return *model_; }
+    constitutive_model const& constitutive() const { return *model_; }
+    /// mesh::is_symmetric (mesh.cpp:47-53): every submesh's model is symmetric (one model per
+    /// device mesh here)
+    bool is_symmetric() const { return model_->is_symmetric(); }
 
     std::int32_t nodes() const { return nodes_; }
     std::int32_t elements() const { return elements_; }
@@ -249,6 +318,11 @@ public:
     void set_parameters(std::vector<double> const& params)
     {
         check(ctx_.get(), neon_mesh_set_parameters(ctx_.get(), params.data()));
+    }
+    /// x = X + u (material_coordinates::update_current_configuration, material_coordinates.cpp:23-27)
+    void set_displacement(std::vector<double> const& u)
+    {
+        check(ctx_.get(), neon_mesh_set_displacement(ctx_.get(), u.data()));
     }
     /// x = X + u and the constitutive update at every quadrature point (mesh.cpp:57-71)
     void update_internal_variables(std::vector<double> const& u, double const time_step = 0.0)
@@ -289,7 +363,29 @@ public:
 private:
     context& ctx_;
     std::int32_t nodes_, elements_;
+    std::unique_ptr<constitutive_model> model_;
 };
+
+inline constitutive_model::constitutive_model(device_mesh& m, int model, material_property material)
+    : mesh_(m), material_(std::move(material))
+{
+    std::int32_t fd = 0, sym = 1;
+    if (neon_model_traits(model, &fd, &sym) != NEON_OK) throw hip_error(NEON_BAD_ARGUMENT, "unknown constitutive model");
+    finite_ = fd != 0;
+    symmetric_ = sym != 0;
+    std::string const all = neon_model_variable_names(model);
+    for (std::size_t b = 0; b < all.size();)
+    {
+        std::size_t const e = std::min(all.find(';', b), all.size());
+        names_.insert(all.substr(b, e - b));
+        b = e + 1;
+    }
+}
+
+inline void constitutive_model::update_internal_variables(double const time_step_size)
+{
+    check(mesh_.ctx().get(), neon_mesh_update_internal_variables(mesh_.ctx().get(), time_step_size));
+}
 
 /// A prescribed displacement history on a set of dofs (boundary.cpp:50-85).
 struct dirichlet_boundary
@@ -349,7 +445,6 @@ struct simulation_options
     /// extension: keep the load history on the device for the J2 adjoint
     bool record_trajectory = false;
     solver_options linear;
-    bool symmetric = true;
     /// "latin_search_direction" of a LATIN step (latin_matrix.hpp:95-104)
     std::optional<double> latin_search_direction;
 };
@@ -385,7 +480,9 @@ public:
         o.solver_tolerance = l.tolerance.value_or(1.0e-5);
         o.solver_max_iterations = l.maximum_iterations.value_or(2000);
         o.solver_criterion = NEON_STOP_ABSOLUTE_PRECONDITIONED;
-        o.solver = options_.symmetric ? NEON_SOLVER_CG : NEON_SOLVER_BICGSTAB;
+        // make_linear_solver(linear_solver, mesh.is_symmetric()) (static_matrix.hpp:109-112,
+        // linear_solver_factory.cpp:150-161)
+        o.solver = mesh_.is_symmetric() ? NEON_SOLVER_CG : NEON_SOLVER_BICGSTAB;
         o.precision = l.precision == "mixed" ? NEON_PRECISION_MIXED : NEON_PRECISION_FP64;
         if (o.precision == NEON_PRECISION_MIXED) o.solver_criterion = NEON_STOP_RELATIVE;
         o.record_trajectory = options_.record_trajectory ? 1 : 0;

@@ -54,6 +54,9 @@ __all__ = [
     "conjugate_gradient_hip",
     "biconjugate_gradient_stabilised_hip",
     "make_linear_solver",
+    "constitutive_model",
+    "material_property",
+    "model_traits",
     "static_matrix",
     "latin_matrix",
     "natural_frequency_matrix",
@@ -214,6 +217,82 @@ def make_linear_solver(solver_data: dict, is_symmetric: bool, context: Context |
     return cls(precision=precision, context=context, **kwargs)
 
 
+def model_traits(model: int) -> tuple[bool, bool]:
+    """(is_finite_deformation, is_symmetric) of a constitutive model (``neon_model_traits``)."""
+    fd, sym = C.c_int32(0), C.c_int32(0)
+    code = _capi.lib().neon_model_traits(int(model), C.byref(fd), C.byref(sym))
+    if code != 0:
+        raise NeonError(code, f"unknown constitutive model {model}")
+    return bool(fd.value), bool(sym.value)
+
+
+class material_property:
+    """The parameters of a device mesh's material (``material_property`` and its isotropic
+    subclasses, src/material/material_property.hpp:13-37, isotropic_elastic_property.hpp):
+    ``name`` and, per element (the reference holds one value per part), the model's four
+    parameters in the order of ``neon_mesh_create``."""
+
+    def __init__(self, name: str, params: np.ndarray, density: float | None = None):
+        self._name = name
+        self.params = np.asarray(params).reshape(-1, 4)
+        self._density = density
+
+    def name(self) -> str:
+        return self._name
+
+    def initial_density(self) -> float:
+        if self._density is None:
+            raise ValueError("Density was requested, but not specified in the input file")
+        return self._density
+
+    def bulk_modulus(self) -> np.ndarray:
+        return self.params[:, 0]
+
+    def shear_modulus(self) -> np.ndarray:
+        return self.params[:, 1]
+
+    def lambda_(self) -> np.ndarray:
+        return self.params[:, 0] - 2.0 * self.params[:, 1] / 3.0
+
+    def elastic_modulus(self) -> np.ndarray:
+        K, G = self.params[:, 0], self.params[:, 1]
+        return 9.0 * K * G / (3.0 * K + G)
+
+    def poissons_ratio(self) -> np.ndarray:
+        K, G = self.params[:, 0], self.params[:, 1]
+        return (3.0 * K - 2.0 * G) / (2.0 * (3.0 * K + G))
+
+
+class constitutive_model:
+    """``constitutive_model`` (src/constitutive/constitutive_model.hpp:19-56) of a
+    :class:`DeviceMesh`: its update launches the device kernels on the mesh's quadrature points,
+    whose state lives on the device (``DeviceMesh.qp_state`` reads it)."""
+
+    def __init__(self, mesh: "DeviceMesh", material_name: str = "", density: float | None = None):
+        self._mesh = mesh
+        self._material = material_property(material_name, mesh.params, density)
+        self._finite, self._symmetric = model_traits(mesh.model)
+        names = _capi.lib().neon_model_variable_names(int(mesh.model)).decode()
+        self._names = set(names.split(";")) if names else set()
+
+    def update_internal_variables(self, time_step_size: float) -> None:
+        """Update the internal variables and tangents at every quadrature point for the
+        displacement the mesh holds (``neon_mesh_update_internal_variables``)."""
+        self._mesh.update_internal_variables(time_step_size)
+
+    def intrinsic_material(self) -> material_property:
+        return self._material
+
+    def is_finite_deformation(self) -> bool:
+        return self._finite
+
+    def is_symmetric(self) -> bool:
+        return self._symmetric
+
+    def variable_names(self) -> set:
+        return self._names
+
+
 class DeviceMesh:
     """Device-resident hex8 mesh, constitutive state and tangent (one per context)."""
 
@@ -232,6 +311,21 @@ class DeviceMesh:
             _capi.f64(self.params), tangent))
         if ageing is not None:
             self.set_ageing(*ageing)
+        self._constitutive = None
+
+    def constitutive(self, material_name: str = "", density: float | None = None) -> constitutive_model:
+        """The mesh's constitutive model (``submesh::constitutive``, submesh.hpp:62)."""
+        if self._constitutive is None or material_name or density is not None:
+            self._constitutive = constitutive_model(self, material_name, density)
+        return self._constitutive
+
+    def is_symmetric(self) -> bool:
+        """Whether the tangent is symmetric: every submesh's model is (mesh.cpp:47-53; one model
+        per device mesh here)."""
+        return model_traits(self.model)[1]
+
+    def is_finite_deformation(self) -> bool:
+        return model_traits(self.model)[0]
 
     def set_ageing(self, scission: float, recombination: float) -> None:
         """The ageing microsphere's scission and recombination probabilities
@@ -442,7 +536,10 @@ class static_matrix:
         o.solver_tolerance = ls.get("tolerance", 1.0e-5)
         o.solver_max_iterations = ls.get("maximum_iterations", 2000)
         o.solver_criterion = simulation.get("criterion", STOP_RELATIVE)
-        o.solver = SOLVER_CG if ls.get("symmetric", True) else SOLVER_BICGSTAB
+        # make_linear_solver(linear_solver, mesh.is_symmetric()) (static_matrix.hpp:109-112,
+        # linear_solver_factory.cpp:150-161): conjugate gradient for a symmetric tangent,
+        # BiCGSTAB otherwise
+        o.solver = SOLVER_CG if mesh.is_symmetric() else SOLVER_BICGSTAB
         o.precision = {"fp64": PRECISION_FP64, "mixed": PRECISION_MIXED}[backend.get("precision", "fp64")]
         o.max_load_steps = simulation.get("max_load_steps", 0)
         o.max_total_iterations = simulation.get("max_total_iterations", 0)

@@ -167,6 +167,9 @@ SIGNATURES = {
          C.c_double, C.c_int32, i32p, f64p],
     ),
     "neon_mesh_create": (C.c_int, [C.c_void_p, C.c_int32, f64p, C.c_int32, i32p, C.c_int32, f64p, C.c_int32]),
+    "neon_model_traits": (C.c_int, [C.c_int32, i32p, i32p]),
+    "neon_model_variable_names": (C.c_char_p, [C.c_int32]),
+    "neon_mesh_model": (C.c_int, [C.c_void_p, i32p]),
     "neon_mesh_set_parameters": (C.c_int, [C.c_void_p, f64p]),
     "neon_mesh_pattern": (C.c_int, [C.c_void_p, P(C.c_int64), i32p, i32p]),
     "neon_mesh_set_displacement": (C.c_int, [C.c_void_p, f64p]),

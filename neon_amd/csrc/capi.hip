@@ -202,6 +202,59 @@ int neon_mesh_create(neon_context* ctx, int32_t n_nodes, const double* X, int32_
     });
 }
 
+namespace
+{
+// constitutive_model::is_finite_deformation / is_symmetric and the `names` each model adds
+// (src/constitutive/mechanics/solid/*.cpp constructors), by model id
+struct model_traits
+{
+    int32_t finite_deformation, symmetric;
+    const char* names;
+};
+bool traits_of(int32_t model, model_traits& t)
+{
+    switch (model & 0xff)
+    {
+    case NEON_MODEL_NEOHOOKE: t = {1, 1, ""}; return true; // compressible_neohooke.hpp:39
+    case NEON_MODEL_J2:                                       // small_strain_J2_plasticity.hpp:32
+        t = {0, 1, "effective_plastic_strain;linearised_plastic_strain;linearised_strain;von_mises_stress"};
+        return true;
+    case NEON_MODEL_LINEAR_ELASTIC: t = {0, 1, "linearised_strain;von_mises_stress"}; return true;
+    case NEON_MODEL_MICROSPHERE_GAUSSIAN:
+    case NEON_MODEL_MICROSPHERE_LANGEVIN:
+    case NEON_MODEL_MICROSPHERE_NONAFFINE: t = {1, 1, ""}; return true;
+    case NEON_MODEL_MICROSPHERE_AGEING:
+        t = {1, 1, "active_segments;active_shear_modulus;inactive_segments;inactive_shear_modulus;reduction_factor"};
+        return true;
+    default: return false;
+    }
+}
+} // namespace
+
+int neon_model_traits(int32_t model, int32_t* finite_deformation, int32_t* symmetric)
+{
+    model_traits t{};
+    if (!traits_of(model, t)) return NEON_BAD_ARGUMENT;
+    if (finite_deformation) *finite_deformation = t.finite_deformation;
+    if (symmetric) *symmetric = t.symmetric;
+    return NEON_OK;
+}
+
+const char* neon_model_variable_names(int32_t model)
+{
+    model_traits t{};
+    return traits_of(model, t) ? t.names : nullptr;
+}
+
+int neon_mesh_model(neon_context* ctx, int32_t* model)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (model) *model = ctx->mesh->model();
+        return NEON_OK;
+    });
+}
+
 int neon_mesh_set_parameters(neon_context* ctx, const double* params)
 {
     return guarded(ctx, [&] {

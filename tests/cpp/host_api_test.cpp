@@ -12,7 +12,9 @@
 #include "neon_hip.hpp"
 
 #include <cmath>
+#include <algorithm>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 namespace
@@ -379,11 +381,62 @@ void eigen_factory_errors()
     o.type = "lanczos";
     expect(throws(o, "OpenCL"), "lanczos needs neon's OpenCL build");
 }
+// constitutive_model traits (host only): the flags and variable names of the reference's
+// constructors (compressible_neohooke.hpp:39, small_strain_J2_plasticity.cpp:19-23,
+// isotropic_linear_elasticity.cpp:17-20)
+void model_traits()
+{
+    std::int32_t fd = -1, sym = -1;
+    expect(neon_model_traits(NEON_MODEL_NEOHOOKE, &fd, &sym) == NEON_OK && fd == 1 && sym == 1,
+           "neo-Hooke: finite deformation, symmetric");
+    expect(neon_model_traits(NEON_MODEL_J2, &fd, &sym) == NEON_OK && fd == 0 && sym == 1,
+           "small-strain J2: small deformation, symmetric");
+    expect(std::string(neon_model_variable_names(NEON_MODEL_J2))
+               == "effective_plastic_strain;linearised_plastic_strain;linearised_strain;von_mises_stress",
+           "small-strain J2 variable names");
+    expect(std::string(neon_model_variable_names(NEON_MODEL_NEOHOOKE)).empty(), "neo-Hooke adds no variable names");
+    expect(neon_model_traits(99, &fd, &sym) == NEON_BAD_ARGUMENT && neon_model_variable_names(99) == nullptr,
+           "unknown model refused");
+}
+
+// the constitutive_model mirror on a device mesh: traits, material, and its update through the
+// C ABI (a homogeneous strain gives sigma = lambda tr(eps) I + 2 G eps at every point)
+void constitutive_interface(neon_hip::context& ctx)
+{
+    std::int32_t nn = 0, ne = 0;
+    neon_mesh_box_size(2, 2, 2, 1.0, 1.0, 1.0, 0.0, &nn, &ne);
+    std::vector<double> X(3 * std::size_t(nn));
+    std::vector<std::int32_t> conn(8 * std::size_t(ne)), ijk(3 * std::size_t(nn));
+    neon_mesh_box(2, 2, 2, 1.0, 1.0, 1.0, 0.0, 1ull, 0.0, X.data(), conn.data(), ijk.data());
+    double const K = 166.0e3, G = 80.0e3;
+    std::vector<double> params;
+    for (std::int32_t e = 0; e < ne; ++e) params.insert(params.end(), {K, G, 1.0e9, 0.0});
+    neon_hip::device_mesh mesh(ctx, X, conn, params, NEON_MODEL_J2, NEON_TANGENT_REFERENCE, "steel");
+    neon_hip::constitutive_model& cm = mesh.constitutive();
+    expect(!cm.is_finite_deformation() && cm.is_symmetric() && mesh.is_symmetric(), "J2 mesh traits");
+    expect(cm.intrinsic_material().name() == "steel" && cm.intrinsic_material().shear_modulus(3) == G,
+           "intrinsic material");
+    expect(cm.variable_names().count("effective_plastic_strain") == 1 && cm.variable_names().size() == 4,
+           "variable names");
+    double const ez = 1e-4;
+    std::vector<double> u(3 * std::size_t(nn), 0.0);
+    for (std::int32_t a = 0; a < nn; ++a) u[3 * a + 2] = ez * X[3 * a + 2];
+    mesh.set_displacement(u);
+    cm.update_internal_variables(0.0);
+    std::vector<double> sigma(9 * 8 * std::size_t(ne));
+    neon_hip::check(ctx.get(), neon_mesh_qp_state(ctx.get(), sigma.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
+    double const lam = K - 2.0 * G / 3.0, szz = (lam + 2.0 * G) * ez, sxx = lam * ez;
+    double err = 0.0;
+    for (std::size_t q = 0; q < 8 * std::size_t(ne); ++q)
+        err = std::max({err, std::abs(sigma[9 * q + 8] - szz), std::abs(sigma[9 * q] - sxx), std::abs(sigma[9 * q + 1])});
+    expect(err <= 1e-9 * szz, "update through constitutive_model gives C : eps");
+}
 } // namespace
 
 int main()
 {
     factory_errors();
+    model_traits();
     eigen_factory_errors();
     load_vectors();
     int devices = 0;
@@ -405,6 +458,7 @@ int main()
     }
     neon_hip::context ctx(0);
     tridiagonal(ctx);
+    constitutive_interface(ctx);
     patch_test(ctx);
     traction_patch_test(ctx);
     bar_frequencies(ctx);

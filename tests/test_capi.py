@@ -104,3 +104,25 @@ def test_make_linear_solver_selection():
         neon_amd.make_linear_solver({"type": "iterative", "device": "gpu"}, True)
     with pytest.raises(ValueError):
         neon_amd.make_linear_solver({"type": "iterative", "device": "gpu", "backend": {"type": "cuda"}}, True)
+
+
+def test_model_traits_follow_the_reference_constructors():
+    # constitutive_model::is_finite_deformation / is_symmetric / names of each model
+    # (compressible_neohooke.hpp:39, small_strain_J2_plasticity.hpp:32 and .cpp:19-23,
+    # isotropic_linear_elasticity.cpp:17-20, gaussian_ageing_affine_microsphere.cpp:30-34)
+    import neon_amd
+
+    assert neon_amd.model_traits(neon_amd.MODEL_NEOHOOKE) == (True, True)
+    assert neon_amd.model_traits(neon_amd.MODEL_J2) == (False, True)
+    assert neon_amd.model_traits(neon_amd.MODEL_LINEAR_ELASTIC) == (False, True)
+    for m in (neon_amd.MODEL_MICROSPHERE_GAUSSIAN, neon_amd.MODEL_MICROSPHERE_LANGEVIN,
+              neon_amd.MODEL_MICROSPHERE_NONAFFINE, neon_amd.MODEL_MICROSPHERE_AGEING,
+              neon_amd.MODEL_MICROSPHERE_GAUSSIAN | neon_amd.SPHERE_BO33):
+        assert neon_amd.model_traits(m) == (True, True)
+    names = _capi.lib().neon_model_variable_names(neon_amd.MODEL_J2).decode().split(";")
+    assert names == sorted(["linearised_strain", "von_mises_stress", "linearised_plastic_strain",
+                            "effective_plastic_strain"])
+    assert _capi.lib().neon_model_variable_names(neon_amd.MODEL_NEOHOOKE).decode() == ""
+    assert _capi.lib().neon_model_variable_names(99) is None
+    with pytest.raises(neon_amd.NeonError):
+        neon_amd.model_traits(99)

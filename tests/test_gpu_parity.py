@@ -299,12 +299,28 @@ def test_newton_parity_block_relative_norms(ctx):
 
 
 # In neon's own J2 mode the plastic history is updated in place at every Newton iterate
-# (small_strain_J2_plasticity.cpp:58-130 writes the internal variables it reads), so the
-# converged displacement depends on the iterates themselves and hence on roundoff: the CPU
-# oracle's result moves by 1.1e-4 (20-step plate) and 6.4e-4 (cut-back plate) relative to
-# max|u| between 1, 2 and 8 threads.  That mode is held to a few times that spread; the
-# consistent mode (return map from the committed history) is held to 1e-8.
-J2_REFERENCE_MODE_U_TOL = 3e-3
+# (small_strain_J2_plasticity.cpp:35-37, 80-86 write the internal variables they read), and the
+# yield test at the start of each step is decided by roundoff (the return map re-runs at the
+# converged state, f = 0 +- eps), so the converged displacement depends on the iterates and hence
+# on the summation order of the linear solves.  neon itself moves with it: its CPU path run
+# with 1, 2, 4 and 8 OpenMP threads and with and without its RCM reordering (all legitimate
+# runs of the reference algorithm) lands up to ~3e-4 (20-step plate) and ~5e-3 (cut-back plate)
+# apart, relative to max|u|.  The HIP result is held to twice that envelope, measured in the
+# same test; the consistent mode (return map from the committed history) is held to 1e-8.
+J2_ENVELOPE_VARIANTS = [(t, r) for r in (0, 1) for t in (1, 2, 4, 8)]
+
+
+def _oracle_envelope(p, u_ref):
+    """The largest distance (relative to max|u_ref|) of the oracle's runs over thread counts and
+    RCM on/off from u_ref, and their Newton iteration counts."""
+    spread, counts = 0.0, []
+    for threads, rcm in J2_ENVELOPE_VARIANTS:
+        m = orc.Model(p.X, p.conn, p.params, model=p.model)
+        code, u, recs = m.newton(orc.newton_options(p.simulation, threads=threads, rcm=rcm), p.boundaries)
+        assert code == orc.OK
+        spread = max(spread, np.abs(u - u_ref).max() / np.abs(u_ref).max())
+        counts.append(len(recs))
+    return spread, counts
 
 
 @pytest.mark.parametrize("tangent", [neon_amd.TANGENT_REFERENCE, neon_amd.TANGENT_CONSISTENT])
@@ -314,8 +330,16 @@ def test_newton_parity_j2_plate(ctx, tangent):
                                   displacement_tolerance=1e-8, norm_mode=1)
     u, recs, u0, recs0, code, code_gpu = _newton_pair(p, ctx, tangent=tangent)
     assert code == orc.OK and code_gpu == 0
-    u_tol = J2_REFERENCE_MODE_U_TOL if tangent == neon_amd.TANGENT_REFERENCE else 1e-8
-    _check_newton(u, recs, u0, recs0, u_tol=u_tol, newton_slack=1)
+    if tangent == neon_amd.TANGENT_CONSISTENT:
+        _check_newton(u, recs, u0, recs0, u_tol=1e-8, newton_slack=1)
+        return
+    spread, counts = _oracle_envelope(p, u0)
+    err = np.abs(u - u0).max() / np.abs(u0).max()
+    print(f"J2 plate, neon's mode: HIP vs oracle {err:.3e}, oracle envelope {spread:.3e}; Newton iterations "
+          f"HIP {len(recs)}, oracle {min(counts)}-{max(counts)}")
+    assert err <= 2.0 * spread, (err, spread)
+    assert min(counts) - 1 <= len(recs) <= max(counts) + 1, (len(recs), counts)
+    _check_newton(u, recs, u0, recs0, u_tol=2.0 * spread, newton_slack=1)
 
 
 @pytest.mark.parametrize("tangent", [neon_amd.TANGENT_REFERENCE, neon_amd.TANGENT_CONSISTENT])
@@ -342,9 +366,12 @@ def test_newton_cutback_recovery(ctx, tangent):
         assert times == times0
         assert err < 1e-8, err
     else:
-        # failed Newton attempts amplify the iterate dependence of neon's mode: the oracle
-        # moves by 6.4e-4 between thread counts and the HIP path lands 3.2e-3 from it
-        assert err < 1e-2, err
+        # failed Newton attempts amplify the iterate dependence of neon's mode (see above)
+        spread, counts = _oracle_envelope(p, u0)
+        print(f"J2 cut-back plate, neon's mode: HIP vs oracle {err:.3e}, oracle envelope {spread:.3e}; Newton "
+              f"iterations HIP {len(recs)}, oracle {min(counts)}-{max(counts)}")
+        assert err <= 2.0 * spread, (err, spread)
+        assert min(counts) - 2 <= len(recs) <= max(counts) + 2, (len(recs), counts)
 
 
 def test_newton_cutback_exhaustion_is_fatal(ctx):
