@@ -78,6 +78,7 @@ extern "C" {
 #define NEON_MODEL_MICROSPHERE_LANGEVIN 4 /* affine_microsphere (Pade inverse Langevin chains) */
 #define NEON_MODEL_MICROSPHERE_NONAFFINE 5 /* nonaffine_microsphere */
 #define NEON_MODEL_MICROSPHERE_AGEING 6   /* gaussian_ageing_affine_microsphere (BAND network) */
+#define NEON_MODEL_J2_DAMAGE 7            /* small_strain_J2_plasticity_damage (unsymmetric tangent) */
 #define NEON_SPHERE_BO33 0x100            /* 33-direction rule instead of the 21-direction one */
 
 #define NEON_TANGENT_REFERENCE 0  /* neon's tangents (neo-Hooke D without 1/J) */

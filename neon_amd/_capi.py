@@ -31,6 +31,7 @@ MODEL_MICROSPHERE_GAUSSIAN = 3
 MODEL_MICROSPHERE_LANGEVIN = 4
 MODEL_MICROSPHERE_NONAFFINE = 5
 MODEL_MICROSPHERE_AGEING = 6
+MODEL_J2_DAMAGE = 7
 SPHERE_BO33 = 0x100  # microsphere rule flag: 33 directions instead of 21
 TANGENT_REFERENCE = 0
 TANGENT_CONSISTENT = 1

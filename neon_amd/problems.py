@@ -159,6 +159,35 @@ def notched_plate_j2(nx=128, ny=64, nz=64, strain=0.005, steps=20, E_mod=200.0e3
                    simulation(steps=steps, **sim), (nx, ny, nz))
 
 
+DAMAGE_CONSTANTS = (5500.0, 250.0, 1220.0, 2.5, 0.6, 2.0)
+
+
+def damage_cube(n=1, top=0.018, period=0.01, steps=10, E_mod=134.0e3, nu=0.3, yield_stress=85.0,
+                newton_iterations=30, **sim):
+    """The viscoplastic damage cube of neon's examples (examples/solid_mechanics/viscoplastic_damage/
+    cube/cube.json): a unit cube (n^3 elements; the example has one), symmetry rollers on x=0,
+    y=0, z=0, top u_z = `top` over t in [0, period] in `steps` fixed increments, the example's
+    steel (E = 134 GPa in MPa units, nu = 0.3, yield 85 MPa) with the damage constants
+    DAMAGE_CONSTANTS (kinematic hardening modulus 5500, softening multiplier 250, plasticity
+    viscous denominator 1220 and exponent 2.5, damage viscous denominator 0.6 and exponent 2),
+    BiCGSTAB (the model's tangent is unsymmetric)."""
+    X, conn, ijk = box_mesh(n, n, n)
+    E = conn.shape[0]
+    params = np.zeros((E, 4))
+    params[:, 0] = E_mod / (3.0 * (1.0 - 2.0 * nu))
+    params[:, 1] = E_mod / (2.0 * (1.0 + nu))
+    params[:, 2] = yield_stress
+    bcs = [
+        {"name": "Xsym", "dofs": face_dofs(ijk, 0, 0, 0), "times": [0.0, period], "values": [0.0, 0.0]},
+        {"name": "Ysym", "dofs": face_dofs(ijk, 1, 0, 1), "times": [0.0, period], "values": [0.0, 0.0]},
+        {"name": "Zsym", "dofs": face_dofs(ijk, 2, 0, 2), "times": [0.0, period], "values": [0.0, 0.0]},
+        {"name": "ZLoad", "dofs": face_dofs(ijk, 2, n, 2), "times": [0.0, period], "values": [0.0, top]},
+    ]
+    simu = simulation(period=period, steps=steps, newton_iterations=newton_iterations, **sim)
+    simu["damage_constants"] = list(DAMAGE_CONSTANTS)
+    return Problem(f"damage_cube{n}", X, conn, ijk, params, _capi.MODEL_J2_DAMAGE, bcs, simu, (n, n, n))
+
+
 def smoothed_log_mu(n, seed, sigma=0.3, passes=3):
     """log mu_e ~ N(0, sigma^2) on an n^3 element grid, smoothed by `passes` 27-point
     (3x3x3, edge-clamped) averages; returns mu_e in element order."""

@@ -361,6 +361,347 @@ inline void outer3(const double* t, const double* u, double* M)
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) M[i * 3 + j] = t[i] * u[j];
 }
+
+// ---- small_strain_J2_plasticity_damage (src/constitutive/mechanics/solid/
+// small_strain_J2_plasticity_damage.cpp:34-290): viscoplastic J2 with nonlinear kinematic
+// hardening and Lemaitre-Chaboche scalar damage.  Tensors are 3x3 row-major; "kinetic" 6-vectors
+// are (11, 22, 33, 23, 13, 12), "kinematic" ones double the shear entries
+// (tensor_operations.hpp:207-214, 373-376).
+
+// Solves A x = b (n x n row-major, destroyed) by Gaussian elimination with complete pivoting, the
+// algorithm behind Eigen's FullPivLU::solve: at step k the entry of largest magnitude in the
+// trailing block is swapped to (k, k) by a row and a column exchange.  Returns false when a pivot
+// is exactly zero.
+bool fullpiv_solve(int n, double* A, const double* b, double* x)
+{
+    int colp[16];
+    double r[16];
+    for (int i = 0; i < n; ++i)
+    {
+        colp[i] = i;
+        r[i] = b[i];
+    }
+    for (int k = 0; k < n; ++k)
+    {
+        int pr = k, pc = k;
+        double big = -1.0;
+        for (int i = k; i < n; ++i)
+            for (int j = k; j < n; ++j)
+                if (std::abs(A[i * n + j]) > big)
+                {
+                    big = std::abs(A[i * n + j]);
+                    pr = i;
+                    pc = j;
+                }
+        if (big == 0.0) return false;
+        if (pr != k)
+        {
+            for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[pr * n + j]);
+            std::swap(r[k], r[pr]);
+        }
+        if (pc != k)
+        {
+            for (int i = 0; i < n; ++i) std::swap(A[i * n + k], A[i * n + pc]);
+            std::swap(colp[k], colp[pc]);
+        }
+        for (int i = k + 1; i < n; ++i)
+        {
+            double const f = A[i * n + k] / A[k * n + k];
+            if (f == 0.0) continue;
+            for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+            r[i] -= f * r[k];
+        }
+    }
+    double y[16];
+    for (int k = n - 1; k >= 0; --k)
+    {
+        double s = r[k];
+        for (int j = k + 1; j < n; ++j) s -= A[k * n + j] * y[j];
+        y[k] = s / A[k * n + k];
+    }
+    for (int k = 0; k < n; ++k) x[colp[k]] = y[k];
+    return true;
+}
+
+inline void kinetic_from6(const double* v, double* a)
+{
+    a[0] = v[0], a[4] = v[1], a[8] = v[2];
+    a[5] = a[7] = v[3];
+    a[2] = a[6] = v[4];
+    a[1] = a[3] = v[5];
+}
+inline void kinematic6(const double* a, double* v)
+{
+    v[0] = a[0], v[1] = a[4], v[2] = a[8];
+    v[3] = 2.0 * a[5], v[4] = 2.0 * a[2], v[5] = 2.0 * a[1];
+}
+// compute_stress_like_vector_voigt (:280-285): C * kinematic(e)
+inline void stress_like6(const double* C, const double* e, double* s)
+{
+    double v[6];
+    kinematic6(e, v);
+    for (int i = 0; i < 6; ++i)
+    {
+        double t = 0.0;
+        for (int j = 0; j < 6; ++j) t += C[i * 6 + j] * v[j];
+        s[i] = t;
+    }
+}
+inline double ddot9(const double* a, const double* b)
+{
+    double s = 0.0;
+    for (int i = 0; i < 9; ++i) s += a[i] * b[i];
+    return s;
+}
+// mandel_notation (tensor_operations.hpp:509-515): shear blocks by sqrt 2 and 2
+inline void mandel6(const double* A, double* M)
+{
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+        {
+            double f = 1.0;
+            if ((i < 3) != (j < 3)) f = std::sqrt(2.0);
+            if (i >= 3 && j >= 3) f = 2.0;
+            M[i * 6 + j] = A[i * 6 + j] * f;
+        }
+}
+
+// Material constants of isotropic_elastic_plastic_damage (isotropic_elastic_plastic_damage.cpp:
+// 9-45): kinematic hardening modulus C, softening multiplier gamma, plasticity viscous
+// denominator s_p and exponent n_p, damage viscous denominator s_d and exponent n_d.
+struct damage_constants
+{
+    double C, gamma, sp, np, sd, nd;
+};
+
+// The residual (:180-189) and Jacobian (:191-236) of the return map's 16 equations at the
+// unknowns y = (plastic multiplier, damage multiplier, kinetic stress (6), kinetic back stress
+// (6), damage, energy release rate), for the trial elastic strain eps_e_t and the point's back
+// stress and damage at the start of the update.  Jm holds the entries that the reference's
+// loop does not overwrite (its initial identity entries) on entry.
+void damage_system(double G, double lam, double sy, double E, const double* Ce, const damage_constants& dc, double dt,
+                   const double* eps_e_t, const double* beta_t, double dmg_t, const double* y, double* res, double* Jm)
+{
+    double const kp = std::pow(dc.sp, -dc.np), kd = std::pow(dc.sd, -dc.nd);
+    double const s32 = std::sqrt(3.0 / 2.0);
+    static const double Pdev[36] = {2.0 / 3, -1.0 / 3, -1.0 / 3, 0, 0, 0, -1.0 / 3, 2.0 / 3, -1.0 / 3, 0, 0, 0,
+                                    -1.0 / 3, -1.0 / 3, 2.0 / 3, 0, 0, 0, 0, 0, 0, 1, 0, 0,
+                                    0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 1}; // voigt::kinetic::deviatoric
+    double const dlp = y[0], dld = y[1];
+    double sig[9], bet[9];
+    kinetic_from6(y + 2, sig);
+    kinetic_from6(y + 8, bet);
+    double const D = y[14], omD = 1.0 - D, Y = y[15];
+    double rel[9], dev[9];
+    for (int i = 0; i < 9; ++i) rel[i] = sig[i] - bet[i];
+    deviatoric3(rel, dev);
+    double const nd = norm9(dev);
+    double const vm = s32 * nd;
+    double N[9];
+    for (int i = 0; i < 9; ++i) N[i] = s32 * dev[i] / (nd * omD);
+    double ee[9];
+    for (int i = 0; i < 9; ++i) ee[i] = eps_e_t[i] - dlp * N[i];
+    // evaluate_yield_function (:252-260), evaluate_damage_yield_function (:262-265)
+    double const fp_raw = vm / omD + 0.5 * dc.gamma / dc.C * ddot9(bet, bet) - sy;
+    double const fd_raw = Y - sy * sy / (2.0 * E);
+    double const fp = std::max(0.0, fp_raw), fd = std::max(0.0, fd_raw);
+    double dfdb[9];
+    for (int i = 0; i < 9; ++i) dfdb[i] = dc.gamma / dc.C * bet[i] - N[i];
+    double const dfdD = vm / (omD * omD);
+    double dNds[36], od[36];
+    outer66(dev, dev, od);
+    for (int k = 0; k < 36; ++k) dNds[k] = s32 / omD * (Pdev[k] / nd - od[k] / (nd * nd * nd));
+    double dNdD[9];
+    for (int i = 0; i < 9; ++i) dNdD[i] = N[i] / omD;
+    // residual (:180-189)
+    double se[9]; // (1 - D) (lambda tr(ee) I + 2 G ee)
+    double const tr = ee[0] + ee[4] + ee[8];
+    for (int i = 0; i < 9; ++i) se[i] = omD * (2.0 * G * ee[i] + (i % 4 == 0 ? lam * tr : 0.0));
+    double Cee[6], Ceem[9];
+    stress_like6(Ce, ee, Cee);
+    kinetic_from6(Cee, Ceem);
+    res[0] = dlp - kp * std::pow(fp, dc.np) * dt;
+    res[1] = dld - kd * std::pow(fd, dc.nd) * dt;
+    {
+        double t[9], v[6];
+        for (int i = 0; i < 9; ++i) t[i] = sig[i] - se[i];
+        kinetic6(t, v);
+        for (int i = 0; i < 6; ++i) res[2 + i] = v[i];
+        for (int i = 0; i < 9; ++i) t[i] = bet[i] - beta_t[i] - dlp * (dc.C * N[i] - dc.gamma * bet[i]);
+        kinetic6(t, v);
+        for (int i = 0; i < 6; ++i) res[8 + i] = v[i];
+    }
+    res[14] = D - dmg_t - dld;
+    res[15] = Y - 0.5 * ddot9(ee, Ceem);
+    // Jacobian (:191-236)
+    double const cfp = -dc.np * kp * std::pow(fp, dc.np - 1.0) * dt;
+    {
+        double v[6], t[9];
+        for (int i = 0; i < 9; ++i) t[i] = cfp * N[i];
+        kinetic6(t, v);
+        for (int j = 0; j < 6; ++j) Jm[0 * 16 + 2 + j] = v[j];
+        for (int i = 0; i < 9; ++i) t[i] = cfp * dfdb[i];
+        kinetic6(t, v);
+        for (int j = 0; j < 6; ++j) Jm[0 * 16 + 8 + j] = v[j];
+    }
+    Jm[0 * 16 + 14] = cfp * dfdD;
+    Jm[1 * 16 + 15] = -dc.nd * kd * std::pow(fd, dc.nd - 1.0) * dt;
+    double omC[36], MomC[36], Mt[36], prod[36], dl_dNds[36];
+    for (int k = 0; k < 36; ++k) omC[k] = omD * Ce[k];
+    {
+        double v[6];
+        stress_like6(omC, N, v);
+        for (int i = 0; i < 6; ++i) Jm[(2 + i) * 16 + 0] = v[i];
+    }
+    mandel6(omC, MomC);
+    for (int k = 0; k < 36; ++k) dl_dNds[k] = dlp * dNds[k];
+    mandel6(dl_dNds, Mt);
+    matmul6(MomC, Mt, prod);
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) Jm[(2 + i) * 16 + 2 + j] = (i == j ? 1.0 : 0.0) + prod[i * 6 + j];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) Jm[(2 + i) * 16 + 8 + j] = -prod[i * 6 + j]; // dN/dbeta = -dN/dsigma
+    {
+        double t[9], v[6];
+        for (int i = 0; i < 9; ++i) t[i] = omD * dlp * dNdD[i] + ee[i];
+        stress_like6(Ce, t, v);
+        for (int i = 0; i < 6; ++i) Jm[(2 + i) * 16 + 14] = v[i];
+        for (int i = 0; i < 9; ++i) t[i] = dc.gamma * bet[i] - dc.C * N[i];
+        kinetic6(t, v);
+        for (int i = 0; i < 6; ++i) Jm[(8 + i) * 16 + 0] = v[i];
+        for (int i = 0; i < 9; ++i) t[i] = -dc.C * dlp * dNdD[i];
+        kinetic6(t, v);
+        for (int i = 0; i < 6; ++i) Jm[(8 + i) * 16 + 14] = v[i];
+    }
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+        {
+            Jm[(8 + i) * 16 + 2 + j] = -dc.C * dlp * dNds[i * 6 + j];
+            Jm[(8 + i) * 16 + 8 + j] = (i == j ? 1.0 + dc.gamma * dlp : 0.0) + dc.C * dlp * dNds[i * 6 + j];
+        }
+    Jm[14 * 16 + 1] = -1.0;
+    Jm[15 * 16 + 0] = ddot9(N, Ceem);
+    for (int j = 0; j < 6; ++j)
+    {
+        double t = 0.0;
+        for (int k = 0; k < 6; ++k) t += dNds[j * 6 + k] * Cee[k];
+        Jm[15 * 16 + 2 + j] = dlp * t;
+        Jm[15 * 16 + 8 + j] = -dlp * t;
+    }
+    Jm[15 * 16 + 14] = dlp * ddot9(dNdD, Ceem);
+}
+
+// perform_radial_return (:128-250) from the trial stress; the in/out state is the QP's back
+// stress, damage, energy release rate (and kinematic hardening = back stress / C); returns the
+// plastic increment times dt (as :249 does) or a negative status on failure.
+int damage_return(double G, double lam, double sy, double E, const double* Ce, const damage_constants& dc, double dt,
+                  const double* eps_e_t, double* sigma_t, double* beta_t, double* dmg_t, double* energy_t,
+                  double* kin_t, double* tangent, double* plastic_increment, int* iterations)
+{
+    double y[16];
+    y[0] = y[1] = 0.0;
+    kinetic6(sigma_t, y + 2);
+    kinetic6(beta_t, y + 8);
+    y[14] = *dmg_t;
+    y[15] = *energy_t;
+    double res[16];
+    for (double& v : res) v = 1.0;
+    double Jm[256] = {};
+    Jm[0 * 16 + 0] = Jm[1 * 16 + 1] = Jm[14 * 16 + 14] = Jm[15 * 16 + 15] = 1.0;
+    for (int i = 0; i < 6; ++i) Jm[(2 + i) * 16 + 2 + i] = Jm[(8 + i) * 16 + 8 + i] = 1.0;
+    double MCe[36];
+    int it = 0;
+    auto rnorm = [&] {
+        double t = 0.0;
+        for (double v : res) t += v * v;
+        return std::sqrt(t);
+    };
+    while (rnorm() > 1.0e-6 && it < 50)
+    {
+        damage_system(G, lam, sy, E, Ce, dc, dt, eps_e_t, beta_t, *dmg_t, y, res, Jm);
+        double A[256], dx[16];
+        std::memcpy(A, Jm, sizeof(A));
+        if (!fullpiv_solve(16, A, res, dx)) return ORC_RETURN_MAP_FAILED;
+        for (int i = 0; i < 16; ++i) y[i] -= dx[i];
+        ++it;
+    }
+    if (iterations) *iterations = it;
+    if (y[14] > 1.0) return ORC_RETURN_MAP_FAILED; // "Damage is greater than one" (:238-243)
+    if (it == 50) return ORC_RETURN_MAP_FAILED;
+    double const pinc = y[0] > 0.0 ? y[0] : 0.0;
+    kinetic_from6(y + 2, sigma_t);
+    kinetic_from6(y + 8, beta_t);
+    *dmg_t = y[14] > *dmg_t ? y[14] : *dmg_t;
+    *energy_t = y[15];
+    for (int i = 0; i < 9; ++i) kin_t[i] = beta_t[i] / dc.C;
+    // (1 - D) mandel(C_e) mandel(inverse(J)[2:8, 2:8]) with the Jacobian of the last iteration (:245-247)
+    double inv22[36];
+    for (int c = 0; c < 6; ++c)
+    {
+        double A[256], e[16] = {}, col[16];
+        std::memcpy(A, Jm, sizeof(A));
+        e[2 + c] = 1.0;
+        if (!fullpiv_solve(16, A, e, col)) return ORC_RETURN_MAP_FAILED;
+        for (int r = 0; r < 6; ++r) inv22[r * 6 + c] = col[2 + r];
+    }
+    mandel6(Ce, MCe);
+    double Mi[36], prod[36];
+    mandel6(inv22, Mi);
+    matmul6(MCe, Mi, prod);
+    for (int k = 0; k < 36; ++k) tangent[k] = (1.0 - *dmg_t) * prod[k];
+    *plastic_increment = pinc * dt;
+    return ORC_OK;
+}
+
+// small_strain_J2_plasticity_damage::update_internal_variables (:34-120) for one QP.  The
+// plastic strain, effective plastic strain, back stress and damage are updated in place, as
+// neon does at every call.
+int j2_damage(const double* H, double bulk, double shear, double sy, const damage_constants& dc, double dt,
+              double* eps_p, double* ebar, double* beta, double* dmg, double* energy, double* kin, double* sigma,
+              double* D, double* vm, int* iterations)
+{
+    double const G = shear, lam = bulk - 2.0 * shear / 3.0;
+    double const E = 9.0 * bulk * shear / (3.0 * bulk + shear); // isotropic_elastic_property.hpp:16
+    double Ce[36];
+    elastic_moduli(bulk, shear, Ce);
+    double strain[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) strain[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i]);
+    double ee[9];
+    for (int i = 0; i < 9; ++i) ee[i] = strain[i] - eps_p[i];
+    double const tr = ee[0] + ee[4] + ee[8];
+    // compute_cauchy_stress_scalar_damage (stress_routines.hpp:38-44)
+    for (int i = 0; i < 9; ++i) sigma[i] = (1.0 - *dmg) * (lam * (i % 4 == 0 ? tr : 0.0) + 2.0 * G * ee[i]);
+    double rel[9];
+    for (int i = 0; i < 9; ++i) rel[i] = sigma[i] - beta[i];
+    *vm = von_mises(rel);
+    {
+        double Cee[6], Ceem[9];
+        stress_like6(Ce, ee, Cee);
+        kinetic_from6(Cee, Ceem);
+        *energy = 0.5 * ddot9(ee, Ceem); // compute_energy_release_rate (:267-272)
+    }
+    if (iterations) *iterations = 0;
+    double const fp = *vm / (1.0 - *dmg) + 0.5 * dc.gamma / dc.C * ddot9(beta, beta) - sy;
+    double const fd = *energy - sy * sy / (2.0 * E);
+    if (fp <= 0.0 && fd <= 0.0)
+    {
+        std::memcpy(D, Ce, sizeof(Ce));
+        return ORC_OK;
+    }
+    double pinc = 0.0;
+    int const s = damage_return(G, lam, sy, E, Ce, dc, dt, ee, sigma, beta, dmg, energy, kin, D, &pinc, iterations);
+    if (s != ORC_OK) return s;
+    for (int i = 0; i < 9; ++i) rel[i] = sigma[i] - beta[i];
+    *vm = von_mises(rel);
+    double dev[9];
+    deviatoric3(rel, dev);
+    double const nd = norm9(dev);
+    for (int i = 0; i < 9; ++i) eps_p[i] += pinc * std::sqrt(3.0 / 2.0) * dev[i] / (nd * (1.0 - *dmg));
+    *ebar += pinc / (1.0 - *dmg);
+    return ORC_OK;
+}
+
 // microsphere.hpp:37-65 (Pade approximations of the inverse Langevin function)
 inline double pade_first(double l, double N) { return (3.0 * N - l * l) / (N - l * l); }
 inline double pade_second(double l, double N) { return (std::pow(l, 4) + 3.0 * N * N) / std::pow(N - l * l, 2); }
@@ -747,6 +1088,10 @@ struct orc_model
     // probabilities and the model's last non-zero time step (a model-wide member in neon)
     vector<double> age, age_o;
     double scission = 0.0, recombination = 0.0, last_dt = 0.0;
+    // J2 with damage: back stress (9 per QP), damage, energy release rate and their committed
+    // copies, kinematic hardening (back stress / C), the material's constants
+    vector<double> beta, dmg, energy, beta_o, dmg_o, energy_o, kin;
+    damage_constants dconst{1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
     // recorded load history: u_n and the committed history h_{n-1} of each converged step
     vector<vector<double>> traj_u, traj_ep, traj_eb;
 };
@@ -800,8 +1145,9 @@ int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int
 {
     if (!out || n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params) return ORC_BAD_ARGUMENT;
     int const base = model & 0xff;
-    if ((model & ~(0xff | ORC_SPHERE_BO33)) != 0 || base > ORC_MODEL_MICROSPHERE_AGEING) return ORC_BAD_ARGUMENT;
-    if (base == ORC_MODEL_MICROSPHERE_AGEING && tangent_mode == ORC_TANGENT_CONSISTENT) return ORC_BAD_ARGUMENT;
+    if ((model & ~(0xff | ORC_SPHERE_BO33)) != 0 || base > ORC_MODEL_J2_DAMAGE) return ORC_BAD_ARGUMENT;
+    if ((base == ORC_MODEL_MICROSPHERE_AGEING || base == ORC_MODEL_J2_DAMAGE) && tangent_mode == ORC_TANGENT_CONSISTENT)
+        return ORC_BAD_ARGUMENT;
     if ((model & ORC_SPHERE_BO33) && !is_microsphere(model)) return ORC_BAD_ARGUMENT;
     for (int64_t i = 0; i < int64_t(n_elem) * 8; ++i)
         if (conn[i] < 0 || conn[i] >= n_nodes) return ORC_BAD_ARGUMENT;
@@ -824,9 +1170,20 @@ int orc_model_create(int32_t n_nodes, const double* X, int32_t n_elem, const int
     m->ebar.assign(Q, 0.0);
     m->vm.assign(Q, 0.0);
     m->D.assign(36 * Q, 0.0);
-    if (model == ORC_MODEL_J2 || model == ORC_MODEL_LINEAR_ELASTIC) // isotropic_linear_elasticity.cpp:22-23
+    if (model == ORC_MODEL_J2 || model == ORC_MODEL_LINEAR_ELASTIC
+        || model == ORC_MODEL_J2_DAMAGE) // isotropic_linear_elasticity.cpp:22-23
         for (size_t q = 0; q < Q; ++q)
             elastic_moduli(m->params[4 * (q / 8)], m->params[4 * (q / 8) + 1], &m->D[36 * q]);
+    if (model == ORC_MODEL_J2_DAMAGE)
+    {
+        m->beta.assign(9 * Q, 0.0);
+        m->kin.assign(9 * Q, 0.0);
+        m->dmg.assign(Q, 0.0);
+        m->energy.assign(Q, 0.0);
+        m->beta_o = m->beta;
+        m->dmg_o = m->dmg;
+        m->energy_o = m->energy;
+    }
     if (base == ORC_MODEL_MICROSPHERE_AGEING)
     {
         // gaussian_ageing_affine_microsphere.cpp:17-50: active modulus G, inactive modulus
@@ -941,6 +1298,20 @@ int orc_update_internal_variables(orc_model* m, const double* u, double dt)
                    m->last_dt, &m->age[11 * q], &m->age_o[11 * q + 5], &m->age[11 * q + 5], &m->sigma[9 * q],
                    &m->D[36 * q]);
     }
+    else if (m->model == ORC_MODEL_J2_DAMAGE)
+    {
+        int failed = 0;
+#pragma omp parallel for schedule(static) reduction(| : failed)
+        for (int64_t q = 0; q < int64_t(Q); ++q)
+        {
+            double const* p = &m->params[4 * (q / 8)];
+            int const s = j2_damage(&m->Hg[9 * q], p[0], p[1], p[2], m->dconst, dt, &m->eps_p[9 * q], &m->ebar[q],
+                                    &m->beta[9 * q], &m->dmg[q], &m->energy[q], &m->kin[9 * q], &m->sigma[9 * q],
+                                    &m->D[36 * q], &m->vm[q], nullptr);
+            if (s != ORC_OK) failed |= 1;
+        }
+        if (failed) status = ORC_RETURN_MAP_FAILED;
+    }
     else if (is_microsphere(m->model))
     {
         int const consistent = m->tangent_mode == ORC_TANGENT_CONSISTENT;
@@ -991,6 +1362,9 @@ int orc_save_internal_variables(orc_model* m, int32_t converged)
         m->ebar_o = m->ebar;
         m->vm_o = m->vm;
         m->age_o = m->age;
+        m->beta_o = m->beta;
+        m->dmg_o = m->dmg;
+        m->energy_o = m->energy;
     }
     else
     {
@@ -1002,8 +1376,57 @@ int orc_save_internal_variables(orc_model* m, int32_t converged)
         m->ebar = m->ebar_o;
         m->vm = m->vm_o;
         m->age = m->age_o;
+        m->beta = m->beta_o;
+        m->dmg = m->dmg_o;
+        m->energy = m->energy_o;
+        // the kinematic hardening variable is the back stress over the modulus at every point
+        for (size_t k = 0; k < m->kin.size(); ++k) m->kin[k] = m->beta[k] / m->dconst.C;
     }
     return ORC_OK;
+}
+
+int orc_model_set_damage(orc_model* m, const double* c)
+{
+    if (!m || !c || m->model != ORC_MODEL_J2_DAMAGE) return ORC_BAD_ARGUMENT;
+    for (int k = 0; k < 6; ++k)
+        if (!(c[k] > 0.0)) return ORC_BAD_ARGUMENT;
+    m->dconst = damage_constants{c[0], c[1], c[2], c[3], c[4], c[5]};
+    return ORC_OK;
+}
+
+int orc_get_damage_state(const orc_model* m, double* back_stress, double* damage, double* energy)
+{
+    if (!m || m->model != ORC_MODEL_J2_DAMAGE) return ORC_BAD_ARGUMENT;
+    if (back_stress) std::copy(m->beta.begin(), m->beta.end(), back_stress);
+    if (damage) std::copy(m->dmg.begin(), m->dmg.end(), damage);
+    if (energy) std::copy(m->energy.begin(), m->energy.end(), energy);
+    return ORC_OK;
+}
+
+int orc_j2_damage_system(const double* eps_e_t, const double* back_stress, double damage, double bulk, double shear,
+                         double yield, const double* c, double dt, const double* y, double* residual, double* J)
+{
+    damage_constants const dc{c[0], c[1], c[2], c[3], c[4], c[5]};
+    double Ce[36];
+    elastic_moduli(bulk, shear, Ce);
+    double const E = 9.0 * bulk * shear / (3.0 * bulk + shear);
+    for (int k = 0; k < 256; ++k) J[k] = 0.0;
+    J[0 * 16 + 0] = J[1 * 16 + 1] = J[14 * 16 + 14] = J[15 * 16 + 15] = 1.0;
+    for (int i = 0; i < 6; ++i) J[(2 + i) * 16 + 2 + i] = J[(8 + i) * 16 + 8 + i] = 1.0;
+    damage_system(shear, bulk - 2.0 * shear / 3.0, yield, E, Ce, dc, dt, eps_e_t, back_stress, damage, y, residual, J);
+    return ORC_OK;
+}
+
+int orc_j2_damage_qp(const double* H, double bulk, double shear, double yield, const double* c, double dt,
+                     double* eps_p, double* ebar, double* back_stress, double* damage, double* energy,
+                     double* kinematic_hardening, double* sigma, double* D, double* von_mises, int32_t* iterations)
+{
+    damage_constants const dc{c[0], c[1], c[2], c[3], c[4], c[5]};
+    int its = 0;
+    int const s = j2_damage(H, bulk, shear, yield, dc, dt, eps_p, ebar, back_stress, damage, energy,
+                            kinematic_hardening, sigma, D, von_mises, &its);
+    if (iterations) *iterations = its;
+    return s;
 }
 
 int orc_model_set_ageing(orc_model* m, double scission, double recombination)

@@ -70,6 +70,7 @@ enum {
     ORC_MODEL_MICROSPHERE_LANGEVIN = 4, /* affine_microsphere */
     ORC_MODEL_MICROSPHERE_NONAFFINE = 5,/* nonaffine_microsphere */
     ORC_MODEL_MICROSPHERE_AGEING = 6,   /* gaussian_ageing_affine_microsphere (BAND) */
+    ORC_MODEL_J2_DAMAGE = 7,            /* small_strain_J2_plasticity_damage */
     ORC_SPHERE_BO33 = 0x100
 };
 
@@ -123,6 +124,26 @@ int orc_save_internal_variables(orc_model* m, int32_t converged);
  * moduli, reduction factor, active and inactive segments, and the secondary network's
  * intermediate Kirchhoff stress (11, 22, 33, 23, 13, 12). */
 int orc_model_set_ageing(orc_model* m, double scission, double recombination);
+
+/* J2 with damage: the material constants of isotropic_elastic_plastic_damage (kinematic hardening
+ * modulus, softening multiplier, plasticity viscous denominator and exponent, damage viscous
+ * denominator and exponent); the per-element params are (K, G, yield stress, unused).  State per
+ * QP: back stress (9), damage, energy release rate; the kinematic hardening variable is the back
+ * stress over the modulus. */
+int orc_model_set_damage(orc_model* m, const double* constants /*6*/);
+int orc_get_damage_state(const orc_model* m, double* back_stress /*9/qp or NULL*/, double* damage /*1/qp or NULL*/,
+                         double* energy /*1/qp or NULL*/);
+/* The return map's residual (16) and Jacobian (16x16 row-major) at unknowns y (plastic and damage
+ * multipliers, kinetic stress (6), kinetic back stress (6), damage, energy release rate), for the
+ * trial elastic strain (3x3) and the back stress (3x3) and damage at the start of the update. */
+int orc_j2_damage_system(const double* eps_e_t, const double* back_stress, double damage, double bulk, double shear,
+                         double yield, const double* constants, double dt, const double* y, double* residual,
+                         double* J);
+/* One QP of small_strain_J2_plasticity_damage::update_internal_variables (in place on eps_p, ebar,
+ * back_stress, damage, energy); kinematic hardening (9) written; iterations of the return map. */
+int orc_j2_damage_qp(const double* H, double bulk, double shear, double yield, const double* constants, double dt,
+                     double* eps_p, double* ebar, double* back_stress, double* damage, double* energy,
+                     double* kinematic_hardening, double* sigma, double* D, double* von_mises, int32_t* iterations);
 int orc_get_ageing_state(const orc_model* m, double* state /*11/qp*/);
 
 /* Per-QP state downloads (QP index e*8 + l). */

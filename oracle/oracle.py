@@ -22,6 +22,7 @@ FATAL, BAD_ARGUMENT = -1, -2
 MODEL_NEOHOOKE, MODEL_J2, MODEL_LINEAR_ELASTIC = 0, 1, 2
 MODEL_MICROSPHERE_GAUSSIAN, MODEL_MICROSPHERE_LANGEVIN, MODEL_MICROSPHERE_NONAFFINE = 3, 4, 5
 MODEL_MICROSPHERE_AGEING = 6
+MODEL_J2_DAMAGE = 7
 SPHERE_BO33 = 0x100
 STOP_RELATIVE, STOP_ABSOLUTE_PRECONDITIONED = 0, 1
 TANGENT_REFERENCE, TANGENT_CONSISTENT = 0, 1
@@ -64,6 +65,12 @@ _SIG = {
     "orc_sphere_rule": [C.c_int32, f64p, f64p],
     "orc_model_set_ageing": [C.c_void_p, C.c_double, C.c_double],
     "orc_get_ageing_state": [C.c_void_p, f64p],
+    "orc_model_set_damage": [C.c_void_p, f64p],
+    "orc_j2_damage_system": [f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_double, f64p, C.c_double, f64p,
+                             f64p, f64p],
+    "orc_get_damage_state": [C.c_void_p, f64p, f64p, f64p],
+    "orc_j2_damage_qp": [f64p, C.c_double, C.c_double, C.c_double, f64p, C.c_double, f64p, f64p, f64p, f64p, f64p,
+                         f64p, f64p, f64p, f64p, i32p],
     "orc_j2_qp": [f64p, C.c_double, C.c_double, C.c_double, C.c_double, f64p, f64p, f64p, f64p, f64p],
     "orc_model_create": [C.c_int32, f64p, C.c_int32, i32p, C.c_int32, f64p, C.c_int32, P(C.c_void_p)],
     "orc_model_destroy": [C.c_void_p],
@@ -169,6 +176,37 @@ def j2_qp(H, bulk, shear, yield_stress, hardening, eps_p=None, ebar=0.0):
     return dict(status=code, sigma=s, D=D, vm=vm[0], eps_p=ep, ebar=eb[0])
 
 
+def j2_damage_qp(H, bulk, shear, yield_stress, constants, dt, state=None):
+    """One QP of small_strain_J2_plasticity_damage::update_internal_variables.  constants =
+    (kinematic hardening modulus, softening multiplier, plasticity viscous denominator and
+    exponent, damage viscous denominator and exponent); state: dict of eps_p, ebar, back_stress,
+    damage, energy (updated in place and returned with sigma, D, vm, kinematic_hardening)."""
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    st = dict(eps_p=np.zeros((3, 3)), ebar=np.zeros(1), back_stress=np.zeros((3, 3)), damage=np.zeros(1),
+              energy=np.zeros(1)) if state is None else state
+    for k in ("ebar", "damage", "energy"):
+        st[k] = np.atleast_1d(np.asarray(st[k], dtype=np.float64)).copy()
+    for k in ("eps_p", "back_stress"):
+        st[k] = np.ascontiguousarray(st[k], dtype=np.float64).copy()
+    c = np.ascontiguousarray(constants, dtype=np.float64)
+    s, D, vm, X, its = np.zeros((3, 3)), np.zeros((6, 6)), np.zeros(1), np.zeros((3, 3)), C.c_int32(0)
+    code = lib().orc_j2_damage_qp(_f(H), bulk, shear, yield_stress, _f(c), dt, _f(st["eps_p"]), _f(st["ebar"]),
+                                  _f(st["back_stress"]), _f(st["damage"]), _f(st["energy"]), _f(X), _f(s), _f(D),
+                                  _f(vm), C.byref(its))
+    st.update(status=code, sigma=s, D=D, vm=vm[0], kinematic_hardening=X, iterations=its.value)
+    return st
+
+
+def j2_damage_system(eps_e_t, back_stress, damage, bulk, shear, yield_stress, constants, dt, y):
+    """Residual (16) and Jacobian (16x16) of the damage model's return map at unknowns y."""
+    r, J = np.zeros(16), np.zeros((16, 16))
+    lib().orc_j2_damage_system(_f(np.ascontiguousarray(eps_e_t, dtype=np.float64)),
+                               _f(np.ascontiguousarray(back_stress, dtype=np.float64)), float(damage), bulk, shear,
+                               yield_stress, _f(np.ascontiguousarray(constants, dtype=np.float64)), dt,
+                               _f(np.ascontiguousarray(y, dtype=np.float64)), _f(r), _f(J))
+    return r, J
+
+
 class Model:
     """Mesh-level oracle model (constitutive state + element matrices + assembly)."""
 
@@ -258,6 +296,19 @@ class Model:
         if lib().orc_get_ageing_state(self.h, _f(out)) != OK:
             raise RuntimeError("orc_get_ageing_state failed")
         return out
+
+    def set_damage(self, constants):
+        c = np.ascontiguousarray(constants, dtype=np.float64)
+        if lib().orc_model_set_damage(self.h, _f(c)) != OK:
+            raise RuntimeError("orc_model_set_damage failed")
+
+    def damage_state(self):
+        """back stress (Q, 3, 3), damage (Q), energy release rate (Q) of the J2 damage model."""
+        Q = 8 * self.n_elem
+        b, d, y = np.zeros(9 * Q), np.zeros(Q), np.zeros(Q)
+        if lib().orc_get_damage_state(self.h, _f(b), _f(d), _f(y)) != OK:
+            raise RuntimeError("orc_get_damage_state failed")
+        return dict(back_stress=b.reshape(Q, 3, 3), damage=d, energy=y)
 
     def latin_force(self, search_direction):
         """orc_latin_force at the model's state (3N)."""
