@@ -29,6 +29,9 @@
  *   neon_mesh_set_ageing       ageing_micromechanical_elastomer (micromechanical_elastomer.cpp:34-53)
  *   neon_mesh_ageing_state     the ageing model's internal variables
  *                              (gaussian_ageing_affine_microsphere.cpp:17-50)
+ *   neon_mesh_set_damage,      isotropic_elastic_plastic_damage (isotropic_elastic_plastic_damage.cpp:9-45)
+ *   neon_mesh_damage_state     and small_strain_J2_plasticity_damage's internal variables
+ *                              (small_strain_J2_plasticity_damage.cpp:17-30)
  *   neon_mesh_assemble         static_matrix::assemble_stiffness + compute_internal_force
  *                              (src/assembler/mechanics/static_matrix.hpp:189-203, 243-277) over
  *                              submesh::tangent_stiffness / internal_force (submesh.cpp:66-160)
@@ -175,6 +178,18 @@ int neon_mesh_set_ageing(neon_context* ctx, double scission, double recombinatio
  * shear modulus, reduction factor, active and inactive segments, secondary network's
  * intermediate stress (11, 22, 33, 23, 13, 12). */
 int neon_mesh_ageing_state(neon_context* ctx, double* state);
+/* J2 with damage (NEON_MODEL_J2_DAMAGE; per-element params (K, G, yield stress, unused)): the
+ * material constants of isotropic_elastic_plastic_damage (isotropic_elastic_plastic_damage.cpp:
+ * 9-45), in the order kinematic_hardening_modulus, softening_multiplier,
+ * plasticity_viscous_denominator, plasticity_viscous_exponent, damage_viscous_denominator,
+ * damage_viscous_exponent (all > 0; 1 before the first call).  The model's rate form integrates
+ * over the time_step of neon_mesh_update_internal_variables (real time: neon passes the load
+ * increment at a step's first update and 0 at its Newton iterates, where nothing flows). */
+int neon_mesh_set_damage(neon_context* ctx, const double* constants);
+/* J2 with damage: per QP (index e*8+l) the back stress (9, row-major), damage and energy release
+ * rate; any pointer may be NULL.  The kinematic hardening variable is the back stress divided by
+ * the kinematic hardening modulus (small_strain_J2_plasticity_damage.cpp:245). */
+int neon_mesh_damage_state(neon_context* ctx, double* back_stress, double* damage, double* energy);
 /* Per-QP state, index e*8+l.  Any pointer may be NULL.  sigma/eps_p 9 per QP (row-major
  * 3x3), D 36 per QP (6x6 Voigt, engineering shear in strain), detF/ebar/vm 1 per QP. */
 int neon_mesh_qp_state(neon_context* ctx, double* sigma, double* D, double* detF, double* eps_p, double* ebar,

@@ -24,6 +24,7 @@ from ._capi import (  # noqa: F401
     MODEL_MICROSPHERE_LANGEVIN,
     MODEL_MICROSPHERE_AGEING,
     MODEL_MICROSPHERE_NONAFFINE,
+    MODEL_J2_DAMAGE,
     MODEL_NEOHOOKE,
     SPHERE_BO33,
     PRECISION_FP64,
@@ -297,8 +298,9 @@ class DeviceMesh:
     """Device-resident hex8 mesh, constitutive state and tangent (one per context)."""
 
     def __init__(self, X: np.ndarray, conn: np.ndarray, params: np.ndarray, model: int = MODEL_NEOHOOKE,
-                 tangent: int = TANGENT_REFERENCE, context: Context | None = None, ageing=None):
-        """`ageing` = (scission, recombination) probabilities of the ageing microsphere model."""
+                 tangent: int = TANGENT_REFERENCE, context: Context | None = None, ageing=None, damage=None):
+        """`ageing` = (scission, recombination) probabilities of the ageing microsphere model;
+        `damage` = the six constants of the J2 damage model (``neon_mesh_set_damage``)."""
         self.ctx = context or Context(0)
         self.X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
         self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1)
@@ -311,7 +313,25 @@ class DeviceMesh:
             _capi.f64(self.params), tangent))
         if ageing is not None:
             self.set_ageing(*ageing)
+        if damage is not None:
+            self.set_damage(damage)
         self._constitutive = None
+
+    def set_damage(self, constants) -> None:
+        """The J2 damage model's material constants: kinematic hardening modulus, softening
+        multiplier, plasticity viscous denominator and exponent, damage viscous denominator and
+        exponent (``neon_mesh_set_damage``)."""
+        c = np.ascontiguousarray(constants, dtype=np.float64)
+        if c.size != 6:
+            raise ValueError("six damage constants")
+        self.ctx.check(_capi.lib().neon_mesh_set_damage(self.ctx.handle, _capi.f64(c)))
+
+    def damage_state(self) -> dict:
+        """back stress (Q, 3, 3), damage (Q), energy release rate (Q) of the J2 damage model."""
+        Q = 8 * self.n_elem
+        b, d, y = np.zeros(9 * Q), np.zeros(Q), np.zeros(Q)
+        self.ctx.check(_capi.lib().neon_mesh_damage_state(self.ctx.handle, _capi.f64(b), _capi.f64(d), _capi.f64(y)))
+        return dict(back_stress=b.reshape(Q, 3, 3), damage=d, energy=y)
 
     def constitutive(self, material_name: str = "", density: float | None = None) -> constitutive_model:
         """The mesh's constitutive model (``submesh::constitutive``, submesh.hpp:62)."""

@@ -169,6 +169,8 @@ SIGNATURES = {
     ),
     "neon_mesh_create": (C.c_int, [C.c_void_p, C.c_int32, f64p, C.c_int32, i32p, C.c_int32, f64p, C.c_int32]),
     "neon_model_traits": (C.c_int, [C.c_int32, i32p, i32p]),
+    "neon_mesh_set_damage": (C.c_int, [C.c_void_p, f64p]),
+    "neon_mesh_damage_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     "neon_model_variable_names": (C.c_char_p, [C.c_int32]),
     "neon_mesh_model": (C.c_int, [C.c_void_p, i32p]),
     "neon_mesh_set_parameters": (C.c_int, [C.c_void_p, f64p]),

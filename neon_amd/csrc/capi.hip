@@ -226,6 +226,11 @@ bool traits_of(int32_t model, model_traits& t)
     case NEON_MODEL_MICROSPHERE_AGEING:
         t = {1, 1, "active_segments;active_shear_modulus;inactive_segments;inactive_shear_modulus;reduction_factor"};
         return true;
+    case NEON_MODEL_J2_DAMAGE: // small_strain_J2_plasticity_damage.hpp:42-44, .cpp:21-29
+        t = {0, 0,
+             "back_stress;damage;effective_plastic_strain;energy_release_rate;kinematic_hardening;"
+             "linearised_plastic_strain;linearised_strain;von_mises_stress"};
+        return true;
     default: return false;
     }
 }
@@ -310,6 +315,25 @@ int neon_mesh_set_ageing(neon_context* ctx, double scission, double recombinatio
     return guarded(ctx, [&] {
         need_mesh(ctx);
         ctx->mesh->set_ageing(scission, recombination);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_set_damage(neon_context* ctx, const double* constants)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (!constants) throw status_error(NEON_BAD_ARGUMENT, "null constants");
+        ctx->mesh->set_damage(constants);
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_damage_state(neon_context* ctx, double* back_stress, double* damage, double* energy)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        ctx->mesh->download_damage_state(back_stress, damage, energy);
         return NEON_OK;
     });
 }

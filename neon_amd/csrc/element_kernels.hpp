@@ -447,20 +447,11 @@ __device__ void ageing_update(const double* F, double J, const double* prm, int 
 }
 
 // ---------------------------------------------------- constitutive update per QP
-// One thread per quadrature point (qp = 8 e + l).
-template <int MODEL>
-__global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const int32_t* __restrict__ conn,
-                                                              const double* __restrict__ X,
-                                                              const double* __restrict__ u, element_params prm,
-                                                              qp_state st, int tangent_mode, int rule,
-                                                              ageing_args ag, uint32_t* __restrict__ status)
+// Deformation measures of quadrature point l of element e (submesh.cpp:210-237): with
+// F_0 = X dN_l, H = (x - X) dN_l F_0^-1 (displacement gradient) and F = (x dN_l) F_0^-1, x = X + u.
+__device__ inline void qp_kinematics(const int32_t* __restrict__ conn, const double* __restrict__ X,
+                                     const double* __restrict__ u, int64_t e, int l, double* H, double* F)
 {
-    int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (qp >= 8ll * E) return;
-    int64_t const e = qp >> 3;
-    int const l = int(qp & 7);
-    int64_t const Q = st.Q;
-
     double F0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fx[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double dXe[8][3];
     for (int a = 0; a < 8; ++a)
@@ -480,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
     }
     double F0i[9];
     inv3(F0, F0i);
-    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 9; ++k) H[k] = 0.0;
     for (int a = 0; a < 8; ++a)
     {
         double B0[3];
@@ -490,10 +481,27 @@ __global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const i
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) H[i * 3 + j] += dXe[a][i] * B0[j];
     }
-    double F[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             F[i * 3 + j] = Fx[i * 3 + 0] * F0i[0 * 3 + j] + Fx[i * 3 + 1] * F0i[1 * 3 + j] + Fx[i * 3 + 2] * F0i[2 * 3 + j];
+}
+
+// One thread per quadrature point (qp = 8 e + l).
+template <int MODEL>
+__global__ __launch_bounds__(kBlock) void update_state_kernel(int32_t E, const int32_t* __restrict__ conn,
+                                                              const double* __restrict__ X,
+                                                              const double* __restrict__ u, element_params prm,
+                                                              qp_state st, int tangent_mode, int rule,
+                                                              ageing_args ag, uint32_t* __restrict__ status)
+{
+    int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (qp >= 8ll * E) return;
+    int64_t const e = qp >> 3;
+    int const l = int(qp & 7);
+    int64_t const Q = st.Q;
+
+    double H[9], F[9];
+    qp_kinematics(conn, X, u, e, l, H, F);
     double const J = det3(F);
     st.detF[qp] = J;
     if (signbit(J)) atomicOr(status, STATUS_NEGATIVE_JACOBIAN);
@@ -755,13 +763,16 @@ __global__ __launch_bounds__(kBlock) void history_copy_kernel(int64_t n, const d
 __global__ __launch_bounds__(kBlock) void expand_tangent_kernel(int64_t Q, const double* __restrict__ coef,
                                                                 const double* __restrict__ sigma,
                                                                 double* __restrict__ D, double* __restrict__ s9,
-                                                                bool full)
+                                                                int kind)
 {
+    // kind 0: structured coefficients; 1: 21 symmetric entries; 2: 36 entries row by row
     int64_t const qp = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (qp >= Q) return;
+    bool const full = kind != 0;
     if (D && full)
         for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < 6; ++j) D[36 * qp + 6 * i + j] = coef[int64_t(upper_index(i, j)) * Q + qp];
+            for (int j = 0; j < 6; ++j)
+                D[36 * qp + 6 * i + j] = coef[int64_t(kind == 2 ? 6 * i + j : upper_index(i, j)) * Q + qp];
     double const A = coef[COEF_A * Q + qp], B = coef[COEF_B * Q + qp], C = coef[COEF_C * Q + qp];
     double n[6];
     for (int k = 0; k < 6; ++k) n[k] = coef[(COEF_N + k) * Q + qp];
@@ -807,7 +818,9 @@ __global__ __launch_bounds__(kBlock) void expand_tangent_kernel(int64_t Q, const
 // FULL: a general symmetric D (21 stored coefficients, microsphere models); phase B then forms
 //   K_ab = sum_l B_a^T D B_b (+ the geometric term) from the engineering-shear columns of B:
 //   column j of B_b has g_b,j in row j and g_b in two shear rows (kShearRow / kShearComp).
-template <typename KT, bool FINITE, bool FULL = false>
+// GENERAL (with FULL): a general D, 36 stored coefficients row by row (the unsymmetric J2
+//   damage tangent); K_ab = B_a^T D B_b is then not K_ba^T.
+template <typename KT, bool FINITE, bool FULL = false, bool GENERAL = false>
 __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict__ elems, int32_t count,
                                                       const int32_t* __restrict__ conn,
                                                       const double* __restrict__ X,
@@ -818,7 +831,7 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
 {
     // coefficients per QP: A, Bt, C, n(6) (n unscaled) or the 21 of a general D; then sigma(6),
     // all scaled by w detJ
-    constexpr int NCOEF = FULL ? kFullCoefs : 9;
+    constexpr int NCOEF = GENERAL ? 36 : FULL ? kFullCoefs : 9;
     constexpr int SIG = NCOEF;
     constexpr int NC = NCOEF + 6;
     __shared__ double s_x[8][3];
@@ -905,7 +918,7 @@ __global__ __launch_bounds__(64) void assemble_kernel(const int32_t* __restrict_
 #pragma unroll
                     for (int i = 0; i < 6; ++i)
 #pragma unroll
-                        for (int j = 0; j < 6; ++j) Dm[i][j] = s_c[l][upper_index(i, j)];
+                        for (int j = 0; j < 6; ++j) Dm[i][j] = s_c[l][GENERAL ? i * 6 + j : upper_index(i, j)];
 #pragma unroll
                     for (int j = 0; j < 3; ++j)
                     {

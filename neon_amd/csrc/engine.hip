@@ -1,12 +1,15 @@
 // Host engine of the gfx950 implicit Newton step: Krylov drivers and the device mesh.
 #include "engine.hpp"
+#include "damage_kernels.hpp"
 #include "element_kernels.hpp"
 #include "krylov_kernels.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include <omp.h>
 
@@ -228,13 +231,36 @@ krylov_engine::krylov_engine(hipStream_t s) : s_(s)
     for (auto& p : part_) p.resize(kMaxPartials);
     state_.resize(2);
     scalar_.resize(1);
+    barrier_.resize(2);
+    NEON_CHECK_HIP(hipMemsetAsync(barrier_.data(), 0, 2 * sizeof(uint32_t), s_));
     NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_state_), sizeof(krylov_state)));
+    NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ring_), 2 * sizeof(krylov_state)));
+    for (auto& e : ring_events_) NEON_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (const char* f = std::getenv("NEON_FUSED_UPDATES")) fused_updates = std::atoi(f) != 0;
     NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_scalar_), sizeof(double)));
+}
+
+// A fused update whose grid barrier timed out leaves its solve's result undefined: fail the
+// solve and re-arm the barrier word.
+void krylov_engine::check_barrier()
+{
+    uint32_t h[2] = {0, 0};
+    NEON_CHECK_HIP(hipMemcpyAsync(h, barrier_.data(), sizeof(h), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    if (h[1] != 0)
+    {
+        NEON_CHECK_HIP(hipMemsetAsync(barrier_.data(), 0, 2 * sizeof(uint32_t), s_));
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
+        throw status_error(NEON_HIP_ERROR, "grid barrier of the fused CG update timed out");
+    }
 }
 
 krylov_engine::~krylov_engine()
 {
     if (host_state_) (void)hipHostFree(host_state_);
+    if (host_ring_) (void)hipHostFree(host_ring_);
+    for (auto& e : ring_events_)
+        if (e) (void)hipEventDestroy(e);
     if (host_scalar_) (void)hipHostFree(host_scalar_);
 }
 
@@ -292,6 +318,89 @@ solve_result finish(const krylov_state& s, int criterion)
 }
 } // namespace
 
+namespace
+{
+// co-resident workgroups of a kernel on the current device (0 when it cannot be queried)
+int coresident_blocks(const void* kernel)
+{
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+template <typename T, int EPT, bool MIXED>
+bool try_fused(hipStream_t s, int64_t n, T* x, T* r, T* p, const T* Ap, const T* minv, const double* part_pAp,
+               int npart, const krylov_state* st, krylov_state* st_next, double* part_rr, double* part_rz,
+               int criterion, double tol, int max_it, uint32_t* bar)
+{
+    // FP32 keeps the old p in registers as well as z; FP64 keeps z only (register budget)
+    constexpr bool keep_p = sizeof(T) == 4;
+    auto kernel = cg_fused_update_kernel<T, EPT, keep_p, MIXED>;
+    static thread_local int capacity = -1;
+    if (capacity < 0) capacity = coresident_blocks(reinterpret_cast<const void*>(kernel));
+    int64_t const grid = (n + int64_t(kBlock) * EPT - 1) / (int64_t(kBlock) * EPT);
+    if (grid < 1 || grid > capacity || grid > kMaxPartials) return false;
+    void* args[] = {&n, &x, &r, &p, &Ap, &minv, &part_pAp, &npart, &st, &st_next, &part_rr, &part_rz, &criterion,
+                    &tol, &max_it, &bar};
+    NEON_CHECK_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kernel), dim3(unsigned(grid)),
+                                              dim3(kBlock), args, 0, s));
+    return true;
+}
+} // namespace
+
+template <typename T, bool MIXED>
+bool krylov_engine::fused_update(int64_t n, T* x, T* r, T* p, const T* Ap, const T* minv, const double* part_pAp,
+                                 int npart, const krylov_state* st, krylov_state* st_next, double* part_rr,
+                                 double* part_rz, int criterion, double tol, int max_it)
+{
+    if (!fused_updates) return false;
+    auto go = [&](auto ept) {
+        constexpr int E = decltype(ept)::value;
+        return try_fused<T, E, MIXED>(s_, n, x, r, p, Ap, minv, part_pAp, npart, st, st_next, part_rr, part_rz,
+                                      criterion, tol, max_it, barrier_.data());
+    };
+    if (fused_ept == 8) return go(std::integral_constant<int, 8>{});
+    if (fused_ept == 16) return go(std::integral_constant<int, 16>{});
+    if (fused_ept == 32) return go(std::integral_constant<int, 32>{});
+    return go(std::integral_constant<int, 8>{}) || go(std::integral_constant<int, 16>{})
+           || go(std::integral_constant<int, 32>{});
+}
+
+// Drives a device-resident Krylov loop: `batch` launches poll_every iterations (advancing
+// `launched`; the state after `launched` iterations is st[launched & 1]).  The host keeps one
+// batch queued ahead of the one whose final state it is waiting for, so the device never idles
+// on a poll; iterations launched after convergence are no-ops.  On return the stream is idle
+// and host_state_ holds the final state.
+template <typename Batch>
+void krylov_engine::run_batches(krylov_state* st, int64_t& launched, Batch&& batch)
+{
+    auto snapshot = [&](int slot) {
+        NEON_CHECK_HIP(hipMemcpyAsync(host_ring_ + slot, st + (launched & 1), sizeof(krylov_state),
+                                      hipMemcpyDeviceToHost, s_));
+        NEON_CHECK_HIP(hipEventRecord(ring_events_[slot], s_));
+    };
+    snapshot(0);
+    NEON_CHECK_HIP(hipEventSynchronize(ring_events_[0]));
+    if (!host_ring_[0].done)
+    {
+        batch();
+        NEON_CHECK_HIP(hipGetLastError());
+        snapshot(1);
+        for (int wait = 1;; wait ^= 1)
+        {
+            batch();
+            NEON_CHECK_HIP(hipGetLastError());
+            snapshot(wait ^ 1);
+            NEON_CHECK_HIP(hipEventSynchronize(ring_events_[wait]));
+            if (host_ring_[wait].done) break;
+        }
+    }
+    NEON_CHECK_HIP(hipMemcpyAsync(host_state_, st + (launched & 1), sizeof(krylov_state), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
 // Jacobi-PCG in FP64.  Per iteration three kernels: SpMV fused with p.Ap, the x/r update
 // fused with r.r and r.z, and the p update with the convergence decision; scalars stay on
 // the device and the host polls the state every `poll_every` iterations.  Iterations past
@@ -314,11 +423,7 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
 
     solve_result res;
     int64_t launched = 0;
-    for (;;)
-    {
-        NEON_CHECK_HIP(hipMemcpyAsync(host_state_, st + (launched & 1), sizeof(krylov_state), hipMemcpyDeviceToHost, s_));
-        NEON_CHECK_HIP(hipStreamSynchronize(s_));
-        if (host_state_->done) break;
+    run_batches(st, launched, [&] {
         for (int k = 0; k < poll_every; ++k, ++launched)
         {
             krylov_state* cur = st + (launched & 1);
@@ -326,13 +431,16 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
             int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP64) : -1;
             A.apply64(s_, p_.data(), Ap_.data(), part_pAp, cur);
             if (prof) prof->end(s_, tok);
+            if (fused_update<double, false>(n, x, r_.data(), p_.data(), Ap_.data(), minv, part_pAp, ns, cur, nxt,
+                                            part_rr, part_rz, criterion, tol, max_it))
+                continue;
             cg_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x, r_.data(), p_.data(), Ap_.data(), minv, part_pAp, ns, cur,
                                                         part_rr, part_rz);
             cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p_.data(), r_.data(), minv, part_rr, part_rz, nb, cur, nxt,
                                                        criterion, tol, max_it);
         }
-        NEON_CHECK_HIP(hipGetLastError());
-    }
+    });
+    if (fused_updates) check_barrier();
     res = finish(*host_state_, criterion);
     // SpMV launches that did work: Eigen's rule counts the converging iteration's SpMV
     // outside `iterations`; the CUDA rule counts every iteration.
@@ -548,11 +656,7 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
     linear_operator Ar = A;
     if (Ar.node_block && Ar.sell) Ar.blocks_override = 2048;
     int64_t launched = 0;
-    for (;;)
-    {
-        NEON_CHECK_HIP(hipMemcpyAsync(host_state_, st + (launched & 1), sizeof(krylov_state), hipMemcpyDeviceToHost, s_));
-        NEON_CHECK_HIP(hipStreamSynchronize(s_));
-        if (host_state_->done) break;
+    run_batches(st, launched, [&] {
         for (int k = 0; k < poll_every; ++k, ++launched)
         {
             krylov_state* cur = st + (launched & 1);
@@ -560,10 +664,13 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP32) : -1;
             A.apply32(s_, p32, Ap32, part_pAp, cur);
             if (prof) prof->end(s_, tok);
-            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, part_rr, part_rz);
             // the reliable-update / true-residual convergence test runs every refine_every
-            // iterations; in between, update_p takes r.r and r.z from the x/r update
+            // iterations; in between, one fused launch does the x/r and p updates
             int const decided = (launched + 1) % refine_every == 0 ? 1 : 0;
+            if (!decided && fused_update<float, true>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, nxt, part_rr,
+                                                      part_rz, NEON_STOP_RELATIVE, tol, max_it))
+                continue;
+            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, part_rr, part_rz);
             if (decided)
             {
                 mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
@@ -574,12 +681,12 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, nb,
                                                          decided, cur, nxt, max_it);
         }
-        NEON_CHECK_HIP(hipGetLastError());
-    }
+    });
     // fold whatever the recurrence holds since the last reliable update
-    mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, st, 1);
+    mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, st + (launched & 1), 1);
     NEON_CHECK_HIP(hipGetLastError());
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    if (fused_updates) check_barrier();
     solve_result res = finish(*host_state_, NEON_STOP_RELATIVE);
     res.refinements = host_state_->restarts;
     int64_t const inner = res.iterations + (host_state_->done == KRYLOV_CONVERGED ? 1 : 0);
@@ -763,10 +870,12 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
 {
     if (n_nodes <= 0 || n_elem <= 0 || !X || !conn || !params)
         throw status_error(NEON_BAD_ARGUMENT, "empty mesh or null pointer");
-    if ((model & ~(0xff | NEON_SPHERE_BO33)) != 0 || (model & 0xff) > NEON_MODEL_MICROSPHERE_AGEING)
+    if ((model & ~(0xff | NEON_SPHERE_BO33)) != 0 || (model & 0xff) > NEON_MODEL_J2_DAMAGE)
         throw status_error(NEON_BAD_ARGUMENT, "unknown model");
     if ((model & 0xff) == NEON_MODEL_MICROSPHERE_AGEING && tangent_mode == NEON_TANGENT_CONSISTENT)
         throw status_error(NEON_BAD_ARGUMENT, "the ageing microsphere model has neon's tangent only");
+    if (model == NEON_MODEL_J2_DAMAGE && tangent_mode == NEON_TANGENT_CONSISTENT)
+        throw status_error(NEON_BAD_ARGUMENT, "the J2 damage model has neon's tangent only");
     if ((model & NEON_SPHERE_BO33) && !is_microsphere(model))
         throw status_error(NEON_BAD_ARGUMENT, "a unit-sphere rule is only for the microsphere models");
     if (tangent_mode != NEON_TANGENT_REFERENCE && tangent_mode != NEON_TANGENT_CONSISTENT)
@@ -956,18 +1065,40 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
     NEON_CHECK_HIP(hipMemcpyAsync(slots_.data(), slots.data(), slots.size(), hipMemcpyHostToDevice, s_));
 
     int64_t const Q = 8ll * E_;
-    int64_t const ncoef = is_microsphere(model_) ? kFullCoefs : 9;
+    int64_t const ncoef = model_ == NEON_MODEL_J2_DAMAGE ? kGeneralCoefs : is_microsphere(model_) ? kFullCoefs : 9;
     sigma_.resize(6 * Q);
     coef_.resize(ncoef * Q);
     detF_.resize(Q);
     NEON_CHECK_HIP(hipMemsetAsync(sigma_.data(), 0, 6 * Q * sizeof(double), s_));
     NEON_CHECK_HIP(hipMemsetAsync(coef_.data(), 0, ncoef * Q * sizeof(double), s_));
-    if (model_ == NEON_MODEL_J2 || model_ == NEON_MODEL_LINEAR_ELASTIC)
+    bool const j2_family = model_ == NEON_MODEL_J2 || model_ == NEON_MODEL_J2_DAMAGE;
+    if (j2_family || model_ == NEON_MODEL_LINEAR_ELASTIC)
     {
         vm_.resize(Q);
         NEON_CHECK_HIP(hipMemsetAsync(vm_.data(), 0, Q * sizeof(double), s_));
     }
-    if (model_ == NEON_MODEL_J2)
+    if (model_ == NEON_MODEL_J2_DAMAGE)
+    {
+        // back stress 6, damage 1, energy release rate 1 per QP and the committed back stress
+        // and damage; the tangent starts at C_e (isotropic_linear_elasticity.cpp:22-23)
+        dmg_state_.resize(8 * Q);
+        dmg_state_old_.resize(7 * Q);
+        NEON_CHECK_HIP(hipMemsetAsync(dmg_state_.data(), 0, 8 * Q * sizeof(double), s_));
+        NEON_CHECK_HIP(hipMemsetAsync(dmg_state_old_.data(), 0, 7 * Q * sizeof(double), s_));
+        std::vector<double> c(36 * size_t(Q));
+        for (int64_t q = 0; q < Q; ++q)
+        {
+            double const* p = params + 4 * (q / 8);
+            double const lam = p[0] - 2.0 * p[1] / 3.0;
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j)
+                    c[size_t(i * 6 + j) * Q + q] = i < 3 && j < 3 ? lam + (i == j ? 2.0 * p[1] : 0.0)
+                                                                  : (i == j ? p[1] : 0.0);
+        }
+        NEON_CHECK_HIP(hipMemcpyAsync(coef_.data(), c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+        NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    }
+    if (j2_family)
     {
         eps_p_.resize(6 * Q);
         ebar_.resize(Q);
@@ -1034,6 +1165,8 @@ void device_mesh::latin_force(const double* u_dev, const double* u_old_dev, doub
     int64_t const Q = 8ll * E_;
     if (sigma_old_.size() != size_t(6 * Q)) throw status_error(NEON_BAD_ARGUMENT, "latin_reset was not called");
     NEON_CHECK_HIP(hipMemsetAsync(f_dev, 0, 3ull * N_ * sizeof(double), s_));
+    if (model_ == NEON_MODEL_J2_DAMAGE)
+        throw status_error(NEON_BAD_ARGUMENT, "LATIN iterations are not available for the J2 damage model");
     bool const full = is_microsphere(model_);
     for (int c = 0; c < colours(); ++c)
     {
@@ -1074,6 +1207,40 @@ void device_mesh::download_ageing_state(double* state)
         for (int k = 0; k < 11; ++k) state[11 * q + k] = a[k * Q + q];
 }
 
+void device_mesh::set_damage(const double* c)
+{
+    if (model_ != NEON_MODEL_J2_DAMAGE) throw status_error(NEON_BAD_ARGUMENT, "the mesh's model is not J2 with damage");
+    for (int k = 0; k < 6; ++k)
+        if (!(c[k] > 0.0)) throw status_error(NEON_BAD_ARGUMENT, "damage constants must be positive");
+    std::copy(c, c + 6, damage_);
+}
+
+void device_mesh::download_damage_state(double* back_stress, double* damage, double* energy)
+{
+    if (model_ != NEON_MODEL_J2_DAMAGE) throw status_error(NEON_BAD_ARGUMENT, "the mesh's model is not J2 with damage");
+    int64_t const Q = 8ll * E_;
+    std::vector<double> a(8 * size_t(Q));
+    NEON_CHECK_HIP(hipMemcpyAsync(a.data(), dmg_state_.data(), a.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    for (int64_t q = 0; q < Q; ++q)
+    {
+        if (back_stress)
+        {
+            // tensor components (11, 22, 33, 23, 13, 12) -> row-major 3 x 3
+            double const* v = a.data();
+            double* o = back_stress + 9 * q;
+            o[0] = v[q];
+            o[4] = v[Q + q];
+            o[8] = v[2 * Q + q];
+            o[5] = o[7] = v[3 * Q + q];
+            o[2] = o[6] = v[4 * Q + q];
+            o[1] = o[3] = v[5 * Q + q];
+        }
+        if (damage) damage[q] = a[6 * Q + q];
+        if (energy) energy[q] = a[7 * Q + q];
+    }
+}
+
 int device_mesh::update_internal_variables(const double* u_dev, double dt)
 {
     int64_t const Q = 8ll * E_;
@@ -1091,6 +1258,13 @@ int device_mesh::update_internal_variables(const double* u_dev, double dt)
                                                           rule, ag, status_.data())
     switch (model_ & 0xff)
     {
+    case NEON_MODEL_J2_DAMAGE:
+        damage_update_kernel<<<grid_for(Q), kBlock, 0, s_>>>(
+            E_, conn_.data(), X_.data(), u_dev, prm, st,
+            damage_state{dmg_state_.data(), dmg_state_.data() + 6 * Q, dmg_state_.data() + 7 * Q},
+            damage_constants{damage_[0], damage_[1], damage_[2], damage_[3], damage_[4], damage_[5]}, dt,
+            status_.data());
+        break;
     case NEON_MODEL_NEOHOOKE: NEON_UPDATE(NEON_MODEL_NEOHOOKE); break;
     case NEON_MODEL_J2: NEON_UPDATE(NEON_MODEL_J2); break;
     case NEON_MODEL_LINEAR_ELASTIC: NEON_UPDATE(NEON_MODEL_LINEAR_ELASTIC); break;
@@ -1126,7 +1300,7 @@ void device_mesh::save_internal_variables(bool converged)
         NEON_CHECK_HIP(hipGetLastError());
         return;
     }
-    if (model_ != NEON_MODEL_J2) return;
+    if (model_ != NEON_MODEL_J2 && model_ != NEON_MODEL_J2_DAMAGE) return;
     if (converged)
     {
         history_copy_kernel<<<grid_for(6 * Q), kBlock, 0, s_>>>(6 * Q, eps_p_.data(), eps_p_old_.data());
@@ -1137,6 +1311,14 @@ void device_mesh::save_internal_variables(bool converged)
         history_copy_kernel<<<grid_for(6 * Q), kBlock, 0, s_>>>(6 * Q, eps_p_old_.data(), eps_p_.data());
         history_copy_kernel<<<grid_for(Q), kBlock, 0, s_>>>(Q, ebar_old_.data(), ebar_.data());
     }
+    if (model_ == NEON_MODEL_J2_DAMAGE)
+    {
+        // back stress and damage (the energy release rate is recomputed before every use)
+        if (converged)
+            history_copy_kernel<<<grid_for(7 * Q), kBlock, 0, s_>>>(7 * Q, dmg_state_.data(), dmg_state_old_.data());
+        else
+            history_copy_kernel<<<grid_for(7 * Q), kBlock, 0, s_>>>(7 * Q, dmg_state_old_.data(), dmg_state_.data());
+    }
     NEON_CHECK_HIP(hipGetLastError());
 }
 
@@ -1146,6 +1328,7 @@ void device_mesh::assemble(const double* u_dev, int precision)
     qp_state st{sigma_.data(), coef_.data(), detF_.data(), eps_p_.data(), ebar_.data(), vm_.data(), Q,
                 eps_p_old_.data(), ebar_old_.data(), age_.data(), age_old_.data()};
     bool const finite = is_finite_model(model_); // is_finite_deformation()
+    bool const general = model_ == NEON_MODEL_J2_DAMAGE;
     bool const full = is_microsphere(model_);
     // neon integrates the small-strain models on the current configuration (submesh.cpp:68,84);
     // the consistent mode integrates them on X
@@ -1178,7 +1361,16 @@ void device_mesh::assemble(const double* u_dev, int precision)
 #define NEON_ASSEMBLE(KT, FIN, FULL, K)                                                                       \
     assemble_kernel<KT, FIN, FULL><<<blocks, 64, 0, s_>>>(el, cnt, conn_.data(), X_.data(), u_dev, st, slots_.data(), \
                                                           bptr_.data(), K, fint_.data())
-        if (precision == NEON_PRECISION_FP64)
+        if (general)
+        {
+            if (precision == NEON_PRECISION_FP64)
+                assemble_kernel<double, false, true, true><<<blocks, 64, 0, s_>>>(
+                    el, cnt, conn_.data(), X_.data(), u_dev, st, slots_.data(), bptr_.data(), k64_.data(), fint_.data());
+            else
+                assemble_kernel<float, false, true, true><<<blocks, 64, 0, s_>>>(
+                    el, cnt, conn_.data(), X_.data(), u_dev, st, slots_.data(), bptr_.data(), k32_.data(), fint_.data());
+        }
+        else if (precision == NEON_PRECISION_FP64)
         {
             if (full)
                 NEON_ASSEMBLE(double, true, true, k64_.data());
@@ -1333,14 +1525,15 @@ void device_mesh::download_qp_state(double* sigma, double* D, double* detF, doub
         if (D) d36.resize(36 * Q);
         if (sigma) s9.resize(9 * Q);
         expand_tangent_kernel<<<grid_for(Q), kBlock, 0, s_>>>(Q, coef_.data(), sigma_.data(), D ? d36.data() : nullptr,
-                                                               sigma ? s9.data() : nullptr, is_microsphere(model_));
+                                                               sigma ? s9.data() : nullptr,
+                                                               model_ == NEON_MODEL_J2_DAMAGE ? 2 : is_microsphere(model_) ? 1 : 0);
         NEON_CHECK_HIP(hipGetLastError());
         if (D) NEON_CHECK_HIP(hipMemcpyAsync(D, d36.data(), 36 * Q * sizeof(double), hipMemcpyDeviceToHost, s_));
         if (sigma) NEON_CHECK_HIP(hipMemcpyAsync(sigma, s9.data(), 9 * Q * sizeof(double), hipMemcpyDeviceToHost, s_));
         NEON_CHECK_HIP(hipStreamSynchronize(s_));
     }
     if (detF) NEON_CHECK_HIP(hipMemcpyAsync(detF, detF_.data(), Q * sizeof(double), hipMemcpyDeviceToHost, s_));
-    if (model_ == NEON_MODEL_J2)
+    if (model_ == NEON_MODEL_J2 || model_ == NEON_MODEL_J2_DAMAGE)
     {
         if (eps_p)
         {

@@ -178,8 +178,20 @@ public:
     // the mixed CG tests the trigger (and, through the true residual, convergence) every this
     // many iterations
     int refine_every = NEON_REFINE_EVERY;
+    // both vector phases of a CG iteration in one cooperative launch (cg_fused_update_kernel)
+    // when the grid it needs is co-resident; otherwise the two-kernel form (environment
+    // NEON_FUSED_UPDATES=0 at engine creation selects the two-kernel form everywhere)
+    bool fused_updates = true;
+    // entries per thread of the fused update (0: the smallest of 8, 16, 32 whose grid fits)
+    int fused_ept = 0;
 
 private:
+    // launches cg_fused_update_kernel on n entries; false (nothing launched) when no
+    // entries-per-thread choice gives a co-resident grid
+    template <typename T, bool MIXED>
+    bool fused_update(int64_t n, T* x, T* r, T* p, const T* Ap, const T* minv, const double* part_pAp, int npart,
+                      const krylov_state* st, krylov_state* st_next, double* part_rr, double* part_rz, int criterion,
+                      double tol, int max_it);
     void ensure(int64_t n);
     void ensure_mixed(int64_t n);
     solve_result cg_mixed_run(const linear_operator& A, const double* b, double* x, double tol, int32_t max_it, int nb,
@@ -188,7 +200,15 @@ private:
     hipStream_t s_;
     device_array<double> part_[5];
     device_array<krylov_state> state_;
+    // grid barrier word of the fused update and its time-out flag (zeroed at creation)
+    device_array<uint32_t> barrier_;
+    void check_barrier();
     krylov_state* host_state_ = nullptr;
+    // poll-ahead ring of run_batches: two pinned state snapshots and their events
+    krylov_state* host_ring_ = nullptr;
+    hipEvent_t ring_events_[2] = {nullptr, nullptr};
+    template <typename Batch>
+    void run_batches(krylov_state* st, int64_t& launched, Batch&& batch);
     double* host_scalar_ = nullptr;
     device_array<double> scalar_;
     int64_t n_ = 0, n_mixed_ = 0;
@@ -243,6 +263,10 @@ public:
     void set_ageing(double scission, double recombination);
     // ageing microsphere: network state, 11 per QP in QP order (host)
     void download_ageing_state(double* state);
+    // J2 damage: the material constants; back stress (9 per QP, row-major), damage and energy
+    // release rate (host, any may be null)
+    void set_damage(const double* constants);
+    void download_damage_state(double* back_stress, double* damage, double* energy);
     void save_internal_variables(bool converged);
     void assemble(const double* u_dev, int precision);
     void enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision);
@@ -327,6 +351,10 @@ private:
     device_array<double> age_, age_old_; // ageing microsphere: 11 x Q network state (SoA)
     device_array<double> sigma_old_;     // LATIN: committed stress (6 x Q)
     double scission_ = 0.0, recombination_ = 0.0, last_dt_ = 0.0;
+    // J2 damage: back stress 6, damage 1, energy release rate 1 per QP (SoA) and the committed
+    // back stress and damage; the material's constants (C, gamma, s_p, n_p, s_d, n_d)
+    device_array<double> dmg_state_, dmg_state_old_;
+    double damage_[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
     device_array<uint32_t> status_;
     std::vector<std::unique_ptr<device_array<double>>> traj_u_, traj_ep_, traj_eb_;
     device_array<double> mu_ep_, mu_eb_;

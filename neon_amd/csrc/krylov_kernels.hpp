@@ -13,6 +13,8 @@
 #include "neon_hip.h"
 
 #include <cfloat>
+#include <type_traits>
+#include <utility>
 
 namespace neon_amd
 {
@@ -1122,6 +1124,218 @@ __global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* 
         s.beta = beta;
         s.it = it_new;
         s.done = done;
+        *st_next = s;
+    }
+}
+
+// --------------------------------------------- fused CG vector update (one launch)
+// Both vector phases of a CG iteration in one cooperative launch, with a grid barrier between
+// them for r.z:
+//   phase 1: alpha = rz / p.Ap;  x += alpha p;  r -= alpha Ap;  z = M^-1 r;  partials of r.r, r.z
+//   phase 2: convergence decision and beta = rz_new / rz;  p = z + beta p.
+// Thread g of the grid owns the 16-byte vectors g, g + T, g + 2T, ... (T threads in all) in
+// both phases and keeps z (and, with KEEP_P, the old p) of its vectors in registers across
+// the barrier, so phase 2 writes p without reading minv and r again (and, with KEEP_P, without
+// reading p): per iteration 8 (KEEP_P) or 9 vector passes instead of the two kernels' 11.
+// The grid is at most the number of co-resident workgroups (cooperative launch), and
+// EPT entries per thread cover n.  MIXED: the FP32 recurrence of the mixed-precision CG on an
+// iteration without a reliable-update test (no convergence decision here, as in
+// mixed_update_p_kernel with decided = 0); otherwise the FP64 CG with `criterion`.
+template <typename T>
+struct vec16_of;
+template <>
+struct vec16_of<double>
+{
+    typedef double2v type;
+    static constexpr int width = 2;
+};
+template <>
+struct vec16_of<float>
+{
+    typedef float4v type;
+    static constexpr int width = 4;
+};
+
+// Waves per SIMD the fused update is compiled for (register budget 512 / waves): with EPT
+// 8 / 16 / 32 entries per thread, 8 / 7 / 4 waves, so that the grid of n / (256 EPT)
+// workgroups fits co-resident on 256 compute units up to 8.4 M entries (EPT 32).
+template <int EPT>
+constexpr int fused_waves()
+{
+    return EPT <= 8 ? 8 : EPT <= 16 ? 7 : 4;
+}
+
+// Grid barrier of a cooperative launch (every workgroup co-resident).  bar[0] counts arrivals:
+// workgroup 0 adds 2^31 - (nblocks - 1), the others 1, so the word's top bit flips exactly
+// when the last workgroup arrives and its low bits return to their value (the word needs no
+// reset between barriers).  Each workgroup's writes before the barrier are released by an
+// agent-scope fence and acquired after it.  The wait is bounded (a poll is a coherent load of
+// about a microsecond; a barrier normally completes within tens of microseconds): after
+// kBarrierSpins polls the workgroup sets bar[1] and goes on; the fused update then ends the
+// solve as non-finite and the host reports the time-out.
+constexpr uint32_t kBarrierSpins = 1u << 17;
+__device__ inline void grid_barrier(uint32_t* bar)
+{
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t const add = blockIdx.x == 0 ? 0x80000000u - (gridDim.x - 1u) : 1u;
+        __threadfence();
+        uint32_t const old = __hip_atomic_fetch_add(bar, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0;
+        while (((old ^ __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 0x80000000u) == 0u)
+        {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins == kBarrierSpins)
+            {
+                __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, K - 1>{}): a loop
+// whose index is a constant expression in every copy, so arrays indexed by it stay in registers
+template <typename F, int... I>
+__device__ inline void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int K, typename F>
+__device__ inline void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, K>{});
+}
+
+template <typename T, int EPT, bool KEEP_P, bool MIXED>
+__global__ __launch_bounds__(kBlock, fused_waves<EPT>()) void cg_fused_update_kernel(
+    int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ Ap,
+    const T* __restrict__ minv, const double* __restrict__ part_pAp, int npart, const krylov_state* __restrict__ st,
+    krylov_state* __restrict__ st_next, double* __restrict__ part_rr, double* __restrict__ part_rz, int criterion,
+    double tol, int max_it, uint32_t* __restrict__ bar)
+{
+    typedef typename vec16_of<T>::type V;
+    constexpr int VW = vec16_of<T>::width;
+    constexpr int K = EPT / VW;
+    static_assert(EPT % VW == 0, "entries per thread: whole 16-byte vectors");
+    krylov_state s = *st;
+    if (s.done)
+    {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
+        return;
+    }
+    double const pAp = reduce_partials<kBlock>(part_pAp, npart);
+    T const alpha = T(s.rz / pAp);
+    int64_t const total = int64_t(gridDim.x) * kBlock;
+    int64_t const g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    int64_t const full = n / VW; // whole vectors; the last thread of the grid takes the rest
+    bool const tail = g == total - 1 && full * VW < n;
+    V zk[K], pk[KEEP_P ? K : 1];
+    double rr = 0.0, rz = 0.0;
+    static_for<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        int64_t const v = g + k * total;
+        if (v < full)
+        {
+            V xv = load_stream<kVectorNT>(reinterpret_cast<const V*>(x) + v);
+            V rv = reinterpret_cast<const V*>(r)[v];
+            V const pv = reinterpret_cast<const V*>(p)[v];
+            V const av = load_stream<kVectorNT>(reinterpret_cast<const V*>(Ap) + v);
+            V const mv = reinterpret_cast<const V*>(minv)[v];
+            xv += alpha * pv;
+            rv = rv - alpha * av;
+            V const zv = mv * rv;
+            static_for<VW>([&](auto jc) {
+                constexpr int jj = decltype(jc)::value;
+                rr += double(rv[jj]) * double(rv[jj]);
+                rz += double(rv[jj]) * double(zv[jj]);
+            });
+            zk[k] = zv;
+            if constexpr (KEEP_P) pk[k] = pv;
+            store_stream<kVectorNT>(reinterpret_cast<V*>(x) + v, xv);
+            reinterpret_cast<V*>(r)[v] = rv;
+        }
+        // one vector group's loads at a time: hoisting every group's loads ahead (the
+        // scheduler's choice) needs more registers than the co-residency budget and spills
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    if (tail)
+        for (int64_t i = full * VW; i < n; ++i)
+        {
+            x[i] += alpha * p[i];
+            T const ri = r[i] - alpha * Ap[i];
+            r[i] = ri;
+            T const zi = minv[i] * ri;
+            rr += double(ri) * double(ri);
+            rz += double(ri) * double(zi);
+        }
+    rr = block_sum<kBlock>(rr);
+    rz = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0)
+    {
+        part_rr[blockIdx.x] = rr;
+        part_rz[blockIdx.x] = rz;
+    }
+    grid_barrier(bar);
+    rr = reduce_partials<kBlock>(part_rr, gridDim.x);
+    rz = reduce_partials<kBlock>(part_rz, gridDim.x);
+    int32_t it_new;
+    int32_t done = KRYLOV_RUNNING;
+    if (MIXED)
+    {
+        it_new = s.it + 1;
+        if (it_new >= max_it) done = KRYLOV_LIMIT;
+    }
+    else if (criterion == NEON_STOP_RELATIVE)
+    {
+        bool const conv = rr < s.threshold; // Eigen breaks before counting the iteration
+        it_new = conv ? s.it : s.it + 1;
+        if (conv)
+            done = KRYLOV_CONVERGED;
+        else if (it_new >= max_it)
+            done = KRYLOV_LIMIT;
+    }
+    else
+    {
+        it_new = s.it + 1; // the CUDA loop counts at the top
+        if (it_new >= max_it)
+            done = KRYLOV_LIMIT;
+        else if (!(sqrt(rz) > tol))
+            done = KRYLOV_CONVERGED;
+    }
+    if (!isfinite(rr) || !isfinite(rz)) done = KRYLOV_NONFINITE;
+    if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) done = KRYLOV_NONFINITE;
+    double const beta64 = rz / s.rz;
+    T const beta = T(beta64);
+    if (done == KRYLOV_RUNNING)
+    {
+        static_for<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            int64_t const v = g + k * total;
+            if (v < full)
+            {
+                V pv;
+                if constexpr (KEEP_P)
+                    pv = pk[k];
+                else
+                    pv = reinterpret_cast<const V*>(p)[v];
+                reinterpret_cast<V*>(p)[v] = zk[k] + beta * pv;
+            }
+        });
+        if (tail)
+            for (int64_t i = full * VW; i < n; ++i) p[i] = minv[i] * r[i] + beta * p[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        s.rr = rr;
+        s.rz = rz;
+        s.beta = beta64;
+        s.it = it_new;
+        s.done = done;
+        if (MIXED) s.refine = 0;
         *st_next = s;
     }
 }

@@ -7,6 +7,9 @@ the CPU oracle (which would take minutes to hours there).
   small against the reactions on the constrained ones, and its total over all dofs vanishes
   (the shape functions sum to one, so every element's forces balance; a check of assembly
   at any state).
+* Mixed-precision block (128^3, 6.44 M dofs): load step 1 to the 1e-8 relative residual in
+  FP64 and mixed precision; the same Newton count, both residuals at the target, and the
+  mixed displacement against the FP64 one.
 * Adjoint block (96^3, smoothed log-normal mu, 4096 observed top nodes): the adjoint gradient
   in log mu against FP64 central differences (step 1e-6) along seeded random directions.
 * J2 notched plate (128 x 64 x 64 cells less the hole of radius 8h, 20 load steps to 0.5 %
@@ -18,9 +21,10 @@ the CPU oracle (which would take minutes to hours there).
   falls with h (on the half-size plate 1.0e-6, 3.3e-7, 6.0e-8, 5.7e-9 at h = 1e-6, 3e-7,
   1e-7, 3e-8, with no roundoff floor yet), and the checks use h = 1e-7.
 
-Run as `python tests/full_size_checks.py [fp64_block adjoint_block j2_plate]` for a JSON line
-per check; `tests/test_gpu_full_size.py` runs the first two at full size and the plate at
-half size.
+Run as `python tests/full_size_checks.py [fp64_block mixed_block_step1 adjoint_block j2_plate
+adjoint_block32]` for a JSON line per check (adjoint_block32: the 32 directions of the
+adjoint workload); `tests/test_gpu_full_size.py` runs each at its workload's size (the adjoint
+block along 8 directions).
 """
 from __future__ import annotations
 
@@ -83,6 +87,34 @@ def fp64_block(ctx, n=64):
     res["top_reaction"] = float(f[top].sum())
     res["free_residual"] = float(np.linalg.norm(f[~fixed]) / np.linalg.norm(f[fixed]))
     res["force_balance"] = float(abs(f[2::3].sum()) / abs(res["top_reaction"]))
+    return res
+
+
+def mixed_block_step1(ctx, n=128):
+    """The mixed-precision workload (128^3 roller block, 6.44 M dofs): load step 1 to the 1e-8
+    relative Newton residual in FP64 and in mixed precision on one mesh; Newton counts, force
+    norms and the mixed path's displacement error against the FP64 path."""
+    p = problems.block_rollers(n, residual_tolerance=1e-8, displacement_tolerance=1e-8, norm_mode=1,
+                               solver_tolerance=1e-10, max_iterations=200000)
+    mesh = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    res = {"check": "mixed_block_step1", "n": n, "dofs": p.n_dofs}
+    sols = {}
+    for prec in ("fp64", "mixed"):
+        sim = dict(p.simulation, max_load_steps=1)
+        sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": prec})
+        sm = neon_amd.static_matrix(mesh, sim, p.boundaries)
+        t = time.time()
+        sols[prec] = sm.solve()
+        res[f"{prec}_seconds"] = time.time() - t
+        res[f"{prec}_newton_iterations"] = len(sm.records)
+        res[f"{prec}_force_norms"] = [r["force_norm"] for r in sm.records]
+        res[f"{prec}_displacement_norms"] = [r["displacement_norm"] for r in sm.records]
+        res[f"{prec}_cg_iterations"] = [r["solver_iterations"] for r in sm.records]
+        res[f"{prec}_status_ok"] = all(r["status"] == 0 for r in sm.records)
+        _progress(f"mixed_block_step1 {n}: {prec} {res[f'{prec}_seconds']:.1f} s, {len(sm.records)} Newton iterations")
+    u64, u32 = sols["fp64"], sols["mixed"]
+    res["mixed_vs_fp64_max"] = float(np.abs(u32 - u64).max() / np.abs(u64).max())
+    res["mixed_vs_fp64_l2"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
     return res
 
 
@@ -157,10 +189,14 @@ def j2_plate(ctx, nx=128, steps=20, rel_steps=(1e-7,)):
 
 
 def main(argv):
-    names = argv or ["fp64_block", "adjoint_block", "j2_plate"]
+    names = argv or ["fp64_block", "mixed_block_step1", "adjoint_block", "j2_plate"]
     ctx = neon_amd.Context(0)
     for name in names:
-        r = {"fp64_block": fp64_block, "adjoint_block": adjoint_block, "j2_plate": j2_plate}[name](ctx)
+        if name == "adjoint_block32":
+            r = adjoint_block(ctx, 96, directions=32)
+        else:
+            r = {"fp64_block": fp64_block, "mixed_block_step1": mixed_block_step1, "adjoint_block": adjoint_block,
+                 "j2_plate": j2_plate}[name](ctx)
         print(json.dumps(r), flush=True)
 
 

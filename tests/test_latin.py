@@ -105,5 +105,9 @@ def test_ingest_latin_step(tmp_path):
         ingest.read(_latin_json(tmp_path / "c", dict(base, latin_scheme="incremental", latin_local_stage="implicit")))
     sim = copy.deepcopy(p.simulation)
     del sim["nonlinear_options"]["latin_search_direction"]
+    class _host_only_mesh:  # latin_matrix checks its options after static_matrix's constructor
+        def is_symmetric(self):
+            return True
+
     with pytest.raises(ValueError, match="latin_search_direction not specified"):
-        neon_amd.latin_matrix(None, sim, p.boundaries)
+        neon_amd.latin_matrix(_host_only_mesh(), sim, p.boundaries)

@@ -336,7 +336,9 @@ def test_model_ids_are_checked():
     with pytest.raises(RuntimeError):
         orc.Model(X, conn, [[1.0, 1.0, 0.0, 0.0]], model=orc.MODEL_J2 | orc.SPHERE_BO33)
     with pytest.raises(RuntimeError):
-        orc.Model(X, conn, [[1.0, 1.0, 0.0, 0.0]], model=7)
+        orc.Model(X, conn, [[1.0, 1.0, 0.0, 0.0]], model=8)
+    with pytest.raises(RuntimeError):  # J2 with damage has no consistent mode
+        orc.Model(X, conn, [[1.0, 1.0, 1.0, 0.0]], model=orc.MODEL_J2_DAMAGE, tangent=orc.TANGENT_CONSISTENT)
     with pytest.raises(RuntimeError):  # the ageing model has no consistent mode
         orc.Model(X, conn, [[1.0, 1.0, 5.0, 0.0]], model=orc.MODEL_MICROSPHERE_AGEING, tangent=orc.TANGENT_CONSISTENT)
 
