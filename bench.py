@@ -49,12 +49,24 @@ def measured_traffic(n):
     return None
 
 
-def spmv_bytes(nnz, nnzb, nodes, n, value_bytes, index_bytes=4):
-    """Algorithmic bytes of one node-block SpMV: values once, one block column per 9 values
-    (`index_bytes`: 2 when the solvers' sliced copy holds 16-bit offsets from the row node, 4
-    for int32 columns), x read once, y written once (row pointers and the sliced copy's
-    padding, which depend on the storage format, are not counted)."""
-    return value_bytes * nnz + index_bytes * nnzb + 2 * value_bytes * n
+def spmv_bytes(nnz, n, value_bytes):
+    """Algorithmic bytes of one node-block SpMV: the nonzero values once, one 16-bit column
+    offset per 3x3 block (nnz / 9 of them), x read once, y written once.  The sliced copy the
+    solvers read stores most column offsets once per step of 64 nodes instead of once per
+    block, and pads short nodes with zero values; neither is counted (`sliced_bytes` gives
+    what the copy holds)."""
+    return value_bytes * nnz + 2 * (nnz // 9) + 2 * value_bytes * n
+
+
+def sliced_bytes(n, value_bytes, sliced):
+    """Bytes one product on the sliced copy reads and writes (`sliced` = ``DeviceMesh.sliced_info()``):
+    the step records (9 x 64 values per step, padding included), a code and an exception word
+    per step, 64 per-lane offsets (2 or 4 bytes) per step without a shared offset, the
+    lane-to-node map, the exception lanes and step pointer of each slice, x once and y once."""
+    ib = 2 if sliced["index16"] else 4
+    own = sliced["steps"] - sliced["shared_steps"]
+    meta = 8 * sliced["steps"] + 64 * ib * own + (4 * 64 + 8) * sliced["slices"] + 4
+    return value_bytes * 576 * sliced["steps"] + meta + 2 * value_bytes * n
 
 
 def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
@@ -202,9 +214,7 @@ def main():
     ctx = neon_amd.Context(0)
     mesh = neon_amd.DeviceMesh(p.X, p.conn, p.params, model=p.model, context=ctx)
     nnz = mesh.nnz()
-    nnzb = nnz // 9
-    # the column bytes the solvers' SpMV needs: 16-bit offsets when they fit
-    ib = 2 if mesh.sliced_info()["index16"] else 4
+    sliced = mesh.sliced_info()
     log(f"setup {time.perf_counter() - t0:.1f} s: {p.n_elem} elements, {p.n_dofs} dofs, nnz {nnz}")
 
     results = {}
@@ -215,7 +225,7 @@ def main():
     r = results[head]
     value = args.steps / r["seconds"]
     vb = 4 if head == "mixed" else 8
-    bytes_spmv = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vb, ib)
+    bytes_spmv = spmv_bytes(nnz, p.n_dofs, vb)
     spmv_avg_ms = r["spmv_ms"] / max(r["spmv_count"], 1)
     achieved = bytes_spmv / (spmv_avg_ms * 1e-3) / 1e9
     line = {
@@ -240,11 +250,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.n) if head == "mixed" else None,
                      "kernel": f"spmv_sell_kernel<{'float' if vb == 4 else 'double'}> (sliced node-block SpMV + p.Ap)",
-                     "algorithmic_bytes": bytes_spmv, "avg_ms": spmv_avg_ms, "launches": r["spmv_count"]},
+                     "algorithmic_bytes": bytes_spmv, "avg_ms": spmv_avg_ms, "launches": r["spmv_count"],
+                     "sliced_layout": dict(sliced, bytes_per_launch=sliced_bytes(p.n_dofs, vb, sliced))},
     }
     for prec, rr in results.items():
         vbp = 4 if prec == "mixed" else 8
-        bs = spmv_bytes(nnz, nnzb, p.n_nodes, p.n_dofs, vbp, ib)
+        bs = spmv_bytes(nnz, p.n_dofs, vbp)
         avg = rr["spmv_ms"] / max(rr["spmv_count"], 1)
         asm_avg = rr["asm_ms"] / max(rr["asm_count"], 1)
         its = [x["solver_iterations"] for x in rr["records"]]

@@ -205,42 +205,21 @@ int neon_mesh_enforce_dirichlet(neon_context* ctx, int32_t n_fixed, const int32_
 int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double tolerance, int32_t max_iterations,
                     const double* b, double* x, int32_t* iterations, double* error);
 
-/* Time reps SpMVs y = K x (x = 1) of the assembled tangent in the given precision with
- * `lanes` (16, 32) lanes per block row, the variant `flags` below and `blocks` workgroups
- * (0: default); average milliseconds per SpMV.  No counterpart in the reference: a tuning and
- * roofline probe. */
-#define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads (CSR-order and sliced forms) */
-#define NEON_SPMV_SWEEP 2       /* workgroups of one XCD sweep its node range together */
-#define NEON_SPMV_PIPELINE 4    /* software-pipelined rows (32 lanes, sweep order) */
-#define NEON_SPMV_LDS 8         /* values staged through LDS in 16-byte loads (32 lanes, sweep order) */
-#define NEON_SPMV_STREAM 16     /* LDS-staged with the next rows' segments in flight (32 lanes, sweep order) */
-#define NEON_SPMV_PROBE_DOT 32   /* probe only: the solvers' form with block partials of x . y */
-#define NEON_SPMV_PROBE_DIRTY 64 /* probe only: each product follows a write of 4 vectors of n (timed apart) */
-#define NEON_SPMV_META 128       /* row pointers of 32 rows per load, shared by shuffles (32 lanes, sweep order) */
-#define NEON_SPMV_SLICED 256     /* sliced copy of the values: 64 nodes per wavefront, one per lane */
-#define NEON_SPMV_SLICED_WIDE 512 /* with NEON_SPMV_SLICED: 1024-thread workgroups instead of 256 */
-#define NEON_SPMV_INDEX32 1024   /* with NEON_SPMV_SLICED: 32-bit columns even where 16-bit offsets from the
-                                    row node fit (the default uses the 16-bit form when every offset fits) */
-int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
-                         int32_t reps, double* avg_ms);
-/* Shape of the SpMV that the solvers on this mesh's tangent use from now on (lanes, flags as
- * above, workgroups for FP64 and for FP32 values; 0: by size).  Default: NEON_SPMV_SLICED |
- * NEON_SPMV_NONTEMPORAL, workgroups by size (one pass, one slice per wavefront).  The pipelined form applies to
- * meshes whose nodes have at most 32 neighbours and falls back to the plain one otherwise.
- * No counterpart in the reference: a tuning knob; results agree to rounding. */
+/* Shape of the SpMV that the solvers on this mesh's tangent use from now on: with
+ * NEON_SPMV_SLICED the sliced copy of the values (64 nodes per wavefront, one per lane), else
+ * the CSR-order values with `lanes` (16, 32) lanes per block row; workgroups for FP64 and for
+ * FP32 values (0: by size).  Default: NEON_SPMV_SLICED | NEON_SPMV_NONTEMPORAL, workgroups by
+ * size (one slice per wavefront).  No counterpart in the reference: a tuning knob; every shape
+ * gives the same products up to the order of the FP32 row sums. */
+#define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads */
+#define NEON_SPMV_SLICED 256    /* the sliced copy of the values */
+#define NEON_SPMV_INDEX32 1024  /* with NEON_SPMV_SLICED: 32-bit per-lane column offsets even where
+                                   16 bits fit (the default uses 16 bits when every offset fits) */
 int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);
 /* The sliced copy's shape: slices (64 nodes each), steps (block columns per lane, padding
- * included) and whether its columns fit 16-bit offsets from the row node (1) or stay 32-bit
- * (0).  No counterpart in the reference. */
-int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int32_t* index16);
-/* Average milliseconds of one assembly (tangent + f_int, all colours) of the held state with
- * `blocks_per_cu` one-wavefront workgroups per compute unit and colour (default 128).  No
- * counterpart in the reference: a tuning probe. */
-int neon_mesh_assembly_bench(neon_context* ctx, int32_t precision, int32_t blocks_per_cu, int32_t reps,
-                             double* avg_ms);
-/* Average milliseconds of a streaming read and of a copy of `bytes` bytes (calibration of the
- * achievable HBM bandwidth and of the traffic counters). */
-int neon_stream_probe(neon_context* ctx, int64_t bytes, int32_t reps, double* read_ms, double* copy_ms);
+ * included), steps whose 64 lanes share one column offset (stored once), and whether the other
+ * steps' per-lane offsets are 16-bit (1) or 32-bit (0).  No counterpart in the reference. */
+int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int64_t* shared_steps, int32_t* index16);
 
 /* ---- Newton driver (static_matrix::solve) ---- */
 typedef struct {

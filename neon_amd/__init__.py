@@ -33,15 +33,9 @@ from ._capi import (  # noqa: F401
     SOLVER_CG,
     SPECTRUM_LOWER,
     SPECTRUM_UPPER,
-    SPMV_LDS,
-    SPMV_META,
     SPMV_SLICED,
-    SPMV_SLICED_WIDE,
     SPMV_INDEX32,
     SPMV_NONTEMPORAL,
-    SPMV_PIPELINE,
-    SPMV_STREAM,
-    SPMV_SWEEP,
     STOP_ABSOLUTE_PRECONDITIONED,
     STOP_RELATIVE,
     TANGENT_CONSISTENT,
@@ -394,9 +388,10 @@ class DeviceMesh:
     def sliced_info(self) -> dict:
         """Shape of the sliced tangent copy (``neon_mesh_sliced_info``)."""
         sl, idx = C.c_int32(0), C.c_int32(0)
-        st = C.c_int64(0)
-        self.ctx.check(_capi.lib().neon_mesh_sliced_info(self.ctx.handle, C.byref(sl), C.byref(st), C.byref(idx)))
-        return {"slices": sl.value, "steps": st.value, "index16": bool(idx.value)}
+        st, sh = C.c_int64(0), C.c_int64(0)
+        self.ctx.check(_capi.lib().neon_mesh_sliced_info(self.ctx.handle, C.byref(sl), C.byref(st), C.byref(sh),
+                                                         C.byref(idx)))
+        return {"slices": sl.value, "steps": st.value, "shared_steps": sh.value, "index16": bool(idx.value)}
 
     def displacement(self) -> np.ndarray:
         """The displacement held by the mesh (``neon_mesh_displacement``)."""

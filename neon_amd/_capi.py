@@ -48,16 +48,11 @@ KERNEL_SPMV_FP32 = 1
 KERNEL_RESIDUAL_FP64 = 2
 KERNEL_ASSEMBLY = 3
 SPMV_NONTEMPORAL = 1
-SPMV_SWEEP = 2
-SPMV_PIPELINE = 4
-SPMV_LDS = 8
-SPMV_STREAM = 16
-SPMV_PROBE_DOT = 32
-SPMV_PROBE_DIRTY = 64
-SPMV_META = 128
 SPMV_SLICED = 256
-SPMV_SLICED_WIDE = 512
 SPMV_INDEX32 = 1024
+# flags of the measurement library's neon_mesh_spmv_bench (include/neon_hip_measure.h)
+MEASURE_DOT = 4096
+MEASURE_DIRTY = 8192
 
 STATUS_NAMES = {
     0: "OK",
@@ -185,10 +180,7 @@ SIGNATURES = {
     "neon_mesh_download": (C.c_int, [C.c_void_p, f64p, f64p]),
     "neon_mesh_enforce_dirichlet": (C.c_int, [C.c_void_p, C.c_int32, i32p, f64p]),
     "neon_mesh_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int32, f64p, f64p, i32p, f64p]),
-    "neon_mesh_spmv_bench": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p]),
     "neon_mesh_spmv_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
-    "neon_mesh_assembly_bench": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, f64p]),
-    "neon_stream_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f64p, f64p]),
     "neon_newton_solve": (
         C.c_int,
         [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), f64p, P(NewtonRecord), C.c_int32, i32p,
@@ -203,7 +195,7 @@ SIGNATURES = {
         [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, i32p, i32p],
     ),
     "neon_mesh_displacement": (C.c_int, [C.c_void_p, f64p]),
-    "neon_mesh_sliced_info": (C.c_int, [C.c_void_p, i32p, P(C.c_int64), i32p]),
+    "neon_mesh_sliced_info": (C.c_int, [C.c_void_p, i32p, P(C.c_int64), P(C.c_int64), i32p]),
     "neon_newton_solve_loaded": (
         C.c_int,
         [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p, P(NewtonRecord),
@@ -225,7 +217,17 @@ SIGNATURES = {
     "neon_random_normal": (C.c_int, [C.c_uint64, C.c_int64, f64p]),
 }
 
+# the measurement library (include/neon_hip_measure.h): timing probes over this library's contexts
+MEASURE_PATH = pathlib.Path(__file__).resolve().parent / "libneon_hip_measure.so"
+MEASURE_SIGNATURES = {
+    "neon_mesh_spmv_bench": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p]),
+    "neon_mesh_spmv_apply": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p, f64p]),
+    "neon_mesh_assembly_bench": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, f64p]),
+    "neon_stream_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f64p, f64p]),
+}
+
 _lib = None
+_measure = None
 
 
 def lib() -> C.CDLL:
@@ -243,6 +245,22 @@ def lib() -> C.CDLL:
             fn.argtypes = args
         _lib = handle
     return _lib
+
+
+def measure_lib() -> C.CDLL:
+    """Load ``libneon_hip_measure.so`` (after ``libneon_hip.so``, whose contexts it takes)."""
+    global _measure
+    if _measure is None:
+        lib()
+        if not MEASURE_PATH.exists():
+            raise RuntimeError(f"{MEASURE_PATH} is missing: build the HIP libraries first")
+        handle = C.CDLL(str(MEASURE_PATH))
+        for name, (res, args) in MEASURE_SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _measure = handle
+    return _measure
 
 
 def ptr(a: np.ndarray | None, ctype):

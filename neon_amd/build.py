@@ -19,6 +19,9 @@ BUILD = HERE / "_build"
 LIB = HERE / "libneon_hip.so"
 
 SOURCES = ["engine.hip", "newton.hip", "eigen.hip", "capi.hip", "meshgen.cpp", "loads.cpp"]
+# the measurement library (include/neon_hip_measure.h): timing probes linked against LIB
+MEASURE_SOURCES = ["measure.hip"]
+MEASURE_LIB = HERE / "libneon_hip_measure.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
@@ -52,18 +55,27 @@ def _compile(src: str) -> pathlib.Path:
 
 
 def build(jobs: int = 4) -> pathlib.Path:
-    """Compile every source and link ``libneon_hip.so`` (always rebuilds)."""
+    """Compile every source and link ``libneon_hip.so`` and the measurement library
+    ``libneon_hip_measure.so`` (always rebuilds)."""
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     BUILD.mkdir(exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
-        objs = list(pool.map(_compile, SOURCES))
+        objs = list(pool.map(_compile, SOURCES + MEASURE_SOURCES))
+    product, measure = objs[: len(SOURCES)], objs[len(SOURCES):]
     tmp = LIB.with_suffix(".so.tmp")
     subprocess.run(
-        [HIPCC, "-shared", "-fopenmp", "--offload-arch=gfx950", *map(str, objs), "-o", str(tmp)],
+        [HIPCC, "-shared", "-fopenmp", "--offload-arch=gfx950", *map(str, product), "-o", str(tmp)],
         check=True,
     )
     os.replace(tmp, LIB)
+    tmp = MEASURE_LIB.with_suffix(".so.tmp")
+    subprocess.run(
+        [HIPCC, "-shared", "-fopenmp", "--offload-arch=gfx950", *map(str, measure), f"-L{HERE}", "-lneon_hip",
+         "-Wl,-rpath,$ORIGIN", "-o", str(tmp)],
+        check=True,
+    )
+    os.replace(tmp, MEASURE_LIB)
     return LIB
 
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <new>
+#include <string>
 #include <vector>
 
 using namespace neon_amd;
@@ -53,6 +54,7 @@ int lanes_for(double avg)
     while (l < 64 && l < avg) l <<= 1;
     return l;
 }
+
 } // namespace
 
 extern "C" {
@@ -436,12 +438,9 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        if ((lanes != 16 && lanes != 32)
-            || (flags & ~(31 | NEON_SPMV_META | NEON_SPMV_SLICED | NEON_SPMV_SLICED_WIDE | NEON_SPMV_INDEX32))
+        if ((lanes != 16 && lanes != 32) || (flags & ~(NEON_SPMV_NONTEMPORAL | NEON_SPMV_SLICED | NEON_SPMV_INDEX32))
             || blocks_fp64 < 0 || blocks_fp32 < 0)
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
-        if ((flags & (NEON_SPMV_PIPELINE | NEON_SPMV_LDS | NEON_SPMV_STREAM | NEON_SPMV_META)) && lanes != 32)
-            throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs use 32 lanes");
         spmv_config& c = ctx->mesh->spmv;
         c.lanes = lanes;
         c.flags = flags;
@@ -451,145 +450,14 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
     });
 }
 
-int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int32_t* index16)
+int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int64_t* shared_steps, int32_t* index16)
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
         if (slices) *slices = ctx->mesh->sliced_slices();
         if (steps) *steps = ctx->mesh->sliced_steps();
+        if (shared_steps) *shared_steps = ctx->mesh->sliced_shared_steps();
         if (index16) *index16 = ctx->mesh->sliced_index16() ? 1 : 0;
-        return NEON_OK;
-    });
-}
-
-int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, int32_t flags, int32_t blocks,
-                         int32_t reps, double* avg_ms)
-{
-    return guarded(ctx, [&] {
-        need_mesh(ctx);
-        device_mesh& m = *ctx->mesh;
-        if (lanes != 16 && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "lanes must be 16 or 32");
-        if (reps <= 0 || !avg_ms) throw status_error(NEON_BAD_ARGUMENT, "bad repetitions");
-        linear_operator A = m.op(precision);
-        A.nb_lanes = lanes;
-        A.nb_nt = (flags & NEON_SPMV_NONTEMPORAL) != 0;
-        A.nb_sweep = (flags & NEON_SPMV_SWEEP) != 0;
-        A.nb_pipeline = (flags & NEON_SPMV_PIPELINE) != 0;
-        A.nb_lds = (flags & NEON_SPMV_LDS) != 0;
-        A.nb_stream = (flags & NEON_SPMV_STREAM) != 0;
-        A.nb_meta = (flags & NEON_SPMV_META) != 0;
-        if (A.nb_meta && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "the row-pointer-chunk SpMV uses 32 lanes");
-        bool const special = A.nb_pipeline || A.nb_lds || A.nb_stream;
-        if (special && lanes != 32)
-            throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs use 32 lanes");
-        if (special && m.max_degree() > 32)
-            throw status_error(NEON_BAD_ARGUMENT, "the pipelined, LDS-staged and streaming SpMVs need rows of at most 32 blocks");
-        A.blocks_override = blocks;
-        A.sell = false;
-        if (flags & NEON_SPMV_SLICED)
-            m.attach_sliced(A, precision, (flags & NEON_SPMV_SLICED_WIDE) != 0, (flags & NEON_SPMV_INDEX32) != 0);
-        int64_t const n = m.dofs();
-        hipStream_t const s = ctx->stream;
-        hipEvent_t e0, e1;
-        NEON_CHECK_HIP(hipEventCreate(&e0));
-        NEON_CHECK_HIP(hipEventCreate(&e1));
-        float ms = 0.0f;
-        // NEON_SPMV_PROBE_DOT: the solvers' form with partials of x.y; NEON_SPMV_PROBE_DIRTY:
-        // each product follows a write of 4 vectors of n entries, as the Krylov updates do
-        device_array<double> part, dirty;
-        bool const dot = (flags & NEON_SPMV_PROBE_DOT) != 0;
-        bool const dirt = (flags & NEON_SPMV_PROBE_DIRTY) != 0;
-        if (dot) part.resize(kMaxPartials);
-        if (dirt) dirty.resize(4 * n);
-        if (precision == NEON_PRECISION_FP64)
-        {
-            device_array<double> x, y;
-            x.resize(n);
-            y.resize(n);
-            fill_ones(s, n, x.data());
-            A.apply64(s, x.data(), y.data(), part.data(), nullptr); // warm
-            NEON_CHECK_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r)
-            {
-                if (dirt) fill_ones(s, 4 * n, dirty.data());
-                A.apply64(s, x.data(), y.data(), part.data(), nullptr);
-            }
-            NEON_CHECK_HIP(hipEventRecord(e1, s));
-            NEON_CHECK_HIP(hipEventSynchronize(e1));
-        }
-        else
-        {
-            device_array<float> x, y;
-            x.resize(n);
-            y.resize(n);
-            fill_ones(s, n, x.data());
-            A.apply32(s, x.data(), y.data(), part.data(), nullptr);
-            NEON_CHECK_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r)
-            {
-                if (dirt) fill_ones(s, 4 * n, dirty.data());
-                A.apply32(s, x.data(), y.data(), part.data(), nullptr);
-            }
-            NEON_CHECK_HIP(hipEventRecord(e1, s));
-            NEON_CHECK_HIP(hipEventSynchronize(e1));
-        }
-        if (dirt)
-        {
-            // subtract the fills, timed alone
-            hipEvent_t f0, f1;
-            NEON_CHECK_HIP(hipEventCreate(&f0));
-            NEON_CHECK_HIP(hipEventCreate(&f1));
-            NEON_CHECK_HIP(hipEventRecord(f0, s));
-            for (int r = 0; r < reps; ++r) fill_ones(s, 4 * n, dirty.data());
-            NEON_CHECK_HIP(hipEventRecord(f1, s));
-            NEON_CHECK_HIP(hipEventSynchronize(f1));
-            float fm = 0.0f, tm = 0.0f;
-            NEON_CHECK_HIP(hipEventElapsedTime(&fm, f0, f1));
-            NEON_CHECK_HIP(hipEventElapsedTime(&tm, e0, e1));
-            (void)hipEventDestroy(f0);
-            (void)hipEventDestroy(f1);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            *avg_ms = double(tm - fm) / reps;
-            return NEON_OK;
-        }
-        NEON_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        *avg_ms = double(ms) / reps;
-        return NEON_OK;
-    });
-}
-
-int neon_mesh_assembly_bench(neon_context* ctx, int32_t precision, int32_t blocks_per_cu, int32_t reps,
-                             double* avg_ms)
-{
-    return guarded(ctx, [&] {
-        need_mesh(ctx);
-        if (blocks_per_cu <= 0 || blocks_per_cu > 1024 || reps <= 0 || !avg_ms)
-            throw status_error(NEON_BAD_ARGUMENT, "bad assembly probe arguments");
-        device_mesh& m = *ctx->mesh;
-        int const keep = m.assembly_blocks_per_cu;
-        m.assembly_blocks_per_cu = blocks_per_cu;
-        double total = 0.0;
-        m.assemble(m.u(), precision); // warm
-        for (int r = 0; r < reps; ++r)
-        {
-            m.assemble(m.u(), precision);
-            total += m.last_assemble_ms;
-        }
-        m.assembly_blocks_per_cu = keep;
-        *avg_ms = total / reps;
-        return NEON_OK;
-    });
-}
-
-int neon_stream_probe(neon_context* ctx, int64_t bytes, int32_t reps, double* read_ms, double* copy_ms)
-{
-    return guarded(ctx, [&] {
-        if (bytes < 8 || reps <= 0) throw status_error(NEON_BAD_ARGUMENT, "bad probe arguments");
-        if (read_ms) *read_ms = stream_probe(ctx->stream, bytes, reps, false);
-        if (copy_ms) *copy_ms = stream_probe(ctx->stream, bytes, reps, true);
         return NEON_OK;
     });
 }

@@ -73,6 +73,15 @@ __device__ inline double reduce_partials(const double* __restrict__ part, int n)
     return block_sum<BS>(s);
 }
 
+// The total of a partials array: n >= 0 reduces its n partials (as reduce_partials); n < 0
+// reads the total that totals_kernel stored after them, at part[kMaxPartials].
+template <int BS>
+__device__ inline double partials_total(const double* __restrict__ part, int n)
+{
+    if (n < 0) return part[kMaxPartials];
+    return reduce_partials<BS>(part, n);
+}
+
 // Contiguous [begin, end) chunk of `total` items for a block, with the blocks of each
 // XCD (block id mod 8 on MI355X) given adjacent chunks so that neighbouring rows share
 // an L2.

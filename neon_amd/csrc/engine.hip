@@ -5,6 +5,7 @@
 #include "krylov_kernels.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -37,13 +38,10 @@ int linear_operator::spmv_blocks() const
 {
     if (node_block && sell)
     {
-        // one wavefront per slice in one pass, or fewer workgroups striding over the slices;
-        // XCD order: a multiple of 8 workgroups
-        int64_t const per = sell_block / 64;
-        int64_t needed = (int64_t(slices) + per - 1) / per;
-        if (nb_sweep) needed = (needed + 7) / 8 * 8;
+        // one wavefront per slice in one pass (4 per workgroup), or fewer workgroups striding
+        // over the slices
+        int64_t const needed = (int64_t(slices) + 3) / 4;
         int64_t const want = blocks_override > 0 ? std::min<int64_t>(blocks_override, needed) : needed;
-        if (nb_sweep) return xcd_blocks(want, kMaxPartials);
         return int(std::max<int64_t>(1, std::min<int64_t>(want, kMaxPartials)));
     }
     int64_t const rows = node_block ? nodes : n;
@@ -68,25 +66,22 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
     if (A.node_block && A.sell)
     {
         if (!sval) throw status_error(NEON_BAD_ARGUMENT, "sliced operator without sliced values");
-        sell_matrix m{A.nodes, A.slices, A.sptr, A.scol, A.scol16};
-#define NEON_SELL(BS, XCD, I16, NT) \
-    spmv_sell_kernel<BS, XCD, I16, NT, VT, XT, YT, AT, DOT, MODE><<<blocks, BS, 0, s>>>(m, sval, x, y, b, part, st, gate)
-        // nontemporal value loads with 256-thread workgroups only
-        int const key = (A.sell_block == 1024 ? 4 : (A.nb_nt ? 8 : 0)) + (A.nb_sweep ? 2 : 0) + (A.scol16 ? 1 : 0);
-        switch (key)
+        sell_matrix m{A.nodes, A.slices, A.sptr, A.snode, A.slane, A.scode, A.sexc, A.soff16, A.soff32};
+#define NEON_SELL(I16, NT) \
+    spmv_sell_kernel<I16, NT, VT, XT, YT, AT, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate)
+        if (A.nb_nt)
         {
-        case 0: NEON_SELL(256, false, false, false); break;
-        case 1: NEON_SELL(256, false, true, false); break;
-        case 2: NEON_SELL(256, true, false, false); break;
-        case 3: NEON_SELL(256, true, true, false); break;
-        case 4: NEON_SELL(1024, false, false, false); break;
-        case 5: NEON_SELL(1024, false, true, false); break;
-        case 6: NEON_SELL(1024, true, false, false); break;
-        case 7: NEON_SELL(1024, true, true, false); break;
-        case 8: NEON_SELL(256, false, false, true); break;
-        case 9: NEON_SELL(256, false, true, true); break;
-        case 10: NEON_SELL(256, true, false, true); break;
-        default: NEON_SELL(256, true, true, true); break;
+            if (A.soff16)
+                NEON_SELL(true, true);
+            else
+                NEON_SELL(false, true);
+        }
+        else
+        {
+            if (A.soff16)
+                NEON_SELL(true, false);
+            else
+                NEON_SELL(false, false);
         }
 #undef NEON_SELL
         NEON_CHECK_HIP(hipGetLastError());
@@ -95,35 +90,15 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
     if (A.node_block)
     {
         nb_matrix m{A.nodes, A.bptr, A.bcol};
-#define NEON_NB(L, NT, SW) \
-    spmv_nb_kernel<L, 1, NT, SW, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate)
-        if (A.nb_meta)
-            spmv_nb_meta_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
-        else if (A.nb_stream)
-            spmv_nb_stream_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
-        else if (A.nb_lds)
-            spmv_nb_lds_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
-        else if (A.nb_pipeline)
+#define NEON_NB(L, NT) \
+    spmv_nb_kernel<L, 1, NT, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate)
+        int const key = (A.nb_lanes == 16 ? 0 : 2) + (A.nb_nt ? 1 : 0);
+        switch (key)
         {
-            if (A.nb_nt)
-                spmv_nb_pipe_kernel<true, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
-            else
-                spmv_nb_pipe_kernel<false, VT, XT, YT, AT, DOT, MODE><<<blocks, kBlock, 0, s>>>(m, val, x, y, b, part, st, gate);
-        }
-        else
-        {
-            int const key = (A.nb_lanes == 16 ? 0 : 4) + (A.nb_nt ? 2 : 0) + (A.nb_sweep ? 1 : 0);
-            switch (key)
-            {
-            case 0: NEON_NB(16, false, false); break;
-            case 1: NEON_NB(16, false, true); break;
-            case 2: NEON_NB(16, true, false); break;
-            case 3: NEON_NB(16, true, true); break;
-            case 4: NEON_NB(32, false, false); break;
-            case 5: NEON_NB(32, false, true); break;
-            case 6: NEON_NB(32, true, false); break;
-            default: NEON_NB(32, true, true); break;
-            }
+        case 0: NEON_NB(16, false); break;
+        case 1: NEON_NB(16, true); break;
+        case 2: NEON_NB(32, false); break;
+        default: NEON_NB(32, true); break;
         }
 #undef NEON_NB
     }
@@ -145,13 +120,14 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
 }
 } // namespace
 
-void linear_operator::apply64(hipStream_t s, const double* x, double* y, double* part, const krylov_state* st) const
+void linear_operator::apply64(hipStream_t s, const double* x, double* y, double* part, const krylov_state* st,
+                              int gate) const
 {
     if (!v64) throw status_error(NEON_BAD_ARGUMENT, "operator has no FP64 values");
     if (part)
-        launch_spmv<double, double, double, double, true, 0>(*this, s, v64, s64, x, y, nullptr, part, st);
+        launch_spmv<double, double, double, double, true, 0>(*this, s, v64, s64, x, y, nullptr, part, st, gate);
     else
-        launch_spmv<double, double, double, double, false, 0>(*this, s, v64, s64, x, y, nullptr, nullptr, st);
+        launch_spmv<double, double, double, double, false, 0>(*this, s, v64, s64, x, y, nullptr, nullptr, st, gate);
 }
 
 void linear_operator::apply32(hipStream_t s, const float* x, float* y, double* part, const krylov_state* st) const
@@ -228,31 +204,14 @@ void kernel_profiler::read(int cls, bool reset, int64_t* count, double* ms)
 // ================================================================== engine
 krylov_engine::krylov_engine(hipStream_t s) : s_(s)
 {
-    for (auto& p : part_) p.resize(kMaxPartials);
+    // partials, then their total (totals_kernel)
+    for (auto& p : part_) p.resize(kMaxPartials + 1);
     state_.resize(2);
     scalar_.resize(1);
-    barrier_.resize(2);
-    NEON_CHECK_HIP(hipMemsetAsync(barrier_.data(), 0, 2 * sizeof(uint32_t), s_));
     NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_state_), sizeof(krylov_state)));
     NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ring_), 2 * sizeof(krylov_state)));
     for (auto& e : ring_events_) NEON_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    if (const char* f = std::getenv("NEON_FUSED_UPDATES")) fused_updates = std::atoi(f) != 0;
     NEON_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_scalar_), sizeof(double)));
-}
-
-// A fused update whose grid barrier timed out leaves its solve's result undefined: fail the
-// solve and re-arm the barrier word.
-void krylov_engine::check_barrier()
-{
-    uint32_t h[2] = {0, 0};
-    NEON_CHECK_HIP(hipMemcpyAsync(h, barrier_.data(), sizeof(h), hipMemcpyDeviceToHost, s_));
-    NEON_CHECK_HIP(hipStreamSynchronize(s_));
-    if (h[1] != 0)
-    {
-        NEON_CHECK_HIP(hipMemsetAsync(barrier_.data(), 0, 2 * sizeof(uint32_t), s_));
-        NEON_CHECK_HIP(hipStreamSynchronize(s_));
-        throw status_error(NEON_HIP_ERROR, "grid barrier of the fused CG update timed out");
-    }
 }
 
 krylov_engine::~krylov_engine()
@@ -279,6 +238,13 @@ void krylov_engine::ensure(int64_t n)
     p_.resize(n);
     Ap_.resize(n);
     n_ = n;
+}
+
+double krylov_engine::read_scalar(const double* d)
+{
+    NEON_CHECK_HIP(hipMemcpyAsync(host_scalar_, d, sizeof(double), hipMemcpyDeviceToHost, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    return *host_scalar_;
 }
 
 double krylov_engine::dot(int64_t n, const double* a, const double* b)
@@ -318,56 +284,6 @@ solve_result finish(const krylov_state& s, int criterion)
 }
 } // namespace
 
-namespace
-{
-// co-resident workgroups of a kernel on the current device (0 when it cannot be queried)
-int coresident_blocks(const void* kernel)
-{
-    int dev = 0, per_cu = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess) return 0;
-    return per_cu * cus;
-}
-
-template <typename T, int EPT, bool MIXED>
-bool try_fused(hipStream_t s, int64_t n, T* x, T* r, T* p, const T* Ap, const T* minv, const double* part_pAp,
-               int npart, const krylov_state* st, krylov_state* st_next, double* part_rr, double* part_rz,
-               int criterion, double tol, int max_it, uint32_t* bar)
-{
-    // FP32 keeps the old p in registers as well as z; FP64 keeps z only (register budget)
-    constexpr bool keep_p = sizeof(T) == 4;
-    auto kernel = cg_fused_update_kernel<T, EPT, keep_p, MIXED>;
-    static thread_local int capacity = -1;
-    if (capacity < 0) capacity = coresident_blocks(reinterpret_cast<const void*>(kernel));
-    int64_t const grid = (n + int64_t(kBlock) * EPT - 1) / (int64_t(kBlock) * EPT);
-    if (grid < 1 || grid > capacity || grid > kMaxPartials) return false;
-    void* args[] = {&n, &x, &r, &p, &Ap, &minv, &part_pAp, &npart, &st, &st_next, &part_rr, &part_rz, &criterion,
-                    &tol, &max_it, &bar};
-    NEON_CHECK_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kernel), dim3(unsigned(grid)),
-                                              dim3(kBlock), args, 0, s));
-    return true;
-}
-} // namespace
-
-template <typename T, bool MIXED>
-bool krylov_engine::fused_update(int64_t n, T* x, T* r, T* p, const T* Ap, const T* minv, const double* part_pAp,
-                                 int npart, const krylov_state* st, krylov_state* st_next, double* part_rr,
-                                 double* part_rz, int criterion, double tol, int max_it)
-{
-    if (!fused_updates) return false;
-    auto go = [&](auto ept) {
-        constexpr int E = decltype(ept)::value;
-        return try_fused<T, E, MIXED>(s_, n, x, r, p, Ap, minv, part_pAp, npart, st, st_next, part_rr, part_rz,
-                                      criterion, tol, max_it, barrier_.data());
-    };
-    if (fused_ept == 8) return go(std::integral_constant<int, 8>{});
-    if (fused_ept == 16) return go(std::integral_constant<int, 16>{});
-    if (fused_ept == 32) return go(std::integral_constant<int, 32>{});
-    return go(std::integral_constant<int, 8>{}) || go(std::integral_constant<int, 16>{})
-           || go(std::integral_constant<int, 32>{});
-}
-
 // Drives a device-resident Krylov loop: `batch` launches poll_every iterations (advancing
 // `launched`; the state after `launched` iterations is st[launched & 1]).  The host keeps one
 // batch queued ahead of the one whose final state it is waiting for, so the device never idles
@@ -401,15 +317,25 @@ void krylov_engine::run_batches(krylov_state* st, int64_t& launched, Batch&& bat
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
 }
 
-// Jacobi-PCG in FP64.  Per iteration three kernels: SpMV fused with p.Ap, the x/r update
-// fused with r.r and r.z, and the p update with the convergence decision; scalars stay on
-// the device and the host polls the state every `poll_every` iterations.  Iterations past
-// convergence are no-ops, so the reported count is exact.
+// Jacobi-PCG in FP64.  Per iteration: SpMV fused with the block partials of p.Ap, their total
+// (one block), the x/r update fused with the partials of r.r and r.z, their totals, and the p
+// update with the convergence decision; scalars stay on the device and the host polls the
+// state every `poll_every` iterations (a batch queued ahead of the one it waits on).
+// Iterations past convergence are no-ops, so the reported count is exact.
 solve_result krylov_engine::cg64(const linear_operator& A, const double* b, double* x, const double* minv,
-                                 int criterion, double tol, int32_t max_it)
+                                 int criterion, double tol, int32_t max_it, const krylov_work* work)
 {
     int64_t const n = A.n;
-    ensure(n);
+    double *r = nullptr, *p = nullptr, *Ap = nullptr;
+    if (work)
+    {
+        r = work->r, p = work->p, Ap = work->Ap;
+    }
+    else
+    {
+        ensure(n);
+        r = r_.data(), p = p_.data(), Ap = Ap_.data();
+    }
     int const nb = vec_blocks(n);
     int const ns = A.spmv_blocks();
     double* part_pAp = part_[0].data();
@@ -417,7 +343,7 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
     double* part_rz = part_[2].data();
     krylov_state* st = state_.data();
 
-    cg_init_kernel<<<nb, kBlock, 0, s_>>>(n, x, r_.data(), p_.data(), b, minv, part_rr, part_rz);
+    cg_init_kernel<<<nb, kBlock, 0, s_>>>(n, x, r, p, b, minv, part_rr, part_rz);
     cg_init_state_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, st, criterion, tol, max_it);
     NEON_CHECK_HIP(hipGetLastError());
 
@@ -429,18 +355,16 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
             krylov_state* cur = st + (launched & 1);
             krylov_state* nxt = st + ((launched + 1) & 1);
             int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP64) : -1;
-            A.apply64(s_, p_.data(), Ap_.data(), part_pAp, cur);
+            A.apply64(s_, p, Ap, part_pAp, cur);
             if (prof) prof->end(s_, tok);
-            if (fused_update<double, false>(n, x, r_.data(), p_.data(), Ap_.data(), minv, part_pAp, ns, cur, nxt,
-                                            part_rr, part_rz, criterion, tol, max_it))
-                continue;
-            cg_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x, r_.data(), p_.data(), Ap_.data(), minv, part_pAp, ns, cur,
+            totals_kernel<<<1, 1024, 0, s_>>>(part_pAp, nullptr, ns, cur);
+            cg_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x, r, p, Ap, minv, part_pAp, -1, cur,
                                                         part_rr, part_rz);
-            cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p_.data(), r_.data(), minv, part_rr, part_rz, nb, cur, nxt,
+            totals_kernel<<<1, 1024, 0, s_>>>(part_rr, part_rz, nb, cur);
+            cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p, r, minv, part_rr, part_rz, -1, cur, nxt,
                                                        criterion, tol, max_it);
         }
     });
-    if (fused_updates) check_barrier();
     res = finish(*host_state_, criterion);
     // SpMV launches that did work: Eigen's rule counts the converging iteration's SpMV
     // outside `iterations`; the CUDA rule counts every iteration.
@@ -450,139 +374,124 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
     return res;
 }
 
-// Jacobi-BiCGSTAB in FP64 (the unsymmetric selection, not on the symmetric hot path): vector
-// work on the device, the handful of scalars per iteration reduced on the device and read by
-// the host.  criterion NEON_STOP_ABSOLUTE_PRECONDITIONED follows
-// biconjugate_gradient_stabilised_cuda::solve (biconjugate_gradient_stabilised_cuda.cpp:31-200);
-// NEON_STOP_RELATIVE follows Eigen's BiCGSTAB with its restart rule (linear_solver.cpp:111-138).
+// Jacobi-BiCGSTAB in FP64 (the unsymmetric selection) with every scalar on the device and the
+// host polling the state like the CG loop (bicg_* kernels).  criterion
+// NEON_STOP_ABSOLUTE_PRECONDITIONED follows biconjugate_gradient_stabilised_cuda::solve
+// (biconjugate_gradient_stabilised_cuda.cpp:31-200); NEON_STOP_RELATIVE follows Eigen's BiCGSTAB
+// with its restart rule (linear_solver.cpp:111-138).
 solve_result krylov_engine::bicgstab64(const linear_operator& A, const double* b, double* x, const double* minv,
-                                       int criterion, double tol, int32_t max_it)
+                                       int criterion, double tol, int32_t max_it, const krylov_work* work)
 {
     int64_t const n = A.n;
-    ensure(n);
-    r0_.resize(n);
-    v_.resize(n);
-    s_vec_.resize(n);
-    t_.resize(n);
-    ph_.resize(n);
-    sh_.resize(n);
-    double* r = r_.data();
-    double* p = p_.data();
-    double* r0 = r0_.data();
-    double* v = v_.data();
-    double* sv = s_vec_.data();
-    double* t = t_.data();
-    double* y = ph_.data();
-    double* z = sh_.data();
-    int const g = grid_for(n);
+    if (!work)
+    {
+        ensure(n);
+        r0_.resize(n);
+        v_.resize(n);
+        s_vec_.resize(n);
+        t_.resize(n);
+        ph_.resize(n);
+        sh_.resize(n);
+    }
+    double* r = work ? work->r : r_.data();
+    double* p = work ? work->p : p_.data();
+    double* r0 = work ? work->r0 : r0_.data();
+    double* v = work ? work->v : v_.data();
+    double* sv = work ? work->s : s_vec_.data();
+    double* t = work ? work->t : t_.data();
+    double* p_hat = work ? work->p_hat : ph_.data();
+    double* s_hat = work ? work->s_hat : sh_.data();
+    int const nb = vec_blocks(n);
+    bool const cuda_rule = criterion == NEON_STOP_ABSOLUTE_PRECONDITIONED;
+    double* part_r0v = part_[0].data();
+    double* part_ss = part_[1].data();
+    double* part_ts = part_[2].data();
+    double* part_tt = part_[3].data();
+    double* part_r0r = part_[4].data();
+    double* part_rr = part_[5].data();
+    krylov_state* st = state_.data();
     size_t const bytes = n * sizeof(double);
-    solve_result res;
-    auto spmv = [&](const double* in, double* out) {
-        A.apply64(s_, in, out, nullptr, nullptr);
-        ++res.spmv;
-    };
+    // x = 0, r = r0 = b, p = v = 0 (x0 = 0 as both callers pass)
     NEON_CHECK_HIP(hipMemsetAsync(x, 0, bytes, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(r0, b, bytes, hipMemcpyDeviceToDevice, s_));
     NEON_CHECK_HIP(hipMemsetAsync(v, 0, bytes, s_));
     NEON_CHECK_HIP(hipMemsetAsync(p, 0, bytes, s_));
-
-    if (criterion == NEON_STOP_ABSOLUTE_PRECONDITIONED)
+    double const bb = dot(n, b, b);
+    krylov_state s0{};
+    s0.rz = bb; // r0 . r = b . b
+    s0.rr = bb;
+    s0.rhs2 = bb;
+    s0.aux0 = 1.0; // Eigen's rho_old, alpha, omega start at 1
+    s0.alpha = 1.0;
+    s0.omega = 1.0;
+    s0.aux1 = bb; // Eigen's |r0|^2
+    if (cuda_rule)
     {
-        double residual = dot(n, r, r0), residual_old = 0.0, alpha = 0.0, omega = 0.0;
-        int32_t k = 0;
-        int status = NEON_OK;
-        while (std::sqrt(residual) > tol && k < max_it)
+        s0.aux0 = 0.0;
+        s0.alpha = s0.omega = 0.0;
+        if (!(std::sqrt(bb) > tol)) s0.done = KRYLOV_CONVERGED;
+        else if (max_it <= 0) s0.done = KRYLOV_LIMIT;
+    }
+    else
+    {
+        s0.threshold = tol * tol * bb;
+        if (bb == 0.0 || !(bb > s0.threshold)) s0.done = KRYLOV_CONVERGED;
+        else if (max_it <= 0) s0.done = KRYLOV_LIMIT;
+    }
+    if (!std::isfinite(bb)) s0.done = KRYLOV_NONFINITE;
+    *host_state_ = s0;
+    NEON_CHECK_HIP(hipMemcpyAsync(st, host_state_, sizeof(krylov_state), hipMemcpyHostToDevice, s_));
+    // the state is a single slot (decision kernels write it between the grid kernels that read
+    // it), so run_batches reads st[launched & 1] = st[0] with launched kept even
+    solve_result res;
+    int64_t launched = 0;
+    run_batches(st, launched, [&] {
+        for (int k = 0; k < poll_every; ++k)
         {
-            ++k;
-            if (k == 1)
-                NEON_CHECK_HIP(hipMemcpyAsync(p, r, bytes, hipMemcpyDeviceToDevice, s_));
-            else
+            if (!cuda_rule)
             {
-                double const beta = residual / residual_old * alpha / omega;
-                lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, p, -omega, v, p); // p - omega v
-                lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, beta, p, p);   // r + beta p
+                bicg_restart_test_kernel<<<1, kBlock, 0, s_>>>(st);
+                A.residual64(s_, b, x, r, st, 1);
+                bicg_restart_kernel<<<nb, kBlock, 0, s_>>>(n, r, r0, part_ss, st);
+                bicg_restart_finish_kernel<<<1, kBlock, 0, s_>>>(part_ss, nb, st);
             }
-            mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, p, y);
-            spmv(y, v);
-            alpha = residual / dot(n, r0, v);
-            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, -alpha, v, sv);
-            double const norm_s = std::sqrt(dot(n, sv, sv));
-            if (norm_s < tol)
-            {
-                lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, alpha, y, x);
-                residual = norm_s * norm_s;
-                break;
-            }
-            mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, sv, z);
-            spmv(z, t);
-            double const ts = dot(n, t, sv), tt = dot(n, t, t);
-            omega = ts / tt;
-            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, alpha, y, x);
-            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, omega, z, x);
-            lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, sv, -omega, t, r);
-            residual_old = residual;
-            residual = dot(n, r0, r);
-            if (omega == 0.0) // is_approx(omega, 0.0) holds only for an exact zero
-            {
-                status = NEON_BREAKDOWN;
-                break;
-            }
+            bicg_p_kernel<<<nb, kBlock, 0, s_>>>(n, p, r, v, minv, p_hat, st, cuda_rule ? 1 : 0);
+            int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP64) : -1;
+            A.apply64(s_, p_hat, v, nullptr, st);
+            if (prof) prof->end(s_, tok);
+            bicg_dot_kernel<<<nb, kBlock, 0, s_>>>(n, r0, v, nullptr, nullptr, part_r0v, nullptr, st, BICG_GATE_NONE);
+            bicg_s_kernel<<<nb, kBlock, 0, s_>>>(n, r, v, minv, sv, s_hat, part_r0v, nb, part_ss, st);
+            bicg_alpha_kernel<<<1, kBlock, 0, s_>>>(part_r0v, part_ss, nb, st, cuda_rule ? 1 : 0, tol);
+            A.apply64(s_, s_hat, t, nullptr, st, BICG_GATE_UNLESS_FLAG);
+            bicg_dot_kernel<<<nb, kBlock, 0, s_>>>(n, t, sv, t, t, part_ts, part_tt, st, BICG_GATE_UNLESS_FLAG);
+            bicg_update_kernel<<<nb, kBlock, 0, s_>>>(n, x, r, r0, p_hat, s_hat, sv, t, part_ts, part_tt, nb, part_r0r,
+                                                      part_rr, st, cuda_rule ? 1 : 0);
+            bicg_finish_kernel<<<1, kBlock, 0, s_>>>(part_ts, part_tt, part_r0r, part_rr, nb, st, cuda_rule ? 1 : 0,
+                                                     tol, max_it);
         }
-        NEON_CHECK_HIP(hipGetLastError());
-        res.iterations = k;
-        res.error = std::sqrt(residual);
-        res.status = status != NEON_OK ? status : (k >= max_it ? NEON_NONCONVERGED : NEON_OK);
-        if (!std::isfinite(residual) && res.status == NEON_OK) res.status = NEON_NAN_OR_INF;
-        return res;
-    }
-
-    // Eigen
-    double const rhs2 = dot(n, r, r);
-    if (rhs2 == 0.0)
+    });
+    krylov_state const& f = *host_state_;
+    res.iterations = f.it;
+    res.spmv = 2ll * f.it + f.restarts;
+    if (prof) prof->collect(f.it);
+    if (cuda_rule)
     {
-        res.status = NEON_OK;
-        return res;
-    }
-    double r0_sq = rhs2;
-    double rho = 1, alpha = 1, w = 1;
-    double const tol2 = tol * tol * rhs2;
-    double const eps2 = DBL_EPSILON * DBL_EPSILON;
-    int32_t i = 0, restarts = 0;
-    double rr = rhs2;
-    while (rr > tol2 && i < max_it)
-    {
-        double const rho_old = rho;
-        rho = dot(n, r0, r);
-        if (std::abs(rho) < eps2 * r0_sq)
+        res.error = std::sqrt(f.rz);
+        switch (f.done)
         {
-            A.residual64(s_, b, x, r);
-            ++res.spmv;
-            NEON_CHECK_HIP(hipMemcpyAsync(r0, r, bytes, hipMemcpyDeviceToDevice, s_));
-            rho = r0_sq = dot(n, r, r);
-            if (restarts++ == 0) i = 0;
+        case KRYLOV_CONVERGED: res.status = NEON_OK; break;
+        case KRYLOV_BREAKDOWN: res.status = NEON_BREAKDOWN; break;
+        case KRYLOV_NONFINITE: res.status = NEON_NAN_OR_INF; break;
+        default: res.status = NEON_NONCONVERGED; break;
         }
-        double const beta = (rho / rho_old) * (alpha / w);
-        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, p, -w, v, p);
-        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, beta, p, p);
-        mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, p, y);
-        spmv(y, v);
-        alpha = rho / dot(n, r0, v);
-        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, r, -alpha, v, sv);
-        mul_kernel<<<g, kBlock, 0, s_>>>(n, minv, sv, z);
-        spmv(z, t);
-        double const tmp = dot(n, t, t);
-        w = tmp > 0.0 ? dot(n, t, sv) / tmp : 0.0;
-        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, alpha, y, w, z, y); // alpha y + w z
-        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, x, 1.0, y, x);
-        lincomb_kernel<<<g, kBlock, 0, s_>>>(n, 1.0, sv, -w, t, r);
-        rr = dot(n, r, r);
-        ++i;
     }
-    NEON_CHECK_HIP(hipGetLastError());
-    res.iterations = i;
-    res.error = std::sqrt(rr / rhs2);
-    res.status = !std::isfinite(rr) ? NEON_NAN_OR_INF : (i >= max_it ? NEON_NONCONVERGED : NEON_OK);
+    else
+    {
+        res.error = f.rhs2 > 0.0 ? std::sqrt(f.rr / f.rhs2) : 0.0;
+        res.status = f.done == KRYLOV_NONFINITE ? NEON_NAN_OR_INF
+                                                : (f.done == KRYLOV_LIMIT ? NEON_NONCONVERGED : NEON_OK);
+    }
     return res;
 }
 
@@ -664,29 +573,27 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP32) : -1;
             A.apply32(s_, p32, Ap32, part_pAp, cur);
             if (prof) prof->end(s_, tok);
+            totals_kernel<<<1, 1024, 0, s_>>>(part_pAp, nullptr, ns, cur);
             // the reliable-update / true-residual convergence test runs every refine_every
-            // iterations; in between, one fused launch does the x/r and p updates
+            // iterations
             int const decided = (launched + 1) % refine_every == 0 ? 1 : 0;
-            if (!decided && fused_update<float, true>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, nxt, part_rr,
-                                                      part_rz, NEON_STOP_RELATIVE, tol, max_it))
-                continue;
-            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, ns, cur, part_rr, part_rz);
+            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, -1, cur, part_rr, part_rz);
+            totals_kernel<<<1, 1024, 0, s_>>>(part_rr, part_rz, nb, cur);
             if (decided)
             {
-                mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
+                mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, -1, cur, reliable_delta);
                 mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, cur, 0);
                 Ar.residual64(s_, b, x, r64, cur, 1);
                 mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
             }
-            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, nb,
-                                                         decided, cur, nxt, max_it);
+            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, -1,
+                                                         nb, decided, cur, nxt, max_it);
         }
     });
     // fold whatever the recurrence holds since the last reliable update
     mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, st + (launched & 1), 1);
     NEON_CHECK_HIP(hipGetLastError());
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
-    if (fused_updates) check_barrier();
     solve_result res = finish(*host_state_, NEON_STOP_RELATIVE);
     res.refinements = host_state_->restarts;
     int64_t const inner = res.iterations + (host_state_->done == KRYLOV_CONVERGED ? 1 : 0);
@@ -715,36 +622,6 @@ void fill_ones(hipStream_t s, int64_t n, float* x)
 {
     fill_kernel<float><<<grid_for(n), kBlock, 0, s>>>(n, 1.0f, x);
     NEON_CHECK_HIP(hipGetLastError());
-}
-
-double stream_probe(hipStream_t s, int64_t bytes, int reps, bool copy)
-{
-    int64_t const n = bytes / 8;
-    device_array<double> a, b, part;
-    a.resize(n);
-    part.resize(4096);
-    if (copy) b.resize(n);
-    fill_ones(s, n, a.data());
-    hipEvent_t e0, e1;
-    NEON_CHECK_HIP(hipEventCreate(&e0));
-    NEON_CHECK_HIP(hipEventCreate(&e1));
-    auto launch = [&]() {
-        if (copy)
-            stream_copy_kernel<double><<<grid_for(n), kBlock, 0, s>>>(n, a.data(), b.data());
-        else
-            stream_sum_kernel<<<4096, kBlock, 0, s>>>(n, a.data(), part.data());
-    };
-    launch();
-    NEON_CHECK_HIP(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r) launch();
-    NEON_CHECK_HIP(hipEventRecord(e1, s));
-    NEON_CHECK_HIP(hipEventSynchronize(e1));
-    NEON_CHECK_HIP(hipGetLastError());
-    float ms = 0.0f;
-    NEON_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return double(ms) / reps;
 }
 
 void convert_to_float(hipStream_t s, int64_t n, const double* x, float* y)
@@ -1002,65 +879,7 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
     NEON_CHECK_HIP(hipMemcpyAsync(elems_.data(), elems.data(), size_t(E_) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(bptr_.data(), bptr_h_.data(), (size_t(N_) + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(bcol_.data(), bcol_h_.data(), size_t(nnzb) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
-    // sliced layout of the pattern (sell_matrix): 64 consecutive nodes per slice, as many steps
-    // as the slice's largest block count; padded steps point at the node itself (node 0 for
-    // lanes past the last node)
-    {
-        slices_ = int32_t((int64_t(N_) + 63) / 64);
-        std::vector<int32_t> sptr(size_t(slices_) + 1, 0);
-        for (int32_t sl = 0; sl < slices_; ++sl)
-        {
-            int32_t md = 0;
-            for (int64_t nd = 64ll * sl; nd < std::min<int64_t>(64ll * sl + 64, N_); ++nd)
-                md = std::max<int32_t>(md, bptr_h_[nd + 1] - bptr_h_[nd]);
-            if (int64_t(sptr[sl]) + md > int64_t(INT32_MAX) / (9 * 64))
-                throw status_error(NEON_BAD_ARGUMENT, "sliced tangent exceeds the index range");
-            sptr[sl + 1] = sptr[sl] + md;
-        }
-        sell_steps_ = sptr[slices_];
-        std::vector<int32_t> scol(size_t(sell_steps_) * 64);
-#pragma omp parallel for schedule(static)
-        for (int32_t sl = 0; sl < slices_; ++sl)
-            for (int lane = 0; lane < 64; ++lane)
-            {
-                int64_t const nd = 64ll * sl + lane;
-                bool const ok = nd < N_;
-                int32_t const b0 = ok ? bptr_h_[nd] : 0, d = ok ? bptr_h_[nd + 1] - b0 : 0;
-                for (int32_t kk = 0; kk < sptr[sl + 1] - sptr[sl]; ++kk)
-                    scol[size_t(sptr[sl] + kk) * 64 + lane] = kk < d ? bcol_h_[size_t(b0) + kk] : int32_t(ok ? nd : 0);
-            }
-        sptr_.resize(sptr.size());
-        scol_.resize(scol.size());
-        NEON_CHECK_HIP(hipMemcpyAsync(sptr_.data(), sptr.data(), sptr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
-        NEON_CHECK_HIP(hipMemcpyAsync(scol_.data(), scol.data(), scol.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
-        // the same columns as 16-bit offsets from the lane's node, when every offset fits
-        std::vector<int16_t> scol16(scol.size());
-        bool fits = true;
-#pragma omp parallel for schedule(static) reduction(&& : fits)
-        for (int32_t sl = 0; sl < slices_; ++sl)
-            for (int lane = 0; lane < 64; ++lane)
-            {
-                int64_t const nd = 64ll * sl + lane;
-                for (int32_t kk = 0; kk < sptr[sl + 1] - sptr[sl]; ++kk)
-                {
-                    size_t const at = size_t(sptr[sl] + kk) * 64 + lane;
-                    // lanes past the last node gather from the last node
-                    int64_t const col = nd < N_ ? int64_t(scol[at]) : int64_t(N_) - 1;
-                    int64_t const off = col - nd;
-                    if (off < INT16_MIN || off > INT16_MAX) fits = false;
-                    scol16[at] = int16_t(off < INT16_MIN ? 0 : (off > INT16_MAX ? 0 : off));
-                }
-            }
-        scol16_ok_ = fits;
-        if (fits)
-        {
-            scol16_.resize(scol16.size());
-            NEON_CHECK_HIP(hipMemcpyAsync(scol16_.data(), scol16.data(), scol16.size() * sizeof(int16_t),
-                                          hipMemcpyHostToDevice, s_));
-        }
-        // the host vectors go out of scope: wait for the copies
-        NEON_CHECK_HIP(hipStreamSynchronize(s_));
-    }
+    build_sliced_pattern();
     NEON_CHECK_HIP(hipMemcpyAsync(self_slot_.data(), self.data(), size_t(N_) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(slots_.data(), slots.data(), slots.size(), hipMemcpyHostToDevice, s_));
 
@@ -1569,6 +1388,163 @@ void device_mesh::download_qp_state(double* sigma, double* D, double* detF, doub
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
 }
 
+// Sliced layout of the pattern (sell_matrix).  Slice s holds the 64 consecutive nodes
+// 64 s ... 64 s + 63 (node order keeps each step's gathers of x on few cache lines), the last
+// slice padded with empty lanes; a slice has as many steps as its largest block count, and step
+// kk of a lane holds the node's block kk (CSR order).  A node's signature is its block count
+// and the offsets (column - node) of its blocks.  When all but at most two of a slice's nodes
+// share one signature (on a structured mesh: all slices but those that straddle two kinds of
+// boundary row), every step is a shared step: its code holds the offset of the common
+// signature (shared_code; 0 past its block count) and its exception word the 16-bit offsets of
+// the (at most two) other nodes, whose lanes the slice's exception word names.  The other
+// slices' steps hold an index into the per-lane offsets (soff16 / soff32).  A lane past its
+// node's block count takes offset 0 (its values are zero); an empty lane (snode -1 - m)
+// gathers like node m, a node of the common signature.
+void device_mesh::build_sliced_pattern()
+{
+    std::vector<uint64_t> sig(N_);
+#pragma omp parallel for schedule(static)
+    for (int64_t nd = 0; nd < N_; ++nd)
+    {
+        int32_t const b0 = bptr_h_[nd], d = bptr_h_[nd + 1] - b0;
+        uint64_t h = 0x9e3779b97f4a7c15ull ^ uint64_t(d);
+        for (int32_t k = 0; k < d; ++k)
+        {
+            h ^= uint64_t(int64_t(bcol_h_[size_t(b0) + k]) - nd) + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+            h *= 0xbf58476d1ce4e5b9ull;
+        }
+        sig[nd] = h;
+    }
+    int64_t const nslices = (int64_t(N_) + 63) / 64;
+    if (nslices > INT32_MAX / 64) throw status_error(NEON_BAD_ARGUMENT, "sliced tangent exceeds the index range");
+    slices_ = int32_t(nslices);
+    auto degree = [&](int32_t nd) { return bptr_h_[nd + 1] - bptr_h_[nd]; };
+    // offset of node nd's block kk (0 past its block count)
+    auto offset = [&](int32_t nd, int32_t kk) -> int64_t {
+        return kk < degree(nd) ? int64_t(bcol_h_[size_t(bptr_h_[nd]) + kk]) - nd : int64_t(0);
+    };
+    std::vector<int32_t> sptr(size_t(slices_) + 1, 0), snode(size_t(slices_) * 64), slane(slices_);
+    // per slice: the lane of a node of the common signature (-1: per-lane offsets)
+    std::vector<int32_t> major(slices_);
+#pragma omp parallel for schedule(static)
+    for (int32_t sl = 0; sl < slices_; ++sl)
+    {
+        int64_t const first = 64ll * sl;
+        int const valid = int(std::min<int64_t>(64, N_ - first));
+        // a signature that all but two lanes share is the signature of one of the first three
+        int best = 0, best_count = 0;
+        for (int c = 0; c < std::min(valid, 3); ++c)
+        {
+            int count = 0;
+            for (int q = 0; q < valid; ++q) count += sig[first + q] == sig[first + c];
+            if (count > best_count) best = c, best_count = count;
+        }
+        int exc[2] = {-1, -1}, ne = 0;
+        bool ok = best_count >= valid - 2;
+        int32_t const ref = int32_t(first + best);
+        for (int q = 0; q < valid && ok; ++q)
+        {
+            int32_t const nd = int32_t(first + q);
+            bool same = sig[nd] == sig[ref] && degree(nd) == degree(ref);
+            // verify the signature (a hash match is not proof)
+            for (int32_t kk = 0; same && kk < degree(ref); ++kk) same = offset(nd, kk) == offset(ref, kk);
+            if (same) continue;
+            if (ne == 2) ok = false;
+            else
+            {
+                exc[ne++] = q;
+                for (int32_t kk = 0; kk < degree(nd) && ok; ++kk)
+                {
+                    int64_t const o = offset(nd, kk);
+                    ok = o >= INT16_MIN && o <= INT16_MAX;
+                }
+            }
+        }
+        for (int32_t kk = 0; kk < degree(ref) && ok; ++kk)
+            ok = offset(ref, kk) > -kSharedBias && offset(ref, kk) < kSharedBias;
+        major[sl] = ok ? best : -1;
+        slane[sl] = ok ? ((exc[0] & 0xff) | ((exc[1] & 0xff) << 8)) : -1;
+        for (int q = 0; q < 64; ++q)
+            snode[size_t(sl) * 64 + q] = q < valid ? int32_t(first + q) : -1 - (ok ? ref : int32_t(first));
+    }
+    for (int32_t sl = 0; sl < slices_; ++sl)
+    {
+        int32_t md = 0;
+        for (int q = 0; q < 64; ++q)
+            if (snode[size_t(sl) * 64 + q] >= 0) md = std::max<int32_t>(md, degree(snode[size_t(sl) * 64 + q]));
+        if (int64_t(sptr[sl]) + md > int64_t(INT32_MAX) / (9 * 64))
+            throw status_error(NEON_BAD_ARGUMENT, "sliced tangent exceeds the index range");
+        sptr[sl + 1] = sptr[sl] + md;
+    }
+    sell_steps_ = sptr[slices_];
+    std::vector<int64_t> lane_steps(size_t(slices_) + 1, 0);
+    for (int32_t sl = 0; sl < slices_; ++sl)
+        lane_steps[size_t(sl) + 1] = lane_steps[sl] + (major[sl] >= 0 ? 0 : sptr[sl + 1] - sptr[sl]);
+    int64_t const own_steps = lane_steps[slices_];
+    if (own_steps > INT32_MAX / 64) throw status_error(NEON_BAD_ARGUMENT, "sliced tangent exceeds the index range");
+    shared_steps_ = sell_steps_ - own_steps;
+    std::vector<int32_t> scode(static_cast<size_t>(std::max<int64_t>(sell_steps_, 1)), 0);
+    std::vector<int32_t> sexc(static_cast<size_t>(std::max<int64_t>(sell_steps_, 1)), 0);
+    std::vector<int32_t> soff32(size_t(std::max<int64_t>(own_steps, 1)) * 64, 0);
+    bool fits = true;
+#pragma omp parallel for schedule(static) reduction(&& : fits)
+    for (int32_t sl = 0; sl < slices_; ++sl)
+    {
+        const int32_t* g = snode.data() + size_t(sl) * 64;
+        int32_t const ks = sptr[sl + 1] - sptr[sl];
+        if (major[sl] >= 0)
+        {
+            int32_t const ref = g[major[sl]];
+            int const e0 = slane[sl] & 0xff, e1 = (slane[sl] >> 8) & 0xff;
+            for (int32_t kk = 0; kk < ks; ++kk)
+            {
+                size_t const at = size_t(sptr[sl]) + kk;
+                scode[at] = shared_code(offset(ref, kk));
+                int32_t const o0 = e0 != 0xff ? int32_t(offset(g[e0], kk)) : 0;
+                int32_t const o1 = e1 != 0xff ? int32_t(offset(g[e1], kk)) : 0;
+                sexc[at] = int32_t((uint32_t(uint16_t(int16_t(o0)))) | (uint32_t(uint16_t(int16_t(o1))) << 16));
+            }
+            continue;
+        }
+        int64_t c = lane_steps[sl];
+        for (int32_t kk = 0; kk < ks; ++kk, ++c)
+        {
+            scode[size_t(sptr[sl]) + kk] = int32_t(c);
+            for (int q = 0; q < 64; ++q)
+            {
+                int64_t const off = g[q] >= 0 ? offset(g[q], kk) : 0;
+                soff32[size_t(c) * 64 + q] = int32_t(off);
+                if (off < INT16_MIN || off > INT16_MAX) fits = false;
+            }
+        }
+    }
+    // slices with per-lane offsets have no exception lanes
+    for (int32_t sl = 0; sl < slices_; ++sl)
+        if (major[sl] < 0) slane[sl] = 0xffff;
+    sptr_.resize(sptr.size());
+    snode_.resize(snode.size());
+    slane_.resize(slane.size());
+    scode_.resize(scode.size());
+    sexc_.resize(sexc.size());
+    soff32_.resize(soff32.size());
+    NEON_CHECK_HIP(hipMemcpyAsync(sptr_.data(), sptr.data(), sptr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(snode_.data(), snode.data(), snode.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(slane_.data(), slane.data(), slane.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(scode_.data(), scode.data(), scode.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(sexc_.data(), sexc.data(), sexc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(soff32_.data(), soff32.data(), soff32.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    soff16_ok_ = fits;
+    std::vector<int16_t> soff16;
+    if (fits)
+    {
+        soff16.assign(soff32.begin(), soff32.end());
+        soff16_.resize(soff16.size());
+        NEON_CHECK_HIP(hipMemcpyAsync(soff16_.data(), soff16.data(), soff16.size() * sizeof(int16_t), hipMemcpyHostToDevice, s_));
+    }
+    // the host vectors go out of scope: wait for the copies
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+}
+
 linear_operator device_mesh::op(int precision) const
 {
     linear_operator A;
@@ -1589,17 +1565,11 @@ linear_operator device_mesh::op(int precision) const
     }
     A.nb_lanes = spmv.lanes;
     A.nb_nt = (spmv.flags & NEON_SPMV_NONTEMPORAL) != 0;
-    A.nb_sweep = (spmv.flags & NEON_SPMV_SWEEP) != 0;
-    // the pipelined kernel holds one block per lane: rows of at most 32 blocks
-    A.nb_pipeline = (spmv.flags & NEON_SPMV_PIPELINE) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
-    A.nb_lds = (spmv.flags & NEON_SPMV_LDS) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
-    A.nb_stream = (spmv.flags & NEON_SPMV_STREAM) != 0 && spmv.lanes == 32 && max_degree_ <= 32;
-    A.nb_meta = (spmv.flags & NEON_SPMV_META) != 0 && spmv.lanes == 32;
     A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
     // nodes of more than 170 neighbours keep the CSR-order values (the sliced copy is packed
     // through LDS rows of 3 * max degree values)
     if ((spmv.flags & NEON_SPMV_SLICED) && sliced_fits())
-        attach_sliced(A, precision, (spmv.flags & NEON_SPMV_SLICED_WIDE) != 0, (spmv.flags & NEON_SPMV_INDEX32) != 0);
+        attach_sliced(A, precision, (spmv.flags & NEON_SPMV_INDEX32) != 0);
     return A;
 }
 
@@ -1622,10 +1592,11 @@ bool device_mesh::sliced_fits() const
     return size_t(64) * 3 * std::max<int32_t>(max_degree_, 1) * sizeof(double) <= 65536;
 }
 
-void device_mesh::attach_sliced(linear_operator& A, int precision, bool wide, bool index32) const
+void device_mesh::attach_sliced(linear_operator& A, int precision, bool index32) const
 {
     if (!sliced_fits()) throw status_error(NEON_BAD_ARGUMENT, "node degree too large for the sliced tangent copy");
-    sell_matrix const m{N_, slices_, sptr_.data(), scol_.data(), nullptr};
+    sell_matrix const m{N_, slices_, sptr_.data(), snode_.data(), slane_.data(), scode_.data(), sexc_.data(), nullptr,
+                        soff32_.data()};
     size_t const vals = size_t(sell_steps_) * 9 * 64;
     if (precision == NEON_PRECISION_FP64)
     {
@@ -1650,11 +1621,14 @@ void device_mesh::attach_sliced(linear_operator& A, int precision, bool wide, bo
         A.s32 = s32_.data();
     }
     A.sell = true;
-    A.sell_block = wide ? 1024 : 256;
     A.slices = slices_;
     A.sptr = sptr_.data();
-    A.scol = scol_.data();
-    A.scol16 = index32 || !scol16_ok_ ? nullptr : scol16_.data();
+    A.snode = snode_.data();
+    A.slane = slane_.data();
+    A.scode = scode_.data();
+    A.sexc = sexc_.data();
+    A.soff32 = soff32_.data();
+    A.soff16 = index32 || !soff16_ok_ ? nullptr : soff16_.data();
 }
 
 namespace

@@ -106,38 +106,45 @@ struct linear_operator
     const int32_t* rp = nullptr;
     const int32_t* ci = nullptr;
     int lanes_per_row = 8;
-    // node-block SpMV shape: lanes per block row (16 or 32), nontemporal value loads, XCD
-    // sweep node order, optional workgroup count (tuning)
+    // node-block SpMV shape: lanes per block row of the CSR-order kernel (16 or 32),
+    // nontemporal value loads, optional workgroup count
     int nb_lanes = 32;
     bool nb_nt = false;
-    bool nb_sweep = false;
-    bool nb_pipeline = false;
-    bool nb_lds = false;
-    bool nb_stream = false;
-    bool nb_meta = false;
     int blocks_override = 0;
     // values
     const double* v64 = nullptr;
     const float* v32 = nullptr;
     // sliced copy of the node-block values (sell_matrix layout); the SpMV uses it when set
     bool sell = false;
-    int sell_block = 256; // threads per workgroup (256 or 1024), one wavefront per slice
     int32_t slices = 0;
     const int32_t* sptr = nullptr;
-    const int32_t* scol = nullptr;
-    const int16_t* scol16 = nullptr; // 16-bit column offsets (used when set)
+    const int32_t* snode = nullptr;
+    const int32_t* slane = nullptr;
+    const int32_t* scode = nullptr;
+    const int32_t* sexc = nullptr;
+    const int16_t* soff16 = nullptr; // 16-bit per-lane offsets (used when set)
+    const int32_t* soff32 = nullptr;
     const double* s64 = nullptr;
     const float* s32 = nullptr;
 
     int spmv_blocks() const;
     // FP64: y = A x (values v64); with part, also block partials of x.y (or w.y if w).
-    void apply64(hipStream_t s, const double* x, double* y, double* part, const krylov_state* st) const;
+    // gate (with st): 1 runs only while st->refine is set, 2 only while it is clear
+    void apply64(hipStream_t s, const double* x, double* y, double* part, const krylov_state* st, int gate = 0) const;
     // FP32 values/vectors, FP32 accumulation, FP64 partials of x.y
     void apply32(hipStream_t s, const float* x, float* y, double* part, const krylov_state* st) const;
     // r = b - A x in FP64 arithmetic from the stored values (v64 if set, else v32)
     // (skipped when st is given and the state is done, or, with gate, no reliable update is due)
     void residual64(hipStream_t s, const double* b, const double* x, double* r, const krylov_state* st = nullptr,
                     int gate = 0) const;
+};
+
+// Device work vectors of n entries each for the FP64 solvers, supplied by the caller instead of
+// the engine's own (cg64 uses r, p, Ap; bicgstab64 r, p, r0, v, s, t, p_hat, s_hat).
+struct krylov_work
+{
+    double *r = nullptr, *p = nullptr, *Ap = nullptr;
+    double *r0 = nullptr, *v = nullptr, *s = nullptr, *t = nullptr, *p_hat = nullptr, *s_hat = nullptr;
 };
 
 class krylov_engine
@@ -149,10 +156,13 @@ public:
     // Jacobi inverse of the operator's diagonal (FP64 values).
     void jacobi64(const linear_operator& A, double* minv, int eigen_rule);
 
+    // work: the caller's work vectors (nothing allocated), else the engine's own (grown to n)
     solve_result cg64(const linear_operator& A, const double* b, double* x, const double* minv, int criterion,
-                      double tol, int32_t max_it);
+                      double tol, int32_t max_it, const krylov_work* work = nullptr);
     solve_result bicgstab64(const linear_operator& A, const double* b, double* x, const double* minv,
-                            int criterion, double tol, int32_t max_it);
+                            int criterion, double tol, int32_t max_it, const krylov_work* work = nullptr);
+    // copy one device double to the host through the engine's pinned word (synchronises)
+    double read_scalar(const double* d);
     // FP32 inner CG with FP64 reliable updates r = b - A x (relative rule only).
     solve_result cg_mixed(const linear_operator& A, const double* b, double* x, int criterion, double tol,
                           int32_t max_it);
@@ -178,31 +188,16 @@ public:
     // the mixed CG tests the trigger (and, through the true residual, convergence) every this
     // many iterations
     int refine_every = NEON_REFINE_EVERY;
-    // both vector phases of a CG iteration in one cooperative launch (cg_fused_update_kernel)
-    // when the grid it needs is co-resident; otherwise the two-kernel form (environment
-    // NEON_FUSED_UPDATES=0 at engine creation selects the two-kernel form everywhere)
-    bool fused_updates = true;
-    // entries per thread of the fused update (0: the smallest of 8, 16, 32 whose grid fits)
-    int fused_ept = 0;
 
 private:
-    // launches cg_fused_update_kernel on n entries; false (nothing launched) when no
-    // entries-per-thread choice gives a co-resident grid
-    template <typename T, bool MIXED>
-    bool fused_update(int64_t n, T* x, T* r, T* p, const T* Ap, const T* minv, const double* part_pAp, int npart,
-                      const krylov_state* st, krylov_state* st_next, double* part_rr, double* part_rz, int criterion,
-                      double tol, int max_it);
     void ensure(int64_t n);
     void ensure_mixed(int64_t n);
     solve_result cg_mixed_run(const linear_operator& A, const double* b, double* x, double tol, int32_t max_it, int nb,
                               int ns, double* part_pAp, double* part_rr, double* part_rz, double* part_rr_t,
                               double* part_rz_t, krylov_state* st, float* minv);
     hipStream_t s_;
-    device_array<double> part_[5];
+    device_array<double> part_[6];
     device_array<krylov_state> state_;
-    // grid barrier word of the fused update and its time-out flag (zeroed at creation)
-    device_array<uint32_t> barrier_;
-    void check_barrier();
     krylov_state* host_state_ = nullptr;
     // poll-ahead ring of run_batches: two pinned state snapshots and their events
     krylov_state* host_ring_ = nullptr;
@@ -276,12 +271,14 @@ public:
     // operator over the assembled tangent (FP64 values, or FP32 values for mixed)
     linear_operator op(int precision) const;
     // points A's SpMV at the sliced copy of the tangent (packed here when the tangent changed
-    // since the last copy); wide: 1024-thread workgroups instead of 256
-    void attach_sliced(linear_operator& A, int precision, bool wide, bool index32 = false) const;
-    // whether every sliced column fits a 16-bit offset from its row node
-    bool sliced_index16() const { return scol16_ok_; }
+    // since the last copy); index32: 32-bit per-lane offsets even where 16 bits fit
+    void attach_sliced(linear_operator& A, int precision, bool index32 = false) const;
+    // whether every per-lane offset of the sliced copy fits 16 bits
+    bool sliced_index16() const { return soff16_ok_; }
     int32_t sliced_slices() const { return slices_; }
     int64_t sliced_steps() const { return sell_steps_; }
+    // steps whose 64 lanes share one column offset
+    int64_t sliced_shared_steps() const { return shared_steps_; }
     // the sliced copy is packed through LDS: nodes of at most 170 neighbours
     bool sliced_fits() const;
     // Jacobi inverse from the node-block diagonal
@@ -325,6 +322,7 @@ public:
     kernel_profiler* prof = nullptr;
 
 private:
+    void build_sliced_pattern();
     hipStream_t s_;
     int32_t N_, E_, model_, tangent_mode_;
     int64_t nnz_ = 0;
@@ -340,10 +338,10 @@ private:
     bool k64_valid_ = false, k32_valid_ = false;
     // sliced layout of the pattern (sell_matrix) and the sliced copies of the tangent
     int32_t slices_ = 0;
-    int64_t sell_steps_ = 0;
-    device_array<int32_t> sptr_, scol_;
-    device_array<int16_t> scol16_;
-    bool scol16_ok_ = false;
+    int64_t sell_steps_ = 0, shared_steps_ = 0;
+    device_array<int32_t> sptr_, snode_, slane_, scode_, sexc_, soff32_;
+    device_array<int16_t> soff16_;
+    bool soff16_ok_ = false;
     mutable device_array<double> s64_;
     mutable device_array<float> s32_;
     mutable bool s64_fresh_ = false, s32_fresh_ = false;
@@ -364,8 +362,6 @@ private:
 void upload_hex8_tables();
 void convert_to_float(hipStream_t s, int64_t n, const double* x, float* y);
 void fill_ones(hipStream_t s, int64_t n, double* x);
-// average ms of a streaming read (copy=false) or copy (copy=true) of `bytes` bytes
-double stream_probe(hipStream_t s, int64_t bytes, int reps, bool copy);
 void fill_ones(hipStream_t s, int64_t n, float* x);
 } // namespace neon_amd
 

@@ -1,5 +1,6 @@
-// Device kernels of the Jacobi-preconditioned Krylov solvers: CSR SpMV in two layouts,
-// fused vector updates with grid reductions, and the device-resident scalar state.
+// Device kernels of the Jacobi-preconditioned Krylov solvers: SpMV in three layouts (generic
+// CSR, node-block CSR, the sliced node-block copy), vector updates fused with their dot-product
+// partials, and the device-resident scalar state.
 //
 // The algorithms follow neon's two solver families:
 //   * Eigen's ConjugateGradient behind conjugate_gradient::solve
@@ -14,7 +15,6 @@
 
 #include <cfloat>
 #include <type_traits>
-#include <utility>
 
 namespace neon_amd
 {
@@ -127,41 +127,25 @@ __device__ inline void nb_rows(nb_matrix A, const VT* __restrict__ val, const XT
         }
 }
 
-// y = A x (node-block layout); with DOT, also writes the block partial of x . y.
-// MODE 0: y = Ax.  MODE 1: y = b - Ax (residual; b given in `bvec`).
-// Node order: each XCD (workgroups b with b % 8 equal) owns one contiguous eighth of the
-// nodes.  SWEEP=false: every workgroup walks its own contiguous chunk of that eighth.
-// SWEEP=true: the workgroups of one XCD advance together through the eighth, workgroup j
-// taking node groups j, j + W, j + 2W, ... (W workgroups per XCD), so the x entries that the
-// concurrently running workgroups gather (neighbours within about one node plane) stay in
-// that XCD's L2 instead of spanning the whole eighth.
-template <int LPR, int ROWS, bool NT, bool SWEEP, typename VT, typename XT, typename YT, typename AT, bool DOT,
-          int MODE>
+// y = A x (node-block layout, CSR-order values); with DOT, also writes the block partial of
+// x . y.  MODE 0: y = Ax.  MODE 1: y = b - Ax (residual; b given in `bvec`).  Each XCD
+// (workgroups b with b % 8 equal) owns one contiguous eighth of the nodes and every workgroup
+// walks its own contiguous chunk of that eighth.  The operator for meshes whose nodes have too
+// many neighbours for the sliced copy, and for the consistent mass.
+template <int LPR, int ROWS, bool NT, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
 __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* __restrict__ val,
                                                          const XT* __restrict__ x, YT* __restrict__ y,
                                                          const double* __restrict__ bvec,
                                                          double* __restrict__ part,
                                                          const krylov_state* __restrict__ st, int gate)
 {
-    if (st && (st->done || (gate && !st->refine))) return;
+    if (st && (st->done || (gate == 1 && !st->refine) || (gate == 2 && st->refine))) return;
     constexpr int G = kBlock / LPR; // lane groups per block
     int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    int64_t start, end, stride;
-    if (SWEEP && (gridDim.x & 7) == 0)
-    {
-        int64_t const per = (int64_t(A.nodes) + 7) / 8;
-        int64_t const beg = int64_t(blockIdx.x & 7) * per;
-        end = beg + per < A.nodes ? beg + per : A.nodes;
-        stride = int64_t(gridDim.x >> 3) * G;
-        start = beg + int64_t(blockIdx.x >> 3) * G + g;
-    }
-    else
-    {
-        int64_t beg;
-        xcd_chunk(A.nodes, beg, end);
-        stride = G;
-        start = beg + g;
-    }
+    int64_t beg, end;
+    xcd_chunk(A.nodes, beg, end);
+    int64_t const stride = G;
+    int64_t const start = beg + g;
     double dot = 0.0;
     for (int64_t n0 = start; n0 < end; n0 += stride * ROWS)
     {
@@ -212,62 +196,70 @@ __global__ __launch_bounds__(kBlock) void spmv_nb_kernel(nb_matrix A, const VT* 
 }
 
 // Sliced node-block layout: a copy of the node-block values in which every wavefront load
-// reads 64 consecutive values.  Slice s holds the 64 consecutive nodes 64 s .. 64 s + 63
-// (lane = node) and sptr[s+1] - sptr[s] steps, the largest block count among them.  Step kk of
-// a node is its kk-th block (ascending column node, as in bcol); steps past the node's block
-// count, and lanes past the last node, hold zero values with the node itself (node 0 past the
-// end) as column.  Value (i, j) of step kk of slice s sits at
-//   sval[((sptr[s] + kk) * 9 + 3 i + j) * 64 + lane],  its column node at scol[(sptr[s] + kk) * 64 + lane].
-// scol16, when set, holds the same columns as 16-bit offsets from the lane's node (64 s + lane;
-// lanes past the last node point at the last node), halving the column bytes.
+// reads one contiguous span.  Slice s holds the 64 consecutive nodes 64 s ... 64 s + 63 (lane l:
+// node snode[64 s + l]; an empty lane past the last node, snode -1 - m, gathers like node m and
+// holds zero values) and sptr[s+1] - sptr[s] steps, the largest block count among them; step kk
+// of a node is its kk-th block (ascending column node, as in bcol), and steps past a node's
+// block count hold zero values.  The 9 values of a step form one record of 9 x 64 values:
+// value t = 3 i + j of lane l sits at
+//   sval[576 (sptr[s] + kk) + sell_pos<VT>(t, l)]:
+// the first 8 in groups of W = 16 / sizeof(VT) per lane (one 16-byte load per lane and group;
+// group g spans 64 W values), the ninth after them (one value per lane).
+// A step's column offsets (column node - lane node): scode[step] < 0 encodes one offset shared
+// by the slice's lanes (shared_offset()) except at most two exception lanes, named by
+// slane[s] (8 bits each, 0xff for none), whose 16-bit offsets are in sexc[step]; scode[step]
+// >= 0 indexes 64 per-lane offsets at soff16 / soff32 [scode * 64 + lane].
+template <typename VT>
+__host__ __device__ constexpr int sell_pos(int t, int lane)
+{
+    constexpr int W = 16 / int(sizeof(VT));
+    return t < 8 ? (t / W) * 64 * W + lane * W + t % W : 512 + lane;
+}
+
 struct sell_matrix
 {
     int32_t nodes;
     int32_t slices;
-    const int32_t* sptr;
-    const int32_t* scol;
-    const int16_t* scol16;
+    const int32_t* sptr;   // [slices + 1] first step of each slice
+    const int32_t* snode;  // [slices * 64] node of each lane (-1 - m: empty lane gathering like node m)
+    const int32_t* slane;  // [slices] exception lanes of a slice's shared steps (8 bits each, 0xff: none)
+    const int32_t* scode;  // [steps] shared_code(offset) or the step's index into the per-lane offsets
+    const int32_t* sexc;   // [steps] 16-bit offsets of the exception lanes (shared steps)
+    const int16_t* soff16; // per-lane offsets when every one fits 16 bits (else null)
+    const int32_t* soff32; // per-lane offsets in 32 bits (used when soff16 is null)
 };
+constexpr int64_t kSharedBias = int64_t(1) << 30;
+__host__ __device__ inline int32_t shared_code(int64_t offset)
+{
+    return int32_t(int64_t(INT32_MIN) + kSharedBias + offset);
+}
+__host__ __device__ inline int64_t shared_offset(int32_t code) { return int64_t(code) - INT32_MIN - kSharedBias; }
 
 // y = A x on the sliced layout, one wavefront per slice, one node (three rows) per lane.  Each
 // lane accumulates its three row sums over its steps in order, so no cross-lane reduction is
 // needed.  With DOT, also writes the block partial of x . y (the stored, rounded y).
-// MODE 0: y = Ax.  MODE 1: y = b - Ax.
-// Slice order: XCD=false, wavefront w of workgroup b takes slices b W + w, stride gridDim W
-// (consecutive slices on different XCDs, so every XCD's L2 fetches all of x).  XCD=true (a
-// multiple of 8 workgroups): the workgroups b with b % 8 = e (one XCD) sweep the e-th eighth
-// of the slices together, so each L2 fetches x only around its own eighth.
+// MODE 0: y = Ax.  MODE 1: y = b - Ax.  Wavefront w of workgroup b takes slices 4 b + w, stride
+// 4 gridDim (consecutive slices on different XCDs).  The slice index, its step range and each
+// step's code are wavefront-uniform (scalar loads).
 // NT: the values are read with nontemporal loads (streamed past the caches, which keep x).
-template <int BS, bool XCD, bool I16, bool NT, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
-__global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* __restrict__ val,
-                                                       const XT* __restrict__ x, YT* __restrict__ y,
-                                                       const double* __restrict__ bvec, double* __restrict__ part,
-                                                       const krylov_state* __restrict__ st, int gate)
+template <bool I16, bool NT, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(256) void spmv_sell_kernel(sell_matrix A, const VT* __restrict__ val,
+                                                        const XT* __restrict__ x, YT* __restrict__ y,
+                                                        const double* __restrict__ bvec, double* __restrict__ part,
+                                                        const krylov_state* __restrict__ st, int gate)
 {
-    if (st && (st->done || (gate && !st->refine))) return;
-    constexpr int W = BS / 64;
+    if (st && (st->done || (gate == 1 && !st->refine) || (gate == 2 && st->refine))) return;
     int const lane = threadIdx.x & 63;
-    int32_t first, stride, last;
-    if (XCD && (gridDim.x & 7) == 0)
-    {
-        int32_t const per = (A.slices + 7) / 8;
-        int32_t const beg = int32_t(blockIdx.x & 7) * per;
-        last = beg + per < A.slices ? beg + per : A.slices;
-        first = beg + int32_t(blockIdx.x >> 3) * W + int32_t(threadIdx.x >> 6);
-        stride = int32_t(gridDim.x >> 3) * W;
-    }
-    else
-    {
-        first = int32_t(blockIdx.x) * W + int32_t(threadIdx.x >> 6);
-        stride = int32_t(gridDim.x) * W;
-        last = A.slices;
-    }
+    int32_t const wave = __builtin_amdgcn_readfirstlane(int32_t(threadIdx.x >> 6));
     double dot = 0.0;
-    for (int32_t s = first; s < last; s += stride)
+    for (int32_t s = int32_t(blockIdx.x) * 4 + wave; s < A.slices; s += int32_t(gridDim.x) * 4)
     {
-        int64_t const n = int64_t(s) * 64 + lane;
-        bool const ok = n < A.nodes;
-        int64_t const row = 3 * (ok ? n : int64_t(A.nodes) - 1);
+        int32_t const node = A.snode[int64_t(s) * 64 + lane];
+        bool const ok = node >= 0;
+        int64_t const nn = ok ? int64_t(node) : -1 - int64_t(node);
+        int64_t const row = 3 * nn;
+        int32_t const exl = A.slane[s];
+        int const ex0 = exl & 0xff, ex1 = (exl >> 8) & 0xff;
         // the node's own b (residual) and x (dot) entries, requested before its blocks
         double br0 = 0.0, br1 = 0.0, br2 = 0.0;
         XT xr0 = XT(0), xr1 = XT(0), xr2 = XT(0);
@@ -290,20 +282,55 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
         // about the same); FP64 accumulation uses the first set only
         AT a0 = AT(0), a1 = AT(0), a2 = AT(0), b0 = AT(0), b1 = AT(0), b2 = AT(0);
         AT c0 = AT(0), c1 = AT(0), c2 = AT(0), d0 = AT(0), d1 = AT(0), d2 = AT(0);
+        // the codes and exception words of up to 64 steps at a time, lane j holding step
+        // kk + j's (coalesced loads); each step reads its own from that lane
+        int32_t codes = 0, excs = 0;
         for (int32_t kk = 0; kk < ks; ++kk)
         {
             int64_t const blk = int64_t(k0) + kk;
-            int64_t const m = I16 ? n + int64_t(A.scol16[blk * 64 + lane]) : int64_t(A.scol[blk * 64 + lane]);
+            if ((kk & 63) == 0)
+            {
+                bool const in = kk + lane < ks;
+                codes = in ? A.scode[blk + lane] : 0;
+                excs = in ? A.sexc[blk + lane] : 0;
+            }
+            int32_t const code = __builtin_amdgcn_readlane(codes, kk & 63);
+            int64_t m;
+            if (code < 0)
+            {
+                int32_t const e = __builtin_amdgcn_readlane(excs, kk & 63);
+                int64_t off = shared_offset(code);
+                if (lane == ex0)
+                    off = int16_t(e & 0xffff);
+                else if (lane == ex1)
+                    off = int16_t(uint32_t(e) >> 16);
+                m = nn + off;
+            }
+            else if constexpr (I16)
+                m = nn + int64_t(A.soff16[int64_t(code) * 64 + lane]);
+            else
+                m = nn + int64_t(A.soff32[int64_t(code) * 64 + lane]);
+            // the step's record: 8 values in 16-byte loads, the ninth alone
+            constexpr int W = 16 / int(sizeof(VT));
+            typedef VT vec_t __attribute__((ext_vector_type(W)));
+            const VT* rec = val + blk * 576;
             VT v[9];
 #pragma unroll
-            for (int t = 0; t < 9; ++t)
+            for (int g = 0; g < 8 / W; ++g)
             {
-                const VT* src = val + (blk * 9 + t) * 64 + lane;
+                const vec_t* src = reinterpret_cast<const vec_t*>(rec + g * 64 * W) + lane;
+                vec_t q;
                 if constexpr (NT)
-                    v[t] = __builtin_nontemporal_load(src);
+                    q = __builtin_nontemporal_load(src);
                 else
-                    v[t] = *src;
+                    q = *src;
+#pragma unroll
+                for (int j = 0; j < W; ++j) v[g * W + j] = q[j];
             }
+            if constexpr (NT)
+                v[8] = __builtin_nontemporal_load(rec + 512 + lane);
+            else
+                v[8] = rec[512 + lane];
             AT const x0 = AT(x[3 * m]), x1 = AT(x[3 * m + 1]), x2 = AT(x[3 * m + 2]);
             AT const n0 = a0 + (AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2);
             AT const n1 = a1 + (AT(v[3]) * x0 + AT(v[4]) * x1 + AT(v[5]) * x2);
@@ -336,7 +363,7 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
     }
     if constexpr (DOT)
     {
-        dot = block_sum<BS>(dot);
+        dot = block_sum<256>(dot);
         if (threadIdx.x == 0) part[blockIdx.x] = dot;
     }
 }
@@ -344,8 +371,8 @@ __global__ __launch_bounds__(BS) void spmv_sell_kernel(sell_matrix A, const VT* 
 // Sliced copy of node-block values (layout above), with coalesced reads: one 256-thread
 // workgroup per slice; for each dof row i the slice's 64 row segments (3 deg contiguous values
 // each, CSR order val[9 bptr[n] + i (3 deg) + 3 kk + j]) are read into LDS (rows of `row_len`
-// = 3 * max degree values), then written out 64 consecutive values at a time.  Steps past a
-// node's block count get zeros.  Dynamic LDS: 64 * row_len values.
+// = 3 * max degree values), then written to the slice's step records (sell_pos).  Steps past a
+// node's block count, and empty lanes, get zeros.  Dynamic LDS: 64 * row_len values.
 template <typename VT>
 __global__ __launch_bounds__(256) void sell_pack_lds_kernel(sell_matrix A, const int32_t* __restrict__ bptr,
                                                             const VT* __restrict__ val, VT* __restrict__ sval,
@@ -358,8 +385,8 @@ __global__ __launch_bounds__(256) void sell_pack_lds_kernel(sell_matrix A, const
     int const tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid < 64)
     {
-        int64_t const n = int64_t(s) * 64 + tid;
-        bool const ok = n < A.nodes;
+        int32_t const n = A.snode[int64_t(s) * 64 + tid];
+        bool const ok = n >= 0;
         sb0[tid] = ok ? bptr[n] : 0;
         sd[tid] = ok ? bptr[n + 1] - bptr[n] : 0;
     }
@@ -378,467 +405,10 @@ __global__ __launch_bounds__(256) void sell_pack_lds_kernel(sell_matrix A, const
         {
             int const kk = c / 3, j = c - 3 * (c / 3);
             VT const v = kk < sd[lane] ? sm[lane * row_len + 3 * kk + j] : VT(0);
-            sval[((int64_t(k0) + kk) * 9 + 3 * i + j) * 64 + lane] = v;
+            sval[(int64_t(k0) + kk) * 576 + sell_pos<VT>(3 * i + j, lane)] = v;
         }
         __syncthreads();
     }
-}
-
-// Node-block SpMV with the row pointers of 32 rows per load (32 lanes per block row, XCD sweep
-// order as above, any row length).  Lane t of a lane group holds bptr of the group's row
-// 32 c + t (and of the row after it) for the current chunk c, the next chunk's being requested
-// a chunk ahead, and each row takes its first block and length by a shuffle instead of a
-// dependent load: a row then waits on two memory round trips (columns with values, then the
-// x gather) instead of three.
-template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
-__global__ __launch_bounds__(kBlock, sizeof(VT) == 8 || sizeof(XT) == 8 ? 6 : 8) void spmv_nb_meta_kernel(nb_matrix A, const VT* __restrict__ val,
-                                                              const XT* __restrict__ x, YT* __restrict__ y,
-                                                              const double* __restrict__ bvec,
-                                                              double* __restrict__ part,
-                                                              const krylov_state* __restrict__ st, int gate)
-{
-    if (st && (st->done || (gate && !st->refine))) return;
-    constexpr int LPR = 32;
-    constexpr int G = kBlock / LPR;
-    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    int64_t start, end, stride;
-    if ((gridDim.x & 7) == 0)
-    {
-        int64_t const per = (int64_t(A.nodes) + 7) / 8;
-        int64_t const beg = int64_t(blockIdx.x & 7) * per;
-        end = beg + per < A.nodes ? beg + per : A.nodes;
-        stride = int64_t(gridDim.x >> 3) * G;
-        start = beg + int64_t(blockIdx.x >> 3) * G + g;
-    }
-    else
-    {
-        int64_t beg;
-        xcd_chunk(A.nodes, beg, end);
-        stride = G;
-        start = beg + g;
-    }
-    int64_t const count = start < end ? (end - start + stride - 1) / stride : 0;
-    // row pointers of row 32 c + l (clamped to the last row; past the end the row is empty)
-    auto meta = [&](int64_t c, int32_t& lo, int32_t& hi) {
-        int64_t const t = c * LPR + l;
-        int64_t const tc = t < count ? t : (count > 0 ? count - 1 : 0);
-        int64_t const n = count > 0 ? start + tc * stride : 0;
-        lo = A.bptr[n];
-        hi = t < count ? A.bptr[n + 1] : lo;
-    };
-    double dot = 0.0;
-    int32_t mlo, mhi, nlo, nhi;
-    meta(0, mlo, mhi);
-    meta(1, nlo, nhi);
-    for (int64_t t = 0; t < count; ++t)
-    {
-        int const ln = int(t % LPR);
-        if (ln == 0 && t > 0)
-        {
-            mlo = nlo;
-            mhi = nhi;
-            meta(t / LPR + 1, nlo, nhi);
-        }
-        int32_t const b0 = __shfl(mlo, ln, LPR);
-        int32_t const d = __shfl(mhi, ln, LPR) - b0;
-        int32_t const n = int32_t(start + t * stride);
-        int64_t const row = 3ll * n + (l < 3 ? l : 2);
-        double const br = MODE == 1 ? bvec[row] : 0.0;
-        XT const xr = DOT ? x[row] : XT(0);
-        bool const ok = true;
-        AT yy[1][3];
-        nb_rows<LPR, 1, false, VT, XT, AT>(A, val, x, &n, &ok, &b0, &d, l, yy);
-        AT yi = l == 1 ? yy[0][1] : yy[0][0];
-        yi = l == 2 ? yy[0][2] : yi;
-        YT const yo = MODE == 1 ? YT(br - double(yi)) : YT(yi);
-        if constexpr (DOT) dot += l < 3 ? double(xr) * double(yo) : 0.0;
-        if (l < 3) y[3ll * n + l] = yo;
-    }
-    if constexpr (DOT)
-    {
-        dot = block_sum<kBlock>(dot);
-        if (threadIdx.x == 0) part[blockIdx.x] = dot;
-    }
-}
-
-// Software-pipelined node-block SpMV (32 lanes per block row, at most 32 blocks per row,
-// XCD sweep order as above).  Iteration t of a lane group gathers x for row t, requests the
-// row pointers of row t+2 and the column indices and 9 values per lane of row t+1, and only
-// then combines row t, so each iteration exposes one memory latency with the next row's
-// loads in flight.  Every load is unconditional from a clamped, valid address and masked
-// afterwards: a load under a branch would force a wait for all outstanding loads at the join.
-template <bool NT, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
-__global__ __launch_bounds__(kBlock) void spmv_nb_pipe_kernel(nb_matrix A, const VT* __restrict__ val,
-                                                              const XT* __restrict__ x, YT* __restrict__ y,
-                                                              const double* __restrict__ bvec,
-                                                              double* __restrict__ part,
-                                                              const krylov_state* __restrict__ st, int gate)
-{
-    if (st && (st->done || (gate && !st->refine))) return;
-    constexpr int LPR = 32;
-    constexpr int G = kBlock / LPR;
-    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    int64_t start, end, stride;
-    if ((gridDim.x & 7) == 0)
-    {
-        int64_t const per = (int64_t(A.nodes) + 7) / 8;
-        int64_t const beg = int64_t(blockIdx.x & 7) * per;
-        end = beg + per < A.nodes ? beg + per : A.nodes;
-        stride = int64_t(gridDim.x >> 3) * G;
-        start = beg + int64_t(blockIdx.x >> 3) * G + g;
-    }
-    else
-    {
-        int64_t beg;
-        xcd_chunk(A.nodes, beg, end);
-        stride = G;
-        start = beg + g;
-    }
-    int64_t const count = start < end ? (end - start + stride - 1) / stride : 0;
-    double dot = 0.0;
-    if (count > 0)
-    {
-        // row pointers of row t (clamped to the last row; degree 0 past the end)
-        auto meta = [&](int64_t t, int32_t& b0, int32_t& d) {
-            int64_t const n = start + (t < count ? t : count - 1) * stride;
-            int32_t const lo = A.bptr[n], hi = A.bptr[n + 1];
-            b0 = lo;
-            d = t < count ? hi - lo : 0;
-        };
-        // lane l's block of a row: column index and 9 values; lanes past the row re-read its
-        // first block (cached) and are masked through their x entries at use
-        auto fetch = [&](int32_t b0, int32_t d, VT* v, int32_t& col, bool& on) {
-            on = l < d;
-            int const q = on ? l : 0;
-            int64_t const rs = 3ll * (d > 0 ? d : 1);
-            col = A.bcol[b0 + q];
-            __builtin_amdgcn_sched_barrier(0);
-            const VT* p = val + 9ll * b0 + 3 * q;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) v[3 * i + j] = ld<NT>(p + i * rs + j);
-        };
-        int32_t b0n, dn; // row pointers of the next row to fetch
-        // one pipeline stage: combine row t (values vc, index cc) while row t+1 is fetched into vn
-        auto stage = [&](int64_t t, const VT* vc, int32_t cc, bool oc, VT* vn, int32_t& cn, bool& on) {
-            AT xs[3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) xs[j] = AT(x[3ll * cc + j]);
-            int32_t b0nn, dnn;
-            meta(t + 2, b0nn, dnn);
-            // issue order x, pointers, indices, values: the next stage then waits only for the
-            // indices while the values are still in flight
-            __builtin_amdgcn_sched_barrier(0);
-            fetch(b0n, dn, vn, cn, on);
-            int64_t const row0 = 3 * (start + t * stride);
-            double xr = 0.0, br = 0.0;
-            if constexpr (DOT) xr = double(x[row0 + (l < 3 ? l : 2)]);
-            if constexpr (MODE == 1) br = bvec[row0 + (l < 3 ? l : 2)];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) xs[j] = oc ? xs[j] : AT(0);
-            AT s0 = AT(vc[0]) * xs[0] + AT(vc[1]) * xs[1] + AT(vc[2]) * xs[2];
-            AT s1 = AT(vc[3]) * xs[0] + AT(vc[4]) * xs[1] + AT(vc[5]) * xs[2];
-            AT s2 = AT(vc[6]) * xs[0] + AT(vc[7]) * xs[1] + AT(vc[8]) * xs[2];
-#pragma unroll
-            for (int off = LPR / 2; off > 0; off >>= 1)
-            {
-                s0 += __shfl_xor(s0, off, LPR);
-                s1 += __shfl_xor(s1, off, LPR);
-                s2 += __shfl_xor(s2, off, LPR);
-            }
-            AT const yi = l == 0 ? s0 : (l == 1 ? s1 : s2);
-            YT const yo = MODE == 1 ? YT(br - double(yi)) : YT(yi);
-            if (l < 3) y[row0 + l] = yo;
-            if constexpr (DOT) dot += l < 3 ? xr * double(yo) : 0.0;
-            b0n = b0nn;
-            dn = dnn;
-        };
-        int32_t b0, d;
-        meta(0, b0, d);
-        meta(1, b0n, dn);
-        // two register sets, alternated by a 2x unrolled loop, so no loaded value is copied
-        VT va[9], vb[9];
-        int32_t ca, cb;
-        bool oa, ob;
-        fetch(b0, d, va, ca, oa);
-        int64_t t = 0;
-        for (; t + 1 < count; t += 2)
-        {
-            stage(t, va, ca, oa, vb, cb, ob);
-            stage(t + 1, vb, cb, ob, va, ca, oa);
-        }
-        if (t < count) stage(t, va, ca, oa, vb, cb, ob);
-    }
-    if constexpr (DOT)
-    {
-        dot = block_sum<kBlock>(dot);
-        if (threadIdx.x == 0) part[blockIdx.x] = dot;
-    }
-}
-
-// LDS-staged node-block SpMV (32 lanes per block row, at most 32 blocks per row, XCD sweep
-// order).  The 8 node rows that one workgroup combines per iteration are consecutive, so their
-// values form one contiguous segment of the value array; the workgroup streams it into LDS
-// with 16-byte loads (every load instruction fully used), then each lane takes its block's
-// 9 values from LDS, gathers x and the row sums reduce across the 32 lanes as in
-// spmv_nb_kernel.  The value array must extend 16 bytes past nnz (loads round the segment end
-// up to a whole vector).
-template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
-__global__ __launch_bounds__(kBlock) void spmv_nb_lds_kernel(nb_matrix A, const VT* __restrict__ val,
-                                                             const XT* __restrict__ x, YT* __restrict__ y,
-                                                             const double* __restrict__ bvec,
-                                                             double* __restrict__ part,
-                                                             const krylov_state* __restrict__ st, int gate)
-{
-    if (st && (st->done || (gate && !st->refine))) return;
-    constexpr int LPR = 32;
-    constexpr int G = kBlock / LPR;              // node rows per iteration
-    constexpr int VPV = 16 / int(sizeof(VT));    // values per 16-byte vector
-    constexpr int CAP = G * 9 * LPR + 2 * VPV;   // segment of G rows of <= 32 blocks + alignment
-    typedef VT vec __attribute__((ext_vector_type(VPV)));
-    __shared__ __align__(16) VT sv[CAP];
-    int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    int64_t const per = (int64_t(A.nodes) + 7) / 8;
-    int64_t beg, end, step, first;
-    if ((gridDim.x & 7) == 0)
-    {
-        beg = int64_t(blockIdx.x & 7) * per;
-        end = beg + per < A.nodes ? beg + per : A.nodes;
-        step = int64_t(gridDim.x >> 3) * G;
-        first = beg + int64_t(blockIdx.x >> 3) * G;
-    }
-    else
-    {
-        xcd_chunk(A.nodes, beg, end);
-        step = G;
-        first = beg;
-    }
-    double dot = 0.0;
-    for (int64_t n0 = first; n0 < end; n0 += step)
-    {
-        int64_t const rows = end - n0 < G ? end - n0 : G;
-        int64_t const s0 = 9ll * A.bptr[n0], s1 = 9ll * A.bptr[n0 + rows];
-        int64_t const a0 = s0 & ~int64_t(VPV - 1);
-        int const nv = int((s1 - a0 + VPV - 1) / VPV);
-        const vec* src = reinterpret_cast<const vec*>(val + a0);
-        vec* dst = reinterpret_cast<vec*>(sv);
-        for (int v = threadIdx.x; v < nv; v += kBlock) dst[v] = __builtin_nontemporal_load(src + v);
-        __syncthreads();
-        if (g < rows)
-        {
-            int64_t const n = n0 + g;
-            int32_t const b0 = A.bptr[n], d = A.bptr[n + 1] - b0;
-            bool const on = l < d;
-            int const q = on ? l : 0;
-            int64_t const m = A.bcol[b0 + q];
-            AT xs[3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) xs[j] = on ? AT(x[3 * m + j]) : AT(0);
-            const VT* pv = sv + (9ll * b0 - a0) + 3 * q;
-            int const rs = 3 * d;
-            AT s0v = AT(pv[0]) * xs[0] + AT(pv[1]) * xs[1] + AT(pv[2]) * xs[2];
-            AT s1v = AT(pv[rs]) * xs[0] + AT(pv[rs + 1]) * xs[1] + AT(pv[rs + 2]) * xs[2];
-            AT s2v = AT(pv[2 * rs]) * xs[0] + AT(pv[2 * rs + 1]) * xs[1] + AT(pv[2 * rs + 2]) * xs[2];
-#pragma unroll
-            for (int off = LPR / 2; off > 0; off >>= 1)
-            {
-                s0v += __shfl_xor(s0v, off, LPR);
-                s1v += __shfl_xor(s1v, off, LPR);
-                s2v += __shfl_xor(s2v, off, LPR);
-            }
-            if (l < 3)
-            {
-                AT const yi = l == 0 ? s0v : (l == 1 ? s1v : s2v);
-                int64_t const row = 3 * n + l;
-                YT yo;
-                if constexpr (MODE == 1)
-                    yo = YT(bvec[row] - double(yi));
-                else
-                    yo = YT(yi);
-                y[row] = yo;
-                if constexpr (DOT) dot += double(x[row]) * double(yo);
-            }
-        }
-        __syncthreads();
-    }
-    if constexpr (DOT)
-    {
-        dot = block_sum<kBlock>(dot);
-        if (threadIdx.x == 0) part[blockIdx.x] = dot;
-    }
-}
-
-// Streaming node-block SpMV (32 lanes per block row, at most 32 blocks per row, XCD sweep
-// order).  Each iteration of a workgroup combines 8 consecutive node rows, whose values, block
-// columns and row pointers are contiguous segments of their arrays.  The workgroup keeps the
-// current rows' segments in LDS and, before combining them, requests the next rows' segments
-// into registers with 16-byte loads (and the row pointers of the rows after), so the value
-// stream never waits on the x gather or the reductions of the rows being combined.  The value
-// array must extend one 16-byte vector past nnz.
-template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
-__global__ __launch_bounds__(kBlock) void spmv_nb_stream_kernel(nb_matrix A, const VT* __restrict__ val,
-                                                                const XT* __restrict__ x, YT* __restrict__ y,
-                                                                const double* __restrict__ bvec,
-                                                                double* __restrict__ part,
-                                                                const krylov_state* __restrict__ st, int gate)
-{
-    if (st && (st->done || (gate && !st->refine))) return;
-    constexpr int LPR = 32;
-    constexpr int G = kBlock / LPR;                   // node rows per iteration
-    constexpr int VPV = 16 / int(sizeof(VT));         // values per 16-byte vector
-    constexpr int NV = (G * 9 * LPR + 2 * VPV) / VPV; // vectors of a segment incl. alignment
-    constexpr int PER = (NV + kBlock - 1) / kBlock;   // vectors per thread
-    typedef VT vec __attribute__((ext_vector_type(VPV)));
-    __shared__ __align__(16) VT sv[NV * VPV];
-    __shared__ int32_t sc[G * LPR];
-    __shared__ int32_t sp[2][G + 1];
-    int const t = threadIdx.x;
-    int const g = t / LPR, l = t % LPR;
-    int64_t beg, end, step, first;
-    if ((gridDim.x & 7) == 0)
-    {
-        int64_t const per = (int64_t(A.nodes) + 7) / 8;
-        beg = int64_t(blockIdx.x & 7) * per;
-        end = beg + per < A.nodes ? beg + per : A.nodes;
-        step = int64_t(gridDim.x >> 3) * G;
-        first = beg + int64_t(blockIdx.x >> 3) * G;
-    }
-    else
-    {
-        xcd_chunk(A.nodes, beg, end);
-        step = G;
-        first = beg;
-    }
-    int64_t const iters = first < end ? (end - first + step - 1) / step : 0;
-    double dot = 0.0;
-    // row pointers bptr[n0 + k], k <= G, of iteration it (clamped at the end of the range)
-    auto pointer = [&](int64_t it) -> int32_t {
-        int64_t const n0 = first + it * step;
-        int64_t const rows = end - n0 < G ? end - n0 : G;
-        return A.bptr[n0 + (t < rows ? t : rows)];
-    };
-    vec pv[PER];
-    int32_t pc = 0, pp = 0;
-    // requests of the segments of iteration it, whose row pointers are in sp[it & 1]
-    auto request = [&](int64_t it) {
-        int32_t const* q = sp[it & 1];
-        int64_t const s0 = 9ll * q[0], s1 = 9ll * q[G];
-        int64_t const a0 = s0 & ~int64_t(VPV - 1);
-        int const nv = int((s1 - a0 + VPV - 1) / VPV);
-        const vec* src = reinterpret_cast<const vec*>(val + a0);
-#pragma unroll
-        for (int k = 0; k < PER; ++k)
-        {
-            int const v = t + k * kBlock;
-            int const vc = v < nv ? v : 0;
-            pv[k] = __builtin_nontemporal_load(src + vc);
-        }
-        int const nc = q[G] - q[0];
-        pc = A.bcol[q[0] + (t < nc ? t : 0)];
-    };
-    auto store = [&](int64_t it) {
-        int32_t const* q = sp[it & 1];
-        int64_t const s0 = 9ll * q[0], s1 = 9ll * q[G];
-        int64_t const a0 = s0 & ~int64_t(VPV - 1);
-        int const nv = int((s1 - a0 + VPV - 1) / VPV);
-        vec* dst = reinterpret_cast<vec*>(sv);
-#pragma unroll
-        for (int k = 0; k < PER; ++k)
-        {
-            int const v = t + k * kBlock;
-            if (v < nv) dst[v] = pv[k];
-        }
-        if (t < q[G] - q[0]) sc[t] = pc;
-    };
-    if (iters > 0)
-    {
-        if (t <= G) sp[0][t] = pointer(0);
-        if (t <= G && iters > 1) sp[1][t] = pointer(1);
-        __syncthreads();
-        request(0);
-        store(0);
-        __syncthreads();
-        for (int64_t it = 0; it < iters; ++it)
-        {
-            bool const more = it + 1 < iters;
-            if (more) request(it + 1);
-            if (t <= G && it + 2 < iters) pp = pointer(it + 2);
-            // combine the current rows from LDS
-            int64_t const n0 = first + it * step;
-            int64_t const rows = end - n0 < G ? end - n0 : G;
-            int32_t const* q = sp[it & 1];
-            if (g < rows)
-            {
-                int64_t const n = n0 + g;
-                int32_t const b0 = q[g], d = q[g + 1] - b0;
-                bool const on = l < d;
-                int const qq = on ? l : 0;
-                int64_t const m = sc[b0 - q[0] + qq];
-                int64_t const row = 3 * n + (l < 3 ? l : 2);
-                double const xr = DOT ? double(x[row]) : 0.0;
-                double const br = MODE == 1 ? bvec[row] : 0.0;
-                AT xs[3];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) xs[j] = on ? AT(x[3 * m + j]) : AT(0);
-                int64_t const a0 = (9ll * q[0]) & ~int64_t(VPV - 1);
-                const VT* pvv = sv + (9ll * b0 - a0) + 3 * qq;
-                int const rs = 3 * d;
-                AT s0v = AT(pvv[0]) * xs[0] + AT(pvv[1]) * xs[1] + AT(pvv[2]) * xs[2];
-                AT s1v = AT(pvv[rs]) * xs[0] + AT(pvv[rs + 1]) * xs[1] + AT(pvv[rs + 2]) * xs[2];
-                AT s2v = AT(pvv[2 * rs]) * xs[0] + AT(pvv[2 * rs + 1]) * xs[1] + AT(pvv[2 * rs + 2]) * xs[2];
-#pragma unroll
-                for (int off = LPR / 2; off > 0; off >>= 1)
-                {
-                    s0v += __shfl_xor(s0v, off, LPR);
-                    s1v += __shfl_xor(s1v, off, LPR);
-                    s2v += __shfl_xor(s2v, off, LPR);
-                }
-                if (l < 3)
-                {
-                    AT const yi = l == 0 ? s0v : (l == 1 ? s1v : s2v);
-                    YT const yo = MODE == 1 ? YT(br - double(yi)) : YT(yi);
-                    y[3 * n + l] = yo;
-                    if constexpr (DOT) dot += xr * double(yo);
-                }
-            }
-            __syncthreads();
-            if (more) store(it + 1);
-            if (t <= G && it + 2 < iters) sp[it & 1][t] = pp;
-            __syncthreads();
-        }
-    }
-    if constexpr (DOT)
-    {
-        dot = block_sum<kBlock>(dot);
-        if (threadIdx.x == 0) part[blockIdx.x] = dot;
-    }
-}
-
-// streaming read of n doubles in 16-byte loads (calibration of the achievable read bandwidth)
-__global__ __launch_bounds__(kBlock) void stream_sum_kernel(int64_t n, const double* __restrict__ v,
-                                                            double* __restrict__ out)
-{
-    int64_t beg, end;
-    xcd_chunk(n / 2, beg, end);
-    typedef double d2 __attribute__((ext_vector_type(2)));
-    const d2* v2 = reinterpret_cast<const d2*>(v);
-    double s = 0.0;
-    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
-    {
-        d2 const t = __builtin_nontemporal_load(v2 + i);
-        s += t.x + t.y;
-    }
-    s = block_sum<kBlock>(s);
-    if (threadIdx.x == 0) out[blockIdx.x] = s;
-}
-
-// copy of n values (calibration of the FETCH_SIZE / WRITE_SIZE counters)
-template <typename VT>
-__global__ __launch_bounds__(kBlock) void stream_copy_kernel(int64_t n, const VT* __restrict__ a, VT* __restrict__ b)
-{
-    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i < n) b[i] = a[i];
 }
 
 // Generic CSR (any sparsity): LPR lanes per row.
@@ -849,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void spmv_csr_kernel(csr_matrix A, const VT
                                                           double* __restrict__ part,
                                                           const krylov_state* __restrict__ st, int gate)
 {
-    if (st && (st->done || (gate && !st->refine))) return;
+    if (st && (st->done || (gate == 1 && !st->refine) || (gate == 2 && st->refine))) return;
     int64_t beg, end;
     xcd_chunk(A.rows, beg, end);
     int const g = threadIdx.x / LPR, l = threadIdx.x % LPR;
@@ -1010,6 +580,28 @@ __device__ inline void store_stream(T* p, T v)
         *p = v;
 }
 
+// part_a[kMaxPartials] = the sum of part_a[0, n) (and the same for part_b when given), in one
+// 1024-thread block, in a fixed order; the consumers then read one total (partials_total with
+// n < 0) instead of every block reducing all n partials.  Skipped when the state is done.
+__global__ __launch_bounds__(1024) void totals_kernel(double* __restrict__ part_a, double* __restrict__ part_b,
+                                                       int n, const krylov_state* __restrict__ st)
+{
+    if (st && st->done) return;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < n; k += 1024)
+    {
+        a += part_a[k];
+        if (part_b) b += part_b[k];
+    }
+    a = block_sum<1024>(a);
+    if (part_b) b = block_sum<1024>(b);
+    if (threadIdx.x == 0)
+    {
+        part_a[kMaxPartials] = a;
+        if (part_b) part_b[kMaxPartials] = b;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double* __restrict__ x,
                                                               double* __restrict__ r,
                                                               const double* __restrict__ p,
@@ -1021,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double*
                                                               double* __restrict__ part_rz)
 {
     if (st->done) return;
-    double const pAp = reduce_partials<kBlock>(part_pAp, npart);
+    double const pAp = partials_total<kBlock>(part_pAp, npart);
     double const alpha = st->rz / pAp;
     int64_t beg, end;
     xcd_chunk_aligned<2>(n, beg, end);
@@ -1080,8 +672,8 @@ __global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* 
         if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
         return;
     }
-    double const rr = reduce_partials<kBlock>(part_rr, npart);
-    double const rz = reduce_partials<kBlock>(part_rz, npart);
+    double const rr = partials_total<kBlock>(part_rr, npart);
+    double const rz = partials_total<kBlock>(part_rz, npart);
     int32_t it_new;
     int32_t done = KRYLOV_RUNNING;
     if (criterion == NEON_STOP_RELATIVE)
@@ -1128,216 +720,262 @@ __global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* 
     }
 }
 
-// --------------------------------------------- fused CG vector update (one launch)
-// Both vector phases of a CG iteration in one cooperative launch, with a grid barrier between
-// them for r.z:
-//   phase 1: alpha = rz / p.Ap;  x += alpha p;  r -= alpha Ap;  z = M^-1 r;  partials of r.r, r.z
-//   phase 2: convergence decision and beta = rz_new / rz;  p = z + beta p.
-// Thread g of the grid owns the 16-byte vectors g, g + T, g + 2T, ... (T threads in all) in
-// both phases and keeps z (and, with KEEP_P, the old p) of its vectors in registers across
-// the barrier, so phase 2 writes p without reading minv and r again (and, with KEEP_P, without
-// reading p): per iteration 8 (KEEP_P) or 9 vector passes instead of the two kernels' 11.
-// The grid is at most the number of co-resident workgroups (cooperative launch), and
-// EPT entries per thread cover n.  MIXED: the FP32 recurrence of the mixed-precision CG on an
-// iteration without a reliable-update test (no convergence decision here, as in
-// mixed_update_p_kernel with decided = 0); otherwise the FP64 CG with `criterion`.
-template <typename T>
-struct vec16_of;
-template <>
-struct vec16_of<double>
-{
-    typedef double2v type;
-    static constexpr int width = 2;
-};
-template <>
-struct vec16_of<float>
-{
-    typedef float4v type;
-    static constexpr int width = 4;
-};
 
-// Waves per SIMD the fused update is compiled for (register budget 512 / waves): with EPT
-// 8 / 16 / 32 entries per thread, 8 / 7 / 4 waves, so that the grid of n / (256 EPT)
-// workgroups fits co-resident on 256 compute units up to 8.4 M entries (EPT 32).
-template <int EPT>
-constexpr int fused_waves()
+// ------------------------------------------------------------- BiCGSTAB (FP64)
+// Device-resident right-preconditioned BiCGSTAB: every scalar stays in one krylov_state, which
+// single-workgroup "decision" kernels update between the grid kernels that read it (kernel
+// boundaries order the two), and the host polls it like the CG loop.  Two stopping rules:
+//   CUDA (biconjugate_gradient_stabilised_cuda.cpp:62-191): the loop runs while
+//     sqrt(r0 . r) > tol and k < max; it stops early when ||s|| < tol (x += alpha p_hat);
+//     omega == 0 is a breakdown; k >= max after the loop fails even if the last test passed.
+//   Eigen (BiCGSTAB.h as called by linear_solver.cpp:111-138): while |r|^2 > tol^2 |b|^2 and
+//     i < max; rho = r0 . r, with a restart (r = b - A x, r0 = r, i = 0 the first time) when
+//     |rho| < eps^2 |r0|^2; omega = t.s / t.t (0 when t.t = 0).
+// State fields: rz = rho (CUDA's "residual" r0.r), aux0 = rho_old, alpha, omega, rr = |r|^2
+// (Eigen), rhs2 = |b|^2, threshold = tol^2 |b|^2, aux1 = |s| (CUDA) or |r0|^2 (Eigen),
+// refine = 1 while this iteration's early exit (CUDA) or restart (Eigen) is due, it = k / i.
+enum : int
 {
-    return EPT <= 8 ? 8 : EPT <= 16 ? 7 : 4;
+    BICG_GATE_NONE = 0,
+    BICG_GATE_IF_FLAG = 1,     // run only when refine is set
+    BICG_GATE_UNLESS_FLAG = 2, // run only when refine is clear
+};
+__device__ inline bool bicg_skip(const krylov_state* st, int gate)
+{
+    return st->done || (gate == BICG_GATE_IF_FLAG && !st->refine) || (gate == BICG_GATE_UNLESS_FLAG && st->refine);
 }
 
-// Grid barrier of a cooperative launch (every workgroup co-resident).  bar[0] counts arrivals:
-// workgroup 0 adds 2^31 - (nblocks - 1), the others 1, so the word's top bit flips exactly
-// when the last workgroup arrives and its low bits return to their value (the word needs no
-// reset between barriers).  Each workgroup's writes before the barrier are released by an
-// agent-scope fence and acquired after it.  The wait is bounded (a poll is a coherent load of
-// about a microsecond; a barrier normally completes within tens of microseconds): after
-// kBarrierSpins polls the workgroup sets bar[1] and goes on; the fused update then ends the
-// solve as non-finite and the host reports the time-out.
-constexpr uint32_t kBarrierSpins = 1u << 17;
-__device__ inline void grid_barrier(uint32_t* bar)
+// partials of a.b (and of c.d when c is set), skipped per gate
+__global__ __launch_bounds__(kBlock) void bicg_dot_kernel(int64_t n, const double* __restrict__ a,
+                                                          const double* __restrict__ b, const double* __restrict__ c,
+                                                          const double* __restrict__ d, double* __restrict__ part_ab,
+                                                          double* __restrict__ part_cd,
+                                                          const krylov_state* __restrict__ st, int gate)
 {
-    __syncthreads();
+    if (bicg_skip(st, gate)) return;
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double s = 0.0, t = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        s += a[i] * b[i];
+        if (c) t += c[i] * d[i];
+    }
+    s = block_sum<kBlock>(s);
+    if (c) t = block_sum<kBlock>(t);
     if (threadIdx.x == 0)
     {
-        uint32_t const add = blockIdx.x == 0 ? 0x80000000u - (gridDim.x - 1u) : 1u;
-        __threadfence();
-        uint32_t const old = __hip_atomic_fetch_add(bar, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t spins = 0;
-        while (((old ^ __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 0x80000000u) == 0u)
-        {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins == kBarrierSpins)
-            {
-                __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-        }
-        __threadfence();
+        part_ab[blockIdx.x] = s;
+        if (c) part_cd[blockIdx.x] = t;
     }
-    __syncthreads();
 }
 
-// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, K - 1>{}): a loop
-// whose index is a constant expression in every copy, so arrays indexed by it stay in registers
-template <typename F, int... I>
-__device__ inline void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+// p = r (first iteration of the CUDA rule) or p = r + beta (p - omega v), beta = (rho / rho_old)
+// (alpha / omega); p_hat = M^-1 p
+__global__ __launch_bounds__(kBlock) void bicg_p_kernel(int64_t n, double* __restrict__ p,
+                                                        const double* __restrict__ r, const double* __restrict__ v,
+                                                        const double* __restrict__ minv, double* __restrict__ p_hat,
+                                                        const krylov_state* __restrict__ st, int first_plain)
 {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int K, typename F>
-__device__ inline void static_for(F&& f)
-{
-    static_for_impl(f, std::make_integer_sequence<int, K>{});
-}
-
-template <typename T, int EPT, bool KEEP_P, bool MIXED>
-__global__ __launch_bounds__(kBlock, fused_waves<EPT>()) void cg_fused_update_kernel(
-    int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ Ap,
-    const T* __restrict__ minv, const double* __restrict__ part_pAp, int npart, const krylov_state* __restrict__ st,
-    krylov_state* __restrict__ st_next, double* __restrict__ part_rr, double* __restrict__ part_rz, int criterion,
-    double tol, int max_it, uint32_t* __restrict__ bar)
-{
-    typedef typename vec16_of<T>::type V;
-    constexpr int VW = vec16_of<T>::width;
-    constexpr int K = EPT / VW;
-    static_assert(EPT % VW == 0, "entries per thread: whole 16-byte vectors");
-    krylov_state s = *st;
-    if (s.done)
+    krylov_state const s = *st;
+    if (s.done) return;
+    bool const plain = first_plain && s.it == 0;
+    double const beta = (s.rz / s.aux0) * (s.alpha / s.omega);
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
+        double const pi = plain ? r[i] : r[i] + beta * (p[i] - s.omega * v[i]);
+        p[i] = pi;
+        p_hat[i] = minv[i] * pi;
+    }
+}
+
+// alpha = rho / (r0 . v); s = r - alpha v; s_hat = M^-1 s; partials of s.s
+__global__ __launch_bounds__(kBlock) void bicg_s_kernel(int64_t n, const double* __restrict__ r,
+                                                        const double* __restrict__ v, const double* __restrict__ minv,
+                                                        double* __restrict__ sv, double* __restrict__ s_hat,
+                                                        const double* __restrict__ part_r0v, int npart,
+                                                        double* __restrict__ part_ss,
+                                                        const krylov_state* __restrict__ st)
+{
+    if (st->done) return;
+    double const alpha = st->rz / reduce_partials<kBlock>(part_r0v, npart);
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double ss = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        double const si = r[i] - alpha * v[i];
+        sv[i] = si;
+        s_hat[i] = minv[i] * si;
+        ss += si * si;
+    }
+    ss = block_sum<kBlock>(ss);
+    if (threadIdx.x == 0) part_ss[blockIdx.x] = ss;
+}
+
+// single block: alpha, |s| and the CUDA rule's early exit (refine = 1 when |s| < tol)
+__global__ __launch_bounds__(kBlock) void bicg_alpha_kernel(const double* __restrict__ part_r0v,
+                                                            const double* __restrict__ part_ss, int npart,
+                                                            krylov_state* __restrict__ st, int cuda_rule, double tol)
+{
+    if (st->done) return;
+    double const r0v = reduce_partials<kBlock>(part_r0v, npart);
+    double const ss = reduce_partials<kBlock>(part_ss, npart);
+    if (threadIdx.x != 0) return;
+    st->alpha = st->rz / r0v;
+    st->aux1 = sqrt(ss);
+    st->refine = cuda_rule && sqrt(ss) < tol ? 1 : 0;
+}
+
+// x += alpha p_hat (+ omega s_hat); unless the early exit is due: omega from t.s and t.t,
+// r = s - omega t, partials of r0.r and r.r
+__global__ __launch_bounds__(kBlock) void bicg_update_kernel(int64_t n, double* __restrict__ x,
+                                                             double* __restrict__ r, const double* __restrict__ r0,
+                                                             const double* __restrict__ p_hat,
+                                                             const double* __restrict__ s_hat,
+                                                             const double* __restrict__ sv,
+                                                             const double* __restrict__ t,
+                                                             const double* __restrict__ part_ts,
+                                                             const double* __restrict__ part_tt, int npart,
+                                                             double* __restrict__ part_r0r,
+                                                             double* __restrict__ part_rr,
+                                                             const krylov_state* __restrict__ st, int cuda_rule)
+{
+    krylov_state const s = *st;
+    if (s.done) return;
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    if (s.refine)
+    {
+        for (int64_t i = beg + threadIdx.x; i < end; i += kBlock) x[i] += s.alpha * p_hat[i];
         return;
     }
-    double const pAp = reduce_partials<kBlock>(part_pAp, npart);
-    T const alpha = T(s.rz / pAp);
-    int64_t const total = int64_t(gridDim.x) * kBlock;
-    int64_t const g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    int64_t const full = n / VW; // whole vectors; the last thread of the grid takes the rest
-    bool const tail = g == total - 1 && full * VW < n;
-    V zk[K], pk[KEEP_P ? K : 1];
-    double rr = 0.0, rz = 0.0;
-    static_for<K>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        int64_t const v = g + k * total;
-        if (v < full)
+    double const ts = reduce_partials<kBlock>(part_ts, npart), tt = reduce_partials<kBlock>(part_tt, npart);
+    double const omega = cuda_rule ? ts / tt : (tt > 0.0 ? ts / tt : 0.0);
+    double r0r = 0.0, rr = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        if (cuda_rule)
         {
-            V xv = load_stream<kVectorNT>(reinterpret_cast<const V*>(x) + v);
-            V rv = reinterpret_cast<const V*>(r)[v];
-            V const pv = reinterpret_cast<const V*>(p)[v];
-            V const av = load_stream<kVectorNT>(reinterpret_cast<const V*>(Ap) + v);
-            V const mv = reinterpret_cast<const V*>(minv)[v];
-            xv += alpha * pv;
-            rv = rv - alpha * av;
-            V const zv = mv * rv;
-            static_for<VW>([&](auto jc) {
-                constexpr int jj = decltype(jc)::value;
-                rr += double(rv[jj]) * double(rv[jj]);
-                rz += double(rv[jj]) * double(zv[jj]);
-            });
-            zk[k] = zv;
-            if constexpr (KEEP_P) pk[k] = pv;
-            store_stream<kVectorNT>(reinterpret_cast<V*>(x) + v, xv);
-            reinterpret_cast<V*>(r)[v] = rv;
+            // x = x + alpha p_hat + omega s_hat as two updates (the two cublasDaxpy calls)
+            double xi = x[i] + s.alpha * p_hat[i];
+            x[i] = xi + omega * s_hat[i];
         }
-        // one vector group's loads at a time: hoisting every group's loads ahead (the
-        // scheduler's choice) needs more registers than the co-residency budget and spills
-        __builtin_amdgcn_sched_barrier(0);
-    });
-    if (tail)
-        for (int64_t i = full * VW; i < n; ++i)
-        {
-            x[i] += alpha * p[i];
-            T const ri = r[i] - alpha * Ap[i];
-            r[i] = ri;
-            T const zi = minv[i] * ri;
-            rr += double(ri) * double(ri);
-            rz += double(ri) * double(zi);
-        }
+        else
+            x[i] += s.alpha * p_hat[i] + omega * s_hat[i]; // x += (alpha y + w z)
+        double const ri = sv[i] - omega * t[i];
+        r[i] = ri;
+        r0r += r0[i] * ri;
+        rr += ri * ri;
+    }
+    r0r = block_sum<kBlock>(r0r);
     rr = block_sum<kBlock>(rr);
-    rz = block_sum<kBlock>(rz);
     if (threadIdx.x == 0)
     {
+        part_r0r[blockIdx.x] = r0r;
         part_rr[blockIdx.x] = rr;
-        part_rz[blockIdx.x] = rz;
     }
-    grid_barrier(bar);
-    rr = reduce_partials<kBlock>(part_rr, gridDim.x);
-    rz = reduce_partials<kBlock>(part_rz, gridDim.x);
-    int32_t it_new;
-    int32_t done = KRYLOV_RUNNING;
-    if (MIXED)
+}
+
+// single block: the end of an iteration (counts, scalars, the loop tests)
+__global__ __launch_bounds__(kBlock) void bicg_finish_kernel(const double* __restrict__ part_ts,
+                                                             const double* __restrict__ part_tt,
+                                                             const double* __restrict__ part_r0r,
+                                                             const double* __restrict__ part_rr, int npart,
+                                                             krylov_state* __restrict__ st, int cuda_rule,
+                                                             double tol, int max_it)
+{
+    if (st->done) return;
+    double ts = 0.0, tt = 0.0, r0r = 0.0, rr = 0.0;
+    if (!st->refine)
     {
-        it_new = s.it + 1;
-        if (it_new >= max_it) done = KRYLOV_LIMIT;
+        ts = reduce_partials<kBlock>(part_ts, npart);
+        tt = reduce_partials<kBlock>(part_tt, npart);
+        r0r = reduce_partials<kBlock>(part_r0r, npart);
+        rr = reduce_partials<kBlock>(part_rr, npart);
     }
-    else if (criterion == NEON_STOP_RELATIVE)
+    if (threadIdx.x != 0) return;
+    krylov_state s = *st;
+    int32_t const k = s.it + 1;
+    if (cuda_rule)
     {
-        bool const conv = rr < s.threshold; // Eigen breaks before counting the iteration
-        it_new = conv ? s.it : s.it + 1;
-        if (conv)
-            done = KRYLOV_CONVERGED;
-        else if (it_new >= max_it)
-            done = KRYLOV_LIMIT;
+        if (s.refine)
+        {
+            // ||s|| < tol: x += alpha p_hat, residual = ||s||^2, leave the loop
+            s.rz = s.aux1 * s.aux1;
+            s.done = k >= max_it ? KRYLOV_LIMIT : KRYLOV_CONVERGED;
+        }
+        else
+        {
+            s.omega = ts / tt;
+            s.aux0 = s.rz;
+            s.rz = r0r;
+            if (s.omega == 0.0) // is_approx(omega, 0.0): an exact zero
+                s.done = KRYLOV_BREAKDOWN;
+            else if (!(sqrt(s.rz) > tol) || k >= max_it)
+                s.done = k >= max_it ? KRYLOV_LIMIT : KRYLOV_CONVERGED;
+            if (!isfinite(s.rz) && s.done != KRYLOV_BREAKDOWN) s.done = KRYLOV_NONFINITE;
+        }
+        s.rr = rr;
     }
     else
     {
-        it_new = s.it + 1; // the CUDA loop counts at the top
-        if (it_new >= max_it)
-            done = KRYLOV_LIMIT;
-        else if (!(sqrt(rz) > tol))
-            done = KRYLOV_CONVERGED;
-    }
-    if (!isfinite(rr) || !isfinite(rz)) done = KRYLOV_NONFINITE;
-    if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) done = KRYLOV_NONFINITE;
-    double const beta64 = rz / s.rz;
-    T const beta = T(beta64);
-    if (done == KRYLOV_RUNNING)
-    {
-        static_for<K>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            int64_t const v = g + k * total;
-            if (v < full)
-            {
-                V pv;
-                if constexpr (KEEP_P)
-                    pv = pk[k];
-                else
-                    pv = reinterpret_cast<const V*>(p)[v];
-                reinterpret_cast<V*>(p)[v] = zk[k] + beta * pv;
-            }
-        });
-        if (tail)
-            for (int64_t i = full * VW; i < n; ++i) p[i] = minv[i] * r[i] + beta * p[i];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-    {
+        s.omega = tt > 0.0 ? ts / tt : 0.0;
         s.rr = rr;
-        s.rz = rz;
-        s.beta = beta64;
-        s.it = it_new;
-        s.done = done;
-        if (MIXED) s.refine = 0;
-        *st_next = s;
+        s.aux0 = s.rz; // rho_old for the next iteration
+        s.rz = r0r;    // r0 . r, the next iteration's rho unless it restarts
+        if (!(rr > s.threshold))
+            s.done = KRYLOV_CONVERGED;
+        else if (k >= max_it)
+            s.done = KRYLOV_LIMIT;
+        if (!isfinite(rr)) s.done = KRYLOV_NONFINITE;
     }
+    s.it = k;
+    s.refine = 0;
+    *st = s;
+}
+
+// Eigen rule, single block at the top of an iteration: the restart test |rho| < eps^2 |r0|^2
+// (refine = 1 when a restart is due)
+__global__ __launch_bounds__(kBlock) void bicg_restart_test_kernel(krylov_state* __restrict__ st)
+{
+    if (threadIdx.x != 0 || st->done) return;
+    double const eps2 = DBL_EPSILON * DBL_EPSILON;
+    st->refine = fabs(st->rz) < eps2 * st->aux1 ? 1 : 0;
+}
+
+// Eigen rule, restart (gated on refine): r0 = r (the fresh residual), partials of r.r
+__global__ __launch_bounds__(kBlock) void bicg_restart_kernel(int64_t n, const double* __restrict__ r,
+                                                              double* __restrict__ r0, double* __restrict__ part_rr,
+                                                              const krylov_state* __restrict__ st)
+{
+    if (bicg_skip(st, BICG_GATE_IF_FLAG)) return;
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double rr = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        r0[i] = r[i];
+        rr += r[i] * r[i];
+    }
+    rr = block_sum<kBlock>(rr);
+    if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+}
+
+// Eigen rule, single block after a restart: rho = |r0|^2 = |r|^2, and the first restart resets
+// the iteration count
+__global__ __launch_bounds__(kBlock) void bicg_restart_finish_kernel(const double* __restrict__ part_rr, int npart,
+                                                                     krylov_state* __restrict__ st)
+{
+    if (st->done || !st->refine) return;
+    double const rr = reduce_partials<kBlock>(part_rr, npart);
+    if (threadIdx.x != 0) return;
+    st->rz = rr;
+    st->aux1 = rr;
+    if (st->restarts++ == 0) st->it = 0;
+    st->refine = 0;
 }
 
 // ------------------------------------------------------- generic vector kernels
@@ -1466,7 +1104,7 @@ __global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, floa
                                                                  double* __restrict__ part_rz)
 {
     if (st->done) return;
-    double const pAp = reduce_partials<kBlock>(part_pAp, npart);
+    double const pAp = partials_total<kBlock>(part_pAp, npart);
     float const alpha = float(st->rz / pAp);
     int64_t beg, end;
     xcd_chunk_aligned<4>(n, beg, end);
@@ -1517,8 +1155,8 @@ __global__ __launch_bounds__(kBlock) void mixed_decide_kernel(const double* __re
                                                               krylov_state* __restrict__ st, double delta)
 {
     if (st->done) return;
-    double const rr = reduce_partials<kBlock>(part_rr, npart);
-    double const rz = reduce_partials<kBlock>(part_rz, npart);
+    double const rr = partials_total<kBlock>(part_rr, npart);
+    double const rz = partials_total<kBlock>(part_rz, npart);
     if (threadIdx.x != 0) return;
     bool const refine = rr < st->threshold || sqrt(rr) < delta * st->aux0 || !isfinite(rr) || !isfinite(rz);
     st->refine = refine ? 1 : 0;
@@ -1580,7 +1218,8 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
                                                                 const double* __restrict__ part_rz_true,
                                                                 const double* __restrict__ part_rr,
                                                                 const double* __restrict__ part_rz, int npart,
-                                                                int decided, const krylov_state* __restrict__ st,
+                                                                int npart_true, int decided,
+                                                                const krylov_state* __restrict__ st,
                                                                 krylov_state* __restrict__ st_next, int max_it)
 {
     krylov_state s = *st;
@@ -1592,15 +1231,15 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
     double rr = s.rr, rz = s.aux1;
     if (!decided)
     {
-        rr = reduce_partials<kBlock>(part_rr, npart);
-        rz = reduce_partials<kBlock>(part_rz, npart);
+        rr = partials_total<kBlock>(part_rr, npart);
+        rz = partials_total<kBlock>(part_rz, npart);
         s.refine = 0;
     }
     bool conv = false;
     if (s.refine)
     {
-        rr = reduce_partials<kBlock>(part_rr_true, npart);
-        rz = reduce_partials<kBlock>(part_rz_true, npart);
+        rr = reduce_partials<kBlock>(part_rr_true, npart_true);
+        rz = reduce_partials<kBlock>(part_rz_true, npart_true);
         conv = rr < s.threshold;
         s.aux0 = sqrt(rr);
         s.restarts += 1; // counts reliable updates

@@ -114,10 +114,12 @@ def cube_clamped(n=10, mu=0.4, bulk=2.0, top=0.01, steps=5, **sim):
                    simulation(steps=steps, **sim), (n, n, n))
 
 
-def block_rollers(n, stretch=0.2, steps=10, jitter=0.1, mu_seed=SEED_MU, **sim):
+def block_rollers(n, stretch=0.2, steps=10, jitter=0.1, mu_seed=SEED_MU, dims=None, **sim):
     """Jittered n^3 neo-Hooke block, mu_e ~ U[0.5, 1.5] MPa, K = 10 mu; rollers on x=0, y=0,
-    z=0 and top u_z = stretch over 10 equal steps (the 64^3 / 128^3 cases)."""
-    X, conn, ijk = box_mesh(n, n, n, jitter=jitter, seed=SEED_JITTER)
+    z=0 and top u_z = stretch over 10 equal steps (the 64^3 / 128^3 cases).  dims = (nx, ny,
+    nz) cells gives a box of unit cells instead (top at z = nz)."""
+    nx, ny, nz = dims if dims is not None else (n, n, n)
+    X, conn, ijk = box_mesh(nx, ny, nz, L=(nx / n, ny / n, nz / n), jitter=jitter, seed=SEED_JITTER)
     E = conn.shape[0]
     mu = uniform(mu_seed, E, 0.5, 1.5)
     params = np.zeros((E, 4))
@@ -127,10 +129,11 @@ def block_rollers(n, stretch=0.2, steps=10, jitter=0.1, mu_seed=SEED_MU, **sim):
         {"name": "x0", "dofs": face_dofs(ijk, 0, 0, 0), "times": [0.0, 1.0], "values": [0.0, 0.0]},
         {"name": "y0", "dofs": face_dofs(ijk, 1, 0, 1), "times": [0.0, 1.0], "values": [0.0, 0.0]},
         {"name": "z0", "dofs": face_dofs(ijk, 2, 0, 2), "times": [0.0, 1.0], "values": [0.0, 0.0]},
-        {"name": "top", "dofs": face_dofs(ijk, 2, n, 2), "times": [0.0, 1.0], "values": [0.0, stretch]},
+        {"name": "top", "dofs": face_dofs(ijk, 2, nz, 2), "times": [0.0, 1.0], "values": [0.0, stretch]},
     ]
-    return Problem(f"block{n}", X, conn, ijk, params, _capi.MODEL_NEOHOOKE, bcs,
-                   simulation(steps=steps, **sim), (n, n, n))
+    name = f"block{n}" if dims is None else f"block{nx}x{ny}x{nz}"
+    return Problem(name, X, conn, ijk, params, _capi.MODEL_NEOHOOKE, bcs, simulation(steps=steps, **sim),
+                   (nx, ny, nz))
 
 
 def notched_plate_j2(nx=128, ny=64, nz=64, strain=0.005, steps=20, E_mod=200.0e3, nu=0.3, H=2000.0,

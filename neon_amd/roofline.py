@@ -3,13 +3,12 @@
     python -m neon_amd.roofline --n 128 --precision mixed --reps 20
 
 builds the n^3 block (the bench's mesh), assembles the tangent at u = 0 in the requested
-precision, runs `reps` SpMVs in the solvers' form (`neon_mesh_spmv_bench`: the sliced copy,
-nontemporal value loads, x . y block partials; the CG loop's kernel instance) and a
-streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the calibration kernel for the
-FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the mean SpMV time, its
-algorithmic bytes (values, block columns (2 bytes each as 16-bit offsets), x read once, y
-written once)
-and the achieved fraction of the 8 TB/s HBM peak.  Run it under `rocprofv3 --kernel-trace
+precision, runs `reps` SpMVs in the solvers' form (`neon_mesh_spmv_bench` of the measurement library
+libneon_hip_measure.so: the sliced copy, nontemporal value loads, x . y block partials; the CG
+loop's kernel instance) and a streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the
+calibration kernel for the FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the
+mean SpMV time, its algorithmic bytes (`spmv_bytes`) and the achieved fraction of the 8 TB/s
+HBM peak.  Run it under `rocprofv3 --kernel-trace
 --stats` or `--pmc` to read the same kernels' durations and traffic counters.
 """
 from __future__ import annotations
@@ -26,11 +25,11 @@ from neon_amd import _capi, problems
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(nnz: int, nodes: int, value_bytes: int, index_bytes: int = 4) -> int:
-    """Values once, one block column per 9 values (`index_bytes` each: 2 for the sliced copy's
-    16-bit offsets, 4 for int32 columns), x read once, y written once."""
+def spmv_bytes(nnz: int, nodes: int, value_bytes: int) -> int:
+    """Algorithmic bytes of one node-block SpMV: the nonzero values once, one 16-bit column
+    offset per 3x3 block, x read once, y written once (as bench.py counts them)."""
     n = 3 * nodes
-    return value_bytes * nnz + index_bytes * (nnz // 9) + 2 * value_bytes * n
+    return value_bytes * nnz + 2 * (nnz // 9) + 2 * value_bytes * n
 
 
 def main(argv=None) -> dict:
@@ -50,19 +49,20 @@ def main(argv=None) -> dict:
     prec = _capi.PRECISION_FP64 if args.precision == "fp64" else _capi.PRECISION_MIXED
     mesh.assemble(prec)
     ms = C.c_double()
-    lib = _capi.lib()
+    lib = _capi.measure_lib()
     # the solvers' form exactly: the sliced copy with nontemporal value loads and the x . y block
     # partials (the same kernel instance as inside the CG loop), workgroups by size; --dirty puts
     # a write of 4 vectors of n before every product, as the CG vector updates do (timed apart)
-    flags = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL | _capi.SPMV_PROBE_DOT
+    flags = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL | _capi.MEASURE_DOT
     if args.dirty:
-        flags |= _capi.SPMV_PROBE_DIRTY
+        flags |= _capi.MEASURE_DIRTY
     cfg = dict(lanes=32, flags=flags, blocks=0)
     ctx.check(lib.neon_mesh_spmv_bench(ctx.handle, prec, cfg["lanes"], cfg["flags"], cfg["blocks"], args.reps,
                                        C.byref(ms)))
     nnz = mesh.nnz()
     vb = 8 if prec == _capi.PRECISION_FP64 else 4
-    b = spmv_bytes(nnz, p.n_nodes, vb, 2 if mesh.sliced_info()["index16"] else 4)
+    sliced = mesh.sliced_info()
+    b = spmv_bytes(nnz, p.n_nodes, vb)
     gbs = b / (ms.value * 1e-3) / 1e9
     read_ms, copy_ms = C.c_double(), C.c_double()
     copy_bytes = args.copy_mb << 20
@@ -74,6 +74,7 @@ def main(argv=None) -> dict:
         "dofs": p.n_dofs,
         "nnz": nnz,
         "shape": cfg,
+        "sliced": sliced,
         "spmv_avg_ms": ms.value,
         "spmv_algorithmic_bytes": b,
         "spmv_gbs": gbs,

@@ -12,8 +12,8 @@ from neon_amd import _capi, problems
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 
 
-def declared_functions():
-    text = (ROOT / "include" / "neon_hip.h").read_text()
+def declared_functions(header="neon_hip.h"):
+    text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(neon_[a-z0-9_]+)\s*\(", text)))
 
@@ -26,6 +26,18 @@ def test_library_exports_every_declared_symbol():
     assert not missing
     # and the Python binding declares a signature for each
     assert set(names) <= set(_capi.SIGNATURES)
+
+
+def test_probes_live_in_the_measurement_library_only():
+    # the timing probes are exported by libneon_hip_measure.so, which loads against the
+    # product library, and not by the product library itself
+    names = declared_functions("neon_hip_measure.h")
+    assert len(names) == 4
+    measure = _capi.measure_lib()
+    assert all(hasattr(measure, n) for n in names)
+    assert set(names) == set(_capi.MEASURE_SIGNATURES)
+    product = _capi.lib()
+    assert not any(hasattr(product, n) for n in names)
 
 
 def test_no_device_is_a_loud_error_here():

@@ -149,27 +149,78 @@ def test_mixed_newton_block(ctx):
 
 
 SPMV_SHAPES = [
-    (32, 0), (16, 0), (32, neon_amd.SPMV_SWEEP), (32, neon_amd.SPMV_SWEEP | neon_amd.SPMV_NONTEMPORAL),
-    (32, neon_amd.SPMV_PIPELINE), (32, neon_amd.SPMV_PIPELINE | neon_amd.SPMV_NONTEMPORAL),
-    (32, neon_amd.SPMV_LDS), (32, neon_amd.SPMV_STREAM), (32, neon_amd.SPMV_META),
-    (32, neon_amd.SPMV_SLICED), (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SLICED_WIDE),
-    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL),
-    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL | neon_amd.SPMV_SWEEP),
+    (32, 0), (16, 0), (32, neon_amd.SPMV_NONTEMPORAL), (16, neon_amd.SPMV_NONTEMPORAL),
+    (32, neon_amd.SPMV_SLICED), (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL | neon_amd.SPMV_INDEX32),
-    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP),
-    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SWEEP | neon_amd.SPMV_SLICED_WIDE),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_INDEX32),
-    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_INDEX32 | neon_amd.SPMV_SLICED_WIDE),
 ]
 
 
-@pytest.mark.parametrize("precision", [neon_amd.PRECISION_FP64, neon_amd.PRECISION_MIXED])
-def test_spmv_shapes_agree(ctx, precision):
-    # every SpMV shape (lanes, XCD sweep, nontemporal, pipelined, LDS-staged, streaming, sliced) solves the device tangent to
-    # the same answer as the plain shape and with the same Krylov count up to rounding;
-    # each solve's true residual meets the tolerance
-    p = problems.block_rollers(12)
+# The sliced copy's two kinds of slice: on the 12^3-cell block (13-node rows) every 64-node
+# slice spans several rows and holds more than two node signatures, so all steps keep per-lane
+# offsets; on the 120 x 3 x 3 box (121-node rows) two thirds of the slices lie in one row, or
+# straddle two rows of one kind, and share each step's offset but for at most two exception
+# lanes (the row ends).
+SLICED_MESHES = [dict(n=12), dict(n=12, dims=(120, 3, 3))]
+
+
+def _sliced_mesh(ctx, spec):
+    p = problems.block_rollers(**spec)
     mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    info = mg.sliced_info()
+    assert info["index16"] and info["slices"] == (p.n_nodes + 63) // 64, info
+    if "dims" in spec:
+        assert 0.6 * info["steps"] < info["shared_steps"] < info["steps"], info
+    else:
+        assert info["shared_steps"] == 0, info
+    return p, mg
+
+
+@pytest.mark.parametrize("spec", SLICED_MESHES)
+@pytest.mark.parametrize("precision", [neon_amd.PRECISION_FP64, neon_amd.PRECISION_MIXED])
+def test_spmv_products_match_the_matrix(ctx, precision, spec):
+    # one product y = K x in every shape (and the solvers' instance with x . y partials) against
+    # scipy's product with the downloaded values: FP64 to 1e-13 of |K||x| per row, FP32 (x
+    # rounded to FP32, FP32 values and accumulation) to 16 FP32 ulps of |K||x| per row, which
+    # a misplaced column or value exceeds by orders of magnitude
+    import ctypes as C
+
+    from neon_amd import _capi
+
+    p, mg = _sliced_mesh(ctx, spec)
+    mg.set_displacement(_smooth_state(p, 0.05, 7))
+    mg.update_internal_variables()
+    mg.assemble(precision)
+    mg.enforce_dirichlet(p.fixed_dofs(), None)
+    rp, ci = mg.pattern()
+    val, _ = mg.download(internal_force=False)
+    A = sp.csr_matrix((val, ci, rp))
+    x = np.random.default_rng(5).standard_normal(p.n_dofs)
+    fp32 = precision == neon_amd.PRECISION_MIXED
+    xr = x.astype(np.float32).astype(np.float64) if fp32 else x
+    ref = A @ xr
+    scale = abs(A) @ np.abs(xr)
+    tol = 16 * np.finfo(np.float32).eps if fp32 else 1e-13
+    lib = _capi.measure_lib()
+    for lanes, flags in SPMV_SHAPES:
+        for extra in (0, _capi.MEASURE_DOT):
+            for blocks in (0, 3):
+                y = np.zeros(p.n_dofs)
+                ctx.check(lib.neon_mesh_spmv_apply(ctx.handle, precision, lanes, flags | extra, blocks,
+                                                   _capi.f64(np.ascontiguousarray(x)), _capi.f64(y)))
+                err = (np.abs(y - ref) / scale).max()
+                print("shape", lanes, flags | extra, blocks, "max error / (|K||x|)", err)
+                assert err <= tol, (lanes, flags, extra, blocks, err)
+
+
+@pytest.mark.parametrize("spec", SLICED_MESHES)
+@pytest.mark.parametrize("precision", [neon_amd.PRECISION_FP64, neon_amd.PRECISION_MIXED])
+def test_spmv_shapes_agree(ctx, precision, spec):
+    # every SpMV shape (CSR-order values with 16 or 32 lanes per block row, the sliced copy with
+    # 16- or 32-bit per-lane offsets, plain or nontemporal value loads) solves the device tangent
+    # to the same answer as the plain shape and with the same Krylov count up to rounding; each
+    # solve's true residual meets the tolerance
+    p, mg = _sliced_mesh(ctx, spec)
     u = _smooth_state(p, 0.05, 7)
     mg.set_displacement(u)
     mg.update_internal_variables()
@@ -180,18 +231,35 @@ def test_spmv_shapes_agree(ctx, precision):
     rp, ci = mg.pattern()
     val, _ = mg.download(internal_force=False)
     A = sp.csr_matrix((val, ci, rp))
-    results = []
+    results, keys = [], []
     for lanes, flags in SPMV_SHAPES:
         # sliced forms: also fewer workgroups than slices (the grid-stride loops)
         for blocks in (0, 64, 3, 8) if flags & neon_amd.SPMV_SLICED else (0, 64):
+            keys.append((lanes, flags, blocks))
             mg.spmv_config(lanes, flags, blocks, blocks)
             x, its, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
             assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b), (lanes, flags, blocks)
             print("shape", lanes, flags, blocks, "iterations", its)
             results.append((x, its))
     x0, its0 = results[0]
-    # the mixed solver tests convergence every 8 iterations, so its counts move in steps of 8
-    slack = 2 if precision == neon_amd.PRECISION_FP64 else 10
+    # the sliced variants on one grid (16/32-bit offsets, plain/nontemporal loads) do the same
+    # arithmetic in the same order: equal counts and equal solutions
+    for blocks in (0, 64, 3, 8):
+        same = [results[i] for i, key in enumerate(keys) if key[1] & neon_amd.SPMV_SLICED and key[2] == blocks]
+        assert len(same) == 4
+        for x, its in same[1:]:
+            assert its == same[0][1] and np.array_equal(x, same[0][0]), blocks
+    if precision == neon_amd.PRECISION_FP64:
+        slack = 2
+    else:
+        # the FP32 recurrence of the mixed solver is sensitive to rounding: to the order of the
+        # FP32 row sums (the CSR-order kernels sum a row over a lane tree, the sliced one in four
+        # interleaved sets along the lane) and of the FP64 dot-product totals.  On the 12^3
+        # block the CSR-order shapes take 279 iterations and the sliced copy 303 to 311 as the
+        # order of the p.Ap total changed, all with products within one FP32 ulp of |K||x|
+        # (test_spmv_products_match_the_matrix) and solutions that agree (below); the counts
+        # may differ by 15 %
+        slack = max(8, int(0.15 * its0))
     for x, its in results[1:]:
         assert abs(its - its0) <= slack, (its, its0)
         assert np.linalg.norm(x - x0) <= 1e-8 * np.linalg.norm(x0)
@@ -200,9 +268,10 @@ def test_spmv_shapes_agree(ctx, precision):
 
 def test_sliced_columns_without_16bit_offsets(ctx):
     # node numbers shuffled over a 32^3-cell block (35,937 nodes): neighbours lie up to
-    # 35,936 apart, beyond 16-bit offsets, so the sliced copy keeps 32-bit columns; the solve
-    # still meets the tolerance against the downloaded matrix, and agrees with the CSR-order
-    # shape; the unshuffled block uses 16-bit offsets
+    # 35,936 apart, beyond 16-bit offsets, so the sliced copy keeps 32-bit per-lane offsets
+    # (and no two nodes share a signature, so no step is shared); the solve still meets the
+    # tolerance against the downloaded matrix, and agrees with the CSR-order shape.  The
+    # unshuffled block uses 16-bit offsets
     p = problems.block_rollers(32)
     perm = np.random.default_rng(3).permutation(p.n_nodes)
     inv = np.argsort(perm)
@@ -212,7 +281,8 @@ def test_sliced_columns_without_16bit_offsets(ctx):
     assert plain.sliced_info()["index16"]
     mg = neon_amd.DeviceMesh(X, conn, p.params, context=ctx)
     info = mg.sliced_info()
-    assert not info["index16"] and info["slices"] == (p.n_nodes + 63) // 64
+    assert not info["index16"] and info["slices"] == (p.n_nodes + 63) // 64, info
+    assert info["shared_steps"] == 0, info
     u = np.zeros(p.n_dofs)
     u.reshape(-1, 3)[:, 2] = 0.01 * X[:, 2]
     mg.set_displacement(u)
@@ -227,6 +297,6 @@ def test_sliced_columns_without_16bit_offsets(ctx):
     A = sp.csr_matrix((val, ci, rp))
     x1, its1, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
     assert np.linalg.norm(b - A @ x1) <= 1e-10 * np.linalg.norm(b)
-    mg.spmv_config(32, neon_amd.SPMV_SWEEP, 0, 0)
+    mg.spmv_config(32, neon_amd.SPMV_NONTEMPORAL, 0, 0)
     x2, its2, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
     assert abs(its1 - its2) <= 2 and np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x2)
