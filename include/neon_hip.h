@@ -44,6 +44,9 @@
  *   neon_natural_frequency     natural_frequency_matrix::solve (natural_frequency_matrix.hpp:54-68)
  *   neon_mesh_mass             natural_frequency_matrix::assemble_mass (natural_frequency_matrix.hpp:106-137)
  *   neon_adjoint_gradient      no counterpart in the reference (new: adjoint of the converged state)
+ *   neon_krylov_solve_csr_noalloc
+ *                              no counterpart in the reference: neon_krylov_solve_csr on device
+ *                              arrays without allocating, for callers that cannot allocate
  *
  * Threading: one host thread per context; a context owns one HIP stream and every
  * device buffer it allocates.  Host pointers are borrowed for the duration of a call.
@@ -134,6 +137,20 @@ int neon_krylov_solve_csr(neon_context* ctx, int solver, int criterion, int prec
                           const int32_t* row_ptr, const int32_t* col, const double* val, const double* b,
                           double* x, double tolerance, int32_t max_iterations, int32_t* iterations,
                           double* error);
+/* The same FP64 solve (NEON_SOLVER_CG or NEON_SOLVER_BICGSTAB, both stopping rules) with every
+ * array in device memory and no allocation: row_ptr, col, val, b and x are device pointers; the
+ * Jacobi inverse and the solver's work vectors live in a static per-GPU scratch array of the
+ * library (9 vectors of 262,144 doubles, 18 MiB, held by each GPU the library runs on until the
+ * process ends), so n is at most 262,144 (NEON_BAD_ARGUMENT beyond).  Malformed CSR arrays
+ * (row_ptr not from 0 to nnz or not monotone, a column outside [0, n)) are found by a device
+ * check and refused with NEON_BAD_ARGUMENT.  Calls that overlap on one GPU share its scratch
+ * and race.  No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture of its own buffers, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+int neon_krylov_solve_csr_noalloc(neon_context* ctx, int solver, int criterion, int32_t n, int64_t nnz,
+                                  const int32_t* row_ptr, const int32_t* col, const double* val, const double* b,
+                                  double* x, double tolerance, int32_t max_iterations, int32_t* iterations,
+                                  double* error);
 
 /* ---- device-resident mesh, constitutive state and tangent ----
  * X: 3 per node, node-interleaved.  conn: 8 per element in hex8 order

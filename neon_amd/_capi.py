@@ -162,6 +162,11 @@ SIGNATURES = {
         [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int32, C.c_int64, i32p, i32p, f64p, f64p, f64p,
          C.c_double, C.c_int32, i32p, f64p],
     ),
+    "neon_krylov_solve_csr_noalloc": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.c_int, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_double, C.c_int32, i32p, f64p],
+    ),
     "neon_mesh_create": (C.c_int, [C.c_void_p, C.c_int32, f64p, C.c_int32, i32p, C.c_int32, f64p, C.c_int32]),
     "neon_model_traits": (C.c_int, [C.c_int32, i32p, i32p]),
     "neon_mesh_set_damage": (C.c_int, [C.c_void_p, f64p]),

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -55,6 +56,57 @@ int lanes_for(double avg)
     return l;
 }
 
+// ---- the allocation-free CSR solve (neon_krylov_solve_csr_noalloc)
+// Its device memory beyond the caller's inputs and outputs lives in `scratch`, one copy per
+// GPU, held until the process ends: a validation flag, then the Jacobi inverse and the
+// solver's work vectors (9 vectors of at most kNoAllocRows entries: minv and BiCGSTAB's eight;
+// CG uses minv and three).  Every call on a GPU shares that GPU's copy, so calls that overlap
+// on one GPU race; each call rewrites what it reads (the flag is cleared by a kernel, the
+// vectors are initialised by the solvers).
+constexpr int64_t kNoAllocRows = int64_t(1) << 18;
+constexpr int64_t kScratchHeader = 32; // doubles before the vectors (256-byte aligned)
+constexpr int64_t kScratchDoubles = kScratchHeader + 9 * kNoAllocRows;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's `scratch` (the current device's), looked up once per device
+double* scratch_address(int device)
+{
+    if (device < 0 || device >= kMaxDevices) throw status_error(NEON_BAD_ARGUMENT, "device index out of range");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device])
+    {
+        void* p = nullptr;
+        hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw status_error(NEON_HIP_ERROR, std::string("hipGetSymbolAddress of the solver scratch failed: ")
+                                                   + hipGetErrorString(e));
+        scratch_ptr[device] = static_cast<double*>(p);
+    }
+    return scratch_ptr[device];
+}
+
+__global__ void clear_flag_kernel(double* flag)
+{
+    if (threadIdx.x == 0) flag[0] = 0.0;
+}
+
+// flag <- 1 if the CSR arrays are malformed: row_ptr[0] != 0, row_ptr[n] != nnz, a decreasing
+// row pointer or a column outside [0, n).  Reads row_ptr[0..n] and col[0..nnz) only.
+__global__ void check_csr_kernel(int32_t n, int64_t nnz, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                 double* flag)
+{
+    int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+    int64_t const t0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (t0 == 0) bad = rp[0] != 0 || int64_t(rp[n]) != nnz;
+    for (int64_t r = t0; r < n; r += stride) bad = bad || rp[r + 1] < rp[r];
+    for (int64_t q = t0; q < nnz; q += stride) bad = bad || ci[q] < 0 || ci[q] >= n;
+    if (bad) flag[0] = 1.0;
+}
 } // namespace
 
 extern "C" {
@@ -80,6 +132,8 @@ int neon_context_create(int device, neon_context** out)
     ctx->device = device;
     int const r = guarded(ctx, [&] {
         NEON_CHECK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        // this device's copy of the allocation-free solve's scratch (the device is current)
+        (void)scratch_address(device);
         ctx->engine = new krylov_engine(ctx->stream);
         ctx->engine->prof = &ctx->profiler;
         return NEON_OK;
@@ -185,6 +239,55 @@ int neon_krylov_solve_csr(neon_context* ctx, int solver, int criterion, int prec
             r = ctx->engine->cg_mixed(A, bd.data(), xd.data(), criterion, tolerance, max_iterations);
         }
         NEON_CHECK_HIP(hipMemcpyAsync(x, xd.data(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+        NEON_CHECK_HIP(hipStreamSynchronize(s));
+        if (iterations) *iterations = r.iterations;
+        if (error) *error = r.error;
+        return r.status;
+    });
+}
+
+int neon_krylov_solve_csr_noalloc(neon_context* ctx, int solver, int criterion, int32_t n, int64_t nnz,
+                                  const int32_t* row_ptr, const int32_t* col, const double* val, const double* b,
+                                  double* x, double tolerance, int32_t max_iterations, int32_t* iterations,
+                                  double* error)
+{
+    return guarded(ctx, [&] {
+        if (n <= 0 || nnz < 0 || !row_ptr || !col || !val || !b || !x) throw status_error(NEON_BAD_ARGUMENT, "bad CSR arguments");
+        if (n > kNoAllocRows)
+            throw status_error(NEON_BAD_ARGUMENT, "the allocation-free solve takes at most " + std::to_string(kNoAllocRows)
+                                                      + " rows (" + std::to_string(n) + " given)");
+        if (nnz > INT32_MAX) throw status_error(NEON_BAD_ARGUMENT, "nnz exceeds 32-bit row pointers");
+        if (solver != NEON_SOLVER_CG && solver != NEON_SOLVER_BICGSTAB) throw status_error(NEON_BAD_ARGUMENT, "unknown solver");
+        if (criterion != NEON_STOP_RELATIVE && criterion != NEON_STOP_ABSOLUTE_PRECONDITIONED)
+            throw status_error(NEON_BAD_ARGUMENT, "unknown stopping rule");
+        hipStream_t const s = ctx->stream;
+        double* const base = scratch_address(ctx->device);
+        double* const flag = base;
+        double* const vec = base + kScratchHeader;
+        clear_flag_kernel<<<1, 64, 0, s>>>(flag);
+        int const grid = int(std::min<int64_t>(1024, std::max<int64_t>(1, (std::max<int64_t>(nnz, n) + 255) / 256)));
+        check_csr_kernel<<<grid, 256, 0, s>>>(n, nnz, row_ptr, col, flag);
+        NEON_CHECK_HIP(hipGetLastError());
+        if (ctx->engine->read_scalar(flag) != 0.0)
+            throw status_error(NEON_BAD_ARGUMENT, "malformed CSR arrays (row pointers or column indices)");
+        linear_operator A;
+        A.n = n;
+        A.node_block = false;
+        A.rp = row_ptr;
+        A.ci = col;
+        A.lanes_per_row = lanes_for(double(nnz) / n);
+        A.v64 = val;
+        double* minv = vec;
+        krylov_work w;
+        double** slots[8] = {&w.r, &w.p, &w.Ap, &w.r0, &w.v, &w.s, &w.t, &w.p_hat};
+        for (int k = 0; k < 8; ++k) *slots[k] = vec + (k + 1) * kNoAllocRows;
+        // BiCGSTAB's eighth vector: CG's Ap slot is free there
+        w.s_hat = w.Ap;
+        ctx->engine->jacobi64(A, minv, criterion == NEON_STOP_RELATIVE ? 1 : 0);
+        solve_result const r =
+            solver == NEON_SOLVER_CG
+                ? ctx->engine->cg64(A, b, x, minv, criterion, tolerance, max_iterations, &w)
+                : ctx->engine->bicgstab64(A, b, x, minv, criterion, tolerance, max_iterations, &w);
         NEON_CHECK_HIP(hipStreamSynchronize(s));
         if (iterations) *iterations = r.iterations;
         if (error) *error = r.error;
