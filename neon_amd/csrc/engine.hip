@@ -318,9 +318,11 @@ void krylov_engine::run_batches(krylov_state* st, int64_t& launched, Batch&& bat
 }
 
 // Jacobi-PCG in FP64.  Per iteration: SpMV fused with the block partials of p.Ap, their total
-// (one block), the x/r update fused with the partials of r.r and r.z, their totals, and the p
-// update with the convergence decision; scalars stay on the device and the host polls the
-// state every `poll_every` iterations (a batch queued ahead of the one it waits on).
+// (one block: the SpMV has one partial per slice quadruple, which every update block would
+// otherwise reduce), the x/r update fused with the partials of r.r and r.z, and the p update
+// with the convergence decision (each of its blocks reduces the x/r update's partials);
+// scalars stay on the device and the host polls the state every `poll_every` iterations (a
+// batch queued ahead of the one it waits on).
 // Iterations past convergence are no-ops, so the reported count is exact.
 solve_result krylov_engine::cg64(const linear_operator& A, const double* b, double* x, const double* minv,
                                  int criterion, double tol, int32_t max_it, const krylov_work* work)
@@ -360,8 +362,7 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
             totals_kernel<<<1, 1024, 0, s_>>>(part_pAp, nullptr, ns, cur);
             cg_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x, r, p, Ap, minv, part_pAp, -1, cur,
                                                         part_rr, part_rz);
-            totals_kernel<<<1, 1024, 0, s_>>>(part_rr, part_rz, nb, cur);
-            cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p, r, minv, part_rr, part_rz, -1, cur, nxt,
+            cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p, r, minv, part_rr, part_rz, nb, cur, nxt,
                                                        criterion, tol, max_it);
         }
     });
@@ -578,15 +579,14 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             // iterations
             int const decided = (launched + 1) % refine_every == 0 ? 1 : 0;
             mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, -1, cur, part_rr, part_rz);
-            totals_kernel<<<1, 1024, 0, s_>>>(part_rr, part_rz, nb, cur);
             if (decided)
             {
-                mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, -1, cur, reliable_delta);
+                mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
                 mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, cur, 0);
                 Ar.residual64(s_, b, x, r64, cur, 1);
                 mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
             }
-            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, -1,
+            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, nb,
                                                          nb, decided, cur, nxt, max_it);
         }
     });
