@@ -185,9 +185,18 @@ def host_info():
                 break
     except (OSError, subprocess.SubprocessError):
         pass
+    cpuinfo = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpuinfo = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "threads_used": cpu_threads(),
-            "lscpu_model": model}
+            "lscpu_model": model, "cpuinfo_model": cpuinfo}
 
 
 def main():

@@ -3,8 +3,9 @@ latin_force_kernel, in the Newton driver) against the CPU oracle's restatement (
 test_latin.py): equal step and iteration sequences, force norms to 1e-8 relative (converged
 ones to 1e-12 of the run's largest) and u to 1e-10
 of max |u|, for the reference's J2 cube and the Langevin microsphere (general D), at s = 0
-(which converges) and on the J2 cube at s = 0.5 (neon's iterations stall and the load step is
-cut back until the minimum step is reached: the same cut-back sequence and last converged u)."""
+(which converges) and on both at s = 0.5 (neon's iterations stall and the load step is cut
+back until the minimum step is reached: the same cut-back sequence, force norms and last
+iterate)."""
 import copy
 
 import numpy as np
@@ -66,6 +67,22 @@ def test_latin_stalling_direction_cut_backs(ctx):
     assert err is not None and code != orc.OK
     _same(recs, recs0)
     # the last iterate of the failed attempt (neither driver resets u on the fatal exit)
+    np.testing.assert_allclose(mesh.displacement(), u0, rtol=0, atol=1e-8 * max(np.abs(u0).max(), 1e-300))
+
+
+def test_latin_general_d_nonzero_direction(ctx, tmp_path):
+    # the Langevin microsphere's general (21-coefficient) D enters the LATIN force through
+    # s D (eps - eps_old) only when s != 0: at s = 0.5 neon's iterations stall on this model as
+    # on the J2 cube (the oracle cuts the step back to the minimum and gives up after 50
+    # iterations), so the device must repeat every iteration's force norm (to 1e-8), the
+    # cut-back sequence and the last iterate
+    print("latin s=0.5 langevin", flush=True)
+    p = tgm._simulation(tmp_path, orc.MODEL_MICROSPHERE_LANGEVIN, neon_files.rollers(
+        [{"name": "z1", "type": "displacement", "time": [0.0, 1.0], "z": [0.0, 0.2]}]))
+    u, err, recs, mesh, code, u0, recs0 = _run(p, ctx, 0.5)
+    assert err is not None and code != orc.OK
+    assert len(recs0) > 10 and max(r["force_norm"] for r in recs0) > 0
+    _same(recs, recs0)
     np.testing.assert_allclose(mesh.displacement(), u0, rtol=0, atol=1e-8 * max(np.abs(u0).max(), 1e-300))
 
 
