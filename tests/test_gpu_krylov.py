@@ -68,6 +68,32 @@ def test_bicgstab_host_csr_parity(ctx, criterion):
         assert res <= 10 * max([res0] + res_t) + 1e-14 * np.linalg.norm(b), (res, res0, res_t)
 
 
+@pytest.mark.parametrize("solver", ["cg", "bicgstab"])
+@pytest.mark.parametrize("criterion", [neon_amd.STOP_RELATIVE, neon_amd.STOP_ABSOLUTE_PRECONDITIONED])
+def test_krylov_iterates_follow_the_oracle(ctx, solver, criterion):
+    # the solvers' k-th iterate against the oracle's (same algorithm, same stopping rule, a
+    # tolerance that never stops them), before roundoff has had time to grow: the oracle's own
+    # iterates move by 2e-15 (BiCGSTAB at k = 10) and 1e-15 (CG at k = 40) of max|x| between
+    # its thread counts, so a wrong coefficient, sign or update order shows at once
+    p = problems.cube_clamped(6)
+    rp, ci, val, b = _system(p)
+    if solver == "cg":
+        ref = orc.cg_eigen if criterion == neon_amd.STOP_RELATIVE else orc.cg_cuda
+        cls, ks, bound = neon_amd.conjugate_gradient_hip, (5, 10, 40), 1e-13
+    else:
+        ref = orc.bicgstab_eigen if criterion == neon_amd.STOP_RELATIVE else orc.bicgstab_cuda
+        cls, ks, bound = neon_amd.biconjugate_gradient_stabilised_hip, (5, 10), 1e-13
+    for k in ks:
+        code0, x0, its0, _ = ref(rp, ci, val, b, 1e-30, k)
+        s = cls(1e-30, k, criterion=criterion, context=ctx)
+        x = np.zeros_like(b)
+        with pytest.raises(neon_amd.computational_error):  # the iteration limit is reached
+            s.solve((rp, ci, val), x, b)
+        err = np.abs(x - x0).max() / np.abs(x0).max()
+        print(f"{solver} rule {criterion}: iterate {k}: {s.iterations} vs {its0} iterations, difference {err:.2e}")
+        assert s.iterations == its0 and err <= bound, (k, err)
+
+
 def test_bicgstab_unsymmetric_system(ctx):
     # an unsymmetric diagonally dominant matrix: BiCGSTAB converges where CG would not apply
     rng = np.random.default_rng(9)
