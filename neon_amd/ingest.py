@@ -67,6 +67,7 @@ class Problem:
     eigen_solver: dict | None = None   # the step's "eigen_solver" block (natural frequency)
     density: np.ndarray | None = None  # (E,) the material's "density", when given
     ageing: tuple | None = None        # ageing microsphere: (scission, recombination) probabilities
+    damage: tuple | None = None        # J2 with damage: the six constants of neon_mesh_set_damage
 
     @property
     def n_nodes(self) -> int:
@@ -102,7 +103,7 @@ def _material_params(material: dict, model: int) -> np.ndarray:
         raise ValueError('"elastic_modulus" and "poissons_ratio" or "bulk_modulus" and "shear_modulus" need to be '
                          "specified as material properties")
     sy = H = 0.0
-    if model == _capi.MODEL_J2:
+    if model in (_capi.MODEL_J2, _capi.MODEL_J2_DAMAGE):
         if "yield_stress" not in material:
             raise ValueError('"yield_stress" was not specified in material properties')
         sy = float(material["yield_stress"])
@@ -121,6 +122,22 @@ def _material_params(material: dict, model: int) -> np.ndarray:
                 raise ValueError('"nonaffine_stretch_parameter" not specified in material data\n')
             H = float(material["nonaffine_stretch_parameter"])
     return np.array([K, G, sy, H])
+
+
+DAMAGE_KEYS = ("kinematic_hardening_modulus", "softening_multiplier", "plasticity_viscous_denominator",
+               "plasticity_viscous_exponent", "damage_viscous_denominator", "damage_viscous_exponent")
+
+
+def _damage_material(material: dict) -> tuple:
+    """The six constants of isotropic_elastic_plastic_damage (isotropic_elastic_plastic_damage.cpp:
+    9-45) in neon_mesh_set_damage's order, with the reference's checks and messages (it checks
+    them in the order softening, kinematic hardening, plasticity exponent and denominator, damage
+    exponent and denominator)."""
+    for key in ("softening_multiplier", "kinematic_hardening_modulus", "plasticity_viscous_exponent",
+                "plasticity_viscous_denominator", "damage_viscous_exponent", "damage_viscous_denominator"):
+        if key not in material:
+            raise ValueError(f'"{key}" needs to be specified')
+    return tuple(float(material[k]) for k in DAMAGE_KEYS)
 
 
 def _ageing_material(material: dict) -> tuple:
@@ -187,7 +204,9 @@ def _model(constitutive: dict) -> int:
         if "damage" in constitutive:
             if constitutive["finite_strain"]:
                 raise ValueError('"damage" is not yet implemented for "finite_strain"')
-            raise ValueError("the J2 damage model does not run on the device path")
+            if constitutive["damage"] == "isotropic_chaboche":
+                return _capi.MODEL_J2_DAMAGE
+            # any other damage type falls through to plain J2 (constitutive_model_factory.cpp:135-157)
         if constitutive["finite_strain"]:
             # finite_strain_J2_plasticity::consistent_tangent receives the logarithmic strain
             # where it expects the trial left Cauchy-Green tensor (finite_strain_J2_plasticity.cpp:
@@ -380,13 +399,14 @@ def read(path: str, step: int = 0) -> Problem:
     material = materials[part["material"]]
     density = np.full(len(conn), float(material["density"])) if "density" in material else None
     ageing = _ageing_material(material)[:2] if (model & 0xFF) == _capi.MODEL_MICROSPHERE_AGEING else None
+    damage = _damage_material(material) if model == _capi.MODEL_J2_DAMAGE else None
     if sim["solution"] == "natural_frequency":
         if density is None:
             # material_property::initial_density (material_property.cpp:34-42)
             raise ValueError("density was requested, but not specified in the input file\n")
         return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, {}, groups,
                        mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], dict(sim["eigen_solver"]),
-                       density, ageing)
+                       density, ageing, damage)
     ls = dict(sim.get("linear_solver", {}))
     if "type" not in ls:
         raise ValueError('A "linear_solver" "type" must be specified')
@@ -399,4 +419,4 @@ def read(path: str, step: int = 0) -> Problem:
                          '"direct" or "iterative"')
     simulation = {"time": sim["time"], "nonlinear_options": sim.get("nonlinear_options", {}), "linear_solver": solver}
     return Problem(root.get("name", part["name"]), X, conn, params, model, boundaries, loads, simulation, groups,
-                   mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], None, density, ageing)
+                   mesh_data.get("visualisation"), mesh_data["name"], sim["solution"], None, density, ageing, damage)

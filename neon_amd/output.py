@@ -39,11 +39,13 @@ _POINTS = _GAUSS * _NODES
 NODAL = ("displacement", "reaction_force")
 COMMON = ("cauchy_stress", "deformation_gradient", "displacement_gradient", "DetF")
 J2_ONLY = ("linearised_strain", "linearised_plastic_strain", "von_mises_stress", "effective_plastic_strain")
+DAMAGE_ONLY = ("back_stress", "kinematic_hardening", "damage", "energy_release_rate")
 # the variable names each model adds (constitutive_model::variable_names): small-strain J2
-# (small_strain_J2_plasticity.cpp:14-26 plus those of isotropic_linear_elasticity.cpp:15-19), linear
-# elasticity (isotropic_linear_elasticity.cpp:15-19);
-# the microsphere models add none
-MODEL_VARIABLES = {_capi.MODEL_J2: J2_ONLY, _capi.MODEL_LINEAR_ELASTIC: ("linearised_strain", "von_mises_stress")}
+# (small_strain_J2_plasticity.cpp:14-26 plus those of isotropic_linear_elasticity.cpp:15-19), J2
+# with damage (those and small_strain_J2_plasticity_damage.cpp:21-29), linear elasticity
+# (isotropic_linear_elasticity.cpp:15-19); the microsphere models add none
+MODEL_VARIABLES = {_capi.MODEL_J2: J2_ONLY, _capi.MODEL_J2_DAMAGE: J2_ONLY + DAMAGE_ONLY,
+                   _capi.MODEL_LINEAR_ELASTIC: ("linearised_strain", "von_mises_stress")}
 
 
 def shape_at_points():
@@ -110,7 +112,7 @@ class vtk_file_output:
         self.fields = list(dict.fromkeys(visualisation.get("fields", [])))
         allowed = NODAL + COMMON + MODEL_VARIABLES.get(model & 0xFF, ())
         for f in self.fields:
-            if f not in NODAL + COMMON + J2_ONLY:
+            if f not in NODAL + COMMON + J2_ONLY + DAMAGE_ONLY:
                 raise ValueError(f"Name {f} is not a valid output variable name")
             if f not in allowed:
                 raise ValueError(f"Internal variable {f} does not exist in mesh")
@@ -153,6 +155,17 @@ class vtk_file_output:
                 out[f] = node_average(qp["vm"].reshape(-1, 1), self.conn, n)
             elif f == "effective_plastic_strain":
                 out[f] = node_average(qp["ebar"].reshape(-1, 1), self.conn, n)
+            elif f == "back_stress":
+                out[f] = node_average(qp["back_stress"].reshape(-1, 9), self.conn, n)
+            elif f == "kinematic_hardening":
+                # neon's J2-damage model allocates this tensor but never writes it
+                # (perform_radial_return takes it without using it,
+                # small_strain_J2_plasticity_damage.cpp:113-115): zero at every write
+                out[f] = np.zeros((n, 9))
+            elif f == "damage":
+                out[f] = node_average(qp["damage"].reshape(-1, 1), self.conn, n)
+            elif f == "energy_release_rate":
+                out[f] = node_average(qp["energy"].reshape(-1, 1), self.conn, n)
         return out
 
     def write(self, step: int, time: float, fields: dict) -> str:
@@ -193,7 +206,7 @@ def solve_with_output(problem, ctx, root: str = ".", precision: str = "fp64", ta
 
     tangent = neon_amd.TANGENT_REFERENCE if tangent is None else tangent
     mesh = neon_amd.DeviceMesh(problem.X, problem.conn, problem.params, model=problem.model, tangent=tangent,
-                               context=ctx, ageing=problem.ageing)
+                               context=ctx, ageing=problem.ageing, damage=getattr(problem, "damage", None))
     sim = dict(problem.simulation)
     sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": precision})
     driver = neon_amd.latin_matrix if problem.solution == "latin" else neon_amd.static_matrix
@@ -205,7 +218,10 @@ def solve_with_output(problem, ctx, root: str = ".", precision: str = "fp64", ta
 
         def on_step(step, time):
             _, f = mesh.download(stiffness=False)
-            writer.write(step, time, writer.fields_at(mesh.displacement(), f, mesh.qp_state()))
+            qp = mesh.qp_state()
+            if problem.model == neon_amd.MODEL_J2_DAMAGE:
+                qp.update(mesh.damage_state())
+            writer.write(step, time, writer.fields_at(mesh.displacement(), f, qp))
     try:
         u = sm.solve(on_step=on_step if writer else None)
     finally:

@@ -132,9 +132,10 @@ def adjoint_block(ctx, n=96, directions=2):
            "adjoint_seconds": time.time() - t, "adjoint_cg_iterations": mesh.adjoint_iterations}
     g = ah.log_mu_gradient(grad4, log_mu)
     rel = []
-    for d in ah.directions(p.n_elem, directions):
+    for k, d in enumerate(ah.directions(p.n_elem, directions)):
         fd = ah.gpu_fd(p, log_mu, u, d, target, ctx)
         rel.append(abs(fd - g @ d) / abs(g @ d))
+        _progress(f"adjoint_block {n}: direction {k + 1} of {directions}: {rel[-1]:.2e} at {time.time() - t0:.1f} s")
     res["fd_relative_errors"] = rel
     res["seconds"] = time.time() - t0
     return res

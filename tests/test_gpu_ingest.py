@@ -1,6 +1,8 @@
 """The reference's example inputs and nonfollower loads through ingest and the device Newton
 (neon_newton_solve / neon_newton_solve_loaded), against the CPU oracle on the same problem and
 against the closed forms of test_ingest.py."""
+import os
+
 import numpy as np
 import pytest
 
@@ -114,3 +116,26 @@ def test_generated_sinusoidal_history(ctx, tmp_path):
     assert code == orc.OK
     _same_newton(recs, recs0)
     np.testing.assert_allclose(u, u0, rtol=0, atol=1e-10 * np.abs(u0).max())
+
+
+def test_viscoplastic_damage_cube_example(ctx, tmp_path):
+    # the reference's viscoplastic damage cube through ingest, the device Newton driver with
+    # BiCGSTAB (the model's tangent is unsymmetric) and the output writer with the example's
+    # fields: the oracle's Newton sequence, u and damage, and the damage fields in the files
+    from neon_amd import output
+
+    p = ingest.read(ti.DAMAGE_CUBE)
+    u, sm, writer = output.solve_with_output(p, ctx, root=str(tmp_path))
+    assert sm.options.solver == neon_amd.SOLVER_BICGSTAB
+    code, u0, recs0, m = ti.oracle_solve(p)
+    assert code == orc.OK
+    _same_newton(sm.records, recs0)
+    np.testing.assert_allclose(u, u0, rtol=0, atol=1e-9 * np.abs(u0).max())
+    d, d0 = sm.mesh.damage_state()["damage"], m.damage_state()["damage"]
+    assert (d0 > 0).all()
+    np.testing.assert_allclose(d, d0, rtol=1e-8, atol=0)
+    # one file per converged load step (10) after the initial write
+    assert len(writer.written) >= 10
+    last = open(os.path.join(str(tmp_path), writer.written[-1])).read()
+    for name in ("damage", "energy_release_rate", "back_stress", "kinematic_hardening", "von_mises_stress"):
+        assert f'Name="{name}"' in last, name

@@ -22,7 +22,7 @@ import pytest
 from scipy.optimize import brentq, fsolve
 
 import neon_amd
-from neon_amd import _capi, ingest
+from neon_amd import _capi, ingest, problems
 from oracle import oracle as orc
 
 import neon_files
@@ -30,11 +30,19 @@ import neon_files
 HERE = os.path.dirname(os.path.abspath(__file__))
 NEOHOOKE_CUBE = os.path.join(HERE, "golden", "neon_examples", "neohooke_cube", "cube.json")
 J2_CUBE = os.path.join(HERE, "golden", "neon_examples", "j2_cube", "cube.json")
+DAMAGE_CUBE = os.path.join(HERE, "golden", "neon_examples", "viscoplastic_damage_cube", "cube.json")
 
 
 def oracle_solve(p, threads=0):
     m = orc.Model(p.X, p.conn, p.params, model=p.model, tangent=orc.TANGENT_REFERENCE)
-    code, u, recs = m.newton(orc.newton_options(p.simulation, threads=threads), p.boundaries, loads=p.loads)
+    damage = getattr(p, "damage", None)
+    if damage is not None:
+        m.set_damage(damage)
+    # the unsymmetric tangent of J2 with damage is solved with BiCGSTAB, as make_linear_solver
+    # picks for a model whose is_symmetric() is false
+    solver = 1 if p.model == orc.MODEL_J2_DAMAGE else 0
+    code, u, recs = m.newton(orc.newton_options(p.simulation, threads=threads, solver=solver), p.boundaries,
+                             loads=p.loads)
     return code, u, recs, m
 
 
@@ -144,6 +152,8 @@ def test_ingest_rejects_what_the_device_path_does_not_run(tmp_path):
         (mat, {"name": "J2_plasticity", "finite_strain": True}, [], "finite-strain J2"),
         (mat, {"name": "J2_plasticity", "finite_strain": True, "damage": "isotropic_chaboche"}, [],
          "not yet implemented"),
+        (dict(mat, yield_stress=85.0), {"name": "J2_plasticity", "finite_strain": False, "damage": "isotropic_chaboche"},
+         [], '"softening_multiplier" needs to be specified'),
         ({"elastic_modulus": 10.0, "poissons_ratio": 0.5}, nh, [], "poissons_ratio"),
         ({"elastic_modulus": 10.0}, nh, [], "bulk_modulus"),
         (mat, nh, [{"name": "z1", "type": "heat_flux", "time": [0, 1], "z": [0, 1]}], "not recognised"),
@@ -357,3 +367,26 @@ def test_generated_history_drives_the_newton_solve(tmp_path):
     assert sorted({round(r["step_time"], 12) for r in recs}) == [0.25, 0.5, 0.75]
     # u_z = -0.05 sin(3 pi / 2) X_z = 0.05 X_z at the end, homogeneous
     np.testing.assert_allclose(u.reshape(-1, 3)[:, 2], 0.05 * p.X[:, 2], rtol=0, atol=1e-12)
+
+
+def test_viscoplastic_damage_cube_example():
+    # the reference's viscoplastic damage cube (examples/solid_mechanics/viscoplastic_damage/cube):
+    # one hex8, symmetry rollers, the top pulled to 1.8 % over 10 ms in 1 ms increments, the
+    # "isotropic_chaboche" J2 model with the material's six damage constants
+    p = ingest.read(DAMAGE_CUBE)
+    assert p.model == _capi.MODEL_J2_DAMAGE
+    assert p.damage == problems.DAMAGE_CONSTANTS
+    E, nu = 134.0e3, 0.3
+    np.testing.assert_allclose(p.params[0], [E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu)), 85.0, 0.0], rtol=1e-15)
+    assert sorted(b["name"] for b in p.boundaries) == ["Xsym", "Ysym", "ZLoad", "Zsym"]
+    # PaStiX maps to the device Krylov solvers at a relative 1e-12
+    assert p.simulation["linear_solver"]["tolerance"] == 1e-12
+    # the oracle (BiCGSTAB on the unsymmetric tangent) runs it to the end with damage growing
+    code, u, recs, m = oracle_solve(p)
+    assert code == orc.OK
+    assert max(r["step_time"] for r in recs) == pytest.approx(0.01)
+    d = m.damage_state()["damage"]
+    assert (d > 0).all() and (d < 1).all()
+    # the same problem as problems.damage_cube(1): the top face's displacement is 0.018
+    top = np.nonzero(p.X[:, 2] == p.X[:, 2].max())[0]
+    np.testing.assert_allclose(u.reshape(-1, 3)[top, 2], 0.018, rtol=1e-14)
