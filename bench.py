@@ -33,16 +33,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_sell_neohooke128_traffic.txt")
+TRAFFIC_PROFILES = {False: os.path.join(ROOT, "profiles", "spmv_sell_neohooke128_traffic.txt"),
+                    True: os.path.join(ROOT, "profiles", "spmv_sym_neohooke128_traffic.txt")}
 
 
-def measured_traffic(n):
-    """HBM bytes per launch of the FP32 sliced node-block SpMV at the 128^3 workload, from the
-    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in the profile file (None
-    for other sizes or without the file)."""
-    if n != 128 or not os.path.exists(TRAFFIC_PROFILE):
+def measured_traffic(n, symmetric):
+    """HBM bytes per launch of the FP32 node-block SpMV (the symmetric or the full sliced copy)
+    at the 128^3 workload, from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    summarised in the profile file (None for other sizes or without the file)."""
+    path = TRAFFIC_PROFILES[bool(symmetric)]
+    if n != 128 or not os.path.exists(path):
         return None
-    with open(TRAFFIC_PROFILE) as fh:
+    with open(path) as fh:
         for line in fh:
             if line.startswith("traffic_bytes_per_launch_fp32"):
                 return int(line.split("=")[1])
@@ -67,6 +69,20 @@ def sliced_bytes(n, value_bytes, sliced):
     own = sliced["steps"] - sliced["shared_steps"]
     meta = 8 * sliced["steps"] + 64 * ib * own + (4 * 64 + 8) * sliced["slices"] + 4
     return value_bytes * 576 * sliced["steps"] + meta + 2 * value_bytes * n
+
+
+def symmetric_bytes(nnz, n, value_bytes):
+    """Algorithmic bytes of one product on the symmetric sliced copy: the stored values once
+    (the diagonal block and the upper blocks of each node, (nnz + 3 n) / 2 values), x read once,
+    y written once.  The lower blocks are the stored upper ones read a second time, from the L2
+    where the schedule keeps them there: not counted (the measured traffic shows what was)."""
+    return value_bytes * (nnz + 3 * n) // 2 + 2 * value_bytes * n
+
+
+def symmetric_copy_bytes(n, value_bytes, symmetric, slices):
+    """Bytes the symmetric copy holds and one product touches once: 1 + offsets records of
+    9 x 64 values per slice (padding included), the schedule's wave slots, x and y."""
+    return value_bytes * 576 * slices * (1 + symmetric["offsets"]) + 4 * symmetric["slots"] + 2 * value_bytes * n
 
 
 def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
@@ -224,6 +240,8 @@ def main():
     mesh = neon_amd.DeviceMesh(p.X, p.conn, p.params, model=p.model, context=ctx)
     nnz = mesh.nnz()
     sliced = mesh.sliced_info()
+    symmetric = mesh.symmetric_info()
+    sym = symmetric["usable"]
     log(f"setup {time.perf_counter() - t0:.1f} s: {p.n_elem} elements, {p.n_dofs} dofs, nnz {nnz}")
 
     results = {}
@@ -234,7 +252,19 @@ def main():
     r = results[head]
     value = args.steps / r["seconds"]
     vb = 4 if head == "mixed" else 8
-    bytes_spmv = spmv_bytes(nnz, p.n_dofs, vb)
+
+    def layout_bytes(value_bytes):
+        return symmetric_bytes(nnz, p.n_dofs, value_bytes) if sym else spmv_bytes(nnz, p.n_dofs, value_bytes)
+
+    bytes_spmv = layout_bytes(vb)
+    vname = "float" if vb == 4 else "double"
+    if sym:
+        kernel = f"spmv_sym_kernel<{vname}> (symmetric sliced node-block SpMV + p.Ap)"
+        layout = dict(symmetric, slices=sliced["slices"],
+                      bytes_per_launch=symmetric_copy_bytes(p.n_dofs, vb, symmetric, sliced["slices"]))
+    else:
+        kernel = f"spmv_sell_kernel<{vname}> (sliced node-block SpMV + p.Ap)"
+        layout = dict(sliced, bytes_per_launch=sliced_bytes(p.n_dofs, vb, sliced))
     spmv_avg_ms = r["spmv_ms"] / max(r["spmv_count"], 1)
     achieved = bytes_spmv / (spmv_avg_ms * 1e-3) / 1e9
     line = {
@@ -257,14 +287,14 @@ def main():
                    "elements": p.n_elem, "dofs": p.n_dofs, "nnz": nnz, "load_steps_timed": r["load_steps"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.n) if head == "mixed" else None,
-                     "kernel": f"spmv_sell_kernel<{'float' if vb == 4 else 'double'}> (sliced node-block SpMV + p.Ap)",
+                     "traffic": measured_traffic(args.n, sym) if head == "mixed" else None,
+                     "kernel": kernel,
                      "algorithmic_bytes": bytes_spmv, "avg_ms": spmv_avg_ms, "launches": r["spmv_count"],
-                     "sliced_layout": dict(sliced, bytes_per_launch=sliced_bytes(p.n_dofs, vb, sliced))},
+                     "symmetric_layout" if sym else "sliced_layout": layout},
     }
     for prec, rr in results.items():
         vbp = 4 if prec == "mixed" else 8
-        bs = spmv_bytes(nnz, p.n_dofs, vbp)
+        bs = layout_bytes(vbp)
         avg = rr["spmv_ms"] / max(rr["spmv_count"], 1)
         asm_avg = rr["asm_ms"] / max(rr["asm_count"], 1)
         its = [x["solver_iterations"] for x in rr["records"]]

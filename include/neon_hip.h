@@ -225,18 +225,28 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
 /* Shape of the SpMV that the solvers on this mesh's tangent use from now on: with
  * NEON_SPMV_SLICED the sliced copy of the values (64 nodes per wavefront, one per lane), else
  * the CSR-order values with `lanes` (16, 32) lanes per block row; workgroups for FP64 and for
- * FP32 values (0: by size).  Default: NEON_SPMV_SLICED | NEON_SPMV_NONTEMPORAL, workgroups by
- * size (one slice per wavefront).  No counterpart in the reference: a tuning knob; every shape
- * gives the same products up to the order of the FP32 row sums. */
+ * FP32 values (0: by size).  Default: NEON_SPMV_SLICED | NEON_SPMV_NONTEMPORAL |
+ * NEON_SPMV_SYMMETRIC, workgroups by size (one slice per wavefront).  No counterpart in the
+ * reference: a tuning knob; every shape gives the same products up to rounding (the order of
+ * the row sums, and with NEON_SPMV_SYMMETRIC each lower block taken as the transpose of the
+ * stored upper one). */
 #define NEON_SPMV_NONTEMPORAL 1 /* stream the values with nontemporal loads */
 #define NEON_SPMV_SLICED 256    /* the sliced copy of the values */
 #define NEON_SPMV_INDEX32 1024  /* with NEON_SPMV_SLICED: 32-bit per-lane column offsets even where
                                    16 bits fit (the default uses 16 bits when every offset fits) */
+#define NEON_SPMV_SYMMETRIC 2048 /* with NEON_SPMV_SLICED: where the model's tangent is symmetric and
+                                    the pattern has at most 15 distinct upper block offsets, a copy
+                                    of the diagonal and upper blocks only, the lower ones read as
+                                    their transposes */
 int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);
 /* The sliced copy's shape: slices (64 nodes each), steps (block columns per lane, padding
  * included), steps whose 64 lanes share one column offset (stored once), and whether the other
  * steps' per-lane offsets are 16-bit (1) or 32-bit (0).  No counterpart in the reference. */
 int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int64_t* shared_steps, int32_t* index16);
+/* The symmetric sliced copy's shape: whether the solvers on this mesh's tangent can use it
+ * (a symmetric model and at most 15 distinct upper block offsets), the number of those offsets
+ * (0 past 15) and the wave slots of its schedule.  No counterpart in the reference. */
+int neon_mesh_symmetric_info(neon_context* ctx, int32_t* usable, int32_t* offsets, int32_t* slots);
 
 /* ---- Newton driver (static_matrix::solve) ---- */
 typedef struct {

@@ -36,6 +36,7 @@ from ._capi import (  # noqa: F401
     SPMV_SLICED,
     SPMV_INDEX32,
     SPMV_NONTEMPORAL,
+    SPMV_SYMMETRIC,
     STOP_ABSOLUTE_PRECONDITIONED,
     STOP_RELATIVE,
     TANGENT_CONSISTENT,
@@ -393,6 +394,14 @@ class DeviceMesh:
                                                          C.byref(idx)))
         return {"slices": sl.value, "steps": st.value, "shared_steps": sh.value, "index16": bool(idx.value)}
 
+    def symmetric_info(self) -> dict:
+        """Shape of the symmetric sliced copy (``neon_mesh_symmetric_info``): whether the solvers
+        can use it, the pattern's distinct upper block offsets and the schedule's wave slots."""
+        us, off, slots = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        self.ctx.check(_capi.lib().neon_mesh_symmetric_info(self.ctx.handle, C.byref(us), C.byref(off),
+                                                            C.byref(slots)))
+        return {"usable": bool(us.value), "offsets": off.value, "slots": slots.value}
+
     def displacement(self) -> np.ndarray:
         """The displacement held by the mesh (``neon_mesh_displacement``)."""
         u = np.zeros(self.n_dofs)
@@ -430,11 +439,12 @@ class DeviceMesh:
         self.ctx.check(_capi.lib().neon_mesh_enforce_dirichlet(self.ctx.handle, len(fixed), _capi.i32(fixed),
                                                                _capi.f64(b)))
 
-    def spmv_config(self, lanes: int = 32, flags: int = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL,
+    def spmv_config(self, lanes: int = 32,
+                    flags: int = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL | _capi.SPMV_SYMMETRIC,
                     blocks_fp64: int = 0, blocks_fp32: int = 0) -> None:
         """Shape of the node-block SpMV used by the solvers on this mesh (tuning knob; the
         defaults are the library's: the sliced copy, one wavefront per 64 nodes, values read with
-        nontemporal loads)."""
+        nontemporal loads, the symmetric copy where the tangent allows it)."""
         self.ctx.check(_capi.lib().neon_mesh_spmv_config(self.ctx.handle, lanes, flags, blocks_fp64, blocks_fp32))
 
     def adjoint_gradient(self, dJdu: np.ndarray, fixed: np.ndarray, tol: float = 1e-12,

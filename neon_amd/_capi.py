@@ -50,6 +50,7 @@ KERNEL_ASSEMBLY = 3
 SPMV_NONTEMPORAL = 1
 SPMV_SLICED = 256
 SPMV_INDEX32 = 1024
+SPMV_SYMMETRIC = 2048
 # flags of the measurement library's neon_mesh_spmv_bench (include/neon_hip_measure.h)
 MEASURE_DOT = 4096
 MEASURE_DIRTY = 8192
@@ -201,6 +202,7 @@ SIGNATURES = {
     ),
     "neon_mesh_displacement": (C.c_int, [C.c_void_p, f64p]),
     "neon_mesh_sliced_info": (C.c_int, [C.c_void_p, i32p, P(C.c_int64), P(C.c_int64), i32p]),
+    "neon_mesh_symmetric_info": (C.c_int, [C.c_void_p, i32p, i32p, i32p]),
     "neon_newton_solve_loaded": (
         C.c_int,
         [C.c_void_p, P(NewtonOptions), C.c_int32, P(Dirichlet), C.c_int32, P(Neumann), f64p, P(NewtonRecord),

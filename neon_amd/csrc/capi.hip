@@ -541,7 +541,8 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
 {
     return guarded(ctx, [&] {
         need_mesh(ctx);
-        if ((lanes != 16 && lanes != 32) || (flags & ~(NEON_SPMV_NONTEMPORAL | NEON_SPMV_SLICED | NEON_SPMV_INDEX32))
+        if ((lanes != 16 && lanes != 32)
+            || (flags & ~(NEON_SPMV_NONTEMPORAL | NEON_SPMV_SLICED | NEON_SPMV_INDEX32 | NEON_SPMV_SYMMETRIC))
             || blocks_fp64 < 0 || blocks_fp32 < 0)
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV configuration");
         spmv_config& c = ctx->mesh->spmv;
@@ -549,6 +550,17 @@ int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32
         c.flags = flags;
         c.blocks64 = blocks_fp64;
         c.blocks32 = blocks_fp32;
+        return NEON_OK;
+    });
+}
+
+int neon_mesh_symmetric_info(neon_context* ctx, int32_t* usable, int32_t* offsets, int32_t* slots)
+{
+    return guarded(ctx, [&] {
+        need_mesh(ctx);
+        if (usable) *usable = ctx->mesh->symmetric_usable() ? 1 : 0;
+        if (offsets) *offsets = ctx->mesh->symmetric_offsets();
+        if (slots) *slots = ctx->mesh->symmetric_slots();
         return NEON_OK;
     });
 }

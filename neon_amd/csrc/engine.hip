@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <tuple>
 #include <type_traits>
 
 #include <omp.h>
@@ -36,6 +37,15 @@ inline int xcd_blocks(int64_t want, int cap)
 // ================================================================ operator
 int linear_operator::spmv_blocks() const
 {
+    if (node_block && sym)
+    {
+        // one wavefront per wave slot in one pass, or fewer workgroups (a multiple of 8, so that
+        // each slot keeps its XCD) striding over the slots
+        int64_t const needed = int64_t(sym_slots) / 4;
+        int64_t want = blocks_override > 0 ? std::min<int64_t>(blocks_override, needed) : needed;
+        want = std::min<int64_t>(want, kMaxPartials);
+        return int(std::max<int64_t>(8, want / 8 * 8));
+    }
     if (node_block && sell)
     {
         // one wavefront per slice in one pass (4 per workgroup), or fewer workgroups striding
@@ -63,6 +73,14 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
                  const double* b, double* part, const krylov_state* st, int gate = 0)
 {
     int const blocks = A.spmv_blocks();
+    if (A.node_block && A.sym)
+    {
+        if (!sval) throw status_error(NEON_BAD_ARGUMENT, "symmetric operator without its values");
+        sym_matrix const m{A.nodes, A.slices, A.sym_steps, A.sym_slots, A.sym_sched, A.sym_off};
+        spmv_sym_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
+        NEON_CHECK_HIP(hipGetLastError());
+        return;
+    }
     if (A.node_block && A.sell)
     {
         if (!sval) throw status_error(NEON_BAD_ARGUMENT, "sliced operator without sliced values");
@@ -564,7 +582,7 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
     // the gated true-residual product is a no-op in all but a few of its launches: a grid of at
     // most 2,048 workgroups striding over the slices keeps those launches short
     linear_operator Ar = A;
-    if (Ar.node_block && Ar.sell) Ar.blocks_override = 2048;
+    if (Ar.node_block && (Ar.sell || Ar.sym)) Ar.blocks_override = 2048;
     int64_t launched = 0;
     run_batches(st, launched, [&] {
         for (int k = 0; k < poll_every; ++k, ++launched)
@@ -880,6 +898,7 @@ device_mesh::device_mesh(hipStream_t s, int32_t n_nodes, const double* X, int32_
     NEON_CHECK_HIP(hipMemcpyAsync(bptr_.data(), bptr_h_.data(), (size_t(N_) + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(bcol_.data(), bcol_h_.data(), size_t(nnzb) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     build_sliced_pattern();
+    build_symmetric_pattern();
     NEON_CHECK_HIP(hipMemcpyAsync(self_slot_.data(), self.data(), size_t(N_) * sizeof(int32_t), hipMemcpyHostToDevice, s_));
     NEON_CHECK_HIP(hipMemcpyAsync(slots_.data(), slots.data(), slots.size(), hipMemcpyHostToDevice, s_));
 
@@ -1218,7 +1237,7 @@ void device_mesh::assemble(const double* u_dev, int precision)
     (void)hipEventDestroy(e1);
     k64_valid_ = precision == NEON_PRECISION_FP64;
     k32_valid_ = precision != NEON_PRECISION_FP64;
-    s64_fresh_ = s32_fresh_ = false;
+    s64_fresh_ = s32_fresh_ = y64_fresh_ = y32_fresh_ = false;
 }
 
 void device_mesh::assemble_mass(const double* density)
@@ -1278,7 +1297,7 @@ void device_mesh::download_mass(double* val)
 void device_mesh::enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision)
 {
     if (n_fixed <= 0) return;
-    s64_fresh_ = s32_fresh_ = false;
+    s64_fresh_ = s32_fresh_ = y64_fresh_ = y32_fresh_ = false;
     if (precision == NEON_PRECISION_FP64)
         dirichlet_kernel<double><<<grid_for(n_fixed), kBlock, 0, s_>>>(n_fixed, fixed_dev, bptr_.data(), bcol_.data(),
                                                                          k64_.data(), b_dev);
@@ -1568,7 +1587,9 @@ linear_operator device_mesh::op(int precision) const
     A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
     // nodes of more than 170 neighbours keep the CSR-order values (the sliced copy is packed
     // through LDS rows of 3 * max degree values)
-    if ((spmv.flags & NEON_SPMV_SLICED) && sliced_fits())
+    if ((spmv.flags & NEON_SPMV_SLICED) && (spmv.flags & NEON_SPMV_SYMMETRIC) && symmetric_usable())
+        attach_symmetric(A, precision);
+    else if ((spmv.flags & NEON_SPMV_SLICED) && sliced_fits())
         attach_sliced(A, precision, (spmv.flags & NEON_SPMV_INDEX32) != 0);
     return A;
 }
@@ -1621,6 +1642,7 @@ void device_mesh::attach_sliced(linear_operator& A, int precision, bool index32)
         A.s32 = s32_.data();
     }
     A.sell = true;
+    A.sym = false;
     A.slices = slices_;
     A.sptr = sptr_.data();
     A.snode = snode_.data();
@@ -1629,6 +1651,147 @@ void device_mesh::attach_sliced(linear_operator& A, int precision, bool index32)
     A.sexc = sexc_.data();
     A.soff32 = soff32_.data();
     A.soff16 = index32 || !soff16_ok_ ? nullptr : soff16_.data();
+}
+
+// The symmetric sliced layout (sym_matrix): the pattern's distinct upper block offsets (at
+// most kMaxSymOffsets, else the layout is not used) and the wave-slot schedule.  Schedule: the
+// slices sorted by (band of the in-plane index, plane), with the plane stride P and the row
+// length L estimated from the offsets of a structured block (ascending 1, L - 1, L, L + 1, ...,
+// P + L + 1: L = smallest offset above 1, plus 1; P = largest - L - 1) and bands of 16 rows;
+// any estimate gives a valid order, a good one makes the transposed reads hit the L2.  Each
+// XCD takes one contiguous eighth of the order.
+void device_mesh::build_symmetric_pattern()
+{
+    sym_ok_ = false;
+    sym_off_h_.clear();
+    sym_slots_ = 0;
+    if (N_ <= 0 || slices_ <= 0) return;
+    std::vector<int64_t> offs;
+    bool many = false;
+#pragma omp parallel
+    {
+        std::vector<int64_t> mine;
+        bool mine_many = false;
+#pragma omp for schedule(static)
+        for (int64_t nd = 0; nd < N_; ++nd)
+        {
+            if (mine_many) continue;
+            for (int32_t b = bptr_h_[nd]; b < bptr_h_[nd + 1]; ++b)
+            {
+                int64_t const o = int64_t(bcol_h_[b]) - nd;
+                if (o <= 0 || std::find(mine.begin(), mine.end(), o) != mine.end()) continue;
+                mine.push_back(o);
+                if (int(mine.size()) > kMaxSymOffsets)
+                {
+                    mine_many = true;
+                    break;
+                }
+            }
+        }
+#pragma omp critical
+        {
+            many = many || mine_many;
+            for (int64_t o : mine)
+                if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+        }
+    }
+    if (many || int(offs.size()) > kMaxSymOffsets) return;
+    std::sort(offs.begin(), offs.end());
+    int64_t const steps = 1 + int64_t(offs.size());
+    if (int64_t(slices_) * steps > int64_t(INT64_MAX) / (576 * 8)) return;
+    // schedule
+    int64_t L = 64, P = int64_t(N_);
+    for (int64_t o : offs)
+        if (o > 1)
+        {
+            L = o + 1;
+            break;
+        }
+    if (!offs.empty() && offs.back() - L - 1 > 0) P = offs.back() - L - 1;
+    int64_t const band = std::max<int64_t>(1, 16 * L);
+    std::vector<int32_t> order(slices_);
+    for (int32_t sl = 0; sl < slices_; ++sl) order[sl] = sl;
+    auto key = [&](int32_t sl) {
+        int64_t const i0 = 64ll * sl;
+        return std::make_tuple((i0 % P) / band, i0 / P, sl);
+    };
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
+    int64_t per = 0;
+    for (int x = 0; x < 8; ++x)
+    {
+        int64_t const b0 = int64_t(slices_) * x / 8, b1 = int64_t(slices_) * (x + 1) / 8;
+        per = std::max<int64_t>(per, (b1 - b0 + 3) / 4);
+    }
+    per = std::max<int64_t>(per, 1);
+    if (per * 32 > INT32_MAX) return;
+    std::vector<int32_t> sched(size_t(per) * 32, -1);
+    for (int x = 0; x < 8; ++x)
+    {
+        int64_t const b0 = int64_t(slices_) * x / 8, b1 = int64_t(slices_) * (x + 1) / 8;
+        for (int64_t e = 0; e < b1 - b0; ++e) sched[size_t(((e / 4) * 8 + x) * 4 + e % 4)] = order[size_t(b0 + e)];
+    }
+    sym_off_h_.assign(offs.begin(), offs.end());
+    sym_off_.resize(std::max<size_t>(sym_off_h_.size(), 1));
+    sym_sched_.resize(sched.size());
+    if (!sym_off_h_.empty())
+        NEON_CHECK_HIP(hipMemcpyAsync(sym_off_.data(), sym_off_h_.data(), sym_off_h_.size() * sizeof(int32_t),
+                                      hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipMemcpyAsync(sym_sched_.data(), sched.data(), sched.size() * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, s_));
+    NEON_CHECK_HIP(hipStreamSynchronize(s_));
+    sym_slots_ = int32_t(sched.size());
+    sym_ok_ = true;
+}
+
+bool device_mesh::symmetric_usable() const { return sym_ok_ && model_is_symmetric(model_); }
+
+namespace
+{
+template <typename VT>
+void pack_symmetric(hipStream_t s, const sym_matrix& m, const int32_t* bptr, const int32_t* bcol, const VT* val,
+                    VT* sval)
+{
+    int64_t const records = int64_t(m.slices) * m.steps;
+    sym_pack_kernel<VT><<<int((records + 3) / 4), 256, 0, s>>>(m, bptr, bcol, val, sval);
+    NEON_CHECK_HIP(hipGetLastError());
+}
+} // namespace
+
+void device_mesh::attach_symmetric(linear_operator& A, int precision) const
+{
+    if (!symmetric_usable()) throw status_error(NEON_BAD_ARGUMENT, "the tangent has no symmetric sliced copy");
+    int32_t const steps = 1 + int32_t(sym_off_h_.size());
+    sym_matrix const m{N_, slices_, steps, sym_slots_, sym_sched_.data(), sym_off_.data()};
+    size_t const vals = size_t(slices_) * steps * 576;
+    if (precision == NEON_PRECISION_FP64)
+    {
+        if (!k64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP64 tangent assembled");
+        if (!y64_fresh_)
+        {
+            y64_.resize(vals);
+            pack_symmetric<double>(s_, m, bptr_.data(), bcol_.data(), k64_.data(), y64_.data());
+            y64_fresh_ = true;
+        }
+        A.s64 = y64_.data();
+    }
+    else
+    {
+        if (!k32_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP32 tangent assembled");
+        if (!y32_fresh_)
+        {
+            y32_.resize(vals);
+            pack_symmetric<float>(s_, m, bptr_.data(), bcol_.data(), k32_.data(), y32_.data());
+            y32_fresh_ = true;
+        }
+        A.s32 = y32_.data();
+    }
+    A.sell = false;
+    A.sym = true;
+    A.slices = slices_;
+    A.sym_steps = steps;
+    A.sym_slots = sym_slots_;
+    A.sym_sched = sym_sched_.data();
+    A.sym_off = sym_off_.data();
 }
 
 namespace

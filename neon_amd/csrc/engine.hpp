@@ -126,6 +126,12 @@ struct linear_operator
     const int32_t* soff32 = nullptr;
     const double* s64 = nullptr;
     const float* s32 = nullptr;
+    // symmetric sliced copy (sym_matrix layout: diagonal and upper blocks); the SpMV uses it
+    // when set, with its values at s64 / s32 and `slices` slices
+    bool sym = false;
+    int32_t sym_steps = 0, sym_slots = 0;
+    const int32_t* sym_sched = nullptr;
+    const int32_t* sym_off = nullptr;
 
     int spmv_blocks() const;
     // FP64: y = A x (values v64); with part, also block partials of x.y (or w.y if w).
@@ -220,10 +226,14 @@ constexpr int64_t kValuePad = 16;
 // Shape of the node-block SpMV used by the Krylov solvers on a mesh tangent (the defaults
 // are the fastest measured at 6.4M dofs on MI355X): lanes per block row, NEON_SPMV_* flags,
 // workgroups for FP64 and FP32 values.
+// constitutive_model::is_symmetric of each model id (only the J2 damage model's tangent is
+// unsymmetric: small_strain_J2_plasticity_damage.hpp:44)
+inline bool model_is_symmetric(int32_t model) { return (model & 0xff) != NEON_MODEL_J2_DAMAGE; }
+
 struct spmv_config
 {
     int lanes = 32;
-    int flags = NEON_SPMV_SLICED | NEON_SPMV_NONTEMPORAL;
+    int flags = NEON_SPMV_SLICED | NEON_SPMV_NONTEMPORAL | NEON_SPMV_SYMMETRIC;
     int blocks64 = 0;
     int blocks32 = 0;
 };
@@ -281,6 +291,14 @@ public:
     int64_t sliced_shared_steps() const { return shared_steps_; }
     // the sliced copy is packed through LDS: nodes of at most 170 neighbours
     bool sliced_fits() const;
+    // points A's SpMV at the symmetric sliced copy (diagonal and upper blocks; packed here when
+    // the tangent changed since the last copy); only where symmetric_usable()
+    void attach_symmetric(linear_operator& A, int precision) const;
+    // a symmetric model and at most 15 distinct upper block offsets in the pattern
+    bool symmetric_usable() const;
+    // upper offsets of the pattern (0 when more than 15) and the symmetric copy's wave slots
+    int32_t symmetric_offsets() const { return sym_ok_ ? int32_t(sym_off_h_.size()) : 0; }
+    int32_t symmetric_slots() const { return sym_slots_; }
     // Jacobi inverse from the node-block diagonal
     void jacobi(double* minv, int eigen_rule, int precision);
     // consistent mass (density per element, host) in the pattern of K, FP64; Dirichlet rows and
@@ -323,6 +341,7 @@ public:
 
 private:
     void build_sliced_pattern();
+    void build_symmetric_pattern();
     hipStream_t s_;
     int32_t N_, E_, model_, tangent_mode_;
     int64_t nnz_ = 0;
@@ -345,6 +364,14 @@ private:
     mutable device_array<double> s64_;
     mutable device_array<float> s32_;
     mutable bool s64_fresh_ = false, s32_fresh_ = false;
+    // symmetric sliced layout: upper offsets, the wave-slot schedule and the copies
+    bool sym_ok_ = false;
+    int32_t sym_slots_ = 0;
+    std::vector<int32_t> sym_off_h_;
+    device_array<int32_t> sym_off_, sym_sched_;
+    mutable device_array<double> y64_;
+    mutable device_array<float> y32_;
+    mutable bool y64_fresh_ = false, y32_fresh_ = false;
     device_array<double> sigma_, coef_, detF_, eps_p_, ebar_, vm_, eps_p_old_, ebar_old_;
     device_array<double> age_, age_old_; // ageing microsphere: 11 x Q network state (SoA)
     device_array<double> sigma_old_;     // LATIN: committed stress (6 x Q)

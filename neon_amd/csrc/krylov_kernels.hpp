@@ -479,6 +479,194 @@ __global__ __launch_bounds__(256) void sell_pack_lds_kernel(sell_matrix A, const
     }
 }
 
+// ------------------------------------------------ symmetric sliced layout
+// For a symmetric tangent whose block pattern has few distinct upper column offsets (column
+// node - row node > 0; a structured hex8 block has 13), the copy keeps the diagonal block and
+// the upper blocks only.  Slice s holds nodes 64 s .. 64 s + 63 (one per lane) in `steps` =
+// 1 + offsets records of 9 x 64 values laid out by sell_pos: record s * steps is the diagonal
+// block, record s * steps + 1 + u the block (i, i + off[u]) of lane node i, zero where node i
+// has no such neighbour.  The product adds, for every lane node i and offset o, the block
+// K(i, i + o) times x(i + o) and the transpose of K(i - o, i) (record (i - o) / 64 * steps + 1
+// + u, lane (i - o) % 64) times x(i - o): every off-diagonal value is read twice, and the
+// schedule makes the second read follow the first closely enough to be served by the L2.
+// Schedule: wave slot q (workgroup q / 4, on XCD (q / 4) % 8) takes slice sched[q] (-1: none).
+// The host orders the slices in tiles (a band of the in-plane index, then plane by plane) and
+// gives each XCD one contiguous eighth of that order, so a slice's neighbours one plane down
+// were read on the same XCD a few workgroups earlier.
+constexpr int kMaxSymOffsets = 15;
+struct sym_matrix
+{
+    int32_t nodes;
+    int32_t slices;
+    int32_t steps;        // records per slice: 1 + offsets
+    int32_t slots;        // wave slots of the schedule (a multiple of 32)
+    const int32_t* sched; // [slots] slice of each wave slot, -1 for none
+    const int32_t* off;   // [steps - 1] the upper offsets, ascending
+};
+
+// y = A x on the symmetric sliced layout (DOT, MODE, gate as in spmv_sell_kernel).  FP32
+// accumulation rotates over four accumulator sets as there; FP64 uses one.
+template <typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
+__global__ __launch_bounds__(256) void spmv_sym_kernel(sym_matrix A, const VT* __restrict__ val,
+                                                       const XT* __restrict__ x, YT* __restrict__ y,
+                                                       const double* __restrict__ bvec, double* __restrict__ part,
+                                                       const krylov_state* __restrict__ st, int gate)
+{
+    if (st && (st->done || (gate == 1 && !st->refine) || (gate == 2 && st->refine))) return;
+    int const lane = threadIdx.x & 63;
+    int32_t const wave = __builtin_amdgcn_readfirstlane(int32_t(threadIdx.x >> 6));
+    int64_t const nodes = A.nodes;
+    int64_t const steps = A.steps;
+    constexpr int W = 16 / int(sizeof(VT));
+    typedef VT vec_t __attribute__((ext_vector_type(W)));
+    // the 9 values of lane l of the record at rec (NT: nontemporal, for a record read once)
+    auto load_rec = [&](const VT* rec, int l, VT(&v)[9], auto nt) {
+#pragma unroll
+        for (int g = 0; g < 8 / W; ++g)
+        {
+            const vec_t* src = reinterpret_cast<const vec_t*>(rec + g * 64 * W) + l;
+            vec_t q;
+            if constexpr (decltype(nt)::value)
+                q = __builtin_nontemporal_load(src);
+            else
+                q = *src;
+#pragma unroll
+            for (int j = 0; j < W; ++j) v[g * W + j] = q[j];
+        }
+        if constexpr (decltype(nt)::value)
+            v[8] = __builtin_nontemporal_load(rec + 512 + l);
+        else
+            v[8] = rec[512 + l];
+    };
+    double dot = 0.0;
+    for (int32_t q = int32_t(blockIdx.x) * 4 + wave; q < A.slots; q += int32_t(gridDim.x) * 4)
+    {
+        int32_t const s = __builtin_amdgcn_readfirstlane(A.sched[q]);
+        if (s < 0) continue;
+        int64_t const i = int64_t(s) * 64 + lane;
+        bool const ok = i < nodes;
+        int64_t const ic = ok ? i : nodes - 1;
+        int64_t const row = 3 * ic;
+        double br0 = 0.0, br1 = 0.0, br2 = 0.0;
+        if constexpr (MODE == 1)
+        {
+            br0 = bvec[row];
+            br1 = bvec[row + 1];
+            br2 = bvec[row + 2];
+        }
+        XT const xo0 = x[row], xo1 = x[row + 1], xo2 = x[row + 2];
+        AT a0 = AT(0), a1 = AT(0), a2 = AT(0), b0 = AT(0), b1 = AT(0), b2 = AT(0);
+        AT c0 = AT(0), c1 = AT(0), c2 = AT(0), d0 = AT(0), d1 = AT(0), d2 = AT(0);
+        // one block's product (transposed: K^T x) into the next accumulator set
+        auto accumulate = [&](VT const(&v)[9], XT x0_, XT x1_, XT x2_, bool tr) {
+            AT const x0 = AT(x0_), x1 = AT(x1_), x2 = AT(x2_);
+            AT n0, n1, n2;
+            if (tr)
+            {
+                n0 = a0 + (AT(v[0]) * x0 + AT(v[3]) * x1 + AT(v[6]) * x2);
+                n1 = a1 + (AT(v[1]) * x0 + AT(v[4]) * x1 + AT(v[7]) * x2);
+                n2 = a2 + (AT(v[2]) * x0 + AT(v[5]) * x1 + AT(v[8]) * x2);
+            }
+            else
+            {
+                n0 = a0 + (AT(v[0]) * x0 + AT(v[1]) * x1 + AT(v[2]) * x2);
+                n1 = a1 + (AT(v[3]) * x0 + AT(v[4]) * x1 + AT(v[5]) * x2);
+                n2 = a2 + (AT(v[6]) * x0 + AT(v[7]) * x1 + AT(v[8]) * x2);
+            }
+            if constexpr (sizeof(AT) == 4)
+            {
+                a0 = b0, a1 = b1, a2 = b2;
+                b0 = c0, b1 = c1, b2 = c2;
+                c0 = d0, c1 = d1, c2 = d2;
+                d0 = n0, d1 = n1, d2 = n2;
+            }
+            else
+            {
+                a0 = n0, a1 = n1, a2 = n2;
+            }
+        };
+        {
+            VT v[9];
+            load_rec(val + int64_t(s) * steps * 576, lane, v, std::true_type{});
+            accumulate(v, xo0, xo1, xo2, false);
+        }
+        int32_t const nu = int32_t(steps) - 1;
+#pragma unroll 2
+        for (int32_t u = 0; u < nu; ++u)
+        {
+            int64_t const o = A.off[u];
+            // the stored block K(i, i + o); past the last node its values are zero
+            int64_t const m = i + o < nodes ? i + o : nodes - 1;
+            VT v[9];
+            load_rec(val + (int64_t(s) * steps + 1 + u) * 576, lane, v, std::false_type{});
+            XT const x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
+            // the transpose of K(i - o, i), stored with node i - o's blocks
+            int64_t const j0 = i - o;
+            bool const jv = j0 >= 0 && j0 < nodes;
+            int64_t const j = jv ? j0 : ic;
+            VT w[9];
+            load_rec(val + ((j >> 6) * steps + 1 + u) * 576, int(j & 63), w, std::false_type{});
+            XT const z0 = jv ? x[3 * j] : XT(0), z1 = jv ? x[3 * j + 1] : XT(0), z2 = jv ? x[3 * j + 2] : XT(0);
+            accumulate(v, x0, x1, x2, false);
+            accumulate(w, z0, z1, z2, true);
+        }
+        AT const r0 = (a0 + b0) + (c0 + d0), r1 = (a1 + b1) + (c1 + d1), r2 = (a2 + b2) + (c2 + d2);
+        YT const y0 = MODE == 1 ? YT(br0 - double(r0)) : YT(r0);
+        YT const y1 = MODE == 1 ? YT(br1 - double(r1)) : YT(r1);
+        YT const y2 = MODE == 1 ? YT(br2 - double(r2)) : YT(r2);
+        if (ok)
+        {
+            y[row] = y0;
+            y[row + 1] = y1;
+            y[row + 2] = y2;
+        }
+        if constexpr (DOT)
+            dot += ok ? double(xo0) * double(y0) + double(xo1) * double(y1) + double(xo2) * double(y2) : 0.0;
+    }
+    if constexpr (DOT)
+    {
+        dot = block_sum<256>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = dot;
+    }
+}
+
+// Symmetric sliced copy of node-block values (CSR order val[9 bptr[n] + i (3 deg) + 3 kk + j]):
+// one wavefront per record; each lane finds its column node (itself for the diagonal record,
+// else i + off[k - 1]) among its row's blocks, zeros where the row has no such block.
+template <typename VT>
+__global__ __launch_bounds__(256) void sym_pack_kernel(sym_matrix A, const int32_t* __restrict__ bptr,
+                                                       const int32_t* __restrict__ bcol, const VT* __restrict__ val,
+                                                       VT* __restrict__ sval)
+{
+    int64_t const rec = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (rec >= int64_t(A.slices) * A.steps) return;
+    int const lane = threadIdx.x & 63;
+    int64_t const s = rec / A.steps;
+    int32_t const k = int32_t(rec - s * A.steps);
+    int64_t const i = s * 64 + lane;
+    VT v[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) v[t] = VT(0);
+    if (i < A.nodes)
+    {
+        int64_t const m = k == 0 ? i : i + A.off[k - 1];
+        int32_t const b0 = bptr[i], deg = bptr[i + 1] - b0;
+        for (int32_t kk = 0; kk < deg; ++kk)
+            if (bcol[b0 + kk] == m)
+            {
+                const VT* src = val + 9ll * b0 + 3 * kk;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[3 * r + c] = src[int64_t(r) * 3 * deg + c];
+                break;
+            }
+    }
+    VT* dst = sval + rec * 576;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) dst[sell_pos<VT>(t, lane)] = v[t];
+}
+
 // Generic CSR (any sparsity): LPR lanes per row.
 template <int LPR, typename VT, typename XT, typename YT, typename AT, bool DOT, int MODE>
 __global__ __launch_bounds__(kBlock) void spmv_csr_kernel(csr_matrix A, const VT* __restrict__ val,

@@ -130,14 +130,17 @@ int neon_mesh_spmv_bench(neon_context* ctx, int32_t precision, int32_t lanes, in
         device_mesh& m = *ctx->mesh;
         if (lanes != 16 && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "lanes must be 16 or 32");
         if (reps <= 0 || !avg_ms || blocks < 0) throw status_error(NEON_BAD_ARGUMENT, "bad repetitions");
-        if (flags & ~(NEON_SPMV_NONTEMPORAL | NEON_SPMV_SLICED | NEON_SPMV_INDEX32 | NEON_MEASURE_DOT | NEON_MEASURE_DIRTY))
+        if (flags & ~(NEON_SPMV_NONTEMPORAL | NEON_SPMV_SLICED | NEON_SPMV_INDEX32 | NEON_SPMV_SYMMETRIC | NEON_MEASURE_DOT | NEON_MEASURE_DIRTY))
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV flags");
         linear_operator A = m.op(precision);
         A.nb_lanes = lanes;
         A.nb_nt = (flags & NEON_SPMV_NONTEMPORAL) != 0;
         A.blocks_override = blocks;
-        A.sell = false;
-        if (flags & NEON_SPMV_SLICED) m.attach_sliced(A, precision, (flags & NEON_SPMV_INDEX32) != 0);
+        A.sell = A.sym = false;
+        if ((flags & NEON_SPMV_SLICED) && (flags & NEON_SPMV_SYMMETRIC) && m.symmetric_usable())
+            m.attach_symmetric(A, precision);
+        else if (flags & NEON_SPMV_SLICED)
+            m.attach_sliced(A, precision, (flags & NEON_SPMV_INDEX32) != 0);
         int64_t const n = m.dofs();
         hipStream_t const s = ctx->stream;
         // NEON_MEASURE_DOT: the solvers' form with partials of x.y; NEON_MEASURE_DIRTY: each
@@ -185,14 +188,17 @@ int neon_mesh_spmv_apply(neon_context* ctx, int32_t precision, int32_t lanes, in
         device_mesh& m = *ctx->mesh;
         if (lanes != 16 && lanes != 32) throw status_error(NEON_BAD_ARGUMENT, "lanes must be 16 or 32");
         if (!x || !y || blocks < 0) throw status_error(NEON_BAD_ARGUMENT, "bad SpMV arguments");
-        if (flags & ~(NEON_SPMV_NONTEMPORAL | NEON_SPMV_SLICED | NEON_SPMV_INDEX32 | NEON_MEASURE_DOT))
+        if (flags & ~(NEON_SPMV_NONTEMPORAL | NEON_SPMV_SLICED | NEON_SPMV_INDEX32 | NEON_SPMV_SYMMETRIC | NEON_MEASURE_DOT))
             throw status_error(NEON_BAD_ARGUMENT, "bad SpMV flags");
         linear_operator A = m.op(precision);
         A.nb_lanes = lanes;
         A.nb_nt = (flags & NEON_SPMV_NONTEMPORAL) != 0;
         A.blocks_override = blocks;
-        A.sell = false;
-        if (flags & NEON_SPMV_SLICED) m.attach_sliced(A, precision, (flags & NEON_SPMV_INDEX32) != 0);
+        A.sell = A.sym = false;
+        if ((flags & NEON_SPMV_SLICED) && (flags & NEON_SPMV_SYMMETRIC) && m.symmetric_usable())
+            m.attach_symmetric(A, precision);
+        else if (flags & NEON_SPMV_SLICED)
+            m.attach_sliced(A, precision, (flags & NEON_SPMV_INDEX32) != 0);
         int64_t const n = m.dofs();
         hipStream_t const s = ctx->stream;
         device_array<double> xd, yd, part;

@@ -5,10 +5,11 @@
 builds the n^3 block (the bench's mesh), assembles the tangent at u = 0 in the requested
 precision, runs `reps` SpMVs in the solvers' form (`neon_mesh_spmv_bench` of the measurement library
 libneon_hip_measure.so: the sliced copy, nontemporal value loads, x . y block partials; the CG
-loop's kernel instance) and a streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the
+loop's kernel instance; the symmetric copy where the mesh allows it unless --full) and a
+streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the
 calibration kernel for the FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the
-mean SpMV time, its algorithmic bytes (`spmv_bytes`) and the achieved fraction of the 8 TB/s
-HBM peak.  Run it under `rocprofv3 --kernel-trace
+mean SpMV time, its algorithmic bytes (`spmv_bytes` or `symmetric_bytes`) and the achieved
+fraction of the 8 TB/s HBM peak.  Run it under `rocprofv3 --kernel-trace
 --stats` or `--pmc` to read the same kernels' durations and traffic counters.
 """
 from __future__ import annotations
@@ -32,6 +33,13 @@ def spmv_bytes(nnz: int, nodes: int, value_bytes: int) -> int:
     return value_bytes * nnz + 2 * (nnz // 9) + 2 * value_bytes * n
 
 
+def symmetric_bytes(nnz: int, nodes: int, value_bytes: int) -> int:
+    """Algorithmic bytes of one product on the symmetric copy: the diagonal and upper blocks'
+    values once ((nnz + 9 nodes) / 2), x read once, y written once (as bench.py counts them)."""
+    n = 3 * nodes
+    return value_bytes * (nnz + 9 * nodes) // 2 + 2 * value_bytes * n
+
+
 def main(argv=None) -> dict:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--n", type=int, default=128)
@@ -39,6 +47,7 @@ def main(argv=None) -> dict:
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--dirty", action="store_true")
+    ap.add_argument("--full", action="store_true", help="the full sliced copy even where the symmetric one applies")
     args = ap.parse_args(argv)
 
     ctx = neon_amd.Context(0)
@@ -50,10 +59,15 @@ def main(argv=None) -> dict:
     mesh.assemble(prec)
     ms = C.c_double()
     lib = _capi.measure_lib()
-    # the solvers' form exactly: the sliced copy with nontemporal value loads and the x . y block
-    # partials (the same kernel instance as inside the CG loop), workgroups by size; --dirty puts
-    # a write of 4 vectors of n before every product, as the CG vector updates do (timed apart)
+    # the solvers' form exactly: the sliced copy (symmetric where usable) with nontemporal value
+    # loads and the x . y block partials (the same kernel instance as inside the CG loop),
+    # workgroups by size; --dirty puts a write of 4 vectors of n before every product, as the CG
+    # vector updates do (timed apart)
+    symmetric = mesh.symmetric_info()
+    sym = symmetric["usable"] and not args.full
     flags = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL | _capi.MEASURE_DOT
+    if sym:
+        flags |= _capi.SPMV_SYMMETRIC
     if args.dirty:
         flags |= _capi.MEASURE_DIRTY
     cfg = dict(lanes=32, flags=flags, blocks=0)
@@ -62,7 +76,7 @@ def main(argv=None) -> dict:
     nnz = mesh.nnz()
     vb = 8 if prec == _capi.PRECISION_FP64 else 4
     sliced = mesh.sliced_info()
-    b = spmv_bytes(nnz, p.n_nodes, vb)
+    b = symmetric_bytes(nnz, p.n_nodes, vb) if sym else spmv_bytes(nnz, p.n_nodes, vb)
     gbs = b / (ms.value * 1e-3) / 1e9
     read_ms, copy_ms = C.c_double(), C.c_double()
     copy_bytes = args.copy_mb << 20
@@ -75,6 +89,7 @@ def main(argv=None) -> dict:
         "nnz": nnz,
         "shape": cfg,
         "sliced": sliced,
+        "symmetric": dict(symmetric, used=sym),
         "spmv_avg_ms": ms.value,
         "spmv_algorithmic_bytes": b,
         "spmv_gbs": gbs,

@@ -179,6 +179,8 @@ SPMV_SHAPES = [
     (32, neon_amd.SPMV_SLICED), (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL | neon_amd.SPMV_INDEX32),
     (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_INDEX32),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL | neon_amd.SPMV_SYMMETRIC),
+    (32, neon_amd.SPMV_SLICED | neon_amd.SPMV_SYMMETRIC),
 ]
 
 
@@ -195,6 +197,7 @@ def _sliced_mesh(ctx, spec):
     mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
     info = mg.sliced_info()
     assert info["index16"] and info["slices"] == (p.n_nodes + 63) // 64, info
+    assert mg.symmetric_info()["usable"]
     if "dims" in spec:
         assert 0.6 * info["steps"] < info["shared_steps"] < info["steps"], info
     else:
@@ -269,12 +272,16 @@ def test_spmv_shapes_agree(ctx, precision, spec):
             results.append((x, its))
     x0, its0 = results[0]
     # the sliced variants on one grid (16/32-bit offsets, plain/nontemporal loads) do the same
-    # arithmetic in the same order: equal counts and equal solutions
+    # arithmetic in the same order: equal counts and equal solutions; so do the two symmetric
+    # shapes (the symmetric copy's loads do not depend on the nontemporal flag)
     for blocks in (0, 64, 3, 8):
-        same = [results[i] for i, key in enumerate(keys) if key[1] & neon_amd.SPMV_SLICED and key[2] == blocks]
-        assert len(same) == 4
-        for x, its in same[1:]:
-            assert its == same[0][1] and np.array_equal(x, same[0][0]), blocks
+        for symmetric, count in ((0, 4), (neon_amd.SPMV_SYMMETRIC, 2)):
+            same = [results[i] for i, key in enumerate(keys)
+                    if key[1] & neon_amd.SPMV_SLICED and key[1] & neon_amd.SPMV_SYMMETRIC == symmetric
+                    and key[2] == blocks]
+            assert len(same) == count
+            for x, its in same[1:]:
+                assert its == same[0][1] and np.array_equal(x, same[0][0]), blocks
     if precision == neon_amd.PRECISION_FP64:
         slack = 2
     else:
