@@ -590,26 +590,50 @@ __global__ __launch_bounds__(256) void spmv_sym_kernel(sym_matrix A, const VT* _
             load_rec(val + int64_t(s) * steps * 576, lane, v, std::true_type{});
             accumulate(v, xo0, xo1, xo2, false);
         }
+        // one offset's operands: the stored block K(i, i + o) and x(i + o), and the transpose
+        // of K(i - o, i) (record (i - o) / 64 * steps + 1 + u, lane (i - o) % 64) and x(i - o);
+        // past the last node the stored values are zero, before the first node x counts as zero
         int32_t const nu = int32_t(steps) - 1;
-#pragma unroll 2
-        for (int32_t u = 0; u < nu; ++u)
+        struct pair_t
         {
-            int64_t const o = A.off[u];
-            // the stored block K(i, i + o); past the last node its values are zero
-            int64_t const m = i + o < nodes ? i + o : nodes - 1;
-            VT v[9];
-            load_rec(val + (int64_t(s) * steps + 1 + u) * 576, lane, v, std::false_type{});
-            XT const x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
-            // the transpose of K(i - o, i), stored with node i - o's blocks
-            int64_t const j0 = i - o;
-            bool const jv = j0 >= 0 && j0 < nodes;
-            int64_t const j = jv ? j0 : ic;
-            VT w[9];
-            load_rec(val + ((j >> 6) * steps + 1 + u) * 576, int(j & 63), w, std::false_type{});
-            XT const z0 = jv ? x[3 * j] : XT(0), z1 = jv ? x[3 * j + 1] : XT(0), z2 = jv ? x[3 * j + 2] : XT(0);
-            accumulate(v, x0, x1, x2, false);
-            accumulate(w, z0, z1, z2, true);
+            VT v[9], w[9];
+            XT x0, x1, x2, z0, z1, z2;
+            bool jv;
+        };
+        auto fetch = [&](int32_t u) {
+            pair_t o;
+            int64_t const off = A.off[u];
+            int64_t const m = i + off < nodes ? i + off : nodes - 1;
+            load_rec(val + (int64_t(s) * steps + 1 + u) * 576, lane, o.v, std::false_type{});
+            o.x0 = x[3 * m], o.x1 = x[3 * m + 1], o.x2 = x[3 * m + 2];
+            int64_t const j0 = i - off;
+            o.jv = j0 >= 0 && j0 < nodes;
+            int64_t const j = o.jv ? j0 : ic;
+            load_rec(val + ((j >> 6) * steps + 1 + u) * 576, int(j & 63), o.w, std::false_type{});
+            o.z0 = x[3 * j], o.z1 = x[3 * j + 1], o.z2 = x[3 * j + 2];
+            return o;
+        };
+        auto accumulate_pair = [&](pair_t const& o) {
+            accumulate(o.v, o.x0, o.x1, o.x2, false);
+            accumulate(o.w, o.jv ? o.z0 : XT(0), o.jv ? o.z1 : XT(0), o.jv ? o.z2 : XT(0), true);
+        };
+        // software-pipelined over two register sets: offset u + 1's loads are issued before
+        // offset u is accumulated (3 % faster at 128^3 in both precisions than a plain loop)
+        int32_t u = 0;
+        if (nu >= 2)
+        {
+            pair_t sa = fetch(0);
+            for (; u + 2 < nu; u += 2)
+            {
+                pair_t const sb = fetch(u + 1);
+                accumulate_pair(sa);
+                sa = fetch(u + 2);
+                accumulate_pair(sb);
+            }
+            accumulate_pair(sa);
+            u += 1;
         }
+        for (; u < nu; ++u) accumulate_pair(fetch(u));
         AT const r0 = (a0 + b0) + (c0 + d0), r1 = (a1 + b1) + (c1 + d1), r2 = (a2 + b2) + (c2 + d2);
         YT const y0 = MODE == 1 ? YT(br0 - double(r0)) : YT(r0);
         YT const y1 = MODE == 1 ? YT(br1 - double(r1)) : YT(r1);
