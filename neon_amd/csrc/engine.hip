@@ -337,8 +337,9 @@ void krylov_engine::run_batches(krylov_state* st, int64_t& launched, Batch&& bat
 
 // Jacobi-PCG in FP64.  Per iteration: SpMV fused with the block partials of p.Ap, their total
 // (one block: the SpMV has one partial per slice quadruple, which every update block would
-// otherwise reduce), the x/r update fused with the partials of r.r and r.z, and the p update
-// with the convergence decision (each of its blocks reduces the x/r update's partials);
+// otherwise reduce), the r update fused with the partials of r.r and r.z, and the x and p
+// updates with the convergence decision (each of its blocks reduces the r update's partials;
+// x += alpha p sits there because that kernel reads p anyway: 10 vector passes, not 11);
 // scalars stay on the device and the host polls the state every `poll_every` iterations (a
 // batch queued ahead of the one it waits on).
 // Iterations past convergence are no-ops, so the reported count is exact.
@@ -378,9 +379,8 @@ solve_result krylov_engine::cg64(const linear_operator& A, const double* b, doub
             A.apply64(s_, p, Ap, part_pAp, cur);
             if (prof) prof->end(s_, tok);
             totals_kernel<<<1, 1024, 0, s_>>>(part_pAp, nullptr, ns, cur);
-            cg_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x, r, p, Ap, minv, part_pAp, -1, cur,
-                                                        part_rr, part_rz);
-            cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p, r, minv, part_rr, part_rz, nb, cur, nxt,
+            cg_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, r, Ap, minv, part_pAp, -1, cur, part_rr, part_rz);
+            cg_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p, x, r, minv, part_pAp, part_rr, part_rz, nb, cur, nxt,
                                                        criterion, tol, max_it);
         }
     });
@@ -596,20 +596,21 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             // the reliable-update / true-residual convergence test runs every refine_every
             // iterations
             int const decided = (launched + 1) % refine_every == 0 ? 1 : 0;
-            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, x32, r32, p32, Ap32, minv, part_pAp, -1, cur, part_rr, part_rz);
+            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, r32, Ap32, minv, part_pAp, -1, cur, part_rr, part_rz);
             if (decided)
             {
                 mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
-                mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, cur, 0);
+                mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, p32, part_pAp, cur, 0);
                 Ar.residual64(s_, b, x, r64, cur, 1);
                 mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
             }
-            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, r32, minv, part_rr_t, part_rz_t, part_rr, part_rz, nb,
+            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, x32, r32, minv, part_pAp, part_rr_t, part_rz_t,
+                                                         part_rr, part_rz, nb,
                                                          nb, decided, cur, nxt, max_it);
         }
     });
     // fold whatever the recurrence holds since the last reliable update
-    mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, st + (launched & 1), 1);
+    mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, p32, part_pAp, st + (launched & 1), 1);
     NEON_CHECK_HIP(hipGetLastError());
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
     solve_result res = finish(*host_state_, NEON_STOP_RELATIVE);

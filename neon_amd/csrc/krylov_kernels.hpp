@@ -882,9 +882,9 @@ __global__ __launch_bounds__(1024) void totals_kernel(double* __restrict__ part_
     }
 }
 
-__global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double* __restrict__ x,
-                                                              double* __restrict__ r,
-                                                              const double* __restrict__ p,
+// r -= alpha Ap with the partials of r.r and r.z (z = M^-1 r); x moves in the p update, which
+// reads p anyway (one vector pass fewer).
+__global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double* __restrict__ r,
                                                               const double* __restrict__ Ap,
                                                               const double* __restrict__ minv,
                                                               const double* __restrict__ part_pAp, int npart,
@@ -898,8 +898,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double*
     int64_t beg, end;
     xcd_chunk_aligned<2>(n, beg, end);
     double rr = 0.0, rz = 0.0;
-    auto body = [&](double& xi, double& ri, double pi, double api, double mi) {
-        xi += alpha * pi;
+    auto body = [&](double& ri, double api, double mi) {
         ri = ri - alpha * api;
         double const zi = mi * ri;
         rr += ri * ri;
@@ -909,22 +908,18 @@ __global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double*
     int64_t const full = beg + (end - beg) / 2 * 2;
     for (int64_t i = beg + 2 * threadIdx.x; i < full; i += 2 * kBlock)
     {
-        double2v xv = load_stream<kVectorNT>(reinterpret_cast<const double2v*>(x + i)),
-                 rv = *reinterpret_cast<const double2v*>(r + i);
-        double2v const pv = *reinterpret_cast<const double2v*>(p + i),
-                       av = load_stream<kVectorNT>(reinterpret_cast<const double2v*>(Ap + i)),
+        double2v rv = *reinterpret_cast<const double2v*>(r + i);
+        double2v const av = load_stream<kVectorNT>(reinterpret_cast<const double2v*>(Ap + i)),
                        mv = *reinterpret_cast<const double2v*>(minv + i);
-        double x0 = xv.x, x1 = xv.y, r0 = rv.x, r1 = rv.y;
-        body(x0, r0, pv.x, av.x, mv.x);
-        body(x1, r1, pv.y, av.y, mv.y);
-        store_stream<kVectorNT>(reinterpret_cast<double2v*>(x + i), double2v{x0, x1});
+        double r0 = rv.x, r1 = rv.y;
+        body(r0, av.x, mv.x);
+        body(r1, av.y, mv.y);
         *reinterpret_cast<double2v*>(r + i) = double2v{r0, r1};
     }
     for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
     {
-        double xi = x[i], ri = r[i];
-        body(xi, ri, p[i], Ap[i], minv[i]);
-        x[i] = xi;
+        double ri = r[i];
+        body(ri, Ap[i], minv[i]);
         r[i] = ri;
     }
     rr = block_sum<kBlock>(rr);
@@ -936,10 +931,13 @@ __global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int64_t n, double*
     }
 }
 
-// Convergence decision, beta, p = z + beta p; block 0 writes the next scalar state.
+// x += alpha p (every iteration that ran: both stopping rules update x before their test), the
+// convergence decision, beta, p = z + beta p; block 0 writes the next scalar state.
 __global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* __restrict__ p,
+                                                             double* __restrict__ x,
                                                              const double* __restrict__ r,
                                                              const double* __restrict__ minv,
+                                                             const double* __restrict__ part_pAp,
                                                              const double* __restrict__ part_rr,
                                                              const double* __restrict__ part_rz, int npart,
                                                              const krylov_state* __restrict__ st,
@@ -952,6 +950,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* 
         if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
         return;
     }
+    double const alpha = s.rz / partials_total<kBlock>(part_pAp, -1);
     double const rr = partials_total<kBlock>(part_rr, npart);
     double const rz = partials_total<kBlock>(part_rz, npart);
     int32_t it_new;
@@ -975,19 +974,27 @@ __global__ __launch_bounds__(kBlock) void cg_update_p_kernel(int64_t n, double* 
     }
     if (!isfinite(rr) || !isfinite(rz)) done = KRYLOV_NONFINITE;
     double const beta = rz / s.rz;
-    if (done == KRYLOV_RUNNING)
+    bool const more = done == KRYLOV_RUNNING;
+    int64_t beg, end;
+    xcd_chunk_aligned<2>(n, beg, end);
+    int64_t const full = beg + (end - beg) / 2 * 2;
+    for (int64_t i = beg + 2 * threadIdx.x; i < full; i += 2 * kBlock)
     {
-        int64_t beg, end;
-        xcd_chunk_aligned<2>(n, beg, end);
-        int64_t const full = beg + (end - beg) / 2 * 2;
-        for (int64_t i = beg + 2 * threadIdx.x; i < full; i += 2 * kBlock)
+        double2v const pv = *reinterpret_cast<const double2v*>(p + i);
+        double2v const xv = load_stream<kVectorNT>(reinterpret_cast<const double2v*>(x + i));
+        store_stream<kVectorNT>(reinterpret_cast<double2v*>(x + i), double2v{xv.x + alpha * pv.x, xv.y + alpha * pv.y});
+        if (more)
         {
             double2v const mv = *reinterpret_cast<const double2v*>(minv + i),
-                           rv = *reinterpret_cast<const double2v*>(r + i),
-                           pv = *reinterpret_cast<const double2v*>(p + i);
+                           rv = *reinterpret_cast<const double2v*>(r + i);
             *reinterpret_cast<double2v*>(p + i) = double2v{mv.x * rv.x + beta * pv.x, mv.y * rv.y + beta * pv.y};
         }
-        for (int64_t i = full + threadIdx.x; i < end; i += kBlock) p[i] = minv[i] * r[i] + beta * p[i];
+    }
+    for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
+    {
+        double const pi = p[i];
+        x[i] = x[i] + alpha * pi;
+        if (more) p[i] = minv[i] * r[i] + beta * pi;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
@@ -1373,9 +1380,9 @@ __global__ __launch_bounds__(kBlock) void mixed_init_state_kernel(const double* 
     *st = s;
 }
 
-__global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, float* __restrict__ x32,
-                                                                 float* __restrict__ r32,
-                                                                 const float* __restrict__ p32,
+// r32 -= alpha Ap32 with FP64 partials of r.r and r.z; x32 moves in the p update (or in the
+// fold of a reliable update)
+__global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, float* __restrict__ r32,
                                                                  const float* __restrict__ Ap32,
                                                                  const float* __restrict__ minv,
                                                                  const double* __restrict__ part_pAp, int npart,
@@ -1389,8 +1396,7 @@ __global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, floa
     int64_t beg, end;
     xcd_chunk_aligned<4>(n, beg, end);
     double rr = 0.0, rz = 0.0;
-    auto body = [&](float& xi, float& ri, float pi, float api, float mi) {
-        xi += alpha * pi;
+    auto body = [&](float& ri, float api, float mi) {
         ri = ri - alpha * api;
         float const zi = mi * ri;
         rr += double(ri) * double(ri);
@@ -1400,24 +1406,20 @@ __global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, floa
     int64_t const full = beg + (end - beg) / 4 * 4;
     for (int64_t i = beg + 4 * threadIdx.x; i < full; i += 4 * kBlock)
     {
-        float4v xv = load_stream<kVectorNT>(reinterpret_cast<const float4v*>(x32 + i)),
-                rv = *reinterpret_cast<const float4v*>(r32 + i);
-        float4v const pv = *reinterpret_cast<const float4v*>(p32 + i),
-                      av = load_stream<kVectorNT>(reinterpret_cast<const float4v*>(Ap32 + i)),
+        float4v rv = *reinterpret_cast<const float4v*>(r32 + i);
+        float4v const av = load_stream<kVectorNT>(reinterpret_cast<const float4v*>(Ap32 + i)),
                       mv = *reinterpret_cast<const float4v*>(minv + i);
-        float x0 = xv.x, x1 = xv.y, x2 = xv.z, x3 = xv.w, r0 = rv.x, r1 = rv.y, r2 = rv.z, r3 = rv.w;
-        body(x0, r0, pv.x, av.x, mv.x);
-        body(x1, r1, pv.y, av.y, mv.y);
-        body(x2, r2, pv.z, av.z, mv.z);
-        body(x3, r3, pv.w, av.w, mv.w);
-        store_stream<kVectorNT>(reinterpret_cast<float4v*>(x32 + i), float4v{x0, x1, x2, x3});
+        float r0 = rv.x, r1 = rv.y, r2 = rv.z, r3 = rv.w;
+        body(r0, av.x, mv.x);
+        body(r1, av.y, mv.y);
+        body(r2, av.z, mv.z);
+        body(r3, av.w, mv.w);
         *reinterpret_cast<float4v*>(r32 + i) = float4v{r0, r1, r2, r3};
     }
     for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
     {
-        float xi = x32[i], ri = r32[i];
-        body(xi, ri, p32[i], Ap32[i], minv[i]);
-        x32[i] = xi;
+        float ri = r32[i];
+        body(ri, Ap32[i], minv[i]);
         r32[i] = ri;
     }
     rr = block_sum<kBlock>(rr);
@@ -1444,17 +1446,23 @@ __global__ __launch_bounds__(kBlock) void mixed_decide_kernel(const double* __re
     st->aux1 = rz;
 }
 
-// x64 += x32, x32 = 0 (when a reliable update is due, or unconditionally with force)
+// x64 += x32 + alpha p32, x32 = 0 when a reliable update is due (the iteration's x step, which
+// the p update then skips); with force, x64 += x32, x32 = 0 unconditionally (the recurrence's
+// remainder after the loop)
 __global__ __launch_bounds__(kBlock) void mixed_fold_kernel(int64_t n, double* __restrict__ x64,
                                                             float* __restrict__ x32,
+                                                            const float* __restrict__ p32,
+                                                            const double* __restrict__ part_pAp,
                                                             const krylov_state* __restrict__ st, int force)
 {
     if (!force && (st->done || !st->refine)) return;
+    float const alpha = force ? 0.0f : float(st->rz / partials_total<kBlock>(part_pAp, -1));
     int64_t beg, end;
     xcd_chunk(n, beg, end);
     for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
-        x64[i] += double(x32[i]);
+        float const xi = force ? x32[i] : x32[i] + alpha * p32[i];
+        x64[i] += double(xi);
         x32[i] = 0.0f;
     }
 }
@@ -1488,12 +1496,15 @@ __global__ __launch_bounds__(kBlock) void mixed_refresh_kernel(int64_t n, const 
     }
 }
 
-// convergence on the true residual, beta, p32 = z + beta p32; block 0 writes the next state
-// (decided = 0: no decision kernel ran this iteration, so r.r and r.z come from the partials
-// of the x/r update and no reliable update is due)
+// x32 += alpha p32 (unless a reliable update folded it already), convergence on the true
+// residual, beta, p32 = z + beta p32; block 0 writes the next state (decided = 0: no decision
+// kernel ran this iteration, so r.r and r.z come from the partials of the x/r update and no
+// reliable update is due)
 __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float* __restrict__ p32,
+                                                                float* __restrict__ x32,
                                                                 const float* __restrict__ r32,
                                                                 const float* __restrict__ minv,
+                                                                const double* __restrict__ part_pAp,
                                                                 const double* __restrict__ part_rr_true,
                                                                 const double* __restrict__ part_rz_true,
                                                                 const double* __restrict__ part_rr,
@@ -1508,6 +1519,7 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
         if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
         return;
     }
+    float const alpha = float(s.rz / partials_total<kBlock>(part_pAp, -1));
     double rr = s.rr, rz = s.aux1;
     if (!decided)
     {
@@ -1515,6 +1527,7 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
         rz = partials_total<kBlock>(part_rz, npart);
         s.refine = 0;
     }
+    bool const step_x = !s.refine;
     bool conv = false;
     if (s.refine)
     {
@@ -1528,20 +1541,36 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
     int32_t done = conv ? KRYLOV_CONVERGED : (it_new >= max_it ? KRYLOV_LIMIT : KRYLOV_RUNNING);
     if (!isfinite(rr) || !isfinite(rz)) done = KRYLOV_NONFINITE;
     float const beta = float(rz / s.rz);
-    if (done == KRYLOV_RUNNING)
+    bool const more = done == KRYLOV_RUNNING;
+    if (step_x || more)
     {
         int64_t beg, end;
         xcd_chunk_aligned<4>(n, beg, end);
         int64_t const full = beg + (end - beg) / 4 * 4;
         for (int64_t i = beg + 4 * threadIdx.x; i < full; i += 4 * kBlock)
         {
-            float4v const mv = *reinterpret_cast<const float4v*>(minv + i),
-                          rv = *reinterpret_cast<const float4v*>(r32 + i),
-                          pv = *reinterpret_cast<const float4v*>(p32 + i);
-            *reinterpret_cast<float4v*>(p32 + i) = float4v{mv.x * rv.x + beta * pv.x, mv.y * rv.y + beta * pv.y,
-                                                           mv.z * rv.z + beta * pv.z, mv.w * rv.w + beta * pv.w};
+            float4v const pv = *reinterpret_cast<const float4v*>(p32 + i);
+            if (step_x)
+            {
+                float4v const xv = load_stream<kVectorNT>(reinterpret_cast<const float4v*>(x32 + i));
+                store_stream<kVectorNT>(reinterpret_cast<float4v*>(x32 + i),
+                                        float4v{xv.x + alpha * pv.x, xv.y + alpha * pv.y, xv.z + alpha * pv.z,
+                                                xv.w + alpha * pv.w});
+            }
+            if (more)
+            {
+                float4v const mv = *reinterpret_cast<const float4v*>(minv + i),
+                              rv = *reinterpret_cast<const float4v*>(r32 + i);
+                *reinterpret_cast<float4v*>(p32 + i) = float4v{mv.x * rv.x + beta * pv.x, mv.y * rv.y + beta * pv.y,
+                                                               mv.z * rv.z + beta * pv.z, mv.w * rv.w + beta * pv.w};
+            }
         }
-        for (int64_t i = full + threadIdx.x; i < end; i += kBlock) p32[i] = minv[i] * r32[i] + beta * p32[i];
+        for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
+        {
+            float const pi = p32[i];
+            if (step_x) x32[i] = x32[i] + alpha * pi;
+            if (more) p32[i] = minv[i] * r32[i] + beta * pi;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
