@@ -146,3 +146,27 @@ def test_symmetric_copy_newton_matches_the_full_copy(ctx, precision):
     for a, b in zip(r0, r1):
         assert abs(a["solver_iterations"] - b["solver_iterations"]) <= max(2, slack * a["solver_iterations"])
     assert np.abs(u1 - u0).max() <= 1e-10 * np.abs(u0).max()
+
+
+@pytest.mark.parametrize("tangent", [neon_amd.TANGENT_REFERENCE, neon_amd.TANGENT_CONSISTENT])
+def test_symmetric_copy_j2_plate_matches_the_full_copy(ctx, tangent):
+    # a 12 x 6 x 6 J2 plate without the hole (a structured pattern, so the symmetric copy
+    # applies) pulled to 0.5 % in 4 steps, well into yield: with either copy the same Newton steps
+    # and iterations, the displacement to 1e-10 of max |u| and the same plastic zone
+    p = problems.notched_plate_j2(12, 6, 6, steps=4, hole_cells=0.0)
+    runs = []
+    for flags in (FULL, SYMMETRIC):
+        mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, model=neon_amd.MODEL_J2, tangent=tangent, context=ctx)
+        assert mg.symmetric_info()["usable"]
+        mg.spmv_config(32, flags)
+        sm = neon_amd.static_matrix(mg, p.simulation, p.boundaries)
+        u = sm.solve()
+        runs.append((u, sm.records, mg.qp_state()["ebar"]))
+        print("flags", flags, "newton", len(sm.records), "krylov", [r["solver_iterations"] for r in sm.records],
+              "yielded", int((runs[-1][2] > 0).sum()), flush=True)
+    (u0, r0, e0), (u1, r1, e1) = runs
+    key = lambda rr: [(round(r["step_time"], 12), r["iteration"], r["status"]) for r in rr]  # noqa: E731
+    assert key(r0) == key(r1)
+    assert (e0 > 0).sum() > 0.25 * e0.size
+    assert np.array_equal(e0 > 0, e1 > 0)
+    assert np.abs(u1 - u0).max() <= 1e-10 * np.abs(u0).max()
