@@ -85,6 +85,27 @@ def symmetric_copy_bytes(n, value_bytes, symmetric, slices):
     return value_bytes * 576 * slices * (1 + symmetric["offsets"]) + 4 * symmetric["slots"] + 2 * value_bytes * n
 
 
+def standalone_products(ctx, mesh, nnz, n, reps=20):
+    """The FP32 products on the mesh's assembled mixed-precision tangent, outside the timed
+    region, through the measurement library (solver form: p.Ap partials, each product after a
+    write of 4 vectors): the symmetric copy (the solvers' choice where it applies) and the full
+    sliced copy, so that the two layouts' times and roofline fractions come from the same run."""
+    import ctypes as C
+
+    from neon_amd import _capi
+
+    lib = _capi.measure_lib()
+    base = _capi.SPMV_SLICED | _capi.SPMV_NONTEMPORAL | _capi.MEASURE_DOT | _capi.MEASURE_DIRTY
+    out = {}
+    for name, flags, nbytes in (("symmetric_copy", base | _capi.SPMV_SYMMETRIC, symmetric_bytes(nnz, n, 4)),
+                                ("full_copy", base, spmv_bytes(nnz, n, 4))):
+        ms = C.c_double()
+        ctx.check(lib.neon_mesh_spmv_bench(ctx.handle, _capi.PRECISION_MIXED, 32, flags, 0, reps, C.byref(ms)))
+        gbs = nbytes / (ms.value * 1e-3) / 1e9
+        out[name] = {"avg_ms": ms.value, "algorithmic_bytes": nbytes, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def run_precision(neon_amd, problems, ctx, mesh, p, precision, steps, warmup):
     sim = dict(p.simulation)
     sim["linear_solver"] = dict(sim["linear_solver"], backend={"type": "hip", "precision": precision})
@@ -326,6 +347,9 @@ def main():
             "l2_rel": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
             "load_step": 1,
         }
+    if sym and "mixed" in results:
+        # both FP32 layouts on the tangent the mixed run left assembled, outside the timed region
+        line["roofline"]["standalone_fp32_products"] = standalone_products(ctx, mesh, nnz, p.n_dofs)
     if not args.no_cpu_baseline:
         its64 = [x["solver_iterations"] for x in results.get("fp64", r)["records"]]
         line["cpu_baseline"] = cpu_baseline(problems, args.n, float(np.mean(its64)), nnz, p.n_elem)
