@@ -2,7 +2,7 @@
 mixed precision (headline) and FP64, each Newton step = assembly + Dirichlet + Jacobi-PCG to
 1e-10 relative + state update, relative Newton residual target 1e-8.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu-baseline]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu-baseline] [--layout full]
 
 Prints one JSON line.  `value` is the mixed-precision Newton steps/s over exactly K timed
 Newton iterations (inputs resident in HBM); `roofline` is the FP32 node-block SpMV of the
@@ -244,6 +244,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--layout", choices=["default", "full"], default="default",
+                    help="full: the full sliced copy of the tangent even where the symmetric copy applies")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition (z-slabs of elements, "
@@ -262,7 +264,9 @@ def main():
     nnz = mesh.nnz()
     sliced = mesh.sliced_info()
     symmetric = mesh.symmetric_info()
-    sym = symmetric["usable"]
+    sym = symmetric["usable"] and args.layout == "default"
+    if args.layout == "full":
+        mesh.spmv_config(32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL)
     log(f"setup {time.perf_counter() - t0:.1f} s: {p.n_elem} elements, {p.n_dofs} dofs, nnz {nnz}")
 
     results = {}
