@@ -9,7 +9,8 @@ loop's kernel instance; the symmetric copy where the mesh allows it unless --ful
 streaming copy of `--copy-mb` MiB (`neon_stream_probe`, the
 calibration kernel for the FETCH_SIZE / WRITE_SIZE counters), and prints one JSON line: the
 mean SpMV time, its algorithmic bytes (`spmv_bytes` or `symmetric_bytes`) and the achieved
-fraction of the 8 TB/s HBM peak.  Run it under `rocprofv3 --kernel-trace
+fraction of the 8 TB/s HBM peak.  With --assembly it times the tangent + f_int assembly instead
+(its canonical flops and algorithmic bytes).  Run it under `rocprofv3 --kernel-trace
 --stats` or `--pmc` to read the same kernels' durations and traffic counters.
 """
 from __future__ import annotations
@@ -40,6 +41,44 @@ def symmetric_bytes(nnz: int, nodes: int, value_bytes: int) -> int:
     return value_bytes * (nnz + 9 * nodes) // 2 + 2 * value_bytes * n
 
 
+ASSEMBLY_FLOP_PER_ELEMENT = 45536  # 8 QPs x (B^T D B 4320 + geometric 528 + kinematics 576 + f_int 168 + constitutive 100)
+FP64_PEAK_TFLOPS = 78.6
+
+
+def assembly_bytes(nnz: int, nodes: int, elems: int, value_bytes: int) -> int:
+    """Algorithmic bytes of one assembly: the tangent written once, coordinates and displacement
+    (48 per node), connectivity and parameters (40 per element), f_int (8 per dof)."""
+    return value_bytes * nnz + 48 * nodes + 40 * elems + 8 * 3 * nodes
+
+
+def assembly(args, ctx, lib, p, mesh, prec) -> dict:
+    """One assembly (all colours, tangent + f_int) of the held state, timed over `reps`, with the
+    streaming copy that calibrates FETCH_SIZE / WRITE_SIZE in the same process."""
+    ms = C.c_double()
+    ctx.check(lib.neon_mesh_assembly_bench(ctx.handle, prec, 128, args.reps, C.byref(ms)))
+    vb = 8 if prec == _capi.PRECISION_FP64 else 4
+    nnz = mesh.nnz()
+    b = assembly_bytes(nnz, p.n_nodes, p.n_elem, vb)
+    flop = ASSEMBLY_FLOP_PER_ELEMENT * p.n_elem
+    read_ms, copy_ms = C.c_double(), C.c_double()
+    copy_bytes = args.copy_mb << 20
+    ctx.check(lib.neon_stream_probe(ctx.handle, copy_bytes, args.reps, C.byref(read_ms), C.byref(copy_ms)))
+    out = {
+        "workload": f"neo-Hooke block {args.n}^3, assembly at u = 0",
+        "precision": args.precision,
+        "elements": p.n_elem,
+        "nnz": nnz,
+        "assembly_avg_ms": ms.value,
+        "assembly_flop": flop,
+        "assembly_tflops": flop / (ms.value * 1e-3) / 1e12,
+        "fp64_peak_frac": flop / (ms.value * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        "assembly_algorithmic_bytes": b,
+        "copy_bytes_each_way": copy_bytes,
+    }
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main(argv=None) -> dict:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--n", type=int, default=128)
@@ -48,6 +87,8 @@ def main(argv=None) -> dict:
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--dirty", action="store_true")
     ap.add_argument("--full", action="store_true", help="the full sliced copy even where the symmetric one applies")
+    ap.add_argument("--assembly", action="store_true",
+                    help="time the tangent + f_int assembly (neon_mesh_assembly_bench) instead of the SpMV")
     args = ap.parse_args(argv)
 
     ctx = neon_amd.Context(0)
@@ -59,6 +100,8 @@ def main(argv=None) -> dict:
     mesh.assemble(prec)
     ms = C.c_double()
     lib = _capi.measure_lib()
+    if args.assembly:
+        return assembly(args, ctx, lib, p, mesh, prec)
     # the solvers' form exactly: the sliced copy (symmetric where usable) with nontemporal value
     # loads and the x . y block partials (the same kernel instance as inside the CG loop),
     # workgroups by size; --dirty puts a write of 4 vectors of n before every product, as the CG
