@@ -235,17 +235,20 @@ int neon_mesh_solve(neon_context* ctx, int solver, int criterion, double toleran
 #define NEON_SPMV_INDEX32 1024  /* with NEON_SPMV_SLICED: 32-bit per-lane column offsets even where
                                    16 bits fit (the default uses 16 bits when every offset fits) */
 #define NEON_SPMV_SYMMETRIC 2048 /* with NEON_SPMV_SLICED: where the model's tangent is symmetric and
-                                    the pattern has at most 15 distinct upper block offsets, a copy
-                                    of the diagonal and upper blocks only, the lower ones read as
-                                    their transposes */
+                                    each 64-node slice's blocks have at most 40 distinct offsets
+                                    (and the copy is at most 3/4 of the full one), a copy of the
+                                    diagonal and upper blocks only, the lower ones read as their
+                                    transposes */
 int neon_mesh_spmv_config(neon_context* ctx, int32_t lanes, int32_t flags, int32_t blocks_fp64, int32_t blocks_fp32);
 /* The sliced copy's shape: slices (64 nodes each), steps (block columns per lane, padding
  * included), steps whose 64 lanes share one column offset (stored once), and whether the other
  * steps' per-lane offsets are 16-bit (1) or 32-bit (0).  No counterpart in the reference. */
 int neon_mesh_sliced_info(neon_context* ctx, int32_t* slices, int64_t* steps, int64_t* shared_steps, int32_t* index16);
 /* The symmetric sliced copy's shape: whether the solvers on this mesh's tangent can use it
- * (a symmetric model and at most 15 distinct upper block offsets), the number of those offsets
- * (0 past 15) and the wave slots of its schedule.  No counterpart in the reference. */
+ * (a symmetric model and a pattern for which it pays: at most 40 distinct block offsets per
+ * 64-node slice, a copy of at most 3/4 of the full one), the largest number of offsets of a
+ * slice (0 when the pattern does not qualify) and the wave slots of its schedule.  No
+ * counterpart in the reference. */
 int neon_mesh_symmetric_info(neon_context* ctx, int32_t* usable, int32_t* offsets, int32_t* slots);
 
 /* ---- Newton driver (static_matrix::solve) ---- */

@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <numeric>
 #include <tuple>
 #include <type_traits>
@@ -76,7 +78,10 @@ void launch_spmv(const linear_operator& A, hipStream_t s, const VT* val, const V
     if (A.node_block && A.sym)
     {
         if (!sval) throw status_error(NEON_BAD_ARGUMENT, "symmetric operator without its values");
-        sym_matrix const m{A.nodes, A.slices, A.sym_steps, A.sym_slots, A.sym_sched, A.sym_off};
+        sym_matrix const m{A.nodes,     A.slices,    A.sym_slots,
+                           A.sym_records, A.sym_sched, A.sym_rdiag,
+                           A.sym_eptr,  reinterpret_cast<const int4v*>(A.sym_ent),
+                           A.sym_rslice, A.sym_roff};
         spmv_sym_kernel<VT, XT, YT, AT, DOT, MODE><<<blocks, 256, 0, s>>>(m, sval, x, y, b, part, st, gate);
         NEON_CHECK_HIP(hipGetLastError());
         return;
@@ -1654,61 +1659,114 @@ void device_mesh::attach_sliced(linear_operator& A, int precision, bool index32)
     A.soff16 = index32 || !soff16_ok_ ? nullptr : soff16_.data();
 }
 
-// The symmetric sliced layout (sym_matrix): the pattern's distinct upper block offsets (at
-// most kMaxSymOffsets, else the layout is not used) and the wave-slot schedule.  Schedule: the
-// slices sorted by (band of the in-plane index, plane), with the plane stride P and the row
-// length L estimated from the offsets of a structured block (ascending 1, L - 1, L, L + 1, ...,
-// P + L + 1: L = smallest offset above 1, plus 1; P = largest - L - 1) and bands of 16 rows;
-// any estimate gives a valid order, a good one makes the transposed reads hit the L2.  Each
-// XCD takes one contiguous eighth of the order.
+// The symmetric sliced layout (sym_matrix).  Per slice of 64 consecutive nodes: the direct list
+// (the upper offsets of its nodes' blocks, one record each after the diagonal record) and the
+// lower list (the offsets of its nodes' lower neighbours), merged by offset into entries that
+// carry the direct record and the records holding the transposed blocks (in slice
+// floor((64 s - o) / 64) and the next one; a lower neighbour j of node i has i = j + o in its
+// own direct list, so the record exists).  The copy is used only where it pays: every slice needs
+// at most kMaxSymEntries entries and the records are at most 3/4 of the full copy's steps.
+// Schedule: the slices sorted by (band of the in-plane index, plane), with the plane stride P and
+// the row length L estimated from the 13 offsets found in most slices (for a structured block
+// ascending 1, L - 1, L, L + 1, ..., P + L + 1: L = smallest above 1, plus 1; P = largest - L -
+// 1) and bands of 16 rows; any estimate gives a valid order, a good one makes the transposed
+// reads hit the L2.  Each XCD takes one contiguous eighth of the order.
 void device_mesh::build_symmetric_pattern()
 {
     sym_ok_ = false;
-    sym_off_h_.clear();
-    sym_slots_ = 0;
+    sym_slots_ = sym_max_entries_ = 0;
+    sym_records_ = 0;
     if (N_ <= 0 || slices_ <= 0) return;
-    std::vector<int64_t> offs;
-    bool many = false;
-#pragma omp parallel
+    std::vector<std::vector<int32_t>> up(slices_), lo(slices_);
+    bool too_many = false;
+#pragma omp parallel for schedule(static) reduction(|| : too_many)
+    for (int32_t sl = 0; sl < slices_; ++sl)
     {
-        std::vector<int64_t> mine;
-        bool mine_many = false;
-#pragma omp for schedule(static)
-        for (int64_t nd = 0; nd < N_; ++nd)
-        {
-            if (mine_many) continue;
+        std::vector<int32_t>& u = up[sl];
+        std::vector<int32_t>& l = lo[sl];
+        int64_t const first = 64ll * sl, last = std::min<int64_t>(first + 64, N_);
+        for (int64_t nd = first; nd < last; ++nd)
             for (int32_t b = bptr_h_[nd]; b < bptr_h_[nd + 1]; ++b)
             {
                 int64_t const o = int64_t(bcol_h_[b]) - nd;
-                if (o <= 0 || std::find(mine.begin(), mine.end(), o) != mine.end()) continue;
-                mine.push_back(o);
-                if (int(mine.size()) > kMaxSymOffsets)
-                {
-                    mine_many = true;
-                    break;
-                }
+                if (o > 0) u.push_back(int32_t(o));
+                if (o < 0) l.push_back(int32_t(-o));
             }
-        }
-#pragma omp critical
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        std::sort(l.begin(), l.end());
+        l.erase(std::unique(l.begin(), l.end()), l.end());
+        std::vector<int32_t> both;
+        std::set_union(u.begin(), u.end(), l.begin(), l.end(), std::back_inserter(both));
+        if (int(both.size()) > kMaxSymEntries) too_many = true;
+    }
+    if (too_many) return;
+    std::vector<int32_t> base(size_t(slices_) + 1, 0);
+    for (int32_t sl = 0; sl < slices_; ++sl) base[sl + 1] = base[sl] + 1 + int32_t(up[sl].size());
+    int64_t const records = base[slices_];
+    if (records > int64_t(3) * sell_steps_ / 4 || records > int64_t(INT32_MAX) / 64) return;
+    // entries
+    std::vector<int32_t> eptr(size_t(slices_) + 1, 0);
+    std::vector<std::vector<int32_t>> ents(slices_);
+    int32_t max_entries = 0;
+#pragma omp parallel for schedule(static) reduction(max : max_entries)
+    for (int32_t sl = 0; sl < slices_; ++sl)
+    {
+        std::vector<int32_t> both;
+        std::set_union(up[sl].begin(), up[sl].end(), lo[sl].begin(), lo[sl].end(), std::back_inserter(both));
+        max_entries = std::max<int32_t>(max_entries, int32_t(both.size()));
+        auto record_of = [&](int64_t t, int32_t o) -> int32_t {
+            // record of slice t's direct block for offset o, or -1
+            if (t < 0 || t >= slices_) return -1;
+            auto const& d = up[t];
+            auto it = std::lower_bound(d.begin(), d.end(), o);
+            return it != d.end() && *it == o ? base[t] + 1 + int32_t(it - d.begin()) : -1;
+        };
+        std::vector<int32_t>& e = ents[sl];
+        for (int32_t o : both)
         {
-            many = many || mine_many;
-            for (int64_t o : mine)
-                if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+            bool const lower = std::binary_search(lo[sl].begin(), lo[sl].end(), o);
+            int64_t const sa = (64ll * sl - o) >> 6; // floor division by 64
+            e.push_back(o);
+            e.push_back(record_of(sl, o));
+            e.push_back(lower ? record_of(sa, o) : -1);
+            e.push_back(lower ? record_of(sa + 1, o) : -1);
         }
     }
-    if (many || int(offs.size()) > kMaxSymOffsets) return;
-    std::sort(offs.begin(), offs.end());
-    int64_t const steps = 1 + int64_t(offs.size());
-    if (int64_t(slices_) * steps > int64_t(INT64_MAX) / (576 * 8)) return;
-    // schedule
+    for (int32_t sl = 0; sl < slices_; ++sl) eptr[sl + 1] = eptr[sl] + int32_t(ents[sl].size() / 4);
+    std::vector<int32_t> ent(static_cast<size_t>(eptr[slices_]) * 4);
+    std::vector<int32_t> rslice(static_cast<size_t>(records)), roff(static_cast<size_t>(records), 0);
+    std::vector<int32_t> rdiag(static_cast<size_t>(slices_));
+#pragma omp parallel for schedule(static)
+    for (int32_t sl = 0; sl < slices_; ++sl)
+    {
+        std::copy(ents[sl].begin(), ents[sl].end(), ent.begin() + size_t(eptr[sl]) * 4);
+        rdiag[sl] = base[sl];
+        rslice[size_t(base[sl])] = sl;
+        for (size_t k = 0; k < up[sl].size(); ++k)
+        {
+            rslice[size_t(base[sl]) + 1 + k] = sl;
+            roff[size_t(base[sl]) + 1 + k] = up[sl][k];
+        }
+    }
+    // schedule: plane stride and row length from the offsets present in most slices
+    std::map<int32_t, int64_t> count;
+    for (int32_t sl = 0; sl < slices_; ++sl)
+        for (int32_t o : up[sl]) ++count[o];
+    std::vector<std::pair<int64_t, int32_t>> byc;
+    for (auto const& kv : count) byc.emplace_back(-kv.second, kv.first);
+    std::sort(byc.begin(), byc.end());
+    std::vector<int32_t> main_offs;
+    for (size_t k = 0; k < byc.size() && k < 13; ++k) main_offs.push_back(byc[k].second);
+    std::sort(main_offs.begin(), main_offs.end());
     int64_t L = 64, P = int64_t(N_);
-    for (int64_t o : offs)
+    for (int32_t o : main_offs)
         if (o > 1)
         {
-            L = o + 1;
+            L = int64_t(o) + 1;
             break;
         }
-    if (!offs.empty() && offs.back() - L - 1 > 0) P = offs.back() - L - 1;
+    if (!main_offs.empty() && int64_t(main_offs.back()) - L - 1 > 0) P = int64_t(main_offs.back()) - L - 1;
     int64_t const band = std::max<int64_t>(1, 16 * L);
     std::vector<int32_t> order(slices_);
     for (int32_t sl = 0; sl < slices_; ++sl) order[sl] = sl;
@@ -1731,20 +1789,40 @@ void device_mesh::build_symmetric_pattern()
         int64_t const b0 = int64_t(slices_) * x / 8, b1 = int64_t(slices_) * (x + 1) / 8;
         for (int64_t e = 0; e < b1 - b0; ++e) sched[size_t(((e / 4) * 8 + x) * 4 + e % 4)] = order[size_t(b0 + e)];
     }
-    sym_off_h_.assign(offs.begin(), offs.end());
-    sym_off_.resize(std::max<size_t>(sym_off_h_.size(), 1));
-    sym_sched_.resize(sched.size());
-    if (!sym_off_h_.empty())
-        NEON_CHECK_HIP(hipMemcpyAsync(sym_off_.data(), sym_off_h_.data(), sym_off_h_.size() * sizeof(int32_t),
-                                      hipMemcpyHostToDevice, s_));
-    NEON_CHECK_HIP(hipMemcpyAsync(sym_sched_.data(), sched.data(), sched.size() * sizeof(int32_t),
-                                  hipMemcpyHostToDevice, s_));
+    auto upload = [&](device_array<int32_t>& d, std::vector<int32_t> const& h) {
+        d.resize(std::max<size_t>(h.size(), 4));
+        if (!h.empty())
+            NEON_CHECK_HIP(hipMemcpyAsync(d.data(), h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice, s_));
+    };
+    upload(sym_sched_, sched);
+    upload(sym_rdiag_, rdiag);
+    upload(sym_eptr_, eptr);
+    upload(sym_ent_, ent);
+    upload(sym_rslice_, rslice);
+    upload(sym_roff_, roff);
+    // the host vectors go out of scope: wait for the copies
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
     sym_slots_ = int32_t(sched.size());
+    sym_records_ = records;
+    sym_max_entries_ = max_entries;
     sym_ok_ = true;
 }
 
 bool device_mesh::symmetric_usable() const { return sym_ok_ && model_is_symmetric(model_); }
+
+sym_matrix device_mesh::sym_layout() const
+{
+    return sym_matrix{N_,
+                      slices_,
+                      sym_slots_,
+                      sym_records_,
+                      sym_sched_.data(),
+                      sym_rdiag_.data(),
+                      sym_eptr_.data(),
+                      reinterpret_cast<const int4v*>(sym_ent_.data()),
+                      sym_rslice_.data(),
+                      sym_roff_.data()};
+}
 
 namespace
 {
@@ -1752,8 +1830,7 @@ template <typename VT>
 void pack_symmetric(hipStream_t s, const sym_matrix& m, const int32_t* bptr, const int32_t* bcol, const VT* val,
                     VT* sval)
 {
-    int64_t const records = int64_t(m.slices) * m.steps;
-    sym_pack_kernel<VT><<<int((records + 3) / 4), 256, 0, s>>>(m, bptr, bcol, val, sval);
+    sym_pack_kernel<VT><<<int((m.records + 3) / 4), 256, 0, s>>>(m, bptr, bcol, val, sval);
     NEON_CHECK_HIP(hipGetLastError());
 }
 } // namespace
@@ -1761,9 +1838,8 @@ void pack_symmetric(hipStream_t s, const sym_matrix& m, const int32_t* bptr, con
 void device_mesh::attach_symmetric(linear_operator& A, int precision) const
 {
     if (!symmetric_usable()) throw status_error(NEON_BAD_ARGUMENT, "the tangent has no symmetric sliced copy");
-    int32_t const steps = 1 + int32_t(sym_off_h_.size());
-    sym_matrix const m{N_, slices_, steps, sym_slots_, sym_sched_.data(), sym_off_.data()};
-    size_t const vals = size_t(slices_) * steps * 576;
+    sym_matrix const m = sym_layout();
+    size_t const vals = size_t(sym_records_) * 576;
     if (precision == NEON_PRECISION_FP64)
     {
         if (!k64_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP64 tangent assembled");
@@ -1789,10 +1865,14 @@ void device_mesh::attach_symmetric(linear_operator& A, int precision) const
     A.sell = false;
     A.sym = true;
     A.slices = slices_;
-    A.sym_steps = steps;
     A.sym_slots = sym_slots_;
+    A.sym_records = sym_records_;
     A.sym_sched = sym_sched_.data();
-    A.sym_off = sym_off_.data();
+    A.sym_rdiag = sym_rdiag_.data();
+    A.sym_eptr = sym_eptr_.data();
+    A.sym_ent = sym_ent_.data();
+    A.sym_rslice = sym_rslice_.data();
+    A.sym_roff = sym_roff_.data();
 }
 
 namespace

@@ -93,6 +93,8 @@ struct solve_result
     int64_t refinements = 0;
 };
 
+struct sym_matrix; // krylov_kernels.hpp
+
 // y = A x for a square matrix held on the device in one of two index layouts.
 struct linear_operator
 {
@@ -129,9 +131,14 @@ struct linear_operator
     // symmetric sliced copy (sym_matrix layout: diagonal and upper blocks); the SpMV uses it
     // when set, with its values at s64 / s32 and `slices` slices
     bool sym = false;
-    int32_t sym_steps = 0, sym_slots = 0;
+    int32_t sym_slots = 0;
+    int64_t sym_records = 0;
     const int32_t* sym_sched = nullptr;
-    const int32_t* sym_off = nullptr;
+    const int32_t* sym_rdiag = nullptr;
+    const int32_t* sym_eptr = nullptr;
+    const int32_t* sym_ent = nullptr; // 4 int32 per entry
+    const int32_t* sym_rslice = nullptr;
+    const int32_t* sym_roff = nullptr;
 
     int spmv_blocks() const;
     // FP64: y = A x (values v64); with part, also block partials of x.y (or w.y if w).
@@ -294,11 +301,14 @@ public:
     // points A's SpMV at the symmetric sliced copy (diagonal and upper blocks; packed here when
     // the tangent changed since the last copy); only where symmetric_usable()
     void attach_symmetric(linear_operator& A, int precision) const;
-    // a symmetric model and at most 15 distinct upper block offsets in the pattern
+    // a symmetric model and a pattern for which the symmetric copy pays (every slice needs at
+    // most kMaxSymEntries offsets and the copy holds at most 3/4 of the full copy's steps)
     bool symmetric_usable() const;
-    // upper offsets of the pattern (0 when more than 15) and the symmetric copy's wave slots
-    int32_t symmetric_offsets() const { return sym_ok_ ? int32_t(sym_off_h_.size()) : 0; }
+    // the largest number of offsets (entries) of a slice (0 when the pattern does not qualify),
+    // the symmetric copy's wave slots and records
+    int32_t symmetric_offsets() const { return sym_ok_ ? sym_max_entries_ : 0; }
     int32_t symmetric_slots() const { return sym_slots_; }
+    int64_t symmetric_records() const { return sym_records_; }
     // Jacobi inverse from the node-block diagonal
     void jacobi(double* minv, int eigen_rule, int precision);
     // consistent mass (density per element, host) in the pattern of K, FP64; Dirichlet rows and
@@ -342,6 +352,7 @@ public:
 private:
     void build_sliced_pattern();
     void build_symmetric_pattern();
+    sym_matrix sym_layout() const;
     hipStream_t s_;
     int32_t N_, E_, model_, tangent_mode_;
     int64_t nnz_ = 0;
@@ -364,11 +375,11 @@ private:
     mutable device_array<double> s64_;
     mutable device_array<float> s32_;
     mutable bool s64_fresh_ = false, s32_fresh_ = false;
-    // symmetric sliced layout: upper offsets, the wave-slot schedule and the copies
+    // symmetric sliced layout: per-slice entries, records, the wave-slot schedule and the copies
     bool sym_ok_ = false;
-    int32_t sym_slots_ = 0;
-    std::vector<int32_t> sym_off_h_;
-    device_array<int32_t> sym_off_, sym_sched_;
+    int32_t sym_slots_ = 0, sym_max_entries_ = 0;
+    int64_t sym_records_ = 0;
+    device_array<int32_t> sym_sched_, sym_rdiag_, sym_eptr_, sym_ent_, sym_rslice_, sym_roff_;
     mutable device_array<double> y64_;
     mutable device_array<float> y32_;
     mutable bool y64_fresh_ = false, y32_fresh_ = false;

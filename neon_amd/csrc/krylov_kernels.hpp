@@ -480,28 +480,37 @@ __global__ __launch_bounds__(256) void sell_pack_lds_kernel(sell_matrix A, const
 }
 
 // ------------------------------------------------ symmetric sliced layout
-// For a symmetric tangent whose block pattern has few distinct upper column offsets (column
-// node - row node > 0; a structured hex8 block has 13), the copy keeps the diagonal block and
-// the upper blocks only.  Slice s holds nodes 64 s .. 64 s + 63 (one per lane) in `steps` =
-// 1 + offsets records of 9 x 64 values laid out by sell_pos: record s * steps is the diagonal
-// block, record s * steps + 1 + u the block (i, i + off[u]) of lane node i, zero where node i
-// has no such neighbour.  The product adds, for every lane node i and offset o, the block
-// K(i, i + o) times x(i + o) and the transpose of K(i - o, i) (record (i - o) / 64 * steps + 1
-// + u, lane (i - o) % 64) times x(i - o): every off-diagonal value is read twice, and the
-// schedule makes the second read follow the first closely enough to be served by the L2.
+// For a symmetric tangent the copy keeps the diagonal block and the upper blocks (column node -
+// row node = o > 0) only.  Slice s holds nodes 64 s .. 64 s + 63 (one per lane); its records of
+// 9 x 64 values (sell_pos layout) are the diagonal block (record rdiag[s]) and one record per
+// offset o in the slice's direct list (the upper offsets of its lanes' nodes): lane l's block
+// (i, i + o), zero where node i has no such neighbour.  The product adds, for every lane node i,
+// K(i, i + o) x(i + o) for the direct offsets and the transpose of K(i - o, i) x(i - o) for the
+// slice's lower offsets (those of its lanes' lower neighbours): that block is lane (i - o) % 64
+// of the record for o of slice (i - o) / 64, which is one of the two slices floor((64 s - o) /
+// 64) and the next one, whose record indices the entry carries.  Every off-diagonal value is
+// read twice, and the schedule makes the second read follow the first closely enough to be
+// served by the L2.  A slice's entries merge the two lists by offset, each {o, direct record or
+// -1, record in slice floor((64 s - o) / 64) or -1, record in the next slice or -1}, so that the
+// direct and the transposed read of one offset are issued together.
 // Schedule: wave slot q (workgroup q / 4, on XCD (q / 4) % 8) takes slice sched[q] (-1: none).
 // The host orders the slices in tiles (a band of the in-plane index, then plane by plane) and
 // gives each XCD one contiguous eighth of that order, so a slice's neighbours one plane down
 // were read on the same XCD a few workgroups earlier.
-constexpr int kMaxSymOffsets = 15;
+constexpr int kMaxSymEntries = 40; // <= 64: a slice's entries fit one wavefront's lanes
+typedef int int4v __attribute__((ext_vector_type(4)));
 struct sym_matrix
 {
     int32_t nodes;
     int32_t slices;
-    int32_t steps;        // records per slice: 1 + offsets
-    int32_t slots;        // wave slots of the schedule (a multiple of 32)
-    const int32_t* sched; // [slots] slice of each wave slot, -1 for none
-    const int32_t* off;   // [steps - 1] the upper offsets, ascending
+    int32_t slots;          // wave slots of the schedule (a multiple of 32)
+    int64_t records;        // records of the copy
+    const int32_t* sched;   // [slots] slice of each wave slot, -1 for none
+    const int32_t* rdiag;   // [slices] record of each slice's diagonal block
+    const int32_t* eptr;    // [slices + 1] entry range of each slice
+    const int4v* ent;       // entries, by slice and ascending offset
+    const int32_t* rslice;  // [records] slice of each record (packing)
+    const int32_t* roff;    // [records] offset of each record, 0 for the diagonal (packing)
 };
 
 // y = A x on the symmetric sliced layout (DOT, MODE, gate as in spmv_sell_kernel).  FP32
@@ -516,7 +525,6 @@ __global__ __launch_bounds__(256) void spmv_sym_kernel(sym_matrix A, const VT* _
     int const lane = threadIdx.x & 63;
     int32_t const wave = __builtin_amdgcn_readfirstlane(int32_t(threadIdx.x >> 6));
     int64_t const nodes = A.nodes;
-    int64_t const steps = A.steps;
     constexpr int W = 16 / int(sizeof(VT));
     typedef VT vec_t __attribute__((ext_vector_type(W)));
     // the 9 values of lane l of the record at rec (NT: nontemporal, for a record read once)
@@ -547,6 +555,7 @@ __global__ __launch_bounds__(256) void spmv_sym_kernel(sym_matrix A, const VT* _
         bool const ok = i < nodes;
         int64_t const ic = ok ? i : nodes - 1;
         int64_t const row = 3 * ic;
+        int64_t const rdiag = A.rdiag[s];
         double br0 = 0.0, br1 = 0.0, br2 = 0.0;
         if constexpr (MODE == 1)
         {
@@ -587,53 +596,63 @@ __global__ __launch_bounds__(256) void spmv_sym_kernel(sym_matrix A, const VT* _
         };
         {
             VT v[9];
-            load_rec(val + int64_t(s) * steps * 576, lane, v, std::true_type{});
+            load_rec(val + rdiag * 576, lane, v, std::true_type{});
             accumulate(v, xo0, xo1, xo2, false);
         }
-        // one offset's operands: the stored block K(i, i + o) and x(i + o), and the transpose
-        // of K(i - o, i) (record (i - o) / 64 * steps + 1 + u, lane (i - o) % 64) and x(i - o);
-        // past the last node the stored values are zero, before the first node x counts as zero
-        int32_t const nu = int32_t(steps) - 1;
+        // one entry's operands: the stored block K(i, i + o) and x(i + o) (zero where the slice
+        // has no direct record for o), and the transpose of K(i - o, i) and x(i - o) (x counts
+        // as zero where node i - o does not exist or holds no block for o: its values are then
+        // read from the slice's own diagonal record, in bounds)
+        int32_t const e0 = A.eptr[s], ne = A.eptr[s + 1] - e0;
+        // the slice's entries (at most kMaxSymEntries <= 64), lane k holding entry k: one
+        // coalesced load, then each entry read from its lane (no scalar-load wait in the loop)
+        int4v const mine = lane < ne ? A.ent[e0 + lane] : int4v{0, 0, -1, -1};
         struct pair_t
         {
             VT v[9], w[9];
             XT x0, x1, x2, z0, z1, z2;
-            bool jv;
+            bool dv, jv;
         };
-        auto fetch = [&](int32_t u) {
+        auto fetch = [&](int32_t m) {
             pair_t o;
-            int64_t const off = A.off[u];
-            int64_t const m = i + off < nodes ? i + off : nodes - 1;
-            load_rec(val + (int64_t(s) * steps + 1 + u) * 576, lane, o.v, std::false_type{});
-            o.x0 = x[3 * m], o.x1 = x[3 * m + 1], o.x2 = x[3 * m + 2];
+            int4v const e{__builtin_amdgcn_readlane(mine.x, m), __builtin_amdgcn_readlane(mine.y, m),
+                          __builtin_amdgcn_readlane(mine.z, m), __builtin_amdgcn_readlane(mine.w, m)};
+            int64_t const off = e.x;
+            int64_t const rd = e.y;
+            o.dv = rd >= 0;
+            int64_t const mi = i + off < nodes ? i + off : nodes - 1;
+            load_rec(val + (o.dv ? rd : rdiag) * 576, lane, o.v, std::false_type{});
+            o.x0 = x[3 * mi], o.x1 = x[3 * mi + 1], o.x2 = x[3 * mi + 2];
             int64_t const j0 = i - off;
-            o.jv = j0 >= 0 && j0 < nodes;
+            int64_t const sa = (int64_t(s) * 64 - off) >> 6; // floor division by 64
+            int64_t const rt = (j0 >> 6) == sa ? int64_t(e.z) : int64_t(e.w);
+            o.jv = j0 >= 0 && j0 < nodes && rt >= 0;
             int64_t const j = o.jv ? j0 : ic;
-            load_rec(val + ((j >> 6) * steps + 1 + u) * 576, int(j & 63), o.w, std::false_type{});
+            load_rec(val + (o.jv ? rt : rdiag) * 576, o.jv ? int(j & 63) : lane, o.w, std::false_type{});
             o.z0 = x[3 * j], o.z1 = x[3 * j + 1], o.z2 = x[3 * j + 2];
             return o;
         };
         auto accumulate_pair = [&](pair_t const& o) {
-            accumulate(o.v, o.x0, o.x1, o.x2, false);
+            accumulate(o.v, o.dv ? o.x0 : XT(0), o.dv ? o.x1 : XT(0), o.dv ? o.x2 : XT(0), false);
             accumulate(o.w, o.jv ? o.z0 : XT(0), o.jv ? o.z1 : XT(0), o.jv ? o.z2 : XT(0), true);
         };
-        // software-pipelined over two register sets: offset u + 1's loads are issued before
-        // offset u is accumulated (3 % faster at 128^3 in both precisions than a plain loop)
-        int32_t u = 0;
-        if (nu >= 2)
+        // software-pipelined over two register sets: entry m + 1's loads are issued before
+        // entry m is accumulated (3 % faster at 128^3 in both precisions than a plain loop)
+        int32_t m = 0;
+        if (ne >= 2)
         {
-            pair_t sa = fetch(0);
-            for (; u + 2 < nu; u += 2)
+            pair_t sa_ = fetch(0);
+            for (; m + 2 < ne; m += 2)
             {
-                pair_t const sb = fetch(u + 1);
-                accumulate_pair(sa);
-                sa = fetch(u + 2);
+                pair_t const sb = fetch(m + 1);
+                accumulate_pair(sa_);
+                sa_ = fetch(m + 2);
                 accumulate_pair(sb);
             }
-            accumulate_pair(sa);
-            u += 1;
+            accumulate_pair(sa_);
+            m += 1;
         }
-        for (; u < nu; ++u) accumulate_pair(fetch(u));
+        for (; m < ne; ++m) accumulate_pair(fetch(m));
         AT const r0 = (a0 + b0) + (c0 + d0), r1 = (a1 + b1) + (c1 + d1), r2 = (a2 + b2) + (c2 + d2);
         YT const y0 = MODE == 1 ? YT(br0 - double(r0)) : YT(r0);
         YT const y1 = MODE == 1 ? YT(br1 - double(r1)) : YT(r1);
@@ -655,25 +674,24 @@ __global__ __launch_bounds__(256) void spmv_sym_kernel(sym_matrix A, const VT* _
 }
 
 // Symmetric sliced copy of node-block values (CSR order val[9 bptr[n] + i (3 deg) + 3 kk + j]):
-// one wavefront per record; each lane finds its column node (itself for the diagonal record,
-// else i + off[k - 1]) among its row's blocks, zeros where the row has no such block.
+// one wavefront per record; each lane finds its column node (itself for a diagonal record, else
+// i + the record's offset) among its row's blocks, zeros where the row has no such block.
 template <typename VT>
 __global__ __launch_bounds__(256) void sym_pack_kernel(sym_matrix A, const int32_t* __restrict__ bptr,
                                                        const int32_t* __restrict__ bcol, const VT* __restrict__ val,
                                                        VT* __restrict__ sval)
 {
     int64_t const rec = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (rec >= int64_t(A.slices) * A.steps) return;
+    if (rec >= A.records) return;
     int const lane = threadIdx.x & 63;
-    int64_t const s = rec / A.steps;
-    int32_t const k = int32_t(rec - s * A.steps);
+    int64_t const s = A.rslice[rec];
     int64_t const i = s * 64 + lane;
     VT v[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) v[t] = VT(0);
     if (i < A.nodes)
     {
-        int64_t const m = k == 0 ? i : i + A.off[k - 1];
+        int64_t const m = i + A.roff[rec];
         int32_t const b0 = bptr[i], deg = bptr[i + 1] - b0;
         for (int32_t kk = 0; kk < deg; ++kk)
             if (bcol[b0 + kk] == m)

@@ -174,20 +174,22 @@ def test_mixed_newton_block(ctx):
     assert sm.records[-1]["force_norm"] <= 1e-8
 
 
-@pytest.mark.parametrize("hole", [1.5, 0.0])
-def test_mixed_newton_j2_plate(ctx, hole):
+@pytest.mark.parametrize("hole, layout", [(1.5, "full"), (1.5, "symmetric"), (0.0, "symmetric")])
+def test_mixed_newton_j2_plate(ctx, hole, layout):
     # the J2 plate (consistent tangent) with the inner PCG in FP32: the radial return, f_int,
     # the Newton residual and the reliable updates stay FP64, so every load step converges to the
     # FP64 oracle's equilibrium and the history follows it: the same load steps (Newton counts
     # may differ: the inexact inner solves cost an iteration here and there), u to 1e-9 of
     # max |u| and the accumulated plastic strain to 1e-9 of its maximum (measured 3e-14 and
-    # 3e-13), the same plastic zone.  With the hole the pattern has 16 distinct upper offsets and
-    # keeps the full sliced copy; without it the symmetric copy applies
+    # 3e-13), the same plastic zone; on the full sliced copy and on the symmetric one, with the
+    # hole (slices near it have offsets of their own) and without
     p = problems.notched_plate_j2(12, 6, 3, steps=20, hole_cells=hole, residual_tolerance=1e-8,
                                   displacement_tolerance=1e-8, norm_mode=1, precision="mixed")
     mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, model=p.model, tangent=neon_amd.TANGENT_CONSISTENT,
                              context=ctx)
-    assert mg.symmetric_info()["usable"] == (hole == 0.0)
+    assert mg.symmetric_info()["usable"]
+    if layout == "full":
+        mg.spmv_config(32, neon_amd.SPMV_SLICED | neon_amd.SPMV_NONTEMPORAL)
     sm = neon_amd.static_matrix(mg, p.simulation, p.boundaries)
     u = sm.solve()
     ref = orc.Model(p.X, p.conn, p.params, model=p.model, tangent=orc.TANGENT_CONSISTENT)
@@ -198,7 +200,7 @@ def test_mixed_newton_j2_plate(ctx, hole):
         sorted({round(r["step_time"], 12) for r in recs0})
     err = np.abs(u - u0).max() / np.abs(u0).max()
     e, e0 = mg.qp_state()["ebar"], ref.qp_state()["ebar"]
-    print(f"hole {hole}: Newton {len(sm.records)} vs {len(recs0)}, u error {err:.2e}, "
+    print(f"hole {hole} {layout}: Newton {len(sm.records)} vs {len(recs0)}, u error {err:.2e}, "
           f"ebar error {np.abs(e - e0).max() / e0.max():.2e}, yielded {(e0 > 0).sum()} of {e0.size}")
     assert err < 1e-9, err
     assert (e0 > 0).sum() > 0.1 * e0.size
