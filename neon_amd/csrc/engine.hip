@@ -557,23 +557,37 @@ solve_result krylov_engine::cg_mixed(const linear_operator& A, const double* b, 
         throw status_error(NEON_BAD_ARGUMENT, "cg_mixed: node-block operators take their Jacobi inverse from the mesh");
     csr_matrix m{int32_t(n), A.rp, A.ci};
     jacobi32_kernel<float><<<grid_for(n), kBlock, 0, s_>>>(m, A.v32, minv, 1);
-    return cg_mixed_run(A, b, x, tol, max_it, nb, ns, part_pAp, part_rr, part_rz, part_rr_t, part_rz_t, st, minv);
+    return cg_mixed_run(A, b, x, tol, max_it, nb, ns, part_pAp, part_rr, part_rz, part_rr_t, part_rz_t, st, minv,
+                        nullptr, nullptr);
 }
 
 solve_result krylov_engine::cg_mixed_nb(const linear_operator& A, const double* b, double* x, const double* minv_in,
-                                        double tol, int32_t max_it)
+                                        double tol, int32_t max_it, const linear_operator* scaled)
 {
     int64_t const n = A.n;
     ensure_mixed(n);
     convert_kernel<double, float><<<grid_for(n), kBlock, 0, s_>>>(n, minv_in, minv32_.data());
+    if (scaled && (scaled->n != n || scaled->spmv_blocks() != A.spmv_blocks()))
+        throw status_error(NEON_BAD_ARGUMENT, "cg_mixed_nb: the scaled operator differs in shape");
     return cg_mixed_run(A, b, x, tol, max_it, vec_blocks(n), A.spmv_blocks(), part_[0].data(), part_[1].data(),
-                        part_[2].data(), part_[3].data(), part_[4].data(), state_.data(), minv32_.data());
+                        part_[2].data(), part_[3].data(), part_[4].data(), state_.data(), minv32_.data(), scaled,
+                        scaled ? minv_in : nullptr);
+}
+
+bool krylov_engine::all_positive(int64_t n, const double* v)
+{
+    int const nb = vec_blocks(n);
+    nonpositive_count_kernel<<<nb, kBlock, 0, s_>>>(n, v, part_[0].data());
+    reduce_to_kernel<<<1, kBlock, 0, s_>>>(part_[0].data(), nb, scalar_.data());
+    NEON_CHECK_HIP(hipGetLastError());
+    return read_scalar(scalar_.data()) == 0.0;
 }
 
 solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double* b, double* x, double tol,
                                          int32_t max_it, int nb, int ns, double* part_pAp, double* part_rr,
                                          double* part_rz, double* part_rr_t, double* part_rz_t, krylov_state* st,
-                                         float* minv)
+                                         float* minv, const linear_operator* scaled_op,
+                                         const double* minv64_scaled)
 {
     int64_t const n = A.n;
     float* r32 = r32_.data();
@@ -581,7 +595,16 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
     float* Ap32 = Ap32_.data();
     float* x32 = x32_.data();
     double* r64 = r64_.data();
-    mixed_init_kernel<<<nb, kBlock, 0, s_>>>(n, b, x, r32, x32, p32, minv, part_rr, part_rz);
+    // scaled: the FP32 recurrence runs on K~ x~ = b~ (K~ = D^-1/2 K D^-1/2, the scaled copy) with
+    // the identity preconditioner, the same iterates as Jacobi-PCG on K x = b in exact
+    // arithmetic; x64 stays unscaled (the folds scale the steps back) and the FP64 residuals
+    // r = b - K x64 use A, the unscaled FP32 values, so convergence is judged on K as without it
+    bool const scaled = minv64_scaled != nullptr;
+    linear_operator const& Arec = scaled ? *scaled_op : A;
+    if (scaled)
+        mixed_init_scaled_kernel<<<nb, kBlock, 0, s_>>>(n, b, x, r32, x32, p32, minv64_scaled, part_rr, part_rz);
+    else
+        mixed_init_kernel<<<nb, kBlock, 0, s_>>>(n, b, x, r32, x32, p32, minv, part_rr, part_rz);
     mixed_init_state_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, st, tol, max_it);
     NEON_CHECK_HIP(hipGetLastError());
     // the gated true-residual product is a no-op in all but a few of its launches: a grid of at
@@ -595,27 +618,45 @@ solve_result krylov_engine::cg_mixed_run(const linear_operator& A, const double*
             krylov_state* cur = st + (launched & 1);
             krylov_state* nxt = st + ((launched + 1) & 1);
             int const tok = prof ? prof->begin(s_, NEON_KERNEL_SPMV_FP32) : -1;
-            A.apply32(s_, p32, Ap32, part_pAp, cur);
+            Arec.apply32(s_, p32, Ap32, part_pAp, cur);
             if (prof) prof->end(s_, tok);
             totals_kernel<<<1, 1024, 0, s_>>>(part_pAp, nullptr, ns, cur);
             // the reliable-update / true-residual convergence test runs every refine_every
             // iterations
             int const decided = (launched + 1) % refine_every == 0 ? 1 : 0;
-            mixed_update_xr_kernel<<<nb, kBlock, 0, s_>>>(n, r32, Ap32, minv, part_pAp, -1, cur, part_rr, part_rz);
+            if (!scaled)
+                mixed_update_xr_kernel<false, true>
+                    <<<nb, kBlock, 0, s_>>>(n, r32, Ap32, minv, part_pAp, -1, cur, part_rr, part_rz);
+            else if (decided)
+                mixed_update_xr_kernel<true, true>
+                    <<<nb, kBlock, 0, s_>>>(n, r32, Ap32, minv, part_pAp, -1, cur, part_rr, part_rz);
+            else
+                mixed_update_xr_kernel<true, false>
+                    <<<nb, kBlock, 0, s_>>>(n, r32, Ap32, nullptr, part_pAp, -1, cur, nullptr, part_rz);
             if (decided)
             {
                 mixed_decide_kernel<<<1, kBlock, 0, s_>>>(part_rr, part_rz, nb, cur, reliable_delta);
-                mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, p32, part_pAp, cur, 0);
+                mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, p32, part_pAp, cur, 0, minv64_scaled);
                 Ar.residual64(s_, b, x, r64, cur, 1);
-                mixed_refresh_kernel<<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, cur, part_rr_t, part_rz_t);
+                if (scaled)
+                    mixed_refresh_kernel<true>
+                        <<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, minv64_scaled, cur, part_rr_t, part_rz_t);
+                else
+                    mixed_refresh_kernel<false>
+                        <<<nb, kBlock, 0, s_>>>(n, r64, r32, minv, nullptr, cur, part_rr_t, part_rz_t);
             }
-            mixed_update_p_kernel<<<nb, kBlock, 0, s_>>>(n, p32, x32, r32, minv, part_pAp, part_rr_t, part_rz_t,
-                                                         part_rr, part_rz, nb,
-                                                         nb, decided, cur, nxt, max_it);
+            if (scaled)
+                mixed_update_p_kernel<true><<<nb, kBlock, 0, s_>>>(n, p32, x32, r32, nullptr, part_pAp, part_rr_t,
+                                                                   part_rz_t, part_rr, part_rz, nb, nb, decided, cur,
+                                                                   nxt, max_it);
+            else
+                mixed_update_p_kernel<false><<<nb, kBlock, 0, s_>>>(n, p32, x32, r32, minv, part_pAp, part_rr_t,
+                                                                    part_rz_t, part_rr, part_rz, nb, nb, decided, cur,
+                                                                    nxt, max_it);
         }
     });
     // fold whatever the recurrence holds since the last reliable update
-    mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, p32, part_pAp, st + (launched & 1), 1);
+    mixed_fold_kernel<<<nb, kBlock, 0, s_>>>(n, x, x32, p32, part_pAp, st + (launched & 1), 1, minv64_scaled);
     NEON_CHECK_HIP(hipGetLastError());
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
     solve_result res = finish(*host_state_, NEON_STOP_RELATIVE);
@@ -1243,7 +1284,7 @@ void device_mesh::assemble(const double* u_dev, int precision)
     (void)hipEventDestroy(e1);
     k64_valid_ = precision == NEON_PRECISION_FP64;
     k32_valid_ = precision != NEON_PRECISION_FP64;
-    s64_fresh_ = s32_fresh_ = y64_fresh_ = y32_fresh_ = false;
+    s64_fresh_ = s32_fresh_ = y64_fresh_ = y32_fresh_ = y32s_fresh_ = false;
 }
 
 void device_mesh::assemble_mass(const double* density)
@@ -1303,7 +1344,7 @@ void device_mesh::download_mass(double* val)
 void device_mesh::enforce_dirichlet(int32_t n_fixed, const int32_t* fixed_dev, double* b_dev, int precision)
 {
     if (n_fixed <= 0) return;
-    s64_fresh_ = s32_fresh_ = y64_fresh_ = y32_fresh_ = false;
+    s64_fresh_ = s32_fresh_ = y64_fresh_ = y32_fresh_ = y32s_fresh_ = false;
     if (precision == NEON_PRECISION_FP64)
         dirichlet_kernel<double><<<grid_for(n_fixed), kBlock, 0, s_>>>(n_fixed, fixed_dev, bptr_.data(), bcol_.data(),
                                                                          k64_.data(), b_dev);
@@ -1570,7 +1611,32 @@ void device_mesh::build_sliced_pattern()
     NEON_CHECK_HIP(hipStreamSynchronize(s_));
 }
 
+bool device_mesh::symmetric_selected() const
+{
+    return (spmv.flags & NEON_SPMV_SLICED) && (spmv.flags & NEON_SPMV_SYMMETRIC) && symmetric_usable();
+}
+
+linear_operator device_mesh::op_scaled(const double* minv) const
+{
+    if (!symmetric_selected()) throw status_error(NEON_BAD_ARGUMENT, "the Jacobi-scaled tangent needs the symmetric copy");
+    linear_operator A = op_base(NEON_PRECISION_MIXED);
+    attach_symmetric(A, NEON_PRECISION_MIXED, minv);
+    return A;
+}
+
 linear_operator device_mesh::op(int precision) const
+{
+    linear_operator A = op_base(precision);
+    // nodes of more than 170 neighbours keep the CSR-order values (the sliced copy is packed
+    // through LDS rows of 3 * max degree values)
+    if (symmetric_selected())
+        attach_symmetric(A, precision);
+    else if ((spmv.flags & NEON_SPMV_SLICED) && sliced_fits())
+        attach_sliced(A, precision, (spmv.flags & NEON_SPMV_INDEX32) != 0);
+    return A;
+}
+
+linear_operator device_mesh::op_base(int precision) const
 {
     linear_operator A;
     A.n = 3ll * N_;
@@ -1591,12 +1657,6 @@ linear_operator device_mesh::op(int precision) const
     A.nb_lanes = spmv.lanes;
     A.nb_nt = (spmv.flags & NEON_SPMV_NONTEMPORAL) != 0;
     A.blocks_override = precision == NEON_PRECISION_FP64 ? spmv.blocks64 : spmv.blocks32;
-    // nodes of more than 170 neighbours keep the CSR-order values (the sliced copy is packed
-    // through LDS rows of 3 * max degree values)
-    if ((spmv.flags & NEON_SPMV_SLICED) && (spmv.flags & NEON_SPMV_SYMMETRIC) && symmetric_usable())
-        attach_symmetric(A, precision);
-    else if ((spmv.flags & NEON_SPMV_SLICED) && sliced_fits())
-        attach_sliced(A, precision, (spmv.flags & NEON_SPMV_INDEX32) != 0);
     return A;
 }
 
@@ -1828,16 +1888,18 @@ namespace
 {
 template <typename VT>
 void pack_symmetric(hipStream_t s, const sym_matrix& m, const int32_t* bptr, const int32_t* bcol, const VT* val,
-                    VT* sval)
+                    VT* sval, const double* scale_minv = nullptr)
 {
-    sym_pack_kernel<VT><<<int((m.records + 3) / 4), 256, 0, s>>>(m, bptr, bcol, val, sval);
+    sym_pack_kernel<VT><<<int((m.records + 3) / 4), 256, 0, s>>>(m, bptr, bcol, val, sval, scale_minv);
     NEON_CHECK_HIP(hipGetLastError());
 }
 } // namespace
 
-void device_mesh::attach_symmetric(linear_operator& A, int precision) const
+void device_mesh::attach_symmetric(linear_operator& A, int precision, const double* scale_minv) const
 {
     if (!symmetric_usable()) throw status_error(NEON_BAD_ARGUMENT, "the tangent has no symmetric sliced copy");
+    if (scale_minv && precision == NEON_PRECISION_FP64)
+        throw status_error(NEON_BAD_ARGUMENT, "the Jacobi-scaled copy holds FP32 values only");
     sym_matrix const m = sym_layout();
     size_t const vals = size_t(sym_records_) * 576;
     if (precision == NEON_PRECISION_FP64)
@@ -1854,13 +1916,27 @@ void device_mesh::attach_symmetric(linear_operator& A, int precision) const
     else
     {
         if (!k32_valid_) throw status_error(NEON_BAD_ARGUMENT, "no FP32 tangent assembled");
-        if (!y32_fresh_)
+        if (scale_minv)
         {
-            y32_.resize(vals);
-            pack_symmetric<float>(s_, m, bptr_.data(), bcol_.data(), k32_.data(), y32_.data());
-            y32_fresh_ = true;
+            // the mixed solver's recurrence copy, D^-1/2 K D^-1/2, beside the unscaled one
+            if (!y32s_fresh_)
+            {
+                y32s_.resize(vals);
+                pack_symmetric<float>(s_, m, bptr_.data(), bcol_.data(), k32_.data(), y32s_.data(), scale_minv);
+                y32s_fresh_ = true;
+            }
+            A.s32 = y32s_.data();
         }
-        A.s32 = y32_.data();
+        else
+        {
+            if (!y32_fresh_)
+            {
+                y32_.resize(vals);
+                pack_symmetric<float>(s_, m, bptr_.data(), bcol_.data(), k32_.data(), y32_.data());
+                y32_fresh_ = true;
+            }
+            A.s32 = y32_.data();
+        }
     }
     A.sell = false;
     A.sym = true;

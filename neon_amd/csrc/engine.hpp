@@ -179,9 +179,14 @@ public:
     // FP32 inner CG with FP64 reliable updates r = b - A x (relative rule only).
     solve_result cg_mixed(const linear_operator& A, const double* b, double* x, int criterion, double tol,
                           int32_t max_it);
-    // same, for a node-block operator whose Jacobi inverse (FP64, from the FP32 diagonal) is given
+    // same, for a node-block operator whose Jacobi inverse (FP64, from the FP32 diagonal) is given;
+    // scaled: the Jacobi-scaled tangent D^-1/2 K D^-1/2 (minv positive, same layout as A), on
+    // which the FP32 recurrence then runs without a preconditioner vector (A still gives the
+    // FP64 residuals)
     solve_result cg_mixed_nb(const linear_operator& A, const double* b, double* x, const double* minv, double tol,
-                             int32_t max_it);
+                             int32_t max_it, const linear_operator* scaled = nullptr);
+    // every entry positive and finite (synchronises)
+    bool all_positive(int64_t n, const double* v);
 
     double dot(int64_t n, const double* a, const double* b);
 
@@ -207,7 +212,8 @@ private:
     void ensure_mixed(int64_t n);
     solve_result cg_mixed_run(const linear_operator& A, const double* b, double* x, double tol, int32_t max_it, int nb,
                               int ns, double* part_pAp, double* part_rr, double* part_rz, double* part_rr_t,
-                              double* part_rz_t, krylov_state* st, float* minv);
+                              double* part_rz_t, krylov_state* st, float* minv, const linear_operator* scaled_op,
+                              const double* minv64_scaled);
     hipStream_t s_;
     device_array<double> part_[6];
     device_array<krylov_state> state_;
@@ -287,6 +293,7 @@ public:
     void download_qp_state(double* sigma, double* D, double* detF, double* eps_p, double* ebar, double* vm);
     // operator over the assembled tangent (FP64 values, or FP32 values for mixed)
     linear_operator op(int precision) const;
+    linear_operator op_base(int precision) const;
     // points A's SpMV at the sliced copy of the tangent (packed here when the tangent changed
     // since the last copy); index32: 32-bit per-lane offsets even where 16 bits fit
     void attach_sliced(linear_operator& A, int precision, bool index32 = false) const;
@@ -300,7 +307,13 @@ public:
     bool sliced_fits() const;
     // points A's SpMV at the symmetric sliced copy (diagonal and upper blocks; packed here when
     // the tangent changed since the last copy); only where symmetric_usable()
-    void attach_symmetric(linear_operator& A, int precision) const;
+    // scale_minv (FP32 values only): attach D^-1/2 K D^-1/2 with D^-1 = scale_minv (positive)
+    void attach_symmetric(linear_operator& A, int precision, const double* scale_minv = nullptr) const;
+    // the solvers' operator is the symmetric copy (flags ask for it and the mesh allows it)
+    bool symmetric_selected() const;
+    // the symmetric FP32 copy of the Jacobi-scaled tangent (minv: the positive Jacobi inverse of
+    // the FP32 diagonal), for the mixed solver
+    linear_operator op_scaled(const double* minv) const;
     // a symmetric model and a pattern for which the symmetric copy pays (every slice needs at
     // most kMaxSymEntries offsets and the copy holds at most 3/4 of the full copy's steps)
     bool symmetric_usable() const;
@@ -381,8 +394,8 @@ private:
     int64_t sym_records_ = 0;
     device_array<int32_t> sym_sched_, sym_rdiag_, sym_eptr_, sym_ent_, sym_rslice_, sym_roff_;
     mutable device_array<double> y64_;
-    mutable device_array<float> y32_;
-    mutable bool y64_fresh_ = false, y32_fresh_ = false;
+    mutable device_array<float> y32_, y32s_;
+    mutable bool y64_fresh_ = false, y32_fresh_ = false, y32s_fresh_ = false;
     device_array<double> sigma_, coef_, detF_, eps_p_, ebar_, vm_, eps_p_old_, ebar_old_;
     device_array<double> age_, age_old_; // ageing microsphere: 11 x Q network state (SoA)
     device_array<double> sigma_old_;     // LATIN: committed stress (6 x Q)

@@ -675,11 +675,13 @@ __global__ __launch_bounds__(256) void spmv_sym_kernel(sym_matrix A, const VT* _
 
 // Symmetric sliced copy of node-block values (CSR order val[9 bptr[n] + i (3 deg) + 3 kk + j]):
 // one wavefront per record; each lane finds its column node (itself for a diagonal record, else
-// i + the record's offset) among its row's blocks, zeros where the row has no such block.
+// i + the record's offset) among its row's blocks, zeros where the row has no such block.  With
+// minv (a positive Jacobi inverse) the copy holds the symmetrically scaled tangent
+// K~(a, b) = sqrt(minv[a]) K(a, b) sqrt(minv[b]) (scaled in FP64, then rounded to VT).
 template <typename VT>
 __global__ __launch_bounds__(256) void sym_pack_kernel(sym_matrix A, const int32_t* __restrict__ bptr,
                                                        const int32_t* __restrict__ bcol, const VT* __restrict__ val,
-                                                       VT* __restrict__ sval)
+                                                       VT* __restrict__ sval, const double* __restrict__ minv)
 {
     int64_t const rec = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (rec >= A.records) return;
@@ -701,6 +703,17 @@ __global__ __launch_bounds__(256) void sym_pack_kernel(sym_matrix A, const int32
                 for (int r = 0; r < 3; ++r)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) v[3 * r + c] = src[int64_t(r) * 3 * deg + c];
+                if (minv)
+                {
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                    {
+                        double const sr = sqrt(minv[3 * i + r]);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            v[3 * r + c] = VT(double(v[3 * r + c]) * sr * sqrt(minv[3 * m + c]));
+                    }
+                }
                 break;
             }
     }
@@ -1378,6 +1391,52 @@ __global__ __launch_bounds__(kBlock) void mixed_init_kernel(int64_t n, const dou
     }
 }
 
+// The same start for the Jacobi-scaled system K~ x~ = b~ (K~ = D^-1/2 K D^-1/2, b~ = D^-1/2 b,
+// x = D^-1/2 x~, D^-1/2 = sqrt(minv64)) that the FP32 recurrence runs on: r32 = p32 =
+// float(b~); x64 (unscaled) = 0; the partials of b.b (the stopping rule's reference) and of
+// r32.r32 (= r.z)
+__global__ __launch_bounds__(kBlock) void mixed_init_scaled_kernel(int64_t n, const double* __restrict__ b,
+                                                                   double* __restrict__ x64, float* __restrict__ r32,
+                                                                   float* __restrict__ x32, float* __restrict__ p32,
+                                                                   const double* __restrict__ minv64,
+                                                                   double* __restrict__ part_rr,
+                                                                   double* __restrict__ part_rz)
+{
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        double const bi = b[i];
+        float const ri = float(sqrt(minv64[i]) * bi);
+        x64[i] = 0.0;
+        x32[i] = 0.0f;
+        r32[i] = ri;
+        p32[i] = ri;
+        rr += bi * bi;
+        rz += double(ri) * double(ri);
+    }
+    rr = block_sum<kBlock>(rr);
+    rz = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0)
+    {
+        part_rr[blockIdx.x] = rr;
+        part_rz[blockIdx.x] = rz;
+    }
+}
+
+// block partials of the number of entries of v that are not positive and finite
+__global__ __launch_bounds__(kBlock) void nonpositive_count_kernel(int64_t n, const double* __restrict__ v,
+                                                                   double* __restrict__ part)
+{
+    int64_t beg, end;
+    xcd_chunk(n, beg, end);
+    double c = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock) c += (v[i] > 0.0 && isfinite(v[i])) ? 0.0 : 1.0;
+    c = block_sum<kBlock>(c);
+    if (threadIdx.x == 0) part[blockIdx.x] = c;
+}
+
 __global__ __launch_bounds__(kBlock) void mixed_init_state_kernel(const double* __restrict__ part_rr,
                                                                   const double* __restrict__ part_rz, int npart,
                                                                   krylov_state* __restrict__ st, double tol,
@@ -1399,7 +1458,10 @@ __global__ __launch_bounds__(kBlock) void mixed_init_state_kernel(const double* 
 }
 
 // r32 -= alpha Ap32 with FP64 partials of r.r and r.z; x32 moves in the p update (or in the
-// fold of a reliable update)
+// fold of a reliable update).  SCALED: the recurrence of the Jacobi-scaled system
+// (r~ = D^-1/2 r, z~ = r~, no minv read): the partials of r~.r~ (= r.z) only, and with RR also
+// those of r.r = sum r~_i^2 / minv_i, which the reliable-update test reads (minv is read then)
+template <bool SCALED, bool RR>
 __global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, float* __restrict__ r32,
                                                                  const float* __restrict__ Ap32,
                                                                  const float* __restrict__ minv,
@@ -1413,20 +1475,31 @@ __global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, floa
     float const alpha = float(st->rz / pAp);
     int64_t beg, end;
     xcd_chunk_aligned<4>(n, beg, end);
+    constexpr bool kMinv = !SCALED || RR;
     double rr = 0.0, rz = 0.0;
     auto body = [&](float& ri, float api, float mi) {
         ri = ri - alpha * api;
-        float const zi = mi * ri;
-        rr += double(ri) * double(ri);
-        rz += double(ri) * double(zi);
+        if constexpr (SCALED)
+        {
+            double const q = double(ri) * double(ri);
+            rz += q;
+            if constexpr (RR) rr += q / double(mi);
+        }
+        else
+        {
+            float const zi = mi * ri;
+            rr += double(ri) * double(ri);
+            rz += double(ri) * double(zi);
+        }
     };
     // four entries per thread in 16-byte accesses, then the chunk's tail
     int64_t const full = beg + (end - beg) / 4 * 4;
     for (int64_t i = beg + 4 * threadIdx.x; i < full; i += 4 * kBlock)
     {
         float4v rv = *reinterpret_cast<const float4v*>(r32 + i);
-        float4v const av = load_stream<kVectorNT>(reinterpret_cast<const float4v*>(Ap32 + i)),
-                      mv = *reinterpret_cast<const float4v*>(minv + i);
+        float4v const av = load_stream<kVectorNT>(reinterpret_cast<const float4v*>(Ap32 + i));
+        float4v mv{1.0f, 1.0f, 1.0f, 1.0f};
+        if constexpr (kMinv) mv = *reinterpret_cast<const float4v*>(minv + i);
         float r0 = rv.x, r1 = rv.y, r2 = rv.z, r3 = rv.w;
         body(r0, av.x, mv.x);
         body(r1, av.y, mv.y);
@@ -1437,14 +1510,14 @@ __global__ __launch_bounds__(kBlock) void mixed_update_xr_kernel(int64_t n, floa
     for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
     {
         float ri = r32[i];
-        body(ri, Ap32[i], minv[i]);
+        body(ri, Ap32[i], kMinv ? minv[i] : 1.0f);
         r32[i] = ri;
     }
-    rr = block_sum<kBlock>(rr);
+    if constexpr (kMinv) rr = block_sum<kBlock>(rr);
     rz = block_sum<kBlock>(rz);
     if (threadIdx.x == 0)
     {
-        part_rr[blockIdx.x] = rr;
+        if constexpr (kMinv) part_rr[blockIdx.x] = rr;
         part_rz[blockIdx.x] = rz;
     }
 }
@@ -1466,12 +1539,14 @@ __global__ __launch_bounds__(kBlock) void mixed_decide_kernel(const double* __re
 
 // x64 += x32 + alpha p32, x32 = 0 when a reliable update is due (the iteration's x step, which
 // the p update then skips); with force, x64 += x32, x32 = 0 unconditionally (the recurrence's
-// remainder after the loop)
+// remainder after the loop).  With minv64 the recurrence is that of the Jacobi-scaled system,
+// so its step is scaled back: x64 += sqrt(minv64) (x32 + alpha p32)
 __global__ __launch_bounds__(kBlock) void mixed_fold_kernel(int64_t n, double* __restrict__ x64,
                                                             float* __restrict__ x32,
                                                             const float* __restrict__ p32,
                                                             const double* __restrict__ part_pAp,
-                                                            const krylov_state* __restrict__ st, int force)
+                                                            const krylov_state* __restrict__ st, int force,
+                                                            const double* __restrict__ minv64)
 {
     if (!force && (st->done || !st->refine)) return;
     float const alpha = force ? 0.0f : float(st->rz / partials_total<kBlock>(part_pAp, -1));
@@ -1480,15 +1555,18 @@ __global__ __launch_bounds__(kBlock) void mixed_fold_kernel(int64_t n, double* _
     for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
         float const xi = force ? x32[i] : x32[i] + alpha * p32[i];
-        x64[i] += double(xi);
+        x64[i] += minv64 ? sqrt(minv64[i]) * double(xi) : double(xi);
         x32[i] = 0.0f;
     }
 }
 
-// r32 = float(r64) after the FP64 residual; partials of r64.r64 and r32.z32
+// r32 = float(r64) after the FP64 residual; partials of r64.r64 and r32.z32.  SCALED: the
+// recurrence holds the scaled residual r32 = float(D^-1/2 r64), and z~ = r~
+template <bool SCALED>
 __global__ __launch_bounds__(kBlock) void mixed_refresh_kernel(int64_t n, const double* __restrict__ r64,
                                                                float* __restrict__ r32,
                                                                const float* __restrict__ minv,
+                                                               const double* __restrict__ minv64,
                                                                const krylov_state* __restrict__ st,
                                                                double* __restrict__ part_rr,
                                                                double* __restrict__ part_rz)
@@ -1500,10 +1578,19 @@ __global__ __launch_bounds__(kBlock) void mixed_refresh_kernel(int64_t n, const 
     for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
         double const r = r64[i];
-        float const ri = float(r);
-        r32[i] = ri;
         rr += r * r;
-        rz += double(ri) * double(minv[i] * ri);
+        if constexpr (SCALED)
+        {
+            float const ri = float(sqrt(minv64[i]) * r);
+            r32[i] = ri;
+            rz += double(ri) * double(ri);
+        }
+        else
+        {
+            float const ri = float(r);
+            r32[i] = ri;
+            rz += double(ri) * double(minv[i] * ri);
+        }
     }
     rr = block_sum<kBlock>(rr);
     rz = block_sum<kBlock>(rz);
@@ -1517,7 +1604,9 @@ __global__ __launch_bounds__(kBlock) void mixed_refresh_kernel(int64_t n, const 
 // x32 += alpha p32 (unless a reliable update folded it already), convergence on the true
 // residual, beta, p32 = z + beta p32; block 0 writes the next state (decided = 0: no decision
 // kernel ran this iteration, so r.r and r.z come from the partials of the x/r update and no
-// reliable update is due)
+// reliable update is due).  SCALED: z = r (no minv read), and without a decision only r.z is
+// reduced (r.r is read only by the decision)
+template <bool SCALED>
 __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float* __restrict__ p32,
                                                                 float* __restrict__ x32,
                                                                 const float* __restrict__ r32,
@@ -1541,7 +1630,7 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
     double rr = s.rr, rz = s.aux1;
     if (!decided)
     {
-        rr = partials_total<kBlock>(part_rr, npart);
+        if constexpr (!SCALED) rr = partials_total<kBlock>(part_rr, npart);
         rz = partials_total<kBlock>(part_rz, npart);
         s.refine = 0;
     }
@@ -1577,17 +1666,26 @@ __global__ __launch_bounds__(kBlock) void mixed_update_p_kernel(int64_t n, float
             }
             if (more)
             {
-                float4v const mv = *reinterpret_cast<const float4v*>(minv + i),
-                              rv = *reinterpret_cast<const float4v*>(r32 + i);
-                *reinterpret_cast<float4v*>(p32 + i) = float4v{mv.x * rv.x + beta * pv.x, mv.y * rv.y + beta * pv.y,
-                                                               mv.z * rv.z + beta * pv.z, mv.w * rv.w + beta * pv.w};
+                float4v const rv = *reinterpret_cast<const float4v*>(r32 + i);
+                if constexpr (SCALED)
+                {
+                    *reinterpret_cast<float4v*>(p32 + i) =
+                        float4v{rv.x + beta * pv.x, rv.y + beta * pv.y, rv.z + beta * pv.z, rv.w + beta * pv.w};
+                }
+                else
+                {
+                    float4v const mv = *reinterpret_cast<const float4v*>(minv + i);
+                    *reinterpret_cast<float4v*>(p32 + i) =
+                        float4v{mv.x * rv.x + beta * pv.x, mv.y * rv.y + beta * pv.y, mv.z * rv.z + beta * pv.z,
+                                mv.w * rv.w + beta * pv.w};
+                }
             }
         }
         for (int64_t i = full + threadIdx.x; i < end; i += kBlock)
         {
             float const pi = p32[i];
             if (step_x) x32[i] = x32[i] + alpha * pi;
-            if (more) p32[i] = minv[i] * r32[i] + beta * pi;
+            if (more) p32[i] = (SCALED ? r32[i] : minv[i] * r32[i]) + beta * pi;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)

@@ -144,9 +144,9 @@ solve_result device_solve(neon_context* ctx, int solver, int criterion, int prec
 {
     device_mesh& m = *ctx->mesh;
     krylov_engine& k = *ctx->engine;
-    linear_operator const A = m.op(precision);
     if (precision == NEON_PRECISION_FP64)
     {
+        linear_operator const A = m.op(precision);
         m.jacobi(minv_dev, criterion == NEON_STOP_RELATIVE ? 1 : 0, precision);
         if (solver == NEON_SOLVER_CG) return k.cg64(A, b_dev, x_dev, minv_dev, criterion, tol, max_it);
         return k.bicgstab64(A, b_dev, x_dev, minv_dev, criterion, tol, max_it);
@@ -155,6 +155,15 @@ solve_result device_solve(neon_context* ctx, int solver, int criterion, int prec
     if (criterion != NEON_STOP_RELATIVE)
         throw status_error(NEON_BAD_ARGUMENT, "mixed precision uses the relative stopping rule");
     m.jacobi(minv_dev, 1, precision);
+    // on the symmetric copy the FP32 recurrence runs on the Jacobi-scaled tangent (no
+    // preconditioner vector in its updates); a diagonal that is not positive keeps the
+    // unscaled Jacobi-PCG
+    linear_operator const A = m.op(precision);
+    if (m.symmetric_selected() && k.all_positive(A.n, minv_dev))
+    {
+        linear_operator const scaled = m.op_scaled(minv_dev);
+        return k.cg_mixed_nb(A, b_dev, x_dev, minv_dev, tol, max_it, &scaled);
+    }
     return k.cg_mixed_nb(A, b_dev, x_dev, minv_dev, tol, max_it);
 }
 
@@ -308,10 +317,10 @@ int newton_solve(neon_context* ctx, const neon_newton_options* o, int32_t n_bc, 
                     sub_kernel<<<blocks(n), kBlock, 0, s>>>(n, fext.data(), m.f_int(), mres.data());
                 else
                     scale_copy_kernel<<<blocks(n), kBlock, 0, s>>>(n, -1.0, m.f_int(), mres.data());
-                linear_operator const A = m.op(precision);
                 if (it == 0)
                 {
                     // apply_displacement_boundaries: r -= K du_p; u += du_p (static_matrix.hpp:322-351)
+                    linear_operator const A = m.op(precision);
                     A.residual64(s, mres.data(), inc.data(), mres.data());
                     add_kernel<<<blocks(n), kBlock, 0, s>>>(n, inc.data(), u);
                     NEON_CHECK_HIP(hipGetLastError());

@@ -132,6 +132,45 @@ def test_symmetric_products_match_the_matrix(ctx, precision, spec):
             assert err <= tol, (extra, blocks, err)
 
 
+def test_mixed_solve_on_the_scaled_copy(ctx):
+    # on the symmetric copy the mixed solver's FP32 recurrence runs on the Jacobi-scaled tangent
+    # D^-1/2 K D^-1/2 (a copy of its own) while its FP64 residuals use the unscaled FP32 values:
+    # the solution meets the tolerance against the downloaded FP32 matrix, agrees with the full
+    # copy's (unscaled Jacobi-PCG) solve, and the products of the unscaled copy afterwards still
+    # match the matrix
+    p = problems.block_rollers(12)
+    mg = neon_amd.DeviceMesh(p.X, p.conn, p.params, context=ctx)
+    mg.set_displacement(tgk._smooth_state(p, 0.05, 7))
+    mg.update_internal_variables()
+    mg.assemble(neon_amd.PRECISION_MIXED)
+    _, f = mg.download(stiffness=False)
+    b = -f.copy()
+    mg.enforce_dirichlet(p.fixed_dofs(), b)
+    rp, ci = mg.pattern()
+    val, _ = mg.download(internal_force=False)
+    A = sp.csr_matrix((val, ci, rp))
+    sols = {}
+    for name, flags in (("full", FULL), ("symmetric", SYMMETRIC)):
+        mg.spmv_config(32, flags)
+        x, its, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
+        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        print(name, "iterations", its, "relative residual against the FP32 matrix", res, flush=True)
+        assert res <= 1e-10
+        sols[name] = (x, its)
+    (x0, i0), (x1, i1) = sols["full"], sols["symmetric"]
+    assert abs(i1 - i0) <= max(8, 0.15 * i0)
+    assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
+    y = np.zeros(p.n_dofs)
+    xr = np.random.default_rng(5).standard_normal(p.n_dofs)
+    xr32 = xr.astype(np.float32).astype(np.float64)
+    ctx.check(_capi.measure_lib().neon_mesh_spmv_apply(ctx.handle, neon_amd.PRECISION_MIXED, 32, SYMMETRIC, 0,
+                                                       _capi.f64(np.ascontiguousarray(xr)), _capi.f64(y)))
+    err = (np.abs(y - A @ xr32) / (abs(A) @ np.abs(xr32))).max()
+    print("unscaled product after the scaled solve: max error / (|K||x|)", err)
+    assert err <= 16 * np.finfo(np.float32).eps
+    mg.spmv_config()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_symmetric_copy_newton_matches_the_full_copy(ctx, precision):
     # the 12^3 block, 2 load steps: the same Newton steps and iterations with either copy, the
