@@ -248,7 +248,7 @@ def main():
                     help="full: the full sliced copy of the tangent even where the symmetric copy applies")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition (z-slabs of elements, "
+        print(json.dumps({"error": "neon_amd runs on one GPU; the multi-GPU decomposition (z-slabs of elements, "
                                    "one halo exchange and two scalar all-reduces per CG iteration) is a design only",
                           "n_gpus": args.gpus}))
         sys.exit(2)

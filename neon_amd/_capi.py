@@ -243,7 +243,7 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(
-                f"{LIB_PATH} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')"
+                f"{LIB_PATH} is missing: build the HIP library first (python -c 'from neon_amd import build; build.build()')"
             )
         handle = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
