@@ -171,6 +171,39 @@ def test_mixed_solve_on_the_scaled_copy(ctx):
     mg.spmv_config()
 
 
+def test_mixed_solve_without_a_positive_diagonal(ctx):
+    # negated material parameters: at u = 0 the tangent is negative definite with a negative
+    # diagonal, on which Jacobi-PCG still converges (M^-1 K is similar to a positive definite
+    # matrix) but the scaled recurrence, which needs sqrt(M^-1), does not apply: the solver keeps
+    # the unscaled recurrence on the symmetric copy (a scaled run would turn the square roots
+    # into NaN), meets the tolerance against the downloaded FP32 matrix and agrees with the full
+    # copy's solve
+    p = problems.block_rollers(8)
+    mg = neon_amd.DeviceMesh(p.X, p.conn, -p.params, context=ctx)
+    assert mg.symmetric_info()["usable"]
+    mg.set_displacement(np.zeros(p.n_dofs))
+    mg.update_internal_variables()
+    mg.assemble(neon_amd.PRECISION_MIXED)
+    b = np.random.default_rng(11).standard_normal(p.n_dofs)
+    mg.enforce_dirichlet(p.fixed_dofs(), b)
+    rp, ci = mg.pattern()
+    val, _ = mg.download(internal_force=False)
+    A = sp.csr_matrix((val, ci, rp))
+    assert (A.diagonal() < 0).all()
+    sols = []
+    for flags in (FULL, SYMMETRIC):
+        mg.spmv_config(32, flags)
+        x, its, _ = mg.solve(b, criterion=neon_amd.STOP_RELATIVE, tol=1e-10)
+        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        print("flags", flags, "iterations", its, "relative residual", res, flush=True)
+        assert np.isfinite(x).all() and res <= 1e-10
+        sols.append((x, its))
+    (x0, i0), (x1, i1) = sols
+    assert abs(i1 - i0) <= max(8, 0.15 * i0)
+    assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
+    mg.spmv_config()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_symmetric_copy_newton_matches_the_full_copy(ctx, precision):
     # the 12^3 block, 2 load steps: the same Newton steps and iterations with either copy, the
